@@ -1,0 +1,237 @@
+/*
+ * Integrand families of the MI355X control-variate integrator.
+ *
+ * The reference integrates any host callable `f(x0, x1, ...)` (src/integrate.h:11-70).
+ * A host lambda cannot run on the GPU, so the C-ABI takes an integrand *descriptor*:
+ * a family id plus a plain parameter block.  Each family is a small functor usable
+ * from host code (g++, oracle and reference driver) and device code (hipcc), so the
+ * three sides integrate exactly the same function.
+ *
+ * Families
+ *   VMX_FAMILY_POLY   sum of monomials  sum_m c_m prod_d x_d^e_md  plus an optional
+ *                     half-space step  h*[n.x > d].  Only +, * and compares: results
+ *                     are bit-identical on CPU and GPU (no FMA contraction).
+ *   VMX_FAMILY_BLOBS  2-D or 3-D (x, y[, t]) anisotropic Gaussian blobs that may move
+ *                     linearly in t, an optional additive step or multiplicative
+ *                     occluder [a x + b y + c t > d], and a separable wave
+ *                     A sin(fx x) cos(fy y).  Configs 1-3 are instances.
+ *   VMX_FAMILY_LOBES5 5-D: three Gaussian lobes over the integrated dims 2..4, two
+ *                     binary half-space steps over all 5 dims and 1 + a cos(2 pi k.x).
+ *
+ * Parameters in the C structs are double precision; `to_T` converts them once to the
+ * evaluation type (float for production, double for the FP64 parity instantiation).
+ */
+#ifndef VILTRUM_AMD_INTEGRANDS_H
+#define VILTRUM_AMD_INTEGRANDS_H
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define VMX_HD __host__ __device__
+#else
+#define VMX_HD
+#endif
+
+#define VMX_MAX_DIM 6
+#define VMX_POLY_MAX_TERMS 16
+#define VMX_MAX_BLOBS 64
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum vmx_family {
+    VMX_FAMILY_POLY = 1,
+    VMX_FAMILY_BLOBS = 2,
+    VMX_FAMILY_LOBES5 = 3
+};
+
+typedef struct vmx_poly_params {
+    int32_t nterms;                         /* 0..VMX_POLY_MAX_TERMS */
+    int32_t has_step;                       /* 1: add step_h*[step_n.x > step_d] */
+    double coef[VMX_POLY_MAX_TERMS];
+    uint8_t expo[VMX_POLY_MAX_TERMS][8];    /* exponent of x_d (d < VMX_MAX_DIM) */
+    double step_n[VMX_MAX_DIM];
+    double step_d;
+    double step_h;
+} vmx_poly_params;
+
+typedef struct vmx_blobs_params {
+    int32_t nblobs;                         /* 0..VMX_MAX_BLOBS */
+    int32_t step_mode;                      /* 0 none, 1 additive h*[n.x>d], 2 occluder f*[n.x>d] */
+    /* per blob: cx, cy, vx, vy, ax, ay, w ;  w*exp(-(ax*dx*dx + ay*dy*dy)),
+       dx = x - (cx + vx*t), dy = y - (cy + vy*t), t = x2 (0 in 2-D) */
+    double blob[VMX_MAX_BLOBS][7];
+    double step[5];                         /* nx, ny, nt, d, h */
+    double wave[3];                         /* A, fx, fy */
+} vmx_blobs_params;
+
+typedef struct vmx_lobes5_params {
+    double centre[3];                       /* lobe centres along dims 2,3,4 */
+    double inv2s2[3];                       /* 1/(2 sigma^2) */
+    double step_n[2][5];                    /* two visibility half-spaces over all dims */
+    double step_d[2];
+    double k[5];                            /* integer frequencies of the cosine */
+    double amp;                             /* cosine amplitude */
+} vmx_lobes5_params;
+
+/* Integrand descriptor handed to every entry point. */
+typedef struct vmx_integrand {
+    int32_t family;                         /* enum vmx_family */
+    int32_t dim;                            /* D, number of integration dimensions */
+    const void* params;                     /* host pointer to the family's parameter block */
+} vmx_integrand;
+
+#ifdef __cplusplus
+}
+
+namespace vmx {
+
+/* `eval<M, D>` takes a math policy M with static exp, sin and cos overloads for float
+   and double; host and device code each supply their own. */
+
+template <typename T>
+struct Poly {
+    int nterms, has_step;
+    T coef[VMX_POLY_MAX_TERMS];
+    uint8_t expo[VMX_POLY_MAX_TERMS][8];
+    T step_n[VMX_MAX_DIM];
+    T step_d, step_h;
+
+    static Poly from(const vmx_poly_params& p) {
+        Poly q{};
+        q.nterms = p.nterms;
+        q.has_step = p.has_step;
+        for (int m = 0; m < VMX_POLY_MAX_TERMS; ++m) {
+            q.coef[m] = T(p.coef[m]);
+            for (int d = 0; d < 8; ++d) q.expo[m][d] = p.expo[m][d];
+        }
+        for (int d = 0; d < VMX_MAX_DIM; ++d) q.step_n[d] = T(p.step_n[d]);
+        q.step_d = T(p.step_d);
+        q.step_h = T(p.step_h);
+        return q;
+    }
+
+    /* Fixed evaluation order: terms in index order, each term the coefficient times
+       x_0^e0 * x_1^e1 * ... by repeated multiplication. */
+    template <class M, int D>
+    VMX_HD T eval(const T* x) const {
+        T s = T(0);
+        for (int m = 0; m < nterms; ++m) {
+            T t = coef[m];
+            for (int d = 0; d < D; ++d)
+                for (int e = 0; e < expo[m][d]; ++e) t = t * x[d];
+            s = s + t;
+        }
+        if (has_step) {
+            T lin = T(0);
+            for (int d = 0; d < D; ++d) lin = lin + step_n[d] * x[d];
+            if (lin > step_d) s = s + step_h;
+        }
+        return s;
+    }
+};
+
+template <typename T>
+struct Blobs {
+    int nblobs, step_mode;
+    T cx[VMX_MAX_BLOBS], cy[VMX_MAX_BLOBS], vx[VMX_MAX_BLOBS], vy[VMX_MAX_BLOBS];
+    T ax[VMX_MAX_BLOBS], ay[VMX_MAX_BLOBS], w[VMX_MAX_BLOBS];
+    T snx, sny, snt, sd, sh;
+    T wa, wfx, wfy;
+
+    static Blobs from(const vmx_blobs_params& p) {
+        Blobs q{};
+        q.nblobs = p.nblobs;
+        q.step_mode = p.step_mode;
+        for (int k = 0; k < VMX_MAX_BLOBS; ++k) {
+            q.cx[k] = T(p.blob[k][0]);
+            q.cy[k] = T(p.blob[k][1]);
+            q.vx[k] = T(p.blob[k][2]);
+            q.vy[k] = T(p.blob[k][3]);
+            q.ax[k] = T(p.blob[k][4]);
+            q.ay[k] = T(p.blob[k][5]);
+            q.w[k] = T(p.blob[k][6]);
+        }
+        q.snx = T(p.step[0]);
+        q.sny = T(p.step[1]);
+        q.snt = T(p.step[2]);
+        q.sd = T(p.step[3]);
+        q.sh = T(p.step[4]);
+        q.wa = T(p.wave[0]);
+        q.wfx = T(p.wave[1]);
+        q.wfy = T(p.wave[2]);
+        return q;
+    }
+
+    template <class M, int D>
+    VMX_HD T eval(const T* x) const {
+        static_assert(D == 2 || D == 3, "blob family is 2-D or 3-D");
+        const T t = (D == 3) ? x[D == 3 ? 2 : 0] : T(0);
+        T s = T(0);
+        for (int k = 0; k < nblobs; ++k) {
+            const T dx = x[0] - (cx[k] + vx[k] * t);
+            const T dy = x[1] - (cy[k] + vy[k] * t);
+            s = s + w[k] * M::exp(-(ax[k] * (dx * dx) + ay[k] * (dy * dy)));
+        }
+        if (wa != T(0)) s = s + wa * (M::sin(wfx * x[0]) * M::cos(wfy * x[1]));
+        if (step_mode != 0) {
+            const T lin = (snx * x[0] + sny * x[1]) + snt * t;
+            const bool in = lin > sd;
+            if (step_mode == 1) {
+                if (in) s = s + sh;
+            } else {
+                if (!in) s = T(0);
+            }
+        }
+        return s;
+    }
+};
+
+template <typename T>
+struct Lobes5 {
+    T c[3], a[3];
+    T n[2][5];
+    T d[2];
+    T k[5];
+    T amp;
+
+    static Lobes5 from(const vmx_lobes5_params& p) {
+        Lobes5 q{};
+        for (int j = 0; j < 3; ++j) {
+            q.c[j] = T(p.centre[j]);
+            q.a[j] = T(p.inv2s2[j]);
+        }
+        for (int s = 0; s < 2; ++s) {
+            for (int j = 0; j < 5; ++j) q.n[s][j] = T(p.step_n[s][j]);
+            q.d[s] = T(p.step_d[s]);
+        }
+        for (int j = 0; j < 5; ++j) q.k[j] = T(p.k[j]);
+        q.amp = T(p.amp);
+        return q;
+    }
+
+    template <class M, int D>
+    VMX_HD T eval(const T* x) const {
+        static_assert(D == 5, "lobes5 family is 5-D");
+        for (int s = 0; s < 2; ++s) {
+            T lin = T(0);
+            for (int j = 0; j < 5; ++j) lin = lin + n[s][j] * x[j];
+            if (!(lin > d[s])) return T(0);
+        }
+        T q = T(0);
+        for (int j = 0; j < 3; ++j) {
+            const T u = x[2 + j] - c[j];
+            q = q + a[j] * (u * u);
+        }
+        T ph = T(0);
+        for (int j = 0; j < 5; ++j) ph = ph + k[j] * x[j];
+        const T two_pi = T(6.283185307179586476925286766559);
+        return M::exp(-q) * (T(1) + amp * M::cos(two_pi * ph));
+    }
+};
+
+}  // namespace vmx
+#endif /* __cplusplus */
+
+#endif /* VILTRUM_AMD_INTEGRANDS_H */

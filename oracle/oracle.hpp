@@ -1,0 +1,703 @@
+/*
+ * TEST INFRASTRUCTURE ONLY -- the oracle.  Nothing in the product (viltrum_amd/, include/)
+ * may include, link or call this file; only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py use it, as the checker and as the timed CPU baseline.
+ *
+ * CPU restatement of viltrum's per-bin control-variate integration path
+ * (integrator_crespo2021 and integrator_adaptive_control_variates_parallel), written
+ * from the behaviour of the reference files cited at each function.  Arithmetic
+ * follows the reference operation by operation, including its mixed float/double
+ * promotions, so that an FP64 (and, modulo libm, FP32) run reproduces the reference
+ * bit for bit:
+ *   - node map    Float(double(p) * double(Float(hi - lo)) + double(lo))
+ *   - Simpson     (p0 + 4.0*p1 + p2) / 6.0 in double, rounded to the value type
+ *   - fold orders first->last for error / approximation, last->first for subrange
+ *   - libstdc++'s push_heap / pop_heap, mt19937, uniform_int_distribution and
+ *     uniform_real_distribution are used directly (the reference's own third-party
+ *     boundary, GCC 11.4 here).
+ *
+ * Pinning (what checks each function; see tests/):
+ *   fill / split / error / heuristics / heap      golden region lists written by the
+ *                                                 reference itself (oracle/_ref), brute
+ *                                                 force on tiny trees, closed forms
+ *   integral_subrange / CV assembly               golden spp=0 bins from oracle/_ref,
+ *                                                 exactness on tensor quadratics,
+ *                                                 partition sums
+ *   residual MC + cv_optimize_weight              golden spp>0 bins from oracle/_ref
+ *                                                 (same libstdc++ RNG), zero residual
+ *                                                 on quadratics, 4-sigma statistics
+ *   blobs2d_exact                                 erf / polygon closed forms checked
+ *                                                 against brute-force quadrature
+ */
+#pragma once
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <random>
+#include <stdexcept>
+#include <thread>
+#include <vector>
+
+#include "../include/viltrum_amd/integrands.h"
+
+namespace vmo {
+
+struct HostMath {
+    static float exp(float x) { return std::exp(x); }
+    static double exp(double x) { return std::exp(x); }
+    static float sin(float x) { return std::sin(x); }
+    static double sin(double x) { return std::sin(x); }
+    static float cos(float x) { return std::cos(x); }
+    static double cos(double x) { return std::cos(x); }
+};
+
+constexpr int pow3(int d) { return d <= 0 ? 1 : 3 * pow3(d - 1); }
+
+enum Heuristic { HEUR_SIZE_RELATIVE = 0, HEUR_DEFAULT_ABSOLUTE = 1 };
+enum Integrator { INT_VR_OPTIMIZE = 0, INT_VR_FIXED = 1, INT_CV_ALPHA1 = 2 };
+
+/* A region: box [lo, hi], its cached volume and 3^D node values, dim 0 fastest
+   (src/newton-cotes/region.h:23-47, src/multiarray/multiarray.h:20-25,
+   src/range.h:17-26).  err/dim is the heuristic's tuple (ExtendedRegion,
+   src/newton-cotes/region.h:442-452). */
+template <typename T, int D>
+struct Region {
+    static constexpr int K = pow3(D);
+    T lo[D], hi[D];
+    T vol;
+    T v[K];
+    T err;
+    int dim;
+};
+
+template <typename T, int D>
+inline T box_volume(const T* lo, const T* hi) {
+    T v = T(1);  // src/range.h:20-23
+    for (int i = 0; i < D; ++i) v *= (hi[i] - lo[i]);
+    return v;
+}
+
+/* Normalised node coordinate p in {0, .5, 1} (or .25/.75 on split) mapped into the
+   region (src/newton-cotes/region.h:40-47). */
+template <typename T>
+inline T node_coord(double p, T lo, T hi) {
+    return T(p * (hi - lo) + lo);
+}
+
+inline int digit(int idx, int d) {
+    for (int i = 0; i < d; ++i) idx /= 3;
+    return idx % 3;
+}
+
+/* ---- rules (src/newton-cotes/rules.h:9-100, src/nested/nested.h:16-33) ---- */
+template <typename T>
+inline T simpson(T p0, T p1, T p2) {
+    return T((p0 + 4.0 * p1 + p2) / 6.0);
+}
+template <typename T>
+inline T trapezoid(T p0, T p2) {
+    return T((p0 + p2) / 2.0);
+}
+template <typename T>
+inline T simpson_at(T t, T p0, T p1, T p2) {
+    const T c0 = p0;
+    const T c1 = -3 * p0 + 4 * p1 - p2;
+    const T c2 = 2 * p0 - 4 * p1 + 2 * p2;
+    return (c2 * t + c1) * t + c0;
+}
+template <typename T>
+inline T simpson_subrange(T a, T b, T p0, T p1, T p2) {
+    const T c0 = p0;
+    const T c1 = -3 * p0 + 4 * p1 - p2;
+    const T c2 = 2 * p0 - 4 * p1 + 2 * p2;
+    return T(((c2 * b / 3.0 + c1 / 2.0) * b + c0) * b - ((c2 * a / 3.0 + c1 / 2.0) * a + c0) * a);
+}
+
+/* ---- error metrics (src/nested/error-metric.h:10-55) ---- */
+template <typename T>
+inline T metric_relative(T a, T b) {
+    const double min_val = 1.e-37;
+    if (std::max(std::abs(a), std::abs(b)) < min_val) return std::abs(b - a);
+    return std::abs(b - a) / std::max(std::abs(a), std::abs(b));
+}
+template <typename T>
+inline T metric_absolute(T a, T b) {
+    return std::abs(b - a);
+}
+
+/* Iterated Simpson over an array of k dims, dim 0 fastest: fold dim 0 repeatedly
+   (fold_all, src/multiarray/fold.h:163-180). `a` is overwritten. */
+template <typename T>
+inline T fold_all_simpson(T* a, int k) {
+    int n = pow3(k);
+    while (n > 1) {
+        n /= 3;
+        for (int j = 0; j < n; ++j) a[j] = simpson(a[3 * j], a[3 * j + 1], a[3 * j + 2]);
+    }
+    return a[0];
+}
+
+/* Region::error(dim, metric) (src/newton-cotes/region.h:387-393): fold `dim` with
+   metric(Simpson, Trapezoid) per 3-point line, then fold_all Simpson over the rest. */
+template <typename T, int D, typename Metric>
+T region_error(const Region<T, D>& r, int dim, Metric metric) {
+    constexpr int KM = pow3(D - 1);
+    T e[KM];
+    int stride = pow3(dim);
+    for (int j = 0; j < KM; ++j) {
+        // j's base-3 digits are the remaining dims in increasing order
+        int rem = j, full = 0, w = 1, q = 0;
+        for (int d = 0; d < D; ++d) {
+            if (d == dim) {
+                w *= 3;
+                continue;
+            }
+            full += (rem % 3) * w;
+            rem /= 3;
+            w *= 3;
+            ++q;
+        }
+        const T p0 = r.v[full], p1 = r.v[full + stride], p2 = r.v[full + 2 * stride];
+        e[j] = metric(simpson(p0, p1, p2), trapezoid(p0, p2));
+    }
+    return r.vol * fold_all_simpson(e, D - 1);
+}
+
+/* error_heuristic_size(error_metric_relative(), 1e-5, 1e-37)
+   (src/nested/error-heuristic.h:21-47). The later dim wins ties (>=). */
+template <typename T, int D>
+void heuristic_size_relative(Region<T, D>& r) {
+    const double size_weight = 1.e-5, min_size = 1.e-37;
+    T max_err = region_error(r, 0, metric_relative<T>);
+    const T ext0 = r.hi[0] - r.lo[0];
+    if (ext0 < min_size || std::isnan(ext0))
+        max_err = 0;
+    else
+        max_err += size_weight * std::abs(ext0);
+    int max_dim = 0;
+    T err = max_err;
+    for (int d = 1; d < D; ++d) {
+        const T ext = r.hi[d] - r.lo[d];
+        err = region_error(r, d, metric_relative<T>) + size_weight * std::abs(ext);
+        if (ext < min_size) err = 0;
+        if (err >= max_err) {
+            max_err = err;
+            max_dim = d;
+        }
+    }
+    r.err = max_err;
+    r.dim = max_dim;
+}
+
+/* error_heuristic_default(error_metric_absolute()) -> Region::max_error_dimension
+   (src/nested/error-heuristic.h:10-19, src/newton-cotes/region.h:401-411). The first
+   dim wins ties (>), and an all-zero region keeps (0, 0). */
+template <typename T, int D>
+void heuristic_default_absolute(Region<T, D>& r) {
+    T max_err = 0;
+    int max_dim = 0;
+    for (int d = 0; d < D; ++d) {
+        const T err = region_error(r, d, metric_absolute<T>);
+        if (err > max_err) {
+            max_err = err;
+            max_dim = d;
+        }
+    }
+    r.err = max_err;
+    r.dim = max_dim;
+}
+
+template <typename T, int D>
+void apply_heuristic(Region<T, D>& r, int heuristic) {
+    if (heuristic == HEUR_SIZE_RELATIVE)
+        heuristic_size_relative(r);
+    else
+        heuristic_default_absolute(r);
+}
+
+/* Root fill (src/multiarray/fill.h:45-72 via region.h:62-68). */
+template <typename T, int D, typename F>
+void fill_region(Region<T, D>& r, const F& f) {
+    for (int idx = 0; idx < Region<T, D>::K; ++idx) {
+        T x[D];
+        for (int d = 0; d < D; ++d) x[d] = node_coord<T>(double(digit(idx, d)) / 2.0, r.lo[d], r.hi[d]);
+        r.v[idx] = f(x);
+    }
+}
+
+/* Region::split(f, dim) with 2 parts (src/newton-cotes/region.h:345-359,
+   src/multiarray/split.h:13-49): child 0 takes parent planes {0, new(.25), 1},
+   child 1 takes {1, new(.75), 2}; new planes are evaluated at parent-relative
+   coordinates through the parent's node map. */
+template <typename T, int D, typename F>
+void split_region(const Region<T, D>& p, int dim, const F& f, Region<T, D>& c0, Region<T, D>& c1) {
+    const int stride = pow3(dim);
+    for (int idx = 0; idx < Region<T, D>::K; ++idx) {
+        const int j = digit(idx, dim);
+        const int base = idx - j * stride;
+        if (j == 0) {
+            c0.v[idx] = p.v[base];
+            c1.v[idx] = p.v[base + stride];
+        } else if (j == 2) {
+            c0.v[idx] = p.v[base + stride];
+            c1.v[idx] = p.v[base + 2 * stride];
+        }
+    }
+    // the whole new plane of child 0 (v = .25) is evaluated before child 1's (v = .75)
+    for (int c = 0; c < 2; ++c) {
+        Region<T, D>& ch = c == 0 ? c0 : c1;
+        for (int idx = 0; idx < Region<T, D>::K; ++idx) {
+            if (digit(idx, dim) != 1) continue;
+            T x[D];
+            for (int d = 0; d < D; ++d)
+                x[d] = node_coord<T>(d == dim ? (c == 0 ? 0.25 : 0.75) : double(digit(idx, d)) / 2.0, p.lo[d],
+                                     p.hi[d]);
+            ch.v[idx] = f(x);
+        }
+    }
+    const T dd = (p.hi[dim] - p.lo[dim]) / T(2);
+    for (int d = 0; d < D; ++d) {
+        c0.lo[d] = c1.lo[d] = p.lo[d];
+        c0.hi[d] = c1.hi[d] = p.hi[d];
+    }
+    c0.hi[dim] = p.lo[dim] + dd * T(1);
+    c1.lo[dim] = c0.hi[dim];
+    c0.vol = box_volume<T, D>(c0.lo, c0.hi);
+    c1.vol = box_volume<T, D>(c1.lo, c1.hi);
+}
+
+/* RegionsGeneratorAdaptiveHeap::generate (src/nested/regions-generator-adaptive-heap.h:
+   18-45): greedy max-heap on err, `iterations` pops, returned in heap-array order. */
+template <typename T, int D, typename F>
+std::vector<Region<T, D>> generate(const F& f, const T* lo, const T* hi, int heuristic, size_t iterations,
+                                   uint64_t* evals = nullptr) {
+    using R = Region<T, D>;
+    std::vector<R> heap;
+    heap.reserve(iterations + 1);
+    R root;
+    for (int d = 0; d < D; ++d) {
+        root.lo[d] = lo[d];
+        root.hi[d] = hi[d];
+    }
+    root.vol = box_volume<T, D>(root.lo, root.hi);
+    fill_region(root, f);
+    apply_heuristic(root, heuristic);
+    heap.push_back(root);
+    auto cmp = [](const R& a, const R& b) { return a.err < b.err; };
+    R c0, c1;
+    for (size_t i = 0; i < iterations; ++i) {
+        const R r = heap.front();
+        split_region(r, r.dim, f, c0, c1);
+        std::pop_heap(heap.begin(), heap.end(), cmp);
+        heap.pop_back();
+        apply_heuristic(c0, heuristic);
+        heap.push_back(c0);
+        std::push_heap(heap.begin(), heap.end(), cmp);
+        apply_heuristic(c1, heuristic);
+        heap.push_back(c1);
+        std::push_heap(heap.begin(), heap.end(), cmp);
+    }
+    if (evals) *evals = uint64_t(R::K) + uint64_t(iterations) * 2 * pow3(D - 1);
+    return heap;
+}
+
+/* ---- region queries used by the per-bin integrator ---- */
+
+/* Range::pos_in_range (src/range.h:44-53). */
+template <typename T>
+inline T pos_in(T x, T lo, T hi) {
+    if (lo >= hi) return lo;
+    return (x - lo) / (hi - lo);
+}
+
+/* Region::integral_subrange -> sub_last (src/newton-cotes/region.h:141-169): every
+   dim folded with Simpson::subrange, last dim first. */
+template <typename T, int D>
+T integral_subrange(const Region<T, D>& r, const T* slo, const T* shi) {
+    T a[D], b[D];
+    for (int d = 0; d < D; ++d) {
+        a[d] = pos_in(slo[d], r.lo[d], r.hi[d]);
+        b[d] = pos_in(shi[d], r.lo[d], r.hi[d]);
+    }
+    T w[Region<T, D>::K];
+    for (int i = 0; i < Region<T, D>::K; ++i) w[i] = r.v[i];
+    int n = Region<T, D>::K;
+    for (int d = D - 1; d >= 0; --d) {
+        n /= 3;
+        for (int j = 0; j < n; ++j) w[j] = simpson_subrange(a[d], b[d], w[j], w[j + n], w[j + 2 * n]);
+    }
+    return r.vol * w[0];
+}
+
+/* Region::approximation_at (src/newton-cotes/region.h:86-107): Horner folds over
+   dims 0, 1, ..., D-1, times volume_from(D) = 1. */
+template <typename T, int D>
+T approximation_at(const Region<T, D>& r, const T* x) {
+    T w[Region<T, D>::K];
+    for (int i = 0; i < Region<T, D>::K; ++i) w[i] = r.v[i];
+    int n = Region<T, D>::K;
+    for (int d = 0; d < D; ++d) {
+        const T t = pos_in(x[d], r.lo[d], r.hi[d]);
+        n /= 3;
+        for (int j = 0; j < n; ++j) w[j] = simpson_at(t, w[3 * j], w[3 * j + 1], w[3 * j + 2]);
+    }
+    return w[0] * T(1);
+}
+
+/* pixels_in_region (src/newton-cotes/region.h:454-463): bin box [start, end) per bin
+   dim; a bin touched by < 1% of its width at the region's upper edge is dropped. */
+template <typename T, int D>
+void pixels_in_region(const Region<T, D>& r, int db, const uint64_t* res, const T* lo, const T* hi, uint64_t* start,
+                      uint64_t* end) {
+    for (int i = 0; i < db; ++i) {
+        start[i] = std::max(uint64_t(0), uint64_t(T(res[i]) * (r.lo[i] - lo[i]) / (hi[i] - lo[i])));
+        end[i] = std::max(start[i] + 1,
+                          std::min(uint64_t(res[i]), uint64_t(0.99f + (T(res[i]) * (r.hi[i] - lo[i]) / (hi[i] - lo[i])))));
+    }
+}
+
+/* Range::intersection (src/range.h:88-97), std::max/std::min semantics kept. */
+template <typename T>
+inline T ref_max(T a, T b) {
+    return (a < b) ? b : a;
+}
+template <typename T>
+inline T ref_min(T a, T b) {
+    return (b < a) ? b : a;
+}
+
+struct Stats {
+    uint64_t regions = 0;
+    uint64_t pairs = 0;
+    uint64_t evals_regions = 0;
+    uint64_t evals_residual = 0;
+    double t_generate = 0, t_stratify = 0, t_bins = 0;
+};
+
+/* Bin -> region lists in region order (serial MutexedTensorVector fill,
+   src/control-variates/regions-integrator-parallel-variance-reduction.h:49-57),
+   as CSR over bins numbered dim 0 fastest. */
+template <typename T, int D>
+void stratify(const std::vector<Region<T, D>>& regs, int db, const uint64_t* res, const T* lo, const T* hi,
+              std::vector<uint64_t>& off, std::vector<uint32_t>& idx) {
+    uint64_t nb = 1;
+    for (int i = 0; i < db; ++i) nb *= res[i];
+    off.assign(nb + 1, 0);
+    uint64_t s[3], e[3];
+    auto for_box = [&](auto&& fn) {
+        for (size_t r = 0; r < regs.size(); ++r) {
+            pixels_in_region(regs[r], db, res, lo, hi, s, e);
+            uint64_t p[3] = {s[0], db > 1 ? s[1] : 0, db > 2 ? s[2] : 0};
+            uint64_t ee[3] = {e[0], db > 1 ? e[1] : 1, db > 2 ? e[2] : 1};
+            for (p[2] = (db > 2 ? s[2] : 0); p[2] < ee[2]; ++p[2])
+                for (p[1] = (db > 1 ? s[1] : 0); p[1] < ee[1]; ++p[1])
+                    for (p[0] = s[0]; p[0] < ee[0]; ++p[0]) {
+                        const uint64_t b = p[0] + (db > 1 ? res[0] * (p[1] + (db > 2 ? res[1] * p[2] : 0)) : 0);
+                        fn(b, uint32_t(r));
+                    }
+        }
+    };
+    for_box([&](uint64_t b, uint32_t) { off[b + 1]++; });
+    for (uint64_t b = 0; b < nb; ++b) off[b + 1] += off[b];
+    idx.assign(off[nb], 0);
+    std::vector<uint64_t> cur(off.begin(), off.end() - 1);
+    for_box([&](uint64_t b, uint32_t r) { idx[cur[b]++] = r; });
+}
+
+/* cv_optimize_weight accumulator (src/control-variates/weight-strategy.h:40-110)
+   with NormDefault = |.| (src/control-variates/norm.h:11-13). */
+template <typename T>
+struct AccOptimize {
+    T sum_f = 0, sum_app = 0;
+    uint64_t size = 0;
+    void set_alpha(double) {}
+    double k_f = 0, k_app = 0, e_f = 0, e_ap = 0, e_ap2 = 0, e_fap = 0;
+    void push(T fs, T as) {
+        if (size == 0) {
+            k_f = std::abs(fs);
+            k_app = std::abs(as);
+        }
+        e_f += (std::abs(fs) - k_f);
+        e_ap += (std::abs(as) - k_app);
+        e_ap2 += (std::abs(as) - k_app) * (std::abs(as) - k_app);
+        e_fap += (std::abs(fs) - k_f) * (std::abs(as) - k_app);
+        sum_f += fs;
+        sum_app += as;
+        ++size;
+    }
+    double covariance() const { return (e_fap - (e_f * e_ap) / double(size)) / double(size - 1); }
+    double variance() const { return (e_ap2 - (e_ap * e_ap) / double(size)) / double(size - 1); }
+    double alpha() const {
+        if (size < 2) return 1.0;
+        const double v = std::max(0.0, variance());
+        if (v <= 0.0) return 1.0;
+        const double c = std::min(v, std::max(0.0, covariance()));
+        return c / v;
+    }
+    T integral(T approximation) const {
+        if (size < 2) return approximation;
+        const double a = alpha();
+        return T((sum_f - a * sum_app) / double(size) + a * approximation);
+    }
+};
+
+/* cv_fixed_weight accumulator (src/control-variates/weight-strategy.h:7-35). */
+template <typename T>
+struct AccFixed {
+    T sum = 0;
+    uint64_t size = 0;
+    double alpha = 1;
+    void set_alpha(double a) { alpha = a; }
+    void push(T fs, T as) {
+        sum += fs - alpha * as;
+        ++size;
+    }
+    T integral(T approximation) const {
+        if (size == 0) return approximation;
+        return T(sum / double(size) + alpha * approximation);
+    }
+};
+
+/* Bin layout of every result array: C order over (i0, i1, i2), i.e. bins[i0][i1][i2]
+   (the vector<vector<T>> accessor of src/integrate.h:139-148). */
+inline uint64_t out_index(int db, const uint64_t* res, const uint64_t* p) {
+    if (db == 1) return p[0];
+    if (db == 2) return p[0] * res[1] + p[1];
+    return (p[0] * res[1] + p[1]) * res[2] + p[2];
+}
+
+/* Per-bin control variate + residual, variance-reduction form
+   (src/control-variates/regions-integrator-parallel-variance-reduction.h:32-109):
+   per-bin mt19937 seeded in dim-0-fastest bin order from mt19937(seed); uniform
+   region roulette (region-russian-roulette.h:9-28); uniform in-cell sampling
+   (region-sampling.h:9-20); bins assigned.  Threads split dim-0 rows as
+   src/foreach.h:43-78 does; results do not depend on the thread count. */
+template <typename T, int D, typename F, typename Acc>
+void integrate_regions_vr(const std::vector<Region<T, D>>& regs, const F& f, const T* lo, const T* hi, int db,
+                          const uint64_t* res, uint64_t spp, uint64_t seed, double alpha, int nthreads, T* out,
+                          Stats* st) {
+    std::vector<uint64_t> off;
+    std::vector<uint32_t> lst;
+    stratify(regs, db, res, lo, hi, off, lst);
+    if (st) st->pairs = lst.size();
+    uint64_t nb = 1;
+    T drange[3];
+    for (int i = 0; i < db; ++i) {
+        nb *= res[i];
+        drange[i] = (hi[i] - lo[i]) / T(res[i]);
+    }
+    const uint64_t factor = nb;
+    std::vector<uint32_t> seeds(nb);
+    {
+        std::mt19937 master(seed);
+        for (uint64_t b = 0; b < nb; ++b) seeds[b] = uint32_t(master());
+    }
+    const uint64_t rows = res[0];
+    std::atomic<uint64_t> next_row{0};
+    auto work = [&]() {
+        std::vector<T> ilo, ihi;
+        for (;;) {
+            const uint64_t i0 = next_row.fetch_add(1);
+            if (i0 >= rows) break;
+            const uint64_t inner = nb / rows;
+            for (uint64_t q = 0; q < inner; ++q) {
+                uint64_t p[3] = {i0, 0, 0};
+                if (db > 1) p[1] = q % res[1];
+                if (db > 2) p[2] = q / res[1];
+                const uint64_t b = p[0] + (db > 1 ? res[0] * (p[1] + (db > 2 ? res[1] * p[2] : 0)) : 0);
+                std::mt19937 rng;
+                rng.seed(seeds[b]);
+                // monte_carlo_per_bin(rng, 1) binds the bin's engine by lvalue and seeds
+                // its own copy from one draw (…parallel-variance-reduction.h:69,
+                // src/monte-carlo/monte-carlo-per-bin.h:28); that draw is consumed here.
+                (void)rng();
+                T blo[D], bhi[D];
+                for (int d = 0; d < D; ++d) {
+                    blo[d] = lo[d];
+                    bhi[d] = hi[d];
+                }
+                for (int i = 0; i < db; ++i) {
+                    blo[i] = lo[i] + p[i] * drange[i];
+                    bhi[i] = lo[i] + (p[i] + 1) * drange[i];
+                }
+                const uint64_t n = off[b + 1] - off[b];
+                ilo.resize(n * D);
+                ihi.resize(n * D);
+                T approximation = 0;
+                for (uint64_t k = 0; k < n; ++k) {
+                    const Region<T, D>& r = regs[lst[off[b] + k]];
+                    bool empty = false;
+                    for (int d = 0; d < D; ++d) {
+                        const T a = ref_max(blo[d], r.lo[d]);
+                        ilo[k * D + d] = a;
+                        ihi[k * D + d] = ref_max(a, ref_min(bhi[d], r.hi[d]));
+                        if (ilo[k * D + d] >= ihi[k * D + d]) empty = true;
+                    }
+                    if (!empty) approximation += double(factor) * integral_subrange(r, &ilo[k * D], &ihi[k * D]);
+                }
+                Acc acc;
+                acc.set_alpha(alpha);
+                if (spp > 0 && n == 0) throw std::runtime_error("bin with no region (undefined in the reference)");
+                if (spp > 0) {
+                    std::uniform_int_distribution<std::size_t> rr(0, n - 1);
+                    for (uint64_t s = 0; s < spp; ++s) {
+                        const std::size_t chosen = rr(rng);
+                        const double rrfactor = double(rr.max() + 1);
+                        const Region<T, D>& r = regs[lst[off[b] + chosen]];
+                        const T* slo = &ilo[chosen * D];
+                        const T* shi = &ihi[chosen * D];
+                        T x[D];
+                        for (int d = 0; d < D; ++d) {
+                            std::uniform_real_distribution<T> dis(slo[d], shi[d]);
+                            x[d] = dis(rng);
+                        }
+                        const T sfactor = box_volume<T, D>(slo, shi);
+                        acc.push(T(f(x) * double(factor) * rrfactor * sfactor),
+                                 T(approximation_at(r, x) * double(factor) * rrfactor * sfactor));
+                    }
+                }
+                out[out_index(db, res, p)] = acc.integral(approximation);
+            }
+        }
+    };
+    const int nt = std::max(1, nthreads);
+    if (nt == 1) {
+        work();
+    } else {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; ++t) th.emplace_back(work);
+        for (auto& t : th) t.join();
+    }
+    if (st) st->evals_residual = nb * spp;
+}
+
+/* Control-variates form with alpha = 1, bins accumulated with +=, one RNG shared by
+   all bins (src/control-variates/regions-integrator-parallel-control-variates.h:17-66).
+   Bins are visited as for_each(parallel) visits them on one thread: dim-0 index outer,
+   remaining dims (dim 1 fastest) inner (src/foreach.h:56-67). */
+template <typename T, int D, typename F>
+void integrate_regions_cv(const std::vector<Region<T, D>>& regs, const F& f, const T* lo, const T* hi, int db,
+                          const uint64_t* res, uint64_t spp, uint64_t seed, T* out, Stats* st) {
+    std::vector<uint64_t> off;
+    std::vector<uint32_t> lst;
+    stratify(regs, db, res, lo, hi, off, lst);
+    if (st) st->pairs = lst.size();
+    uint64_t nb = 1;
+    T drange[3];
+    for (int i = 0; i < db; ++i) {
+        nb *= res[i];
+        drange[i] = (hi[i] - lo[i]) / T(res[i]);
+    }
+    const uint64_t factor = nb;
+    std::mt19937 rng(seed);
+    std::vector<T> ilo, ihi;
+    const uint64_t inner = nb / res[0];
+    for (uint64_t i0 = 0; i0 < res[0]; ++i0) {
+        for (uint64_t q = 0; q < inner; ++q) {
+            uint64_t p[3] = {i0, 0, 0};
+            if (db > 1) p[1] = q % res[1];
+            if (db > 2) p[2] = q / res[1];
+            const uint64_t b = p[0] + (db > 1 ? res[0] * (p[1] + (db > 2 ? res[1] * p[2] : 0)) : 0);
+            T& bin = out[out_index(db, res, p)];
+            T blo[D], bhi[D];
+            for (int d = 0; d < D; ++d) {
+                blo[d] = lo[d];
+                bhi[d] = hi[d];
+            }
+            for (int i = 0; i < db; ++i) {
+                blo[i] = lo[i] + p[i] * drange[i];
+                bhi[i] = lo[i] + (p[i] + 1) * drange[i];
+            }
+            const uint64_t n = off[b + 1] - off[b];
+            ilo.resize(n * D);
+            ihi.resize(n * D);
+            std::vector<char> empty(n, 0);
+            for (uint64_t k = 0; k < n; ++k) {
+                const Region<T, D>& r = regs[lst[off[b] + k]];
+                for (int d = 0; d < D; ++d) {
+                    const T a = ref_max(blo[d], r.lo[d]);
+                    ilo[k * D + d] = a;
+                    ihi[k * D + d] = ref_max(a, ref_min(bhi[d], r.hi[d]));
+                    if (ilo[k * D + d] >= ihi[k * D + d]) empty[k] = 1;
+                }
+                if (!empty[k]) bin += double(factor) * integral_subrange(r, &ilo[k * D], &ihi[k * D]);
+            }
+            if (spp > 0 && n == 0) throw std::runtime_error("bin with no region (undefined in the reference)");
+            std::uniform_int_distribution<std::size_t> rr(0, n - 1);
+            for (uint64_t s = 0; s < spp; ++s) {
+                const std::size_t chosen = rr(rng);
+                const Region<T, D>& r = regs[lst[off[b] + chosen]];
+                if (empty[chosen]) continue;
+                const T* slo = &ilo[chosen * D];
+                const T* shi = &ihi[chosen * D];
+                T x[D];
+                for (int d = 0; d < D; ++d) {
+                    std::uniform_real_distribution<T> dis(slo[d], shi[d]);
+                    x[d] = dis(rng);
+                }
+                bin += (f(x) - approximation_at(r, x)) * double(factor) * box_volume<T, D>(slo, shi) * double(n) /
+                       double(spp);
+            }
+        }
+    }
+    if (st) st->evals_residual = nb * spp;
+}
+
+/* ---- FP64 closed form for 2-D static blob family bins (truth for RMSE) ---- */
+
+inline double gauss_int(double a, double b, double c, double alpha) {
+    const double s = std::sqrt(alpha);
+    return 0.5 * std::sqrt(M_PI / alpha) * (std::erf(s * (b - c)) - std::erf(s * (a - c)));
+}
+
+/* Area of {(x, y) in [x0,x1]x[y0,y1] : nx*x + ny*y > d}: clip the rectangle against the
+   half-plane (Sutherland-Hodgman, one edge) and take the shoelace area. */
+inline double halfplane_rect_area(double x0, double x1, double y0, double y1, double nx, double ny, double d) {
+    const double px[4] = {x0, x1, x1, x0}, py[4] = {y0, y0, y1, y1};
+    double qx[8], qy[8];
+    int m = 0;
+    for (int i = 0; i < 4; ++i) {
+        const int j = (i + 1) % 4;
+        const double si = nx * px[i] + ny * py[i] - d, sj = nx * px[j] + ny * py[j] - d;
+        if (si > 0) {
+            qx[m] = px[i];
+            qy[m] = py[i];
+            ++m;
+        }
+        if ((si > 0) != (sj > 0)) {
+            const double t = si / (si - sj);
+            qx[m] = px[i] + t * (px[j] - px[i]);
+            qy[m] = py[i] + t * (py[j] - py[i]);
+            ++m;
+        }
+    }
+    double a = 0;
+    for (int i = 0; i < m; ++i) {
+        const int j = (i + 1) % m;
+        a += qx[i] * qy[j] - qx[j] * qy[i];
+    }
+    return std::abs(a) * 0.5;
+}
+
+/* Exact integral of a 2-D blob-family integrand (static blobs, step_mode 0 or 1) over a
+   rectangle. */
+inline double blobs2d_exact_rect(const vmx_blobs_params& p, double x0, double x1, double y0, double y1) {
+    double s = 0;
+    for (int k = 0; k < p.nblobs; ++k) {
+        const double* b = p.blob[k];
+        s += b[6] * gauss_int(x0, x1, b[0], b[4]) * gauss_int(y0, y1, b[1], b[5]);
+    }
+    const double A = p.wave[0], fx = p.wave[1], fy = p.wave[2];
+    if (A != 0) {
+        const double ix = (fx == 0) ? 0.0 : (std::cos(fx * x0) - std::cos(fx * x1)) / fx;
+        const double iy = (fy == 0) ? (y1 - y0) : (std::sin(fy * y1) - std::sin(fy * y0)) / fy;
+        s += A * ix * iy;
+    }
+    if (p.step_mode == 1) s += p.step[4] * halfplane_rect_area(x0, x1, y0, y1, p.step[0], p.step[1], p.step[3]);
+    return s;
+}
+
+}  // namespace vmo

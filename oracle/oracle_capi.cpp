@@ -1,0 +1,351 @@
+/*
+ * TEST INFRASTRUCTURE ONLY -- C entry points of the oracle (liboracle.so), called from
+ * tests/ and from bench.py's cpu_baseline leg through ctypes.  Never linked by the
+ * product.  Dispatches (scalar type, D, integrand family) onto oracle.hpp.
+ */
+#include <chrono>
+#include <cstdio>
+#include <string>
+
+#include "oracle.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+/* Integrand wrapper; optionally logs every evaluation (x, f(x)) in call order. */
+template <typename T, int D, typename P>
+struct Fn {
+    P p;
+    std::vector<double>* log = nullptr;
+    T operator()(const T* x) const {
+        const T y = p.template eval<vmo::HostMath, D>(x);
+        if (log) {
+            for (int d = 0; d < D; ++d) log->push_back(double(x[d]));
+            log->push_back(double(y));
+        }
+        return y;
+    }
+};
+
+template <typename T, int D, typename Body>
+int with_f(const vmx_integrand* f, Body&& body, std::vector<double>* log = nullptr) {
+    if (f->family == VMX_FAMILY_POLY) {
+        Fn<T, D, vmx::Poly<T>> fn{vmx::Poly<T>::from(*static_cast<const vmx_poly_params*>(f->params)), log};
+        return body(fn);
+    }
+    if constexpr (D == 2 || D == 3) {
+        if (f->family == VMX_FAMILY_BLOBS) {
+            Fn<T, D, vmx::Blobs<T>> fn{vmx::Blobs<T>::from(*static_cast<const vmx_blobs_params*>(f->params)), log};
+            return body(fn);
+        }
+    }
+    if constexpr (D == 5) {
+        if (f->family == VMX_FAMILY_LOBES5) {
+            Fn<T, D, vmx::Lobes5<T>> fn{vmx::Lobes5<T>::from(*static_cast<const vmx_lobes5_params*>(f->params)),
+                                        log};
+            return body(fn);
+        }
+    }
+    g_err = "unsupported integrand family for this dimension";
+    return -2;
+}
+
+template <template <typename, int> class Job, typename... A>
+int dispatch(int scalar_bytes, int D, A&&... a) {
+    try {
+#define VMO_CASE(DD)                                                              \
+    case DD:                                                                      \
+        return scalar_bytes == 4 ? Job<float, DD>::run(std::forward<A>(a)...)    \
+                                 : Job<double, DD>::run(std::forward<A>(a)...);
+        if (scalar_bytes != 4 && scalar_bytes != 8) {
+            g_err = "scalar_bytes must be 4 or 8";
+            return -1;
+        }
+        switch (D) {
+            VMO_CASE(1)
+            VMO_CASE(2)
+            VMO_CASE(3)
+            VMO_CASE(4)
+            VMO_CASE(5)
+            VMO_CASE(6)
+            default:
+                g_err = "D must be in 1..6";
+                return -1;
+        }
+#undef VMO_CASE
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -3;
+    }
+}
+
+template <typename T, int D>
+void load_box(const double* lo, const double* hi, T* tlo, T* thi) {
+    for (int d = 0; d < D; ++d) {
+        tlo[d] = T(lo[d]);
+        thi[d] = T(hi[d]);
+    }
+}
+
+template <typename T, int D>
+void store_regions(const std::vector<vmo::Region<T, D>>& regs, double* out_lo, double* out_hi, double* out_nodes,
+                   double* out_err, int32_t* out_dim) {
+    for (size_t r = 0; r < regs.size(); ++r) {
+        for (int d = 0; d < D; ++d) {
+            if (out_lo) out_lo[r * D + d] = regs[r].lo[d];
+            if (out_hi) out_hi[r * D + d] = regs[r].hi[d];
+        }
+        if (out_nodes)
+            for (int k = 0; k < vmo::Region<T, D>::K; ++k) out_nodes[r * vmo::Region<T, D>::K + k] = regs[r].v[k];
+        if (out_err) out_err[r] = regs[r].err;
+        if (out_dim) out_dim[r] = regs[r].dim;
+    }
+}
+
+template <typename T, int D>
+struct GenerateJob {
+    static int run(const vmx_integrand* f, const double* lo, const double* hi, int heuristic, uint64_t iterations,
+                   double* out_lo, double* out_hi, double* out_nodes, double* out_err, int32_t* out_dim,
+                   double* out_evals, uint64_t* out_nevals) {
+        T tlo[D], thi[D];
+        load_box<T, D>(lo, hi, tlo, thi);
+        std::vector<double> log;
+        int rc = with_f<T, D>(
+            f,
+            [&](const auto& fn) {
+                auto regs = vmo::generate<T, D>(fn, tlo, thi, heuristic, iterations);
+                store_regions(regs, out_lo, out_hi, out_nodes, out_err, out_dim);
+                return int(regs.size());
+            },
+            out_evals ? &log : nullptr);
+        if (out_evals) {
+            std::copy(log.begin(), log.end(), out_evals);
+            *out_nevals = log.size() / (D + 1);
+        }
+        return rc;
+    }
+};
+
+struct Timer {
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    double lap() {
+        auto t1 = std::chrono::steady_clock::now();
+        double s = std::chrono::duration<double>(t1 - t0).count();
+        t0 = t1;
+        return s;
+    }
+};
+
+template <typename T, int D, typename F>
+int run_regions(int integrator, double alpha, const F& fn, const std::vector<vmo::Region<T, D>>& regs, const T* tlo,
+                const T* thi, int db, const uint64_t* res, uint64_t spp, uint64_t seed, int nthreads,
+                double* bins_inout, vmo::Stats* st) {
+    uint64_t nb = 1;
+    for (int i = 0; i < db; ++i) nb *= res[i];
+    std::vector<T> out(nb);
+    for (uint64_t b = 0; b < nb; ++b) out[b] = T(bins_inout[b]);
+    if (integrator == vmo::INT_VR_OPTIMIZE)
+        vmo::integrate_regions_vr<T, D, F, vmo::AccOptimize<T>>(regs, fn, tlo, thi, db, res, spp, seed, alpha,
+                                                                  nthreads, out.data(), st);
+    else if (integrator == vmo::INT_VR_FIXED)
+        vmo::integrate_regions_vr<T, D, F, vmo::AccFixed<T>>(regs, fn, tlo, thi, db, res, spp, seed, alpha, nthreads,
+                                                               out.data(), st);
+    else if (integrator == vmo::INT_CV_ALPHA1)
+        vmo::integrate_regions_cv<T, D, F>(regs, fn, tlo, thi, db, res, spp, seed, out.data(), st);
+    else {
+        g_err = "unknown integrator";
+        return -1;
+    }
+    for (uint64_t b = 0; b < nb; ++b) bins_inout[b] = double(out[b]);
+    return 0;
+}
+
+template <typename T, int D>
+struct IntegrateJob {
+    static int run(int integrator, int heuristic, double alpha, const vmx_integrand* f, const double* lo,
+                   const double* hi, int db, const uint64_t* res, uint64_t iterations, uint64_t spp, uint64_t seed,
+                   int nthreads, double* bins_inout, vmo::Stats* st) {
+        if (db < 1 || db > 3 || db > D) {
+            g_err = "bin dims must be 1..min(3, D)";
+            return -1;
+        }
+        T tlo[D], thi[D];
+        load_box<T, D>(lo, hi, tlo, thi);
+        return with_f<T, D>(f, [&](const auto& fn) {
+            Timer tm;
+            uint64_t ev = 0;
+            auto regs = vmo::generate<T, D>(fn, tlo, thi, heuristic, iterations, &ev);
+            const double tg = tm.lap();
+            int rc = run_regions<T, D>(integrator, alpha, fn, regs, tlo, thi, db, res, spp, seed, nthreads,
+                                       bins_inout, st);
+            if (st) {
+                st->regions = regs.size();
+                st->evals_regions = ev;
+                st->t_generate = tg;
+                st->t_bins = tm.lap();
+            }
+            return rc;
+        });
+    }
+};
+
+template <typename T, int D>
+struct RegionsJob {
+    static int run(int integrator, double alpha, const vmx_integrand* f, const double* lo, const double* hi,
+                   uint64_t nregions, const double* reg_lo, const double* reg_hi, const double* reg_nodes, int db,
+                   const uint64_t* res, uint64_t spp, uint64_t seed, int nthreads, double* bins_inout,
+                   vmo::Stats* st) {
+        if (db < 1 || db > 3 || db > D) {
+            g_err = "bin dims must be 1..min(3, D)";
+            return -1;
+        }
+        T tlo[D], thi[D];
+        load_box<T, D>(lo, hi, tlo, thi);
+        std::vector<vmo::Region<T, D>> regs(nregions);
+        for (uint64_t r = 0; r < nregions; ++r) {
+            for (int d = 0; d < D; ++d) {
+                regs[r].lo[d] = T(reg_lo[r * D + d]);
+                regs[r].hi[d] = T(reg_hi[r * D + d]);
+            }
+            regs[r].vol = vmo::box_volume<T, D>(regs[r].lo, regs[r].hi);
+            for (int k = 0; k < vmo::Region<T, D>::K; ++k)
+                regs[r].v[k] = T(reg_nodes[r * vmo::Region<T, D>::K + k]);
+            regs[r].err = 0;
+            regs[r].dim = 0;
+        }
+        return with_f<T, D>(f, [&](const auto& fn) {
+            return run_regions<T, D>(integrator, alpha, fn, regs, tlo, thi, db, res, spp, seed, nthreads, bins_inout,
+                                     st);
+        });
+    }
+};
+
+template <typename T, int D>
+struct EvalJob {
+    static int run(const vmx_integrand* f, uint64_t n, const double* x, double* out) {
+        return with_f<T, D>(f, [&](const auto& fn) {
+            for (uint64_t i = 0; i < n; ++i) {
+                T p[D];
+                for (int d = 0; d < D; ++d) p[d] = T(x[i * D + d]);
+                out[i] = double(fn(p));
+            }
+            return 0;
+        });
+    }
+};
+
+/* Heuristic (err, dim) of one region given its box and nodes. */
+template <typename T, int D>
+struct HeuristicJob {
+    static int run(int heuristic, uint64_t n, const double* reg_lo, const double* reg_hi, const double* reg_nodes,
+                   double* out_err, int32_t* out_dim) {
+        for (uint64_t r = 0; r < n; ++r) {
+            vmo::Region<T, D> g;
+            for (int d = 0; d < D; ++d) {
+                g.lo[d] = T(reg_lo[r * D + d]);
+                g.hi[d] = T(reg_hi[r * D + d]);
+            }
+            g.vol = vmo::box_volume<T, D>(g.lo, g.hi);
+            for (int k = 0; k < vmo::Region<T, D>::K; ++k) g.v[k] = T(reg_nodes[r * vmo::Region<T, D>::K + k]);
+            vmo::apply_heuristic(g, heuristic);
+            out_err[r] = g.err;
+            out_dim[r] = g.dim;
+        }
+        return 0;
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+typedef struct vmo_stats {
+    uint64_t regions, pairs, evals_regions, evals_residual;
+    double t_generate, t_stratify, t_bins;
+} vmo_stats;
+
+static void copy_stats(const vmo::Stats& s, vmo_stats* o) {
+    if (!o) return;
+    o->regions = s.regions;
+    o->pairs = s.pairs;
+    o->evals_regions = s.evals_regions;
+    o->evals_residual = s.evals_residual;
+    o->t_generate = s.t_generate;
+    o->t_stratify = s.t_stratify;
+    o->t_bins = s.t_bins;
+}
+
+const char* vmo_last_error(void) { return g_err.c_str(); }
+
+/* Region list of RegionsGeneratorAdaptiveHeap::generate in heap-array order, and, if
+   out_evals is given, every integrand evaluation in call order as rows (x.., f(x)).
+   Returns the number of regions. */
+int vmo_generate(int scalar_bytes, const vmx_integrand* f, const double* lo, const double* hi, int heuristic,
+                 uint64_t iterations, double* out_lo, double* out_hi, double* out_nodes, double* out_err,
+                 int32_t* out_dim, double* out_evals, uint64_t* out_nevals) {
+    return dispatch<GenerateJob>(scalar_bytes, f->dim, f, lo, hi, heuristic, iterations, out_lo, out_hi, out_nodes,
+                                 out_err, out_dim, out_evals, out_nevals);
+}
+
+/* Full integrate: generate + per-bin CV + residual.  bins_inout uses C order
+   bins[i0][i1][i2]; it is read (the alpha=1 form accumulates) and written. */
+int vmo_integrate(int scalar_bytes, int integrator, int heuristic, double alpha, const vmx_integrand* f,
+                  const double* lo, const double* hi, int db, const uint64_t* res, uint64_t iterations,
+                  uint64_t spp, uint64_t seed, int nthreads, double* bins_inout, vmo_stats* st) {
+    vmo::Stats s;
+    int rc = dispatch<IntegrateJob>(scalar_bytes, f->dim, integrator, heuristic, alpha, f, lo, hi, db, res,
+                                    iterations, spp, seed, nthreads, bins_inout, &s);
+    copy_stats(s, st);
+    return rc;
+}
+
+/* Per-bin phase only, on a caller-given region list (any order). */
+int vmo_integrate_regions(int scalar_bytes, int integrator, double alpha, const vmx_integrand* f, const double* lo,
+                          const double* hi, uint64_t nregions, const double* reg_lo, const double* reg_hi,
+                          const double* reg_nodes, int db, const uint64_t* res, uint64_t spp, uint64_t seed,
+                          int nthreads, double* bins_inout, vmo_stats* st) {
+    vmo::Stats s;
+    int rc = dispatch<RegionsJob>(scalar_bytes, f->dim, integrator, alpha, f, lo, hi, nregions, reg_lo, reg_hi,
+                                  reg_nodes, db, res, spp, seed, nthreads, bins_inout, &s);
+    copy_stats(s, st);
+    return rc;
+}
+
+int vmo_eval(int scalar_bytes, const vmx_integrand* f, uint64_t n, const double* x, double* out) {
+    return dispatch<EvalJob>(scalar_bytes, f->dim, f, n, x, out);
+}
+
+int vmo_heuristic(int scalar_bytes, int D, int heuristic, uint64_t n, const double* reg_lo, const double* reg_hi,
+                  const double* reg_nodes, double* out_err, int32_t* out_dim) {
+    return dispatch<HeuristicJob>(scalar_bytes, D, heuristic, n, reg_lo, reg_hi, reg_nodes, out_err, out_dim);
+}
+
+/* FP64 closed-form bins for a 2-D static blob integrand over [lo, hi], C order. */
+int vmo_blobs2d_exact_bins(const vmx_blobs_params* p, const double* lo, const double* hi, uint64_t res0,
+                           uint64_t res1, double* out) {
+    for (int k = 0; k < p->nblobs; ++k)
+        if (p->blob[k][2] != 0 || p->blob[k][3] != 0) {
+            g_err = "closed form needs static blobs";
+            return -1;
+        }
+    if (p->step_mode == 2) {
+        g_err = "closed form needs step_mode 0 or 1";
+        return -1;
+    }
+    const double dx = (hi[0] - lo[0]) / double(res0), dy = (hi[1] - lo[1]) / double(res1);
+    for (uint64_t i = 0; i < res0; ++i)
+        for (uint64_t j = 0; j < res1; ++j) {
+            const double x0 = lo[0] + i * dx, x1 = lo[0] + (i + 1) * dx;
+            const double y0 = lo[1] + j * dy, y1 = lo[1] + (j + 1) * dy;
+            out[i * res1 + j] = double(res0 * res1) * vmo::blobs2d_exact_rect(*p, x0, x1, y0, y1);
+        }
+    return 0;
+}
+
+/* Exact integral over one rectangle (used to label per-region true errors). */
+double vmo_blobs2d_exact_rect(const vmx_blobs_params* p, double x0, double x1, double y0, double y1) {
+    return vmo::blobs2d_exact_rect(*p, x0, x1, y0, y1);
+}
+
+}  // extern "C"

@@ -1,0 +1,248 @@
+/*
+ * TEST INFRASTRUCTURE ONLY -- project-written C entry points around the *reference's
+ * own* headers (adolfomunoz/viltrum, header-only C++17), compiled in place from
+ * /root/reference by oracle/Makefile into oracle/_ref/libviltrum_ref.so.  No reference
+ * source is copied into this repository; this file only instantiates the reference's
+ * public templates on the project's integrand families:
+ *   - viltrum::regions_generator_adaptive_heap(...).generate  (region lists, heap order)
+ *   - viltrum::integrate(integrator, bins, res, f, range)     (bins)
+ * Used to write tests/golden fixtures and to cross-check the oracle restatement.
+ */
+#include "viltrum.h"
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../include/viltrum_amd/integrands.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+struct HostMath {
+    static float exp(float x) { return std::exp(x); }
+    static double exp(double x) { return std::exp(x); }
+    static float sin(float x) { return std::sin(x); }
+    static double sin(double x) { return std::sin(x); }
+    static float cos(float x) { return std::cos(x); }
+    static double cos(double x) { return std::cos(x); }
+};
+
+/* Integrand wrapper; optionally logs every evaluation (x, f(x)) in call order. */
+template <typename T, std::size_t D, typename P>
+struct Fn {
+    P p;
+    std::vector<double>* log = nullptr;
+    T operator()(const std::array<T, D>& x) const {
+        const T y = p.template eval<HostMath, int(D)>(x.data());
+        if (log) {
+            for (std::size_t d = 0; d < D; ++d) log->push_back(double(x[d]));
+            log->push_back(double(y));
+        }
+        return y;
+    }
+};
+
+template <typename T, std::size_t D, typename Body>
+int with_f(const vmx_integrand* f, Body&& body, std::vector<double>* log = nullptr) {
+    if (f->family == VMX_FAMILY_POLY)
+        return body(
+            Fn<T, D, vmx::Poly<T>>{vmx::Poly<T>::from(*static_cast<const vmx_poly_params*>(f->params)), log});
+    if constexpr (D == 2 || D == 3) {
+        if (f->family == VMX_FAMILY_BLOBS)
+            return body(
+                Fn<T, D, vmx::Blobs<T>>{vmx::Blobs<T>::from(*static_cast<const vmx_blobs_params*>(f->params)), log});
+    }
+    if constexpr (D == 5) {
+        if (f->family == VMX_FAMILY_LOBES5)
+            return body(Fn<T, D, vmx::Lobes5<T>>{
+                vmx::Lobes5<T>::from(*static_cast<const vmx_lobes5_params*>(f->params)), log});
+    }
+    g_err = "unsupported integrand family for this dimension";
+    return -2;
+}
+
+template <typename T, std::size_t D>
+viltrum::Range<T, D> make_range(const double* lo, const double* hi) {
+    std::array<T, D> a, b;
+    for (std::size_t d = 0; d < D; ++d) {
+        a[d] = T(lo[d]);
+        b[d] = T(hi[d]);
+    }
+    return viltrum::Range<T, D>(a, b);
+}
+
+template <typename Body>
+int pick_heuristic(int heuristic, Body&& body) {
+    if (heuristic == 0) return body(viltrum::error_heuristic_size(viltrum::error_metric_relative(), 1.e-5));
+    return body(viltrum::error_heuristic_default(viltrum::error_metric_absolute()));
+}
+
+template <typename T, std::size_t D>
+int generate(const vmx_integrand* f, const double* lo, const double* hi, int heuristic, uint64_t iterations,
+             double* out_lo, double* out_hi, double* out_err, int32_t* out_dim, double* out_evals,
+             uint64_t* out_nevals) {
+    auto range = make_range<T, D>(lo, hi);
+    std::vector<double> log;
+    int rc = with_f<T, D>(
+        f,
+        [&](const auto& fn) {
+            return pick_heuristic(heuristic, [&](const auto& eh) {
+                viltrum::LoggerNull lg;
+                auto gen = viltrum::regions_generator_adaptive_heap(
+                    viltrum::nested(viltrum::simpson, viltrum::trapezoidal), eh, std::size_t(iterations));
+                auto regs = gen.generate(std::array<std::size_t, 1>{1}, fn, range, lg);
+                for (std::size_t r = 0; r < regs.size(); ++r) {
+                    const auto& rg = regs[r];
+                    for (std::size_t d = 0; d < D; ++d) {
+                        out_lo[r * D + d] = rg.range().min(d);
+                        out_hi[r * D + d] = rg.range().max(d);
+                    }
+                    out_err[r] = double(std::get<0>(rg.extra()));
+                    out_dim[r] = int32_t(std::get<1>(rg.extra()));
+                }
+                return int(regs.size());
+            });
+        },
+        out_evals ? &log : nullptr);
+    if (out_evals) {
+        std::copy(log.begin(), log.end(), out_evals);
+        *out_nevals = log.size() / (D + 1);
+    }
+    return rc;
+}
+
+template <typename T, std::size_t D, std::size_t DB>
+int integrate_db(int integrator, int heuristic, double alpha, const vmx_integrand* f, const double* lo,
+                 const double* hi, const uint64_t* res, uint64_t iterations, uint64_t spp, uint64_t seed,
+                 double* bins_inout) {
+    auto range = make_range<T, D>(lo, hi);
+    std::array<std::size_t, DB> r;
+    std::size_t nb = 1;
+    for (std::size_t i = 0; i < DB; ++i) {
+        r[i] = res[i];
+        nb *= res[i];
+    }
+    std::vector<T> out(nb);
+    for (std::size_t b = 0; b < nb; ++b) out[b] = T(bins_inout[b]);
+    auto bins = [&](const std::array<std::size_t, DB>& p) -> T& {
+        std::size_t k = p[0];
+        for (std::size_t i = 1; i < DB; ++i) k = k * r[i] + p[i];
+        return out[k];
+    };
+    int rc = with_f<T, D>(f, [&](const auto& fn) {
+        return pick_heuristic(heuristic, [&](const auto& eh) {
+            auto gen = viltrum::regions_generator_adaptive_heap(viltrum::nested(viltrum::simpson, viltrum::trapezoidal),
+                                                                eh, std::size_t(iterations));
+            if (integrator == 0) {
+                auto in = viltrum::integrator_region_based(
+                    gen, viltrum::regions_integrator_parallel_variance_reduction(
+                             viltrum::rr_uniform_region(), viltrum::cv_optimize_weight(),
+                             viltrum::region_sampling_uniform(), std::mt19937(std::size_t(seed)), spp));
+                viltrum::integrate(in, bins, r, fn, range);
+            } else if (integrator == 1) {
+                auto in = viltrum::integrator_region_based(
+                    gen, viltrum::regions_integrator_parallel_variance_reduction(
+                             viltrum::rr_uniform_region(), viltrum::cv_fixed_weight(alpha),
+                             viltrum::region_sampling_uniform(), std::mt19937(std::size_t(seed)), spp));
+                viltrum::integrate(in, bins, r, fn, range);
+            } else {
+                auto in = viltrum::integrator_region_based(
+                    gen, viltrum::regions_integrator_parallel_control_variates(spp, std::size_t(seed)));
+                viltrum::integrate(in, bins, r, fn, range);
+            }
+            return 0;
+        });
+    });
+    for (std::size_t b = 0; b < nb; ++b) bins_inout[b] = double(out[b]);
+    return rc;
+}
+
+template <typename T, std::size_t D>
+int integrate_any(int integrator, int heuristic, double alpha, const vmx_integrand* f, const double* lo,
+                  const double* hi, int db, const uint64_t* res, uint64_t iterations, uint64_t spp, uint64_t seed,
+                  double* bins_inout) {
+    if (db == 1)
+        return integrate_db<T, D, 1>(integrator, heuristic, alpha, f, lo, hi, res, iterations, spp, seed, bins_inout);
+    if constexpr (D >= 2) {
+        if (db == 2)
+            return integrate_db<T, D, 2>(integrator, heuristic, alpha, f, lo, hi, res, iterations, spp, seed,
+                                         bins_inout);
+    }
+    if constexpr (D >= 3) {
+        if (db == 3)
+            return integrate_db<T, D, 3>(integrator, heuristic, alpha, f, lo, hi, res, iterations, spp, seed,
+                                         bins_inout);
+    }
+    g_err = "bin dims must be 1..min(3, D)";
+    return -1;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* vmr_last_error(void) { return g_err.c_str(); }
+
+/* Heap-array-ordered region list (ranges, err, dim) and, if out_evals is given, every
+   integrand evaluation in call order as rows (x_0..x_{D-1}, f(x)); the node values a
+   region holds are exactly these evaluations.  Returns the number of regions. */
+int vmr_generate(int scalar_bytes, const vmx_integrand* f, const double* lo, const double* hi, int heuristic,
+                 uint64_t iterations, double* out_lo, double* out_hi, double* out_err, int32_t* out_dim,
+                 double* out_evals, uint64_t* out_nevals) {
+    try {
+#define VMR_G(DD)                                                                                                  \
+    case DD:                                                                                                       \
+        return scalar_bytes == 4                                                                                   \
+                   ? generate<float, DD>(f, lo, hi, heuristic, iterations, out_lo, out_hi, out_err, out_dim,       \
+                                         out_evals, out_nevals)                                                    \
+                   : generate<double, DD>(f, lo, hi, heuristic, iterations, out_lo, out_hi, out_err, out_dim,      \
+                                          out_evals, out_nevals);
+        switch (f->dim) {
+            VMR_G(1)
+            VMR_G(2)
+            VMR_G(3)
+            VMR_G(4)
+            VMR_G(5)
+            default:
+                g_err = "D must be 1..5";
+                return -1;
+        }
+#undef VMR_G
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -3;
+    }
+}
+
+/* integrator: 0 integrator_crespo2021-style VR with cv_optimize_weight, 1 VR with
+   cv_fixed_weight(alpha), 2 the alpha=1 control-variates form (bins accumulated). */
+int vmr_integrate(int scalar_bytes, int integrator, int heuristic, double alpha, const vmx_integrand* f,
+                  const double* lo, const double* hi, int db, const uint64_t* res, uint64_t iterations, uint64_t spp,
+                  uint64_t seed, double* bins_inout) {
+    try {
+#define VMR_I(DD)                                                                                                  \
+    case DD:                                                                                                       \
+        return scalar_bytes == 4 ? integrate_any<float, DD>(integrator, heuristic, alpha, f, lo, hi, db, res,     \
+                                                            iterations, spp, seed, bins_inout)                     \
+                                 : integrate_any<double, DD>(integrator, heuristic, alpha, f, lo, hi, db, res,    \
+                                                             iterations, spp, seed, bins_inout);
+        switch (f->dim) {
+            VMR_I(1)
+            VMR_I(2)
+            VMR_I(3)
+            VMR_I(4)
+            VMR_I(5)
+            default:
+                g_err = "D must be 1..5";
+                return -1;
+        }
+#undef VMR_I
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -3;
+    }
+}
+
+}  // extern "C"

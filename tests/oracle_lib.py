@@ -1,0 +1,195 @@
+"""ctypes access to the checkers: the oracle (oracle/liboracle.so, a CPU restatement of
+the reference algorithm) and, when it has been built here, the reference itself
+(oracle/_ref/libviltrum_ref.so, the reference's own headers instantiated by
+oracle/ref_driver.cc).  Test infrastructure only."""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libviltrum_ref.so")
+
+INT_VR_OPTIMIZE, INT_VR_FIXED, INT_CV_ALPHA1 = 0, 1, 2
+HEUR_SIZE_RELATIVE, HEUR_DEFAULT_ABSOLUTE = 0, 1
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_up = C.POINTER(C.c_uint64)
+
+
+class OracleStats(C.Structure):
+    _fields_ = [("regions", C.c_uint64), ("pairs", C.c_uint64), ("evals_regions", C.c_uint64),
+                ("evals_residual", C.c_uint64), ("t_generate", C.c_double), ("t_stratify", C.c_double),
+                ("t_bins", C.c_double)]
+
+
+def _ptr(a, t=_dp):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def _load(path):
+    if not os.path.exists(path):
+        return None
+    return C.CDLL(path)
+
+
+_oracle = None
+_ref = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        lib = _load(ORACLE_SO)
+        if lib is None:
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        lib.vmo_last_error.restype = C.c_char_p
+        _oracle = lib
+    return _oracle
+
+
+def ref():
+    """The reference build, or None where it was not built (no /root/reference)."""
+    global _ref
+    if _ref is None:
+        lib = _load(REF_SO)
+        if lib is None:
+            return None
+        lib.vmr_last_error.restype = C.c_char_p
+        _ref = lib
+    return _ref
+
+
+def n_evals(D, iterations):
+    return 3 ** D + iterations * 2 * 3 ** (D - 1)
+
+
+class Regions:
+    def __init__(self, lo, hi, err, dim, nodes=None, evals=None):
+        self.lo, self.hi, self.err, self.dim, self.nodes, self.evals = lo, hi, err, dim, nodes, evals
+
+    def __len__(self):
+        return len(self.err)
+
+
+def _box(D, lo, hi):
+    lo = np.zeros(D) if lo is None else np.asarray(lo, dtype=np.float64)
+    hi = np.ones(D) if hi is None else np.asarray(hi, dtype=np.float64)
+    return np.ascontiguousarray(lo), np.ascontiguousarray(hi)
+
+
+def generate(f, iterations, scalar=8, heuristic=HEUR_SIZE_RELATIVE, lo=None, hi=None, use_ref=False, evals=False):
+    """Region list in the reference's heap-array order."""
+    D = f.dim
+    lo, hi = _box(D, lo, hi)
+    N = iterations + 1
+    olo, ohi = np.zeros(N * D), np.zeros(N * D)
+    err, dim = np.zeros(N), np.zeros(N, dtype=np.int32)
+    ev = np.zeros(n_evals(D, iterations) * (D + 1)) if evals else None
+    nev = C.c_uint64(0)
+    if use_ref:
+        lib = ref()
+        nodes = None
+        rc = lib.vmr_generate(C.c_int(scalar), f.ptr(), _ptr(lo), _ptr(hi), C.c_int(heuristic),
+                              C.c_uint64(iterations), _ptr(olo), _ptr(ohi), _ptr(err), _ptr(dim, _ip),
+                              _ptr(ev), C.byref(nev))
+        if rc < 0:
+            raise RuntimeError(lib.vmr_last_error().decode())
+    else:
+        lib = oracle()
+        nodes = np.zeros(N * 3 ** D)
+        rc = lib.vmo_generate(C.c_int(scalar), f.ptr(), _ptr(lo), _ptr(hi), C.c_int(heuristic),
+                              C.c_uint64(iterations), _ptr(olo), _ptr(ohi), _ptr(nodes), _ptr(err),
+                              _ptr(dim, _ip), _ptr(ev), C.byref(nev))
+        if rc < 0:
+            raise RuntimeError(lib.vmo_last_error().decode())
+        nodes = nodes.reshape(N, 3 ** D)
+    if ev is not None:
+        ev = ev[: nev.value * (D + 1)].reshape(-1, D + 1)
+    return Regions(olo.reshape(N, D), ohi.reshape(N, D), err, dim, nodes, ev)
+
+
+def integrate(f, res, iterations, spp, seed, scalar=8, integrator=INT_VR_OPTIMIZE, heuristic=HEUR_SIZE_RELATIVE,
+              alpha=1.0, lo=None, hi=None, nthreads=1, use_ref=False, bins_in=None):
+    """Bins in C order (bins[i0][i1]); returns (bins, stats or None)."""
+    D = f.dim
+    lo, hi = _box(D, lo, hi)
+    res = tuple(int(r) for r in res)
+    r = np.ascontiguousarray(np.array(res, dtype=np.uint64))
+    bins = np.zeros(int(np.prod(res))) if bins_in is None else np.array(bins_in, dtype=np.float64).ravel().copy()
+    if use_ref:
+        lib = ref()
+        rc = lib.vmr_integrate(C.c_int(scalar), C.c_int(integrator), C.c_int(heuristic), C.c_double(alpha), f.ptr(),
+                               _ptr(lo), _ptr(hi), C.c_int(len(res)), _ptr(r, _up), C.c_uint64(iterations),
+                               C.c_uint64(spp), C.c_uint64(seed), _ptr(bins))
+        if rc < 0:
+            raise RuntimeError(lib.vmr_last_error().decode())
+        return bins.reshape(res), None
+    lib = oracle()
+    st = OracleStats()
+    rc = lib.vmo_integrate(C.c_int(scalar), C.c_int(integrator), C.c_int(heuristic), C.c_double(alpha), f.ptr(),
+                           _ptr(lo), _ptr(hi), C.c_int(len(res)), _ptr(r, _up), C.c_uint64(iterations),
+                           C.c_uint64(spp), C.c_uint64(seed), C.c_int(nthreads), _ptr(bins), C.byref(st))
+    if rc < 0:
+        raise RuntimeError(lib.vmo_last_error().decode())
+    return bins.reshape(res), st
+
+
+def integrate_regions(f, regs_lo, regs_hi, regs_nodes, res, spp, seed, scalar=8, integrator=INT_VR_OPTIMIZE,
+                      alpha=1.0, lo=None, hi=None, nthreads=1, bins_in=None):
+    """Per-bin phase of the oracle on a caller-given region list."""
+    D = f.dim
+    lo, hi = _box(D, lo, hi)
+    res = tuple(int(x) for x in res)
+    r = np.ascontiguousarray(np.array(res, dtype=np.uint64))
+    bins = np.zeros(int(np.prod(res))) if bins_in is None else np.array(bins_in, dtype=np.float64).ravel().copy()
+    rl = np.ascontiguousarray(regs_lo, dtype=np.float64)
+    rh = np.ascontiguousarray(regs_hi, dtype=np.float64)
+    rn = np.ascontiguousarray(regs_nodes, dtype=np.float64)
+    st = OracleStats()
+    lib = oracle()
+    rc = lib.vmo_integrate_regions(C.c_int(scalar), C.c_int(integrator), C.c_double(alpha), f.ptr(), _ptr(lo),
+                                   _ptr(hi), C.c_uint64(rl.shape[0]), _ptr(rl), _ptr(rh), _ptr(rn),
+                                   C.c_int(len(res)), _ptr(r, _up), C.c_uint64(spp), C.c_uint64(seed),
+                                   C.c_int(nthreads), _ptr(bins), C.byref(st))
+    if rc < 0:
+        raise RuntimeError(lib.vmo_last_error().decode())
+    return bins.reshape(res), st
+
+
+def evaluate(f, x, scalar=8):
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, f.dim)
+    out = np.zeros(x.shape[0])
+    lib = oracle()
+    rc = lib.vmo_eval(C.c_int(scalar), f.ptr(), C.c_uint64(x.shape[0]), _ptr(x), _ptr(out))
+    if rc < 0:
+        raise RuntimeError(lib.vmo_last_error().decode())
+    return out
+
+
+def heuristic(D, lo, hi, nodes, scalar=8, heuristic=HEUR_SIZE_RELATIVE):
+    lo = np.ascontiguousarray(lo, dtype=np.float64).reshape(-1, D)
+    hi = np.ascontiguousarray(hi, dtype=np.float64).reshape(-1, D)
+    nodes = np.ascontiguousarray(nodes, dtype=np.float64).reshape(lo.shape[0], -1)
+    err = np.zeros(lo.shape[0])
+    dim = np.zeros(lo.shape[0], dtype=np.int32)
+    lib = oracle()
+    rc = lib.vmo_heuristic(C.c_int(scalar), C.c_int(D), C.c_int(heuristic), C.c_uint64(lo.shape[0]), _ptr(lo),
+                           _ptr(hi), _ptr(nodes), _ptr(err), _ptr(dim, _ip))
+    if rc < 0:
+        raise RuntimeError(lib.vmo_last_error().decode())
+    return err, dim
+
+
+def blobs2d_exact_bins(f, res0, res1, lo=(0.0, 0.0), hi=(1.0, 1.0)):
+    lib = oracle()
+    lo = np.ascontiguousarray(lo, dtype=np.float64)
+    hi = np.ascontiguousarray(hi, dtype=np.float64)
+    out = np.zeros(res0 * res1)
+    rc = lib.vmo_blobs2d_exact_bins(C.byref(f.params), _ptr(lo), _ptr(hi), C.c_uint64(res0), C.c_uint64(res1),
+                                    _ptr(out))
+    if rc < 0:
+        raise RuntimeError(lib.vmo_last_error().decode())
+    return out.reshape(res0, res1)
