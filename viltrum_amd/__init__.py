@@ -1,0 +1,251 @@
+"""viltrum_amd -- MI355X per-bin control-variate integration.
+
+Python mirror of the reference's call surface over the C ABI (include/viltrum_amd.h):
+
+    integrate(integrator, bins, f, rng)        src/integrate.h:72-79, 139-148
+    integrate(integrator, f, rng) -> value     src/integrate.h:105-130
+    integrator_crespo2021(iterations, spp, seed)
+                                               src/control-variates/integrator-crespo2021.h:7-22
+    integrator_adaptive_control_variates_parallel(heuristic, iterations, spp, seed)
+                                               src/control-variates/integrator-adaptive-control-variates.h:17-20
+    integrator_variance_reduction_fixed(alpha, iterations, spp, seed, heuristic)
+                                               regions_integrator_parallel_variance_reduction(rr, alpha, ...)
+                                               src/control-variates/regions-integrator-parallel-variance-reduction.h:127-130
+
+`bins` is a numpy array whose shape is the bin resolution (1 to 3 dims, C order like
+bins[i0][i1]); float32 bins run the FP32 production path, float64 bins the FP64 path.
+Integrands are descriptors from viltrum_amd.integrands (a GPU cannot call a Python
+function).  Every call runs on the GPU through libviltrum_amd.so; there is no CPU path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import integrands
+from .integrands import Integrand
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libviltrum_amd.so")
+
+F32, F64 = 4, 8
+HEUR_SIZE_RELATIVE, HEUR_DEFAULT_ABSOLUTE = 0, 1
+FORM_VR_OPTIMIZE, FORM_VR_FIXED, FORM_CV_ALPHA1 = 0, 1, 2
+ORDER_CANONICAL, ORDER_REFERENCE = 0, 1
+REFINER_HEURISTIC, REFINER_LEARNED = 0, 1
+
+SYMBOLS = ["vmx_ctx_create", "vmx_ctx_destroy", "vmx_last_error", "vmx_integrate", "vmx_integrate_device",
+           "vmx_export_regions", "vmx_eval"]
+
+
+class VmxError(RuntimeError):
+    pass
+
+
+class Problem(C.Structure):
+    _fields_ = [
+        ("scalar", C.c_int32), ("bin_dims", C.c_int32), ("heuristic", C.c_int32), ("form", C.c_int32),
+        ("order", C.c_int32), ("refiner", C.c_int32), ("alpha", C.c_double),
+        ("lo", C.c_double * 6), ("hi", C.c_double * 6), ("res", C.c_uint64 * 3),
+        ("regions", C.c_uint64), ("spp", C.c_uint64), ("seed", C.c_uint64),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("regions", C.c_uint64), ("nodes", C.c_uint64), ("rounds", C.c_uint64), ("pairs", C.c_uint64),
+        ("footprints", C.c_uint64), ("evals_regions", C.c_uint64), ("evals_residual", C.c_uint64),
+        ("ties_unresolved", C.c_uint64), ("ms_generate", C.c_double), ("ms_select", C.c_double),
+        ("ms_stratify", C.c_double), ("ms_bins", C.c_double), ("ms_total", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """The HIP library; raises if it has not been built (no fallback exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise VmxError("libviltrum_amd.so not built: run `python -m viltrum_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        L.vmx_ctx_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
+        L.vmx_ctx_destroy.argtypes = [C.c_void_p]
+        L.vmx_last_error.argtypes = [C.c_void_p]
+        L.vmx_last_error.restype = C.c_char_p
+        L.vmx_integrate.argtypes = [C.c_void_p, C.POINTER(Problem), C.c_void_p, C.c_void_p, C.POINTER(Stats)]
+        L.vmx_integrate_device.argtypes = [C.c_void_p, C.POINTER(Problem), C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.POINTER(Stats)]
+        L.vmx_export_regions.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_void_p]
+        L.vmx_eval.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_uint64, C.c_void_p, C.c_void_p]
+        _lib = L
+    return _lib
+
+
+class Context:
+    """Owns device buffers and a stream on one GPU (vmx_ctx)."""
+
+    def __init__(self, device=0):
+        self.ptr = C.c_void_p()
+        rc = lib().vmx_ctx_create(device, C.byref(self.ptr))
+        if rc != 0:
+            raise VmxError("vmx_ctx_create failed (%d): no usable GPU %d" % (rc, device))
+
+    def close(self):
+        if self.ptr:
+            lib().vmx_ctx_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise VmxError("%s (status %d)" % (lib().vmx_last_error(self.ptr).decode(), rc))
+
+    def integrate_host(self, problem, f, bins):
+        """bins: contiguous numpy array of the evaluation type, filled in place."""
+        st = Stats()
+        self._check(lib().vmx_integrate(self.ptr, C.byref(problem), C.cast(f.ptr(), C.c_void_p),
+                                        bins.ctypes.data_as(C.c_void_p), C.byref(st)))
+        return st
+
+    def integrate_device(self, problem, f, bins_ptr, stream=None):
+        st = Stats()
+        self._check(lib().vmx_integrate_device(self.ptr, C.byref(problem), C.cast(f.ptr(), C.c_void_p),
+                                               C.c_void_p(bins_ptr), C.c_void_p(stream or 0), C.byref(st)))
+        return st
+
+    def export_regions(self, D):
+        n = C.c_uint64(0)
+        L = lib()
+        self._check(L.vmx_export_regions(self.ptr, C.byref(n), None, None, None, None, None))
+        N = n.value
+        lo, hi = np.zeros((N, D)), np.zeros((N, D))
+        nodes, err = np.zeros((N, 3 ** D)), np.zeros(N)
+        dim = np.zeros(N, dtype=np.int32)
+        self._check(L.vmx_export_regions(self.ptr, C.byref(n), lo.ctypes.data_as(C.c_void_p),
+                                         hi.ctypes.data_as(C.c_void_p), nodes.ctypes.data_as(C.c_void_p),
+                                         err.ctypes.data_as(C.c_void_p), dim.ctypes.data_as(C.c_void_p)))
+        return dict(lo=lo, hi=hi, nodes=nodes, err=err, dim=dim)
+
+    def eval(self, f, x, scalar=F64):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, f.dim)
+        out = np.zeros(x.shape[0])
+        self._check(lib().vmx_eval(self.ptr, C.cast(f.ptr(), C.c_void_p), scalar, x.shape[0],
+                                   x.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p)))
+        return out
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class Range:
+    """Box [lo, hi] (src/range.h:17-139)."""
+
+    def __init__(self, lo, hi):
+        self.lo = [float(v) for v in lo]
+        self.hi = [float(v) for v in hi]
+        if len(self.lo) != len(self.hi):
+            raise ValueError("lo/hi dims differ")
+
+    @property
+    def dims(self):
+        return len(self.lo)
+
+
+def range_primary(D):
+    """[0,1]^D (src/range.h:373-376)."""
+    return Range([0.0] * D, [1.0] * D)
+
+
+class Integrator:
+    """Parameters of a region-based CV integrator (IntegratorRegionBased,
+    src/newton-cotes/integrator-region-based.h:5-29)."""
+
+    def __init__(self, iterations, spp, seed, heuristic=HEUR_SIZE_RELATIVE, form=FORM_VR_OPTIMIZE, alpha=1.0,
+                 order=ORDER_CANONICAL, refiner=REFINER_HEURISTIC):
+        self.iterations = int(iterations)
+        self.spp = int(spp)
+        self.seed = int(seed)
+        self.heuristic = heuristic
+        self.form = form
+        self.alpha = float(alpha)
+        self.order = order
+        self.refiner = refiner
+        self.last_stats = None
+
+    def problem(self, f, res, rng, scalar):
+        if rng.dims != f.dim:
+            raise ValueError("range has %d dims, integrand %d" % (rng.dims, f.dim))
+        if not 1 <= len(res) <= min(3, f.dim):
+            raise ValueError("bins must have 1..min(3, D) dims")
+        p = Problem()
+        p.scalar = scalar
+        p.bin_dims = len(res)
+        p.heuristic = self.heuristic
+        p.form = self.form
+        p.order = self.order
+        p.refiner = self.refiner
+        p.alpha = self.alpha
+        for d in range(f.dim):
+            p.lo[d] = rng.lo[d]
+            p.hi[d] = rng.hi[d]
+        for i, r in enumerate(res):
+            p.res[i] = int(r)
+        p.regions = self.iterations + 1
+        p.spp = self.spp
+        p.seed = self.seed
+        return p
+
+
+def integrator_crespo2021(iterations, spp, seed=0, **kw):
+    """Nested Simpson/trapezoid regions, error_heuristic_size(relative, 1e-5), per-bin
+    residual with cv_optimize_weight (src/control-variates/integrator-crespo2021.h:7-22)."""
+    return Integrator(iterations, spp, seed, HEUR_SIZE_RELATIVE, FORM_VR_OPTIMIZE, **kw)
+
+
+def integrator_adaptive_control_variates_parallel(iterations, spp, seed=0, heuristic=HEUR_DEFAULT_ABSOLUTE, **kw):
+    """alpha = 1 control variates, bins accumulated
+    (src/control-variates/integrator-adaptive-control-variates.h:17-20)."""
+    return Integrator(iterations, spp, seed, heuristic, FORM_CV_ALPHA1, **kw)
+
+
+def integrator_variance_reduction_fixed(alpha, iterations, spp, seed=0, heuristic=HEUR_SIZE_RELATIVE, **kw):
+    """cv_fixed_weight(alpha) residual
+    (src/control-variates/regions-integrator-parallel-variance-reduction.h:127-130)."""
+    return Integrator(iterations, spp, seed, heuristic, FORM_VR_FIXED, alpha=alpha, **kw)
+
+
+def integrate(integrator, *args, ctx=None):
+    """integrate(integrator, bins, f, range)  -> fills bins (numpy, float32/float64)
+    integrate(integrator, f, range[, dtype])  -> scalar (one bin)."""
+    ctx = ctx or default_context()
+    if isinstance(args[0], np.ndarray):
+        bins, f, rng = args[:3]
+        if bins.dtype not in (np.float32, np.float64) or not bins.flags.c_contiguous:
+            raise TypeError("bins must be a C-contiguous float32 or float64 array")
+        scalar = F32 if bins.dtype == np.float32 else F64
+        p = integrator.problem(f, bins.shape, rng, scalar)
+        integrator.last_stats = ctx.integrate_host(p, f, bins)
+        return bins
+    f, rng = args[:2]
+    dtype = args[2] if len(args) > 2 else np.float64
+    one = np.zeros(1, dtype=dtype)
+    integrate(integrator, one, f, rng, ctx=ctx)
+    return float(one[0])
