@@ -1,0 +1,143 @@
+#!/usr/bin/env python3
+"""Benchmark of the per-bin control-variate path on one MI355X.
+
+Step = one full integration (adaptive regions, control-variate integral and residual
+Monte Carlo per bin) of the workload, integrand parameters already resident and bins
+written to device memory.  Prints ONE JSON line.
+
+Workload (default): 3-D integrand (2 bin dims + time) over [0,1]^3 into 512x512 FP32
+bins, 32 moving anisotropic Gaussian blobs times a seeded step occluder, 16384 regions,
+64 residual samples per bin, seed 0 -- the plain GPU path.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "bins/sec at fixed RMSE vs FP64 oracle: learned+batched vs plain GPU path"
+PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 peak
+PEAK_HBM_GBS = 8000.0
+
+
+def workloads():
+    from viltrum_amd import integrands as I
+    return {
+        "config2": dict(desc="3-D moving-blob integrand (32 blobs x step occluder), 512x512 fp32 bins, "
+                             "16384 regions, 64 residual spp, seed 0, plain GPU path",
+                        f=I.config2(), res=(512, 512), regions=16384, spp=64, seed=0),
+        "config1": dict(desc="2-D config-1 integrand, 64x64 fp32 bins, 1024 regions, 16 spp, seed 0",
+                        f=I.config1(), res=(64, 64), regions=1024, spp=16, seed=0),
+    }
+
+
+def cpu_baseline(w, nthreads):
+    """The oracle (C++ restatement of integrator_crespo2021, FP32, per-bin mt19937,
+    threads over dim-0 rows) on the host cores: median of 3 full runs."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        O.integrate(w["f"], w["res"], w["regions"] - 1, w["spp"], w["seed"], scalar=4,
+                    integrator=O.INT_VR_OPTIMIZE, heuristic=O.HEUR_SIZE_RELATIVE, nthreads=nthreads)
+        times.append(time.perf_counter() - t0)
+    t = sorted(times)[1]
+    nb = int(np.prod(w["res"]))
+    return {"value": nb / t, "unit": "bins/s", "cores": nthreads, "kind": "port",
+            "sample": "full workload (%d bins, %d regions, %d spp), median of 3 runs, %.2f s each"
+                      % (nb, w["regions"], w["spp"], t)}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="config2")
+    ap.add_argument("--truth-spp", type=int, default=1024)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import viltrum_amd as V
+    from viltrum_amd import hip
+    from viltrum_amd.integrands import flops_per_eval
+
+    w = workloads()[args.workload]
+    f = w["f"]
+    ctx = V.Context(0)
+    integ = V.integrator_crespo2021(w["regions"] - 1, w["spp"], seed=w["seed"])
+    prob = integ.problem(f, w["res"], V.range_primary(f.dim), V.F32)
+    nb = int(np.prod(w["res"]))
+    bins = hip.DeviceBuffer(nb * 4)
+    stream = hip.Stream()
+
+    for _ in range(args.warmup):
+        ctx.integrate_device(prob, f, bins.address, stream.address)
+    stream.synchronize()
+    e0, e1 = hip.Event(), hip.Event()
+    stats = []
+    hip.device_synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        stats.append(ctx.integrate_device(prob, f, bins.address, stream.address).as_dict())
+    e1.record(stream)
+    e1.synchronize()
+    hip.device_synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = max(wall * 1e3, e0.elapsed_ms(e1)) / args.steps
+
+    out = np.zeros(w["res"], dtype=np.float32)
+    bins.to_host(out)
+    truth, se = ctx.integrate_mc(f, w["res"], args.truth_spp, seed=12345, stratified=True)
+    rmse = float(np.sqrt(np.mean((out.astype(np.float64) - truth) ** 2)))
+    truth_se = float(np.sqrt(np.mean(se ** 2)))
+
+    st = stats[-1]
+    flops, special = flops_per_eval(f)
+    ms_k = float(np.mean([s["ms_bins"] for s in stats]))
+    achieved = st["evals_residual"] * flops / (ms_k * 1e-3) / 1e12
+    evals = st["evals_regions"] + st["evals_residual"]
+    res = {
+        "metric": METRIC,
+        "value": nb / (ms_step * 1e-3),
+        "unit": "bins/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": w["desc"], "bins": list(w["res"]), "regions": w["regions"], "spp": w["spp"],
+                   "path": "plain"},
+        "evals_per_s": evals / (ms_step * 1e-3),
+        "rmse_vs_fp64": rmse,
+        "fp64_truth": {"kind": "GPU FP64 stratified per-bin MC", "spp": args.truth_spp, "stderr_rms": truth_se},
+        "phase_ms": {k: float(np.mean([s[k] for s in stats])) for k in
+                     ("ms_generate", "ms_select", "ms_stratify", "ms_bins", "ms_total")},
+        "roofline": {"bound": "valu", "kernel": "k_bins (residual Monte Carlo)", "achieved": achieved,
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": None, "flops_per_eval": flops, "special_per_eval": special,
+                     "kernel_ms": ms_k},
+        "stats": {k: st[k] for k in ("regions", "nodes", "rounds", "pairs", "evals_regions", "evals_residual",
+                                     "ties_unresolved")},
+    }
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline(w, min(16, os.cpu_count() or 1))
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

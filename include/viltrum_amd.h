@@ -25,6 +25,10 @@
  *       the region list the reference hands to Logger::log(seq_regions)
  *       src/newton-cotes/integrator-region-based.h:19, i.e. the output of
  *       RegionsGeneratorAdaptiveHeap::generate src/nested/regions-generator-adaptive-heap.h:18-45.
+ *   vmx_integrate_mc
+ *       viltrum::integrate(monte_carlo_per_bin(rng, spp), bins, resolution, f, range):
+ *       MonteCarloPerBin::integrate src/monte-carlo/monte-carlo-per-bin.h:40-65 (plain
+ *       per-bin Monte Carlo; here it also provides the FP64 truth for RMSE).
  *   vmx_eval
  *       the integrand call f(x) of src/integrate.h:11-70 (device evaluation, for tests).
  *
@@ -127,6 +131,13 @@ int vmx_integrate_device(vmx_ctx* ctx, const vmx_problem* p, const vmx_integrand
    Pass NULL arrays to query *n.  lo/hi: n*D, nodes: n*3^D, err: n, dim: n. */
 int vmx_export_regions(vmx_ctx* ctx, uint64_t* n, double* lo, double* hi, double* nodes, double* err,
                        int32_t* dim);
+
+/* Plain per-bin Monte Carlo with p->spp samples per bin (p->scalar, p->seed, range,
+   bin_dims, res; region fields ignored): bins = volume * mean f over the bin's box.
+   stratified != 0 jitters samples over a g x g grid of the first two bin dims.
+   stderr_out (optional, FP64) receives each bin's standard error.  Host arrays. */
+int vmx_integrate_mc(vmx_ctx* ctx, const vmx_problem* p, const vmx_integrand* f, int32_t stratified, void* bins,
+                     double* stderr_out);
 
 /* Evaluate the integrand on the device at n points x (n*D, row-major). */
 int vmx_eval(vmx_ctx* ctx, const vmx_integrand* f, int32_t scalar, uint64_t n, const double* x, double* out);

@@ -35,7 +35,7 @@ ORDER_CANONICAL, ORDER_REFERENCE = 0, 1
 REFINER_HEURISTIC, REFINER_LEARNED = 0, 1
 
 SYMBOLS = ["vmx_ctx_create", "vmx_ctx_destroy", "vmx_last_error", "vmx_integrate", "vmx_integrate_device",
-           "vmx_export_regions", "vmx_eval"]
+           "vmx_export_regions", "vmx_integrate_mc", "vmx_eval"]
 
 
 class VmxError(RuntimeError):
@@ -83,6 +83,8 @@ def lib():
         L.vmx_export_regions.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_void_p, C.c_void_p]
         L.vmx_eval.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_uint64, C.c_void_p, C.c_void_p]
+        L.vmx_integrate_mc.argtypes = [C.c_void_p, C.POINTER(Problem), C.c_void_p, C.c_int32, C.c_void_p,
+                                       C.c_void_p]
         _lib = L
     return _lib
 
@@ -136,6 +138,16 @@ class Context:
                                          hi.ctypes.data_as(C.c_void_p), nodes.ctypes.data_as(C.c_void_p),
                                          err.ctypes.data_as(C.c_void_p), dim.ctypes.data_as(C.c_void_p)))
         return dict(lo=lo, hi=hi, nodes=nodes, err=err, dim=dim)
+
+    def integrate_mc(self, f, res, spp, seed, rng=None, dtype=np.float64, stratified=True):
+        """Plain per-bin Monte Carlo (monte_carlo_per_bin); returns (bins, stderr)."""
+        rng = rng or range_primary(f.dim)
+        p = Integrator(0, spp, seed).problem(f, res, rng, F32 if dtype == np.float32 else F64)
+        bins = np.zeros(res, dtype=dtype)
+        se = np.zeros(res, dtype=np.float64)
+        self._check(lib().vmx_integrate_mc(self.ptr, C.byref(p), C.cast(f.ptr(), C.c_void_p), int(stratified),
+                                           bins.ctypes.data_as(C.c_void_p), se.ctypes.data_as(C.c_void_p)))
+        return bins, se
 
     def eval(self, f, x, scalar=F64):
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, f.dim)

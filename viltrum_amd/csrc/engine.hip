@@ -39,6 +39,7 @@ struct vmx_ctx {
         size_t bytes = 0;
     };
     Buf tree, regs, lists, misc, params, bins;
+    hipEvent_t ev[8] = {};
     // last result (for vmx_export_regions): gathered from device memory on demand
     int last_D = 0;
     uint64_t last_n = 0;
@@ -399,12 +400,17 @@ int run(vmx_ctx* ctx, const vmx_problem& p, const void* fparams, T* bins_dev, hi
     ba.form = p.form;
     ba.alpha = p.alpha;
     HIP_CHECK(hipMemsetAsync(d_flags, 0, sizeof(int32_t), st));
+    HIP_CHECK(hipEventRecord(ctx->ev[0], st));
     vmx::k_bins<T, D, P, Math><<<bblocks, 256, 0, st>>>(ba, rg, d_off, d_lst, d_f, bins_dev, d_flags);
     HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventRecord(ctx->ev[1], st));
     int32_t flags = 0;
     HIP_CHECK(hipMemcpyAsync(&flags, d_flags, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
-    const double ms_bins = ms_since(t_bins);
+    float ms_kernel_bins = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms_kernel_bins, ctx->ev[0], ctx->ev[1]));
+    const double ms_bins = ms_kernel_bins;
+    (void)t_bins;
     if (flags & vmx::FLAG_EMPTY_BIN) throw VmxError(VMX_ERR_NUMERIC, "a bin overlaps no region while spp > 0");
 
     // region list export, gathered lazily from the device buffers of this call
@@ -576,9 +582,106 @@ int eval_T(vmx_ctx* ctx, const vmx_integrand& f, uint64_t n, const double* x, do
 }
 }  // namespace
 
+template <typename T, typename Math, int D, typename P>
+int mc_run(vmx_ctx* ctx, const vmx_problem& p, const void* fparams, int stratified, void* bins, double* stderr_out) {
+    const int db = p.bin_dims;
+    uint64_t nb = 1;
+    for (int i = 0; i < db; ++i) nb *= p.res[i];
+    const P fp = P::from(*static_cast<const typename ParamOf<P>::type*>(fparams));
+    vmx::BinArgs<T> ba{};
+    double vtot = 1;
+    for (int d = 0; d < D; ++d) {
+        ba.lo[d] = T(p.lo[d]);
+        ba.hi[d] = T(p.hi[d]);
+        vtot *= p.hi[d] - p.lo[d];
+    }
+    for (int i = 0; i < 3; ++i) {
+        ba.res[i] = i < db ? p.res[i] : 1;
+        ba.drange[i] = i < db ? (ba.hi[i] - ba.lo[i]) / T(p.res[i]) : T(0);
+    }
+    ba.nbins = nb;
+    ba.seed_lo = uint32_t(p.seed);
+    ba.seed_hi = uint32_t(p.seed >> 32);
+    ba.db = db;
+    uint32_t grid = 1;
+    if (stratified && db >= 2) {
+        grid = uint32_t(std::sqrt(double(p.spp)));
+        while (grid > 1 && p.spp % (uint64_t(grid) * grid) != 0) --grid;
+    }
+    Carve c(ensure(ctx->misc, sizeof(P) + sizeof(double) * nb * 2 + 4096));
+    P* d_f = c.take<P>(1);
+    double* d_s = c.take<double>(nb);
+    double* d_s2 = c.take<double>(nb);
+    hipStream_t st = ctx->stream;
+    HIP_CHECK(hipMemcpyAsync(d_f, &fp, sizeof(P), hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipMemsetAsync(d_s, 0, sizeof(double) * nb, st));
+    HIP_CHECK(hipMemsetAsync(d_s2, 0, sizeof(double) * nb, st));
+    const unsigned blocks = unsigned((nb + 255) / 256);
+    const uint64_t chunk = 64;
+    for (uint64_t s0 = 0; s0 < p.spp; s0 += chunk) {
+        const uint64_t cnt = std::min(chunk, p.spp - s0);
+        vmx::k_mc<T, D, P, Math><<<blocks, 256, 0, st>>>(ba, d_f, s0, cnt, grid, d_s, d_s2);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(st));
+    }
+    std::vector<double> s(nb), s2(nb);
+    HIP_CHECK(hipMemcpyAsync(s.data(), d_s, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipMemcpyAsync(s2.data(), d_s2, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    const double n = double(p.spp);
+    for (uint64_t b = 0; b < nb; ++b) {
+        const double mean = s[b] / n;
+        const double var = n > 1 ? std::max(0.0, (s2[b] - n * mean * mean) / (n - 1)) : 0.0;
+        const double val = vtot * mean;
+        if (sizeof(T) == 4)
+            static_cast<float*>(bins)[b] = float(val);
+        else
+            static_cast<double*>(bins)[b] = val;
+        if (stderr_out) stderr_out[b] = vtot * std::sqrt(var / n);
+    }
+    return VMX_OK;
+}
+
+template <typename T, typename Math>
+int mc_dispatch(vmx_ctx* ctx, const vmx_problem& p, const vmx_integrand& f, int stratified, void* bins, double* se) {
+    switch (f.family) {
+        case VMX_FAMILY_POLY:
+            switch (f.dim) {
+                case 1: return mc_run<T, Math, 1, vmx::Poly<T>>(ctx, p, f.params, stratified, bins, se);
+                case 2: return mc_run<T, Math, 2, vmx::Poly<T>>(ctx, p, f.params, stratified, bins, se);
+                case 3: return mc_run<T, Math, 3, vmx::Poly<T>>(ctx, p, f.params, stratified, bins, se);
+                case 4: return mc_run<T, Math, 4, vmx::Poly<T>>(ctx, p, f.params, stratified, bins, se);
+                case 5: return mc_run<T, Math, 5, vmx::Poly<T>>(ctx, p, f.params, stratified, bins, se);
+                default: break;
+            }
+            break;
+        case VMX_FAMILY_BLOBS:
+            if (f.dim == 2) return mc_run<T, Math, 2, vmx::Blobs<T>>(ctx, p, f.params, stratified, bins, se);
+            if (f.dim == 3) return mc_run<T, Math, 3, vmx::Blobs<T>>(ctx, p, f.params, stratified, bins, se);
+            break;
+        case VMX_FAMILY_LOBES5:
+            if (f.dim == 5) return mc_run<T, Math, 5, vmx::Lobes5<T>>(ctx, p, f.params, stratified, bins, se);
+            break;
+        default: break;
+    }
+    throw VmxError(VMX_ERR_UNSUPPORTED, "integrand family / dimension not built");
+}
+
 }  // namespace
 
 extern "C" {
+
+int vmx_integrate_mc(vmx_ctx* ctx, const vmx_problem* p, const vmx_integrand* f, int32_t stratified, void* bins,
+                     double* stderr_out) {
+    return guarded(ctx, [&]() -> int {
+        validate(p, f);
+        if (!bins) throw VmxError(VMX_ERR_INVALID, "null bins");
+        if (p->spp < 1) throw VmxError(VMX_ERR_INVALID, "spp must be >= 1");
+        if (p->scalar == VMX_F32) return mc_dispatch<float, vmx::FastMath>(ctx, *p, *f, stratified, bins, stderr_out);
+        return mc_dispatch<double, vmx::PreciseMath>(ctx, *p, *f, stratified, bins, stderr_out);
+    });
+}
+
 
 int vmx_ctx_create(int device, vmx_ctx** out) {
     if (!out) return VMX_ERR_INVALID;
@@ -593,6 +696,11 @@ int vmx_ctx_create(int device, vmx_ctx** out) {
         delete c;
         return VMX_ERR_DEVICE;
     }
+    for (hipEvent_t& e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete c;
+            return VMX_ERR_DEVICE;
+        }
     *out = c;
     return VMX_OK;
 }
@@ -602,6 +710,8 @@ void vmx_ctx_destroy(vmx_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     for (vmx_ctx::Buf* b : {&ctx->tree, &ctx->regs, &ctx->lists, &ctx->misc, &ctx->params, &ctx->bins})
         if (b->p) (void)hipFree(b->p);
+    for (hipEvent_t e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

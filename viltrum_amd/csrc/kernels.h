@@ -516,6 +516,57 @@ __global__ void k_bins(BinArgs<T> a, Regs<T> r, const uint32_t* __restrict__ off
     bins[b] = T(result);
 }
 
+/* Plain per-bin Monte Carlo, the reference's monte_carlo_per_bin
+   (src/monte-carlo/monte-carlo-per-bin.h:40-65): `count` uniform samples per bin from
+   sample index `first`, accumulated into FP64 sum and sum of squares.  With
+   `stratified`, sample s of a bin is jittered inside cell s % (g*g) of a g x g grid over
+   the first two bin dims (g*g divides the sample count).  Chunked launches keep every
+   kernel short. */
+template <typename T, int D, typename F, typename M>
+__global__ void k_mc(BinArgs<T> a, const F* __restrict__ f, uint64_t first, uint64_t count, uint32_t grid,
+                     double* __restrict__ sum, double* __restrict__ sum2) {
+    const uint64_t b = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (b >= a.nbins) return;
+    const int db = a.db;
+    uint32_t p[3] = {0, 0, 0};
+    {
+        uint64_t rem = b;
+        for (int i = db - 1; i >= 0; --i) {
+            p[i] = uint32_t(rem % a.res[i]);
+            rem /= a.res[i];
+        }
+    }
+    T blo[D], bhi[D];
+    for (int d = 0; d < D; ++d) {
+        blo[d] = a.lo[d];
+        bhi[d] = a.hi[d];
+    }
+    for (int i = 0; i < db; ++i) {
+        blo[i] = a.lo[i] + T(p[i]) * a.drange[i];
+        bhi[i] = a.lo[i] + T(p[i] + 1) * a.drange[i];
+    }
+    double s1 = 0, s2 = 0;
+    for (uint64_t s = first; s < first + count; ++s) {
+        const SampleDraws<T, D> u(uint32_t(s), b, a.stream ^ 0x5bd1e995u, a.seed_lo, a.seed_hi);
+        T x[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            T c = u.coord(d);
+            if (grid > 1 && d < 2 && d < db) {
+                const uint32_t cell = uint32_t(s % (uint64_t(grid) * grid));
+                const uint32_t ci = d == 0 ? cell % grid : cell / grid;
+                c = (T(ci) + c) / T(grid);
+            }
+            x[d] = blo[d] + c * (bhi[d] - blo[d]);
+        }
+        const double y = double(f->template eval<M, D>(x));
+        s1 += y;
+        s2 += y * y;
+    }
+    sum[b] += s1;
+    sum2[b] += s2;
+}
+
 /* dst[i*stride + k] = src[order[i]*stride + k] */
 template <typename T>
 __global__ void k_gather(const T* __restrict__ src, const int32_t* __restrict__ order, uint32_t n, int stride,

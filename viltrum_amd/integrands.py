@@ -242,3 +242,29 @@ def config3_batch(seed=0, count=256):
         f = c1_family(sub, nb) if dim == 2 else moving_blobs(sub, nb, dim=3)
         out.append(dict(integrand=f, res=(side, side), regions=regions, spp=spp))
     return out
+
+
+def flops_per_eval(f):
+    """Static FP32 operation count of one integrand evaluation, read off the source in
+    include/viltrum_amd/integrands.h (add/sub/mul/neg/compare = 1, FMA = 2; the
+    functions are written without FMAs).  Returns (flops, special) where `special`
+    counts exp/sin/cos calls, never converted to flops."""
+    if f.family == FAMILY_BLOBS:
+        p = f.params
+        flops = 14 * p.nblobs + 1  # per blob: 6 (centre, dx, dy) + 8 (quadratic form, exp arg, w*, s+)
+        special = p.nblobs
+        if p.wave[0] != 0:
+            flops += 5
+            special += 2
+        if p.step_mode != 0:
+            flops += 6 + (1 if p.step_mode == 1 else 0)
+        return flops, special
+    if f.family == FAMILY_LOBES5:
+        return 49, 2
+    if f.family == FAMILY_POLY:
+        p = f.params
+        flops = sum(1 + sum(p.expo[m][d] for d in range(f.dim)) for m in range(p.nterms))
+        if p.has_step:
+            flops += 2 * f.dim + 2
+        return flops, 0
+    raise ValueError("unknown family")
