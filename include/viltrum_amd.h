@@ -127,6 +127,13 @@ int vmx_integrate(vmx_ctx* ctx, const vmx_problem* p, const vmx_integrand* f, vo
 int vmx_integrate_device(vmx_ctx* ctx, const vmx_problem* p, const vmx_integrand* f, void* bins_device,
                          void* stream, vmx_stats* stats);
 
+/* n independent problems in one launch sequence (one cooperative generation kernel per
+   (family, dim) group; every problem keeps its own range, resolution, budget, spp,
+   seed and form).  All problems share the scalar type, region order and bin_dims.
+   bins_device[i]: device bins of problem i; stats: n entries or NULL. */
+int vmx_integrate_batch(vmx_ctx* ctx, uint32_t n, const vmx_problem* problems, const vmx_integrand* integrands,
+                        void* const* bins_device, void* stream, vmx_stats* stats);
+
 /* Region list of the last vmx_integrate* call on this context, in its final order.
    Pass NULL arrays to query *n.  lo/hi: n*D, nodes: n*3^D, err: n, dim: n. */
 int vmx_export_regions(vmx_ctx* ctx, uint64_t* n, double* lo, double* hi, double* nodes, double* err,

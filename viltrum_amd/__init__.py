@@ -35,7 +35,7 @@ ORDER_CANONICAL, ORDER_REFERENCE = 0, 1
 REFINER_HEURISTIC, REFINER_LEARNED = 0, 1
 
 SYMBOLS = ["vmx_ctx_create", "vmx_ctx_destroy", "vmx_last_error", "vmx_integrate", "vmx_integrate_device",
-           "vmx_export_regions", "vmx_integrate_mc", "vmx_eval"]
+           "vmx_integrate_batch", "vmx_export_regions", "vmx_integrate_mc", "vmx_eval"]
 
 
 class VmxError(RuntimeError):
@@ -80,6 +80,8 @@ def lib():
         L.vmx_integrate.argtypes = [C.c_void_p, C.POINTER(Problem), C.c_void_p, C.c_void_p, C.POINTER(Stats)]
         L.vmx_integrate_device.argtypes = [C.c_void_p, C.POINTER(Problem), C.c_void_p, C.c_void_p, C.c_void_p,
                                            C.POINTER(Stats)]
+        L.vmx_integrate_batch.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_void_p]
         L.vmx_export_regions.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_void_p, C.c_void_p]
         L.vmx_eval.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_uint64, C.c_void_p, C.c_void_p]
@@ -125,6 +127,17 @@ class Context:
         self._check(lib().vmx_integrate_device(self.ptr, C.byref(problem), C.cast(f.ptr(), C.c_void_p),
                                                C.c_void_p(bins_ptr), C.c_void_p(stream or 0), C.byref(st)))
         return st
+
+    def integrate_batch(self, problems, fs, bins_ptrs, stream=None):
+        """n problems in one launch sequence; bins_ptrs are device addresses."""
+        n = len(problems)
+        P = (Problem * n)(*problems)
+        F = (integrands.IntegrandDesc * n)(*[f.desc for f in fs])
+        B = (C.c_void_p * n)(*bins_ptrs)
+        S = (Stats * n)()
+        self._check(lib().vmx_integrate_batch(self.ptr, n, C.cast(P, C.c_void_p), C.cast(F, C.c_void_p),
+                                              C.cast(B, C.c_void_p), C.c_void_p(stream or 0), C.cast(S, C.c_void_p)))
+        return list(S)
 
     def export_regions(self, D):
         n = C.c_uint64(0)

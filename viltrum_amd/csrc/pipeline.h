@@ -1,0 +1,998 @@
+/*
+ * Device-resident, batch-capable pipeline (J integrals of one (scalar, D, family) per
+ * launch sequence).  Every array is indexed by an absolute slot; job j owns
+ *   node slots    [node_base_j, node_base_j + node_cap_j)
+ *   regions       [reg_base_j,  reg_base_j + N_j)
+ *   bins          [bin_base_j,  bin_base_j + nbins_j)     (lists, offsets)
+ *
+ * 1. k_roots, then refinement rounds of every job: per round k_theta_round (boundary
+ *    bucket of the N-1 largest path-minimum errors, from per-job histograms) and
+ *    k_expand_round (teams of lanes split the new candidates that reach it).
+ * 2. Exact selection of the N-1 split nodes: everything above the boundary bucket is
+ *    split; inside it one workgroup per job radix-selects on the full key
+ *    (b, b1, codes), which is unique per node, so the result is deterministic.
+ * 3. Leaves (unsplit children of split nodes) in canonical order, then footprints:
+ *    leaves that share their bin-dim extent share their pixel box, and their FP64
+ *    bin-dim marginals add up (the control-variate integral is linear in the nodes).
+ * 4. Per-bin footprint lists and the per-bin control variate + residual kernel.
+ */
+#ifndef VMX_PIPELINE_H
+#define VMX_PIPELINE_H
+
+#include "kernels.h"
+
+namespace vmx {
+
+constexpr uint32_t NONE = 0xFFFFFFFFu;
+constexpr int MAX_JOBS = 4096;
+constexpr int GEN_THREADS = 256;
+
+enum : uint32_t { JF_CAPACITY = 1, JF_DEPTH = 2, JF_ROUNDS = 4, JF_EMPTY_BIN = 16, JF_LEAVES = 32 };
+
+template <typename T>
+struct Job {
+    T lo[VMX_MAX_DIM], hi[VMX_MAX_DIM], drange[3];
+    uint64_t res[3];
+    uint64_t nbins, bin_base, spp;
+    void* bins;  // T*, device, C order
+    double alpha;
+    uint32_t seed_lo, seed_hi, stream;
+    uint32_t N, node_base, node_cap, reg_base;
+    int32_t db, mb, form, heur;
+};
+
+struct JobRt {
+    uint32_t count, flags, kbound, above, nbucket, nleaves, ties, pad;
+};
+
+template <typename T>
+struct TreeArrays {
+    T *lo, *hi, *v, *err, *b, *b1;
+    int32_t *dim, *parent, *child;
+    uint32_t *code, *job;
+    uint8_t* popped;
+};
+
+/* ------------------------------------------------------------------ block helpers */
+/* In-place inclusive suffix sum of s[0..255] (s[t] = sum_{i>=t}); 256 threads. */
+__device__ inline void suffix_sum256(uint32_t* s) {
+    const int t = threadIdx.x;
+    for (int off = 1; off < 256; off <<= 1) {
+        const uint32_t add = (t + off < 256) ? s[t + off] : 0u;
+        __syncthreads();
+        s[t] += add;
+        __syncthreads();
+    }
+}
+
+/* Bucket (16-bit key of b) holding the need-th largest b of one job, and the number of
+   nodes in strictly higher buckets.  Requires total >= need >= 1.  256 threads. */
+__device__ inline void boundary_bucket(const uint32_t* h8, const uint32_t* h16, uint32_t need, uint32_t* sh,
+                                       uint32_t& k_out, uint32_t& above_out) {
+    const int t = threadIdx.x;
+    __shared__ uint32_t sel_c, sel_above;
+    sh[t] = h8[t];
+    __syncthreads();
+    suffix_sum256(sh);
+    if (sh[t] >= need && (t == 255 || sh[t + 1] < need)) {
+        sel_c = uint32_t(t);
+        sel_above = (t == 255) ? 0u : sh[t + 1];
+    }
+    __syncthreads();
+    const uint32_t c = sel_c, above = sel_above;
+    __syncthreads();
+    sh[t] = h16[c * 256 + t];
+    __syncthreads();
+    suffix_sum256(sh);
+    const uint32_t need2 = need - above;
+    if (sh[t] >= need2 && (t == 255 || sh[t + 1] < need2)) {
+        sel_c = c * 256 + uint32_t(t);
+        sel_above = above + ((t == 255) ? 0u : sh[t + 1]);
+    }
+    __syncthreads();
+    k_out = sel_c;
+    above_out = sel_above;
+    __syncthreads();
+}
+
+/* ------------------------------------------------------------------ teams
+   A team of TEAM lanes (a sub-wave or a whole wave) owns one region split: lanes share
+   the new-plane evaluations, the per-line nested errors and the Simpson folds through
+   LDS.  Each fold output is computed exactly as the sequential fold computes it, so
+   the results are bit-identical. */
+template <int D>
+struct TeamShape {
+    static constexpr int TEAM = D <= 2 ? 8 : (D == 3 ? 32 : 64);
+};
+
+template <typename T, int D>
+struct TeamLds {
+    static constexpr int K = pow3(D), KM = pow3(D - 1);
+    T pv[K];
+    T cv[2][K];
+    T e[2][2 * D * KM];  // double buffer of line errors / folds
+    T box[2][2][D];      // child c: lo, hi
+    T pbox[2][D];
+    int32_t node, slot, dim, job, go;
+};
+
+/* Line errors and folds for `nc` regions held in L.cv[c] with boxes L.box[c]; lanes
+   `lane` of `team`; returns (err, dim) of region c in lane c (c < nc).  Every lane of
+   the block must call it (it contains __syncthreads). */
+template <typename T, int D, int TEAM>
+__device__ inline void team_heuristic(TeamLds<T, D>& L, int nc, int heur, int lane, bool active, T& err_out,
+                                      int& dim_out) {
+    constexpr int KM = pow3(D - 1);
+    const int n_lines = nc * D * KM;
+    if (active)
+        for (int q = lane; q < n_lines; q += TEAM) {
+            const int c = q / (D * KM), rem = q % (D * KM), d = rem / KM, jl = rem % KM;
+            const int below = pow3(d);
+            const int full = (jl % below) + (jl / below) * below * 3;
+            const T* v = L.cv[c];
+            const T p0 = v[full], p1 = v[full + below], p2 = v[full + 2 * below];
+            const T sm = simpson(p0, p1, p2), tz = trapezoid(p0, p2);
+            L.e[0][q] = heur == 0 ? metric_relative(sm, tz) : metric_absolute(sm, tz);
+        }
+    __syncthreads();
+    int buf = 0, n = KM;
+    for (int lev = 0; lev < D - 1; ++lev) {
+        n /= 3;
+        if (active)
+            for (int q = lane; q < nc * D * n; q += TEAM) {
+                const int g = q / n, k = q % n;
+                const T* in = L.e[buf] + g * KM;
+                L.e[buf ^ 1][g * KM + k] = simpson(in[3 * k], in[3 * k + 1], in[3 * k + 2]);
+            }
+        buf ^= 1;
+        __syncthreads();
+    }
+    if (active && lane < nc) {
+        const int c = lane;
+        const T* lo = L.box[c][0];
+        const T* hi = L.box[c][1];
+        const T vol = box_volume<T, D>(lo, hi);
+        const T* E = L.e[buf] + c * D * KM;  // E[d * KM] = folded error of dim d
+        if (heur == 0) {
+            const double size_weight = 1.e-5, min_size = 1.e-37;
+            T max_err = vol * E[0];
+            const T ext0 = hi[0] - lo[0];
+            if (ext0 < min_size || visnan(ext0))
+                max_err = 0;
+            else
+                max_err += size_weight * vabs(ext0);
+            int max_dim = 0;
+            for (int d = 1; d < D; ++d) {
+                const T ext = hi[d] - lo[d];
+                T err = vol * E[d * KM] + size_weight * vabs(ext);
+                if (ext < min_size) err = 0;
+                if (err >= max_err) {
+                    max_err = err;
+                    max_dim = d;
+                }
+            }
+            err_out = max_err;
+            dim_out = max_dim;
+        } else {
+            T max_err = 0;
+            int max_dim = 0;
+            for (int d = 0; d < D; ++d) {
+                const T err = vol * E[d * KM];
+                if (err > max_err) {
+                    max_err = err;
+                    max_dim = d;
+                }
+            }
+            err_out = max_err;
+            dim_out = max_dim;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ roots
+   One block per job: 256 lanes evaluate the 3^D nodes, one team scores the root. */
+template <typename T, int D, typename F, typename M>
+__global__ void __launch_bounds__(256) k_roots(const Job<T>* __restrict__ jobs, JobRt* rt, const F* __restrict__ fs,
+                                               TreeArrays<T> t, uint32_t* hist16, uint32_t* hist8, int32_t* cand0,
+                                               uint32_t* ncand0) {
+    constexpr int K = pow3(D);
+    constexpr int TEAM = TeamShape<D>::TEAM;
+    __shared__ TeamLds<T, D> L;
+    const uint32_t j = blockIdx.x;
+    const Job<T>& jb = jobs[j];
+    const F& f = fs[j];
+    const size_t s = jb.node_base;
+    const int tid = threadIdx.x;
+    if (tid < D) {
+        L.box[0][0][tid] = jb.lo[tid];
+        L.box[0][1][tid] = jb.hi[tid];
+    }
+    __syncthreads();
+    for (int idx = tid; idx < K; idx += blockDim.x) {
+        T x[D];
+        for (int d = 0; d < D; ++d) x[d] = node_coord<T>(double(digit3(idx, d)) / 2.0, L.box[0][0][d], L.box[0][1][d]);
+        const T y = f.template eval<M, D>(x);
+        L.cv[0][idx] = y;
+        t.v[s * K + idx] = y;
+    }
+    __syncthreads();
+    T e = 0;
+    int dm = 0;
+    team_heuristic<T, D, TEAM>(L, 1, jb.heur, tid, tid < TEAM, e, dm);
+    if (tid == 0) {
+        for (int d = 0; d < D; ++d) {
+            t.lo[s * D + d] = L.box[0][0][d];
+            t.hi[s * D + d] = L.box[0][1][d];
+            t.code[s * D + d] = 1u;
+        }
+        t.err[s] = e;
+        t.dim[s] = dm;
+        t.b[s] = e;
+        t.b1[s] = T(INFINITY);
+        t.parent[s] = -1;
+        t.child[s] = -1;
+        t.job[s] = j;
+        rt[j].count = jb.node_base + 1;
+        const uint32_t k = key16(e);
+        atomicAdd(&hist16[size_t(j) * 65536 + k], 1u);
+        atomicAdd(&hist8[size_t(j) * 256 + (k >> 8)], 1u);
+        cand0[s] = int32_t(s);
+        ncand0[j] = jb.N > 1 ? 1u : 0u;
+    }
+}
+
+template <typename T>
+struct GenArgs {
+    const Job<T>* jobs;
+    JobRt* rt;
+    TreeArrays<T> t;
+    uint32_t* hist16;
+    uint32_t* hist8;
+    int32_t* cand[2];
+    uint32_t* ncand;  // [2*J]
+    T* theta;         // [J]
+    uint32_t* rounds; // rounds that had candidates
+    uint32_t J;
+    uint32_t max_rounds;
+};
+
+/* Expansion of one candidate per team; all lanes of the block call it the same number
+   of times (block-uniform loop), teams without a candidate idle through the syncs. */
+template <typename T, int D, typename F, typename M>
+__device__ inline void team_expand(const GenArgs<T>& g, const F* __restrict__ fs, TeamLds<T, D>& L, int par,
+                                   const uint32_t* pre, uint64_t w, bool has) {
+    constexpr int K = pow3(D), KM = pow3(D - 1);
+    constexpr int TEAM = TeamShape<D>::TEAM;
+    const int lane = threadIdx.x % TEAM;
+    const uint32_t J = g.J;
+    TreeArrays<T> t = g.t;
+    if (lane == 0) {
+        L.go = 0;
+        if (has) {
+            uint32_t lo = 0, hi = J;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) / 2;
+                if (pre[mid] <= w)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            const uint32_t j = lo;
+            const Job<T>& jb = g.jobs[j];
+            const int32_t node = g.cand[par][jb.node_base + (uint32_t(w) - pre[j])];
+            if (t.b[node] >= g.theta[j]) {
+                const uint32_t slot = atomicAdd(&g.rt[j].count, 2u);
+                if (slot + 2 > jb.node_base + jb.node_cap) {
+                    atomicOr(&g.rt[j].flags, JF_CAPACITY);
+                } else {
+                    L.go = 1;
+                    L.node = node;
+                    L.slot = int32_t(slot);
+                    L.job = int32_t(j);
+                    const int dim = t.dim[node];
+                    L.dim = dim;
+                    T plo[D], phi[D];
+                    for (int d = 0; d < D; ++d) {
+                        plo[d] = t.lo[size_t(node) * D + d];
+                        phi[d] = t.hi[size_t(node) * D + d];
+                        L.pbox[0][d] = plo[d];
+                        L.pbox[1][d] = phi[d];
+                    }
+                    split_boxes<T, D>(plo, phi, dim, L.box[0][0], L.box[0][1], L.box[1][0], L.box[1][1]);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const bool go = L.go != 0;
+    if (go)
+        for (int k = lane; k < K; k += TEAM) L.pv[k] = t.v[size_t(L.node) * K + k];
+    __syncthreads();
+    if (go) {
+        const int dim = L.dim;
+        const int stride = pow3(dim);
+        const F& f = fs[L.job];
+        for (int idx = lane; idx < K; idx += TEAM) {
+            const int jd = digit3(idx, dim);
+            if (jd == 1) continue;
+            const int base = idx - jd * stride;
+            L.cv[0][idx] = L.pv[jd == 0 ? base : base + stride];
+            L.cv[1][idx] = L.pv[jd == 0 ? base + stride : base + 2 * stride];
+        }
+        for (int q = lane; q < 2 * KM; q += TEAM) {
+            const int c = q / KM, jj = q % KM;
+            T x[D];
+            const int idx = split_new_node<T, D>(L.pbox[0], L.pbox[1], dim, c, jj, x);
+            L.cv[c][idx] = f.template eval<M, D>(x);
+        }
+    }
+    __syncthreads();
+    if (go) {
+        const size_t s0 = size_t(L.slot);
+        for (int q = lane; q < 2 * K; q += TEAM) t.v[(s0 + q / K) * K + q % K] = L.cv[q / K][q % K];
+    }
+    const int heur = go ? g.jobs[L.job].heur : 0;
+    T e = 0;
+    int dm = 0;
+    team_heuristic<T, D, TEAM>(L, 2, heur, lane, go, e, dm);
+    if (go && lane < 2) {
+        const int c = lane;
+        const uint32_t j = uint32_t(L.job);
+        const int32_t node = L.node;
+        const int dim = L.dim;
+        const size_t s = size_t(L.slot) + c;
+        const uint32_t pcode = t.code[size_t(node) * D + dim];
+        if (pcode >= 0x80000000u) atomicOr(&g.rt[j].flags, JF_DEPTH);
+        for (int d = 0; d < D; ++d) {
+            t.lo[s * D + d] = L.box[c][0][d];
+            t.hi[s * D + d] = L.box[c][1][d];
+            t.code[s * D + d] = (d == dim) ? (pcode * 2u + uint32_t(c)) : t.code[size_t(node) * D + d];
+        }
+        t.err[s] = e;
+        t.dim[s] = dm;
+        const T bp = t.b[node], b1p = t.b1[node];
+        T nb, nb1;  // first two entries of the path's suffix-minimum stack
+        if (e < bp) {
+            nb = e;
+            nb1 = T(INFINITY);
+        } else if (e < b1p) {
+            nb = bp;
+            nb1 = e;
+        } else {
+            nb = bp;
+            nb1 = b1p;
+        }
+        t.b[s] = nb;
+        t.b1[s] = nb1;
+        t.parent[s] = node;
+        t.child[s] = -1;
+        t.job[s] = j;
+        const uint32_t k = key16(nb);
+        atomicAdd(&g.hist16[size_t(j) * 65536 + k], 1u);
+        atomicAdd(&g.hist8[size_t(j) * 256 + (k >> 8)], 1u);
+        if (c == 0) {
+            t.child[node] = int32_t(s);
+            const uint32_t o = atomicAdd(&g.ncand[(par ^ 1) * J + j], 2u);
+            const uint32_t nbase = g.jobs[j].node_base;
+            g.cand[par ^ 1][nbase + o] = int32_t(s);
+            g.cand[par ^ 1][nbase + o + 1] = int32_t(s + 1);
+        }
+    }
+    __syncthreads();
+}
+
+/* Round kernels.  A round is k_theta_round (one block per job: the boundary bucket of
+   the job's N-1 largest b among its nodes so far, as a lower bound theta) followed by
+   k_expand_round (teams expand the job's new candidates with b >= theta).  Kernel
+   boundaries order the rounds, so no grid-wide barrier or cache flush is needed inside
+   a kernel.  A round with no candidates exits at once; the host launches rounds in
+   chunks and checks for remaining candidates between chunks. */
+template <typename T>
+__global__ void __launch_bounds__(256) k_theta_round(GenArgs<T> g, int par) {
+    __shared__ uint32_t sh[256];
+    const uint32_t j = blockIdx.x;
+    const Job<T>& jb = g.jobs[j];
+    // the counter this round's expansion fills (last read by the previous round)
+    if (threadIdx.x == 0) g.ncand[(par ^ 1) * g.J + j] = 0;
+    if (g.ncand[par * g.J + j] == 0) return;
+    const uint32_t total = g.rt[j].count - jb.node_base;
+    const uint32_t need = jb.N - 1;
+    T th;
+    if (need == 0) {
+        th = T(INFINITY);
+    } else if (total < need) {
+        th = T(-INFINITY);
+    } else {
+        uint32_t k, above;
+        boundary_bucket(g.hist8 + size_t(j) * 256, g.hist16 + size_t(j) * 65536, need, sh, k, above);
+        key16_floor(k, th);
+    }
+    if (threadIdx.x == 0) g.theta[j] = th;
+}
+
+template <typename T, int D, typename F, typename M>
+__global__ void __launch_bounds__(GEN_THREADS) k_expand_round(GenArgs<T> g, const F* __restrict__ fs, int par) {
+    constexpr int TEAM = TeamShape<D>::TEAM;
+    constexpr int TPB = GEN_THREADS / TEAM;  // teams per block
+    __shared__ uint32_t pre[MAX_JOBS + 1];
+    __shared__ TeamLds<T, D> lds[TPB];
+    const uint32_t J = g.J;
+    const int team = threadIdx.x / TEAM;
+    for (uint32_t j = threadIdx.x; j < J; j += blockDim.x) pre[j + 1] = g.ncand[par * J + j];
+    if (threadIdx.x == 0) pre[0] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (uint32_t j = 1; j <= J; ++j) pre[j] += pre[j - 1];
+    __syncthreads();
+    const uint32_t total = pre[J];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && total > 0) atomicAdd(g.rounds, 1u);
+    for (uint64_t base = uint64_t(blockIdx.x) * TPB; base < total; base += uint64_t(gridDim.x) * TPB) {
+        const uint64_t w = base + team;
+        team_expand<T, D, F, M>(g, fs, lds[team], par, pre, w, w < total);
+    }
+}
+
+/* ------------------------------------------------------------------ selection */
+template <typename T>
+__global__ void k_sel_bound(const Job<T>* __restrict__ jobs, JobRt* rt, const uint32_t* hist16,
+                            const uint32_t* hist8) {
+    __shared__ uint32_t sh[256];
+    const uint32_t j = blockIdx.x;
+    const uint32_t need = jobs[j].N - 1;
+    uint32_t k = NONE, above = 0;
+    if (need > 0) boundary_bucket(hist8 + size_t(j) * 256, hist16 + size_t(j) * 65536, need, sh, k, above);
+    if (threadIdx.x == 0) {
+        rt[j].kbound = k;
+        rt[j].above = above;
+        rt[j].nbucket = 0;
+        rt[j].nleaves = 0;
+        rt[j].ties = 0;
+    }
+}
+
+/* Nodes above the boundary bucket are split; nodes inside it are gathered. */
+template <typename T>
+__global__ void k_sel_gather(const Job<T>* __restrict__ jobs, JobRt* rt, TreeArrays<T> t, uint32_t cap_total,
+                             int32_t* bucket) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= cap_total) return;
+    const uint32_t j = t.job[s];
+    uint8_t p = 0;
+    if (j != NONE && s < rt[j].count && rt[j].kbound != NONE) {
+        const uint32_t k = key16(t.b[s]);
+        if (k > rt[j].kbound) {
+            p = 1;
+        } else if (k == rt[j].kbound) {
+            const uint32_t pos = atomicAdd(&rt[j].nbucket, 1u);
+            bucket[jobs[j].node_base + pos] = int32_t(s);
+        }
+    }
+    t.popped[s] = p;
+}
+
+/* Digit `pass` (8 bits, most significant first) of the selection key of node s:
+   b below its top 16 bits, b1, then the bitwise complement of every dim's code (so a
+   larger key means split earlier; smaller codes win ties). */
+template <typename T, int D>
+__device__ inline uint32_t sel_digit(const TreeArrays<T>& t, int32_t s, int pass) {
+    constexpr int BB = int(sizeof(T));  // bytes of b and b1
+    constexpr int nb_b = BB - 2;        // low bytes of b (top 2 bytes fixed by the bucket)
+    auto byte_of = [](uint64_t x, int i, int nbytes) { return uint32_t((x >> (8 * (nbytes - 1 - i))) & 0xFFu); };
+    if (pass < nb_b) {
+        const uint64_t x = (BB == 4) ? uint64_t(__float_as_uint(float(t.b[s])))
+                                     : uint64_t(__double_as_longlong(double(t.b[s])));
+        return byte_of(x, pass + 2, BB);
+    }
+    pass -= nb_b;
+    if (pass < BB) {
+        const uint64_t x = (BB == 4) ? uint64_t(__float_as_uint(float(t.b1[s])))
+                                     : uint64_t(__double_as_longlong(double(t.b1[s])));
+        return byte_of(x, pass, BB);
+    }
+    pass -= BB;
+    const int d = pass / 4;
+    return byte_of(uint64_t(~t.code[size_t(s) * D + d]), pass % 4, 4);
+}
+
+/* One workgroup per job: exact radix select inside the boundary bucket.  bstate per
+   bucket item: 0 undecided, 1 split, 2 kept. */
+template <typename T, int D>
+__global__ void k_sel_resolve(const Job<T>* __restrict__ jobs, JobRt* rt, TreeArrays<T> t,
+                              const int32_t* __restrict__ bucket, uint8_t* bstate) {
+    constexpr int PASSES = (int(sizeof(T)) - 2) + int(sizeof(T)) + 4 * D;
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t sel_digit_s, sel_above_s;
+    const uint32_t j = blockIdx.x;
+    const Job<T>& jb = jobs[j];
+    if (rt[j].kbound == NONE) return;
+    const uint32_t nb = rt[j].nbucket;
+    const int32_t* lst = bucket + jb.node_base;
+    uint8_t* st = bstate + jb.node_base;
+    uint32_t rank = (jb.N - 1) - rt[j].above;  // split this many bucket items, largest first
+    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) st[i] = 0;
+    __syncthreads();
+    for (int pass = 0; pass < PASSES; ++pass) {
+        hist[threadIdx.x] = 0;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x)
+            if (st[i] == 0) atomicAdd(&hist[sel_digit<T, D>(t, lst[i], pass)], 1u);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t above = 0;
+            int dsel = 0;
+            for (int dg = 255; dg >= 0; --dg) {
+                if (above + hist[dg] >= rank) {
+                    dsel = dg;
+                    break;
+                }
+                above += hist[dg];
+            }
+            sel_digit_s = uint32_t(dsel);
+            sel_above_s = above;
+        }
+        __syncthreads();
+        const uint32_t dsel = sel_digit_s;
+        for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x)
+            if (st[i] == 0) {
+                const uint32_t dg = sel_digit<T, D>(t, lst[i], pass);
+                if (dg > dsel)
+                    st[i] = 1;
+                else if (dg < dsel)
+                    st[i] = 2;
+            }
+        rank -= sel_above_s;
+        __syncthreads();
+    }
+    // exactly one undecided item is left (the key is unique per node): the rank-th
+    // largest, split too
+    __shared__ int32_t kstar;
+    if (threadIdx.x == 0) kstar = -1;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x)
+        if (st[i] == 0) kstar = lst[i];
+    __syncthreads();
+    if (kstar < 0) {
+        if (threadIdx.x == 0) atomicOr(&rt[j].flags, JF_LEAVES);
+        return;
+    }
+    const T bk = t.b[kstar], b1k = t.b1[kstar];
+    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) {
+        const int32_t s = lst[i];
+        const bool split = st[i] != 2;
+        t.popped[s] = split ? 1 : 0;
+        if (!split && t.b[s] == bk && t.b1[s] == b1k) atomicAdd(&rt[j].ties, 1u);
+    }
+}
+
+/* Unsplit children of split nodes (or the root when N = 1), appended per job. */
+template <typename T>
+__global__ void k_leaves(const Job<T>* __restrict__ jobs, JobRt* rt, TreeArrays<T> t, uint32_t cap_total,
+                         int32_t* leaf_slot) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    bool leaf = false;
+    uint32_t j = NONE;
+    if (s < cap_total) {
+        j = t.job[s];
+        if (j != NONE && s < rt[j].count && !t.popped[s]) {
+            const int32_t p = t.parent[s];
+            leaf = (p >= 0) ? (t.popped[p] != 0) : (jobs[j].N == 1);
+        }
+    }
+    // wave-aggregated append: lanes of the leader's job share one atomic
+    const uint64_t act = __ballot(leaf);
+    if (act == 0) return;
+    const int leader = __ffsll((unsigned long long)act) - 1;
+    const uint32_t jl = __shfl(j, leader);
+    const uint64_t same = __ballot(leaf && j == jl);
+    uint32_t base = 0;
+    if (int(threadIdx.x % 64) == leader) base = atomicAdd(&rt[jl].nleaves, uint32_t(__popcll(same)));
+    base = __shfl(base, leader);
+    if (!leaf) return;
+    uint32_t pos;
+    if (j == jl) {
+        const uint64_t below = same & ((1ull << (threadIdx.x % 64)) - 1ull);
+        pos = base + uint32_t(__popcll(below));
+    } else {
+        pos = atomicAdd(&rt[j].nleaves, 1u);
+    }
+    const Job<T>& jb = jobs[j];
+    if (pos >= jb.N) {
+        atomicOr(&rt[j].flags, JF_LEAVES);
+        return;
+    }
+    leaf_slot[jb.reg_base + pos] = int32_t(s);
+}
+
+/* Sort keys of one LSD pass over the leaves (canonical order: job, code_0, ...,
+   code_{D-1}).  which = -1: job; which = d: code_d (32 bits). */
+template <int D>
+__global__ void k_leaf_keys(const int32_t* __restrict__ leaf_slot, const uint32_t* __restrict__ code,
+                            const uint32_t* __restrict__ job, uint32_t n, int which, uint32_t* keys) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int32_t sl = leaf_slot[r];
+    if (sl < 0) {  // unfilled (the host rejects the batch before any slot is used)
+        keys[r] = NONE;
+        return;
+    }
+    const size_t s = size_t(sl);
+    keys[r] = which < 0 ? job[s] : code[s * D + which];
+}
+
+/* ------------------------------------------------------------------ regions */
+template <typename T>
+struct RegionArrays {
+    T *lo, *hi, *vol, *coef;
+    double* marg;
+    uint32_t* job;
+    uint32_t* fpflag;  // [n+1]
+    uint32_t* fpid;    // [n+1] exclusive scan of fpflag
+};
+
+template <typename T>
+struct FootArrays {
+    T *lo, *hi;          // [F*3] bin dims
+    double* scale;       // [F]
+    double* marg;        // [F*mb]
+    uint32_t *first, *count, *box, *job;  // box [F*6]
+};
+
+template <typename T, int D>
+__global__ void k_prepare2(const Job<T>* __restrict__ jobs, const int32_t* __restrict__ leaf_slot, TreeArrays<T> t,
+                           RegionArrays<T> r, uint32_t n, int mb, int merge) {
+    constexpr int K = pow3(D);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t s = size_t(leaf_slot[i]);
+    const uint32_t j = t.job[s];
+    const Job<T>& jb = jobs[j];
+    T lo[D], hi[D], v[K];
+    for (int d = 0; d < D; ++d) {
+        lo[d] = t.lo[s * D + d];
+        hi[d] = t.hi[s * D + d];
+        r.lo[size_t(i) * D + d] = lo[d];
+        r.hi[size_t(i) * D + d] = hi[d];
+    }
+    for (int k = 0; k < K; ++k) v[k] = t.v[s * K + k];
+    r.vol[i] = box_volume<T, D>(lo, hi);
+    double m[27];
+    bin_marginal<T, double, D>(v, jb.db, m);
+    for (int k = 0; k < mb; ++k) r.marg[size_t(i) * mb + k] = m[k];
+    T c[K];
+    monomials<T, T, D>(v, c);
+    for (int k = 0; k < K; ++k) r.coef[size_t(i) * K + k] = c[k];
+    r.job[i] = j;
+    uint32_t flag = 1;
+    if (merge && i != jb.reg_base) {
+        const size_t sp = size_t(leaf_slot[i - 1]);
+        flag = 0;
+        for (int d = 0; d < jb.db; ++d)
+            if (t.code[sp * D + d] != t.code[s * D + d]) flag = 1;
+    }
+    r.fpflag[i] = flag;
+    if (i == 0) r.fpflag[n] = 0;
+}
+
+/* One thread per footprint: bin-dim box, pixel box, and merged FP64 marginal
+   (sum of vol_r * marginal_r over its regions, in region order) or, unmerged, the
+   single region's (vol, marginal) so per-region sums keep the reference's rounding. */
+template <typename T, int D>
+__global__ void k_footprints(const Job<T>* __restrict__ jobs, RegionArrays<T> r, FootArrays<T> f, uint32_t n,
+                             int mb, int merge) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !r.fpflag[i]) return;
+    const uint32_t id = r.fpid[i];
+    uint32_t e = i + 1;
+    while (e < n && !r.fpflag[e]) ++e;
+    const uint32_t j = r.job[i];
+    const Job<T>& jb = jobs[j];
+    f.first[id] = i;
+    f.count[id] = e - i;
+    f.job[id] = j;
+    for (int d = 0; d < 3; ++d) {
+        uint64_t a = 0, z = 1;
+        if (d < jb.db) {
+            const T lo = r.lo[size_t(i) * D + d], hi = r.hi[size_t(i) * D + d];
+            f.lo[size_t(id) * 3 + d] = lo;
+            f.hi[size_t(id) * 3 + d] = hi;
+            pixel_range<T>(lo, hi, jb.lo[d], jb.hi[d], jb.res[d], a, z);
+        }
+        f.box[size_t(id) * 6 + 2 * d] = uint32_t(a);
+        f.box[size_t(id) * 6 + 2 * d + 1] = uint32_t(z);
+    }
+    if (merge) {
+        double acc[27];
+        for (int k = 0; k < mb; ++k) acc[k] = 0.0;
+        for (uint32_t q = i; q < e; ++q) {
+            const double vol = double(r.vol[q]);
+            for (int k = 0; k < mb; ++k) acc[k] += vol * r.marg[size_t(q) * mb + k];
+        }
+        for (int k = 0; k < mb; ++k) f.marg[size_t(id) * mb + k] = acc[k];
+        f.scale[id] = 1.0;
+    } else {
+        for (int k = 0; k < mb; ++k) f.marg[size_t(id) * mb + k] = r.marg[size_t(i) * mb + k];
+        f.scale[id] = double(r.vol[i]);
+    }
+}
+
+/* Count (Fill = false) or place (Fill = true) footprint ids into their bins' lists:
+   one wave per footprint, lanes over the pixel box. */
+template <typename T, bool Fill>
+__global__ void k_stratify2(const Job<T>* __restrict__ jobs, FootArrays<T> f, const uint32_t* __restrict__ nfp_ptr,
+                            uint32_t* __restrict__ cnt, const uint32_t* __restrict__ off, uint32_t* __restrict__ lst) {
+    const uint32_t w = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const uint32_t lane = threadIdx.x % 64;
+    if (w >= *nfp_ptr) return;
+    const Job<T>& jb = jobs[f.job[w]];
+    const uint32_t* bx = f.box + size_t(w) * 6;
+    const uint32_t n0 = bx[1] - bx[0], n1 = bx[3] - bx[2], n2 = bx[5] - bx[4];
+    const uint64_t total = uint64_t(n0) * n1 * n2;
+    for (uint64_t c = lane; c < total; c += 64) {
+        const uint32_t p0 = bx[0] + uint32_t(c % n0);
+        const uint32_t p1 = bx[2] + uint32_t((c / n0) % n1);
+        const uint32_t p2 = bx[4] + uint32_t(c / (uint64_t(n0) * n1));
+        uint64_t local = p0;
+        if (jb.db > 1) local = local * jb.res[1] + p1;
+        if (jb.db > 2) local = local * jb.res[2] + p2;
+        const uint64_t bi = jb.bin_base + local;
+        const uint32_t k = atomicAdd(&cnt[bi], 1u);
+        if (Fill) lst[off[bi] + k] = w;
+    }
+}
+
+template <typename T>
+__device__ inline uint32_t job_of_bin(const Job<T>* jobs, uint32_t J, uint64_t b) {
+    uint32_t lo = 0, hi = J;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (jobs[mid].bin_base <= b)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+constexpr int BIN_MAXF = 64;  // footprints per bin cached in LDS (longer lists walk memory)
+
+template <typename T>
+struct BinLds {
+    uint32_t fi[BIN_MAXF];
+    uint32_t cum[BIN_MAXF + 1];
+    double contrib[BIN_MAXF];
+    double approx;
+    uint32_t n, nf, o, job;
+    uint64_t b;
+};
+
+/* sum over the `width` lanes of a team (width a power of two <= 64) */
+__device__ inline double team_sum(double v, int width) {
+    for (int m = width / 2; m > 0; m >>= 1) v += __shfl_xor(v, m, width);
+    return v;
+}
+
+/* A team of L lanes per bin (all jobs): control variate over the bin's footprints
+   (FP64, summed in list order by one lane), then the spp residual samples spread over
+   the lanes -- a region drawn uniformly among the bin's regions (binary search over
+   footprint counts in LDS), a point uniform in bin x region -- and the estimator of the
+   job's form (…regions-integrator-parallel-variance-reduction.h:67-103,
+   …regions-integrator-parallel-control-variates.h:39-65, weight-strategy.h:7-110).
+   Moments use the reference's shift by the bin's first sample; lane partial sums are
+   Kahan-compensated in the evaluation type and reduced across the team in FP64. */
+template <typename T, int D, int DB, typename F, typename M>
+__global__ void __launch_bounds__(256) k_bins3(const Job<T>* __restrict__ jobs, uint32_t J, uint64_t total_bins,
+                                               int L, RegionArrays<T> r, FootArrays<T> fa,
+                                               const uint32_t* __restrict__ off, const uint32_t* __restrict__ lst,
+                                               const F* __restrict__ fs, JobRt* rt) {
+    constexpr int K = pow3(D);
+    extern __shared__ unsigned char smem_raw[];
+    BinLds<T>* lds = reinterpret_cast<BinLds<T>*>(smem_raw);
+    const int teams = blockDim.x / L;
+    const int team = threadIdx.x / L, lane = threadIdx.x % L;
+    BinLds<T>& B = lds[team];
+    for (uint64_t base = uint64_t(blockIdx.x) * teams; base < total_bins; base += uint64_t(gridDim.x) * teams) {
+        const uint64_t gb = base + team;
+        const bool active = gb < total_bins;
+        uint32_t j = 0, o = 0, nf = 0;
+        if (active) {
+            j = job_of_bin(jobs, J, gb);
+            o = off[gb];
+            nf = off[gb + 1] - o;
+        }
+        const Job<T>& jb = jobs[j];
+        const uint64_t b = active ? gb - jb.bin_base : 0;
+        constexpr int db = DB;
+        uint32_t p[3] = {0, 0, 0};
+        {
+            uint64_t rem = b;
+#pragma unroll
+            for (int i = db - 1; i >= 0; --i) {
+                p[i] = uint32_t(rem % jb.res[i]);
+                rem /= jb.res[i];
+            }
+        }
+        T blo[D], bhi[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            blo[d] = jb.lo[d];
+            bhi[d] = jb.hi[d];
+        }
+#pragma unroll
+        for (int i = 0; i < db; ++i) {
+            blo[i] = jb.lo[i] + T(p[i]) * jb.drange[i];
+            bhi[i] = jb.lo[i] + T(p[i] + 1) * jb.drange[i];
+        }
+        const double factor = double(jb.nbins);
+        // control-variate terms of the cached footprints
+        const uint32_t nc = nf < uint32_t(BIN_MAXF) ? nf : uint32_t(BIN_MAXF);
+        if (active)
+            for (uint32_t k = lane; k < nc; k += L) {
+                const uint32_t fi = lst[o + k];
+                B.fi[k] = fi;
+                B.cum[k + 1] = fa.count[fi];
+                double aa[db], bb[db];
+                bool empty = false;
+#pragma unroll
+                for (int d = 0; d < db; ++d) {
+                    const T rl = fa.lo[size_t(fi) * 3 + d], rh = fa.hi[size_t(fi) * 3 + d];
+                    const T il = vmax(blo[d], rl);
+                    const T ih = vmax(il, vmin(bhi[d], rh));
+                    if (il >= ih) empty = true;
+                    aa[d] = double(pos_in(il, rl, rh));
+                    bb[d] = double(pos_in(ih, rl, rh));
+                }
+                B.contrib[k] = empty ? 0.0
+                                     : factor * (fa.scale[fi] *
+                                                 marginal_subrange_n<double, db>(fa.marg + size_t(fi) * jb.mb, aa, bb));
+            }
+        __syncthreads();
+        if (active && lane == 0) {
+            T* bins = static_cast<T*>(jb.bins);
+            double approx = (jb.form == VMX_FORM_CV_ALPHA1) ? double(bins[b]) : 0.0;
+            uint32_t n = 0;
+            B.cum[0] = 0;
+            for (uint32_t k = 0; k < nf; ++k) {
+                if (k < nc) {
+                    approx += B.contrib[k];
+                    n += B.cum[k + 1];
+                    B.cum[k + 1] = n;
+                } else {  // beyond the LDS cache: same terms, same order, from memory
+                    const uint32_t fi = lst[o + k];
+                    n += fa.count[fi];
+                    double aa[db], bb[db];
+                    bool empty = false;
+#pragma unroll
+                    for (int d = 0; d < db; ++d) {
+                        const T rl = fa.lo[size_t(fi) * 3 + d], rh = fa.hi[size_t(fi) * 3 + d];
+                        const T il = vmax(blo[d], rl);
+                        const T ih = vmax(il, vmin(bhi[d], rh));
+                        if (il >= ih) empty = true;
+                        aa[d] = double(pos_in(il, rl, rh));
+                        bb[d] = double(pos_in(ih, rl, rh));
+                    }
+                    if (!empty)
+                        approx += factor * (fa.scale[fi] *
+                                            marginal_subrange_n<double, db>(fa.marg + size_t(fi) * jb.mb, aa, bb));
+                }
+            }
+            B.approx = approx;
+            B.n = n;
+        }
+        __syncthreads();
+        const uint32_t n = active ? B.n : 0;
+        const double approx = active ? B.approx : 0.0;
+        const bool sampling = active && jb.spp > 0 && n > 0;
+        if (active && jb.spp > 0 && n == 0 && lane == 0) atomicOr(&rt[j].flags, JF_EMPTY_BIN);
+        // region index -> region: binary search over cached prefix counts, memory walk beyond
+        auto region_of = [&](uint32_t idx) -> uint32_t {
+            if (idx < B.cum[nc] || nc == nf) {
+                uint32_t lo = 0, hi = nc;
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) / 2;
+                    if (B.cum[mid] <= idx)
+                        lo = mid;
+                    else
+                        hi = mid;
+                }
+                return fa.first[B.fi[lo]] + (idx - B.cum[lo]);
+            }
+            idx -= B.cum[nc];
+            for (uint32_t k = nc;; ++k) {
+                const uint32_t fi = lst[o + k];
+                const uint32_t c = fa.count[fi];
+                if (idx < c || k + 1 == nf) return fa.first[fi] + (idx < c ? idx : c - 1);
+                idx -= c;
+            }
+        };
+        // per-lane accumulators
+        KahanSum<T> sf, sa, mf, ma, maa, mfa;
+        double acc1 = 0.0;  // alpha = 1 form
+        T k_f = 0, k_a = 0;
+        const uint64_t spp = jb.spp;
+        bool first = true;
+        for (uint64_t s0 = 0; s0 < spp; s0 += L) {
+            const uint64_t s = s0 + lane;
+            const bool live = sampling && s < spp;
+            T fsv = 0, asv = 0;
+            bool empty = true;
+            double w = 0.0;
+            if (live) {
+                const SampleDraws<T, D> u(uint32_t(s), b, jb.stream, jb.seed_lo, jb.seed_hi);
+                const uint32_t reg = region_of(uniform_index(u.u[0], n));
+                T x[D], t[D], ilo[D], ihi[D];
+                empty = false;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    const T rl = r.lo[size_t(reg) * D + d], rh = r.hi[size_t(reg) * D + d];
+                    ilo[d] = vmax(blo[d], rl);
+                    ihi[d] = vmax(ilo[d], vmin(bhi[d], rh));
+                    if (ilo[d] >= ihi[d]) empty = true;
+                    x[d] = ilo[d] + u.coord(d) * (ihi[d] - ilo[d]);
+                    t[d] = pos_in(x[d], rl, rh);
+                }
+                const T fx = fs[j].template eval<M, D>(x);
+                const T px = eval_monomials<T, D>(r.coef + size_t(reg) * K, t);
+                w = factor * double(n) * double(box_volume<T, D>(ilo, ihi));
+                if (jb.form == VMX_FORM_CV_ALPHA1) {
+                    if (!empty) acc1 += double(fx - px) * w / double(spp);
+                } else {
+                    fsv = T(double(fx) * w);
+                    asv = T(double(px) * w);
+                }
+            }
+            if (jb.form == VMX_FORM_CV_ALPHA1) continue;
+            if (first) {  // the bin's first sample (lane 0, s = 0) shifts the moments
+                k_f = __shfl(vabs(fsv), 0, L);
+                k_a = __shfl(vabs(asv), 0, L);
+                first = false;
+            }
+            if (live) {
+                sf.add(fsv);
+                sa.add(asv);
+                const T df = vabs(fsv) - k_f, da = vabs(asv) - k_a;
+                mf.add(df);
+                ma.add(da);
+                maa.add(da * da);
+                mfa.add(df * da);
+            }
+        }
+        if (jb.form == VMX_FORM_CV_ALPHA1) {
+            const double tot = team_sum(acc1, L);
+            if (active && lane == 0) static_cast<T*>(jb.bins)[b] = T(approx + tot);
+        } else {
+            const double Sf = team_sum(double(sf.s) - double(sf.c), L);
+            const double Sa = team_sum(double(sa.s) - double(sa.c), L);
+            const double Ef = team_sum(double(mf.s) - double(mf.c), L);
+            const double Ea = team_sum(double(ma.s) - double(ma.c), L);
+            const double Eaa = team_sum(double(maa.s) - double(maa.c), L);
+            const double Efa = team_sum(double(mfa.s) - double(mfa.c), L);
+            if (active && lane == 0) {
+                double result;
+                const double ns = double(spp);
+                if (!sampling) {
+                    result = approx;
+                } else if (jb.form == VMX_FORM_VR_FIXED) {
+                    result = (Sf - jb.alpha * Sa) / ns + jb.alpha * approx;
+                } else if (spp < 2) {
+                    result = approx;
+                } else {
+                    const double cov = (Efa - (Ef * Ea) / ns) / (ns - 1);
+                    const double var = (Eaa - (Ea * Ea) / ns) / (ns - 1);
+                    double alpha = 1.0;
+                    const double v = var > 0.0 ? var : 0.0;
+                    if (v > 0.0) {
+                        const double c = cov > 0.0 ? (cov < v ? cov : v) : 0.0;
+                        alpha = c / v;
+                    }
+                    result = (Sf - alpha * Sa) / ns + alpha * approx;
+                }
+                static_cast<T*>(jb.bins)[b] = T(result);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace vmx
+
+#endif

@@ -225,19 +225,26 @@ VMX_HD inline void bin_marginal(const T* v, int db, A* out) {
     for (int i = 0; i < n; ++i) out[i] = w[i];
 }
 
-/* integral_subrange restricted to the bin dims on a marginal m (3^db entries):
-   folds dim db-1 first, dim 0 last.  a/b are normalised bounds. */
-template <typename A>
-VMX_HD inline A marginal_subrange(const A* m, int db, const A* a, const A* b) {
-    A w[27];
-    int n = 1;
-    for (int i = 0; i < db; ++i) n *= 3;
-    for (int i = 0; i < n; ++i) w[i] = m[i];
-    for (int d = db - 1; d >= 0; --d) {
-        n /= 3;
+/* integral_subrange restricted to the bin dims on a marginal m (3^DB entries): folds
+   dim DB-1 first, dim 0 last.  a/b are normalised bounds. */
+template <typename A, int DB>
+VMX_HD inline A marginal_subrange_n(const A* m, const A* a, const A* b) {
+    A w[pow3(DB)];
+#pragma unroll
+    for (int i = 0; i < pow3(DB); ++i) w[i] = m[i];
+#pragma unroll
+    for (int d = DB - 1; d >= 0; --d) {
+        const int n = pow3(d);
+#pragma unroll
         for (int j = 0; j < n; ++j) w[j] = simpson_subrange(a[d], b[d], w[j], w[j + n], w[j + 2 * n]);
     }
     return w[0];
+}
+template <typename A>
+VMX_HD inline A marginal_subrange(const A* m, int db, const A* a, const A* b) {
+    if (db == 1) return marginal_subrange_n<A, 1>(m, a, b);
+    if (db == 2) return marginal_subrange_n<A, 2>(m, a, b);
+    return marginal_subrange_n<A, 3>(m, a, b);
 }
 
 /* pixels_in_region: bin index box [start, end) along bin dim i. */
@@ -271,17 +278,25 @@ VMX_HD inline void monomials(const T* v, C* c) {
     }
 }
 
-/* Evaluate monomial form at normalised t (Horner, last dim outermost). */
+/* Evaluate the monomial form at normalised t: nested Horner, dim 0 innermost, written
+   as template recursion so every coefficient stays in registers. */
+template <typename C, int DD>
+struct MonoHorner {
+    VMX_HD static C eval(const C* c, const C* t) {
+        constexpr int S = pow3(DD - 1);
+        const C a0 = MonoHorner<C, DD - 1>::eval(c, t);
+        const C a1 = MonoHorner<C, DD - 1>::eval(c + S, t);
+        const C a2 = MonoHorner<C, DD - 1>::eval(c + 2 * S, t);
+        return (a2 * t[DD - 1] + a1) * t[DD - 1] + a0;
+    }
+};
+template <typename C>
+struct MonoHorner<C, 0> {
+    VMX_HD static C eval(const C* c, const C*) { return c[0]; }
+};
 template <typename C, int D>
 VMX_HD inline C eval_monomials(const C* c, const C* t) {
-    C w[pow3(D)];
-    int n = pow3(D);
-    for (int i = 0; i < n; ++i) w[i] = c[i];
-    for (int d = 0; d < D; ++d) {
-        n /= 3;
-        for (int j = 0; j < n; ++j) w[j] = (w[3 * j + 2] * t[d] + w[3 * j + 1]) * t[d] + w[3 * j];
-    }
-    return w[0];
+    return MonoHorner<C, D>::eval(c, t);
 }
 
 }  // namespace vmx
