@@ -1,0 +1,84 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures of tests/golden/ from the reference build
+(oracle/_ref/libviltrum_ref.so: adolfomunoz/viltrum's own headers, compiled in this
+container by oracle/Makefile).  Only numbers are stored (npz, float64 holds every FP32
+and FP64 value exactly); nothing of the reference travels.
+
+    make -C oracle && python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import oracle_lib as O  # noqa: E402
+from viltrum_amd import integrands as I  # noqa: E402
+
+
+def integrand_cases():
+    return {
+        "config1": I.config1(),
+        "blobs3d": I.moving_blobs(3, 8),
+        "c1fam": I.c1_family(5, 6),
+        "poly2": I.poly([(1.0, (3, 1)), (-2.0, (1, 4)), (0.5, (0, 0))], 2, step=([1.0, 1.0], 1.0, 0.5)),
+        "poly3": I.poly([(1.0, (2, 1, 3)), (0.25, (0, 5, 1))], 3, step=([1.0, -0.5, 0.7], 0.3, 1.0)),
+        "lobes5": I.config5(2),
+    }
+
+
+REGION_CASES = [  # (integrand, scalar, heuristic, iterations)
+    ("config1", 8, 0, 0), ("config1", 8, 0, 1), ("config1", 8, 0, 2), ("config1", 8, 0, 7), ("config1", 8, 0, 63),
+    ("config1", 8, 0, 1023), ("config1", 4, 0, 1023), ("config1", 8, 1, 63), ("config1", 4, 1, 255),
+    ("blobs3d", 8, 0, 300), ("blobs3d", 4, 0, 300), ("c1fam", 8, 0, 400), ("poly2", 8, 0, 200),
+    ("poly2", 8, 1, 200), ("poly3", 8, 0, 150), ("lobes5", 8, 0, 40), ("lobes5", 4, 0, 40),
+]
+
+BIN_CASES = [  # (integrand, scalar, form, heuristic, res, iterations, spp, seed, alpha)
+    ("config1", 8, 0, 0, (16, 16), 1023, 0, 0, 1.0),
+    ("config1", 8, 0, 0, (64, 64), 1023, 0, 0, 1.0),
+    ("config1", 4, 0, 0, (64, 64), 1023, 0, 0, 1.0),
+    ("config1", 8, 0, 0, (32, 32), 255, 16, 7, 1.0),
+    ("config1", 4, 0, 0, (32, 32), 255, 16, 7, 1.0),
+    ("config1", 8, 1, 0, (24, 20), 200, 8, 3, 0.5),
+    ("config1", 8, 2, 1, (16, 16), 127, 4, 11, 1.0),
+    ("config1", 8, 0, 0, (40,), 100, 5, 2, 1.0),
+    ("blobs3d", 8, 0, 0, (8, 8), 300, 4, 5, 1.0),
+    ("blobs3d", 8, 0, 0, (4, 4, 4), 100, 3, 5, 1.0),
+    ("poly3", 8, 0, 0, (8, 8), 150, 0, 0, 1.0),
+    ("lobes5", 8, 0, 0, (6, 6), 40, 2, 9, 1.0),
+]
+
+
+def main():
+    if O.ref() is None:
+        sys.exit("reference build missing: make -C oracle (needs /root/reference)")
+    fs = integrand_cases()
+    out = {}
+    for name, scalar, heur, it in REGION_CASES:
+        f = fs[name]
+        g = O.generate(f, it, scalar=scalar, heuristic=heur, use_ref=True, evals=True)
+        key = "regions/%s/s%d/h%d/it%d" % (name, scalar, heur, it)
+        out[key + "/lo"] = g.lo
+        out[key + "/hi"] = g.hi
+        out[key + "/err"] = g.err
+        out[key + "/dim"] = g.dim
+        out[key + "/evals"] = g.evals
+    for name, scalar, form, heur, res, it, spp, seed, alpha in BIN_CASES:
+        f = fs[name]
+        b, _ = O.integrate(f, res, it, spp, seed, scalar=scalar, integrator=form, heuristic=heur, alpha=alpha,
+                           use_ref=True)
+        key = "bins/%s/s%d/f%d/h%d/r%s/it%d/spp%d/seed%d/a%g" % (name, scalar, form, heur, "x".join(map(str, res)),
+                                                                it, spp, seed, alpha)
+        out[key] = b
+    path = os.path.join(HERE, "reference_fixtures.npz")
+    np.savez_compressed(path, **out)
+    print("wrote %s (%d arrays, %.0f KB)" % (path, len(out), os.path.getsize(path) / 1024))
+
+
+if __name__ == "__main__":
+    main()
