@@ -36,6 +36,52 @@ def workloads():
     }
 
 
+def run_config3(args, V, hip):
+    """256 integrals of uneven resolution, budget and spp (2-D config-1 family and 3-D
+    moving blobs, 4..64 blobs) in one batched call, against the same integrals one call
+    each.  value = total bins / batched time."""
+    from viltrum_amd import integrands as I
+    items = I.config3_batch(seed=0, count=256)
+    ctx = V.Context(0)
+    probs, fs, bufs = [], [], []
+    for it in items:
+        f = it["integrand"]
+        integ = V.integrator_crespo2021(it["regions"] - 1, it["spp"], seed=len(probs))
+        probs.append(integ.problem(f, it["res"], V.range_primary(f.dim), V.F32))
+        fs.append(f)
+        bufs.append(hip.DeviceBuffer(int(np.prod(it["res"])) * 4))
+    stream = hip.Stream()
+    total_bins = sum(int(np.prod(it["res"])) for it in items)
+    ptrs = [b.address for b in bufs]
+    for _ in range(args.warmup):
+        ctx.integrate_batch(probs, fs, ptrs, stream.address)
+    hip.device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = ctx.integrate_batch(probs, fs, ptrs, stream.address)
+    hip.device_synchronize()
+    ms_batch = (time.perf_counter() - t0) * 1e3 / args.steps
+    t0 = time.perf_counter()
+    for p, f, ptr in zip(probs, fs, ptrs):
+        ctx.integrate_device(p, f, ptr, stream.address)
+    hip.device_synchronize()
+    ms_seq = (time.perf_counter() - t0) * 1e3
+    evals = sum(x.evals_regions + x.evals_residual for x in st)
+    res = {
+        "metric": METRIC, "value": total_bins / (ms_batch * 1e-3), "unit": "bins/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_batch, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "256 seeded integrals (2-D config-1 family / 3-D moving blobs, 4..64 blobs), "
+                               "res 16^2..256^2, budgets logU[256,32768], spp {4,16,64}, one batched call",
+                   "integrals": len(items), "bins": total_bins, "path": "batched"},
+        "evals_per_s": evals / (ms_batch * 1e-3),
+        "sequential": {"ms": ms_seq, "bins_per_s": total_bins / (ms_seq * 1e-3)},
+        "batch_speedup": ms_seq / ms_batch,
+        "phase_ms": {k: float(getattr(st[0], k)) for k in ("ms_generate", "ms_select", "ms_stratify", "ms_bins")},
+    }
+    print(json.dumps(res), flush=True)
+
+
 def cpu_baseline(w, nthreads):
     """The oracle (C++ restatement of integrator_crespo2021, FP32, per-bin mt19937,
     threads over dim-0 rows) on the host cores: median of 3 full runs."""
@@ -71,6 +117,8 @@ def main():
     from viltrum_amd import hip
     from viltrum_amd.integrands import flops_per_eval
 
+    if args.workload == "config3":
+        return run_config3(args, V, hip)
     w = workloads()[args.workload]
     f = w["f"]
     ctx = V.Context(0)

@@ -1,0 +1,263 @@
+/*
+ * C++17 drop-in for viltrum's per-bin control-variate path on an MI355X.
+ *
+ * The call shape of the reference is kept (src/integrate.h:72-173):
+ *
+ *     viltrum::integrate(integrator, bins, resolution, f, range);   // accessor bins
+ *     viltrum::integrate(integrator, bins_vector_of_vectors, f, range);
+ *     viltrum::integrate(integrator, bins_vector, f, range);
+ *     auto v = viltrum::integrate(integrator, f, range);            // one bin
+ *
+ * with the integrators of the reference's factories:
+ *
+ *     integrator_crespo2021(iterations, spp, seed)                      (integrator-crespo2021.h:7-22)
+ *     integrator_adaptive_control_variates_parallel(iterations, spp, seed, heuristic)
+ *                                                                       (integrator-adaptive-control-variates.h:17-20)
+ *     integrator_variance_reduction_fixed(alpha, iterations, spp, seed) (regions-integrator-parallel-variance-reduction.h:127-130)
+ *
+ * What differs: the integrand is a device-evaluable descriptor (viltrum_amd::blobs,
+ * poly, lobes5) rather than an arbitrary host lambda, which a GPU cannot call; passing
+ * a lambda is a compile-time error, not a silent CPU fallback.  The bin value type picks
+ * the evaluation precision (float: FP32 evaluation with FP64 assembly; double: FP64).
+ *
+ * Include this header instead of <viltrum.h>; `namespace viltrum` is an alias of
+ * viltrum_amd unless VILTRUM_AMD_NO_ALIAS is defined (do not include both libraries'
+ * headers in one translation unit).  Link with -lviltrum_amd.
+ */
+#ifndef VILTRUM_AMD_VILTRUM_HPP
+#define VILTRUM_AMD_VILTRUM_HPP
+
+#include <array>
+#include <cstddef>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../viltrum_amd.h"
+
+namespace viltrum_amd {
+
+/* ---------------------------------------------------------------- ranges */
+template <typename T, std::size_t DIM>
+class Range {
+    std::array<T, DIM> lo_, hi_;
+
+public:
+    static constexpr std::size_t dimensions = DIM;
+    using value_type = T;
+    Range(const std::array<T, DIM>& a, const std::array<T, DIM>& b) : lo_(a), hi_(b) {}
+    const std::array<T, DIM>& min() const { return lo_; }
+    const std::array<T, DIM>& max() const { return hi_; }
+    T min(std::size_t i) const { return lo_[i]; }
+    T max(std::size_t i) const { return hi_[i]; }
+    T volume() const {
+        T v = T(1);
+        for (std::size_t i = 0; i < DIM; ++i) v *= (hi_[i] - lo_[i]);
+        return v;
+    }
+};
+
+template <typename T, std::size_t DIM>
+Range<T, DIM> range(const std::array<T, DIM>& a, const std::array<T, DIM>& b) {
+    return Range<T, DIM>(a, b);
+}
+
+template <std::size_t N, typename T = float>
+Range<T, N> range_primary() {
+    std::array<T, N> a, b;
+    a.fill(T(0));
+    b.fill(T(1));
+    return Range<T, N>(a, b);
+}
+
+/* ---------------------------------------------------------------- integrands */
+struct Integrand {
+    vmx_integrand desc{};
+    std::shared_ptr<void> params;
+};
+
+inline Integrand blobs(const vmx_blobs_params& p, int dim) {
+    Integrand f;
+    auto q = std::make_shared<vmx_blobs_params>(p);
+    f.params = q;
+    f.desc = vmx_integrand{VMX_FAMILY_BLOBS, dim, q.get()};
+    return f;
+}
+inline Integrand poly(const vmx_poly_params& p, int dim) {
+    Integrand f;
+    auto q = std::make_shared<vmx_poly_params>(p);
+    f.params = q;
+    f.desc = vmx_integrand{VMX_FAMILY_POLY, dim, q.get()};
+    return f;
+}
+inline Integrand lobes5(const vmx_lobes5_params& p) {
+    Integrand f;
+    auto q = std::make_shared<vmx_lobes5_params>(p);
+    f.params = q;
+    f.desc = vmx_integrand{VMX_FAMILY_LOBES5, 5, q.get()};
+    return f;
+}
+
+/* ---------------------------------------------------------------- context */
+class Context {
+    vmx_ctx* ctx_ = nullptr;
+
+public:
+    explicit Context(int device = 0) {
+        if (vmx_ctx_create(device, &ctx_) != VMX_OK) throw std::runtime_error("viltrum_amd: no usable GPU");
+    }
+    ~Context() { vmx_ctx_destroy(ctx_); }
+    Context(const Context&) = delete;
+    Context& operator=(const Context&) = delete;
+    vmx_ctx* get() const { return ctx_; }
+    void check(int rc) const {
+        if (rc != VMX_OK) throw std::runtime_error(std::string("viltrum_amd: ") + vmx_last_error(ctx_));
+    }
+    /* one context per host thread (a context is not shared between threads) */
+    static Context& thread_default() {
+        thread_local Context c(0);
+        return c;
+    }
+};
+
+class LoggerNull {
+public:
+    template <typename Data>
+    void log(const Data&) {}
+};
+
+/* ---------------------------------------------------------------- integrators */
+class IntegratorGpu {
+    vmx_problem base_{};
+
+public:
+    IntegratorGpu(std::size_t iterations, std::size_t spp, std::size_t seed, int heuristic, int form, double alpha,
+                  int order = VMX_ORDER_CANONICAL) {
+        std::memset(&base_, 0, sizeof(base_));
+        base_.heuristic = heuristic;
+        base_.form = form;
+        base_.order = order;
+        base_.refiner = VMX_REFINER_HEURISTIC;
+        base_.alpha = alpha;
+        base_.regions = iterations + 1;
+        base_.spp = spp;
+        base_.seed = seed;
+    }
+    const vmx_problem& base() const { return base_; }
+
+    /* IntegratorRegionBased::integrate (src/newton-cotes/integrator-region-based.h:13-22):
+       bins(pos) is assigned (variance-reduction forms) or accumulated (alpha = 1 form),
+       pos in dim-0-outer C order. */
+    template <typename Bins, std::size_t DB, typename Float, std::size_t DIM, typename Logger>
+    void integrate(Bins& bins, const std::array<std::size_t, DB>& res, const Integrand& f,
+                   const Range<Float, DIM>& range, Logger& logger, Context& ctx = Context::thread_default()) const {
+        static_assert(DB >= 1 && DB <= 3 && DB <= DIM, "1..3 bin dimensions, no more than the integrand's");
+        using T = std::decay_t<decltype(bins(std::array<std::size_t, DB>()))>;
+        static_assert(std::is_same<T, float>::value || std::is_same<T, double>::value,
+                      "bins must hold float (FP32 path) or double (FP64 path)");
+        if (f.desc.dim != int(DIM)) throw std::invalid_argument("viltrum_amd: range and integrand dims differ");
+        vmx_problem p = base_;
+        p.scalar = sizeof(T) == 4 ? VMX_F32 : VMX_F64;
+        p.bin_dims = int(DB);
+        for (std::size_t d = 0; d < DIM; ++d) {
+            p.lo[d] = double(range.min(d));
+            p.hi[d] = double(range.max(d));
+        }
+        std::size_t nb = 1;
+        for (std::size_t i = 0; i < DB; ++i) {
+            p.res[i] = res[i];
+            nb *= res[i];
+        }
+        std::vector<T> flat(nb, T(0));
+        auto for_each_pos = [&](auto&& fn) {
+            std::array<std::size_t, DB> pos{};
+            for (std::size_t k = 0; k < nb; ++k) {
+                std::size_t rem = k;
+                for (std::size_t i = DB; i-- > 0;) {
+                    pos[i] = rem % res[i];
+                    rem /= res[i];
+                }
+                fn(k, pos);
+            }
+        };
+        if (p.form == VMX_FORM_CV_ALPHA1) for_each_pos([&](std::size_t k, const auto& pos) { flat[k] = bins(pos); });
+        vmx_stats st{};
+        ctx.check(vmx_integrate(ctx.get(), &p, &f.desc, flat.data(), &st));
+        for_each_pos([&](std::size_t k, const auto& pos) { bins(pos) = flat[k]; });
+        logger.log(st);
+    }
+
+    template <typename Bins, std::size_t DB, typename F, typename Float, std::size_t DIM, typename Logger>
+    void integrate(Bins&, const std::array<std::size_t, DB>&, const F&, const Range<Float, DIM>&, Logger&) const {
+        static_assert(std::is_same<F, Integrand>::value,
+                      "viltrum_amd integrates device-evaluable integrand descriptors (viltrum_amd::blobs, poly, "
+                      "lobes5); a host lambda cannot run on the GPU");
+    }
+};
+
+inline IntegratorGpu integrator_crespo2021(std::size_t iterations, std::size_t spp, std::size_t seed = 0) {
+    return IntegratorGpu(iterations, spp, seed, VMX_HEUR_SIZE_RELATIVE, VMX_FORM_VR_OPTIMIZE, 1.0);
+}
+inline IntegratorGpu integrator_adaptive_control_variates_parallel(std::size_t iterations, std::size_t spp,
+                                                                   std::size_t seed = 0,
+                                                                   int heuristic = VMX_HEUR_DEFAULT_ABSOLUTE) {
+    return IntegratorGpu(iterations, spp, seed, heuristic, VMX_FORM_CV_ALPHA1, 1.0);
+}
+inline IntegratorGpu integrator_variance_reduction_fixed(double alpha, std::size_t iterations, std::size_t spp,
+                                                         std::size_t seed = 0,
+                                                         int heuristic = VMX_HEUR_SIZE_RELATIVE) {
+    return IntegratorGpu(iterations, spp, seed, heuristic, VMX_FORM_VR_FIXED, alpha);
+}
+
+/* ---------------------------------------------------------------- integrate overloads
+   (src/integrate.h:72-79, 93-97, 105-130, 132-148, 169-173) */
+template <typename Integrator, typename Bins, std::size_t DB, typename Float, std::size_t DIM, typename Logger>
+void integrate(const Integrator& integrator, Bins& bins, const std::array<std::size_t, DB>& resolution,
+               const Integrand& f, const Range<Float, DIM>& range, Logger& logger) {
+    integrator.integrate(bins, resolution, f, range, logger);
+}
+
+template <typename Integrator, typename Bins, std::size_t DB, typename Float, std::size_t DIM>
+void integrate(const Integrator& integrator, Bins& bins, const std::array<std::size_t, DB>& resolution,
+               const Integrand& f, const Range<Float, DIM>& range) {
+    LoggerNull log;
+    integrate(integrator, bins, resolution, f, range, log);
+}
+
+template <typename Integrator, typename T, typename Float, std::size_t DIM>
+void integrate(const Integrator& integrator, std::vector<std::vector<T>>& bins, const Integrand& f,
+               const Range<Float, DIM>& range) {
+    std::size_t max1 = 0;
+    for (const auto& v : bins) max1 = v.size() > max1 ? v.size() : max1;
+    for (auto& v : bins) v.resize(max1);
+    auto b = [&bins](const std::array<std::size_t, 2>& i) -> T& { return bins[i[0]][i[1]]; };
+    const std::array<std::size_t, 2> res{bins.size(), max1};
+    integrate(integrator, b, res, f, range);
+}
+
+template <typename Integrator, typename T, typename Float, std::size_t DIM>
+void integrate(const Integrator& integrator, std::vector<T>& bins, const Integrand& f, const Range<Float, DIM>& range) {
+    auto b = [&bins](const std::array<std::size_t, 1>& i) -> T& { return bins[i[0]]; };
+    const std::array<std::size_t, 1> res{bins.size()};
+    integrate(integrator, b, res, f, range);
+}
+
+template <typename Integrator, typename Float, std::size_t DIM>
+Float integrate(const Integrator& integrator, const Integrand& f, const Range<Float, DIM>& range) {
+    Float sol(0);
+    auto b = [&sol](const std::array<std::size_t, 1>&) -> Float& { return sol; };
+    const std::array<std::size_t, 1> res{1};
+    integrate(integrator, b, res, f, range);
+    return sol;
+}
+
+}  // namespace viltrum_amd
+
+#ifndef VILTRUM_AMD_NO_ALIAS
+namespace viltrum = viltrum_amd;
+#endif
+
+#endif

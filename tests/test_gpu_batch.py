@@ -1,0 +1,70 @@
+"""Batching: many integrals of uneven resolution, budget, spp and form in one launch
+sequence give, integral by integral, exactly what each gives alone."""
+import numpy as np
+import pytest
+
+import viltrum_amd as V
+from viltrum_amd import hip
+from viltrum_amd import integrands as I
+
+pytestmark = pytest.mark.gpu
+
+
+def _problems():
+    specs = [  # (integrand, res, iterations, spp, seed, form)
+        (I.config1(), (16, 16), 255, 16, 1, V.FORM_VR_OPTIMIZE),
+        (I.c1_family(3, 9), (32, 20), 600, 4, 2, V.FORM_VR_OPTIMIZE),
+        (I.moving_blobs(7, 12), (24, 24), 900, 8, 3, V.FORM_VR_OPTIMIZE),
+        (I.c1_family(4, 2), (8, 64), 50, 64, 4, V.FORM_VR_FIXED),
+        (I.moving_blobs(8, 4), (16, 16), 0, 3, 5, V.FORM_CV_ALPHA1),
+        (I.config1(), (64, 64), 2000, 0, 6, V.FORM_VR_OPTIMIZE),
+        (I.c1_family(5, 30), (48, 48), 3000, 16, 7, V.FORM_VR_OPTIMIZE),
+    ]
+    out = []
+    for f, res, it, spp, seed, form in specs:
+        heur = V.HEUR_DEFAULT_ABSOLUTE if form == V.FORM_CV_ALPHA1 else V.HEUR_SIZE_RELATIVE
+        integ = V.Integrator(it, spp, seed, heuristic=heur, form=form, alpha=0.6)
+        out.append((f, res, integ))
+    return out
+
+
+@pytest.mark.parametrize("scalar", [V.F32, V.F64])
+def test_batch_equals_individual_runs(ctx, scalar):
+    dt = np.float32 if scalar == V.F32 else np.float64
+    items = _problems()
+    singles = []
+    for f, res, integ in items:
+        b = np.zeros(res, dtype=dt)
+        V.integrate(integ, b, f, V.range_primary(f.dim), ctx=ctx)
+        singles.append(b)
+    bufs = [hip.DeviceBuffer(int(np.prod(res)) * np.dtype(dt).itemsize) for _, res, _ in items]
+    for buf in bufs:
+        buf.zero()
+    probs = [integ.problem(f, res, V.range_primary(f.dim), scalar) for f, res, integ in items]
+    stats = ctx.integrate_batch(probs, [f for f, _, _ in items], [b.address for b in bufs])
+    for k, (f, res, integ) in enumerate(items):
+        got = bufs[k].to_host(np.zeros(res, dtype=dt))
+        assert np.array_equal(got, singles[k]), k
+        assert stats[k].regions == integ.iterations + 1
+    for b in bufs:
+        b.free()
+
+
+def test_batch_of_one_hundred_small_integrals(ctx):
+    rng = np.random.default_rng(0)
+    items = []
+    for k in range(100):
+        f = I.c1_family(1000 + k, int(rng.integers(1, 10)))
+        side = int(rng.choice([4, 8, 16]))
+        items.append((f, (side, side), V.integrator_crespo2021(int(rng.integers(0, 400)), int(rng.choice([2, 4, 8])),
+                                                                seed=k)))
+    bufs = [hip.DeviceBuffer(int(np.prod(res)) * 4) for _, res, _ in items]
+    probs = [integ.problem(f, res, V.range_primary(2), V.F32) for f, res, integ in items]
+    ctx.integrate_batch(probs, [f for f, _, _ in items], [b.address for b in bufs])
+    for k in (0, 37, 99):
+        f, res, integ = items[k]
+        b = np.zeros(res, dtype=np.float32)
+        V.integrate(integ, b, f, V.range_primary(2), ctx=ctx)
+        assert np.array_equal(bufs[k].to_host(np.zeros(res, dtype=np.float32)), b)
+    for b in bufs:
+        b.free()

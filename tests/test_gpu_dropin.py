@@ -1,0 +1,25 @@
+"""The C++ drop-in header used the way the reference's own examples are written
+(tests/cpp/dropin.cpp) gives what the Python mirror gives."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import viltrum_amd as V
+from viltrum_amd import integrands as I
+
+pytestmark = pytest.mark.gpu
+EXE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "dropin")
+
+
+def test_cpp_dropin_matches_python(ctx):
+    assert os.path.exists(EXE), "build tests/cpp/dropin (python tests/cpp/build.py)"
+    out = subprocess.run([EXE], capture_output=True, text=True, timeout=120, check=True).stdout.split("\n")
+    scalar = float(out[0].split()[1])
+    assert abs(scalar - 2.0 / 3.0) < 1e-15
+    b = np.zeros((32, 32), dtype=np.float32)
+    V.integrate(V.integrator_crespo2021(255, 16, seed=1), b, I.config1(), V.range_primary(2), ctx=ctx)
+    vals = [float(x) for x in out[1].split()[1:]]
+    assert vals[0] == float(b[3, 7]) and vals[1] == float(b[20, 11])
+    assert abs(vals[2] - float(b.astype(np.float64).sum())) < 1e-6 * abs(vals[2])

@@ -174,7 +174,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st) {
         jb.alpha = p.alpha;
         jb.seed_lo = uint32_t(p.seed);
         jb.seed_hi = uint32_t(p.seed >> 32);
-        jb.stream = j;
+        jb.stream = 0;  // random streams depend on the seed only, not on the batch position
         jb.N = uint32_t(p.regions);
         jb.reg_base = uint32_t(n_reg);
         n_reg += p.regions;
