@@ -82,22 +82,162 @@ def run_config3(args, V, hip):
     print(json.dumps(res), flush=True)
 
 
-def cpu_baseline(w, nthreads):
+def run_config4(args, V, hip):
+    """Learned refinement + batching against the plain GPU path at equal RMSE.
+
+    Setup (untimed): train the priority network on the GPU (pairs the heuristic method
+    generates on 512 seeded config-1 style integrands), then take 64 held-out integrands
+    at 512x512 FP32 bins with their FP64 closed-form bins.  Target RMSE: the plain path
+    (heuristic priority, one integral per call) at 16384 regions and 16 spp.  Each path
+    then picks, over region budgets {4096, 16384}, the spp that reaches the target
+    (RMSE ~ 1/sqrt(spp) from a 16-spp probe, then checked and stepped) and the faster
+    of the two budgets.  A step is one learned+batched pass over the 64 integrals at its
+    chosen (budget, spp); value = their bins / step time.  The plain path at its chosen
+    point is timed the same way beside it."""
+    from viltrum_amd import integrands as I
+    from viltrum_amd import refiner as R
+    ctx = V.Context(0)
+    rep = R.train(ctx, count=args.train_integrands, regions=args.train_regions, epochs=args.train_epochs, seed=0,
+                  stride=args.train_stride)
+    fs = R.held_out(args.integrals)
+    res = (512, 512)
+    nb = res[0] * res[1]
+    truths = [ctx.exact_bins(f, res) for f in fs]
+    bufs = [hip.DeviceBuffer(nb * 4) for _ in fs]
+    ptrs = [b.address for b in bufs]
+    stream = hip.Stream()
+
+    def problems(regions, spp, learned):
+        it = R.integrator(regions - 1, spp, seed=0, learned=learned)
+        return [it.problem(f, res, V.range_primary(2), V.F32) for f in fs]
+
+    def run_plain(ps):
+        return [ctx.integrate_device(p, f, ptr, stream.address) for p, f, ptr in zip(ps, fs, ptrs)]
+
+    def run_batch(ps):
+        return ctx.integrate_batch(ps, fs, ptrs, stream.address)
+
+    def rmse():
+        hip.device_synchronize()
+        se = 0.0
+        out = np.zeros(res, dtype=np.float32)
+        for b, t in zip(bufs, truths):
+            b.to_host(out)
+            se += float(np.sum((out.astype(np.float64) - t) ** 2))
+        return float(np.sqrt(se / (nb * len(fs))))
+
+    def timed(fn, ps, steps):
+        for _ in range(args.warmup):
+            fn(ps)
+        hip.device_synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st = fn(ps)
+        hip.device_synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps, st
+
+    budgets = (4096, 16384)
+    run_plain(problems(16384, 16, False))
+    target = rmse()
+
+    def choose(fn, learned):
+        """(regions, spp, rmse, ms) per budget: the smallest spp whose RMSE <= target."""
+        pts = {}
+        for n in budgets:
+            seen = {}
+            fn(problems(n, 16, learned))
+            seen[16] = rmse()
+            spp = max(1, int(np.ceil(16 * (seen[16] / target) ** 2)))
+            for _ in range(12):
+                if spp not in seen:
+                    fn(problems(n, spp, learned))
+                    seen[spp] = rmse()
+                if seen[spp] <= target:
+                    if spp > 1 and (spp - 1) not in seen:
+                        fn(problems(n, spp - 1, learned))
+                        seen[spp - 1] = rmse()
+                    if spp > 1 and seen[spp - 1] <= target:
+                        spp -= 1
+                        continue
+                    break
+                spp = max(spp + 1, int(np.ceil(spp * 1.1)))
+            ms, _ = timed(fn, problems(n, spp, learned), 2)
+            pts[n] = dict(regions=n, spp=spp, rmse=seen[spp], ms=ms)
+        return pts
+
+    plain_pts = choose(run_plain, False)
+    learn_pts = choose(run_batch, True)
+    bp = min(plain_pts.values(), key=lambda x: x["ms"])
+    bl = min(learn_pts.values(), key=lambda x: x["ms"])
+    p_learn = problems(bl["regions"], bl["spp"], True)
+    ms_learn, st = timed(run_batch, p_learn, args.steps)
+    rmse_learn = rmse()
+    p_plain = problems(bp["regions"], bp["spp"], False)
+    ms_plain, st_plain = timed(run_plain, p_plain, max(1, args.steps // 2))
+    rmse_plain = rmse()
+    # the two changes apart: batching alone (heuristic priority, plain's point) and the
+    # learned priority alone (one call per integral, learned's point)
+    ms_bheur, _ = timed(run_batch, p_plain, max(1, args.steps // 2))
+    ms_lone, _ = timed(run_plain, p_learn, max(1, args.steps // 2))
+    total_bins = nb * len(fs)
+    value = total_bins / (ms_learn * 1e-3)
+    plain_bps = total_bins / (ms_plain * 1e-3)
+    flops = sum(st[i].evals_residual * I.flops_per_eval(f)[0] for i, f in enumerate(fs))
+    ms_k = float(st[0].ms_bins)
+    achieved = flops / (ms_k * 1e-3) / 1e12
+    evals = sum(x.evals_regions + x.evals_residual for x in st)
+    out = {
+        "metric": METRIC, "value": value, "unit": "bins/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_learn, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "learned refinement + batching vs plain GPU path at equal RMSE: %d held-out "
+                               "config-1 style integrands (1..8 blobs, step, wave), 512x512 fp32 bins"
+                               % len(fs),
+                   "integrals": len(fs), "bins": total_bins, "target": "plain path at 16384 regions, 16 spp",
+                   "learned_batched": {"regions": bl["regions"], "spp": bl["spp"]},
+                   "plain": {"regions": bp["regions"], "spp": bp["spp"]}, "path": "learned+batched"},
+        "plain_bins_per_s": plain_bps,
+        "speedup_vs_plain": value / plain_bps,
+        "batched_heuristic_bins_per_s": total_bins / (ms_bheur * 1e-3),
+        "learned_unbatched_bins_per_s": total_bins / (ms_lone * 1e-3),
+        "rmse_vs_fp64": {"target": target, "learned_batched": rmse_learn, "plain": rmse_plain},
+        "sweep": {"plain": list(plain_pts.values()), "learned_batched": list(learn_pts.values())},
+        "fp64_truth": {"kind": "FP64 closed-form per-bin integrals (erf, polygon area, separable wave)"},
+        "evals_per_s": evals / (ms_learn * 1e-3),
+        "phase_ms": {k: float(getattr(st[0], k)) for k in ("ms_generate", "ms_select", "ms_stratify", "ms_bins",
+                                                            "ms_total")},
+        "plain_phase_ms_per_integral": {k: float(np.mean([getattr(x, k) for x in st_plain])) for k in
+                                        ("ms_generate", "ms_select", "ms_stratify", "ms_bins", "ms_total")},
+        "roofline": {"bound": "valu", "kernel": "k_bins3 (per-bin CV + residual Monte Carlo, batched)",
+                     "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel_ms": ms_k},
+        "training": {k: rep[k] for k in ("pairs", "steps", "loss_first", "loss_train", "loss_valid", "ms_pairs",
+                                         "ms_train")},
+    }
+    if not args.no_cpu_baseline:
+        w = dict(f=fs[0], res=res, regions=bp["regions"], spp=bp["spp"], seed=0)
+        cb = cpu_baseline(w, min(16, os.cpu_count() or 1), runs=1)
+        cb["sample"] = "1 of the %d integrals, plain method at the plain path's point: " % len(fs) + cb["sample"]
+        out["cpu_baseline"] = cb
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(w, nthreads, runs=3):
     """The oracle (C++ restatement of integrator_crespo2021, FP32, per-bin mt19937,
     threads over dim-0 rows) on the host cores: median of 3 full runs."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     times = []
-    for _ in range(3):
+    for _ in range(runs):
         t0 = time.perf_counter()
         O.integrate(w["f"], w["res"], w["regions"] - 1, w["spp"], w["seed"], scalar=4,
                     integrator=O.INT_VR_OPTIMIZE, heuristic=O.HEUR_SIZE_RELATIVE, nthreads=nthreads)
         times.append(time.perf_counter() - t0)
-    t = sorted(times)[1]
+    t = sorted(times)[len(times) // 2]
     nb = int(np.prod(w["res"]))
     return {"value": nb / t, "unit": "bins/s", "cores": nthreads, "kind": "port",
-            "sample": "full workload (%d bins, %d regions, %d spp), median of 3 runs, %.2f s each"
-                      % (nb, w["regions"], w["spp"], t)}
+            "sample": "%d bins, %d regions, %d spp, median of %d runs, %.2f s each"
+                      % (nb, w["regions"], w["spp"], runs, t)}
 
 
 def main():
@@ -108,6 +248,12 @@ def main():
     ap.add_argument("--workload", default="config2")
     ap.add_argument("--truth-spp", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--integrals", type=int, default=64, help="config4: held-out integrals")
+    ap.add_argument("--train-integrands", type=int, default=512, help="config4: training integrands")
+    ap.add_argument("--train-regions", type=int, default=4096,
+                    help="config4: training budgets cycle through 1/4, 1/2, 1, 2 times this")
+    ap.add_argument("--train-epochs", type=int, default=8)
+    ap.add_argument("--train-stride", type=int, default=2, help="config4: keep every n-th generated region")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -119,6 +265,8 @@ def main():
 
     if args.workload == "config3":
         return run_config3(args, V, hip)
+    if args.workload == "config4":
+        return run_config4(args, V, hip)
     w = workloads()[args.workload]
     f = w["f"]
     ctx = V.Context(0)

@@ -83,7 +83,8 @@ enum vmx_order {
 /* Region refinement priority. */
 enum vmx_refiner {
     VMX_REFINER_HEURISTIC = 0, /* the error heuristic above */
-    VMX_REFINER_LEARNED = 1    /* the trained MLP of the context (vmx_refiner_load) */
+    VMX_REFINER_LEARNED = 1    /* the context's priority network (vmx_refiner_train / _set);
+                                  FP32, dims 1..4, canonical order */
 };
 
 typedef struct vmx_problem {
@@ -127,9 +128,9 @@ int vmx_integrate(vmx_ctx* ctx, const vmx_problem* p, const vmx_integrand* f, vo
 int vmx_integrate_device(vmx_ctx* ctx, const vmx_problem* p, const vmx_integrand* f, void* bins_device,
                          void* stream, vmx_stats* stats);
 
-/* n independent problems in one launch sequence (one cooperative generation kernel per
-   (family, dim) group; every problem keeps its own range, resolution, budget, spp,
-   seed and form).  All problems share the scalar type, region order and bin_dims.
+/* n independent problems in one launch sequence per (family, dim) group (every problem
+   keeps its own range, resolution, budget, spp, seed, form and refiner).  All problems
+   share the scalar type, region order and bin_dims.
    bins_device[i]: device bins of problem i; stats: n entries or NULL. */
 int vmx_integrate_batch(vmx_ctx* ctx, uint32_t n, const vmx_problem* problems, const vmx_integrand* integrands,
                         void* const* bins_device, void* stream, vmx_stats* stats);
@@ -145,6 +146,75 @@ int vmx_export_regions(vmx_ctx* ctx, uint64_t* n, double* lo, double* hi, double
    stderr_out (optional, FP64) receives each bin's standard error.  Host arrays. */
 int vmx_integrate_mc(vmx_ctx* ctx, const vmx_problem* p, const vmx_integrand* f, int32_t stratified, void* bins,
                      double* stderr_out);
+
+/* ---- Learned refinement priority.  No counterpart in the reference: there the error
+   heuristic (src/nested/error-heuristic.h:21-47) alone orders splits; it stays the split-
+   dimension rule, the fallback for a non-finite network output and the baseline.
+   A 4-layer MLP (F -> 64 -> 64 -> 64 -> 1, BF16 weights, FP32 accumulation on the matrix
+   cores) scores each new region from its features; it is trained on the device on
+   (region, FP64 true error) pairs the method itself generates.  Parameters are FP32,
+   laid out W0[KP][64] b0[64] W1[64][64] b1 W2[64][64] b2 w3[64] b3 mu sigma. */
+
+typedef struct vmx_train_opts {
+    uint32_t epochs;  /* passes over the training pairs (0: 8) */
+    uint32_t batch;   /* pairs per Adam step, a multiple of 64 (0: 1024) */
+    float lr;         /* initial step size, cosine-decayed to lr/20 (0: 2e-3) */
+    float holdout;    /* fraction of pairs held out for the validation loss (0: 0.05) */
+    uint64_t seed;    /* initialisation, split and shuffling */
+    int32_t target;   /* FP64 target E: 0 (default) the residual energy, the integral over
+                         the region of (f - P)^2; 1 the within-bin residual variance summed
+                         over the region's bins; 2 the reduction of 1 by one split along the
+                         heuristic's dim */
+    uint32_t stride;  /* keep every stride-th generated region as a pair (0: 1) */
+} vmx_train_opts;
+
+typedef struct vmx_train_report {
+    uint64_t pairs;       /* (region, true error) pairs generated */
+    uint64_t steps;       /* Adam steps */
+    double loss_first;    /* mean loss 0.5 (y - (t - mu)/sigma)^2 over the first epoch */
+    double loss_train;    /* ... over the last epoch */
+    double loss_valid;    /* on the held-out pairs after training */
+    double target_mean, target_std;  /* mu, sigma */
+    double ms_pairs, ms_train;
+} vmx_train_report;
+
+/* Sizes for integrands of dimension dim (1..4): parameter count, feature count F and
+   the padded feature row width KP of vmx_refiner_pairs / _forward / _grad rows. */
+int vmx_refiner_size(int32_t dim, uint64_t* n_params, int32_t* n_features, int32_t* row_width);
+
+/* Training pairs of the regions the heuristic method generates for n problems of one
+   family and dim (every stride-th node of the refinement rounds): features as BF16 bit
+   patterns (count x KP) and targets log2(E / (s^2 vol)), E the FP64 target of
+   opts->target (bins of each problem's resolution), s = max |node value|.  Only
+   opts->target and opts->stride are read (NULL: defaults).  NULL features/targets: only
+   *count is set. */
+int vmx_refiner_pairs(vmx_ctx* ctx, uint32_t n, const vmx_problem* problems, const vmx_integrand* integrands,
+                      const vmx_train_opts* opts, uint64_t cap, uint16_t* features, float* targets,
+                      uint64_t* count);
+
+/* Generate pairs as above and train the network for their dim on the device (Adam,
+   minibatches of 64-row tiles, deterministic); the context keeps the parameters. */
+int vmx_refiner_train(vmx_ctx* ctx, uint32_t n, const vmx_problem* problems, const vmx_integrand* integrands,
+                      const vmx_train_opts* opts, vmx_train_report* report);
+
+/* Copy the context's network for dim out (vmx_refiner_size floats) or in. */
+int vmx_refiner_get(vmx_ctx* ctx, int32_t dim, float* params);
+int vmx_refiner_set(vmx_ctx* ctx, int32_t dim, const float* params);
+
+/* Raw network outputs y (standardised log2 error) of n feature rows (BF16, n x KP), with
+   host params (NULL: the context's network). */
+int vmx_refiner_forward(vmx_ctx* ctx, int32_t dim, const float* params, uint64_t n, const uint16_t* features,
+                        float* y);
+
+/* Gradient of the training loss (1/n) sum 0.5 (y - (t - mu)/sigma)^2 with respect to the
+   trained parameters (all but mu, sigma) at host params, and the loss. */
+int vmx_refiner_grad(vmx_ctx* ctx, int32_t dim, const float* params, uint64_t n, const uint16_t* features,
+                     const float* targets, float* grad, double* loss);
+
+/* FP64 closed-form bins (bin count * integral over the bin, host array, C order) of a
+   2-D static blob integrand with an additive step (step_mode 0 or 1): the truth for
+   RMSE where one exists.  The reference has no counterpart. */
+int vmx_exact_bins(vmx_ctx* ctx, const vmx_problem* p, const vmx_integrand* f, double* bins);
 
 /* Evaluate the integrand on the device at n points x (n*D, row-major). */
 int vmx_eval(vmx_ctx* ctx, const vmx_integrand* f, int32_t scalar, uint64_t n, const double* x, double* out);

@@ -1,0 +1,541 @@
+/*
+ * TEST INFRASTRUCTURE ONLY -- CPU restatement of the learned refinement priority:
+ * region features, the FP64 training targets (residual energy, within-bin residual
+ * variance and its split gain), and the 4-layer MLP forward and backward pass.
+ * Part of liboracle.so; never linked by the product.  The reference has no learned
+ * component, so these follow the definitions documented in the product headers
+ * (features, target, network, rounding) and are pinned by mathematics instead:
+ *   vmo_nn_grad (exact mode)     central finite differences of vmo_nn_forward
+ *   vmo_nn_forward (exact mode)  an independent numpy evaluation in the tests
+ *   vmo_region_features          hand-computed regions (tests)
+ *   vmo_true_error               closed forms on polynomial integrands (tests)
+ *   vmo_bin_variances            brute-force midpoint sums over fine grids (tests)
+ * The "emulate" mode rounds exactly where the device kernels round (BF16 weights,
+ * activations and deltas; FP32 sums), so device results agree to accumulation order.
+ */
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../include/viltrum_amd/integrands.h"
+
+namespace {
+
+constexpr int W = 64;
+
+int pow3i(int d) { return d <= 0 ? 1 : 3 * pow3i(d - 1); }
+
+uint16_t to_bf16(float x) {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return uint16_t((u >> 16) | 0x40u);
+    const uint32_t lsb = (u >> 16) & 1u;
+    u += 0x7fffu + lsb;
+    return uint16_t(u >> 16);
+}
+float from_bf16(uint16_t h) {
+    const uint32_t u = uint32_t(h) << 16;
+    float x;
+    std::memcpy(&x, &u, 4);
+    return x;
+}
+float rbf(float x) { return from_bf16(to_bf16(x)); }
+
+struct Net {
+    int kp;
+    // offsets of the flat parameter array
+    size_t w0, b0, w1, b1, w2, b2, w3, b3, mu, sigma, trained;
+    explicit Net(int k) : kp(k) {
+        w0 = 0;
+        b0 = w0 + size_t(kp) * W;
+        w1 = b0 + W;
+        b1 = w1 + W * W;
+        w2 = b1 + W;
+        b2 = w2 + W * W;
+        w3 = b2 + W;
+        b3 = w3 + W;
+        trained = b3 + 1;
+        mu = trained;
+        sigma = trained + 1;
+    }
+};
+
+/* Activations of one row, kept for the backward pass. */
+struct Acts {
+    std::vector<double> x, z0, h0, z1, h1, z2, h2;
+    double y;
+};
+
+/* Forward pass of one row.  emulate: device rounding; else exact FP64. */
+void forward_row(const Net& n, const float* p, const uint16_t* xr, bool emulate, Acts& a) {
+    const int kp = n.kp;
+    a.x.assign(kp, 0.0);
+    for (int k = 0; k < kp; ++k) a.x[k] = from_bf16(xr[k]);
+    auto wv = [&](size_t i) -> double { return emulate ? double(rbf(p[i])) : double(p[i]); };
+    auto layer = [&](const std::vector<double>& in, int K, size_t ow, size_t ob, std::vector<double>& z,
+                     std::vector<double>& h) {
+        z.assign(W, 0.0);
+        h.assign(W, 0.0);
+        for (int o = 0; o < W; ++o) {
+            double s = 0.0;
+            for (int k = 0; k < K; ++k) s += in[k] * wv(ow + size_t(k) * W + o);
+            if (emulate) {
+                const float zf = float(s) + p[ob + o];
+                z[o] = zf;
+                h[o] = rbf(zf > 0.f ? zf : 0.f);
+            } else {
+                z[o] = s + double(p[ob + o]);
+                h[o] = z[o] > 0.0 ? z[o] : 0.0;
+            }
+        }
+    };
+    layer(a.x, kp, n.w0, n.b0, a.z0, a.h0);
+    layer(a.h0, W, n.w1, n.b1, a.z1, a.h1);
+    layer(a.h1, W, n.w2, n.b2, a.z2, a.h2);
+    if (emulate) {
+        float q[4];
+        for (int part = 0; part < 4; ++part) {
+            float s = 0.f;
+            for (int k = 0; k < 16; ++k) s += float(a.h2[part * 16 + k]) * rbf(p[n.w3 + part * 16 + k]);
+            q[part] = s;
+        }
+        a.y = double(((q[0] + q[1]) + (q[2] + q[3])) + p[n.b3]);
+    } else {
+        double s = 0.0;
+        for (int k = 0; k < W; ++k) s += a.h2[k] * double(p[n.w3 + k]);
+        a.y = s + double(p[n.b3]);
+    }
+}
+
+thread_local std::string g_err;
+
+}  // namespace
+
+extern "C" {
+
+const char* vmo_nn_last_error(void) { return g_err.c_str(); }
+
+/* feature count F and padded row width KP for dimension D */
+int vmo_nn_dims(int D, int* F, int* KP) {
+    if (D < 1 || D > 6) return -1;
+    const int f = 4 * D + pow3i(D) + 1;
+    if (F) *F = f;
+    if (KP) *KP = (f + 31) / 32 * 32;
+    return 0;
+}
+
+uint16_t vmo_f2bf(float x) { return to_bf16(x); }
+
+/* y[i] = raw network output of row i */
+int vmo_nn_forward(int kp, const float* params, uint64_t n, const uint16_t* X, int emulate, double* y) {
+    if (kp % 32 != 0 || kp <= 0) return -1;
+    Net net(kp);
+    Acts a;
+    for (uint64_t i = 0; i < n; ++i) {
+        forward_row(net, params, X + i * kp, emulate != 0, a);
+        y[i] = a.y;
+    }
+    return 0;
+}
+
+/* Gradient of L = (1/n) sum_i 0.5 (y_i - (t_i - mu)/sigma)^2 over the trained
+   parameters.  emulate: device rounding and summation blocking (64-row tiles, tiles
+   summed in order); else exact FP64. */
+int vmo_nn_grad(int kp, const float* params, uint64_t n, const uint16_t* X, const float* t, int emulate,
+                double* grad, double* loss) {
+    if (kp % 32 != 0 || kp <= 0 || n == 0) return -1;
+    Net net(kp);
+    const bool em = emulate != 0;
+    const size_t P = net.trained;
+    std::vector<double> total(P, 0.0);
+    double ltotal = 0.0;
+    auto wv = [&](size_t i) -> double { return em ? double(rbf(params[i])) : double(params[i]); };
+    const float inv_f = 1.f / float(n);
+    const uint64_t tile = em ? 64 : n;
+    std::vector<Acts> acts;
+    for (uint64_t base = 0; base < n; base += tile) {
+        const uint64_t rows = std::min<uint64_t>(tile, n - base);
+        acts.assign(rows, Acts());
+        std::vector<double> dy(rows), part(P, 0.0);
+        double lsum = 0.0;
+        for (uint64_t r = 0; r < rows; ++r) {
+            forward_row(net, params, X + (base + r) * kp, em, acts[r]);
+            if (em) {
+                const float tn = (t[base + r] - params[net.mu]) / params[net.sigma];
+                const float d = float(acts[r].y) - tn;
+                dy[r] = d * inv_f;
+                lsum += double(0.5f * d * d * inv_f);
+            } else {
+                const double tn = (double(t[base + r]) - double(params[net.mu])) / double(params[net.sigma]);
+                const double d = acts[r].y - tn;
+                dy[r] = d / double(n);
+                lsum += 0.5 * d * d / double(n);
+            }
+        }
+        // output layer
+        for (int k = 0; k < W; ++k) {
+            double s = 0.0;
+            float sf = 0.f;
+            for (uint64_t r = 0; r < rows; ++r) {
+                if (em)
+                    sf += float(acts[r].h2[k]) * float(dy[r]);
+                else
+                    s += acts[r].h2[k] * dy[r];
+            }
+            part[net.w3 + k] = em ? double(sf) : s;
+        }
+        {
+            double s = 0.0;
+            float sf = 0.f;
+            for (uint64_t r = 0; r < rows; ++r) {
+                if (em)
+                    sf += float(dy[r]);
+                else
+                    s += dy[r];
+            }
+            part[net.b3] = em ? double(sf) : s;
+        }
+        // deltas of the last hidden layer
+        std::vector<std::vector<double>> g(rows, std::vector<double>(W));
+        for (uint64_t r = 0; r < rows; ++r)
+            for (int k = 0; k < W; ++k) {
+                if (em)
+                    g[r][k] = acts[r].h2[k] > 0 ? double(rbf(float(dy[r]) * rbf(params[net.w3 + k]))) : 0.0;
+                else
+                    g[r][k] = acts[r].z2[k] > 0 ? dy[r] * double(params[net.w3 + k]) : 0.0;
+            }
+        // hidden layers 2, 1, then the input layer
+        for (int layer = 2; layer >= 0; --layer) {
+            const size_t ow = layer == 2 ? net.w2 : (layer == 1 ? net.w1 : net.w0);
+            const size_t ob = layer == 2 ? net.b2 : (layer == 1 ? net.b1 : net.b0);
+            const int K = layer == 0 ? kp : W;
+            auto hin = [&](uint64_t r, int k) -> double {
+                return layer == 2 ? acts[r].h1[k] : (layer == 1 ? acts[r].h0[k] : acts[r].x[k]);
+            };
+            for (int k = 0; k < K; ++k)
+                for (int o = 0; o < W; ++o) {
+                    double s = 0.0;
+                    for (uint64_t r = 0; r < rows; ++r) s += hin(r, k) * g[r][o];
+                    part[ow + size_t(k) * W + o] = em ? double(float(s)) : s;
+                }
+            for (int o = 0; o < W; ++o) {
+                double s = 0.0;
+                float sf = 0.f;
+                for (uint64_t r = 0; r < rows; ++r) {
+                    if (em)
+                        sf += float(g[r][o]);
+                    else
+                        s += g[r][o];
+                }
+                part[ob + o] = em ? double(sf) : s;
+            }
+            if (layer == 0) break;
+            std::vector<std::vector<double>> gn(rows, std::vector<double>(W));
+            for (uint64_t r = 0; r < rows; ++r)
+                for (int k = 0; k < W; ++k) {
+                    double s = 0.0;
+                    for (int o = 0; o < W; ++o) s += g[r][o] * wv(ow + size_t(k) * W + o);
+                    const double hk = layer == 2 ? acts[r].h1[k] : acts[r].h0[k];
+                    const double zk = layer == 2 ? acts[r].z1[k] : acts[r].z0[k];
+                    if (em)
+                        gn[r][k] = hk > 0 ? double(rbf(float(s))) : 0.0;
+                    else
+                        gn[r][k] = zk > 0 ? s : 0.0;
+                }
+            g.swap(gn);
+        }
+        if (em) {
+            for (size_t i = 0; i < P; ++i) total[i] = double(float(total[i]) + float(part[i]));
+            ltotal += lsum;
+        } else {
+            for (size_t i = 0; i < P; ++i) total[i] += part[i];
+            ltotal += lsum;
+        }
+    }
+    for (size_t i = 0; i < P; ++i) grad[i] = total[i];
+    if (loss) *loss = ltotal;
+    return 0;
+}
+
+/* Features of one FP32 region (values given as doubles that hold floats): centre,
+   -depth/16, (log2 bins - depth)/16 per dim, nodes / s, Simpson-folded |S - T| per dim
+   / s, depth / 32, zero padding to KP.  Computed in float with the documented operation
+   order.  lres[d]: log2 of the bin count along d (0 for integrated dims). */
+int vmo_region_features(int D, const double* lo, const double* hi, const double* rlo, const double* rhi,
+                        const double* v, const uint32_t* code, const float* lres, float* x, float* s_out,
+                        int* depth_out) {
+    if (D < 1 || D > 6) return -1;
+    int F = 0, KP = 0;
+    vmo_nn_dims(D, &F, &KP);
+    const int K = pow3i(D), KM = pow3i(D - 1);
+    float s = 0.f;
+    for (int k = 0; k < K; ++k) {
+        const float a = std::fabs(float(v[k]));
+        s = (s < a) ? a : s;
+    }
+    const float se = s < 0x1p-100f ? 0x1p-100f : s;
+    const float inv = 1.f / se;
+    int depth = 0;
+    for (int d = 0; d < D; ++d) {
+        const float l = float(lo[d]), h = float(hi[d]), rl = float(rlo[d]), rh = float(rhi[d]);
+        x[d] = (0.5f * (l + h) - rl) / (rh - rl);
+        int nd = 0;
+        for (uint32_t c = code[d]; c > 1u; c >>= 1) ++nd;
+        depth += nd;
+        x[D + d] = -float(nd) * (1.f / 16.f);
+        x[2 * D + d] = (lres[d] - float(nd)) * (1.f / 16.f);
+    }
+    for (int k = 0; k < K; ++k) x[3 * D + k] = float(v[k]) * inv;
+    for (int d = 0; d < D; ++d) {
+        const int below = pow3i(d);
+        float acc = 0.f;
+        for (int jl = 0; jl < KM; ++jl) {
+            const int full = (jl % below) + (jl / below) * below * 3;
+            const float p0 = float(v[full]), p1 = float(v[full + below]), p2 = float(v[full + 2 * below]);
+            const float S = float((p0 + 4.0 * p1 + p2) / 6.0);
+            const float T = float((p0 + p2) / 2.0);
+            float w = 1.f;
+            int r = jl;
+            for (int e = 0; e < D - 1; ++e) {
+                w *= (r % 3 == 1) ? (4.f / 6.f) : (1.f / 6.f);
+                r /= 3;
+            }
+            acc += w * std::fabs(S - T);
+        }
+        x[3 * D + K + d] = acc * inv;
+    }
+    x[4 * D + K] = float(depth) * (1.f / 32.f);
+    for (int i = F; i < KP; ++i) x[i] = 0.f;
+    if (s_out) *s_out = s;
+    if (depth_out) *depth_out = depth;
+    return 0;
+}
+
+/* integral over the box of (f - P)^2, P the tensor quadratic through the 3^D node values
+   v (node k at {0, 1/2, 1}^D, dim 0 fastest), by a composite 3-point Gauss-Legendre rule
+   with Q cells per dim; f evaluated in double. */
+/* f(x) in double for any built family */
+int vmo_eval_f64(const vmx_integrand* f, const double* x, double* y) {
+    struct M {
+        static double exp(double a) { return std::exp(a); }
+        static double sin(double a) { return std::sin(a); }
+        static double cos(double a) { return std::cos(a); }
+    };
+    const int D = f->dim;
+    switch (f->family) {
+        case VMX_FAMILY_POLY: {
+            const auto p = vmx::Poly<double>::from(*static_cast<const vmx_poly_params*>(f->params));
+            switch (D) {
+                case 1: *y = p.eval<M, 1>(x); return 0;
+                case 2: *y = p.eval<M, 2>(x); return 0;
+                case 3: *y = p.eval<M, 3>(x); return 0;
+                case 4: *y = p.eval<M, 4>(x); return 0;
+                case 5: *y = p.eval<M, 5>(x); return 0;
+                default: return -1;
+            }
+        }
+        case VMX_FAMILY_BLOBS: {
+            const auto p = vmx::Blobs<double>::from(*static_cast<const vmx_blobs_params*>(f->params));
+            if (D == 2) *y = p.eval<M, 2>(x);
+            else if (D == 3) *y = p.eval<M, 3>(x);
+            else return -1;
+            return 0;
+        }
+        case VMX_FAMILY_LOBES5: {
+            const auto p = vmx::Lobes5<double>::from(*static_cast<const vmx_lobes5_params*>(f->params));
+            if (D != 5) return -1;
+            *y = p.eval<M, 5>(x);
+            return 0;
+        }
+        default: return -1;
+    }
+}
+
+double vmo_true_error(const vmx_integrand* f, const double* lo, const double* hi, const double* v, int Q) {
+    const int D = f->dim;
+    if (D < 1 || D > 6 || Q < 1) return NAN;
+    const int K = pow3i(D), G = 3 * Q;
+    auto eval = [&](const double* x) -> double {
+        struct M {
+            static double exp(double a) { return std::exp(a); }
+            static double sin(double a) { return std::sin(a); }
+            static double cos(double a) { return std::cos(a); }
+        };
+        switch (f->family) {
+            case VMX_FAMILY_POLY: {
+                const auto p = vmx::Poly<double>::from(*static_cast<const vmx_poly_params*>(f->params));
+                switch (D) {
+                    case 1: return p.eval<M, 1>(x);
+                    case 2: return p.eval<M, 2>(x);
+                    case 3: return p.eval<M, 3>(x);
+                    case 4: return p.eval<M, 4>(x);
+                    default: return p.eval<M, 5>(x);
+                }
+            }
+            case VMX_FAMILY_BLOBS: {
+                const auto p = vmx::Blobs<double>::from(*static_cast<const vmx_blobs_params*>(f->params));
+                return D == 2 ? p.eval<M, 2>(x) : p.eval<M, 3>(x);
+            }
+            default: {
+                const auto p = vmx::Lobes5<double>::from(*static_cast<const vmx_lobes5_params*>(f->params));
+                return p.eval<M, 5>(x);
+            }
+        }
+    };
+    const double gx[3] = {0.5 - 0.5 * std::sqrt(0.6), 0.5, 0.5 + 0.5 * std::sqrt(0.6)};
+    const double gw[3] = {5.0 / 18.0, 8.0 / 18.0, 5.0 / 18.0};
+    long total = 1;
+    for (int d = 0; d < D; ++d) total *= G;
+    double vol = 1.0;
+    for (int d = 0; d < D; ++d) vol *= hi[d] - lo[d];
+    double acc = 0.0;
+    std::vector<double> u(D), x(D);
+    for (long q = 0; q < total; ++q) {
+        long r = q;
+        double w = 1.0;
+        for (int d = 0; d < D; ++d) {
+            const int i = int(r % G);
+            r /= G;
+            u[d] = (double(i / 3) + gx[i % 3]) / double(Q);
+            w *= gw[i % 3] / double(Q);
+            x[d] = lo[d] + u[d] * (hi[d] - lo[d]);
+        }
+        double P = 0.0;
+        for (int k = 0; k < K; ++k) {
+            double b = v[k];
+            int kk = k;
+            for (int d = 0; d < D; ++d) {
+                const int j = kk % 3;
+                kk /= 3;
+                const double t = u[d];
+                b *= j == 0 ? 2.0 * (t - 0.5) * (t - 1.0) : (j == 1 ? -4.0 * t * (t - 1.0) : 2.0 * t * (t - 0.5));
+            }
+            P += b;
+        }
+        const double res = eval(x.data()) - P;
+        acc += w * res * res;
+    }
+    return vol * acc;
+}
+
+/* Within-bin residual variance of the box [lo, hi] with node values vp, summed over the
+   bins of the grid (bins of width step[d] from glo[d] along dims d < db) that meet it:
+   sum_b [int_{b^r} g^2 - (int_{b^r} g)^2 / |b^r|], g = f - P.  With v0, v1 non-null,
+   also the same sum for the two children split at the middle of `dim` (*Vc).  Boxes
+   meeting more than maxb bins along a dim use maxb evenly spaced bins (index
+   i0 + floor((k + 1/2) cnt / maxb)), scaled by cnt / maxb.  Each bin piece, cut at the
+   split plane, is integrated by 3-point Gauss-Legendre per dim and sub-piece. */
+int vmo_bin_variances(const vmx_integrand* f, int db, const double* glo, const double* step, const uint64_t* res,
+                      const double* lo, const double* hi, const double* vp, int dim, const double* v0,
+                      const double* v1, int maxb, double* Vp, double* Vc) {
+    const int D = f->dim;
+    if (D < 1 || D > 6 || db < 0 || db > D || maxb < 1) return -1;
+    const double mid = lo[dim] + (hi[dim] - lo[dim]) / 2.0;
+    std::vector<double> clo[2], chi[2];
+    for (int c = 0; c < 2; ++c) {
+        clo[c].assign(lo, lo + D);
+        chi[c].assign(hi, hi + D);
+    }
+    chi[0][dim] = mid;
+    clo[1][dim] = mid;
+    std::vector<std::vector<uint64_t>> bins(db);
+    double scale = 1.0;
+    for (int d = 0; d < db; ++d) {
+        long a = long(std::floor((lo[d] - glo[d]) / step[d]));
+        long b = long(std::ceil((hi[d] - glo[d]) / step[d]));
+        a = std::max(0L, std::min(a, long(res[d]) - 1));
+        b = std::min(b, long(res[d]));
+        b = std::max(b, a + 1);
+        const uint64_t cnt = uint64_t(b - a);
+        if (cnt <= uint64_t(maxb)) {
+            for (uint64_t k = 0; k < cnt; ++k) bins[d].push_back(uint64_t(a) + k);
+        } else {
+            for (int k = 0; k < maxb; ++k)
+                bins[d].push_back(uint64_t(a) + uint64_t((k + 0.5) * double(cnt) / double(maxb)));
+            scale *= double(cnt) / double(maxb);
+        }
+    }
+    auto feval = [&](const double* x) {
+        double y = 0;
+        vmo_eval_f64(f, x, &y);
+        return y;
+    };
+    const double gx[3] = {0.5 - 0.5 * std::sqrt(0.6), 0.5, 0.5 + 0.5 * std::sqrt(0.6)};
+    const double gw[3] = {5.0 / 18.0, 8.0 / 18.0, 5.0 / 18.0};
+    auto quad = [&](const double* u, const double* plo, const double* phi, const double* v) {
+        // tensor quadratic through v on [plo, phi] at the point u (absolute coordinates)
+        const int K = pow3i(D);
+        double P = 0.0;
+        for (int k = 0; k < K; ++k) {
+            double b = v[k];
+            int kk = k;
+            for (int d = 0; d < D; ++d) {
+                const int j = kk % 3;
+                kk /= 3;
+                const double t = (u[d] - plo[d]) / (phi[d] - plo[d]);
+                b *= j == 0 ? 2.0 * (t - 0.5) * (t - 1.0) : (j == 1 ? -4.0 * t * (t - 1.0) : 2.0 * t * (t - 0.5));
+            }
+            P += b;
+        }
+        return P;
+    };
+    const bool kids = v0 && v1;
+    double vp_sum = 0.0, vc_sum = 0.0;
+    long total = 1;
+    for (int d = 0; d < db; ++d) total *= long(bins[d].size());
+    for (long idx = 0; idx < total; ++idx) {
+        std::vector<double> a(lo, lo + D), b(hi, hi + D);
+        long r = idx;
+        bool empty = false;
+        for (int d = 0; d < db; ++d) {
+            const uint64_t bin = bins[d][r % long(bins[d].size())];
+            r /= long(bins[d].size());
+            a[d] = std::max(lo[d], glo[d] + double(bin) * step[d]);
+            b[d] = std::min(hi[d], glo[d] + double(bin + 1) * step[d]);
+            if (!(a[d] < b[d])) empty = true;
+        }
+        if (empty) continue;
+        double vol = 1.0;
+        for (int d = 0; d < D; ++d) vol *= b[d] - a[d];
+        double p1 = 0.0, p2 = 0.0;
+        for (int c = 0; c < 2; ++c) {
+            std::vector<double> sa(a), sb(b);
+            sa[dim] = std::max(a[dim], clo[c][dim]);
+            sb[dim] = std::min(b[dim], chi[c][dim]);
+            if (!(sa[dim] < sb[dim])) continue;
+            double c1 = 0.0, c2 = 0.0, cv = 1.0;
+            for (int d = 0; d < D; ++d) cv *= sb[d] - sa[d];
+            long pts = 1;
+            for (int d = 0; d < D; ++d) pts *= 3;
+            std::vector<double> x(D);
+            for (long q = 0; q < pts; ++q) {
+                long rq = q;
+                double w = 1.0;
+                for (int d = 0; d < D; ++d) {
+                    const int g = int(rq % 3);
+                    rq /= 3;
+                    x[d] = sa[d] + gx[g] * (sb[d] - sa[d]);
+                    w *= gw[g] * (sb[d] - sa[d]);
+                }
+                const double fx = feval(x.data());
+                const double gp = fx - quad(x.data(), lo, hi, vp);
+                p1 += w * gp;
+                p2 += w * gp * gp;
+                if (kids) {
+                    const double gc = fx - quad(x.data(), clo[c].data(), chi[c].data(), c == 0 ? v0 : v1);
+                    c1 += w * gc;
+                    c2 += w * gc * gc;
+                }
+            }
+            if (kids) vc_sum += scale * (c2 - c1 * c1 / cv);
+        }
+        vp_sum += scale * (p2 - p1 * p1 / vol);
+    }
+    *Vp = vp_sum;
+    if (Vc) *Vc = vc_sum;
+    return 0;
+}
+
+}  // extern "C"

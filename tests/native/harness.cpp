@@ -1,13 +1,15 @@
 /*
  * TEST INFRASTRUCTURE: the product's device arithmetic (viltrum_amd/csrc/region_math.h,
- * philox.h) compiled for the host with g++, exposed through C entry points so tests can
+ * philox.h, features.h) compiled for the host with g++, exposed through C entry points so tests can
  * check it against the oracle on the CPU.  Built by tests/native/build.py.
  */
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 
 #include "../../viltrum_amd/csrc/philox.h"
 #include "../../viltrum_amd/csrc/region_math.h"
+#include "../../viltrum_amd/csrc/features.h"
 
 namespace {
 template <typename T, int D>
@@ -131,4 +133,32 @@ void vmh_pixel_range(int scalar, double rlo, double rhi, double lo, double hi, u
     else
         vmx::pixel_range<double>(rlo, rhi, lo, hi, res, se[0], se[1]);
 }
+
+/* the region features of the priority network (FP32 regions), as the kernels compute them */
+int vmh_region_features(int D, const double* lo, const double* hi, const double* rlo, const double* rhi,
+                        const double* v, const uint32_t* code, const float* lres, float* x, float* s, int* depth) {
+    auto run = [&](auto dd) {
+        constexpr int DD = decltype(dd)::value;
+        constexpr int K = vmx::pow3(DD), KP = vmx::nn::Dims<DD>::KP;
+        float l[DD], h[DD], rl[DD], rh[DD], vv[K];
+        for (int d = 0; d < DD; ++d) {
+            l[d] = float(lo[d]);
+            h[d] = float(hi[d]);
+            rl[d] = float(rlo[d]);
+            rh[d] = float(rhi[d]);
+        }
+        for (int k = 0; k < K; ++k) vv[k] = float(v[k]);
+        vmx::nn::region_features<float, DD, KP>(l, h, rl, rh, vv, code, lres, [x](int i, float val) { x[i] = val; },
+                                                *s, *depth);
+        return 0;
+    };
+    switch (D) {
+        case 1: return run(std::integral_constant<int, 1>{});
+        case 2: return run(std::integral_constant<int, 2>{});
+        case 3: return run(std::integral_constant<int, 3>{});
+        case 4: return run(std::integral_constant<int, 4>{});
+        default: return -1;
+    }
+}
+uint16_t vmh_f2bf(float x) { return vmx::nn::f2bf(x); }
 }

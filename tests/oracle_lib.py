@@ -193,3 +193,108 @@ def blobs2d_exact_bins(f, res0, res1, lo=(0.0, 0.0), hi=(1.0, 1.0)):
     if rc < 0:
         raise RuntimeError(lib.vmo_last_error().decode())
     return out.reshape(res0, res1)
+
+
+# ---------------------------------------------------------------- learned priority
+
+def nn_dims(D):
+    F, KP = C.c_int(), C.c_int()
+    if oracle().vmo_nn_dims(C.c_int(D), C.byref(F), C.byref(KP)) != 0:
+        raise ValueError("bad D")
+    return F.value, KP.value
+
+
+def f2bf(x):
+    """float32 array -> BF16 bit patterns (round to nearest even), via the oracle."""
+    lib = oracle()
+    lib.vmo_f2bf.restype = C.c_uint16
+    lib.vmo_f2bf.argtypes = [C.c_float]
+    x = np.asarray(x, dtype=np.float32)
+    return np.array([lib.vmo_f2bf(float(v)) for v in x.ravel()], dtype=np.uint16).reshape(x.shape)
+
+
+def nn_forward(kp, params, X, emulate=True):
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.uint16)
+    y = np.zeros(X.shape[0])
+    rc = oracle().vmo_nn_forward(C.c_int(kp), params.ctypes.data_as(C.c_void_p), C.c_uint64(X.shape[0]),
+                                 X.ctypes.data_as(C.c_void_p), C.c_int(int(emulate)), _ptr(y))
+    if rc != 0:
+        raise RuntimeError("vmo_nn_forward failed")
+    return y
+
+
+def nn_grad(kp, params, X, t, emulate=True):
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.uint16)
+    t = np.ascontiguousarray(t, dtype=np.float32)
+    g = np.zeros(params.size - 2)
+    loss = C.c_double()
+    rc = oracle().vmo_nn_grad(C.c_int(kp), params.ctypes.data_as(C.c_void_p), C.c_uint64(X.shape[0]),
+                              X.ctypes.data_as(C.c_void_p), t.ctypes.data_as(C.c_void_p), C.c_int(int(emulate)),
+                              _ptr(g), C.byref(loss))
+    if rc != 0:
+        raise RuntimeError("vmo_nn_grad failed")
+    return g, loss.value
+
+
+def lres_of(res, D):
+    """log2 of the bin counts per dim as the library computes it (float of the double
+    log2), 0 for integrated dims."""
+    out = np.zeros(D, dtype=np.float32)
+    for d, r in enumerate(res):
+        out[d] = np.float32(np.log2(float(r)))
+    return out
+
+
+def region_features(D, lo, hi, rlo, rhi, v, code, lres=None):
+    F, KP = nn_dims(D)
+    x = np.zeros(KP, dtype=np.float32)
+    s = C.c_float()
+    depth = C.c_int()
+    a = [np.ascontiguousarray(z, dtype=np.float64) for z in (lo, hi, rlo, rhi, v)]
+    code = np.ascontiguousarray(code, dtype=np.uint32)
+    lres = np.ascontiguousarray(np.zeros(D) if lres is None else lres, dtype=np.float32)
+    rc = oracle().vmo_region_features(C.c_int(D), *[_ptr(z) for z in a], code.ctypes.data_as(C.c_void_p),
+                                      lres.ctypes.data_as(C.c_void_p), x.ctypes.data_as(C.c_void_p), C.byref(s),
+                                      C.byref(depth))
+    if rc != 0:
+        raise RuntimeError("vmo_region_features failed")
+    return x, s.value, depth.value
+
+
+def true_error(f, lo, hi, v, Q):
+    lib = oracle()
+    lib.vmo_true_error.restype = C.c_double
+    a = [np.ascontiguousarray(z, dtype=np.float64) for z in (lo, hi, v)]
+    return lib.vmo_true_error(f.ptr(), *[_ptr(z) for z in a], C.c_int(Q))
+
+
+def eval_f64(f, x):
+    lib = oracle()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = C.c_double()
+    if lib.vmo_eval_f64(f.ptr(), _ptr(x), C.byref(y)) != 0:
+        raise RuntimeError("vmo_eval_f64 failed")
+    return y.value
+
+
+def bin_variances(f, res, lo, hi, vp, dim, v0=None, v1=None, glo=None, ghi=None, maxb=4):
+    """(V of the region, V of its children or None) on the bin grid res over [glo, ghi]."""
+    D = f.dim
+    db = len(res)
+    glo = np.zeros(db) if glo is None else np.asarray(glo, dtype=np.float64)
+    ghi = np.ones(db) if ghi is None else np.asarray(ghi, dtype=np.float64)
+    step = np.ascontiguousarray((ghi - glo) / np.asarray(res, dtype=np.float64))
+    glo = np.ascontiguousarray(glo)
+    r = np.ascontiguousarray(np.asarray(res, dtype=np.uint64))
+    a = [np.ascontiguousarray(z, dtype=np.float64) for z in (lo, hi, vp)]
+    kids = v0 is not None
+    k0 = np.ascontiguousarray(v0, dtype=np.float64) if kids else None
+    k1 = np.ascontiguousarray(v1, dtype=np.float64) if kids else None
+    Vp, Vc = C.c_double(), C.c_double()
+    rc = oracle().vmo_bin_variances(f.ptr(), C.c_int(db), _ptr(glo), _ptr(step), _ptr(r, _up), *[_ptr(z) for z in a],
+                                    C.c_int(dim), _ptr(k0), _ptr(k1), C.c_int(maxb), C.byref(Vp), C.byref(Vc))
+    if rc != 0:
+        raise RuntimeError("vmo_bin_variances failed")
+    return Vp.value, (Vc.value if kids else None)

@@ -1,0 +1,211 @@
+"""GPU checks of the learned refinement priority: the MFMA forward and backward kernels
+against the CPU oracle, the training pairs (features, FP64 true error) against the
+oracle's restatement on the exported regions, training on the device, and what the
+trained network gains over the heuristic at equal region budget."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import viltrum_amd as V
+from viltrum_amd import integrands as I
+from viltrum_amd import refiner as R
+
+from test_refiner_oracle import layout, random_net, random_rows
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("D,n", [(2, 1000), (3, 77), (4, 130)])
+def test_forward_matches_oracle(ctx, D, n):
+    F, kp = O.nn_dims(D)
+    assert V.refiner_size(D) == (layout(kp)["trained"] + 2, F, kp)
+    p = random_net(kp, F, 50 + D)
+    X = random_rows(n, kp, F, 60 + D)
+    y = ctx.refiner_forward(D, X, params=p)
+    ye = O.nn_forward(kp, p, X, emulate=True)
+    # the FP32 sums of the matrix cores differ from the oracle's in order only; a BF16
+    # rounding that lands differently moves an output by a fraction of a BF16 step
+    diff = np.abs(y - ye)
+    assert np.median(diff) < 1e-5 * (1 + np.median(np.abs(ye)))
+    assert np.max(diff) < 2e-2 * (1 + np.max(np.abs(ye)))
+
+
+@pytest.mark.parametrize("D,n", [(2, 200), (3, 64)])
+def test_grad_matches_oracle(ctx, D, n):
+    F, kp = O.nn_dims(D)
+    p = random_net(kp, F, 70 + D)
+    X = random_rows(n, kp, F, 80 + D)
+    t = np.random.default_rng(90 + D).normal(-7, 3, n).astype(np.float32)
+    g, loss = ctx.refiner_grad(D, p, X, t)
+    ge, le = O.nn_grad(kp, p, X, t, emulate=True)
+    gx, lx = O.nn_grad(kp, p, X, t, emulate=False)
+    assert abs(loss - le) < 1e-4 * le
+    L = layout(kp)
+    for a, b in [("w0", "b0"), ("b0", "w1"), ("w1", "b1"), ("b1", "w2"), ("w2", "b2"), ("b2", "w3"), ("w3", "b3")]:
+        ref = ge[L[a]:L[b]]
+        d = np.linalg.norm(g[L[a]:L[b]] - ref) / np.linalg.norm(ref)
+        assert d < 0.05, (a, d)
+        dx = np.linalg.norm(g[L[a]:L[b]] - gx[L[a]:L[b]]) / np.linalg.norm(gx[L[a]:L[b]])
+        assert dx < 0.2, (a, dx)
+    assert abs(g[L["b3"]] - ge[L["b3"]]) < 1e-4 * (1 + abs(ge[L["b3"]]))
+
+
+def _codes(lo, hi, rlo, rhi):
+    """dyadic codes of regions of the range (extent 2^-n, offset index)"""
+    D = lo.shape[1]
+    out = np.zeros(lo.shape, dtype=np.uint32)
+    for d in range(D):
+        ext = (hi[:, d] - lo[:, d]) / (rhi[d] - rlo[d])
+        n = np.rint(-np.log2(ext)).astype(np.int64)
+        idx = np.rint((lo[:, d] - rlo[d]) / (hi[:, d] - lo[:, d])).astype(np.int64)
+        out[:, d] = (1 << n) + idx
+    return out
+
+
+def children_nodes(f, lo, hi, v, dim):
+    """Node values of the two children of a split along dim: the parent's planes, and the
+    new middle planes evaluated in FP64 (as the training-pair kernel does)."""
+    D = len(lo)
+    out = []
+    for c in range(2):
+        vc = np.zeros(3 ** D)
+        for k in range(3 ** D):
+            dg = [(k // 3 ** d) % 3 for d in range(D)]
+            j = dg[dim]
+            if j == 1:
+                x = [((0.25 if c == 0 else 0.75) if d == dim else dg[d] / 2.0) * (hi[d] - lo[d]) + lo[d]
+                     for d in range(D)]
+                vc[k] = O.eval_f64(f, np.array(x))
+            else:
+                pj = (0 if j == 0 else 1) if c == 0 else (1 if j == 0 else 2)
+                vc[k] = v[k + (pj - j) * 3 ** dim]
+        out.append(vc)
+    return out
+
+
+@pytest.mark.parametrize("target", [V.TARGET_ENERGY, V.TARGET_BINVAR, V.TARGET_GAIN])
+def test_pairs_match_oracle(ctx, target):
+    f = I.config4_integrand(7)
+    res = (32, 32)
+    it = V.integrator_crespo2021(255, 0, 0)
+    p = it.problem(f, res, V.range_primary(2), V.F32)
+    X, t = ctx.refiner_pairs([p], [f], target=target)
+    assert X.shape[0] >= 2 * 256 - 1 and X.shape[1] == 32
+    assert np.all(np.isfinite(t))
+    # every third generated region (node slots are claimed in scheduling order, so which
+    # regions those are varies between runs; each is one of the full set's)
+    Xs, ts = ctx.refiner_pairs([p], [f], target=target, stride=3)
+    assert Xs.shape[0] == (X.shape[0] + 2) // 3
+    rows = {tuple(X[i]): t[i] for i in range(X.shape[0])}
+    for i in range(Xs.shape[0]):
+        assert rows[tuple(Xs[i])] == ts[i]
+    # the final regions of the same problem are among the pairs
+    bins = np.zeros(res, dtype=np.float32)
+    ctx.integrate_host(p, f, bins)
+    regs = ctx.export_regions(2)
+    codes = _codes(regs["lo"], regs["hi"], [0, 0], [1, 1])
+    keys = {tuple(X[i]): i for i in range(X.shape[0])}
+    lres = O.lres_of(res, 2)
+    checked = 0
+    for r in range(0, regs["lo"].shape[0], 9):
+        lo, hi, v, dim = regs["lo"][r], regs["hi"][r], regs["nodes"][r], int(regs["dim"][r])
+        x_o, s_o, d_o = O.region_features(2, lo, hi, [0, 0], [1, 1], v, codes[r], lres=lres)
+        xb = O.f2bf(x_o)
+        assert tuple(xb) in keys, (r, xb)  # the device features of this region, bit for bit
+        i = keys[tuple(xb)]
+        vol = float(np.prod(hi - lo))
+        se = max(float(s_o), 2.0 ** -100)
+        if target == V.TARGET_ENERGY:
+            E = O.true_error(f, lo, hi, v, 3)
+            scale = E
+        else:
+            v0, v1 = children_nodes(f, lo, hi, v, dim)
+            Vp, Vc = O.bin_variances(f, res, lo, hi, v, dim, v0, v1, maxb=4)
+            E = Vp - Vc if target == V.TARGET_GAIN else Vp
+            scale = Vp
+        E_dev = 2.0 ** float(t[i]) * se * se * vol
+        floor = 2.0 ** -60 * se * se * vol
+        if E <= floor:
+            assert t[i] == -60.0 or E_dev <= 2 * floor
+        else:
+            assert abs(E_dev - E) <= 1e-6 * scale + 1e-6 * E + floor, (r, E_dev, E, scale)
+        checked += 1
+    assert checked >= 20
+
+
+@pytest.fixture(scope="module")
+def trained(ctx):
+    rep = R.train(ctx, count=128, regions=2048, res=(512, 512), epochs=6, batch=1024, lr=2e-3, seed=1)
+    return rep
+
+
+def test_training_learns(trained):
+    rep = trained
+    assert rep["pairs"] >= 128 * (2 * 1024 - 1)
+    # standardised targets: predicting the mean gives a loss of 0.5
+    assert rep["loss_valid"] < 0.3, rep
+    assert rep["loss_train"] < 0.7 * rep["loss_first"], rep
+    assert rep["ms_train"] > 0
+
+
+def test_set_get_roundtrip(ctx, trained):
+    p = ctx.refiner_get(2)
+    assert p.size == V.refiner_size(2)[0] and np.all(np.isfinite(p)) and p[-1] > 0
+    ctx.refiner_set(2, p)
+    np.testing.assert_array_equal(ctx.refiner_get(2), p)
+    with pytest.raises(V.VmxError):
+        bad = p.copy()
+        bad[-1] = 0.0
+        ctx.refiner_set(2, bad)
+
+
+def test_learned_refinement_is_a_valid_partition(ctx, trained):
+    f = R.held_out(1)[0]
+    bins = np.zeros((64, 64), dtype=np.float32)
+    it = R.integrator(2047, 4, 3, learned=True)
+    V.integrate(it, bins, f, V.range_primary(2), ctx=ctx)
+    st = it.last_stats
+    assert st.regions == 2048 and np.all(np.isfinite(bins))
+    regs = ctx.export_regions(2)
+    vol = np.prod(regs["hi"] - regs["lo"], axis=1)
+    assert abs(vol.sum() - 1.0) < 1e-6
+    assert np.all(regs["err"] > 0)
+    # unbiased: the control variate is the exact integral of the fitted polynomial
+    truth = O.blobs2d_exact_bins(f, 64, 64)
+    err = bins.astype(np.float64) - truth
+    se = np.std(err) / np.sqrt(err.size)
+    assert abs(err.mean()) < 5 * se + 1e-7
+
+
+def rmse_pair(ctx, f, res, regions, spp, seed):
+    truth = O.blobs2d_exact_bins(f, res[0], res[1])
+    out = []
+    for learned in (False, True):
+        bins = np.zeros(res, dtype=np.float32)
+        V.integrate(R.integrator(regions - 1, spp, seed, learned=learned), bins, f, V.range_primary(2), ctx=ctx)
+        out.append(float(np.sqrt(np.mean((bins.astype(np.float64) - truth) ** 2))))
+    return out
+
+
+def test_learned_priority_gains_over_heuristic(ctx, trained):
+    """Held-out integrands, 512x512 bins, equal region budget (4096) and spp: the learned
+    priority lowers the RMSE against the closed-form per-bin integrals (geometric mean of
+    the per-integrand ratios; equal RMSE then needs ratio^2 as many samples)."""
+    ratios = []
+    for f in R.held_out(8):
+        h, l = rmse_pair(ctx, f, (512, 512), 4096, 4, 11)
+        ratios.append(l / h)
+    gm = float(np.exp(np.mean(np.log(ratios))))
+    print("learned/heuristic RMSE ratios", np.round(ratios, 3), "geometric mean", round(gm, 3))
+    assert gm < 0.9, ratios
+
+
+@pytest.mark.parametrize("seed", [0, 5])
+def test_exact_bins_match_oracle(ctx, seed):
+    f = I.config4_integrand(R.EVAL_SEED0 + seed)
+    got = ctx.exact_bins(f, (96, 80))
+    want = O.blobs2d_exact_bins(f, 96, 80)
+    np.testing.assert_allclose(got, want, rtol=1e-11, atol=1e-12 * np.max(np.abs(want)))
+    f1 = I.config1()
+    np.testing.assert_allclose(ctx.exact_bins(f1, (64, 64)), O.blobs2d_exact_bins(f1, 64, 64), rtol=1e-11,
+                               atol=1e-12)

@@ -1,4 +1,4 @@
-"""The product's device arithmetic (viltrum_amd/csrc/region_math.h, philox.h) compiled
+"""The product's device arithmetic (viltrum_amd/csrc/region_math.h, philox.h, features.h) compiled
 for the host, against the oracle and against published Philox known answers."""
 import ctypes as C
 import os
@@ -143,3 +143,49 @@ def test_pixel_range_clamps_and_matches_rule():
     assert tuple(se) == (2, 5)
     H.vmh_pixel_range(4, 0.99999994, 1.0, 0.0, 1.0, 1 << 24, P(se, up))   # rounding lands on res: clamped
     assert se[0] < (1 << 24) and se[1] <= (1 << 24)
+
+
+@pytest.mark.parametrize("D", [1, 2, 3, 4])
+def test_region_features_match_oracle(D):
+    """The kernels' feature function (features.h, host build) against the oracle's
+    restatement, bit for bit, on random FP32 regions."""
+    rng = np.random.default_rng(100 + D)
+    F, kp = O.nn_dims(D)
+    H.vmh_region_features.restype = C.c_int
+    for trial in range(40):
+        depth = rng.integers(0, 8, D)
+        code = np.array([(1 << int(n)) | int(rng.integers(0, 1 << int(n))) if n else 1 for n in depth],
+                        dtype=np.uint32)
+        rlo, rhi = np.full(D, -0.5), np.full(D, 1.5)
+        ext = (rhi - rlo) / 2.0 ** depth
+        lo = rlo + (code - (1 << depth)) * ext
+        hi = lo + ext
+        v = rng.normal(0, 1, 3 ** D).astype(np.float32).astype(np.float64) * (10.0 ** rng.integers(-20, 5))
+        if trial == 0:
+            v[:] = 0.0  # scale floor
+        res = [int(r) for r in rng.integers(1, 700, min(D, 3))]
+        lres = np.zeros(D, dtype=np.float32)
+        lres[:len(res)] = O.lres_of(res, len(res))
+        x_o, s_o, d_o = O.region_features(D, lo, hi, rlo, rhi, v, code, lres=lres)
+        x_h = np.zeros(kp, dtype=np.float32)
+        s_h, d_h = C.c_float(), C.c_int()
+        assert H.vmh_region_features(D, P(lo), P(hi), P(rlo), P(rhi), P(v), P(code, C.POINTER(C.c_uint32)),
+                                     P(lres, C.POINTER(C.c_float)), P(x_h, C.POINTER(C.c_float)), C.byref(s_h),
+                                     C.byref(d_h)) == 0
+        assert d_h.value == d_o == int(depth.sum())
+        assert s_h.value == s_o
+        np.testing.assert_array_equal(x_h, x_o)
+
+
+def test_bf16_rounding_matches_oracle():
+    H.vmh_f2bf.restype = C.c_uint16
+    H.vmh_f2bf.argtypes = [C.c_float]
+    rng = np.random.default_rng(5)
+    xs = np.concatenate([rng.normal(0, 1, 500).astype(np.float32),
+                         np.array([0.0, -0.0, 1.00390625, 1.005859375, 3.0e38, -3.4e38, np.inf, -np.inf],
+                                  dtype=np.float32)])
+    want = O.f2bf(xs)
+    got = np.array([H.vmh_f2bf(float(v)) for v in xs], dtype=np.uint16)
+    np.testing.assert_array_equal(got, want)
+    # round to nearest even at the halfway points
+    assert H.vmh_f2bf(1.00390625) == 0x3F80 and H.vmh_f2bf(1.01171875) == 0x3F82

@@ -35,7 +35,9 @@ ORDER_CANONICAL, ORDER_REFERENCE = 0, 1
 REFINER_HEURISTIC, REFINER_LEARNED = 0, 1
 
 SYMBOLS = ["vmx_ctx_create", "vmx_ctx_destroy", "vmx_last_error", "vmx_integrate", "vmx_integrate_device",
-           "vmx_integrate_batch", "vmx_export_regions", "vmx_integrate_mc", "vmx_eval"]
+           "vmx_integrate_batch", "vmx_export_regions", "vmx_integrate_mc", "vmx_eval", "vmx_refiner_size",
+           "vmx_refiner_pairs", "vmx_refiner_train", "vmx_refiner_get", "vmx_refiner_set", "vmx_refiner_forward",
+           "vmx_refiner_grad", "vmx_exact_bins"]
 
 
 class VmxError(RuntimeError):
@@ -58,6 +60,23 @@ class Stats(C.Structure):
         ("ties_unresolved", C.c_uint64), ("ms_generate", C.c_double), ("ms_select", C.c_double),
         ("ms_stratify", C.c_double), ("ms_bins", C.c_double), ("ms_total", C.c_double),
     ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+TARGET_ENERGY, TARGET_BINVAR, TARGET_GAIN = 0, 1, 2
+
+
+class TrainOpts(C.Structure):
+    _fields_ = [("epochs", C.c_uint32), ("batch", C.c_uint32), ("lr", C.c_float), ("holdout", C.c_float),
+                ("seed", C.c_uint64), ("target", C.c_int32), ("stride", C.c_uint32)]
+
+
+class TrainReport(C.Structure):
+    _fields_ = [("pairs", C.c_uint64), ("steps", C.c_uint64), ("loss_first", C.c_double),
+                ("loss_train", C.c_double), ("loss_valid", C.c_double), ("target_mean", C.c_double),
+                ("target_std", C.c_double), ("ms_pairs", C.c_double), ("ms_train", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -87,8 +106,29 @@ def lib():
         L.vmx_eval.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_uint64, C.c_void_p, C.c_void_p]
         L.vmx_integrate_mc.argtypes = [C.c_void_p, C.POINTER(Problem), C.c_void_p, C.c_int32, C.c_void_p,
                                        C.c_void_p]
+        L.vmx_refiner_size.argtypes = [C.c_int32, C.POINTER(C.c_uint64), C.POINTER(C.c_int32),
+                                       C.POINTER(C.c_int32)]
+        L.vmx_refiner_pairs.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.POINTER(TrainOpts),
+                                        C.c_uint64, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)]
+        L.vmx_refiner_train.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.POINTER(TrainOpts),
+                                        C.POINTER(TrainReport)]
+        L.vmx_refiner_get.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
+        L.vmx_refiner_set.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
+        L.vmx_refiner_forward.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]
+        L.vmx_refiner_grad.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.POINTER(C.c_double)]
+        L.vmx_exact_bins.argtypes = [C.c_void_p, C.POINTER(Problem), C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
+
+
+def refiner_size(dim):
+    """(parameter count, feature count F, padded row width KP) of the priority network."""
+    n, f, kp = C.c_uint64(), C.c_int32(), C.c_int32()
+    rc = lib().vmx_refiner_size(dim, C.byref(n), C.byref(f), C.byref(kp))
+    if rc != 0:
+        raise VmxError("no priority network for dimension %d (status %d)" % (dim, rc))
+    return n.value, f.value, kp.value
 
 
 class Context:
@@ -161,6 +201,80 @@ class Context:
         self._check(lib().vmx_integrate_mc(self.ptr, C.byref(p), C.cast(f.ptr(), C.c_void_p), int(stratified),
                                            bins.ctypes.data_as(C.c_void_p), se.ctypes.data_as(C.c_void_p)))
         return bins, se
+
+    # ---- learned refinement priority
+    @staticmethod
+    def _problems(problems, fs):
+        n = len(problems)
+        return n, (Problem * n)(*problems), (integrands.IntegrandDesc * n)(*[f.desc for f in fs])
+
+    def refiner_pairs(self, problems, fs, target=TARGET_ENERGY, stride=1):
+        """(features uint16 [n, KP] BF16 bits, targets float32 [n]) of the regions the
+        heuristic method generates for the problems (one family and dim)."""
+        n, P, F = self._problems(problems, fs)
+        o = TrainOpts(0, 0, 0.0, 0.0, 0, target, stride)
+        cnt = C.c_uint64()
+        L = lib()
+        self._check(L.vmx_refiner_pairs(self.ptr, n, C.cast(P, C.c_void_p), C.cast(F, C.c_void_p), C.byref(o), 0,
+                                        None, None, C.byref(cnt)))
+        kp = refiner_size(fs[0].dim)[2]
+        X = np.zeros((cnt.value, kp), dtype=np.uint16)
+        t = np.zeros(cnt.value, dtype=np.float32)
+        self._check(L.vmx_refiner_pairs(self.ptr, n, C.cast(P, C.c_void_p), C.cast(F, C.c_void_p), C.byref(o),
+                                        cnt.value, X.ctypes.data_as(C.c_void_p), t.ctypes.data_as(C.c_void_p),
+                                        C.byref(cnt)))
+        return X[:cnt.value], t[:cnt.value]
+
+    def refiner_train(self, problems, fs, epochs=8, batch=1024, lr=2e-3, holdout=0.05, seed=0,
+                      target=TARGET_ENERGY, stride=1):
+        n, P, F = self._problems(problems, fs)
+        o = TrainOpts(epochs, batch, lr, holdout, seed, target, stride)
+        r = TrainReport()
+        self._check(lib().vmx_refiner_train(self.ptr, n, C.cast(P, C.c_void_p), C.cast(F, C.c_void_p), C.byref(o),
+                                            C.byref(r)))
+        return r.as_dict()
+
+    def refiner_get(self, dim):
+        p = np.zeros(refiner_size(dim)[0], dtype=np.float32)
+        self._check(lib().vmx_refiner_get(self.ptr, dim, p.ctypes.data_as(C.c_void_p)))
+        return p
+
+    def refiner_set(self, dim, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.size != refiner_size(dim)[0]:
+            raise VmxError("expected %d parameters" % refiner_size(dim)[0])
+        self._check(lib().vmx_refiner_set(self.ptr, dim, p.ctypes.data_as(C.c_void_p)))
+
+    def refiner_forward(self, dim, X, params=None):
+        X = np.ascontiguousarray(X, dtype=np.uint16)
+        y = np.zeros(X.shape[0], dtype=np.float32)
+        pp = None
+        if params is not None:
+            params = np.ascontiguousarray(params, dtype=np.float32)
+            pp = params.ctypes.data_as(C.c_void_p)
+        self._check(lib().vmx_refiner_forward(self.ptr, dim, pp, X.shape[0], X.ctypes.data_as(C.c_void_p),
+                                              y.ctypes.data_as(C.c_void_p)))
+        return y
+
+    def refiner_grad(self, dim, params, X, t):
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        X = np.ascontiguousarray(X, dtype=np.uint16)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(params.size - 2, dtype=np.float32)
+        loss = C.c_double()
+        self._check(lib().vmx_refiner_grad(self.ptr, dim, params.ctypes.data_as(C.c_void_p), X.shape[0],
+                                           X.ctypes.data_as(C.c_void_p), t.ctypes.data_as(C.c_void_p),
+                                           g.ctypes.data_as(C.c_void_p), C.byref(loss)))
+        return g, loss.value
+
+    def exact_bins(self, f, res, rng=None):
+        """FP64 closed-form bins of a 2-D static blob integrand (the RMSE truth)."""
+        rng = rng or range_primary(f.dim)
+        p = Integrator(0, 0, 0).problem(f, res, rng, F64)
+        out = np.zeros(res, dtype=np.float64)
+        self._check(lib().vmx_exact_bins(self.ptr, C.byref(p), C.cast(f.ptr(), C.c_void_p),
+                                         out.ctypes.data_as(C.c_void_p)))
+        return out
 
     def eval(self, f, x, scalar=F64):
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, f.dim)

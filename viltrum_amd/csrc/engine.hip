@@ -23,6 +23,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <random>
 #include <limits>
 #include <type_traits>
 #include <new>
@@ -34,6 +35,7 @@
 #include <rocprim/rocprim.hpp>
 
 #include "../../include/viltrum_amd.h"
+#include "exact.h"
 #include "pipeline.h"
 
 struct vmx_ctx {
@@ -46,6 +48,9 @@ struct vmx_ctx {
         size_t bytes = 0;
     };
     Buf tree, regs, lists, misc, params, bins, bins2;
+    Buf nn[VMX_MAX_DIM + 1];                       // FP32 priority-network parameters per dim
+    bool nn_ok[VMX_MAX_DIM + 1] = {};
+    Buf tr_x, tr_t, tr_idx, tr_part, tr_loss, tr_state;  // training work space
     hipEvent_t ev[8] = {};
     // last result (for vmx_export_regions): gathered from device memory on demand
     int last_D = 0;
@@ -103,6 +108,20 @@ struct Measure {
 
 template <typename P>
 struct ParamOf;
+template <typename P>
+struct F64Of;
+template <typename T>
+struct F64Of<vmx::Poly<T>> {
+    using type = vmx::Poly<double>;
+};
+template <typename T>
+struct F64Of<vmx::Blobs<T>> {
+    using type = vmx::Blobs<double>;
+};
+template <typename T>
+struct F64Of<vmx::Lobes5<T>> {
+    using type = vmx::Lobes5<double>;
+};
 template <typename T>
 struct ParamOf<vmx::Poly<T>> {
     using type = vmx_poly_params;
@@ -128,13 +147,35 @@ struct JobIn {
     vmx_stats* stats;
 };
 
+/* Training-pair mode of run_batch: stop after generation, report the node count and,
+   if alloc hands back device arrays for that many rows, fill them. */
+struct PairsReq {
+    uint64_t count = 0;
+    int kind = vmx::nn::TARGET_ENERGY;
+    uint32_t stride = 1;
+    std::function<void(uint64_t, uint16_t**, float**)> alloc;
+};
+
+constexpr int LEARN_MAX_DIM = 4;  // the fused network fits the expansion kernel's LDS up to here
+
+template <typename T, int D, typename P, typename Math>
+void launch_expand(bool learn, const vmx::GenArgs<T>& g, const P* d_f, int par, uint32_t blocks, hipStream_t st) {
+    if constexpr (std::is_same<T, float>::value && D <= LEARN_MAX_DIM) {
+        if (learn) {
+            vmx::k_expand_round<T, D, P, Math, true><<<blocks, vmx::GEN_THREADS, 0, st>>>(g, d_f, par);
+            return;
+        }
+    }
+    vmx::k_expand_round<T, D, P, Math, false><<<blocks, vmx::GEN_THREADS, 0, st>>>(g, d_f, par);
+}
+
 template <typename T>
 struct RocprimTemp {
     size_t sort_bytes = 0, scan_bytes = 0;
 };
 
 template <typename T, int D, typename P, typename Math>
-int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st) {
+int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsReq* pairs_req = nullptr) {
     constexpr int K = vmx::pow3(D);
     const auto t_start = std::chrono::steady_clock::now();
     const uint32_t J = uint32_t(in.size());
@@ -180,7 +221,18 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st) {
         n_reg += p.regions;
         jb.form = p.form;
         jb.heur = p.heuristic;
+        jb.learn = (!pairs_req && p.refiner == VMX_REFINER_LEARNED) ? 1 : 0;
+        for (int d = 0; d < VMX_MAX_DIM; ++d) jb.lres[d] = d < jb.db ? float(std::log2(double(p.res[d]))) : 0.f;
         fparams[j] = P::from(*static_cast<const typename ParamOf<P>::type*>(in[j].params));
+    }
+    bool learn = false;
+    for (const auto& jb : jobs) learn |= jb.learn != 0;
+    if (learn) {
+        if (!std::is_same<T, float>::value || D > LEARN_MAX_DIM)
+            throw VmxError(VMX_ERR_UNSUPPORTED, "learned refiner: FP32 problems of dimension 1..4 only");
+        if (!ctx->nn_ok[D]) throw VmxError(VMX_ERR_UNSUPPORTED, "learned refiner: no network for this dimension");
+        if (order != VMX_ORDER_CANONICAL)
+            throw VmxError(VMX_ERR_INVALID, "learned refiner: canonical region order only");
     }
     if (n_reg > 0x7FFFFFF0ull || total_bins > 0xFFFFFFF0ull) throw VmxError(VMX_ERR_CAPACITY, "batch too large");
     int mb_max = 1;
@@ -280,12 +332,13 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st) {
         g.theta = d_theta;
         g.rounds = d_rounds;
         g.J = J;
+        g.net = learn ? static_cast<const float*>(ctx->nn[D].p) : nullptr;
         std::vector<uint32_t> ncand_h(2 * J);
         int par = 0;
         for (int launched = 0;;) {
             for (int k = 0; k < round_chunk; ++k, ++launched, par ^= 1) {
                 vmx::k_theta_round<T><<<J, 256, 0, st>>>(g, par);
-                vmx::k_expand_round<T, D, P, Math><<<gen_blocks, vmx::GEN_THREADS, 0, st>>>(g, d_f, par);
+                launch_expand<T, D, P, Math>(learn, g, d_f, par, gen_blocks, st);
             }
             HIP_CHECK(hipGetLastError());
             HIP_CHECK(hipMemcpyAsync(ncand_h.data(), d_ncand, sizeof(uint32_t) * 2 * J, hipMemcpyDeviceToHost, st));
@@ -313,6 +366,38 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st) {
         break;
     }
     const double ms_gen = ms_since(t_start);
+
+    if (pairs_req) {
+        PairsReq* pairs = pairs_req;
+        // training pairs: every node of every job, job-major in slot order
+        std::vector<uint64_t> base(J);
+        uint64_t total = 0;
+        const uint32_t stride = std::max<uint32_t>(1, pairs->stride);
+        for (uint32_t j = 0; j < J; ++j) {
+            base[j] = total;
+            total += (rt[j].count - jobs[j].node_base + stride - 1) / stride;
+        }
+        pairs->count = total;
+        uint16_t* X = nullptr;
+        float* tg = nullptr;
+        if (pairs->alloc) pairs->alloc(total, &X, &tg);
+        if (X && tg && total > 0) {
+            using F64 = typename F64Of<P>::type;
+            std::vector<F64> f64(J);
+            for (uint32_t j = 0; j < J; ++j)
+                f64[j] = F64::from(*static_cast<const typename ParamOf<P>::type*>(in[j].params));
+            Carve c(ensure(ctx->misc, sizeof(uint64_t) * J + sizeof(F64) * J + 1024));
+            uint64_t* d_base = c.take<uint64_t>(J);
+            F64* d_f64 = c.take<F64>(J);
+            HIP_CHECK(hipMemcpyAsync(d_base, base.data(), sizeof(uint64_t) * J, hipMemcpyHostToDevice, st));
+            HIP_CHECK(hipMemcpyAsync(d_f64, f64.data(), sizeof(F64) * J, hipMemcpyHostToDevice, st));
+            vmx::k_train_pairs<T, D, F64><<<(cap_total + 127) / 128, 128, 0, st>>>(d_jobs, t, cap_total, d_base, d_f64,
+                                                                                pairs->kind, stride, X, tg);
+            HIP_CHECK(hipGetLastError());
+            HIP_CHECK(hipStreamSynchronize(st));
+        }
+        return VMX_OK;
+    }
 
     // ---------------- 2. selection and canonical leaf order
     const auto t_sel = std::chrono::steady_clock::now();
@@ -572,24 +657,25 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st) {
 }
 
 template <typename T, typename Math>
-int dispatch_T(vmx_ctx* ctx, int family, int D, const std::vector<JobIn>& jobs, hipStream_t st) {
+int dispatch_T(vmx_ctx* ctx, int family, int D, const std::vector<JobIn>& jobs, hipStream_t st,
+               PairsReq* pairs = nullptr) {
     switch (family) {
         case VMX_FAMILY_POLY:
             switch (D) {
-                case 1: return run_batch<T, 1, vmx::Poly<T>, Math>(ctx, jobs, st);
-                case 2: return run_batch<T, 2, vmx::Poly<T>, Math>(ctx, jobs, st);
-                case 3: return run_batch<T, 3, vmx::Poly<T>, Math>(ctx, jobs, st);
-                case 4: return run_batch<T, 4, vmx::Poly<T>, Math>(ctx, jobs, st);
-                case 5: return run_batch<T, 5, vmx::Poly<T>, Math>(ctx, jobs, st);
+                case 1: return run_batch<T, 1, vmx::Poly<T>, Math>(ctx, jobs, st, pairs);
+                case 2: return run_batch<T, 2, vmx::Poly<T>, Math>(ctx, jobs, st, pairs);
+                case 3: return run_batch<T, 3, vmx::Poly<T>, Math>(ctx, jobs, st, pairs);
+                case 4: return run_batch<T, 4, vmx::Poly<T>, Math>(ctx, jobs, st, pairs);
+                case 5: return run_batch<T, 5, vmx::Poly<T>, Math>(ctx, jobs, st, pairs);
                 default: break;
             }
             break;
         case VMX_FAMILY_BLOBS:
-            if (D == 2) return run_batch<T, 2, vmx::Blobs<T>, Math>(ctx, jobs, st);
-            if (D == 3) return run_batch<T, 3, vmx::Blobs<T>, Math>(ctx, jobs, st);
+            if (D == 2) return run_batch<T, 2, vmx::Blobs<T>, Math>(ctx, jobs, st, pairs);
+            if (D == 3) return run_batch<T, 3, vmx::Blobs<T>, Math>(ctx, jobs, st, pairs);
             break;
         case VMX_FAMILY_LOBES5:
-            if (D == 5) return run_batch<T, 5, vmx::Lobes5<T>, Math>(ctx, jobs, st);
+            if (D == 5) return run_batch<T, 5, vmx::Lobes5<T>, Math>(ctx, jobs, st, pairs);
             break;
         default: break;
     }
@@ -638,7 +724,8 @@ void validate(const vmx_problem* p, const vmx_integrand* f) {
     if (p->form < VMX_FORM_VR_OPTIMIZE || p->form > VMX_FORM_CV_ALPHA1) throw VmxError(VMX_ERR_INVALID, "unknown form");
     if (p->order != VMX_ORDER_CANONICAL && p->order != VMX_ORDER_REFERENCE)
         throw VmxError(VMX_ERR_INVALID, "unknown region order");
-    if (p->refiner != VMX_REFINER_HEURISTIC) throw VmxError(VMX_ERR_UNSUPPORTED, "learned refiner not loaded");
+    if (p->refiner != VMX_REFINER_HEURISTIC && p->refiner != VMX_REFINER_LEARNED)
+        throw VmxError(VMX_ERR_INVALID, "unknown refiner");
     for (int d = 0; d < f->dim; ++d)
         if (!(p->hi[d] > p->lo[d])) throw VmxError(VMX_ERR_INVALID, "range must have hi > lo");
 }
@@ -717,6 +804,219 @@ int eval_T(vmx_ctx* ctx, const vmx_integrand& f, uint64_t n, const double* x, do
     return VMX_OK;
 }
 }  // namespace
+
+/* ------------------------------------------------------------------ priority network */
+int kp_of(int dim) {
+    switch (dim) {
+        case 1: return vmx::nn::Dims<1>::KP;
+        case 2: return vmx::nn::Dims<2>::KP;
+        case 3: return vmx::nn::Dims<3>::KP;
+        case 4: return vmx::nn::Dims<4>::KP;
+        default: throw VmxError(VMX_ERR_UNSUPPORTED, "learned refiner: dimension 1..4");
+    }
+}
+int nf_of(int dim) {
+    switch (dim) {
+        case 1: return vmx::nn::Dims<1>::F;
+        case 2: return vmx::nn::Dims<2>::F;
+        case 3: return vmx::nn::Dims<3>::F;
+        case 4: return vmx::nn::Dims<4>::F;
+        default: throw VmxError(VMX_ERR_UNSUPPORTED, "learned refiner: dimension 1..4");
+    }
+}
+uint64_t n_trained(int kp) { return uint64_t(kp) * 64 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1; }
+uint64_t n_total(int kp) { return n_trained(kp) + 2; }
+
+template <typename Fn>
+void with_kp(int kp, Fn&& fn) {
+    if (kp == 32)
+        fn(std::integral_constant<int, 32>{});
+    else if (kp == 64)
+        fn(std::integral_constant<int, 64>{});
+    else if (kp == 96)
+        fn(std::integral_constant<int, 96>{});
+    else if (kp == 128)
+        fn(std::integral_constant<int, 128>{});
+    else
+        throw VmxError(VMX_ERR_UNSUPPORTED, "learned refiner: unsupported feature width");
+}
+
+void nn_forward(int kp, const float* params, const uint16_t* X, uint64_t n, float* y, hipStream_t st) {
+    with_kp(kp, [&](auto K) {
+        constexpr int KP = decltype(K)::value;
+        const unsigned blocks = unsigned(std::min<uint64_t>((n + 63) / 64, 8192));
+        vmx::nn::k_mlp_forward<KP><<<blocks, 256, 0, st>>>(params, X, n, y);
+    });
+    HIP_CHECK(hipGetLastError());
+}
+
+void nn_grad(int kp, const float* params, const uint16_t* X, const float* tg, const uint32_t* idx, uint32_t count,
+             float inv_batch, float* part, float* loss, int want_grad, hipStream_t st) {
+    with_kp(kp, [&](auto K) {
+        constexpr int KP = decltype(K)::value;
+        const unsigned blocks = (count + 63) / 64;
+        vmx::nn::k_mlp_grad<KP><<<blocks, 256, 0, st>>>(params, X, tg, idx, count, inv_batch, part, loss, want_grad);
+    });
+    HIP_CHECK(hipGetLastError());
+}
+
+/* Runs generation for n problems of one family and dim and fills training pairs into
+   ctx->tr_x / ctx->tr_t (device).  Returns the number of pairs. */
+uint64_t make_pairs(vmx_ctx* ctx, uint32_t n, const vmx_problem* problems, const vmx_integrand* integrands,
+                    bool fill, const vmx_train_opts& o) {
+    if (o.target < 0 || o.target > vmx::nn::TARGET_GAIN)
+        throw VmxError(VMX_ERR_INVALID, "unknown training target");
+    if (n == 0) throw VmxError(VMX_ERR_INVALID, "no problems");
+    if (!problems || !integrands) throw VmxError(VMX_ERR_INVALID, "null argument");
+    std::vector<JobIn> group;
+    for (uint32_t i = 0; i < n; ++i) {
+        validate(&problems[i], &integrands[i]);
+        if (integrands[i].family != integrands[0].family || integrands[i].dim != integrands[0].dim ||
+            problems[i].scalar != problems[0].scalar)
+            throw VmxError(VMX_ERR_INVALID, "training pairs: one family, dim and scalar per call");
+        if (problems[i].order != VMX_ORDER_CANONICAL) throw VmxError(VMX_ERR_INVALID, "canonical order only");
+        group.push_back({&problems[i], integrands[i].params, nullptr, nullptr});
+    }
+    const int dim = integrands[0].dim;
+    const int kp = kp_of(dim);
+    PairsReq req;
+    req.kind = o.target;
+    req.stride = o.stride ? o.stride : 1;
+    if (fill)
+        req.alloc = [ctx, kp](uint64_t rows, uint16_t** X, float** tg) {
+            *X = static_cast<uint16_t*>(ensure(ctx->tr_x, sizeof(uint16_t) * rows * kp));
+            *tg = static_cast<float*>(ensure(ctx->tr_t, sizeof(float) * rows));
+        };
+    if (problems[0].scalar == VMX_F32)
+        dispatch_T<float, vmx::FastMath>(ctx, integrands[0].family, dim, group, ctx->stream, &req);
+    else
+        dispatch_T<double, vmx::PreciseMath>(ctx, integrands[0].family, dim, group, ctx->stream, &req);
+    return req.count;
+}
+
+float uniform_pm(std::mt19937_64& rng, float a) { return a * (2.f * float(double(rng() >> 11) * 0x1p-53) - 1.f); }
+
+/* Trains the priority network for `dim` on the `count` pairs in ctx->tr_x / tr_t. */
+void train_net(vmx_ctx* ctx, int dim, uint64_t count, const vmx_train_opts& o, vmx_train_report* rep) {
+    const int kp = kp_of(dim), F = nf_of(dim);
+    const uint64_t P = n_trained(kp), PT = n_total(kp);
+    hipStream_t st = ctx->stream;
+    const uint32_t epochs = o.epochs ? o.epochs : 8;
+    const uint32_t batch = o.batch ? o.batch : 1024;
+    if (batch % 64 != 0 || batch > 65536) throw VmxError(VMX_ERR_INVALID, "batch must be a multiple of 64, <= 65536");
+    const float lr0 = o.lr > 0 ? o.lr : 2e-3f;
+    const double holdout = (o.holdout > 0 && o.holdout < 0.5) ? o.holdout : 0.05;
+    if (count < 256 || count > 0xFFFFFFF0ull) throw VmxError(VMX_ERR_INVALID, "need 256 .. 2^32 training pairs");
+    const uint16_t* X = static_cast<const uint16_t*>(ctx->tr_x.p);
+    const float* tg = static_cast<const float*>(ctx->tr_t.p);
+
+    std::vector<float> th(count);
+    HIP_CHECK(hipMemcpyAsync(th.data(), tg, sizeof(float) * count, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    std::mt19937_64 rng(o.seed);
+    std::vector<uint32_t> perm(count);
+    for (uint64_t i = 0; i < count; ++i) perm[i] = uint32_t(i);
+    std::shuffle(perm.begin(), perm.end(), rng);
+    const uint64_t n_valid = std::max<uint64_t>(64, uint64_t(double(count) * holdout));
+    const uint64_t n_train = count - n_valid;
+    double sm = 0, sq = 0;
+    for (uint64_t i = 0; i < n_train; ++i) sm += th[perm[i]];
+    const double mean = sm / double(n_train);
+    for (uint64_t i = 0; i < n_train; ++i) sq += (th[perm[i]] - mean) * (th[perm[i]] - mean);
+    const double sd = std::max(1e-3, std::sqrt(sq / double(n_train)));
+
+    // He-uniform initialisation; zero weights on the padding inputs
+    using namespace vmx::nn;
+    std::vector<float> p(PT, 0.f);
+    const uint64_t o_b0 = uint64_t(kp) * 64, o_w1 = o_b0 + 64, o_b1 = o_w1 + 4096, o_w2 = o_b1 + 64,
+                   o_b2 = o_w2 + 4096, o_w3 = o_b2 + 64, o_b3 = o_w3 + 64;
+    for (int k = 0; k < F; ++k)
+        for (int j = 0; j < 64; ++j) p[uint64_t(k) * 64 + j] = uniform_pm(rng, std::sqrt(6.f / float(F)));
+    for (int i = 0; i < 4096; ++i) {
+        p[o_w1 + i] = uniform_pm(rng, std::sqrt(6.f / 64.f));
+        p[o_w2 + i] = uniform_pm(rng, std::sqrt(6.f / 64.f));
+    }
+    for (int i = 0; i < 64; ++i) p[o_w3 + i] = uniform_pm(rng, std::sqrt(3.f / 64.f));
+    (void)o_b3;
+    p[P] = float(mean);
+    p[P + 1] = float(sd);
+
+    const uint32_t nblk = batch / 64;
+    const uint64_t steps = (n_train + batch - 1) / batch;
+    const uint64_t vblk = (n_valid + 63) / 64;
+    Measure m;
+    m.take<float>(PT);             // params
+    m.take<float>(P);              // m
+    m.take<float>(P);              // v
+    m.take<uint32_t>(count);       // idx: train then valid
+    Carve c(ensure(ctx->tr_state, m.used));
+    float* d_p = c.take<float>(PT);
+    float* d_m = c.take<float>(P);
+    float* d_v = c.take<float>(P);
+    uint32_t* d_idx = c.take<uint32_t>(count);
+    float* d_part = static_cast<float*>(ensure(ctx->tr_part, sizeof(float) * nblk * P));
+    const uint64_t nloss = std::max<uint64_t>(steps * nblk, vblk);
+    float* d_loss = static_cast<float*>(ensure(ctx->tr_loss, sizeof(float) * nloss));
+    HIP_CHECK(hipMemcpyAsync(d_p, p.data(), sizeof(float) * PT, hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipMemsetAsync(d_m, 0, sizeof(float) * P, st));
+    HIP_CHECK(hipMemsetAsync(d_v, 0, sizeof(float) * P, st));
+    HIP_CHECK(hipMemcpyAsync(d_idx + n_train, perm.data() + n_train, sizeof(uint32_t) * n_valid,
+                             hipMemcpyHostToDevice, st));
+
+    const auto t0 = std::chrono::steady_clock::now();
+    const double total_steps = double(steps) * epochs;
+    const float b1 = 0.9f, b2 = 0.999f;
+    uint64_t t = 0;
+    std::vector<float> lh(steps * nblk);
+    double first = 0, last = 0;
+    std::vector<uint32_t> tr(perm.begin(), perm.begin() + n_train);
+    for (uint32_t e = 0; e < epochs; ++e) {
+        std::shuffle(tr.begin(), tr.end(), rng);
+        HIP_CHECK(hipMemcpyAsync(d_idx, tr.data(), sizeof(uint32_t) * n_train, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemsetAsync(d_loss, 0, sizeof(float) * steps * nblk, st));
+        for (uint64_t s = 0; s < steps; ++s, ++t) {
+            const uint64_t off = s * batch;
+            const uint32_t cnt = uint32_t(std::min<uint64_t>(batch, n_train - off));
+            nn_grad(kp, d_p, X, tg, d_idx + off, cnt, 1.f / float(cnt), d_part, d_loss + s * nblk, 1, st);
+            const double frac = double(t) / total_steps;
+            const float lr = float(lr0 * (0.05 + 0.95 * 0.5 * (1.0 + std::cos(3.141592653589793 * frac))));
+            const float c1 = float(1.0 / (1.0 - std::pow(double(b1), double(t + 1))));
+            const float c2 = float(1.0 / (1.0 - std::pow(double(b2), double(t + 1))));
+            k_adam<<<unsigned((P + 255) / 256), 256, 0, st>>>(d_p, d_m, d_v, d_part, (cnt + 63) / 64, uint32_t(P), lr,
+                                                              b1, b2, 1e-8f, c1, c2);
+            HIP_CHECK(hipGetLastError());
+        }
+        HIP_CHECK(hipMemcpyAsync(lh.data(), d_loss, sizeof(float) * steps * nblk, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        double sum = 0;
+        for (float x : lh) sum += x;
+        const double mean_loss = sum / double(steps);
+        if (e == 0) first = mean_loss;
+        last = mean_loss;
+    }
+    const double ms_train = ms_since(t0);
+    // held-out loss
+    nn_grad(kp, d_p, X, tg, d_idx + n_train, uint32_t(n_valid), 1.f / float(n_valid), d_part, d_loss, 0, st);
+    std::vector<float> vl(vblk);
+    HIP_CHECK(hipMemcpyAsync(vl.data(), d_loss, sizeof(float) * vblk, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    double vsum = 0;
+    for (float x : vl) vsum += x;
+    // keep the parameters
+    float* keep = static_cast<float*>(ensure(ctx->nn[dim], sizeof(float) * PT));
+    HIP_CHECK(hipMemcpyAsync(keep, d_p, sizeof(float) * PT, hipMemcpyDeviceToDevice, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    ctx->nn_ok[dim] = true;
+    if (rep) {
+        rep->steps = t;
+        rep->loss_first = first;
+        rep->loss_train = last;
+        rep->loss_valid = vsum;
+        rep->target_mean = mean;
+        rep->target_std = sd;
+        rep->ms_train = ms_train;
+    }
+}
 
 template <typename T, typename Math, int D, typename P>
 int mc_run(vmx_ctx* ctx, const vmx_problem& p, const void* fparams, int stratified, void* bins, double* stderr_out) {
@@ -805,6 +1105,48 @@ int mc_dispatch(vmx_ctx* ctx, const vmx_problem& p, const vmx_integrand& f, int 
 
 }  // namespace
 
+namespace {
+/* device copies of host network inputs, in ctx->misc */
+struct NetIn {
+    float* p;
+    uint16_t* X;
+    float* t;
+    float* y;
+    uint32_t* idx;
+};
+NetIn upload_net_inputs(vmx_ctx* ctx, int dim, const float* params, uint64_t n, const uint16_t* X, const float* t) {
+    const int kp = kp_of(dim);
+    const uint64_t PT = n_total(kp);
+    Measure m;
+    m.take<float>(PT);
+    m.take<uint16_t>(n * kp);
+    m.take<float>(n);
+    m.take<float>(n);
+    m.take<uint32_t>(n);
+    Carve c(ensure(ctx->misc, m.used));
+    NetIn r{};
+    r.p = c.take<float>(PT);
+    r.X = c.take<uint16_t>(n * kp);
+    r.t = c.take<float>(n);
+    r.y = c.take<float>(n);
+    r.idx = c.take<uint32_t>(n);
+    if (params) {
+        HIP_CHECK(hipMemcpyAsync(r.p, params, sizeof(float) * PT, hipMemcpyHostToDevice, ctx->stream));
+    } else {
+        if (!ctx->nn_ok[dim]) throw VmxError(VMX_ERR_INVALID, "no network for this dimension");
+        HIP_CHECK(hipMemcpyAsync(r.p, ctx->nn[dim].p, sizeof(float) * PT, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    HIP_CHECK(hipMemcpyAsync(r.X, X, sizeof(uint16_t) * n * kp, hipMemcpyHostToDevice, ctx->stream));
+    if (t) HIP_CHECK(hipMemcpyAsync(r.t, t, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+    std::vector<uint32_t> id(n);
+    for (uint64_t i = 0; i < n; ++i) id[i] = uint32_t(i);
+    HIP_CHECK(hipMemcpyAsync(r.idx, id.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return r;
+}
+
+}  // namespace
+
 extern "C" {
 
 int vmx_integrate_mc(vmx_ctx* ctx, const vmx_problem* p, const vmx_integrand* f, int32_t stratified, void* bins,
@@ -844,8 +1186,11 @@ int vmx_ctx_create(int device, vmx_ctx** out) {
 void vmx_ctx_destroy(vmx_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
-    for (vmx_ctx::Buf* b : {&ctx->tree, &ctx->regs, &ctx->lists, &ctx->misc, &ctx->params, &ctx->bins, &ctx->bins2})
+    for (vmx_ctx::Buf* b : {&ctx->tree, &ctx->regs, &ctx->lists, &ctx->misc, &ctx->params, &ctx->bins, &ctx->bins2,
+                            &ctx->tr_x, &ctx->tr_t, &ctx->tr_idx, &ctx->tr_part, &ctx->tr_loss, &ctx->tr_state})
         if (b->p) (void)hipFree(b->p);
+    for (vmx_ctx::Buf& b : ctx->nn)
+        if (b.p) (void)hipFree(b.p);
     for (hipEvent_t e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -911,6 +1256,158 @@ int vmx_export_regions(vmx_ctx* ctx, uint64_t* n, double* lo, double* hi, double
     });
 }
 
+
+int vmx_refiner_size(int32_t dim, uint64_t* n_params, int32_t* n_features, int32_t* row_width) {
+    try {
+        const int kp = kp_of(dim);
+        if (n_params) *n_params = n_total(kp);
+        if (n_features) *n_features = nf_of(dim);
+        if (row_width) *row_width = kp;
+        return VMX_OK;
+    } catch (const VmxError& e) {
+        return e.code;
+    }
+}
+
+int vmx_refiner_pairs(vmx_ctx* ctx, uint32_t n, const vmx_problem* problems, const vmx_integrand* integrands,
+                      const vmx_train_opts* opts, uint64_t cap, uint16_t* features, float* targets, uint64_t* count) {
+    return guarded(ctx, [&]() -> int {
+        if (!count) throw VmxError(VMX_ERR_INVALID, "null count");
+        vmx_train_opts o{};
+        if (opts) o = *opts;
+        const bool fill = features && targets;
+        const uint64_t rows = make_pairs(ctx, n, problems, integrands, fill, o);
+        *count = rows;
+        if (!fill) return int(VMX_OK);
+        if (rows > cap) throw VmxError(VMX_ERR_CAPACITY, "more pairs than cap");
+        const int kp = kp_of(integrands[0].dim);
+        HIP_CHECK(hipMemcpyAsync(features, ctx->tr_x.p, sizeof(uint16_t) * rows * kp, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+        HIP_CHECK(hipMemcpyAsync(targets, ctx->tr_t.p, sizeof(float) * rows, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return int(VMX_OK);
+    });
+}
+
+int vmx_refiner_train(vmx_ctx* ctx, uint32_t n, const vmx_problem* problems, const vmx_integrand* integrands,
+                      const vmx_train_opts* opts, vmx_train_report* report) {
+    return guarded(ctx, [&]() -> int {
+        vmx_train_opts o{};
+        if (opts) o = *opts;
+        if (report) std::memset(report, 0, sizeof(*report));
+        const auto t0 = std::chrono::steady_clock::now();
+        const uint64_t rows = make_pairs(ctx, n, problems, integrands, true, o);
+        const double ms_pairs = ms_since(t0);
+        train_net(ctx, integrands[0].dim, rows, o, report);
+        if (report) {
+            report->pairs = rows;
+            report->ms_pairs = ms_pairs;
+        }
+        return int(VMX_OK);
+    });
+}
+
+int vmx_refiner_get(vmx_ctx* ctx, int32_t dim, float* params) {
+    return guarded(ctx, [&]() -> int {
+        const int kp = kp_of(dim);
+        if (!params) throw VmxError(VMX_ERR_INVALID, "null params");
+        if (!ctx->nn_ok[dim]) throw VmxError(VMX_ERR_INVALID, "no network for this dimension");
+        HIP_CHECK(hipMemcpy(params, ctx->nn[dim].p, sizeof(float) * n_total(kp), hipMemcpyDeviceToHost));
+        return int(VMX_OK);
+    });
+}
+
+int vmx_refiner_set(vmx_ctx* ctx, int32_t dim, const float* params) {
+    return guarded(ctx, [&]() -> int {
+        const int kp = kp_of(dim);
+        if (!params) throw VmxError(VMX_ERR_INVALID, "null params");
+        const uint64_t PT = n_total(kp);
+        for (uint64_t i = 0; i < PT; ++i)
+            if (!std::isfinite(params[i])) throw VmxError(VMX_ERR_INVALID, "non-finite parameter");
+        if (!(params[PT - 1] > 0.f)) throw VmxError(VMX_ERR_INVALID, "target scale must be > 0");
+        float* d = static_cast<float*>(ensure(ctx->nn[dim], sizeof(float) * PT));
+        HIP_CHECK(hipMemcpy(d, params, sizeof(float) * PT, hipMemcpyHostToDevice));
+        ctx->nn_ok[dim] = true;
+        return int(VMX_OK);
+    });
+}
+
+int vmx_refiner_forward(vmx_ctx* ctx, int32_t dim, const float* params, uint64_t n, const uint16_t* features,
+                        float* y) {
+    return guarded(ctx, [&]() -> int {
+        if (!features || !y) throw VmxError(VMX_ERR_INVALID, "null argument");
+        if (n == 0) return int(VMX_OK);
+        if (n > (1ull << 26)) throw VmxError(VMX_ERR_INVALID, "at most 2^26 rows");
+        const NetIn in = upload_net_inputs(ctx, dim, params, n, features, nullptr);
+        nn_forward(kp_of(dim), in.p, in.X, n, in.y, ctx->stream);
+        HIP_CHECK(hipMemcpyAsync(y, in.y, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return int(VMX_OK);
+    });
+}
+
+int vmx_refiner_grad(vmx_ctx* ctx, int32_t dim, const float* params, uint64_t n, const uint16_t* features,
+                     const float* targets, float* grad, double* loss) {
+    return guarded(ctx, [&]() -> int {
+        if (!params || !features || !targets || !grad) throw VmxError(VMX_ERR_INVALID, "null argument");
+        if (n == 0 || n > (1ull << 20)) throw VmxError(VMX_ERR_INVALID, "1 .. 2^20 rows");
+        const int kp = kp_of(dim);
+        const uint64_t P = n_trained(kp);
+        if (!(params[P + 1] > 0.f)) throw VmxError(VMX_ERR_INVALID, "target scale must be > 0");
+        const NetIn in = upload_net_inputs(ctx, dim, params, n, features, targets);
+        const uint32_t nblk = uint32_t((n + 63) / 64);
+        float* d_part = static_cast<float*>(ensure(ctx->tr_part, sizeof(float) * nblk * P));
+        float* d_loss = static_cast<float*>(ensure(ctx->tr_loss, sizeof(float) * nblk));
+        float* d_grad = static_cast<float*>(ensure(ctx->params, sizeof(float) * P));
+        nn_grad(kp, in.p, in.X, in.t, in.idx, uint32_t(n), 1.f / float(n), d_part, d_loss, 1, ctx->stream);
+        vmx::nn::k_sum_parts<<<unsigned((P + 255) / 256), 256, 0, ctx->stream>>>(d_part, nblk, uint32_t(P), d_grad);
+        HIP_CHECK(hipGetLastError());
+        std::vector<float> lh(nblk);
+        HIP_CHECK(hipMemcpyAsync(grad, d_grad, sizeof(float) * P, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_CHECK(hipMemcpyAsync(lh.data(), d_loss, sizeof(float) * nblk, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        if (loss) {
+            double sm = 0;
+            for (float x : lh) sm += x;
+            *loss = sm;
+        }
+        return int(VMX_OK);
+    });
+}
+
+int vmx_exact_bins(vmx_ctx* ctx, const vmx_problem* p, const vmx_integrand* f, double* bins) {
+    return guarded(ctx, [&]() -> int {
+        validate(p, f);
+        if (!bins) throw VmxError(VMX_ERR_INVALID, "null bins");
+        if (f->family != VMX_FAMILY_BLOBS || f->dim != 2 || p->bin_dims != 2)
+            throw VmxError(VMX_ERR_UNSUPPORTED, "closed form: 2-D blob integrands with 2 bin dims only");
+        const auto& q = *static_cast<const vmx_blobs_params*>(f->params);
+        if (q.step_mode == 2) throw VmxError(VMX_ERR_UNSUPPORTED, "closed form: additive step only");
+        vmx::ExactBlobs2D e{};
+        e.nblobs = q.nblobs;
+        e.step_mode = q.step_mode;
+        for (int k = 0; k < q.nblobs; ++k) {
+            if (q.blob[k][2] != 0.0 || q.blob[k][3] != 0.0)
+                throw VmxError(VMX_ERR_UNSUPPORTED, "closed form: static blobs only");
+            for (int i = 0; i < 7; ++i) e.blob[k][i] = q.blob[k][i];
+        }
+        for (int i = 0; i < 5; ++i) e.step[i] = q.step[i];
+        for (int i = 0; i < 3; ++i) e.wave[i] = q.wave[i];
+        const uint64_t nb = p->res[0] * p->res[1];
+        Carve c(ensure(ctx->misc, sizeof(e) + sizeof(double) * nb + 1024));
+        vmx::ExactBlobs2D* d_e = c.take<vmx::ExactBlobs2D>(1);
+        double* d_o = c.take<double>(nb);
+        hipStream_t st = ctx->stream;
+        HIP_CHECK(hipMemcpyAsync(d_e, &e, sizeof(e), hipMemcpyHostToDevice, st));
+        const double dx = (p->hi[0] - p->lo[0]) / double(p->res[0]), dy = (p->hi[1] - p->lo[1]) / double(p->res[1]);
+        vmx::k_exact_bins<<<unsigned((nb + 255) / 256), 256, 0, st>>>(d_e, p->lo[0], p->lo[1], dx, dy, p->res[0],
+                                                                       p->res[1], d_o);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipMemcpyAsync(bins, d_o, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        return int(VMX_OK);
+    });
+}
 
 int vmx_eval(vmx_ctx* ctx, const vmx_integrand* f, int32_t scalar, uint64_t n, const double* x, double* out) {
     return guarded(ctx, [&]() -> int {

@@ -19,7 +19,10 @@
 #ifndef VMX_PIPELINE_H
 #define VMX_PIPELINE_H
 
+#include <type_traits>
+
 #include "kernels.h"
+#include "refiner.h"
 
 namespace vmx {
 
@@ -39,6 +42,8 @@ struct Job {
     uint32_t seed_lo, seed_hi, stream;
     uint32_t N, node_base, node_cap, reg_base;
     int32_t db, mb, form, heur;
+    int32_t learn, pad_;  // learn: refinement priority from the network (refiner.h)
+    float lres[VMX_MAX_DIM];  // log2 of the bin count per dim (0 for integrated dims)
 };
 
 struct JobRt {
@@ -225,15 +230,18 @@ __global__ void __launch_bounds__(256) k_roots(const Job<T>* __restrict__ jobs, 
             t.hi[s * D + d] = L.box[0][1][d];
             t.code[s * D + d] = 1u;
         }
-        t.err[s] = e;
+        // with a learned priority the root goes first whatever its score (it is the
+        // only region), so its children are ranked by their own priorities
+        const T pe = jb.learn ? T(INFINITY) : e;
+        t.err[s] = pe;
         t.dim[s] = dm;
-        t.b[s] = e;
+        t.b[s] = pe;
         t.b1[s] = T(INFINITY);
         t.parent[s] = -1;
         t.child[s] = -1;
         t.job[s] = j;
         rt[j].count = jb.node_base + 1;
-        const uint32_t k = key16(e);
+        const uint32_t k = key16(pe);
         atomicAdd(&hist16[size_t(j) * 65536 + k], 1u);
         atomicAdd(&hist8[size_t(j) * 256 + (k >> 8)], 1u);
         cand0[s] = int32_t(s);
@@ -254,13 +262,27 @@ struct GenArgs {
     uint32_t* rounds; // rounds that had candidates
     uint32_t J;
     uint32_t max_rounds;
+    const float* net;  // FP32 parameters of the priority network (learned jobs)
+};
+
+/* LDS of the fused priority network: the block's 2*TPB new regions form the rows of one
+   forward pass per expansion step. */
+template <int D>
+struct LearnLds {
+    static constexpr int KP = nn::Dims<D>::KP;
+    static constexpr int TPB = GEN_THREADS / TeamShape<D>::TEAM;
+    static constexpr int R = (2 * TPB + 15) / 16 * 16;
+    nn::NetW<KP> net;
+    alignas(16) uint16_t xs[R][KP + 8];
+    alignas(16) uint16_t h[2][R][nn::WS];
+    float y[R];
 };
 
 /* Expansion of one candidate per team; all lanes of the block call it the same number
    of times (block-uniform loop), teams without a candidate idle through the syncs. */
-template <typename T, int D, typename F, typename M>
+template <typename T, int D, typename F, typename M, bool LEARN>
 __device__ inline void team_expand(const GenArgs<T>& g, const F* __restrict__ fs, TeamLds<T, D>& L, int par,
-                                   const uint32_t* pre, uint64_t w, bool has) {
+                                   const uint32_t* pre, uint64_t w, bool has, LearnLds<D>* ll, int team) {
     constexpr int K = pow3(D), KM = pow3(D - 1);
     constexpr int TEAM = TeamShape<D>::TEAM;
     const int lane = threadIdx.x % TEAM;
@@ -335,6 +357,37 @@ __device__ inline void team_expand(const GenArgs<T>& g, const F* __restrict__ fs
     T e = 0;
     int dm = 0;
     team_heuristic<T, D, TEAM>(L, 2, heur, lane, go, e, dm);
+    if constexpr (LEARN) {
+        // learned priority of both children (lane c scores child c); the heuristic's
+        // split dim is kept, and its error stands in where the network output is NaN
+        constexpr int KP = nn::Dims<D>::KP;
+        const bool lrn = go && g.jobs[L.job].learn != 0;
+        float fsc = 0.f;
+        int fdepth = 0;
+        if (lane < 2) {
+            uint16_t* row = ll->xs[team * 2 + lane];
+            if (lrn) {
+                const int c = lane;
+                const Job<T>& jb = g.jobs[L.job];
+                uint32_t code[D];
+                for (int d = 0; d < D; ++d) code[d] = t.code[size_t(L.node) * D + d];
+                code[L.dim] = code[L.dim] * 2u + uint32_t(c);
+                nn::region_features<T, D, KP>(
+                    L.box[c][0], L.box[c][1], jb.lo, jb.hi, L.cv[c], code, jb.lres,
+                    [row](int i, float x) { row[i] = nn::f2bf(x); }, fsc, fdepth);
+            } else {
+                for (int k = 0; k < KP; ++k) row[k] = 0;
+            }
+        }
+        __syncthreads();
+        nn::forward_block<KP>(ll->net, &ll->xs[0][0], LearnLds<D>::R, &ll->h[0][0][0], &ll->h[1][0][0],
+                              &ll->h[0][0][0], ll->y);
+        if (lrn && lane < 2) {
+            const float lp = ll->y[team * 2 + lane] * ll->net.sigma + ll->net.mu +
+                             2.f * log2f(vmax(fsc, 0x1p-100f)) - float(fdepth);
+            if (lp == lp) e = T(exp2f(vmin(vmax(lp, -126.f), 126.f)));
+        }
+    }
     if (go && lane < 2) {
         const int c = lane;
         const uint32_t j = uint32_t(L.job);
@@ -410,16 +463,25 @@ __global__ void __launch_bounds__(256) k_theta_round(GenArgs<T> g, int par) {
     if (threadIdx.x == 0) g.theta[j] = th;
 }
 
-template <typename T, int D, typename F, typename M>
+template <typename T, int D, typename F, typename M, bool LEARN>
 __global__ void __launch_bounds__(GEN_THREADS) k_expand_round(GenArgs<T> g, const F* __restrict__ fs, int par) {
     constexpr int TEAM = TeamShape<D>::TEAM;
     constexpr int TPB = GEN_THREADS / TEAM;  // teams per block
     __shared__ uint32_t pre[MAX_JOBS + 1];
     __shared__ TeamLds<T, D> lds[TPB];
+    __shared__ typename std::conditional<LEARN, LearnLds<D>, char>::type learn_lds;
+    LearnLds<D>* ll = nullptr;
     const uint32_t J = g.J;
     const int team = threadIdx.x / TEAM;
     for (uint32_t j = threadIdx.x; j < J; j += blockDim.x) pre[j + 1] = g.ncand[par * J + j];
     if (threadIdx.x == 0) pre[0] = 0;
+    if constexpr (LEARN) {
+        ll = &learn_lds;
+        nn::load_net<LearnLds<D>::KP>(ll->net, g.net);
+        constexpr int KP = LearnLds<D>::KP;
+        for (int i = threadIdx.x + 2 * TPB * KP; i < LearnLds<D>::R * KP; i += blockDim.x)
+            ll->xs[i / KP][i % KP] = 0;  // padding rows stay zero
+    }
     __syncthreads();
     if (threadIdx.x == 0)
         for (uint32_t j = 1; j <= J; ++j) pre[j] += pre[j - 1];
@@ -428,8 +490,75 @@ __global__ void __launch_bounds__(GEN_THREADS) k_expand_round(GenArgs<T> g, cons
     if (blockIdx.x == 0 && threadIdx.x == 0 && total > 0) atomicAdd(g.rounds, 1u);
     for (uint64_t base = uint64_t(blockIdx.x) * TPB; base < total; base += uint64_t(gridDim.x) * TPB) {
         const uint64_t w = base + team;
-        team_expand<T, D, F, M>(g, fs, lds[team], par, pre, w, w < total);
+        team_expand<T, D, F, M, LEARN>(g, fs, lds[team], par, pre, w, w < total, ll, team);
     }
+}
+
+/* ------------------------------------------------------------------ training pairs
+   Features and FP64 target (nn::TargetKind) of every stride-th node the rounds
+   created (the regions the method itself generates), one thread per node; job j's
+   nodes land at rows out_base[j] + (slot - node_base_j) / stride. */
+template <typename T, int D, typename F64>
+__global__ void k_train_pairs(const Job<T>* __restrict__ jobs, TreeArrays<T> t, uint32_t cap_total,
+                              const uint64_t* __restrict__ out_base, const F64* __restrict__ fs64, int kind,
+                              uint32_t stride, uint16_t* __restrict__ X, float* __restrict__ tgt) {
+    constexpr int K = pow3(D), KM = pow3(D - 1), KP = nn::Dims<D>::KP;
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= cap_total) return;
+    const uint32_t j = t.job[s];
+    if (j == NONE) return;
+    const Job<T>& jb = jobs[j];
+    const uint32_t local = s - jb.node_base;
+    if (local % stride != 0) return;
+    const uint64_t row = out_base[j] + local / stride;
+    const T* lo = t.lo + size_t(s) * D;
+    const T* hi = t.hi + size_t(s) * D;
+    const T* v = t.v + size_t(s) * K;
+    uint16_t* xr = X + row * KP;
+    float sc = 0.f;
+    int depth = 0;
+    nn::region_features<T, D, KP>(lo, hi, jb.lo, jb.hi, v, t.code + size_t(s) * D, jb.lres,
+                                  [xr](int i, float x) { xr[i] = nn::f2bf(x); }, sc, depth);
+    double vol = 1.0;
+    for (int d = 0; d < D; ++d) vol *= double(hi[d]) - double(lo[d]);
+    const F64& f = fs64[j];
+    double e;
+    if (kind == nn::TARGET_ENERGY) {
+        e = nn::true_error<T, D, F64, PreciseMath>(f, lo, hi, v);
+    } else {
+        nn::BinGrid g{};
+        g.db = jb.db;
+        for (int d = 0; d < jb.db; ++d) {
+            g.glo[d] = double(jb.lo[d]);
+            g.step[d] = (double(jb.hi[d]) - double(jb.lo[d])) / double(jb.res[d]);
+            g.res[d] = jb.res[d];
+        }
+        double dlo[D], dhi[D], vp[K];
+        for (int d = 0; d < D; ++d) {
+            dlo[d] = double(lo[d]);
+            dhi[d] = double(hi[d]);
+        }
+        for (int k = 0; k < K; ++k) vp[k] = double(v[k]);
+        const int dim = t.dim[s];
+        double Vp = 0.0, Vc = 0.0;
+        if (kind == nn::TARGET_BINVAR) {
+            nn::bin_variances<D, F64, PreciseMath>(f, g, dlo, dhi, vp, dim, nullptr, nullptr, Vp, Vc);
+            e = Vp;
+        } else {
+            // children of the heuristic's split, new planes evaluated in FP64
+            double v0[K], v1[K];
+            split_inherit<double, D>(vp, dim, v0, v1);
+            for (int c = 0; c < 2; ++c)
+                for (int jj = 0; jj < KM; ++jj) {
+                    double x[D];
+                    const int idx = split_new_node<double, D>(dlo, dhi, dim, c, jj, x);
+                    (c == 0 ? v0 : v1)[idx] = f.template eval<PreciseMath, D>(x);
+                }
+            nn::bin_variances<D, F64, PreciseMath>(f, g, dlo, dhi, vp, dim, v0, v1, Vp, Vc);
+            e = Vp - Vc;
+        }
+    }
+    tgt[row] = nn::error_target(e, sc, vol);
 }
 
 /* ------------------------------------------------------------------ selection */

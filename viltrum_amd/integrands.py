@@ -213,6 +213,14 @@ def c1_family(seed, nblobs):
                  name="c1fam_s%d_k%d" % (seed, nblobs))
 
 
+def config4_integrand(seed):
+    """Config-1 style integrand of the learned-refinement workload: 1..8 blobs (count drawn
+    first from SplitMix64(seed)), then c1_family with the next draw as its seed."""
+    rng = SplitMix64(seed)
+    nb = rng.integer(1, 8)
+    return c1_family(rng.next() & 0xFFFFFFFF, nb)
+
+
 def config5(seed):
     """5-D: 3 Gaussian lobes over dims 2..4 (sigma U[.05,.3], centre U[.2,.8]), two
     half-space steps over all dims, 1 + 0.3 cos(2 pi k.x), k integer in [1,8]^5.
