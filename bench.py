@@ -1,13 +1,18 @@
 #!/usr/bin/env python3
-"""Benchmark of the per-bin control-variate path on one MI355X.
+"""Benchmark of the per-bin control-variate path on one MI355X.  Prints ONE JSON line.
 
-Step = one full integration (adaptive regions, control-variate integral and residual
-Monte Carlo per bin) of the workload, integrand parameters already resident and bins
-written to device memory.  Prints ONE JSON line.
+A step is one full integration pass (adaptive regions, control-variate integral and
+residual Monte Carlo per bin) over the workload, integrand parameters resident and bins
+written to device memory.
 
-Workload (default): 3-D integrand (2 bin dims + time) over [0,1]^3 into 512x512 FP32
-bins, 32 moving anisotropic Gaussian blobs times a seeded step occluder, 16384 regions,
-64 residual samples per bin, seed 0 -- the plain GPU path.
+Workloads:
+  config4 (default)  64 held-out 2-D config-1 style integrands at 512x512 FP32 bins:
+                     the learned-priority batched path against the plain GPU path at
+                     equal RMSE vs FP64 closed-form bins (value = learned+batched bins/s)
+  config2            3-D moving-blob integrand (32 blobs x step occluder), 512x512 bins,
+                     16384 regions, 64 spp, plain GPU path
+  config3            256 uneven integrals in one batched call vs one call each
+  config1            2-D config-1 integrand, 64x64 bins, 1024 regions, 16 spp
 """
 import argparse
 import json
@@ -245,7 +250,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="config2")
+    ap.add_argument("--workload", default="config4")
     ap.add_argument("--truth-spp", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--integrals", type=int, default=64, help="config4: held-out integrals")

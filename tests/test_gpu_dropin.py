@@ -21,5 +21,6 @@ def test_cpp_dropin_matches_python(ctx):
     b = np.zeros((32, 32), dtype=np.float32)
     V.integrate(V.integrator_crespo2021(255, 16, seed=1), b, I.config1(), V.range_primary(2), ctx=ctx)
     vals = [float(x) for x in out[1].split()[1:]]
-    assert vals[0] == float(b[3, 7]) and vals[1] == float(b[20, 11])
+    # 9 significant digits identify a float32 exactly
+    assert np.float32(vals[0]) == b[3, 7] and np.float32(vals[1]) == b[20, 11]
     assert abs(vals[2] - float(b.astype(np.float64).sum())) < 1e-6 * abs(vals[2])

@@ -254,7 +254,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     P* d_f = nullptr;
     uint32_t *d_hist16 = nullptr, *d_hist8 = nullptr, *d_ncand = nullptr, *d_rounds = nullptr;
     T* d_theta = nullptr;
-    int32_t* d_cand[2] = {nullptr, nullptr};
+    uint32_t* d_cstart = nullptr;
     uint32_t cap_total = 0;
     std::vector<vmx::JobRt> rt(J);
     // ---------------- 1. generation (retried with more node capacity if needed)
@@ -281,8 +281,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         m.take<uint8_t>(cap_total);
         m.take<uint32_t>(size_t(J) * 65536);
         m.take<uint32_t>(size_t(J) * 256);
-        m.take<int32_t>(cap_total);
-        m.take<int32_t>(cap_total);
+        m.take<uint32_t>(2 * J);
         m.take<uint32_t>(2 * J);
         m.take<T>(J);
         m.take<uint32_t>(4);
@@ -304,8 +303,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         t.popped = c.take<uint8_t>(cap_total);
         d_hist16 = c.take<uint32_t>(size_t(J) * 65536);
         d_hist8 = c.take<uint32_t>(size_t(J) * 256);
-        d_cand[0] = c.take<int32_t>(cap_total);
-        d_cand[1] = c.take<int32_t>(cap_total);
+        d_cstart = c.take<uint32_t>(2 * J);
         d_ncand = c.take<uint32_t>(2 * J);
         d_theta = c.take<T>(J);
         uint32_t* small = c.take<uint32_t>(4);
@@ -318,7 +316,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         HIP_CHECK(hipMemsetAsync(d_hist8, 0, sizeof(uint32_t) * J * 256, st));
         HIP_CHECK(hipMemsetAsync(d_ncand, 0, sizeof(uint32_t) * 2 * J, st));
         HIP_CHECK(hipMemsetAsync(small, 0, sizeof(uint32_t) * 4, st));
-        vmx::k_roots<T, D, P, Math><<<J, 256, 0, st>>>(d_jobs, d_rt, d_f, t, d_hist16, d_hist8, d_cand[0], d_ncand);
+        vmx::k_roots<T, D, P, Math><<<J, 256, 0, st>>>(d_jobs, d_rt, d_f, t, d_hist16, d_hist8, d_cstart, d_ncand);
         HIP_CHECK(hipGetLastError());
         vmx::GenArgs<T> g{};
         g.jobs = d_jobs;
@@ -326,8 +324,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         g.t = t;
         g.hist16 = d_hist16;
         g.hist8 = d_hist8;
-        g.cand[0] = d_cand[0];
-        g.cand[1] = d_cand[1];
+        g.cstart = d_cstart;
         g.ncand = d_ncand;
         g.theta = d_theta;
         g.rounds = d_rounds;

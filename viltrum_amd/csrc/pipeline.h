@@ -198,7 +198,7 @@ __device__ inline void team_heuristic(TeamLds<T, D>& L, int nc, int heur, int la
    One block per job: 256 lanes evaluate the 3^D nodes, one team scores the root. */
 template <typename T, int D, typename F, typename M>
 __global__ void __launch_bounds__(256) k_roots(const Job<T>* __restrict__ jobs, JobRt* rt, const F* __restrict__ fs,
-                                               TreeArrays<T> t, uint32_t* hist16, uint32_t* hist8, int32_t* cand0,
+                                               TreeArrays<T> t, uint32_t* hist16, uint32_t* hist8, uint32_t* cstart0,
                                                uint32_t* ncand0) {
     constexpr int K = pow3(D);
     constexpr int TEAM = TeamShape<D>::TEAM;
@@ -244,7 +244,7 @@ __global__ void __launch_bounds__(256) k_roots(const Job<T>* __restrict__ jobs, 
         const uint32_t k = key16(pe);
         atomicAdd(&hist16[size_t(j) * 65536 + k], 1u);
         atomicAdd(&hist8[size_t(j) * 256 + (k >> 8)], 1u);
-        cand0[s] = int32_t(s);
+        cstart0[j] = uint32_t(s);
         ncand0[j] = jb.N > 1 ? 1u : 0u;
     }
 }
@@ -256,13 +256,22 @@ struct GenArgs {
     TreeArrays<T> t;
     uint32_t* hist16;
     uint32_t* hist8;
-    int32_t* cand[2];
-    uint32_t* ncand;  // [2*J]
+    uint32_t* cstart;  // [2*J] first slot of the round's candidates (they are contiguous)
+    uint32_t* ncand;   // [2*J] number of candidates
     T* theta;         // [J]
     uint32_t* rounds; // rounds that had candidates
     uint32_t J;
     uint32_t max_rounds;
     const float* net;  // FP32 parameters of the priority network (learned jobs)
+};
+
+/* Per-team slot requests of one expansion step, combined into one atomic per block and
+   job (teams of a step hold consecutive candidates, so a job's teams are one run). */
+template <int TPB>
+struct SlotLds {
+    uint32_t job[TPB];
+    int32_t slot[TPB];
+    uint8_t want[TPB];
 };
 
 /* LDS of the fused priority network: the block's 2*TPB new regions form the rows of one
@@ -282,14 +291,17 @@ struct LearnLds {
    of times (block-uniform loop), teams without a candidate idle through the syncs. */
 template <typename T, int D, typename F, typename M, bool LEARN>
 __device__ inline void team_expand(const GenArgs<T>& g, const F* __restrict__ fs, TeamLds<T, D>& L, int par,
-                                   const uint32_t* pre, uint64_t w, bool has, LearnLds<D>* ll, int team) {
+                                   const uint32_t* pre, uint64_t w, bool has, LearnLds<D>* ll, int team,
+                                   SlotLds<GEN_THREADS / TeamShape<D>::TEAM>& sl) {
     constexpr int K = pow3(D), KM = pow3(D - 1);
     constexpr int TEAM = TeamShape<D>::TEAM;
+    constexpr int TPB = GEN_THREADS / TEAM;
     const int lane = threadIdx.x % TEAM;
     const uint32_t J = g.J;
     TreeArrays<T> t = g.t;
     if (lane == 0) {
         L.go = 0;
+        sl.want[team] = 0;
         if (has) {
             uint32_t lo = 0, hi = J;
             while (hi - lo > 1) {
@@ -300,30 +312,56 @@ __device__ inline void team_expand(const GenArgs<T>& g, const F* __restrict__ fs
                     hi = mid;
             }
             const uint32_t j = lo;
-            const Job<T>& jb = g.jobs[j];
-            const int32_t node = g.cand[par][jb.node_base + (uint32_t(w) - pre[j])];
-            if (t.b[node] >= g.theta[j]) {
-                const uint32_t slot = atomicAdd(&g.rt[j].count, 2u);
-                if (slot + 2 > jb.node_base + jb.node_cap) {
-                    atomicOr(&g.rt[j].flags, JF_CAPACITY);
-                } else {
-                    L.go = 1;
-                    L.node = node;
-                    L.slot = int32_t(slot);
-                    L.job = int32_t(j);
-                    const int dim = t.dim[node];
-                    L.dim = dim;
-                    T plo[D], phi[D];
-                    for (int d = 0; d < D; ++d) {
-                        plo[d] = t.lo[size_t(node) * D + d];
-                        phi[d] = t.hi[size_t(node) * D + d];
-                        L.pbox[0][d] = plo[d];
-                        L.pbox[1][d] = phi[d];
-                    }
-                    split_boxes<T, D>(plo, phi, dim, L.box[0][0], L.box[0][1], L.box[1][0], L.box[1][1]);
-                }
-            }
+            const int32_t node = int32_t(g.cstart[par * J + j] + (uint32_t(w) - pre[j]));
+            L.node = node;
+            sl.job[team] = j;
+            sl.want[team] = t.b[node] >= g.theta[j] ? 1 : 0;
         }
+    }
+    __syncthreads();
+    // two child slots per expanding team: one atomic per run of teams of the same job
+    if (threadIdx.x == 0) {
+        for (int a = 0; a < TPB;) {
+            if (!sl.want[a]) {
+                sl.slot[a] = -1;
+                ++a;
+                continue;
+            }
+            const uint32_t j = sl.job[a];
+            int b = a, cnt = 0;
+            for (; b < TPB && (!sl.want[b] || sl.job[b] == j); ++b) cnt += sl.want[b];
+            const Job<T>& jb = g.jobs[j];
+            const uint32_t base = atomicAdd(&g.rt[j].count, 2u * uint32_t(cnt));
+            const bool fits = base + 2u * uint32_t(cnt) <= jb.node_base + jb.node_cap;
+            if (fits)
+                atomicAdd(&g.ncand[(par ^ 1) * J + j], 2u * uint32_t(cnt));
+            else
+                atomicOr(&g.rt[j].flags, JF_CAPACITY);
+            int k = 0;
+            for (int c = a; c < b; ++c) {
+                sl.slot[c] = (sl.want[c] && fits) ? int32_t(base + 2u * uint32_t(k)) : -1;
+                k += sl.want[c];
+            }
+            a = b;
+        }
+    }
+    __syncthreads();
+    if (lane == 0 && sl.want[team] && sl.slot[team] >= 0) {
+        const uint32_t j = sl.job[team];
+        const int32_t node = L.node;
+        L.go = 1;
+        L.slot = sl.slot[team];
+        L.job = int32_t(j);
+        const int dim = t.dim[node];
+        L.dim = dim;
+        T plo[D], phi[D];
+        for (int d = 0; d < D; ++d) {
+            plo[d] = t.lo[size_t(node) * D + d];
+            phi[d] = t.hi[size_t(node) * D + d];
+            L.pbox[0][d] = plo[d];
+            L.pbox[1][d] = phi[d];
+        }
+        split_boxes<T, D>(plo, phi, dim, L.box[0][0], L.box[0][1], L.box[1][0], L.box[1][1]);
     }
     __syncthreads();
     const bool go = L.go != 0;
@@ -423,13 +461,7 @@ __device__ inline void team_expand(const GenArgs<T>& g, const F* __restrict__ fs
         const uint32_t k = key16(nb);
         atomicAdd(&g.hist16[size_t(j) * 65536 + k], 1u);
         atomicAdd(&g.hist8[size_t(j) * 256 + (k >> 8)], 1u);
-        if (c == 0) {
-            t.child[node] = int32_t(s);
-            const uint32_t o = atomicAdd(&g.ncand[(par ^ 1) * J + j], 2u);
-            const uint32_t nbase = g.jobs[j].node_base;
-            g.cand[par ^ 1][nbase + o] = int32_t(s);
-            g.cand[par ^ 1][nbase + o + 1] = int32_t(s + 1);
-        }
+        if (c == 0) t.child[node] = int32_t(s);
     }
     __syncthreads();
 }
@@ -445,8 +477,11 @@ __global__ void __launch_bounds__(256) k_theta_round(GenArgs<T> g, int par) {
     __shared__ uint32_t sh[256];
     const uint32_t j = blockIdx.x;
     const Job<T>& jb = g.jobs[j];
-    // the counter this round's expansion fills (last read by the previous round)
-    if (threadIdx.x == 0) g.ncand[(par ^ 1) * g.J + j] = 0;
+    // this round's children are appended from the job's current count on
+    if (threadIdx.x == 0) {
+        g.cstart[(par ^ 1) * g.J + j] = g.rt[j].count;
+        g.ncand[(par ^ 1) * g.J + j] = 0;
+    }
     if (g.ncand[par * g.J + j] == 0) return;
     const uint32_t total = g.rt[j].count - jb.node_base;
     const uint32_t need = jb.N - 1;
@@ -469,6 +504,7 @@ __global__ void __launch_bounds__(GEN_THREADS) k_expand_round(GenArgs<T> g, cons
     constexpr int TPB = GEN_THREADS / TEAM;  // teams per block
     __shared__ uint32_t pre[MAX_JOBS + 1];
     __shared__ TeamLds<T, D> lds[TPB];
+    __shared__ SlotLds<TPB> slots;
     __shared__ typename std::conditional<LEARN, LearnLds<D>, char>::type learn_lds;
     LearnLds<D>* ll = nullptr;
     const uint32_t J = g.J;
@@ -490,7 +526,7 @@ __global__ void __launch_bounds__(GEN_THREADS) k_expand_round(GenArgs<T> g, cons
     if (blockIdx.x == 0 && threadIdx.x == 0 && total > 0) atomicAdd(g.rounds, 1u);
     for (uint64_t base = uint64_t(blockIdx.x) * TPB; base < total; base += uint64_t(gridDim.x) * TPB) {
         const uint64_t w = base + team;
-        team_expand<T, D, F, M, LEARN>(g, fs, lds[team], par, pre, w, w < total, ll, team);
+        team_expand<T, D, F, M, LEARN>(g, fs, lds[team], par, pre, w, w < total, ll, team, slots);
     }
 }
 
