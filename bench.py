@@ -259,6 +259,8 @@ def main():
                     help="config4: training budgets cycle through 1/4, 1/2, 1, 2 times this")
     ap.add_argument("--train-epochs", type=int, default=8)
     ap.add_argument("--train-stride", type=int, default=2, help="config4: keep every n-th generated region")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device to device (a known-size kernel for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -267,6 +269,15 @@ def main():
     import viltrum_amd as V
     from viltrum_amd import hip
     from viltrum_amd.integrands import flops_per_eval
+
+    if args.calibrate_copy > 0:
+        nbytes = args.calibrate_copy << 20
+        src, dst = hip.DeviceBuffer(nbytes), hip.DeviceBuffer(nbytes)
+        src.zero()
+        hip.copy_device(dst, src, nbytes)
+        hip.device_synchronize()
+        src.free()
+        dst.free()
 
     if args.workload == "config3":
         return run_config3(args, V, hip)

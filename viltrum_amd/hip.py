@@ -95,5 +95,12 @@ class Event:
         return ms.value
 
 
+D2D = 3
+
+
+def copy_device(dst, src, nbytes):
+    check(hip().hipMemcpy(dst.ptr, src.ptr, nbytes, D2D), "hipMemcpy D2D")
+
+
 def device_synchronize():
     check(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
