@@ -10,7 +10,8 @@
  *      replayed on the host over the GPU-computed errors
  *      (src/nested/regions-generator-adaptive-heap.h:21-41).
  *   3. k_prepare2 (boxes, volumes, FP64 bin-dim marginals, monomial coefficients),
- *      footprints, per-bin footprint lists (k_stratify2, device scan, k_sort_lists).
+ *      footprints, per-bin footprint lists (pairs emitted per footprint at scanned
+ *      offsets, stable radix sort by bin, boundary offsets).
  *   4. k_bins2: per-bin control variate + residual Monte Carlo.
  * Host synchronisation: after generation (capacity check) and once to size the bin
  * lists.
@@ -485,11 +486,10 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
 
     // ---------------- 3. regions, footprints, bin lists
     const auto t_str = std::chrono::steady_clock::now();
-    size_t scan_reg_bytes = 0, scan_bin_bytes = 0;
+    size_t scan_reg_bytes = 0;
     HIP_CHECK(hipError_t(rocprim::exclusive_scan(nullptr, scan_reg_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr, 0u,
                                                 size_t(n_reg + 1), rocprim::plus<uint32_t>(), st)));
-    HIP_CHECK(hipError_t(rocprim::exclusive_scan(nullptr, scan_bin_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                                0u, size_t(total_bins + 1), rocprim::plus<uint32_t>(), st)));
+
     Measure mr;
     mr.take<T>(n_reg * D);
     mr.take<T>(n_reg * D);
@@ -505,9 +505,10 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     mr.take<double>(n_reg * mb_max);
     for (int i = 0; i < 3; ++i) mr.take<uint32_t>(n_reg);
     mr.take<uint32_t>(n_reg * 6);
+    mr.take<uint32_t>(n_reg + 1);
+    mr.take<uint32_t>(n_reg + 1);
     mr.take<uint32_t>(total_bins + 1);
-    mr.take<uint32_t>(total_bins + 1);
-    mr.take<char>(std::max(scan_reg_bytes, scan_bin_bytes) + 256);
+    mr.take<char>(scan_reg_bytes + 256);
     Carve cr(ensure(ctx->lists, mr.used));
     vmx::RegionArrays<T> ra{};
     ra.lo = cr.take<T>(n_reg * D);
@@ -527,9 +528,10 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     fa.count = cr.take<uint32_t>(n_reg);
     fa.job = cr.take<uint32_t>(n_reg);
     fa.box = cr.take<uint32_t>(n_reg * 6);
-    uint32_t* d_cnt = cr.take<uint32_t>(total_bins + 1);
+    fa.npairs = cr.take<uint32_t>(n_reg + 1);
+    uint32_t* d_pofs = cr.take<uint32_t>(n_reg + 1);
     uint32_t* d_off = cr.take<uint32_t>(total_bins + 1);
-    void* d_scan_tmp = cr.take<char>(std::max(scan_reg_bytes, scan_bin_bytes) + 256);
+    void* d_scan_tmp = cr.take<char>(scan_reg_bytes + 256);
     // mb differs per job only through db; all jobs of a batch share the footprint stride
     for (const auto& jb : jobs)
         if (jb.mb != mb_max) throw VmxError(VMX_ERR_INVALID, "one bin_dims per batch");
@@ -538,27 +540,45 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     size_t bytes = scan_reg_bytes;
     HIP_CHECK(hipError_t(rocprim::exclusive_scan(d_scan_tmp, bytes, ra.fpflag, ra.fpid, 0u, size_t(n_reg + 1),
                                                 rocprim::plus<uint32_t>(), st)));
+    HIP_CHECK(hipMemsetAsync(fa.npairs, 0, sizeof(uint32_t) * (n_reg + 1), st));
     vmx::k_footprints<T, D><<<reg_blocks, 256, 0, st>>>(d_jobs, ra, fa, uint32_t(n_reg), mb_max, merge);
     HIP_CHECK(hipGetLastError());
     const uint32_t* d_nfp = ra.fpid + n_reg;
-    const unsigned strat_blocks = unsigned((n_reg * 64 + 255) / 256);
-    HIP_CHECK(hipMemsetAsync(d_cnt, 0, sizeof(uint32_t) * (total_bins + 1), st));
-    vmx::k_stratify2<T, false><<<strat_blocks, 256, 0, st>>>(d_jobs, fa, d_nfp, d_cnt, nullptr, nullptr);
-    HIP_CHECK(hipGetLastError());
-    bytes = scan_bin_bytes;
-    HIP_CHECK(hipError_t(rocprim::exclusive_scan(d_scan_tmp, bytes, d_cnt, d_off, 0u, size_t(total_bins + 1),
+    // per-bin footprint lists: pairs emitted per footprint at scanned offsets, then a
+    // stable radix sort by bin (each list ends up in ascending footprint order)
+    bytes = scan_reg_bytes;
+    HIP_CHECK(hipError_t(rocprim::exclusive_scan(d_scan_tmp, bytes, fa.npairs, d_pofs, 0u, size_t(n_reg + 1),
                                                 rocprim::plus<uint32_t>(), st)));
     uint32_t pairs = 0, nfp = 0;
-    HIP_CHECK(hipMemcpyAsync(&pairs, d_off + total_bins, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipMemcpyAsync(&pairs, d_pofs + n_reg, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipMemcpyAsync(&nfp, d_nfp, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipMemcpyAsync(rt.data(), d_rt, sizeof(vmx::JobRt) * J, hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
-    uint32_t* d_lst = static_cast<uint32_t*>(ensure(ctx->bins2, sizeof(uint32_t) * (size_t(pairs) + 1)));
-    HIP_CHECK(hipMemsetAsync(d_cnt, 0, sizeof(uint32_t) * (total_bins + 1), st));
-    vmx::k_stratify2<T, true><<<strat_blocks, 256, 0, st>>>(d_jobs, fa, d_nfp, d_cnt, d_off, d_lst);
-    HIP_CHECK(hipGetLastError());
-    const unsigned bin_blocks = unsigned((total_bins + 255) / 256);
-    vmx::k_sort_lists<<<bin_blocks, 256, 0, st>>>(d_off, total_bins, d_lst);
+    unsigned key_bits = 1;
+    while (key_bits < 32 && (uint64_t(1) << key_bits) < total_bins) ++key_bits;
+    size_t sort_pairs_bytes = 0;
+    HIP_CHECK(hipError_t(rocprim::radix_sort_pairs(nullptr, sort_pairs_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                  (uint32_t*)nullptr, (uint32_t*)nullptr, size_t(pairs), 0, key_bits,
+                                                  st)));
+    Measure mp;
+    for (int i = 0; i < 4; ++i) mp.take<uint32_t>(size_t(pairs) + 1);
+    mp.take<char>(sort_pairs_bytes + 256);
+    Carve cp(ensure(ctx->bins2, mp.used));
+    uint32_t* d_keys_in = cp.take<uint32_t>(size_t(pairs) + 1);
+    uint32_t* d_vals_in = cp.take<uint32_t>(size_t(pairs) + 1);
+    uint32_t* d_keys_out = cp.take<uint32_t>(size_t(pairs) + 1);
+    uint32_t* d_lst = cp.take<uint32_t>(size_t(pairs) + 1);
+    void* d_sort_pairs_tmp = cp.take<char>(sort_pairs_bytes + 256);
+    if (pairs > 0) {
+        const unsigned emit_blocks = unsigned((uint64_t(nfp) * 64 + 255) / 256);
+        vmx::k_emit_pairs<T><<<emit_blocks, 256, 0, st>>>(d_jobs, fa, d_nfp, d_pofs, d_keys_in, d_vals_in);
+        HIP_CHECK(hipGetLastError());
+        size_t sb = sort_pairs_bytes;
+        HIP_CHECK(hipError_t(rocprim::radix_sort_pairs(d_sort_pairs_tmp, sb, d_keys_in, d_keys_out, d_vals_in, d_lst,
+                                                      size_t(pairs), 0, key_bits, st)));
+    }
+    vmx::k_bin_offsets<<<unsigned((uint64_t(pairs) + 1 + 255) / 256), 256, 0, st>>>(d_keys_out, pairs, total_bins,
+                                                                                 d_off);
     HIP_CHECK(hipGetLastError());
     const double ms_str = ms_since(t_str);
 

@@ -48,22 +48,6 @@ __device__ inline uint32_t key16(double b) { return uint32_t(uint64_t(__double_a
 __device__ inline void key16_floor(uint32_t k, float& out) { out = __uint_as_float(k << 16); }
 __device__ inline void key16_floor(uint32_t k, double& out) { out = __longlong_as_double((long long)(uint64_t(k) << 48)); }
 
-/* Sort each bin's list ascending (list order = final region order). */
-__global__ void k_sort_lists(const uint32_t* __restrict__ off, uint64_t nb, uint32_t* __restrict__ lst) {
-    const uint64_t b = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (b >= nb) return;
-    const uint32_t a = off[b], e = off[b + 1];
-    for (uint32_t i = a + 1; i < e; ++i) {
-        const uint32_t x = lst[i];
-        uint32_t j = i;
-        while (j > a && lst[j - 1] > x) {
-            lst[j] = lst[j - 1];
-            --j;
-        }
-        lst[j] = x;
-    }
-}
-
 template <typename T>
 struct KahanSum {
     T s = 0, c = 0;

@@ -14,7 +14,8 @@
  * 3. Leaves (unsplit children of split nodes) in canonical order, then footprints:
  *    leaves that share their bin-dim extent share their pixel box, and their FP64
  *    bin-dim marginals add up (the control-variate integral is linear in the nodes).
- * 4. Per-bin footprint lists and the per-bin control variate + residual kernel.
+ * 4. Per-bin footprint lists (pairs sorted by bin) and the per-bin control variate +
+ *    residual kernel.
  */
 #ifndef VMX_PIPELINE_H
 #define VMX_PIPELINE_H
@@ -800,6 +801,7 @@ struct FootArrays {
     double* scale;       // [F]
     double* marg;        // [F*mb]
     uint32_t *first, *count, *box, *job;  // box [F*6]
+    uint32_t* npairs;    // [F+1] bins in the pixel box (exclusive-scanned into pair offsets)
 };
 
 template <typename T, int D>
@@ -865,6 +867,10 @@ __global__ void k_footprints(const Job<T>* __restrict__ jobs, RegionArrays<T> r,
         f.box[size_t(id) * 6 + 2 * d] = uint32_t(a);
         f.box[size_t(id) * 6 + 2 * d + 1] = uint32_t(z);
     }
+    {
+        const uint32_t* bx = f.box + size_t(id) * 6;
+        f.npairs[id] = (bx[1] - bx[0]) * (bx[3] - bx[2]) * (bx[5] - bx[4]);
+    }
     if (merge) {
         double acc[27];
         for (int k = 0; k < mb; ++k) acc[k] = 0.0;
@@ -882,27 +888,41 @@ __global__ void k_footprints(const Job<T>* __restrict__ jobs, RegionArrays<T> r,
 
 /* Count (Fill = false) or place (Fill = true) footprint ids into their bins' lists:
    one wave per footprint, lanes over the pixel box. */
-template <typename T, bool Fill>
-__global__ void k_stratify2(const Job<T>* __restrict__ jobs, FootArrays<T> f, const uint32_t* __restrict__ nfp_ptr,
-                            uint32_t* __restrict__ cnt, const uint32_t* __restrict__ off, uint32_t* __restrict__ lst) {
+/* (bin, footprint) pairs of every footprint's pixel box, one wave per footprint, at the
+   footprint's offset of the scanned pair counts: no atomics, and footprint ids ascend
+   along the array, so a stable sort by bin leaves each bin's list in region order. */
+template <typename T>
+__global__ void k_emit_pairs(const Job<T>* __restrict__ jobs, FootArrays<T> f, const uint32_t* __restrict__ nfp_ptr,
+                             const uint32_t* __restrict__ pofs, uint32_t* __restrict__ keys,
+                             uint32_t* __restrict__ vals) {
     const uint32_t w = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
     const uint32_t lane = threadIdx.x % 64;
     if (w >= *nfp_ptr) return;
     const Job<T>& jb = jobs[f.job[w]];
     const uint32_t* bx = f.box + size_t(w) * 6;
-    const uint32_t n0 = bx[1] - bx[0], n1 = bx[3] - bx[2], n2 = bx[5] - bx[4];
-    const uint64_t total = uint64_t(n0) * n1 * n2;
-    for (uint64_t c = lane; c < total; c += 64) {
-        const uint32_t p0 = bx[0] + uint32_t(c % n0);
-        const uint32_t p1 = bx[2] + uint32_t((c / n0) % n1);
-        const uint32_t p2 = bx[4] + uint32_t(c / (uint64_t(n0) * n1));
+    const uint32_t n0 = bx[1] - bx[0], n1 = bx[3] - bx[2];
+    const uint32_t total = f.npairs[w];
+    const uint32_t o = pofs[w];
+    for (uint32_t c = lane; c < total; c += 64) {
+        const uint32_t p0 = bx[0] + c % n0;
+        const uint32_t p1 = bx[2] + (c / n0) % n1;
+        const uint32_t p2 = bx[4] + c / (n0 * n1);
         uint64_t local = p0;
         if (jb.db > 1) local = local * jb.res[1] + p1;
         if (jb.db > 2) local = local * jb.res[2] + p2;
-        const uint64_t bi = jb.bin_base + local;
-        const uint32_t k = atomicAdd(&cnt[bi], 1u);
-        if (Fill) lst[off[bi] + k] = w;
+        keys[o + c] = uint32_t(jb.bin_base + local);
+        vals[o + c] = w;
     }
+}
+
+/* off[b] = first position of bin b in the bin-sorted pair keys (off[B] = P). */
+__global__ void k_bin_offsets(const uint32_t* __restrict__ keys, uint32_t P, uint64_t total_bins,
+                              uint32_t* __restrict__ off) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i > P) return;
+    const uint64_t prev = i == 0 ? 0 : uint64_t(keys[i - 1]) + 1;
+    const uint64_t cur = i == P ? total_bins + 1 : uint64_t(keys[i]) + 1;
+    for (uint64_t b = prev; b < cur; ++b) off[b] = uint32_t(i);
 }
 
 template <typename T>
@@ -930,6 +950,15 @@ struct BinLds {
     uint64_t b;
 };
 
+/* Makes a team's LDS writes visible to its other lanes.  A team never spans waves
+   (L <= 64) and a wave issues its LDS operations in order, so a wave-scope fence that
+   also stops the compiler from moving LDS accesses across it is enough. */
+__device__ inline void team_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 /* sum over the `width` lanes of a team (width a power of two <= 64) */
 __device__ inline double team_sum(double v, int width) {
     for (int m = width / 2; m > 0; m >>= 1) v += __shfl_xor(v, m, width);
@@ -955,9 +984,10 @@ __global__ void __launch_bounds__(256) k_bins3(const Job<T>* __restrict__ jobs, 
     const int teams = blockDim.x / L;
     const int team = threadIdx.x / L, lane = threadIdx.x % L;
     BinLds<T>& B = lds[team];
-    for (uint64_t base = uint64_t(blockIdx.x) * teams; base < total_bins; base += uint64_t(gridDim.x) * teams) {
-        const uint64_t gb = base + team;
-        const bool active = gb < total_bins;
+    // every team walks its own bins: no block-wide barrier, so a bin with a long list or
+    // many samples holds up only its own wave
+    for (uint64_t gb = uint64_t(blockIdx.x) * teams + team; gb < total_bins; gb += uint64_t(gridDim.x) * teams) {
+        const bool active = true;
         uint32_t j = 0, o = 0, nf = 0;
         if (active) {
             j = job_of_bin(jobs, J, gb);
@@ -1010,7 +1040,7 @@ __global__ void __launch_bounds__(256) k_bins3(const Job<T>* __restrict__ jobs, 
                                      : factor * (fa.scale[fi] *
                                                  marginal_subrange_n<double, db>(fa.marg + size_t(fi) * jb.mb, aa, bb));
             }
-        __syncthreads();
+        team_sync();
         if (active && lane == 0) {
             T* bins = static_cast<T*>(jb.bins);
             double approx = (jb.form == VMX_FORM_CV_ALPHA1) ? double(bins[b]) : 0.0;
@@ -1043,7 +1073,7 @@ __global__ void __launch_bounds__(256) k_bins3(const Job<T>* __restrict__ jobs, 
             B.approx = approx;
             B.n = n;
         }
-        __syncthreads();
+        team_sync();
         const uint32_t n = active ? B.n : 0;
         const double approx = active ? B.approx : 0.0;
         const bool sampling = active && jb.spp > 0 && n > 0;
@@ -1154,7 +1184,7 @@ __global__ void __launch_bounds__(256) k_bins3(const Job<T>* __restrict__ jobs, 
                 static_cast<T*>(jb.bins)[b] = T(result);
             }
         }
-        __syncthreads();
+        team_sync();
     }
 }
 
