@@ -213,7 +213,7 @@ def run_config4(args, V, hip):
                                                             "ms_total")},
         "plain_phase_ms_per_integral": {k: float(np.mean([getattr(x, k) for x in st_plain])) for k in
                                         ("ms_generate", "ms_select", "ms_stratify", "ms_bins", "ms_total")},
-        "roofline": {"bound": "valu", "kernel": "k_bins3 (per-bin CV + residual Monte Carlo, batched)",
+        "roofline": {"bound": "valu", "kernel": "k_bins1 (per-bin CV + residual Monte Carlo, batched)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel_ms": ms_k},
         "training": {k: rep[k] for k in ("pairs", "steps", "loss_first", "loss_train", "loss_valid", "ms_pairs",
@@ -339,7 +339,7 @@ def main():
         "fp64_truth": {"kind": "GPU FP64 stratified per-bin MC", "spp": args.truth_spp, "stderr_rms": truth_se},
         "phase_ms": {k: float(np.mean([s[k] for s in stats])) for k in
                      ("ms_generate", "ms_select", "ms_stratify", "ms_bins", "ms_total")},
-        "roofline": {"bound": "valu", "kernel": "k_bins (residual Monte Carlo)", "achieved": achieved,
+        "roofline": {"bound": "valu", "kernel": "k_bins1 (per-bin CV + residual Monte Carlo)", "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": None, "flops_per_eval": flops, "special_per_eval": special,
                      "kernel_ms": ms_k},

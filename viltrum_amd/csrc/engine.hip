@@ -12,7 +12,7 @@
  *   3. k_prepare2 (boxes, volumes, FP64 bin-dim marginals, monomial coefficients),
  *      footprints, per-bin footprint lists (pairs emitted per footprint at scanned
  *      offsets, stable radix sort by bin, boundary offsets).
- *   4. k_bins2: per-bin control variate + residual Monte Carlo.
+ *   4. k_bins1: per-bin control variate + residual Monte Carlo, one lane per bin.
  * Host synchronisation: after generation (capacity check) and once to size the bin
  * lists.
  */
@@ -583,29 +583,22 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     const double ms_str = ms_since(t_str);
 
     // ---------------- 4. bins
-    // lanes per bin: one residual sample per lane up to a wave
-    uint64_t spp_max = 0;
-    for (const auto& jb : jobs) spp_max = std::max(spp_max, jb.spp);
-    int L = 8;
-    while (L < 64 && uint64_t(L) * 8 < spp_max) L *= 2;
-    if (const char* e = std::getenv("VMX_BIN_LANES")) L = std::max(1, std::min(64, std::atoi(e)));
-    if (L & (L - 1)) throw VmxError(VMX_ERR_INVALID, "lanes per bin must be a power of two");
-    const int teams = 256 / L;
-    const size_t smem = sizeof(vmx::BinLds<T>) * teams;
-    const uint64_t want_blocks = (total_bins + teams - 1) / teams;
-    const unsigned bins_grid = unsigned(std::min<uint64_t>(want_blocks, uint64_t(n_cu) * 64));
+    // one lane per bin; a grid of at most 64 blocks per CU walks the bins
+    const uint64_t want_blocks = (total_bins + 255) / 256;
+    unsigned bins_grid = unsigned(std::min<uint64_t>(want_blocks, uint64_t(n_cu) * 64));
+    if (const char* e = std::getenv("VMX_BIN_BLOCKS")) bins_grid = unsigned(std::max(1, std::atoi(e)));
     HIP_CHECK(hipEventRecord(ctx->ev[0], st));
     const int db = jobs[0].db;
     if (db == 1)
-        vmx::k_bins3<T, D, 1, P, Math><<<bins_grid, 256, smem, st>>>(d_jobs, J, total_bins, L, ra, fa, d_off, d_lst,
-                                                                     d_f, d_rt);
+        vmx::k_bins1<T, D, 1, P, Math><<<bins_grid, 256, 0, st>>>(d_jobs, J, total_bins, ra, fa, d_off, d_lst, d_f,
+                                                                  d_rt);
     else if constexpr (D >= 2) {
         if (db == 2)
-            vmx::k_bins3<T, D, 2, P, Math><<<bins_grid, 256, smem, st>>>(d_jobs, J, total_bins, L, ra, fa, d_off,
-                                                                         d_lst, d_f, d_rt);
+            vmx::k_bins1<T, D, 2, P, Math><<<bins_grid, 256, 0, st>>>(d_jobs, J, total_bins, ra, fa, d_off, d_lst,
+                                                                      d_f, d_rt);
         else if constexpr (D >= 3)
-            vmx::k_bins3<T, D, 3, P, Math><<<bins_grid, 256, smem, st>>>(d_jobs, J, total_bins, L, ra, fa, d_off,
-                                                                         d_lst, d_f, d_rt);
+            vmx::k_bins1<T, D, 3, P, Math><<<bins_grid, 256, 0, st>>>(d_jobs, J, total_bins, ra, fa, d_off, d_lst,
+                                                                      d_f, d_rt);
     }
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipEventRecord(ctx->ev[1], st));

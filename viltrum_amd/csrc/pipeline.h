@@ -938,70 +938,37 @@ __device__ inline uint32_t job_of_bin(const Job<T>* jobs, uint32_t J, uint64_t b
     return lo;
 }
 
-constexpr int BIN_MAXF = 64;  // footprints per bin cached in LDS (longer lists walk memory)
+constexpr int LANE_MAXF = 16;  // footprints per bin cached in LDS (longer lists walk memory)
 
-template <typename T>
-struct BinLds {
-    uint32_t fi[BIN_MAXF];
-    uint32_t cum[BIN_MAXF + 1];
-    double contrib[BIN_MAXF];
-    double approx;
-    uint32_t n, nf, o, job;
-    uint64_t b;
-};
-
-/* Makes a team's LDS writes visible to its other lanes.  A team never spans waves
-   (L <= 64) and a wave issues its LDS operations in order, so a wave-scope fence that
-   also stops the compiler from moving LDS accesses across it is enough. */
-__device__ inline void team_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-/* sum over the `width` lanes of a team (width a power of two <= 64) */
-__device__ inline double team_sum(double v, int width) {
-    for (int m = width / 2; m > 0; m >>= 1) v += __shfl_xor(v, m, width);
-    return v;
-}
-
-/* A team of L lanes per bin (all jobs): control variate over the bin's footprints
-   (FP64, summed in list order by one lane), then the spp residual samples spread over
-   the lanes -- a region drawn uniformly among the bin's regions (binary search over
-   footprint counts in LDS), a point uniform in bin x region -- and the estimator of the
-   job's form (…regions-integrator-parallel-variance-reduction.h:67-103,
+/* One lane per bin (all jobs of the batch).  The lane sums the control variate over the
+   bin's footprints in list order (FP64), then draws the spp residual samples -- a region
+   uniform among the bin's regions (binary search over footprint counts cached in LDS),
+   a point uniform in bin x region -- and forms the estimator of the job's form
+   (…regions-integrator-parallel-variance-reduction.h:67-103,
    …regions-integrator-parallel-control-variates.h:39-65, weight-strategy.h:7-110).
-   Moments use the reference's shift by the bin's first sample; lane partial sums are
-   Kahan-compensated in the evaluation type and reduced across the team in FP64. */
+   Moments use the reference's shift by the bin's first sample; sums are
+   Kahan-compensated in the evaluation type and finished in FP64.  Summation depends on
+   the job alone, so a batched integral gives exactly what it gives alone. */
 template <typename T, int D, int DB, typename F, typename M>
-__global__ void __launch_bounds__(256) k_bins3(const Job<T>* __restrict__ jobs, uint32_t J, uint64_t total_bins,
-                                               int L, RegionArrays<T> r, FootArrays<T> fa,
+__global__ void __launch_bounds__(256) k_bins1(const Job<T>* __restrict__ jobs, uint32_t J, uint64_t total_bins,
+                                               RegionArrays<T> r, FootArrays<T> fa,
                                                const uint32_t* __restrict__ off, const uint32_t* __restrict__ lst,
                                                const F* __restrict__ fs, JobRt* rt) {
     constexpr int K = pow3(D);
-    extern __shared__ unsigned char smem_raw[];
-    BinLds<T>* lds = reinterpret_cast<BinLds<T>*>(smem_raw);
-    const int teams = blockDim.x / L;
-    const int team = threadIdx.x / L, lane = threadIdx.x % L;
-    BinLds<T>& B = lds[team];
-    // every team walks its own bins: no block-wide barrier, so a bin with a long list or
-    // many samples holds up only its own wave
-    for (uint64_t gb = uint64_t(blockIdx.x) * teams + team; gb < total_bins; gb += uint64_t(gridDim.x) * teams) {
-        const bool active = true;
-        uint32_t j = 0, o = 0, nf = 0;
-        if (active) {
-            j = job_of_bin(jobs, J, gb);
-            o = off[gb];
-            nf = off[gb + 1] - o;
-        }
+    __shared__ uint32_t s_fi[LANE_MAXF][256];
+    __shared__ uint32_t s_cum[LANE_MAXF + 1][256];
+    const int tid = threadIdx.x;
+    for (uint64_t gb = uint64_t(blockIdx.x) * blockDim.x + tid; gb < total_bins;
+         gb += uint64_t(gridDim.x) * blockDim.x) {
+        const uint32_t j = job_of_bin(jobs, J, gb);
         const Job<T>& jb = jobs[j];
-        const uint64_t b = active ? gb - jb.bin_base : 0;
-        constexpr int db = DB;
+        const uint32_t o = off[gb], nf = off[gb + 1] - o;
+        const uint64_t b = gb - jb.bin_base;
         uint32_t p[3] = {0, 0, 0};
         {
             uint64_t rem = b;
 #pragma unroll
-            for (int i = db - 1; i >= 0; --i) {
+            for (int i = DB - 1; i >= 0; --i) {
                 p[i] = uint32_t(rem % jb.res[i]);
                 rem /= jb.res[i];
             }
@@ -1013,85 +980,56 @@ __global__ void __launch_bounds__(256) k_bins3(const Job<T>* __restrict__ jobs, 
             bhi[d] = jb.hi[d];
         }
 #pragma unroll
-        for (int i = 0; i < db; ++i) {
+        for (int i = 0; i < DB; ++i) {
             blo[i] = jb.lo[i] + T(p[i]) * jb.drange[i];
             bhi[i] = jb.lo[i] + T(p[i] + 1) * jb.drange[i];
         }
         const double factor = double(jb.nbins);
-        // control-variate terms of the cached footprints
-        const uint32_t nc = nf < uint32_t(BIN_MAXF) ? nf : uint32_t(BIN_MAXF);
-        if (active)
-            for (uint32_t k = lane; k < nc; k += L) {
-                const uint32_t fi = lst[o + k];
-                B.fi[k] = fi;
-                B.cum[k + 1] = fa.count[fi];
-                double aa[db], bb[db];
-                bool empty = false;
-#pragma unroll
-                for (int d = 0; d < db; ++d) {
-                    const T rl = fa.lo[size_t(fi) * 3 + d], rh = fa.hi[size_t(fi) * 3 + d];
-                    const T il = vmax(blo[d], rl);
-                    const T ih = vmax(il, vmin(bhi[d], rh));
-                    if (il >= ih) empty = true;
-                    aa[d] = double(pos_in(il, rl, rh));
-                    bb[d] = double(pos_in(ih, rl, rh));
-                }
-                B.contrib[k] = empty ? 0.0
-                                     : factor * (fa.scale[fi] *
-                                                 marginal_subrange_n<double, db>(fa.marg + size_t(fi) * jb.mb, aa, bb));
+        T* bins = static_cast<T*>(jb.bins);
+        // control variate: footprint terms in list order
+        double approx = (jb.form == VMX_FORM_CV_ALPHA1) ? double(bins[b]) : 0.0;
+        const uint32_t nc = nf < uint32_t(LANE_MAXF) ? nf : uint32_t(LANE_MAXF);
+        uint32_t n = 0;
+        s_cum[0][tid] = 0;
+        for (uint32_t k = 0; k < nf; ++k) {
+            const uint32_t fi = lst[o + k];
+            n += fa.count[fi];
+            if (k < nc) {
+                s_fi[k][tid] = fi;
+                s_cum[k + 1][tid] = n;
             }
-        team_sync();
-        if (active && lane == 0) {
-            T* bins = static_cast<T*>(jb.bins);
-            double approx = (jb.form == VMX_FORM_CV_ALPHA1) ? double(bins[b]) : 0.0;
-            uint32_t n = 0;
-            B.cum[0] = 0;
-            for (uint32_t k = 0; k < nf; ++k) {
-                if (k < nc) {
-                    approx += B.contrib[k];
-                    n += B.cum[k + 1];
-                    B.cum[k + 1] = n;
-                } else {  // beyond the LDS cache: same terms, same order, from memory
-                    const uint32_t fi = lst[o + k];
-                    n += fa.count[fi];
-                    double aa[db], bb[db];
-                    bool empty = false;
+            double aa[DB], bb[DB];
+            bool empty = false;
 #pragma unroll
-                    for (int d = 0; d < db; ++d) {
-                        const T rl = fa.lo[size_t(fi) * 3 + d], rh = fa.hi[size_t(fi) * 3 + d];
-                        const T il = vmax(blo[d], rl);
-                        const T ih = vmax(il, vmin(bhi[d], rh));
-                        if (il >= ih) empty = true;
-                        aa[d] = double(pos_in(il, rl, rh));
-                        bb[d] = double(pos_in(ih, rl, rh));
-                    }
-                    if (!empty)
-                        approx += factor * (fa.scale[fi] *
-                                            marginal_subrange_n<double, db>(fa.marg + size_t(fi) * jb.mb, aa, bb));
-                }
+            for (int d = 0; d < DB; ++d) {
+                const T rl = fa.lo[size_t(fi) * 3 + d], rh = fa.hi[size_t(fi) * 3 + d];
+                const T il = vmax(blo[d], rl);
+                const T ih = vmax(il, vmin(bhi[d], rh));
+                if (il >= ih) empty = true;
+                aa[d] = double(pos_in(il, rl, rh));
+                bb[d] = double(pos_in(ih, rl, rh));
             }
-            B.approx = approx;
-            B.n = n;
+            approx += empty ? 0.0
+                            : factor * (fa.scale[fi] * marginal_subrange_n<double, DB>(fa.marg + size_t(fi) * jb.mb,
+                                                                                       aa, bb));
         }
-        team_sync();
-        const uint32_t n = active ? B.n : 0;
-        const double approx = active ? B.approx : 0.0;
-        const bool sampling = active && jb.spp > 0 && n > 0;
-        if (active && jb.spp > 0 && n == 0 && lane == 0) atomicOr(&rt[j].flags, JF_EMPTY_BIN);
-        // region index -> region: binary search over cached prefix counts, memory walk beyond
+        const uint64_t spp = jb.spp;
+        const bool sampling = spp > 0 && n > 0;
+        if (spp > 0 && n == 0) atomicOr(&rt[j].flags, JF_EMPTY_BIN);
+        // region index -> region: binary search over the cached prefix counts, memory walk beyond
         auto region_of = [&](uint32_t idx) -> uint32_t {
-            if (idx < B.cum[nc] || nc == nf) {
+            if (idx < s_cum[nc][tid] || nc == nf) {
                 uint32_t lo = 0, hi = nc;
                 while (hi - lo > 1) {
                     const uint32_t mid = (lo + hi) / 2;
-                    if (B.cum[mid] <= idx)
+                    if (s_cum[mid][tid] <= idx)
                         lo = mid;
                     else
                         hi = mid;
                 }
-                return fa.first[B.fi[lo]] + (idx - B.cum[lo]);
+                return fa.first[s_fi[lo][tid]] + (idx - s_cum[lo][tid]);
             }
-            idx -= B.cum[nc];
+            idx -= s_cum[nc][tid];
             for (uint32_t k = nc;; ++k) {
                 const uint32_t fi = lst[o + k];
                 const uint32_t c = fa.count[fi];
@@ -1099,92 +1037,74 @@ __global__ void __launch_bounds__(256) k_bins3(const Job<T>* __restrict__ jobs, 
                 idx -= c;
             }
         };
-        // per-lane accumulators
         KahanSum<T> sf, sa, mf, ma, maa, mfa;
         double acc1 = 0.0;  // alpha = 1 form
         T k_f = 0, k_a = 0;
-        const uint64_t spp = jb.spp;
-        bool first = true;
-        for (uint64_t s0 = 0; s0 < spp; s0 += L) {
-            const uint64_t s = s0 + lane;
-            const bool live = sampling && s < spp;
-            T fsv = 0, asv = 0;
-            bool empty = true;
-            double w = 0.0;
-            if (live) {
-                const SampleDraws<T, D> u(uint32_t(s), b, jb.stream, jb.seed_lo, jb.seed_hi);
-                const uint32_t reg = region_of(uniform_index(u.u[0], n));
-                T x[D], t[D], ilo[D], ihi[D];
-                empty = false;
+        const uint64_t ns_run = sampling ? spp : 0;
+        for (uint64_t s = 0; s < ns_run; ++s) {
+            const SampleDraws<T, D> u(uint32_t(s), b, jb.stream, jb.seed_lo, jb.seed_hi);
+            const uint32_t reg = region_of(uniform_index(u.u[0], n));
+            T x[D], t[D], ilo[D], ihi[D];
+            bool empty = false;
 #pragma unroll
-                for (int d = 0; d < D; ++d) {
-                    const T rl = r.lo[size_t(reg) * D + d], rh = r.hi[size_t(reg) * D + d];
-                    ilo[d] = vmax(blo[d], rl);
-                    ihi[d] = vmax(ilo[d], vmin(bhi[d], rh));
-                    if (ilo[d] >= ihi[d]) empty = true;
-                    x[d] = ilo[d] + u.coord(d) * (ihi[d] - ilo[d]);
-                    t[d] = pos_in(x[d], rl, rh);
-                }
-                const T fx = fs[j].template eval<M, D>(x);
-                const T px = eval_monomials<T, D>(r.coef + size_t(reg) * K, t);
-                w = factor * double(n) * double(box_volume<T, D>(ilo, ihi));
-                if (jb.form == VMX_FORM_CV_ALPHA1) {
-                    if (!empty) acc1 += double(fx - px) * w / double(spp);
-                } else {
-                    fsv = T(double(fx) * w);
-                    asv = T(double(px) * w);
-                }
+            for (int d = 0; d < D; ++d) {
+                const T rl = r.lo[size_t(reg) * D + d], rh = r.hi[size_t(reg) * D + d];
+                ilo[d] = vmax(blo[d], rl);
+                ihi[d] = vmax(ilo[d], vmin(bhi[d], rh));
+                if (ilo[d] >= ihi[d]) empty = true;
+                x[d] = ilo[d] + u.coord(d) * (ihi[d] - ilo[d]);
+                t[d] = pos_in(x[d], rl, rh);
             }
-            if (jb.form == VMX_FORM_CV_ALPHA1) continue;
-            if (first) {  // the bin's first sample (lane 0, s = 0) shifts the moments
-                k_f = __shfl(vabs(fsv), 0, L);
-                k_a = __shfl(vabs(asv), 0, L);
-                first = false;
+            const T fx = fs[j].template eval<M, D>(x);
+            const T px = eval_monomials<T, D>(r.coef + size_t(reg) * K, t);
+            const double w = factor * double(n) * double(box_volume<T, D>(ilo, ihi));
+            if (jb.form == VMX_FORM_CV_ALPHA1) {
+                if (!empty) acc1 += double(fx - px) * w / double(spp);
+                continue;
             }
-            if (live) {
-                sf.add(fsv);
-                sa.add(asv);
-                const T df = vabs(fsv) - k_f, da = vabs(asv) - k_a;
-                mf.add(df);
-                ma.add(da);
-                maa.add(da * da);
-                mfa.add(df * da);
+            const T fsv = T(double(fx) * w), asv = T(double(px) * w);
+            if (s == 0) {  // the bin's first sample shifts the moments
+                k_f = vabs(fsv);
+                k_a = vabs(asv);
             }
+            sf.add(fsv);
+            sa.add(asv);
+            const T df = vabs(fsv) - k_f, da = vabs(asv) - k_a;
+            mf.add(df);
+            ma.add(da);
+            maa.add(da * da);
+            mfa.add(df * da);
         }
         if (jb.form == VMX_FORM_CV_ALPHA1) {
-            const double tot = team_sum(acc1, L);
-            if (active && lane == 0) static_cast<T*>(jb.bins)[b] = T(approx + tot);
+            bins[b] = T(approx + acc1);
+            continue;
+        }
+        double result;
+        const double nsd = double(spp);
+        if (!sampling || (jb.form != VMX_FORM_VR_FIXED && spp < 2)) {
+            result = approx;
         } else {
-            const double Sf = team_sum(double(sf.s) - double(sf.c), L);
-            const double Sa = team_sum(double(sa.s) - double(sa.c), L);
-            const double Ef = team_sum(double(mf.s) - double(mf.c), L);
-            const double Ea = team_sum(double(ma.s) - double(ma.c), L);
-            const double Eaa = team_sum(double(maa.s) - double(maa.c), L);
-            const double Efa = team_sum(double(mfa.s) - double(mfa.c), L);
-            if (active && lane == 0) {
-                double result;
-                const double ns = double(spp);
-                if (!sampling) {
-                    result = approx;
-                } else if (jb.form == VMX_FORM_VR_FIXED) {
-                    result = (Sf - jb.alpha * Sa) / ns + jb.alpha * approx;
-                } else if (spp < 2) {
-                    result = approx;
-                } else {
-                    const double cov = (Efa - (Ef * Ea) / ns) / (ns - 1);
-                    const double var = (Eaa - (Ea * Ea) / ns) / (ns - 1);
-                    double alpha = 1.0;
-                    const double v = var > 0.0 ? var : 0.0;
-                    if (v > 0.0) {
-                        const double c = cov > 0.0 ? (cov < v ? cov : v) : 0.0;
-                        alpha = c / v;
-                    }
-                    result = (Sf - alpha * Sa) / ns + alpha * approx;
+            const double Sf = double(sf.s) - double(sf.c);
+            const double Sa = double(sa.s) - double(sa.c);
+            if (jb.form == VMX_FORM_VR_FIXED) {
+                result = (Sf - jb.alpha * Sa) / nsd + jb.alpha * approx;
+            } else {
+                const double Ef = double(mf.s) - double(mf.c);
+                const double Ea = double(ma.s) - double(ma.c);
+                const double Eaa = double(maa.s) - double(maa.c);
+                const double Efa = double(mfa.s) - double(mfa.c);
+                const double cov = (Efa - (Ef * Ea) / nsd) / (nsd - 1);
+                const double var = (Eaa - (Ea * Ea) / nsd) / (nsd - 1);
+                double alpha = 1.0;
+                const double v = var > 0.0 ? var : 0.0;
+                if (v > 0.0) {
+                    const double c = cov > 0.0 ? (cov < v ? cov : v) : 0.0;
+                    alpha = c / v;
                 }
-                static_cast<T*>(jb.bins)[b] = T(result);
+                result = (Sf - alpha * Sa) / nsd + alpha * approx;
             }
         }
-        team_sync();
+        bins[b] = T(result);
     }
 }
 
