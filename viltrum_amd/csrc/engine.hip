@@ -583,10 +583,8 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     const double ms_str = ms_since(t_str);
 
     // ---------------- 4. bins
-    // one lane per bin; a grid of at most 64 blocks per CU walks the bins
-    const uint64_t want_blocks = (total_bins + 255) / 256;
-    unsigned bins_grid = unsigned(std::min<uint64_t>(want_blocks, uint64_t(n_cu) * 64));
-    if (const char* e = std::getenv("VMX_BIN_BLOCKS")) bins_grid = unsigned(std::max(1, std::atoi(e)));
+    // one lane per bin, one block per 256 consecutive bins
+    const unsigned bins_grid = unsigned((total_bins + 255) / 256);
     HIP_CHECK(hipEventRecord(ctx->ev[0], st));
     const int db = jobs[0].db;
     if (db == 1)

@@ -940,7 +940,8 @@ __device__ inline uint32_t job_of_bin(const Job<T>* jobs, uint32_t J, uint64_t b
 
 constexpr int LANE_MAXF = 16;  // footprints per bin cached in LDS (longer lists walk memory)
 
-/* One lane per bin (all jobs of the batch).  The lane sums the control variate over the
+/* One lane per bin (all jobs of the batch), one block per 256 consecutive bins.  The
+   lane sums the control variate over the
    bin's footprints in list order (FP64), then draws the spp residual samples -- a region
    uniform among the bin's regions (binary search over footprint counts cached in LDS),
    a point uniform in bin x region -- and forms the estimator of the job's form
@@ -957,9 +958,31 @@ __global__ void __launch_bounds__(256) k_bins1(const Job<T>* __restrict__ jobs, 
     constexpr int K = pow3(D);
     __shared__ uint32_t s_fi[LANE_MAXF][256];
     __shared__ uint32_t s_cum[LANE_MAXF + 1][256];
+    // the integrand's parameters in LDS when the block's bins belong to one job (the
+    // common case): every evaluation then reads them as LDS broadcasts
+    __shared__ alignas(16) unsigned char f_raw[sizeof(F)];
+    __shared__ int block_job;
     const int tid = threadIdx.x;
-    for (uint64_t gb = uint64_t(blockIdx.x) * blockDim.x + tid; gb < total_bins;
-         gb += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t base = uint64_t(blockIdx.x) * blockDim.x;
+    if (tid == 0) {
+        const uint64_t last = (base + blockDim.x < total_bins ? base + blockDim.x : total_bins) - 1;
+        const uint32_t j0 = job_of_bin(jobs, J, base), j1 = job_of_bin(jobs, J, last);
+        block_job = j0 == j1 ? int(j0) : -1;
+    }
+    __syncthreads();
+    const int bj = block_job;
+    if (bj >= 0) {
+        const unsigned char* src = reinterpret_cast<const unsigned char*>(&fs[bj]);
+        for (int i = tid; i < int(sizeof(F) / 4); i += blockDim.x)
+            reinterpret_cast<uint32_t*>(f_raw)[i] = reinterpret_cast<const uint32_t*>(src)[i];
+        if (tid == 0)
+            for (int i = int(sizeof(F) / 4) * 4; i < int(sizeof(F)); ++i) f_raw[i] = src[i];
+    }
+    __syncthreads();
+    const F& f_lds = *reinterpret_cast<const F*>(f_raw);
+    {
+        const uint64_t gb = base + tid;
+        if (gb >= total_bins) return;
         const uint32_t j = job_of_bin(jobs, J, gb);
         const Job<T>& jb = jobs[j];
         const uint32_t o = off[gb], nf = off[gb + 1] - o;
@@ -1055,7 +1078,7 @@ __global__ void __launch_bounds__(256) k_bins1(const Job<T>* __restrict__ jobs, 
                 x[d] = ilo[d] + u.coord(d) * (ihi[d] - ilo[d]);
                 t[d] = pos_in(x[d], rl, rh);
             }
-            const T fx = fs[j].template eval<M, D>(x);
+            const T fx = bj >= 0 ? f_lds.template eval<M, D>(x) : fs[j].template eval<M, D>(x);
             const T px = eval_monomials<T, D>(r.coef + size_t(reg) * K, t);
             const double w = factor * double(n) * double(box_volume<T, D>(ilo, ihi));
             if (jb.form == VMX_FORM_CV_ALPHA1) {
@@ -1077,7 +1100,7 @@ __global__ void __launch_bounds__(256) k_bins1(const Job<T>* __restrict__ jobs, 
         }
         if (jb.form == VMX_FORM_CV_ALPHA1) {
             bins[b] = T(approx + acc1);
-            continue;
+            return;
         }
         double result;
         const double nsd = double(spp);
