@@ -134,9 +134,10 @@ struct Poly {
 
 template <typename T>
 struct Blobs {
+    /* one blob per row, (cx, cy, vx, vy, ax, ay, w, 0): a blob is two (FP32) or four
+       (FP64) 16-byte loads */
+    alignas(16) T blob[VMX_MAX_BLOBS][8];
     int nblobs, step_mode;
-    T cx[VMX_MAX_BLOBS], cy[VMX_MAX_BLOBS], vx[VMX_MAX_BLOBS], vy[VMX_MAX_BLOBS];
-    T ax[VMX_MAX_BLOBS], ay[VMX_MAX_BLOBS], w[VMX_MAX_BLOBS];
     T snx, sny, snt, sd, sh;
     T wa, wfx, wfy;
 
@@ -145,13 +146,8 @@ struct Blobs {
         q.nblobs = p.nblobs;
         q.step_mode = p.step_mode;
         for (int k = 0; k < VMX_MAX_BLOBS; ++k) {
-            q.cx[k] = T(p.blob[k][0]);
-            q.cy[k] = T(p.blob[k][1]);
-            q.vx[k] = T(p.blob[k][2]);
-            q.vy[k] = T(p.blob[k][3]);
-            q.ax[k] = T(p.blob[k][4]);
-            q.ay[k] = T(p.blob[k][5]);
-            q.w[k] = T(p.blob[k][6]);
+            for (int i = 0; i < 7; ++i) q.blob[k][i] = T(p.blob[k][i]);
+            q.blob[k][7] = T(0);
         }
         q.snx = T(p.step[0]);
         q.sny = T(p.step[1]);
@@ -170,9 +166,10 @@ struct Blobs {
         const T t = (D == 3) ? x[D == 3 ? 2 : 0] : T(0);
         T s = T(0);
         for (int k = 0; k < nblobs; ++k) {
-            const T dx = x[0] - (cx[k] + vx[k] * t);
-            const T dy = x[1] - (cy[k] + vy[k] * t);
-            s = s + w[k] * M::exp(-(ax[k] * (dx * dx) + ay[k] * (dy * dy)));
+            const T* b = blob[k];
+            const T dx = x[0] - (b[0] + b[2] * t);
+            const T dy = x[1] - (b[1] + b[3] * t);
+            s = s + b[6] * M::exp(-(b[4] * (dx * dx) + b[5] * (dy * dy)));
         }
         if (wa != T(0)) s = s + wa * (M::sin(wfx * x[0]) * M::cos(wfy * x[1]));
         if (step_mode != 0) {
