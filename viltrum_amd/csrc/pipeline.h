@@ -951,7 +951,7 @@ constexpr int LANE_MAXF = 16;  // footprints per bin cached in LDS (longer lists
    Kahan-compensated in the evaluation type and finished in FP64.  Summation depends on
    the job alone, so a batched integral gives exactly what it gives alone. */
 template <typename T, int D, int DB, typename F, typename M>
-__global__ void __launch_bounds__(256) k_bins1(const Job<T>* __restrict__ jobs, uint32_t J, uint64_t total_bins,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D <= 3 && sizeof(T) == 4 ? 4 : 1))) k_bins1(const Job<T>* __restrict__ jobs, uint32_t J, uint64_t total_bins,
                                                RegionArrays<T> r, FootArrays<T> fa,
                                                const uint32_t* __restrict__ off, const uint32_t* __restrict__ lst,
                                                const F* __restrict__ fs, JobRt* rt) {
