@@ -53,6 +53,8 @@ struct vmx_ctx {
     bool nn_ok[VMX_MAX_DIM + 1] = {};
     Buf tr_x, tr_t, tr_idx, tr_part, tr_loss, tr_state;  // training work space
     hipEvent_t ev[8] = {};
+    void* pinned = nullptr;  // host staging for small device-to-host readbacks
+    size_t pinned_bytes = 0;
     // last result (for vmx_export_regions): gathered from device memory on demand
     int last_D = 0;
     uint64_t last_n = 0;
@@ -83,6 +85,18 @@ void* ensure(vmx_ctx::Buf& b, size_t bytes) {
         b.bytes = bytes;
     }
     return b.p;
+}
+
+void* ensure_pinned(vmx_ctx* ctx, size_t bytes) {
+    if (ctx->pinned_bytes < bytes) {
+        if (ctx->pinned) HIP_CHECK(hipHostFree(ctx->pinned));
+        ctx->pinned = nullptr;
+        ctx->pinned_bytes = 0;
+        const size_t want = std::max<size_t>(bytes, 1 << 16);
+        HIP_CHECK(hipHostMalloc(&ctx->pinned, want, hipHostMallocDefault));
+        ctx->pinned_bytes = want;
+    }
+    return ctx->pinned;
 }
 
 /* Carves aligned sub-arrays out of one allocation. */
@@ -245,7 +259,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     uint32_t gen_blocks = uint32_t(n_cu) * 2;
     if (const char* e = std::getenv("VMX_GEN_BLOCKS")) gen_blocks = std::max(1, std::atoi(e));
     int round_chunk = 8;
-    if (const char* e = std::getenv("VMX_ROUND_CHUNK")) round_chunk = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("VMX_ROUND_CHUNK")) round_chunk = std::max(2, std::atoi(e) & ~1);
 
     uint32_t rounds = 0;
     uint64_t cap_mult = 8;
@@ -331,31 +345,53 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         g.rounds = d_rounds;
         g.J = J;
         g.net = learn ? static_cast<const float*>(ctx->nn[D].p) : nullptr;
-        std::vector<uint32_t> ncand_h(2 * J);
-        int par = 0;
-        for (int launched = 0;;) {
-            for (int k = 0; k < round_chunk; ++k, ++launched, par ^= 1) {
-                vmx::k_theta_round<T><<<J, 256, 0, st>>>(g, par);
-                launch_expand<T, D, P, Math>(learn, g, d_f, par, gen_blocks, st);
+        // rounds in chunks of round_chunk (even, so each chunk starts at parity 0); after
+        // each chunk the candidate counts, job states and round counter are copied to
+        // pinned memory.  The host checks chunk k while chunk k+1 runs, so the device is
+        // never idle waiting for the host; the chunk after the last busy one finds no
+        // candidates and its kernels exit at once.
+        const size_t slot_bytes = ((sizeof(uint32_t) * 2 * J + sizeof(vmx::JobRt) * J + 16) + 255) & ~size_t(255);
+        char* pin = static_cast<char*>(ensure_pinned(ctx, 2 * slot_bytes));
+        auto slot = [&](int k) { return pin + size_t(k & 1) * slot_bytes; };
+        auto launch_chunk = [&](int k) {
+            for (int r = 0; r < round_chunk; ++r) {
+                vmx::k_theta_round<T><<<J, 256, 0, st>>>(g, r & 1);
+                launch_expand<T, D, P, Math>(learn, g, d_f, r & 1, gen_blocks, st);
             }
             HIP_CHECK(hipGetLastError());
-            HIP_CHECK(hipMemcpyAsync(ncand_h.data(), d_ncand, sizeof(uint32_t) * 2 * J, hipMemcpyDeviceToHost, st));
-            HIP_CHECK(hipStreamSynchronize(st));
+            char* sl = slot(k);
+            HIP_CHECK(hipMemcpyAsync(sl, d_ncand, sizeof(uint32_t) * 2 * J, hipMemcpyDeviceToHost, st));
+            HIP_CHECK(hipMemcpyAsync(sl + sizeof(uint32_t) * 2 * J, d_rt, sizeof(vmx::JobRt) * J,
+                                     hipMemcpyDeviceToHost, st));
+            HIP_CHECK(hipMemcpyAsync(sl + sizeof(uint32_t) * 2 * J + sizeof(vmx::JobRt) * J, small, 16,
+                                     hipMemcpyDeviceToHost, st));
+            HIP_CHECK(hipEventRecord(ctx->ev[2 + (k & 1)], st));
+        };
+        launch_chunk(0);
+        int k = 0;
+        for (;; ++k) {
+            if ((k + 2) * round_chunk > 4096) {
+                HIP_CHECK(hipStreamSynchronize(st));
+                throw VmxError(VMX_ERR_CAPACITY, "more than 4096 refinement rounds");
+            }
+            launch_chunk(k + 1);
+            HIP_CHECK(hipEventSynchronize(ctx->ev[2 + (k & 1)]));
+            const uint32_t* nc = reinterpret_cast<const uint32_t*>(slot(k));
             bool pending = false;
-            for (uint32_t j = 0; j < J; ++j) pending |= ncand_h[par * J + j] != 0;
+            for (uint32_t j = 0; j < J; ++j) pending |= nc[j] != 0;
             if (!pending) break;
-            if (launched >= 4096) throw VmxError(VMX_ERR_CAPACITY, "more than 4096 refinement rounds");
         }
+        std::memcpy(rt.data(), slot(k) + sizeof(uint32_t) * 2 * J, sizeof(vmx::JobRt) * J);
         uint32_t small_h[4];
-        HIP_CHECK(hipMemcpyAsync(rt.data(), d_rt, sizeof(vmx::JobRt) * J, hipMemcpyDeviceToHost, st));
-        HIP_CHECK(hipMemcpyAsync(small_h, small, sizeof(small_h), hipMemcpyDeviceToHost, st));
-        HIP_CHECK(hipStreamSynchronize(st));
+        std::memcpy(small_h, slot(k) + sizeof(uint32_t) * 2 * J + sizeof(vmx::JobRt) * J, 16);
         rounds = small_h[0];
-        bool cap_fail = false;
+        bool cap_fail = false, too_deep = false;
         for (const auto& r : rt) {
-            if (r.flags & vmx::JF_DEPTH) throw VmxError(VMX_ERR_CAPACITY, "region tree deeper than 31 splits in one dim");
-            if (r.flags & vmx::JF_CAPACITY) cap_fail = true;
+            too_deep |= (r.flags & vmx::JF_DEPTH) != 0;
+            cap_fail |= (r.flags & vmx::JF_CAPACITY) != 0;
         }
+        if (too_deep || cap_fail) HIP_CHECK(hipStreamSynchronize(st));  // before buffers change hands
+        if (too_deep) throw VmxError(VMX_ERR_CAPACITY, "region tree deeper than 31 splits in one dim");
         if (cap_fail) {
             if (attempt >= 3) throw VmxError(VMX_ERR_CAPACITY, "node capacity exceeded");
             cap_mult *= 2;
@@ -439,12 +475,8 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         }
         if (vin != d_leaf)
             HIP_CHECK(hipMemcpyAsync(d_leaf, vin, sizeof(int32_t) * n_reg, hipMemcpyDeviceToDevice, st));
-        // every job must have exactly N leaves before any kernel indexes by them
-        HIP_CHECK(hipMemcpyAsync(rt.data(), d_rt, sizeof(vmx::JobRt) * J, hipMemcpyDeviceToHost, st));
-        HIP_CHECK(hipStreamSynchronize(st));
-        for (uint32_t j = 0; j < J; ++j)
-            if ((rt[j].flags & vmx::JF_LEAVES) || rt[j].nleaves != jobs[j].N)
-                throw VmxError(VMX_ERR_CAPACITY, "region selection did not close");
+        // a job with fewer than N leaves leaves -1 slots behind: k_prepare2 turns them into
+        // empty regions and the host rejects the result at its next synchronisation
     } else {
         // the reference's heap-array order: replay push_heap / pop_heap on the errors
         std::vector<int32_t> all_leaves(n_reg);
@@ -550,10 +582,20 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     HIP_CHECK(hipError_t(rocprim::exclusive_scan(d_scan_tmp, bytes, fa.npairs, d_pofs, 0u, size_t(n_reg + 1),
                                                 rocprim::plus<uint32_t>(), st)));
     uint32_t pairs = 0, nfp = 0;
-    HIP_CHECK(hipMemcpyAsync(&pairs, d_pofs + n_reg, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    HIP_CHECK(hipMemcpyAsync(&nfp, d_nfp, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    HIP_CHECK(hipMemcpyAsync(rt.data(), d_rt, sizeof(vmx::JobRt) * J, hipMemcpyDeviceToHost, st));
-    HIP_CHECK(hipStreamSynchronize(st));
+    {
+        char* pin = static_cast<char*>(ensure_pinned(ctx, 16 + sizeof(vmx::JobRt) * J));
+        HIP_CHECK(hipMemcpyAsync(pin, d_pofs + n_reg, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(pin + 4, d_nfp, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(pin + 16, d_rt, sizeof(vmx::JobRt) * J, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        std::memcpy(&pairs, pin, 4);
+        std::memcpy(&nfp, pin + 4, 4);
+        std::memcpy(rt.data(), pin + 16, sizeof(vmx::JobRt) * J);
+    }
+    if (order == VMX_ORDER_CANONICAL)
+        for (uint32_t j = 0; j < J; ++j)
+            if ((rt[j].flags & vmx::JF_LEAVES) || rt[j].nleaves != jobs[j].N)
+                throw VmxError(VMX_ERR_CAPACITY, "region selection did not close");
     unsigned key_bits = 1;
     while (key_bits < 32 && (uint64_t(1) << key_bits) < total_bins) ++key_bits;
     size_t sort_pairs_bytes = 0;
@@ -600,8 +642,12 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     }
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipEventRecord(ctx->ev[1], st));
-    HIP_CHECK(hipMemcpyAsync(rt.data(), d_rt, sizeof(vmx::JobRt) * J, hipMemcpyDeviceToHost, st));
-    HIP_CHECK(hipStreamSynchronize(st));
+    {
+        char* pin = static_cast<char*>(ensure_pinned(ctx, sizeof(vmx::JobRt) * J));
+        HIP_CHECK(hipMemcpyAsync(pin, d_rt, sizeof(vmx::JobRt) * J, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        std::memcpy(rt.data(), pin, sizeof(vmx::JobRt) * J);
+    }
     float ms_k = 0.f;
     HIP_CHECK(hipEventElapsedTime(&ms_k, ctx->ev[0], ctx->ev[1]));
     for (uint32_t j = 0; j < J; ++j)
@@ -1199,6 +1245,7 @@ void vmx_ctx_destroy(vmx_ctx* ctx) {
         if (b->p) (void)hipFree(b->p);
     for (vmx_ctx::Buf& b : ctx->nn)
         if (b.p) (void)hipFree(b.p);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     for (hipEvent_t e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

@@ -810,6 +810,16 @@ __global__ void k_prepare2(const Job<T>* __restrict__ jobs, const int32_t* __res
     constexpr int K = pow3(D);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (leaf_slot[i] < 0) {  // selection did not close (the host rejects the result): an empty region
+        for (int d = 0; d < D; ++d) r.lo[size_t(i) * D + d] = r.hi[size_t(i) * D + d] = T(0);
+        r.vol[i] = T(0);
+        for (int k = 0; k < mb; ++k) r.marg[size_t(i) * mb + k] = 0.0;
+        for (int k = 0; k < K; ++k) r.coef[size_t(i) * K + k] = T(0);
+        r.job[i] = 0;
+        r.fpflag[i] = 1;
+        if (i == 0) r.fpflag[n] = 0;
+        return;
+    }
     const size_t s = size_t(leaf_slot[i]);
     const uint32_t j = t.job[s];
     const Job<T>& jb = jobs[j];
@@ -830,7 +840,7 @@ __global__ void k_prepare2(const Job<T>* __restrict__ jobs, const int32_t* __res
     for (int k = 0; k < K; ++k) r.coef[size_t(i) * K + k] = c[k];
     r.job[i] = j;
     uint32_t flag = 1;
-    if (merge && i != jb.reg_base) {
+    if (merge && i != jb.reg_base && leaf_slot[i - 1] >= 0) {
         const size_t sp = size_t(leaf_slot[i - 1]);
         flag = 0;
         for (int d = 0; d < jb.db; ++d)
