@@ -68,3 +68,20 @@ def test_batch_of_one_hundred_small_integrals(ctx):
         assert np.array_equal(bufs[k].to_host(np.zeros(res, dtype=np.float32)), b)
     for b in bufs:
         b.free()
+
+
+def test_repeated_calls_replay_the_same_rounds(ctx):
+    """The refinement rounds run as a captured graph that later calls with the same
+    buffers replay: repeated calls, with other work in between, give identical bins."""
+    f = I.moving_blobs(21, 10)
+    integ = V.integrator_crespo2021(1500, 8, seed=4)
+    first = np.zeros((40, 40), dtype=np.float32)
+    V.integrate(integ, first, f, V.range_primary(3), ctx=ctx)
+    for k in range(4):
+        if k == 2:  # a different problem in between (other buffers, a new capture)
+            other = np.zeros((8, 8), dtype=np.float32)
+            V.integrate(V.integrator_crespo2021(300, 4, seed=1), other, I.c1_family(5, 3), V.range_primary(2),
+                        ctx=ctx)
+        again = np.zeros((40, 40), dtype=np.float32)
+        V.integrate(integ, again, f, V.range_primary(3), ctx=ctx)
+        assert np.array_equal(again, first), k

@@ -55,6 +55,13 @@ struct vmx_ctx {
     hipEvent_t ev[8] = {};
     void* pinned = nullptr;  // host staging for small device-to-host readbacks
     size_t pinned_bytes = 0;
+    // one chunk of refinement rounds captured as a graph, reused while its launch
+    // arguments (buffers, shapes, stream) stay the same
+    struct RoundGraph {
+        std::vector<unsigned char> key;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    } rounds;
     // last result (for vmx_export_regions): gathered from device memory on demand
     int last_D = 0;
     uint64_t last_n = 0;
@@ -353,12 +360,39 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         const size_t slot_bytes = ((sizeof(uint32_t) * 2 * J + sizeof(vmx::JobRt) * J + 16) + 255) & ~size_t(255);
         char* pin = static_cast<char*>(ensure_pinned(ctx, 2 * slot_bytes));
         auto slot = [&](int k) { return pin + size_t(k & 1) * slot_bytes; };
-        auto launch_chunk = [&](int k) {
-            for (int r = 0; r < round_chunk; ++r) {
-                vmx::k_theta_round<T><<<J, 256, 0, st>>>(g, r & 1);
-                launch_expand<T, D, P, Math>(learn, g, d_f, r & 1, gen_blocks, st);
+        // the chunk's 2 * round_chunk kernels as a graph: one launch instead of many
+        // small ones (the late rounds are launch-bound); recaptured only when an argument
+        // changes
+        {
+            std::vector<unsigned char> key(sizeof(g) + 64);
+            std::memcpy(key.data(), &g, sizeof(g));
+            const void* fn = reinterpret_cast<const void*>(&launch_expand<T, D, P, Math>);
+            const void* parts[4] = {fn, d_f, st, reinterpret_cast<const void*>(uintptr_t(learn))};
+            std::memcpy(key.data() + sizeof(g), parts, sizeof(parts));
+            const uint32_t shape[2] = {gen_blocks, uint32_t(round_chunk)};
+            std::memcpy(key.data() + sizeof(g) + sizeof(parts), shape, sizeof(shape));
+            if (ctx->rounds.key != key || !ctx->rounds.exec) {
+                if (ctx->rounds.exec) HIP_CHECK(hipGraphExecDestroy(ctx->rounds.exec));
+                if (ctx->rounds.graph) HIP_CHECK(hipGraphDestroy(ctx->rounds.graph));
+                ctx->rounds.exec = nullptr;
+                ctx->rounds.graph = nullptr;
+                ctx->rounds.key.clear();
+                HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+                for (int r = 0; r < round_chunk; ++r) {
+                    vmx::k_theta_round<T><<<J, 256, 0, st>>>(g, r & 1);
+                    launch_expand<T, D, P, Math>(learn, g, d_f, r & 1, gen_blocks, st);
+                }
+                hipGraph_t graph = nullptr;
+                const hipError_t e1 = hipStreamEndCapture(st, &graph);
+                HIP_CHECK(e1);
+                HIP_CHECK(hipGetLastError());
+                HIP_CHECK(hipGraphInstantiate(&ctx->rounds.exec, graph, nullptr, nullptr, 0));
+                ctx->rounds.graph = graph;
+                ctx->rounds.key = key;
             }
-            HIP_CHECK(hipGetLastError());
+        }
+        auto launch_chunk = [&](int k) {
+            HIP_CHECK(hipGraphLaunch(ctx->rounds.exec, st));
             char* sl = slot(k);
             HIP_CHECK(hipMemcpyAsync(sl, d_ncand, sizeof(uint32_t) * 2 * J, hipMemcpyDeviceToHost, st));
             HIP_CHECK(hipMemcpyAsync(sl + sizeof(uint32_t) * 2 * J, d_rt, sizeof(vmx::JobRt) * J,
@@ -1246,6 +1280,8 @@ void vmx_ctx_destroy(vmx_ctx* ctx) {
     for (vmx_ctx::Buf& b : ctx->nn)
         if (b.p) (void)hipFree(b.p);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->rounds.exec) (void)hipGraphExecDestroy(ctx->rounds.exec);
+    if (ctx->rounds.graph) (void)hipGraphDestroy(ctx->rounds.graph);
     for (hipEvent_t e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
