@@ -263,7 +263,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     // launch shapes: expansion grid and lanes per bin (VMX_* overrides for tuning)
     int dev = ctx->device, n_cu = 0;
     HIP_CHECK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
-    uint32_t gen_blocks = uint32_t(n_cu) * 2;
+    uint32_t gen_blocks = uint32_t(n_cu) * 4;
     if (const char* e = std::getenv("VMX_GEN_BLOCKS")) gen_blocks = std::max(1, std::atoi(e));
     int round_chunk = 8;
     if (const char* e = std::getenv("VMX_ROUND_CHUNK")) round_chunk = std::max(2, std::atoi(e) & ~1);
@@ -274,7 +274,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     vmx::Job<T>* d_jobs = nullptr;
     vmx::JobRt* d_rt = nullptr;
     P* d_f = nullptr;
-    uint32_t *d_hist16 = nullptr, *d_hist8 = nullptr, *d_ncand = nullptr, *d_rounds = nullptr;
+    uint32_t *d_hist16 = nullptr, *d_ncand = nullptr, *d_rounds = nullptr;
     T* d_theta = nullptr;
     uint32_t* d_cstart = nullptr;
     uint32_t cap_total = 0;
@@ -302,7 +302,6 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         m.take<uint32_t>(cap_total);
         m.take<uint8_t>(cap_total);
         m.take<uint32_t>(size_t(J) * 65536);
-        m.take<uint32_t>(size_t(J) * 256);
         m.take<uint32_t>(2 * J);
         m.take<uint32_t>(2 * J);
         m.take<T>(J);
@@ -324,7 +323,6 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         t.job = c.take<uint32_t>(cap_total);
         t.popped = c.take<uint8_t>(cap_total);
         d_hist16 = c.take<uint32_t>(size_t(J) * 65536);
-        d_hist8 = c.take<uint32_t>(size_t(J) * 256);
         d_cstart = c.take<uint32_t>(2 * J);
         d_ncand = c.take<uint32_t>(2 * J);
         d_theta = c.take<T>(J);
@@ -335,17 +333,15 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         HIP_CHECK(hipMemsetAsync(d_rt, 0, sizeof(vmx::JobRt) * J, st));
         HIP_CHECK(hipMemsetAsync(t.job, 0xFF, sizeof(uint32_t) * cap_total, st));
         HIP_CHECK(hipMemsetAsync(d_hist16, 0, sizeof(uint32_t) * J * 65536, st));
-        HIP_CHECK(hipMemsetAsync(d_hist8, 0, sizeof(uint32_t) * J * 256, st));
         HIP_CHECK(hipMemsetAsync(d_ncand, 0, sizeof(uint32_t) * 2 * J, st));
         HIP_CHECK(hipMemsetAsync(small, 0, sizeof(uint32_t) * 4, st));
-        vmx::k_roots<T, D, P, Math><<<J, 256, 0, st>>>(d_jobs, d_rt, d_f, t, d_hist16, d_hist8, d_cstart, d_ncand);
+        vmx::k_roots<T, D, P, Math><<<J, 256, 0, st>>>(d_jobs, d_rt, d_f, t, d_hist16, d_cstart, d_ncand);
         HIP_CHECK(hipGetLastError());
         vmx::GenArgs<T> g{};
         g.jobs = d_jobs;
         g.rt = d_rt;
         g.t = t;
         g.hist16 = d_hist16;
-        g.hist8 = d_hist8;
         g.cstart = d_cstart;
         g.ncand = d_ncand;
         g.theta = d_theta;
@@ -492,7 +488,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     const unsigned reg_blocks = unsigned((n_reg + 255) / 256);
     if (order == VMX_ORDER_CANONICAL) {
         HIP_CHECK(hipMemsetAsync(d_leaf, 0xFF, sizeof(int32_t) * n_reg, st));
-        vmx::k_sel_bound<T><<<J, 256, 0, st>>>(d_jobs, d_rt, d_hist16, d_hist8);
+        vmx::k_sel_bound<T><<<J, 256, 0, st>>>(d_jobs, d_rt, d_hist16);
         vmx::k_sel_gather<T><<<cap_blocks, 256, 0, st>>>(d_jobs, d_rt, t, cap_total, d_bucket);
         vmx::k_sel_resolve<T, D><<<J, 256, 0, st>>>(d_jobs, d_rt, t, d_bucket, d_bstate);
         vmx::k_leaves<T><<<cap_blocks, 256, 0, st>>>(d_jobs, d_rt, t, cap_total, d_leaf);

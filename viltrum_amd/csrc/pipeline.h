@@ -72,12 +72,22 @@ __device__ inline void suffix_sum256(uint32_t* s) {
 }
 
 /* Bucket (16-bit key of b) holding the need-th largest b of one job, and the number of
-   nodes in strictly higher buckets.  Requires total >= need >= 1.  256 threads. */
-__device__ inline void boundary_bucket(const uint32_t* h8, const uint32_t* h16, uint32_t need, uint32_t* sh,
-                                       uint32_t& k_out, uint32_t& above_out) {
+   nodes in strictly higher buckets.  Requires total >= need >= 1.  256 threads.  The
+   coarse counts (top 8 bits of the key) are the row sums of the fine histogram, formed
+   here rather than kept by atomics: nodes of a round cluster in few coarse buckets. */
+__device__ inline void boundary_bucket(const uint32_t* h16, uint32_t need, uint32_t* sh, uint32_t& k_out,
+                                       uint32_t& above_out) {
     const int t = threadIdx.x;
     __shared__ uint32_t sel_c, sel_above;
-    sh[t] = h8[t];
+    {
+        const uint4* row = reinterpret_cast<const uint4*>(h16 + size_t(t) * 256);
+        uint32_t acc = 0;
+        for (int i = 0; i < 64; ++i) {
+            const uint4 q = row[i];
+            acc += q.x + q.y + q.z + q.w;
+        }
+        sh[t] = acc;
+    }
     __syncthreads();
     suffix_sum256(sh);
     if (sh[t] >= need && (t == 255 || sh[t + 1] < need)) {
@@ -199,7 +209,7 @@ __device__ inline void team_heuristic(TeamLds<T, D>& L, int nc, int heur, int la
    One block per job: 256 lanes evaluate the 3^D nodes, one team scores the root. */
 template <typename T, int D, typename F, typename M>
 __global__ void __launch_bounds__(256) k_roots(const Job<T>* __restrict__ jobs, JobRt* rt, const F* __restrict__ fs,
-                                               TreeArrays<T> t, uint32_t* hist16, uint32_t* hist8, uint32_t* cstart0,
+                                               TreeArrays<T> t, uint32_t* hist16, uint32_t* cstart0,
                                                uint32_t* ncand0) {
     constexpr int K = pow3(D);
     constexpr int TEAM = TeamShape<D>::TEAM;
@@ -244,7 +254,6 @@ __global__ void __launch_bounds__(256) k_roots(const Job<T>* __restrict__ jobs, 
         rt[j].count = jb.node_base + 1;
         const uint32_t k = key16(pe);
         atomicAdd(&hist16[size_t(j) * 65536 + k], 1u);
-        atomicAdd(&hist8[size_t(j) * 256 + (k >> 8)], 1u);
         cstart0[j] = uint32_t(s);
         ncand0[j] = jb.N > 1 ? 1u : 0u;
     }
@@ -256,7 +265,6 @@ struct GenArgs {
     JobRt* rt;
     TreeArrays<T> t;
     uint32_t* hist16;
-    uint32_t* hist8;
     uint32_t* cstart;  // [2*J] first slot of the round's candidates (they are contiguous)
     uint32_t* ncand;   // [2*J] number of candidates
     T* theta;         // [J]
@@ -461,7 +469,6 @@ __device__ inline void team_expand(const GenArgs<T>& g, const F* __restrict__ fs
         t.job[s] = j;
         const uint32_t k = key16(nb);
         atomicAdd(&g.hist16[size_t(j) * 65536 + k], 1u);
-        atomicAdd(&g.hist8[size_t(j) * 256 + (k >> 8)], 1u);
         if (c == 0) t.child[node] = int32_t(s);
     }
     __syncthreads();
@@ -493,7 +500,7 @@ __global__ void __launch_bounds__(256) k_theta_round(GenArgs<T> g, int par) {
         th = T(-INFINITY);
     } else {
         uint32_t k, above;
-        boundary_bucket(g.hist8 + size_t(j) * 256, g.hist16 + size_t(j) * 65536, need, sh, k, above);
+        boundary_bucket(g.hist16 + size_t(j) * 65536, need, sh, k, above);
         key16_floor(k, th);
     }
     if (threadIdx.x == 0) g.theta[j] = th;
@@ -600,13 +607,12 @@ __global__ void k_train_pairs(const Job<T>* __restrict__ jobs, TreeArrays<T> t, 
 
 /* ------------------------------------------------------------------ selection */
 template <typename T>
-__global__ void k_sel_bound(const Job<T>* __restrict__ jobs, JobRt* rt, const uint32_t* hist16,
-                            const uint32_t* hist8) {
+__global__ void k_sel_bound(const Job<T>* __restrict__ jobs, JobRt* rt, const uint32_t* hist16) {
     __shared__ uint32_t sh[256];
     const uint32_t j = blockIdx.x;
     const uint32_t need = jobs[j].N - 1;
     uint32_t k = NONE, above = 0;
-    if (need > 0) boundary_bucket(hist8 + size_t(j) * 256, hist16 + size_t(j) * 65536, need, sh, k, above);
+    if (need > 0) boundary_bucket(hist16 + size_t(j) * 65536, need, sh, k, above);
     if (threadIdx.x == 0) {
         rt[j].kbound = k;
         rt[j].above = above;
