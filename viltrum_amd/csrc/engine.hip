@@ -50,6 +50,7 @@ struct vmx_ctx {
     };
     Buf tree, regs, lists, misc, params, bins, bins2;
     Buf nn[VMX_MAX_DIM + 1];                       // FP32 priority-network parameters per dim
+    Buf nn_lds[VMX_MAX_DIM + 1];                   // ... and their LDS image (nn::NetW<KP>)
     bool nn_ok[VMX_MAX_DIM + 1] = {};
     Buf tr_x, tr_t, tr_idx, tr_part, tr_loss, tr_state;  // training work space
     hipEvent_t ev[8] = {};
@@ -347,7 +348,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         g.theta = d_theta;
         g.rounds = d_rounds;
         g.J = J;
-        g.net = learn ? static_cast<const float*>(ctx->nn[D].p) : nullptr;
+        g.net = learn ? ctx->nn_lds[D].p : nullptr;
         // rounds in chunks of round_chunk (even, so each chunk starts at parity 0); after
         // each chunk the candidate counts, job states and round counter are copied to
         // pinned memory.  The host checks chunk k while chunk k+1 runs, so the device is
@@ -925,6 +926,18 @@ void with_kp(int kp, Fn&& fn) {
         throw VmxError(VMX_ERR_UNSUPPORTED, "learned refiner: unsupported feature width");
 }
 
+/* Builds ctx->nn_lds[dim] from the FP32 parameters in ctx->nn[dim]. */
+void pack_net(vmx_ctx* ctx, int dim) {
+    const int kp = kp_of(dim);
+    with_kp(kp, [&](auto K) {
+        constexpr int KP = decltype(K)::value;
+        auto* out = static_cast<vmx::nn::NetW<KP>*>(ensure(ctx->nn_lds[dim], sizeof(vmx::nn::NetW<KP>)));
+        vmx::nn::k_pack_net<KP><<<1, 256, 0, ctx->stream>>>(static_cast<const float*>(ctx->nn[dim].p), out);
+    });
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+}
+
 void nn_forward(int kp, const float* params, const uint16_t* X, uint64_t n, float* y, hipStream_t st) {
     with_kp(kp, [&](auto K) {
         constexpr int KP = decltype(K)::value;
@@ -1090,6 +1103,7 @@ void train_net(vmx_ctx* ctx, int dim, uint64_t count, const vmx_train_opts& o, v
     float* keep = static_cast<float*>(ensure(ctx->nn[dim], sizeof(float) * PT));
     HIP_CHECK(hipMemcpyAsync(keep, d_p, sizeof(float) * PT, hipMemcpyDeviceToDevice, st));
     HIP_CHECK(hipStreamSynchronize(st));
+    pack_net(ctx, dim);
     ctx->nn_ok[dim] = true;
     if (rep) {
         rep->steps = t;
@@ -1275,6 +1289,8 @@ void vmx_ctx_destroy(vmx_ctx* ctx) {
         if (b->p) (void)hipFree(b->p);
     for (vmx_ctx::Buf& b : ctx->nn)
         if (b.p) (void)hipFree(b.p);
+    for (vmx_ctx::Buf& b : ctx->nn_lds)
+        if (b.p) (void)hipFree(b.p);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->rounds.exec) (void)hipGraphExecDestroy(ctx->rounds.exec);
     if (ctx->rounds.graph) (void)hipGraphDestroy(ctx->rounds.graph);
@@ -1414,6 +1430,7 @@ int vmx_refiner_set(vmx_ctx* ctx, int32_t dim, const float* params) {
         if (!(params[PT - 1] > 0.f)) throw VmxError(VMX_ERR_INVALID, "target scale must be > 0");
         float* d = static_cast<float*>(ensure(ctx->nn[dim], sizeof(float) * PT));
         HIP_CHECK(hipMemcpy(d, params, sizeof(float) * PT, hipMemcpyHostToDevice));
+        pack_net(ctx, dim);
         ctx->nn_ok[dim] = true;
         return int(VMX_OK);
     });

@@ -271,7 +271,7 @@ struct GenArgs {
     uint32_t* rounds; // rounds that had candidates
     uint32_t J;
     uint32_t max_rounds;
-    const float* net;  // FP32 parameters of the priority network (learned jobs)
+    const void* net;   // the priority network's LDS image (nn::NetW<KP>, packed once per network)
 };
 
 /* Per-team slot requests of one expansion step, combined into one atomic per block and
@@ -519,19 +519,23 @@ __global__ void __launch_bounds__(GEN_THREADS) k_expand_round(GenArgs<T> g, cons
     const int team = threadIdx.x / TEAM;
     for (uint32_t j = threadIdx.x; j < J; j += blockDim.x) pre[j + 1] = g.ncand[par * J + j];
     if (threadIdx.x == 0) pre[0] = 0;
-    if constexpr (LEARN) {
-        ll = &learn_lds;
-        nn::load_net<LearnLds<D>::KP>(ll->net, g.net);
-        constexpr int KP = LearnLds<D>::KP;
-        for (int i = threadIdx.x + 2 * TPB * KP; i < LearnLds<D>::R * KP; i += blockDim.x)
-            ll->xs[i / KP][i % KP] = 0;  // padding rows stay zero
-    }
     __syncthreads();
     if (threadIdx.x == 0)
         for (uint32_t j = 1; j <= J; ++j) pre[j] += pre[j - 1];
     __syncthreads();
     const uint32_t total = pre[J];
     if (blockIdx.x == 0 && threadIdx.x == 0 && total > 0) atomicAdd(g.rounds, 1u);
+    if (uint64_t(blockIdx.x) * TPB >= total) return;  // no candidate for this block (uniform)
+    if constexpr (LEARN) {
+        ll = &learn_lds;
+        constexpr int KP = LearnLds<D>::KP;
+        const uint4* src = static_cast<const uint4*>(g.net);
+        uint4* dst = reinterpret_cast<uint4*>(&ll->net);
+        for (int i = threadIdx.x; i < int(sizeof(nn::NetW<KP>) / 16); i += blockDim.x) dst[i] = src[i];
+        for (int i = threadIdx.x + 2 * TPB * KP; i < LearnLds<D>::R * KP; i += blockDim.x)
+            ll->xs[i / KP][i % KP] = 0;  // padding rows stay zero
+        __syncthreads();
+    }
     for (uint64_t base = uint64_t(blockIdx.x) * TPB; base < total; base += uint64_t(gridDim.x) * TPB) {
         const uint64_t w = base + team;
         team_expand<T, D, F, M, LEARN>(g, fs, lds[team], par, pre, w, w < total, ll, team, slots);

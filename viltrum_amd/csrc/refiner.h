@@ -68,7 +68,7 @@ __device__ inline f32x4_t mm_tile_tn(const uint16_t* a, int lda, const uint16_t*
 
 /* Forward weights in LDS. */
 template <int KP>
-struct alignas(16) NetW {
+struct alignas(16) NetW {  // size a multiple of 16 bytes (copied as 16-byte words)
     uint16_t w0t[W][KP + 8];
     uint16_t w1t[W][WS];
     uint16_t w2t[W][WS];
@@ -100,6 +100,13 @@ __device__ inline void load_net(NetW<KP>& n, const float* __restrict__ p) {
         n.mu = p[L::mu];
         n.sigma = p[L::sigma];
     }
+}
+
+/* The LDS image of a network (what load_net builds), written once to device memory so
+   kernels that only run the forward pass copy it with 16-byte loads. */
+template <int KP>
+__global__ void k_pack_net(const float* __restrict__ p, NetW<KP>* out) {
+    load_net<KP>(*out, p);
 }
 
 template <int KP>
