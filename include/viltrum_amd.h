@@ -186,8 +186,9 @@ int vmx_refiner_size(int32_t dim, uint64_t* n_params, int32_t* n_features, int32
    family and dim (every stride-th node of the refinement rounds): features as BF16 bit
    patterns (count x KP) and targets log2(E / (s^2 vol)), E the FP64 target of
    opts->target (bins of each problem's resolution), s = max |node value|.  Only
-   opts->target and opts->stride are read (NULL: defaults).  NULL features/targets: only
-   *count is set. */
+   opts->target and opts->stride are read (NULL: defaults).  *count is set to the number
+   of pairs and min(count, cap) rows are written (NULL features/targets: none).  Rows
+   come in the order the device claimed node slots, which varies between calls. */
 int vmx_refiner_pairs(vmx_ctx* ctx, uint32_t n, const vmx_problem* problems, const vmx_integrand* integrands,
                       const vmx_train_opts* opts, uint64_t cap, uint16_t* features, float* targets,
                       uint64_t* count);

@@ -218,12 +218,14 @@ class Context:
         self._check(L.vmx_refiner_pairs(self.ptr, n, C.cast(P, C.c_void_p), C.cast(F, C.c_void_p), C.byref(o), 0,
                                         None, None, C.byref(cnt)))
         kp = refiner_size(fs[0].dim)[2]
-        X = np.zeros((cnt.value, kp), dtype=np.uint16)
-        t = np.zeros(cnt.value, dtype=np.float32)
+        cap = cnt.value + 4096
+        X = np.zeros((cap, kp), dtype=np.uint16)
+        t = np.zeros(cap, dtype=np.float32)
         self._check(L.vmx_refiner_pairs(self.ptr, n, C.cast(P, C.c_void_p), C.cast(F, C.c_void_p), C.byref(o),
-                                        cnt.value, X.ctypes.data_as(C.c_void_p), t.ctypes.data_as(C.c_void_p),
+                                        cap, X.ctypes.data_as(C.c_void_p), t.ctypes.data_as(C.c_void_p),
                                         C.byref(cnt)))
-        return X[:cnt.value], t[:cnt.value]
+        m = min(cnt.value, cap)
+        return X[:m], t[:m]
 
     def refiner_train(self, problems, fs, epochs=8, batch=1024, lr=2e-3, holdout=0.05, seed=0,
                       target=TARGET_ENERGY, stride=1):

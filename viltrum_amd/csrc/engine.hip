@@ -1382,11 +1382,11 @@ int vmx_refiner_pairs(vmx_ctx* ctx, uint32_t n, const vmx_problem* problems, con
         const uint64_t rows = make_pairs(ctx, n, problems, integrands, fill, o);
         *count = rows;
         if (!fill) return int(VMX_OK);
-        if (rows > cap) throw VmxError(VMX_ERR_CAPACITY, "more pairs than cap");
+        const uint64_t out = rows < cap ? rows : cap;
         const int kp = kp_of(integrands[0].dim);
-        HIP_CHECK(hipMemcpyAsync(features, ctx->tr_x.p, sizeof(uint16_t) * rows * kp, hipMemcpyDeviceToHost,
+        HIP_CHECK(hipMemcpyAsync(features, ctx->tr_x.p, sizeof(uint16_t) * out * kp, hipMemcpyDeviceToHost,
                                  ctx->stream));
-        HIP_CHECK(hipMemcpyAsync(targets, ctx->tr_t.p, sizeof(float) * rows, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_CHECK(hipMemcpyAsync(targets, ctx->tr_t.p, sizeof(float) * out, hipMemcpyDeviceToHost, ctx->stream));
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
         return int(VMX_OK);
     });
