@@ -359,8 +359,10 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         auto slot = [&](int k) { return pin + size_t(k & 1) * slot_bytes; };
         // the chunk's 2 * round_chunk kernels as a graph: one launch instead of many
         // small ones (the late rounds are launch-bound); recaptured only when an argument
-        // changes
-        {
+        // changes.  VMX_NO_GRAPH=1 launches them one by one (e.g. under a kernel tracer
+        // that does not follow graph launches).
+        static const bool use_graph = !(std::getenv("VMX_NO_GRAPH") && std::atoi(std::getenv("VMX_NO_GRAPH")));
+        if (use_graph) {
             std::vector<unsigned char> key(sizeof(g) + 64);
             std::memcpy(key.data(), &g, sizeof(g));
             const void* fn = reinterpret_cast<const void*>(&launch_expand<T, D, P, Math>);
@@ -389,7 +391,15 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
             }
         }
         auto launch_chunk = [&](int k) {
-            HIP_CHECK(hipGraphLaunch(ctx->rounds.exec, st));
+            if (use_graph) {
+                HIP_CHECK(hipGraphLaunch(ctx->rounds.exec, st));
+            } else {
+                for (int r = 0; r < round_chunk; ++r) {
+                    vmx::k_theta_round<T><<<J, 256, 0, st>>>(g, r & 1);
+                    launch_expand<T, D, P, Math>(learn, g, d_f, r & 1, gen_blocks, st);
+                }
+                HIP_CHECK(hipGetLastError());
+            }
             char* sl = slot(k);
             HIP_CHECK(hipMemcpyAsync(sl, d_ncand, sizeof(uint32_t) * 2 * J, hipMemcpyDeviceToHost, st));
             HIP_CHECK(hipMemcpyAsync(sl + sizeof(uint32_t) * 2 * J, d_rt, sizeof(vmx::JobRt) * J,
@@ -445,6 +455,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         pairs->count = total;
         uint16_t* X = nullptr;
         float* tg = nullptr;
+        HIP_CHECK(hipStreamSynchronize(st));  // the last (idle) round chunk is done
         if (pairs->alloc) pairs->alloc(total, &X, &tg);
         if (X && tg && total > 0) {
             using F64 = typename F64Of<P>::type;
