@@ -160,20 +160,29 @@ struct Blobs {
         return q;
     }
 
+    /* Sums in blob order.  M::fma(a, b, c) is a*b + c under the policy's rounding; with
+       the unfused policies every value is the plain expression
+       s + w*exp(-(ax*(dx*dx) + ay*(dy*dy))), dx = x - (cx + vx*t). */
     template <class M, int D>
     VMX_HD T eval(const T* x) const {
         static_assert(D == 2 || D == 3, "blob family is 2-D or 3-D");
-        const T t = (D == 3) ? x[D == 3 ? 2 : 0] : T(0);
         T s = T(0);
         for (int k = 0; k < nblobs; ++k) {
             const T* b = blob[k];
-            const T dx = x[0] - (b[0] + b[2] * t);
-            const T dy = x[1] - (b[1] + b[3] * t);
-            s = s + b[6] * M::exp(-(b[4] * (dx * dx) + b[5] * (dy * dy)));
+            T dx, dy;
+            if constexpr (D == 3) {
+                dx = x[0] - M::fma(b[2], x[2], b[0]);
+                dy = x[1] - M::fma(b[3], x[2], b[1]);
+            } else {  // t = 0: the centre is (cx, cy)
+                dx = x[0] - b[0];
+                dy = x[1] - b[1];
+            }
+            s = M::fma(b[6], M::exp(-M::fma(b[5], dy * dy, b[4] * (dx * dx))), s);
         }
         if (wa != T(0)) s = s + wa * (M::sin(wfx * x[0]) * M::cos(wfy * x[1]));
         if (step_mode != 0) {
-            const T lin = (snx * x[0] + sny * x[1]) + snt * t;
+            T lin = snx * x[0] + sny * x[1];
+            if constexpr (D == 3) lin = M::fma(snt, x[2], lin);
             const bool in = lin > sd;
             if (step_mode == 1) {
                 if (in) s = s + sh;
@@ -213,18 +222,17 @@ struct Lobes5 {
         static_assert(D == 5, "lobes5 family is 5-D");
         for (int s = 0; s < 2; ++s) {
             T lin = T(0);
-            for (int j = 0; j < 5; ++j) lin = lin + n[s][j] * x[j];
+            for (int j = 0; j < 5; ++j) lin = M::fma(n[s][j], x[j], lin);
             if (!(lin > d[s])) return T(0);
         }
         T q = T(0);
         for (int j = 0; j < 3; ++j) {
             const T u = x[2 + j] - c[j];
-            q = q + a[j] * (u * u);
+            q = M::fma(a[j], u * u, q);
         }
         T ph = T(0);
-        for (int j = 0; j < 5; ++j) ph = ph + k[j] * x[j];
-        const T two_pi = T(6.283185307179586476925286766559);
-        return M::exp(-q) * (T(1) + amp * M::cos(two_pi * ph));
+        for (int j = 0; j < 5; ++j) ph = M::fma(k[j], x[j], ph);
+        return M::exp(-q) * (T(1) + amp * M::cos2pi(ph));
     }
 };
 

@@ -52,6 +52,10 @@ struct HostMath {
     static double sin(double x) { return std::sin(x); }
     static float cos(float x) { return std::cos(x); }
     static double cos(double x) { return std::cos(x); }
+    template <typename T>
+    static T fma(T a, T b, T c) { return a * b + c; }  // unfused, as an x86-64 build of a user lambda
+    template <typename T>
+    static T cos2pi(T p) { return std::cos(T(6.283185307179586476925286766559) * p); }
 };
 
 constexpr int pow3(int d) { return d <= 0 ? 1 : 3 * pow3(d - 1); }

@@ -322,6 +322,8 @@ int vmo_eval_f64(const vmx_integrand* f, const double* x, double* y) {
         static double exp(double a) { return std::exp(a); }
         static double sin(double a) { return std::sin(a); }
         static double cos(double a) { return std::cos(a); }
+        static double fma(double a, double b, double c) { return a * b + c; }
+        static double cos2pi(double p) { return std::cos(6.283185307179586476925286766559 * p); }
     };
     const int D = f->dim;
     switch (f->family) {
@@ -362,6 +364,8 @@ double vmo_true_error(const vmx_integrand* f, const double* lo, const double* hi
             static double exp(double a) { return std::exp(a); }
             static double sin(double a) { return std::sin(a); }
             static double cos(double a) { return std::cos(a); }
+            static double fma(double a, double b, double c) { return a * b + c; }
+            static double cos2pi(double p) { return std::cos(6.283185307179586476925286766559 * p); }
         };
         switch (f->family) {
             case VMX_FAMILY_POLY: {

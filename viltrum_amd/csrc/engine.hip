@@ -108,13 +108,16 @@ void* ensure_pinned(vmx_ctx* ctx, size_t bytes) {
 }
 
 /* Carves aligned sub-arrays out of one allocation. */
+/* Sub-allocations of one buffer, 256-byte aligned; taking more than the buffer holds
+   is a bug in the caller's size computation and throws. */
 struct Carve {
     char* base;
-    size_t used = 0;
-    explicit Carve(void* p) : base(static_cast<char*>(p)) {}
+    size_t used = 0, cap;
+    explicit Carve(const vmx_ctx::Buf& b) : base(static_cast<char*>(b.p)), cap(b.bytes) {}
     template <typename X>
     X* take(size_t n) {
         used = (used + 255) & ~size_t(255);
+        if (used + n * sizeof(X) > cap) throw VmxError(VMX_ERR_CAPACITY, "work buffer carved past its size");
         X* r = reinterpret_cast<X*>(base + used);
         used += n * sizeof(X);
         return r;
@@ -183,13 +186,14 @@ constexpr int LEARN_MAX_DIM = 4;  // the fused network fits the expansion kernel
 
 template <typename T, int D, typename P, typename Math>
 void launch_expand(bool learn, const vmx::GenArgs<T>& g, const P* d_f, int par, uint32_t blocks, hipStream_t st) {
+    const size_t pre_bytes = size_t(g.J + 1) * sizeof(uint32_t);  // the per-job candidate prefix
     if constexpr (std::is_same<T, float>::value && D <= LEARN_MAX_DIM) {
         if (learn) {
-            vmx::k_expand_round<T, D, P, Math, true><<<blocks, vmx::GEN_THREADS, 0, st>>>(g, d_f, par);
+            vmx::k_expand_round<T, D, P, Math, true><<<blocks, vmx::GEN_THREADS, pre_bytes, st>>>(g, d_f, par);
             return;
         }
     }
-    vmx::k_expand_round<T, D, P, Math, false><<<blocks, vmx::GEN_THREADS, 0, st>>>(g, d_f, par);
+    vmx::k_expand_round<T, D, P, Math, false><<<blocks, vmx::GEN_THREADS, pre_bytes, st>>>(g, d_f, par);
 }
 
 template <typename T>
@@ -270,6 +274,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     if (const char* e = std::getenv("VMX_ROUND_CHUNK")) round_chunk = std::max(2, std::atoi(e) & ~1);
 
     uint32_t rounds = 0;
+    uint32_t max_depth[VMX_MAX_DIM] = {};
     uint64_t cap_mult = 8;
     vmx::TreeArrays<T> t{};
     vmx::Job<T>* d_jobs = nullptr;
@@ -306,8 +311,9 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         m.take<uint32_t>(2 * J);
         m.take<uint32_t>(2 * J);
         m.take<T>(J);
-        m.take<uint32_t>(4);
-        Carve c(ensure(ctx->tree, m.used));
+        m.take<uint32_t>(16);
+        ensure(ctx->tree, m.used);
+        Carve c(ctx->tree);
         d_jobs = c.take<vmx::Job<T>>(J);
         d_rt = c.take<vmx::JobRt>(J);
         d_f = c.take<P>(J);
@@ -327,7 +333,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         d_cstart = c.take<uint32_t>(2 * J);
         d_ncand = c.take<uint32_t>(2 * J);
         d_theta = c.take<T>(J);
-        uint32_t* small = c.take<uint32_t>(4);
+        uint32_t* small = c.take<uint32_t>(16);  // [0] rounds, [8 + d] deepest split count along dim d
         d_rounds = small;
         HIP_CHECK(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(vmx::Job<T>) * J, hipMemcpyHostToDevice, st));
         HIP_CHECK(hipMemcpyAsync(d_f, fparams.data(), sizeof(P) * J, hipMemcpyHostToDevice, st));
@@ -335,7 +341,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         HIP_CHECK(hipMemsetAsync(t.job, 0xFF, sizeof(uint32_t) * cap_total, st));
         HIP_CHECK(hipMemsetAsync(d_hist16, 0, sizeof(uint32_t) * J * 65536, st));
         HIP_CHECK(hipMemsetAsync(d_ncand, 0, sizeof(uint32_t) * 2 * J, st));
-        HIP_CHECK(hipMemsetAsync(small, 0, sizeof(uint32_t) * 4, st));
+        HIP_CHECK(hipMemsetAsync(small, 0, sizeof(uint32_t) * 16, st));
         vmx::k_roots<T, D, P, Math><<<J, 256, 0, st>>>(d_jobs, d_rt, d_f, t, d_hist16, d_cstart, d_ncand);
         HIP_CHECK(hipGetLastError());
         vmx::GenArgs<T> g{};
@@ -347,6 +353,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         g.ncand = d_ncand;
         g.theta = d_theta;
         g.rounds = d_rounds;
+        g.depth = small + 8;
         g.J = J;
         g.net = learn ? ctx->nn_lds[D].p : nullptr;
         // rounds in chunks of round_chunk (even, so each chunk starts at parity 0); after
@@ -354,7 +361,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         // pinned memory.  The host checks chunk k while chunk k+1 runs, so the device is
         // never idle waiting for the host; the chunk after the last busy one finds no
         // candidates and its kernels exit at once.
-        const size_t slot_bytes = ((sizeof(uint32_t) * 2 * J + sizeof(vmx::JobRt) * J + 16) + 255) & ~size_t(255);
+        const size_t slot_bytes = ((sizeof(uint32_t) * 2 * J + sizeof(vmx::JobRt) * J + 64) + 255) & ~size_t(255);
         char* pin = static_cast<char*>(ensure_pinned(ctx, 2 * slot_bytes));
         auto slot = [&](int k) { return pin + size_t(k & 1) * slot_bytes; };
         // the chunk's 2 * round_chunk kernels as a graph: one launch instead of many
@@ -404,7 +411,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
             HIP_CHECK(hipMemcpyAsync(sl, d_ncand, sizeof(uint32_t) * 2 * J, hipMemcpyDeviceToHost, st));
             HIP_CHECK(hipMemcpyAsync(sl + sizeof(uint32_t) * 2 * J, d_rt, sizeof(vmx::JobRt) * J,
                                      hipMemcpyDeviceToHost, st));
-            HIP_CHECK(hipMemcpyAsync(sl + sizeof(uint32_t) * 2 * J + sizeof(vmx::JobRt) * J, small, 16,
+            HIP_CHECK(hipMemcpyAsync(sl + sizeof(uint32_t) * 2 * J + sizeof(vmx::JobRt) * J, small, 64,
                                      hipMemcpyDeviceToHost, st));
             HIP_CHECK(hipEventRecord(ctx->ev[2 + (k & 1)], st));
         };
@@ -423,9 +430,10 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
             if (!pending) break;
         }
         std::memcpy(rt.data(), slot(k) + sizeof(uint32_t) * 2 * J, sizeof(vmx::JobRt) * J);
-        uint32_t small_h[4];
-        std::memcpy(small_h, slot(k) + sizeof(uint32_t) * 2 * J + sizeof(vmx::JobRt) * J, 16);
+        uint32_t small_h[16];
+        std::memcpy(small_h, slot(k) + sizeof(uint32_t) * 2 * J + sizeof(vmx::JobRt) * J, 64);
         rounds = small_h[0];
+        for (int d = 0; d < D; ++d) max_depth[d] = small_h[8 + d];
         bool cap_fail = false, too_deep = false;
         for (const auto& r : rt) {
             too_deep |= (r.flags & vmx::JF_DEPTH) != 0;
@@ -462,7 +470,8 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
             std::vector<F64> f64(J);
             for (uint32_t j = 0; j < J; ++j)
                 f64[j] = F64::from(*static_cast<const typename ParamOf<P>::type*>(in[j].params));
-            Carve c(ensure(ctx->misc, sizeof(uint64_t) * J + sizeof(F64) * J + 1024));
+            ensure(ctx->misc, sizeof(uint64_t) * J + sizeof(F64) * J + 1024);
+            Carve c(ctx->misc);
             uint64_t* d_base = c.take<uint64_t>(J);
             F64* d_f64 = c.take<F64>(J);
             HIP_CHECK(hipMemcpyAsync(d_base, base.data(), sizeof(uint64_t) * J, hipMemcpyHostToDevice, st));
@@ -477,24 +486,42 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
 
     // ---------------- 2. selection and canonical leaf order
     const auto t_sel = std::chrono::steady_clock::now();
+    // canonical order in one sort when (job, code_0, ..., code_{D-1}) fits a 64-bit key:
+    // code_d < 2^(max_depth_d + 1); LSD passes over 32-bit keys otherwise
+    vmx::Widths<D> widths{};
+    int job_bits = 0;
+    while (job_bits < 32 && (uint64_t(1) << job_bits) < J) ++job_bits;
+    int key_bits64 = job_bits;
+    for (int d = 0; d < D; ++d) {
+        widths.w[d] = int(max_depth[d]) + 1;
+        key_bits64 += widths.w[d];
+    }
+    const bool one_sort = key_bits64 <= 64;
+    size_t sort_bytes = 0, sort64_bytes = 0;
+    HIP_CHECK(hipError_t(rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                  (int32_t*)nullptr, (int32_t*)nullptr, size_t(n_reg), 0, 32, st)));
+    HIP_CHECK(hipError_t(rocprim::radix_sort_pairs(nullptr, sort64_bytes, (uint64_t*)nullptr, (uint64_t*)nullptr,
+                                                  (int32_t*)nullptr, (int32_t*)nullptr, size_t(n_reg), 0,
+                                                  std::max(1, std::min(64, key_bits64)), st)));
+    sort_bytes = std::max(sort_bytes, sort64_bytes);
     Measure ms;
     ms.take<int32_t>(cap_total);   // bucket
     ms.take<uint8_t>(cap_total);   // bstate
     ms.take<int32_t>(n_reg);       // leaf slots
     ms.take<int32_t>(n_reg);       // sort values tmp
-    ms.take<uint32_t>(n_reg);      // sort keys
-    ms.take<uint32_t>(n_reg);      // sort keys tmp
-    size_t sort_bytes = 0;
-    HIP_CHECK(hipError_t(rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                                  (int32_t*)nullptr, (int32_t*)nullptr, size_t(n_reg), 0, 32, st)));
+    ms.take<uint64_t>(n_reg);      // sort keys
+    ms.take<uint64_t>(n_reg);      // sort keys tmp
     ms.take<char>(sort_bytes + 256);
-    Carve cs(ensure(ctx->regs, ms.used));
+    ensure(ctx->regs, ms.used);
+    Carve cs(ctx->regs);
     int32_t* d_bucket = cs.take<int32_t>(cap_total);
     uint8_t* d_bstate = cs.take<uint8_t>(cap_total);
     int32_t* d_leaf = cs.take<int32_t>(n_reg);
     int32_t* d_leaf_tmp = cs.take<int32_t>(n_reg);
-    uint32_t* d_keys = cs.take<uint32_t>(n_reg);
-    uint32_t* d_keys_tmp = cs.take<uint32_t>(n_reg);
+    uint64_t* d_keys64 = cs.take<uint64_t>(n_reg);
+    uint64_t* d_keys64_tmp = cs.take<uint64_t>(n_reg);
+    uint32_t* d_keys = reinterpret_cast<uint32_t*>(d_keys64);
+    uint32_t* d_keys_tmp = reinterpret_cast<uint32_t*>(d_keys64_tmp);
     void* d_sort_tmp = cs.take<char>(sort_bytes + 256);
     const unsigned cap_blocks = (cap_total + 255) / 256;
     const unsigned reg_blocks = unsigned((n_reg + 255) / 256);
@@ -505,15 +532,23 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         vmx::k_sel_resolve<T, D><<<J, 256, 0, st>>>(d_jobs, d_rt, t, d_bucket, d_bstate);
         vmx::k_leaves<T><<<cap_blocks, 256, 0, st>>>(d_jobs, d_rt, t, cap_total, d_leaf);
         HIP_CHECK(hipGetLastError());
-        // LSD passes: code_{D-1} ... code_0, then job
         int32_t* vin = d_leaf;
         int32_t* vout = d_leaf_tmp;
-        for (int which = D - 1; which >= (J > 1 ? -1 : 0); --which) {
-            vmx::k_leaf_keys<D><<<reg_blocks, 256, 0, st>>>(vin, t.code, t.job, uint32_t(n_reg), which, d_keys);
+        if (one_sort) {
+            vmx::k_leaf_keys64<D><<<reg_blocks, 256, 0, st>>>(d_leaf, t.code, t.job, uint32_t(n_reg), widths, d_keys64,
+                                                              d_leaf_tmp);
             size_t bytes = sort_bytes;
-            HIP_CHECK(hipError_t(rocprim::radix_sort_pairs(d_sort_tmp, bytes, d_keys, d_keys_tmp, vin, vout,
-                                                          size_t(n_reg), 0, 32, st)));
-            std::swap(vin, vout);
+            HIP_CHECK(hipError_t(rocprim::radix_sort_pairs(d_sort_tmp, bytes, d_keys64, d_keys64_tmp, d_leaf_tmp, d_leaf,
+                                                          size_t(n_reg), 0, std::max(1, key_bits64), st)));
+        } else {
+            // LSD passes: code_{D-1} ... code_0, then job
+            for (int which = D - 1; which >= (J > 1 ? -1 : 0); --which) {
+                vmx::k_leaf_keys<D><<<reg_blocks, 256, 0, st>>>(vin, t.code, t.job, uint32_t(n_reg), which, d_keys);
+                size_t bytes = sort_bytes;
+                HIP_CHECK(hipError_t(rocprim::radix_sort_pairs(d_sort_tmp, bytes, d_keys, d_keys_tmp, vin, vout,
+                                                              size_t(n_reg), 0, 32, st)));
+                std::swap(vin, vout);
+            }
         }
         if (vin != d_leaf)
             HIP_CHECK(hipMemcpyAsync(d_leaf, vin, sizeof(int32_t) * n_reg, hipMemcpyDeviceToDevice, st));
@@ -583,7 +618,8 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     mr.take<uint32_t>(n_reg + 1);
     mr.take<uint32_t>(total_bins + 1);
     mr.take<char>(scan_reg_bytes + 256);
-    Carve cr(ensure(ctx->lists, mr.used));
+    ensure(ctx->lists, mr.used);
+    Carve cr(ctx->lists);
     vmx::RegionArrays<T> ra{};
     ra.lo = cr.take<T>(n_reg * D);
     ra.hi = cr.take<T>(n_reg * D);
@@ -638,31 +674,37 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         for (uint32_t j = 0; j < J; ++j)
             if ((rt[j].flags & vmx::JF_LEAVES) || rt[j].nleaves != jobs[j].N)
                 throw VmxError(VMX_ERR_CAPACITY, "region selection did not close");
-    unsigned key_bits = 1;
-    while (key_bits < 32 && (uint64_t(1) << key_bits) < total_bins) ++key_bits;
-    size_t sort_pairs_bytes = 0;
-    HIP_CHECK(hipError_t(rocprim::radix_sort_pairs(nullptr, sort_pairs_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                                  (uint32_t*)nullptr, (uint32_t*)nullptr, size_t(pairs), 0, key_bits,
-                                                  st)));
+    // per-bin footprint lists by counting sort: count, scan, place, then every list in
+    // ascending footprint id (region order)
+    size_t scan_bins_bytes = 0;
+    HIP_CHECK(hipError_t(rocprim::exclusive_scan(nullptr, scan_bins_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr, 0u,
+                                                size_t(total_bins + 1), rocprim::plus<uint32_t>(), st)));
     Measure mp;
-    for (int i = 0; i < 4; ++i) mp.take<uint32_t>(size_t(pairs) + 1);
-    mp.take<char>(sort_pairs_bytes + 256);
-    Carve cp(ensure(ctx->bins2, mp.used));
-    uint32_t* d_keys_in = cp.take<uint32_t>(size_t(pairs) + 1);
-    uint32_t* d_vals_in = cp.take<uint32_t>(size_t(pairs) + 1);
-    uint32_t* d_keys_out = cp.take<uint32_t>(size_t(pairs) + 1);
+    mp.take<uint32_t>(total_bins + 1);                        // counts
+    mp.take<uint32_t>(size_t(pairs) + 1);                     // lists
+    mp.take<uint32_t>(size_t(pairs) + 1);                     // copies of very long lists
+    mp.take<uint32_t>(size_t(pairs) / vmx::SORT_SHORT + 2);   // queue of long lists
+    mp.take<char>(scan_bins_bytes + 256);
+    ensure(ctx->bins2, mp.used);
+    Carve cp(ctx->bins2);
+    uint32_t* d_cnt = cp.take<uint32_t>(total_bins + 1);
     uint32_t* d_lst = cp.take<uint32_t>(size_t(pairs) + 1);
-    void* d_sort_pairs_tmp = cp.take<char>(sort_pairs_bytes + 256);
+    uint32_t* d_lst_tmp = cp.take<uint32_t>(size_t(pairs) + 1);
+    uint32_t* d_qlong = cp.take<uint32_t>(size_t(pairs) / vmx::SORT_SHORT + 2);
+    void* d_scan_bins_tmp = cp.take<char>(scan_bins_bytes + 256);
+    HIP_CHECK(hipMemsetAsync(d_cnt, 0, sizeof(uint32_t) * (total_bins + 1), st));
+    HIP_CHECK(hipMemsetAsync(d_qlong, 0, sizeof(uint32_t), st));
+    const unsigned fp_blocks = unsigned((uint64_t(nfp) * 64 + 255) / 256);
+    if (pairs > 0)
+        vmx::k_bin_pass<T, false><<<fp_blocks, 256, 0, st>>>(d_jobs, fa, d_nfp, d_cnt, nullptr, nullptr);
+    bytes = scan_bins_bytes;
+    HIP_CHECK(hipError_t(rocprim::exclusive_scan(d_scan_bins_tmp, bytes, d_cnt, d_off, 0u, size_t(total_bins + 1),
+                                                rocprim::plus<uint32_t>(), st)));
     if (pairs > 0) {
-        const unsigned emit_blocks = unsigned((uint64_t(nfp) * 64 + 255) / 256);
-        vmx::k_emit_pairs<T><<<emit_blocks, 256, 0, st>>>(d_jobs, fa, d_nfp, d_pofs, d_keys_in, d_vals_in);
-        HIP_CHECK(hipGetLastError());
-        size_t sb = sort_pairs_bytes;
-        HIP_CHECK(hipError_t(rocprim::radix_sort_pairs(d_sort_pairs_tmp, sb, d_keys_in, d_keys_out, d_vals_in, d_lst,
-                                                      size_t(pairs), 0, key_bits, st)));
+        vmx::k_bin_pass<T, true><<<fp_blocks, 256, 0, st>>>(d_jobs, fa, d_nfp, d_cnt, d_off, d_lst);
+        vmx::k_sort_lists<<<unsigned((total_bins + 255) / 256), 256, 0, st>>>(d_off, total_bins, d_lst, d_qlong);
+        vmx::k_sort_long<<<1024, 256, 0, st>>>(d_off, d_qlong, d_lst, d_lst_tmp);
     }
-    vmx::k_bin_offsets<<<unsigned((uint64_t(pairs) + 1 + 255) / 256), 256, 0, st>>>(d_keys_out, pairs, total_bins,
-                                                                                 d_off);
     HIP_CHECK(hipGetLastError());
     const double ms_str = ms_since(t_str);
 
@@ -702,7 +744,8 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
                                                    int32_t* odim) {
         const size_t N = n_reg;
         const size_t total = N * (2 * D + K + 1);
-        Carve c(ensure(ctx->misc, sizeof(T) * total + sizeof(int32_t) * N + 4096));
+        ensure(ctx->misc, sizeof(T) * total + sizeof(int32_t) * N + 4096);
+        Carve c(ctx->misc);
         T* tmp = c.take<T>(total);
         int32_t* tdim = c.take<int32_t>(N);
         const unsigned g1 = unsigned((N * K + 255) / 256);
@@ -849,7 +892,8 @@ template <typename T, typename Math>
 int eval_T(vmx_ctx* ctx, const vmx_integrand& f, uint64_t n, const double* x, double* out) {
     const int D = f.dim;
     const size_t fbytes = 8192;
-    Carve c(ensure(ctx->misc, fbytes + sizeof(double) * n * (D + 1) + 1024));
+    ensure(ctx->misc, fbytes + sizeof(double) * n * (D + 1) + 1024);
+    Carve c(ctx->misc);
     void* d_f = c.take<char>(fbytes);
     double* d_x = c.take<double>(n * D);
     double* d_o = c.take<double>(n);
@@ -1057,7 +1101,8 @@ void train_net(vmx_ctx* ctx, int dim, uint64_t count, const vmx_train_opts& o, v
     m.take<float>(P);              // m
     m.take<float>(P);              // v
     m.take<uint32_t>(count);       // idx: train then valid
-    Carve c(ensure(ctx->tr_state, m.used));
+    ensure(ctx->tr_state, m.used);
+    Carve c(ctx->tr_state);
     float* d_p = c.take<float>(PT);
     float* d_m = c.take<float>(P);
     float* d_v = c.take<float>(P);
@@ -1153,7 +1198,8 @@ int mc_run(vmx_ctx* ctx, const vmx_problem& p, const void* fparams, int stratifi
         grid = uint32_t(std::sqrt(double(p.spp)));
         while (grid > 1 && p.spp % (uint64_t(grid) * grid) != 0) --grid;
     }
-    Carve c(ensure(ctx->misc, sizeof(P) + sizeof(double) * nb * 2 + 4096));
+    ensure(ctx->misc, sizeof(P) + sizeof(double) * nb * 2 + 4096);
+    Carve c(ctx->misc);
     P* d_f = c.take<P>(1);
     double* d_s = c.take<double>(nb);
     double* d_s2 = c.take<double>(nb);
@@ -1232,7 +1278,8 @@ NetIn upload_net_inputs(vmx_ctx* ctx, int dim, const float* params, uint64_t n, 
     m.take<float>(n);
     m.take<float>(n);
     m.take<uint32_t>(n);
-    Carve c(ensure(ctx->misc, m.used));
+    ensure(ctx->misc, m.used);
+    Carve c(ctx->misc);
     NetIn r{};
     r.p = c.take<float>(PT);
     r.X = c.take<uint16_t>(n * kp);
@@ -1509,7 +1556,8 @@ int vmx_exact_bins(vmx_ctx* ctx, const vmx_problem* p, const vmx_integrand* f, d
         for (int i = 0; i < 5; ++i) e.step[i] = q.step[i];
         for (int i = 0; i < 3; ++i) e.wave[i] = q.wave[i];
         const uint64_t nb = p->res[0] * p->res[1];
-        Carve c(ensure(ctx->misc, sizeof(e) + sizeof(double) * nb + 1024));
+        ensure(ctx->misc, sizeof(e) + sizeof(double) * nb + 1024);
+        Carve c(ctx->misc);
         vmx::ExactBlobs2D* d_e = c.take<vmx::ExactBlobs2D>(1);
         double* d_o = c.take<double>(nb);
         hipStream_t st = ctx->stream;

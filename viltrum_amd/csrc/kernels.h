@@ -24,22 +24,35 @@
 namespace vmx {
 
 /* ---- math policies for integrand evaluation ---- */
-struct PreciseMath {
+/* Each policy also fixes how the integrands round: `fma(a, b, c)` is a*b + c, and
+   `cos2pi(p)` is cos(2 pi p). */
+struct PreciseMath {  // the host's arithmetic: correctly rounded libm, two roundings per a*b + c
     __device__ static float exp(float x) { return expf(x); }
     __device__ static double exp(double x) { return ::exp(x); }
     __device__ static float sin(float x) { return sinf(x); }
     __device__ static double sin(double x) { return ::sin(x); }
     __device__ static float cos(float x) { return cosf(x); }
     __device__ static double cos(double x) { return ::cos(x); }
+    template <typename T>
+    __device__ static T fma(T a, T b, T c) { return a * b + c; }
+    template <typename T>
+    __device__ static T cos2pi(T p) { return cos(T(6.283185307179586476925286766559) * p); }
 };
-/* FP32 production: hardware exp2-based exponential; sin/cos keep full range reduction. */
+/* FP32 production: the hardware transcendentals (v_exp_f32, v_sin_f32, v_cos_f32; a few
+   ulp, arguments here stay within a few hundred radians) and fused multiply-adds.
+   FP64 is unchanged. */
 struct FastMath {
     __device__ static float exp(float x) { return __expf(x); }
     __device__ static double exp(double x) { return ::exp(x); }
-    __device__ static float sin(float x) { return sinf(x); }
+    __device__ static float sin(float x) { return __sinf(x); }
     __device__ static double sin(double x) { return ::sin(x); }
-    __device__ static float cos(float x) { return cosf(x); }
+    __device__ static float cos(float x) { return __cosf(x); }
     __device__ static double cos(double x) { return ::cos(x); }
+    __device__ static float fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+    __device__ static double fma(double a, double b, double c) { return a * b + c; }
+    // exact reduction to [-1/2, 1/2] revolutions, then the hardware cosine (revolutions)
+    __device__ static float cos2pi(float p) { return __builtin_amdgcn_cosf(p - rintf(p)); }
+    __device__ static double cos2pi(double p) { return ::cos(6.283185307179586476925286766559 * p); }
 };
 
 /* 16-bit histogram key of a non-negative error: its top 16 bits (order preserving). */

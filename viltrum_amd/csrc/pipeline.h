@@ -269,6 +269,7 @@ struct GenArgs {
     uint32_t* ncand;   // [2*J] number of candidates
     T* theta;         // [J]
     uint32_t* rounds; // rounds that had candidates
+    uint32_t* depth;  // [D] most splits along each dim over all nodes (block maxima)
     uint32_t J;
     uint32_t max_rounds;
     const void* net;   // the priority network's LDS image (nn::NetW<KP>, packed once per network)
@@ -281,6 +282,7 @@ struct SlotLds {
     uint32_t job[TPB];
     int32_t slot[TPB];
     uint8_t want[TPB];
+    uint32_t depth[VMX_MAX_DIM];  // the block's deepest new code per dim
 };
 
 /* LDS of the fused priority network: the block's 2*TPB new regions form the rows of one
@@ -443,6 +445,7 @@ __device__ inline void team_expand(const GenArgs<T>& g, const F* __restrict__ fs
         const size_t s = size_t(L.slot) + c;
         const uint32_t pcode = t.code[size_t(node) * D + dim];
         if (pcode >= 0x80000000u) atomicOr(&g.rt[j].flags, JF_DEPTH);
+        atomicMax(&sl.depth[dim], 31u - uint32_t(__clz(pcode * 2u + uint32_t(c))));
         for (int d = 0; d < D; ++d) {
             t.lo[s * D + d] = L.box[c][0][d];
             t.hi[s * D + d] = L.box[c][1][d];
@@ -510,7 +513,7 @@ template <typename T, int D, typename F, typename M, bool LEARN>
 __global__ void __launch_bounds__(GEN_THREADS) k_expand_round(GenArgs<T> g, const F* __restrict__ fs, int par) {
     constexpr int TEAM = TeamShape<D>::TEAM;
     constexpr int TPB = GEN_THREADS / TEAM;  // teams per block
-    __shared__ uint32_t pre[MAX_JOBS + 1];
+    extern __shared__ uint32_t pre[];        // J + 1 entries: the launch's dynamic LDS
     __shared__ TeamLds<T, D> lds[TPB];
     __shared__ SlotLds<TPB> slots;
     __shared__ typename std::conditional<LEARN, LearnLds<D>, char>::type learn_lds;
@@ -526,6 +529,7 @@ __global__ void __launch_bounds__(GEN_THREADS) k_expand_round(GenArgs<T> g, cons
     const uint32_t total = pre[J];
     if (blockIdx.x == 0 && threadIdx.x == 0 && total > 0) atomicAdd(g.rounds, 1u);
     if (uint64_t(blockIdx.x) * TPB >= total) return;  // no candidate for this block (uniform)
+    if (threadIdx.x < D) slots.depth[threadIdx.x] = 0;  // ordered before use by team_expand's barriers
     if constexpr (LEARN) {
         ll = &learn_lds;
         constexpr int KP = LearnLds<D>::KP;
@@ -540,6 +544,8 @@ __global__ void __launch_bounds__(GEN_THREADS) k_expand_round(GenArgs<T> g, cons
         const uint64_t w = base + team;
         team_expand<T, D, F, M, LEARN>(g, fs, lds[team], par, pre, w, w < total, ll, team, slots);
     }
+    __syncthreads();
+    if (threadIdx.x < D && slots.depth[threadIdx.x] != 0) atomicMax(&g.depth[threadIdx.x], slots.depth[threadIdx.x]);
 }
 
 /* ------------------------------------------------------------------ training pairs
@@ -779,6 +785,12 @@ __global__ void k_leaves(const Job<T>* __restrict__ jobs, JobRt* rt, TreeArrays<
     leaf_slot[jb.reg_base + pos] = int32_t(s);
 }
 
+/* Field widths (bits) of the codes in a 64-bit leaf key */
+template <int D>
+struct Widths {
+    int w[D];
+};
+
 /* Sort keys of one LSD pass over the leaves (canonical order: job, code_0, ...,
    code_{D-1}).  which = -1: job; which = d: code_d (32 bits). */
 template <int D>
@@ -793,6 +805,26 @@ __global__ void k_leaf_keys(const int32_t* __restrict__ leaf_slot, const uint32_
     }
     const size_t s = size_t(sl);
     keys[r] = which < 0 ? job[s] : code[s * D + which];
+}
+
+/* One 64-bit key per leaf, (job, code_0, ..., code_{D-1}) in fields of `width` bits
+   (code_d < 2^width_d), so one sort gives the lexicographic order of the LSD passes. */
+template <int D>
+__global__ void k_leaf_keys64(const int32_t* __restrict__ leaf_slot, const uint32_t* __restrict__ code,
+                              const uint32_t* __restrict__ job, uint32_t n, Widths<D> wd, uint64_t* keys,
+                              int32_t* vals) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int32_t sl = leaf_slot[r];
+    vals[r] = sl;
+    if (sl < 0) {
+        keys[r] = ~uint64_t(0);
+        return;
+    }
+    const size_t s = size_t(sl);
+    uint64_t k = job[s];
+    for (int d = 0; d < D; ++d) k = (k << wd.w[d]) | code[s * D + d];
+    keys[r] = k;
 }
 
 /* ------------------------------------------------------------------ regions */
@@ -906,43 +938,124 @@ __global__ void k_footprints(const Job<T>* __restrict__ jobs, RegionArrays<T> r,
     }
 }
 
-/* Count (Fill = false) or place (Fill = true) footprint ids into their bins' lists:
-   one wave per footprint, lanes over the pixel box. */
-/* (bin, footprint) pairs of every footprint's pixel box, one wave per footprint, at the
-   footprint's offset of the scanned pair counts: no atomics, and footprint ids ascend
-   along the array, so a stable sort by bin leaves each bin's list in region order. */
-template <typename T>
-__global__ void k_emit_pairs(const Job<T>* __restrict__ jobs, FootArrays<T> f, const uint32_t* __restrict__ nfp_ptr,
-                             const uint32_t* __restrict__ pofs, uint32_t* __restrict__ keys,
-                             uint32_t* __restrict__ vals) {
+/* q = c / n for the box walks below: a float estimate, then exact integer fix-up. */
+__device__ inline uint32_t div_fix(uint32_t c, uint32_t n, float inv) {
+    uint32_t q = uint32_t(float(c) * inv);
+    int64_t r = int64_t(c) - int64_t(q) * n;
+    while (r < 0) {
+        --q;
+        r += n;
+    }
+    while (r >= int64_t(n)) {
+        ++q;
+        r -= n;
+    }
+    return q;
+}
+
+/* Per-bin footprint lists by counting sort, one wave per footprint walking its pixel
+   box with the last bin dim fastest (consecutive lanes, consecutive bins).
+   PLACE = false: cnt[bin] += 1.  PLACE = true (cnt = the counts, off = their exclusive
+   scan): each footprint id goes to off[bin] + (--cnt[bin]), so every list is filled
+   but in scheduling order; k_sort_lists then puts each list in ascending footprint id
+   (region order), which fixes the per-bin summation order. */
+template <typename T, bool PLACE>
+__global__ void k_bin_pass(const Job<T>* __restrict__ jobs, FootArrays<T> f, const uint32_t* __restrict__ nfp_ptr,
+                           uint32_t* __restrict__ cnt, const uint32_t* __restrict__ off, uint32_t* __restrict__ lst) {
     const uint32_t w = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
     const uint32_t lane = threadIdx.x % 64;
     if (w >= *nfp_ptr) return;
     const Job<T>& jb = jobs[f.job[w]];
     const uint32_t* bx = f.box + size_t(w) * 6;
-    const uint32_t n0 = bx[1] - bx[0], n1 = bx[3] - bx[2];
+    const uint32_t n0 = bx[1] - bx[0], n1 = bx[3] - bx[2], n2 = bx[5] - bx[4];
     const uint32_t total = f.npairs[w];
-    const uint32_t o = pofs[w];
+    const uint64_t r1 = jb.db > 1 ? jb.res[1] : 1, r2 = jb.db > 2 ? jb.res[2] : 1;
+    const float i2 = 1.0f / float(n2), i12 = 1.0f / float(n1);
     for (uint32_t c = lane; c < total; c += 64) {
-        const uint32_t p0 = bx[0] + c % n0;
-        const uint32_t p1 = bx[2] + (c / n0) % n1;
-        const uint32_t p2 = bx[4] + c / (n0 * n1);
-        uint64_t local = p0;
-        if (jb.db > 1) local = local * jb.res[1] + p1;
-        if (jb.db > 2) local = local * jb.res[2] + p2;
-        keys[o + c] = uint32_t(jb.bin_base + local);
-        vals[o + c] = w;
+        const uint32_t q2 = div_fix(c, n2, i2);  // c = (a0 * n1 + a1) * n2 + a2
+        const uint32_t a2 = c - q2 * n2;
+        const uint32_t a0 = div_fix(q2, n1, i12);
+        const uint32_t a1 = q2 - a0 * n1;
+        const uint64_t local = (uint64_t(bx[0] + a0) * r1 + (bx[2] + a1)) * r2 + (bx[4] + a2);
+        const uint64_t bin = jb.bin_base + local;
+        if constexpr (PLACE) {
+            const uint32_t k = atomicSub(&cnt[bin], 1u) - 1u;
+            lst[off[bin] + k] = w;
+        } else {
+            atomicAdd(&cnt[bin], 1u);
+        }
     }
 }
 
-/* off[b] = first position of bin b in the bin-sorted pair keys (off[B] = P). */
-__global__ void k_bin_offsets(const uint32_t* __restrict__ keys, uint32_t P, uint64_t total_bins,
-                              uint32_t* __restrict__ off) {
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i > P) return;
-    const uint64_t prev = i == 0 ? 0 : uint64_t(keys[i - 1]) + 1;
-    const uint64_t cur = i == P ? total_bins + 1 : uint64_t(keys[i]) + 1;
-    for (uint64_t b = prev; b < cur; ++b) off[b] = uint32_t(i);
+constexpr uint32_t SORT_SHORT = 32;  // longer lists are sorted by a workgroup each
+
+/* Ascending footprint ids in every bin's list: insertion sort per lane for short lists,
+   staged in the lane's own LDS column (no barriers); longer ones are queued for
+   k_sort_long. */
+__global__ void __launch_bounds__(256) k_sort_lists(const uint32_t* __restrict__ off, uint64_t total_bins,
+                                                    uint32_t* __restrict__ lst, uint32_t* __restrict__ qlong) {
+    __shared__ uint32_t sv[SORT_SHORT][256];
+    const int tid = threadIdx.x;
+    const uint64_t b = uint64_t(blockIdx.x) * blockDim.x + tid;
+    if (b >= total_bins) return;
+    const uint32_t o = off[b], n = off[b + 1] - o;
+    if (n <= 1) return;
+    if (n > SORT_SHORT) {
+        const uint32_t k = atomicAdd(&qlong[0], 1u);
+        qlong[1 + k] = uint32_t(b);
+        return;
+    }
+    for (uint32_t i = 0; i < n; ++i) sv[i][tid] = lst[o + i];
+    bool moved = false;
+    for (uint32_t i = 1; i < n; ++i) {
+        const uint32_t v = sv[i][tid];
+        uint32_t j = i;
+        while (j > 0 && sv[j - 1][tid] > v) {
+            sv[j][tid] = sv[j - 1][tid];
+            --j;
+        }
+        sv[j][tid] = v;
+        moved |= j != i;
+    }
+    if (moved)
+        for (uint32_t i = 0; i < n; ++i) lst[o + i] = sv[i][tid];
+}
+
+/* One workgroup per queued long list: every entry goes to its rank among the list's
+   entries (ids within a list are distinct), read from a copy in LDS, or for very long
+   lists from a copy in `tmp` read back with device-scope loads.  Blocks stride over the
+   queue; the loop bounds are block-uniform. */
+constexpr uint32_t SORT_LDS = 8192;
+__global__ void __launch_bounds__(256) k_sort_long(const uint32_t* __restrict__ off, const uint32_t* __restrict__ qlong,
+                                                   uint32_t* __restrict__ lst, uint32_t* __restrict__ tmp) {
+    __shared__ uint32_t sb[SORT_LDS];
+    const uint32_t nq = qlong[0];
+    for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
+        const uint32_t b = qlong[1 + q];
+        const uint32_t o = off[b], n = off[b + 1] - o;
+        if (n <= SORT_LDS) {
+            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) sb[i] = lst[o + i];
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+                const uint32_t v = sb[i];
+                uint32_t r = 0;
+                for (uint32_t k = 0; k < n; ++k) r += sb[k] < v ? 1u : 0u;
+                lst[o + r] = v;
+            }
+        } else {
+            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) tmp[o + i] = lst[o + i];
+            __threadfence();
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+                const uint32_t v = __hip_atomic_load(&tmp[o + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                uint32_t r = 0;
+                for (uint32_t k = 0; k < n; ++k)
+                    r += __hip_atomic_load(&tmp[o + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < v ? 1u : 0u;
+                lst[o + r] = v;
+            }
+        }
+        __syncthreads();
+    }
 }
 
 template <typename T>
@@ -960,6 +1073,19 @@ __device__ inline uint32_t job_of_bin(const Job<T>* jobs, uint32_t J, uint64_t b
 
 constexpr int LANE_MAXF = 16;  // footprints per bin cached in LDS (longer lists walk memory)
 
+/* A residual sample's position in its region.  FP64: pos_in, the reference's quotient.
+   FP32: x times the hardware reciprocal (about 1 ulp); only the polynomial's value at
+   the sample depends on it, while the control-variate integral keeps pos_in. */
+template <typename T>
+__device__ inline T sample_pos(T x, T lo, T hi) {
+    return pos_in(x, lo, hi);
+}
+template <>
+__device__ inline float sample_pos<float>(float x, float lo, float hi) {
+    if (lo >= hi) return lo;
+    return (x - lo) * __builtin_amdgcn_rcpf(hi - lo);
+}
+
 /* One lane per bin (all jobs of the batch), one block per 256 consecutive bins.  The
    lane sums the control variate over the
    bin's footprints in list order (FP64), then draws the spp residual samples -- a region
@@ -976,7 +1102,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D <= 3
                                                const uint32_t* __restrict__ off, const uint32_t* __restrict__ lst,
                                                const F* __restrict__ fs, JobRt* rt) {
     constexpr int K = pow3(D);
-    __shared__ uint32_t s_fi[LANE_MAXF][256];
+    // per lane: first region and running region count of its first LANE_MAXF footprints
+    __shared__ uint32_t s_first[LANE_MAXF][256];
     __shared__ uint32_t s_cum[LANE_MAXF + 1][256];
     // the integrand's parameters in LDS when the block's bins belong to one job (the
     // common case): every evaluation then reads them as LDS broadcasts
@@ -1032,13 +1159,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D <= 3
         // control variate: footprint terms in list order
         double approx = (jb.form == VMX_FORM_CV_ALPHA1) ? double(bins[b]) : 0.0;
         const uint32_t nc = nf < uint32_t(LANE_MAXF) ? nf : uint32_t(LANE_MAXF);
-        uint32_t n = 0;
+        uint32_t n = 0, first0 = 0;
         s_cum[0][tid] = 0;
         for (uint32_t k = 0; k < nf; ++k) {
             const uint32_t fi = lst[o + k];
             n += fa.count[fi];
             if (k < nc) {
-                s_fi[k][tid] = fi;
+                const uint32_t f0 = fa.first[fi];
+                if (k == 0) first0 = f0;
+                s_first[k][tid] = f0;
                 s_cum[k + 1][tid] = n;
             }
             double aa[DB], bb[DB];
@@ -1059,8 +1188,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D <= 3
         const uint64_t spp = jb.spp;
         const bool sampling = spp > 0 && n > 0;
         if (spp > 0 && n == 0) atomicOr(&rt[j].flags, JF_EMPTY_BIN);
-        // region index -> region: binary search over the cached prefix counts, memory walk beyond
+        // region index -> region: one footprint is one contiguous run of regions; else a
+        // binary search over the cached prefix counts, memory walk beyond
         auto region_of = [&](uint32_t idx) -> uint32_t {
+            if (nf == 1) return first0 + idx;
             if (idx < s_cum[nc][tid] || nc == nf) {
                 uint32_t lo = 0, hi = nc;
                 while (hi - lo > 1) {
@@ -1070,7 +1201,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D <= 3
                     else
                         hi = mid;
                 }
-                return fa.first[s_fi[lo][tid]] + (idx - s_cum[lo][tid]);
+                return s_first[lo][tid] + (idx - s_cum[lo][tid]);
             }
             idx -= s_cum[nc][tid];
             for (uint32_t k = nc;; ++k) {
@@ -1080,6 +1211,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D <= 3
                 idx -= c;
             }
         };
+        constexpr bool FP32 = sizeof(T) == 4;
+        const T wn = T(factor * double(n));  // FP32 sample weight factor B * n
         KahanSum<T> sf, sa, mf, ma, maa, mfa;
         double acc1 = 0.0;  // alpha = 1 form
         T k_f = 0, k_a = 0;
@@ -1095,17 +1228,26 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D <= 3
                 ilo[d] = vmax(blo[d], rl);
                 ihi[d] = vmax(ilo[d], vmin(bhi[d], rh));
                 if (ilo[d] >= ihi[d]) empty = true;
-                x[d] = ilo[d] + u.coord(d) * (ihi[d] - ilo[d]);
-                t[d] = pos_in(x[d], rl, rh);
+                x[d] = mul_add<FP32>(u.coord(d), ihi[d] - ilo[d], ilo[d]);
+                t[d] = sample_pos(x[d], rl, rh);
             }
             const T fx = bj >= 0 ? f_lds.template eval<M, D>(x) : fs[j].template eval<M, D>(x);
-            const T px = eval_monomials<T, D>(r.coef + size_t(reg) * K, t);
-            const double w = factor * double(n) * double(box_volume<T, D>(ilo, ihi));
+            const T px = eval_monomials<T, D, FP32>(r.coef + size_t(reg) * K, t);
+            const T vol = box_volume<T, D>(ilo, ihi);
             if (jb.form == VMX_FORM_CV_ALPHA1) {
-                if (!empty) acc1 += double(fx - px) * w / double(spp);
+                if (!empty) acc1 += double(fx - px) * (factor * double(n) * double(vol)) / double(spp);
                 continue;
             }
-            const T fsv = T(double(fx) * w), asv = T(double(px) * w);
+            T fsv, asv;
+            if constexpr (FP32) {  // weights in FP32: B * n * vol is O(n), far from the range limits
+                const T w = wn * vol;
+                fsv = fx * w;
+                asv = px * w;
+            } else {
+                const double w = factor * double(n) * double(vol);
+                fsv = T(double(fx) * w);
+                asv = T(double(px) * w);
+            }
             if (s == 0) {  // the bin's first sample shifts the moments
                 k_f = vabs(fsv);
                 k_a = vabs(asv);

@@ -278,25 +278,39 @@ VMX_HD inline void monomials(const T* v, C* c) {
     }
 }
 
+/* a*b + c, fused (one rounding) when FUSED */
+template <bool FUSED, typename C>
+VMX_HD inline C mul_add(C a, C b, C c) {
+    if constexpr (FUSED)
+        return C(__builtin_fma(a, b, c));
+    else
+        return a * b + c;
+}
+template <>
+VMX_HD inline float mul_add<true, float>(float a, float b, float c) {
+    return __builtin_fmaf(a, b, c);
+}
+
 /* Evaluate the monomial form at normalised t: nested Horner, dim 0 innermost, written
-   as template recursion so every coefficient stays in registers. */
-template <typename C, int DD>
+   as template recursion so every coefficient stays in registers.  FUSED: each Horner
+   step is one fused multiply-add. */
+template <typename C, int DD, bool FUSED = false>
 struct MonoHorner {
     VMX_HD static C eval(const C* c, const C* t) {
         constexpr int S = pow3(DD - 1);
-        const C a0 = MonoHorner<C, DD - 1>::eval(c, t);
-        const C a1 = MonoHorner<C, DD - 1>::eval(c + S, t);
-        const C a2 = MonoHorner<C, DD - 1>::eval(c + 2 * S, t);
-        return (a2 * t[DD - 1] + a1) * t[DD - 1] + a0;
+        const C a0 = MonoHorner<C, DD - 1, FUSED>::eval(c, t);
+        const C a1 = MonoHorner<C, DD - 1, FUSED>::eval(c + S, t);
+        const C a2 = MonoHorner<C, DD - 1, FUSED>::eval(c + 2 * S, t);
+        return mul_add<FUSED>(mul_add<FUSED>(a2, t[DD - 1], a1), t[DD - 1], a0);
     }
 };
-template <typename C>
-struct MonoHorner<C, 0> {
+template <typename C, bool FUSED>
+struct MonoHorner<C, 0, FUSED> {
     VMX_HD static C eval(const C* c, const C*) { return c[0]; }
 };
-template <typename C, int D>
+template <typename C, int D, bool FUSED = false>
 VMX_HD inline C eval_monomials(const C* c, const C* t) {
-    return MonoHorner<C, D>::eval(c, t);
+    return MonoHorner<C, D, FUSED>::eval(c, t);
 }
 
 }  // namespace vmx
