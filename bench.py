@@ -58,6 +58,7 @@ def run_config3(args, V, hip):
     stream = hip.Stream()
     total_bins = sum(int(np.prod(it["res"])) for it in items)
     ptrs = [b.address for b in bufs]
+    tune = ctx.autotune(probs, fs, ptrs, stream.address)
     for _ in range(args.warmup):
         ctx.integrate_batch(probs, fs, ptrs, stream.address)
     hip.device_synchronize()
@@ -82,6 +83,7 @@ def run_config3(args, V, hip):
         "evals_per_s": evals / (ms_batch * 1e-3),
         "sequential": {"ms": ms_seq, "bins_per_s": total_bins / (ms_seq * 1e-3)},
         "batch_speedup": ms_seq / ms_batch,
+        "autotune": tune,
         "phase_ms": {k: float(getattr(st[0], k)) for k in ("ms_generate", "ms_select", "ms_stratify", "ms_bins")},
     }
     print(json.dumps(res), flush=True)
@@ -142,6 +144,11 @@ def run_config4(args, V, hip):
         return (time.perf_counter() - t0) * 1e3 / steps, st
 
     budgets = (4096, 16384)
+    # launch shapes of each path's workload class, picked by the library's autotuner
+    # (setup, untimed, like the training)
+    tune = {"learned_batched": ctx.autotune(problems(4096, 16, True), fs, ptrs, stream.address),
+            "heuristic_batched": ctx.autotune(problems(4096, 16, False), fs, ptrs, stream.address),
+            "plain": ctx.autotune(problems(4096, 16, False)[:1], fs[:1], ptrs[:1], stream.address)}
     run_plain(problems(16384, 16, False))
     target = rmse()
 
@@ -218,6 +225,7 @@ def run_config4(args, V, hip):
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel_ms": ms_k},
         "training": {k: rep[k] for k in ("pairs", "steps", "loss_first", "loss_train", "loss_valid", "ms_pairs",
                                          "ms_train")},
+        "autotune": tune,
     }
     if not args.no_cpu_baseline:
         w = dict(f=fs[0], res=res, regions=bp["regions"], spp=bp["spp"], seed=0)
@@ -244,6 +252,7 @@ def run_config5(args, V, hip):
     bufs = [hip.DeviceBuffer(nb * 4) for _ in fs]
     ptrs = [b.address for b in bufs]
     stream = hip.Stream()
+    tune = ctx.autotune(ps, fs, ptrs, stream.address)
     for _ in range(args.warmup):
         ctx.integrate_batch(ps, fs, ptrs, stream.address)
     hip.device_synchronize()
@@ -287,6 +296,7 @@ def run_config5(args, V, hip):
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel_ms": ms_k},
         "stats": {"nodes": int(sum(x.nodes for x in st)), "rounds": int(st[0].rounds), "pairs": int(st[0].pairs)},
+        "autotune": tune,
     }
     print(json.dumps(res_out), flush=True)
 

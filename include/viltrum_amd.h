@@ -217,6 +217,26 @@ int vmx_refiner_grad(vmx_ctx* ctx, int32_t dim, const float* params, uint64_t n,
    RMSE where one exists.  The reference has no counterpart. */
 int vmx_exact_bins(vmx_ctx* ctx, const vmx_problem* p, const vmx_integrand* f, double* bins);
 
+/* ---- Launch-shape autotuner.  No counterpart in the reference (its parallel loops have
+   no launch shapes).  Runs the batch of vmx_integrate_batch once per candidate shape
+   (expansion workgroups per CU 2/4/8, refinement rounds per graph launch 4/8, the
+   per-bin kernel's workgroup-to-bin mapping contiguous per XCD or round-robin), keeps
+   the fastest for later calls of the same family, dimension, scalar type, refiner,
+   batch size and bin count (each within a factor of 2), and leaves that batch's
+   results in bins_device.  Launch shapes never change results: the random streams and
+   every summation order are independent of them. */
+typedef struct vmx_tune_report {
+    int32_t gen_blocks_per_cu; /* chosen: expansion workgroups per compute unit */
+    int32_t round_chunk;       /* chosen: refinement rounds per graph launch */
+    int32_t bins_xcd;          /* chosen: 1 = each XCD takes a contiguous run of bin tiles */
+    int32_t trials;            /* shapes timed */
+    double ms_best;            /* wall time of one call at the chosen shape */
+    double ms_default;         /* ... at the built-in default shape */
+} vmx_tune_report;
+
+int vmx_autotune(vmx_ctx* ctx, uint32_t n, const vmx_problem* problems, const vmx_integrand* integrands,
+                 void* const* bins_device, void* stream, vmx_tune_report* report);
+
 /* Evaluate the integrand on the device at n points x (n*D, row-major). */
 int vmx_eval(vmx_ctx* ctx, const vmx_integrand* f, int32_t scalar, uint64_t n, const double* x, double* out);
 

@@ -85,3 +85,44 @@ def test_repeated_calls_replay_the_same_rounds(ctx):
         again = np.zeros((40, 40), dtype=np.float32)
         V.integrate(integ, again, f, V.range_primary(3), ctx=ctx)
         assert np.array_equal(again, first), k
+
+
+def _run_batch(ctx, items, scalar=V.F32):
+    dt = np.float32 if scalar == V.F32 else np.float64
+    bufs = [hip.DeviceBuffer(int(np.prod(res)) * np.dtype(dt).itemsize) for _, res, _ in items]
+    probs = [integ.problem(f, res, V.range_primary(f.dim), scalar) for f, res, integ in items]
+    ctx.integrate_batch(probs, [f for f, _, _ in items], [b.address for b in bufs])
+    out = [b.to_host(np.zeros(res, dtype=dt)) for b, (_, res, _) in zip(bufs, items)]
+    for b in bufs:
+        b.free()
+    return out
+
+
+@pytest.mark.parametrize("env", [{"VMX_BINS_XCD": "1"}, {"VMX_GEN_PER_CU": "2", "VMX_ROUND_CHUNK": "4"},
+                                 {"VMX_GEN_PER_CU": "8", "VMX_BINS_XCD": "1"}], ids=["xcd", "g2r4", "g8xcd"])
+def test_launch_shapes_do_not_change_results(ctx, monkeypatch, env):
+    items = [it for it in _problems() if it[0].dim == 2]
+    base = _run_batch(ctx, items)
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    other = _run_batch(ctx, items)
+    for k in range(len(items)):
+        assert np.array_equal(base[k], other[k]), k
+
+
+def test_autotune_picks_a_shape_and_keeps_results(ctx):
+    items = [it for it in _problems() if it[0].dim == 2]
+    base = _run_batch(ctx, items)
+    bufs = [hip.DeviceBuffer(int(np.prod(res)) * 4) for _, res, _ in items]
+    probs = [integ.problem(f, res, V.range_primary(f.dim), V.F32) for f, res, integ in items]
+    rep = ctx.autotune(probs, [f for f, _, _ in items], [b.address for b in bufs])
+    assert rep["trials"] == 12 and rep["gen_blocks_per_cu"] in (2, 4, 8) and rep["round_chunk"] in (4, 8)
+    assert rep["bins_xcd"] in (0, 1) and 0 < rep["ms_best"] <= rep["ms_default"]
+    for k, (_, res, _) in enumerate(items):
+        assert np.array_equal(bufs[k].to_host(np.zeros(res, dtype=np.float32)), base[k]), k
+    for b in bufs:
+        b.free()
+    # later calls of the same class run at the chosen shape, with the same results
+    again = _run_batch(ctx, items)
+    for k in range(len(items)):
+        assert np.array_equal(again[k], base[k]), k

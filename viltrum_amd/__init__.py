@@ -37,7 +37,7 @@ REFINER_HEURISTIC, REFINER_LEARNED = 0, 1
 SYMBOLS = ["vmx_ctx_create", "vmx_ctx_destroy", "vmx_last_error", "vmx_integrate", "vmx_integrate_device",
            "vmx_integrate_batch", "vmx_export_regions", "vmx_integrate_mc", "vmx_eval", "vmx_refiner_size",
            "vmx_refiner_pairs", "vmx_refiner_train", "vmx_refiner_get", "vmx_refiner_set", "vmx_refiner_forward",
-           "vmx_refiner_grad", "vmx_exact_bins"]
+           "vmx_refiner_grad", "vmx_exact_bins", "vmx_autotune"]
 
 
 class VmxError(RuntimeError):
@@ -82,6 +82,14 @@ class TrainReport(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class TuneReport(C.Structure):
+    _fields_ = [("gen_blocks_per_cu", C.c_int32), ("round_chunk", C.c_int32), ("bins_xcd", C.c_int32),
+                ("trials", C.c_int32), ("ms_best", C.c_double), ("ms_default", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -101,6 +109,8 @@ def lib():
                                            C.POINTER(Stats)]
         L.vmx_integrate_batch.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                           C.c_void_p]
+        L.vmx_autotune.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.POINTER(TuneReport)]
         L.vmx_export_regions.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_void_p, C.c_void_p]
         L.vmx_eval.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_uint64, C.c_void_p, C.c_void_p]
@@ -178,6 +188,18 @@ class Context:
         self._check(lib().vmx_integrate_batch(self.ptr, n, C.cast(P, C.c_void_p), C.cast(F, C.c_void_p),
                                               C.cast(B, C.c_void_p), C.c_void_p(stream or 0), C.cast(S, C.c_void_p)))
         return list(S)
+
+    def autotune(self, problems, fs, bins_ptrs, stream=None):
+        """Time the batch at each candidate launch shape and keep the fastest for this
+        workload class (vmx_autotune); the batch's results stay in bins_ptrs."""
+        n = len(problems)
+        P = (Problem * n)(*problems)
+        F = (integrands.IntegrandDesc * n)(*[f.desc for f in fs])
+        B = (C.c_void_p * n)(*bins_ptrs)
+        rep = TuneReport()
+        self._check(lib().vmx_autotune(self.ptr, n, C.cast(P, C.c_void_p), C.cast(F, C.c_void_p),
+                                       C.cast(B, C.c_void_p), C.c_void_p(stream or 0), C.byref(rep)))
+        return rep.as_dict()
 
     def export_regions(self, D):
         n = C.c_uint64(0)

@@ -23,10 +23,13 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <array>
 #include <functional>
+#include <map>
 #include <random>
 #include <limits>
 #include <type_traits>
+#include <typeinfo>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -63,6 +66,14 @@ struct vmx_ctx {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
     } rounds;
+    // launch shapes: the autotuner's choices per workload class, and the shape a tuning
+    // trial forces (keys of the classes the trial ran are collected in tuned_keys)
+    struct Shape {
+        int gen_per_cu = 4, round_chunk = 8, bins_xcd = 0;
+    };
+    std::map<std::array<uint64_t, 6>, Shape> tune;
+    const Shape* trial = nullptr;
+    std::vector<std::array<uint64_t, 6>> tuned_keys;
     // last result (for vmx_export_regions): gathered from device memory on demand
     int last_D = 0;
     uint64_t last_n = 0;
@@ -108,6 +119,15 @@ void* ensure_pinned(vmx_ctx* ctx, size_t bytes) {
 }
 
 /* Carves aligned sub-arrays out of one allocation. */
+inline int ilog2(uint64_t x) {
+    int r = 0;
+    while (x > 1) {
+        x >>= 1;
+        ++r;
+    }
+    return r;
+}
+
 /* Sub-allocations of one buffer, 256-byte aligned; taking more than the buffer holds
    is a bug in the caller's size computation and throws. */
 struct Carve {
@@ -265,13 +285,25 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     int mb_max = 1;
     for (const auto& jb : jobs) mb_max = std::max(mb_max, int(jb.mb));
 
-    // launch shapes: expansion grid and lanes per bin (VMX_* overrides for tuning)
+    // launch shapes: a tuning trial's, else VMX_* environment overrides, else the
+    // autotuner's choice for this workload class, else the defaults
     int dev = ctx->device, n_cu = 0;
     HIP_CHECK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
-    uint32_t gen_blocks = uint32_t(n_cu) * 4;
-    if (const char* e = std::getenv("VMX_GEN_BLOCKS")) gen_blocks = std::max(1, std::atoi(e));
-    int round_chunk = 8;
-    if (const char* e = std::getenv("VMX_ROUND_CHUNK")) round_chunk = std::max(2, std::atoi(e) & ~1);
+    const std::array<uint64_t, 6> tune_key = {uint64_t(typeid(P).hash_code()), uint64_t(D), sizeof(T),
+                                              uint64_t(learn), uint64_t(ilog2(J)), uint64_t(ilog2(total_bins))};
+    vmx_ctx::Shape shape;
+    if (ctx->trial) {
+        shape = *ctx->trial;
+        ctx->tuned_keys.push_back(tune_key);
+    } else {
+        const auto it = ctx->tune.find(tune_key);
+        if (it != ctx->tune.end()) shape = it->second;
+        if (const char* e = std::getenv("VMX_GEN_PER_CU")) shape.gen_per_cu = std::max(1, std::atoi(e));
+        if (const char* e = std::getenv("VMX_ROUND_CHUNK")) shape.round_chunk = std::max(2, std::atoi(e) & ~1);
+        if (const char* e = std::getenv("VMX_BINS_XCD")) shape.bins_xcd = std::atoi(e) != 0;
+    }
+    const uint32_t gen_blocks = uint32_t(n_cu) * uint32_t(shape.gen_per_cu);
+    const int round_chunk = shape.round_chunk;
 
     uint32_t rounds = 0;
     uint32_t max_depth[VMX_MAX_DIM] = {};
@@ -715,14 +747,14 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     const int db = jobs[0].db;
     if (db == 1)
         vmx::k_bins1<T, D, 1, P, Math><<<bins_grid, 256, 0, st>>>(d_jobs, J, total_bins, ra, fa, d_off, d_lst, d_f,
-                                                                  d_rt);
+                                                                  d_rt, shape.bins_xcd);
     else if constexpr (D >= 2) {
         if (db == 2)
             vmx::k_bins1<T, D, 2, P, Math><<<bins_grid, 256, 0, st>>>(d_jobs, J, total_bins, ra, fa, d_off, d_lst,
-                                                                      d_f, d_rt);
+                                                                      d_f, d_rt, shape.bins_xcd);
         else if constexpr (D >= 3)
             vmx::k_bins1<T, D, 3, P, Math><<<bins_grid, 256, 0, st>>>(d_jobs, J, total_bins, ra, fa, d_off, d_lst,
-                                                                      d_f, d_rt);
+                                                                      d_f, d_rt, shape.bins_xcd);
     }
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipEventRecord(ctx->ev[1], st));
@@ -1387,6 +1419,78 @@ int vmx_integrate_batch(vmx_ctx* ctx, uint32_t n, const vmx_problem* problems, c
         }
         hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
         return integrate_groups(ctx, n, ps.data(), fs.data(), bins_device, ss.data(), st);
+    });
+}
+
+int vmx_autotune(vmx_ctx* ctx, uint32_t n, const vmx_problem* problems, const vmx_integrand* integrands,
+                 void* const* bins_device, void* stream, vmx_tune_report* report) {
+    return guarded(ctx, [&]() -> int {
+        if (n == 0) return int(VMX_OK);
+        if (!problems || !integrands || !bins_device) throw VmxError(VMX_ERR_INVALID, "null argument");
+        std::vector<const vmx_problem*> ps(n);
+        std::vector<const vmx_integrand*> fs(n);
+        std::vector<vmx_stats*> ss(n, nullptr);
+        for (uint32_t i = 0; i < n; ++i) {
+            validate(&problems[i], &integrands[i]);
+            if (!bins_device[i]) throw VmxError(VMX_ERR_INVALID, "null bins");
+            ps[i] = &problems[i];
+            fs[i] = &integrands[i];
+        }
+        hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+        // one call's wall time at a shape: the faster of two after a warm-up (the first
+        // call at a new shape also captures its round graph)
+        auto time_shape = [&](const vmx_ctx::Shape& sh) {
+            ctx->trial = &sh;
+            double best = 1e300;
+            for (int r = 0; r < 3; ++r) {
+                HIP_CHECK(hipStreamSynchronize(st));
+                const auto t0 = std::chrono::steady_clock::now();
+                integrate_groups(ctx, n, ps.data(), fs.data(), bins_device, ss.data(), st);
+                HIP_CHECK(hipStreamSynchronize(st));
+                if (r > 0) best = std::min(best, ms_since(t0));
+            }
+            ctx->trial = nullptr;
+            return best;
+        };
+        struct Reset {
+            vmx_ctx* c;
+            ~Reset() { c->trial = nullptr; }
+        } reset{ctx};
+        ctx->tuned_keys.clear();
+        const vmx_ctx::Shape def;
+        const double ms_def = time_shape(def);
+        vmx_ctx::Shape best = def;
+        double ms_best = ms_def;
+        int trials = 1;
+        for (int g : {2, 4, 8})
+            for (int rc : {4, 8})
+                for (int x : {0, 1}) {
+                    vmx_ctx::Shape sh;
+                    sh.gen_per_cu = g;
+                    sh.round_chunk = rc;
+                    sh.bins_xcd = x;
+                    if (g == def.gen_per_cu && rc == def.round_chunk && x == def.bins_xcd) continue;
+                    const double ms = time_shape(sh);
+                    ++trials;
+                    if (ms < ms_best) {
+                        ms_best = ms;
+                        best = sh;
+                    }
+                }
+        for (const auto& k : ctx->tuned_keys) ctx->tune[k] = best;
+        ctx->tuned_keys.clear();
+        // leave the chosen shape's results in the bins
+        integrate_groups(ctx, n, ps.data(), fs.data(), bins_device, ss.data(), st);
+        HIP_CHECK(hipStreamSynchronize(st));
+        if (report) {
+            report->gen_blocks_per_cu = best.gen_per_cu;
+            report->round_chunk = best.round_chunk;
+            report->bins_xcd = best.bins_xcd;
+            report->trials = trials;
+            report->ms_best = ms_best;
+            report->ms_default = ms_def;
+        }
+        return int(VMX_OK);
     });
 }
 

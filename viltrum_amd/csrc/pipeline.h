@@ -1071,6 +1071,16 @@ __device__ inline uint32_t job_of_bin(const Job<T>* jobs, uint32_t J, uint64_t b
     return lo;
 }
 
+/* Workgroups are dispatched to the 8 XCDs round-robin (workgroup w runs on XCD w % 8).
+   This maps them to tiles so that each XCD takes one contiguous run of tiles: neighbouring
+   bins share regions, so each XCD's L2 holds one neighbourhood instead of all of them.
+   A bijection of [0, G). */
+__device__ inline uint32_t xcd_tile(uint32_t w, uint32_t G) {
+    constexpr uint32_t NX = 8;
+    const uint32_t q = G / NX, r = G % NX, x = w % NX, i = w / NX;
+    return x * q + (x < r ? x : r) + i;
+}
+
 constexpr int LANE_MAXF = 16;  // footprints per bin cached in LDS (longer lists walk memory)
 
 /* A residual sample's position in its region.  FP64: pos_in, the reference's quotient.
@@ -1100,7 +1110,7 @@ template <typename T, int D, int DB, typename F, typename M>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D <= 3 && sizeof(T) == 4 ? 4 : 1))) k_bins1(const Job<T>* __restrict__ jobs, uint32_t J, uint64_t total_bins,
                                                RegionArrays<T> r, FootArrays<T> fa,
                                                const uint32_t* __restrict__ off, const uint32_t* __restrict__ lst,
-                                               const F* __restrict__ fs, JobRt* rt) {
+                                               const F* __restrict__ fs, JobRt* rt, int xcd) {
     constexpr int K = pow3(D);
     // per lane: first region and running region count of its first LANE_MAXF footprints
     __shared__ uint32_t s_first[LANE_MAXF][256];
@@ -1110,7 +1120,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D <= 3
     __shared__ alignas(16) unsigned char f_raw[sizeof(F)];
     __shared__ int block_job;
     const int tid = threadIdx.x;
-    const uint64_t base = uint64_t(blockIdx.x) * blockDim.x;
+    const uint64_t base = uint64_t(xcd ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x) * blockDim.x;
     if (tid == 0) {
         const uint64_t last = (base + blockDim.x < total_bins ? base + blockDim.x : total_bins) - 1;
         const uint32_t j0 = job_of_bin(jobs, J, base), j1 = job_of_bin(jobs, J, last);
