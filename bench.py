@@ -143,6 +143,27 @@ def run_config4(args, V, hip):
         hip.device_synchronize()
         return (time.perf_counter() - t0) * 1e3 / steps, st
 
+    if args.only_timed:
+        # the learned+batched step alone at a given (regions, spp), launch shapes from the
+        # VMX_* environment (no tuning runs): for kernel profiles of the timed step
+        regions, spp = (int(x) for x in args.only_timed.split(","))
+        ms, st = timed(run_batch, problems(regions, spp, True), args.steps)
+        flops = sum(st[i].evals_residual * I.flops_per_eval(f)[0] for i, f in enumerate(fs))
+        ms_k = float(st[0].ms_bins)
+        print(json.dumps({
+            "metric": METRIC, "value": nb * len(fs) / (ms * 1e-3), "unit": "bins/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+            "config": {"workload": "learned+batched step only: %d held-out integrands, 512x512 fp32 bins, "
+                                   "%d regions, %d spp" % (len(fs), regions, spp), "path": "learned+batched"},
+            "rmse_vs_fp64": rmse(),
+            "phase_ms": {k: float(getattr(st[0], k)) for k in ("ms_generate", "ms_select", "ms_stratify",
+                                                                "ms_bins", "ms_total")},
+            "roofline": {"bound": "valu", "kernel": "k_bins1 (per-bin CV + residual Monte Carlo, batched)",
+                         "achieved": flops / (ms_k * 1e-3) / 1e12, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                         "frac": flops / (ms_k * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, "traffic": None,
+                         "kernel_ms": ms_k}}), flush=True)
+        return
     budgets = (4096, 16384)
     # launch shapes of each path's workload class, picked by the library's autotuner
     # (setup, untimed, like the training)
@@ -333,6 +354,8 @@ def main():
                     help="config4: training budgets cycle through 1/4, 1/2, 1, 2 times this")
     ap.add_argument("--train-epochs", type=int, default=8)
     ap.add_argument("--train-stride", type=int, default=2, help="config4: keep every n-th generated region")
+    ap.add_argument("--only-timed", default="", metavar="REGIONS,SPP",
+                    help="config4: run only the learned+batched step at this point (for kernel profiles)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device (a known-size kernel for HBM counter scaling)")
     args = ap.parse_args()

@@ -54,7 +54,11 @@ def _check(g, o, rel_floor=0.0):
     (I.config1(), (16, 16), 127, 16),
     (I.c1_family(11, 5), (12, 10), 200, 8),
     (I.moving_blobs(4, 6), (8, 8), 150, 16),
-], ids=["config1", "c1fam", "blobs3d"])
+    # 5-D: every bin column meets hundreds of regions, so bin lists are far longer than
+    # the per-bin kernel's LDS cache and the region lookup searches the stored prefixes
+    (I.poly([(1.0, (1, 2, 0, 3, 1)), (2.0, (4, 0, 1, 0, 0))], 5, step=([1, 1, 1, 1, 1], 2.5, 0.25), name="poly5"),
+     (4, 4), 400, 8),
+], ids=["config1", "c1fam", "blobs3d", "poly5_long_lists"])
 @pytest.mark.parametrize("form,alpha", [(V.FORM_VR_OPTIMIZE, 1.0), (V.FORM_VR_FIXED, 0.5), (V.FORM_CV_ALPHA1, 1.0)],
                          ids=["vr_opt", "vr_fixed", "cv_alpha1"])
 def test_residual_statistics_fp64(ctx, f, res, it, spp, form, alpha):

@@ -417,7 +417,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
                 ctx->rounds.key.clear();
                 HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
                 for (int r = 0; r < round_chunk; ++r) {
-                    vmx::k_theta_round<T><<<J, 256, 0, st>>>(g, r & 1);
+                    vmx::k_theta_round<T><<<J, vmx::THETA_THREADS, 0, st>>>(g, r & 1);
                     launch_expand<T, D, P, Math>(learn, g, d_f, r & 1, gen_blocks, st);
                 }
                 hipGraph_t graph = nullptr;
@@ -434,7 +434,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
                 HIP_CHECK(hipGraphLaunch(ctx->rounds.exec, st));
             } else {
                 for (int r = 0; r < round_chunk; ++r) {
-                    vmx::k_theta_round<T><<<J, 256, 0, st>>>(g, r & 1);
+                    vmx::k_theta_round<T><<<J, vmx::THETA_THREADS, 0, st>>>(g, r & 1);
                     launch_expand<T, D, P, Math>(learn, g, d_f, r & 1, gen_blocks, st);
                 }
                 HIP_CHECK(hipGetLastError());
@@ -716,6 +716,8 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     mp.take<uint32_t>(size_t(pairs) + 1);                     // lists
     mp.take<uint32_t>(size_t(pairs) + 1);                     // copies of very long lists
     mp.take<uint32_t>(size_t(pairs) / vmx::SORT_SHORT + 2);   // queue of long lists
+    mp.take<uint32_t>(size_t(pairs) + 1);                     // prefix counts of long lists
+    mp.take<uint32_t>(size_t(pairs) + 1);                     // first regions of long lists
     mp.take<char>(scan_bins_bytes + 256);
     ensure(ctx->bins2, mp.used);
     Carve cp(ctx->bins2);
@@ -723,6 +725,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     uint32_t* d_lst = cp.take<uint32_t>(size_t(pairs) + 1);
     uint32_t* d_lst_tmp = cp.take<uint32_t>(size_t(pairs) + 1);
     uint32_t* d_qlong = cp.take<uint32_t>(size_t(pairs) / vmx::SORT_SHORT + 2);
+    vmx::ListPrefix lp{cp.take<uint32_t>(size_t(pairs) + 1), cp.take<uint32_t>(size_t(pairs) + 1)};
     void* d_scan_bins_tmp = cp.take<char>(scan_bins_bytes + 256);
     HIP_CHECK(hipMemsetAsync(d_cnt, 0, sizeof(uint32_t) * (total_bins + 1), st));
     HIP_CHECK(hipMemsetAsync(d_qlong, 0, sizeof(uint32_t), st));
@@ -734,8 +737,10 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
                                                 rocprim::plus<uint32_t>(), st)));
     if (pairs > 0) {
         vmx::k_bin_pass<T, true><<<fp_blocks, 256, 0, st>>>(d_jobs, fa, d_nfp, d_cnt, d_off, d_lst);
-        vmx::k_sort_lists<<<unsigned((total_bins + 255) / 256), 256, 0, st>>>(d_off, total_bins, d_lst, d_qlong);
+        vmx::k_sort_lists<T><<<unsigned((total_bins + 255) / 256), 256, 0, st>>>(d_off, total_bins, d_lst, d_qlong,
+                                                                                 fa, lp);
         vmx::k_sort_long<<<1024, 256, 0, st>>>(d_off, d_qlong, d_lst, d_lst_tmp);
+        vmx::k_list_prefix<T><<<1024, 256, 0, st>>>(d_off, d_qlong, d_lst, fa, lp);
     }
     HIP_CHECK(hipGetLastError());
     const double ms_str = ms_since(t_str);
@@ -747,14 +752,14 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     const int db = jobs[0].db;
     if (db == 1)
         vmx::k_bins1<T, D, 1, P, Math><<<bins_grid, 256, 0, st>>>(d_jobs, J, total_bins, ra, fa, d_off, d_lst, d_f,
-                                                                  d_rt, shape.bins_xcd);
+                                                                  d_rt, lp, shape.bins_xcd);
     else if constexpr (D >= 2) {
         if (db == 2)
             vmx::k_bins1<T, D, 2, P, Math><<<bins_grid, 256, 0, st>>>(d_jobs, J, total_bins, ra, fa, d_off, d_lst,
-                                                                      d_f, d_rt, shape.bins_xcd);
+                                                                      d_f, d_rt, lp, shape.bins_xcd);
         else if constexpr (D >= 3)
             vmx::k_bins1<T, D, 3, P, Math><<<bins_grid, 256, 0, st>>>(d_jobs, J, total_bins, ra, fa, d_off, d_lst,
-                                                                      d_f, d_rt, shape.bins_xcd);
+                                                                      d_f, d_rt, lp, shape.bins_xcd);
     }
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipEventRecord(ctx->ev[1], st));

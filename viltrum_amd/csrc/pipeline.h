@@ -60,48 +60,71 @@ struct TreeArrays {
 };
 
 /* ------------------------------------------------------------------ block helpers */
-/* In-place inclusive suffix sum of s[0..255] (s[t] = sum_{i>=t}); 256 threads. */
+/* In-place inclusive suffix sum of s[0..255] (s[t] = sum_{i>=t}); threads 0..255 of a
+   block of 256 or more, all of which call it.  Wave-level shuffles within each of the
+   four 64-entry quarters, then one exchange of the quarter totals. */
 __device__ inline void suffix_sum256(uint32_t* s) {
+    __shared__ uint32_t quarter[4];
     const int t = threadIdx.x;
-    for (int off = 1; off < 256; off <<= 1) {
-        const uint32_t add = (t + off < 256) ? s[t + off] : 0u;
-        __syncthreads();
-        s[t] += add;
-        __syncthreads();
+    const int lane = t & 63;
+    uint32_t v = t < 256 ? s[t] : 0u;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t u = __shfl_down(v, off, 64);
+        if (lane + off < 64) v += u;
     }
+    if (t < 256 && lane == 0) quarter[t >> 6] = v;
+    __syncthreads();
+    if (t < 256) {
+        uint32_t add = 0;
+        for (int q = (t >> 6) + 1; q < 4; ++q) add += quarter[q];
+        s[t] = v + add;
+    }
+    __syncthreads();
 }
 
 /* Bucket (16-bit key of b) holding the need-th largest b of one job, and the number of
-   nodes in strictly higher buckets.  Requires total >= need >= 1.  256 threads.  The
-   coarse counts (top 8 bits of the key) are the row sums of the fine histogram, formed
-   here rather than kept by atomics: nodes of a round cluster in few coarse buckets. */
+   nodes in strictly higher buckets.  Requires total >= need >= 1.  NT threads (a
+   multiple of 256), all of which call it.  The coarse counts (top 8 bits of the key)
+   are the row sums of the fine histogram, formed here (NT/256 threads per row) rather
+   than kept by atomics: nodes of a round cluster in few coarse buckets. */
+template <int NT>
 __device__ inline void boundary_bucket(const uint32_t* h16, uint32_t need, uint32_t* sh, uint32_t& k_out,
                                        uint32_t& above_out) {
+    static_assert(NT >= 256 && NT % 256 == 0, "whole rows per thread group");
+    constexpr int PER = NT / 256;
     const int t = threadIdx.x;
+    __shared__ uint32_t part[NT];
     __shared__ uint32_t sel_c, sel_above;
     {
-        const uint4* row = reinterpret_cast<const uint4*>(h16 + size_t(t) * 256);
+        const int row = t / PER, q = t % PER;
+        const uint4* p = reinterpret_cast<const uint4*>(h16 + size_t(row) * 256) + q * (64 / PER);
         uint32_t acc = 0;
-        for (int i = 0; i < 64; ++i) {
-            const uint4 q = row[i];
-            acc += q.x + q.y + q.z + q.w;
+        for (int i = 0; i < 64 / PER; ++i) {
+            const uint4 v = p[i];
+            acc += v.x + v.y + v.z + v.w;
         }
-        sh[t] = acc;
+        part[t] = acc;
+    }
+    __syncthreads();
+    if (t < 256) {
+        uint32_t a = 0;
+        for (int i = 0; i < PER; ++i) a += part[t * PER + i];
+        sh[t] = a;
     }
     __syncthreads();
     suffix_sum256(sh);
-    if (sh[t] >= need && (t == 255 || sh[t + 1] < need)) {
+    if (t < 256 && sh[t] >= need && (t == 255 || sh[t + 1] < need)) {
         sel_c = uint32_t(t);
         sel_above = (t == 255) ? 0u : sh[t + 1];
     }
     __syncthreads();
     const uint32_t c = sel_c, above = sel_above;
     __syncthreads();
-    sh[t] = h16[c * 256 + t];
+    if (t < 256) sh[t] = h16[c * 256 + t];
     __syncthreads();
     suffix_sum256(sh);
     const uint32_t need2 = need - above;
-    if (sh[t] >= need2 && (t == 255 || sh[t + 1] < need2)) {
+    if (t < 256 && sh[t] >= need2 && (t == 255 || sh[t + 1] < need2)) {
         sel_c = c * 256 + uint32_t(t);
         sel_above = above + ((t == 255) ? 0u : sh[t + 1]);
     }
@@ -483,8 +506,9 @@ __device__ inline void team_expand(const GenArgs<T>& g, const F* __restrict__ fs
    boundaries order the rounds, so no grid-wide barrier or cache flush is needed inside
    a kernel.  A round with no candidates exits at once; the host launches rounds in
    chunks and checks for remaining candidates between chunks. */
+constexpr int THETA_THREADS = 256;
 template <typename T>
-__global__ void __launch_bounds__(256) k_theta_round(GenArgs<T> g, int par) {
+__global__ void __launch_bounds__(THETA_THREADS) k_theta_round(GenArgs<T> g, int par) {
     __shared__ uint32_t sh[256];
     const uint32_t j = blockIdx.x;
     const Job<T>& jb = g.jobs[j];
@@ -503,7 +527,7 @@ __global__ void __launch_bounds__(256) k_theta_round(GenArgs<T> g, int par) {
         th = T(-INFINITY);
     } else {
         uint32_t k, above;
-        boundary_bucket(g.hist16 + size_t(j) * 65536, need, sh, k, above);
+        boundary_bucket<THETA_THREADS>(g.hist16 + size_t(j) * 65536, need, sh, k, above);
         key16_floor(k, th);
     }
     if (threadIdx.x == 0) g.theta[j] = th;
@@ -622,7 +646,7 @@ __global__ void k_sel_bound(const Job<T>* __restrict__ jobs, JobRt* rt, const ui
     const uint32_t j = blockIdx.x;
     const uint32_t need = jobs[j].N - 1;
     uint32_t k = NONE, above = 0;
-    if (need > 0) boundary_bucket(hist16 + size_t(j) * 65536, need, sh, k, above);
+    if (need > 0) boundary_bucket<256>(hist16 + size_t(j) * 65536, need, sh, k, above);
     if (threadIdx.x == 0) {
         rt[j].kbound = k;
         rt[j].above = above;
@@ -979,21 +1003,37 @@ __global__ void k_bin_pass(const Job<T>* __restrict__ jobs, FootArrays<T> f, con
         const uint64_t local = (uint64_t(bx[0] + a0) * r1 + (bx[2] + a1)) * r2 + (bx[4] + a2);
         const uint64_t bin = jb.bin_base + local;
         if constexpr (PLACE) {
-            const uint32_t k = atomicSub(&cnt[bin], 1u) - 1u;
-            lst[off[bin] + k] = w;
+            const uint32_t o = off[bin];
+            if (off[bin + 1] - o == 1) {  // the bin's only footprint: no cursor needed
+                lst[o] = w;
+            } else {
+                const uint32_t k = atomicSub(&cnt[bin], 1u) - 1u;
+                lst[o + k] = w;
+            }
         } else {
             atomicAdd(&cnt[bin], 1u);
         }
     }
 }
 
-constexpr uint32_t SORT_SHORT = 32;  // longer lists are sorted by a workgroup each
+constexpr uint32_t SORT_SHORT = 16;  // longer lists are sorted by a workgroup each
+constexpr int LANE_MAXF = 16;  // footprints per bin the per-bin kernel caches in LDS
+
+/* For lists longer than LANE_MAXF the per-bin kernel finds a sample's region by binary
+   search over the list's exclusive prefix region counts `pcum` and first regions
+   `pfirst`, stored beside the list. */
+struct ListPrefix {
+    uint32_t* pcum;
+    uint32_t* pfirst;
+};
 
 /* Ascending footprint ids in every bin's list: insertion sort per lane for short lists,
    staged in the lane's own LDS column (no barriers); longer ones are queued for
    k_sort_long. */
+template <typename T>
 __global__ void __launch_bounds__(256) k_sort_lists(const uint32_t* __restrict__ off, uint64_t total_bins,
-                                                    uint32_t* __restrict__ lst, uint32_t* __restrict__ qlong) {
+                                                    uint32_t* __restrict__ lst, uint32_t* __restrict__ qlong,
+                                                    FootArrays<T> fa, ListPrefix lp) {
     __shared__ uint32_t sv[SORT_SHORT][256];
     const int tid = threadIdx.x;
     const uint64_t b = uint64_t(blockIdx.x) * blockDim.x + tid;
@@ -1019,6 +1059,15 @@ __global__ void __launch_bounds__(256) k_sort_lists(const uint32_t* __restrict__
     }
     if (moved)
         for (uint32_t i = 0; i < n; ++i) lst[o + i] = sv[i][tid];
+    if (n > uint32_t(LANE_MAXF)) {
+        uint32_t c = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint32_t fi = sv[i][tid];
+            lp.pcum[o + i] = c;
+            lp.pfirst[o + i] = fa.first[fi];
+            c += fa.count[fi];
+        }
+    }
 }
 
 /* One workgroup per queued long list: every entry goes to its rank among the list's
@@ -1058,6 +1107,43 @@ __global__ void __launch_bounds__(256) k_sort_long(const uint32_t* __restrict__ 
     }
 }
 
+/* Prefix region counts of the queued long lists (sorted by now), one workgroup per list:
+   a block-wide scan per 256-entry chunk. */
+template <typename T>
+__global__ void __launch_bounds__(256) k_list_prefix(const uint32_t* __restrict__ off, const uint32_t* __restrict__ qlong,
+                                                     const uint32_t* __restrict__ lst, FootArrays<T> fa, ListPrefix lp) {
+    __shared__ uint32_t sc[256];
+    const uint32_t nq = qlong[0];
+    const int t = threadIdx.x;
+    for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
+        const uint32_t b = qlong[1 + q];
+        const uint32_t o = off[b], n = off[b + 1] - o;
+        uint32_t carry = 0;
+        for (uint32_t c0 = 0; c0 < n; c0 += 256) {
+            const uint32_t i = c0 + uint32_t(t);
+            uint32_t fi = 0, cnt = 0;
+            if (i < n) {
+                fi = lst[o + i];
+                cnt = fa.count[fi];
+            }
+            sc[t] = cnt;
+            __syncthreads();
+            for (int step = 1; step < 256; step <<= 1) {  // inclusive scan
+                const uint32_t add = t >= step ? sc[t - step] : 0u;
+                __syncthreads();
+                sc[t] += add;
+                __syncthreads();
+            }
+            if (i < n) {
+                lp.pcum[o + i] = carry + sc[t] - cnt;
+                lp.pfirst[o + i] = fa.first[fi];
+            }
+            carry += sc[255];
+            __syncthreads();
+        }
+    }
+}
+
 template <typename T>
 __device__ inline uint32_t job_of_bin(const Job<T>* jobs, uint32_t J, uint64_t b) {
     uint32_t lo = 0, hi = J;
@@ -1081,7 +1167,6 @@ __device__ inline uint32_t xcd_tile(uint32_t w, uint32_t G) {
     return x * q + (x < r ? x : r) + i;
 }
 
-constexpr int LANE_MAXF = 16;  // footprints per bin cached in LDS (longer lists walk memory)
 
 /* A residual sample's position in its region.  FP64: pos_in, the reference's quotient.
    FP32: x times the hardware reciprocal (about 1 ulp); only the polynomial's value at
@@ -1110,7 +1195,7 @@ template <typename T, int D, int DB, typename F, typename M>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D <= 3 && sizeof(T) == 4 ? 4 : 1))) k_bins1(const Job<T>* __restrict__ jobs, uint32_t J, uint64_t total_bins,
                                                RegionArrays<T> r, FootArrays<T> fa,
                                                const uint32_t* __restrict__ off, const uint32_t* __restrict__ lst,
-                                               const F* __restrict__ fs, JobRt* rt, int xcd) {
+                                               const F* __restrict__ fs, JobRt* rt, ListPrefix lp, int xcd) {
     constexpr int K = pow3(D);
     // per lane: first region and running region count of its first LANE_MAXF footprints
     __shared__ uint32_t s_first[LANE_MAXF][256];
@@ -1199,7 +1284,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D <= 3
         const bool sampling = spp > 0 && n > 0;
         if (spp > 0 && n == 0) atomicOr(&rt[j].flags, JF_EMPTY_BIN);
         // region index -> region: one footprint is one contiguous run of regions; else a
-        // binary search over the cached prefix counts, memory walk beyond
+        // binary search over the prefix counts, cached in LDS for the first LANE_MAXF
+        // footprints and stored beside the list beyond
         auto region_of = [&](uint32_t idx) -> uint32_t {
             if (nf == 1) return first0 + idx;
             if (idx < s_cum[nc][tid] || nc == nf) {
@@ -1213,13 +1299,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D <= 3
                 }
                 return s_first[lo][tid] + (idx - s_cum[lo][tid]);
             }
-            idx -= s_cum[nc][tid];
-            for (uint32_t k = nc;; ++k) {
-                const uint32_t fi = lst[o + k];
-                const uint32_t c = fa.count[fi];
-                if (idx < c || k + 1 == nf) return fa.first[fi] + (idx < c ? idx : c - 1);
-                idx -= c;
+            uint32_t lo = nc, hi = nf;  // lp.pcum[o + nc] = s_cum[nc] <= idx
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) / 2;
+                if (lp.pcum[o + mid] <= idx)
+                    lo = mid;
+                else
+                    hi = mid;
             }
+            return lp.pfirst[o + lo] + (idx - lp.pcum[o + lo]);
         };
         constexpr bool FP32 = sizeof(T) == 4;
         const T wn = T(factor * double(n));  // FP32 sample weight factor B * n
