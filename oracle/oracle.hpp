@@ -39,6 +39,8 @@
 #include <random>
 #include <stdexcept>
 #include <thread>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../include/viltrum_amd/integrands.h"
@@ -473,6 +475,108 @@ inline uint64_t out_index(int db, const uint64_t* res, const uint64_t* p) {
     return (p[0] * res[1] + p[1]) * res[2] + p[2];
 }
 
+/* The residual's integrand for one bin.  A plain integrand only consumes the draw that
+   monte_carlo_per_bin(rng, 1) takes from the bin's engine; an integrand with a
+   `bind(rng)` (the Fubini form below) builds its per-bin callable from it. */
+template <class F, class = void>
+struct HasBind : std::false_type {};
+template <class F>
+struct HasBind<F, std::void_t<decltype(std::declval<const F&>().bind(std::declval<std::mt19937&>()))>>
+    : std::true_type {};
+
+template <class F>
+auto bind_bin(const F& f, std::mt19937& rng) {
+    if constexpr (HasBind<F>::value) {
+        return f.bind(rng);
+    } else {
+        (void)rng();
+        return [&f](const auto* x) { return f(x); };
+    }
+}
+
+/* ---- Fubini control variate: integrator_crespo2021_infinite<RD>
+   (src/control-variates/integrator-crespo2021.h:24-44).  Regions and bins live in the
+   first RD dims; the node values and the residual's integrand integrate the remaining
+   dims of the (float) range by Monte Carlo (src/combination/fubini.h:51-54,
+   src/combination/regions-generator-fubini.h:16-22).  Float only: the reference's
+   rest-dims lambda takes std::array<float, ...>. */
+
+/* An engine after `copies` copy-constructions of the Monte Carlo object holding it: the
+   class declares only a copy constructor, which seeds the new engine from one draw of
+   the source's (src/monte-carlo/monte-carlo.h:29-33, monte-carlo-per-bin.h:28-32). */
+inline std::mt19937 copied_engine(std::mt19937 e, int copies) {
+    for (int k = 0; k < copies; ++k) {
+        const auto s = e();
+        e = std::mt19937(s);
+    }
+    return e;
+}
+
+/* Node values: MonteCarlo::integrate over the rest range with `mc` samples
+   (src/monte-carlo/monte-carlo.h:39-60, one bin): uniform float draws per rest dim, the
+   is_inside test, sol += f * (1 * volume / mc) into a float; one engine serves every
+   node value in evaluation order. */
+template <int RD, int DF, typename F>
+struct FubiniNodes {
+    const F& f;
+    float rlo[DF], rhi[DF];  // entries RD..DF-1: the rest range
+    float vol;               // Range<float, DF-RD>::volume: product of extents in float
+    uint64_t mc;
+    mutable std::mt19937 eng;
+    float operator()(const float* x) const {
+        const double factor = 1.0 * double(vol) / double(mc);
+        float sol = 0.0f;
+        for (uint64_t i = 0; i < mc; ++i) {
+            float xf[DF];
+            for (int d = 0; d < RD; ++d) xf[d] = x[d];
+            for (int d = RD; d < DF; ++d) {
+                std::uniform_real_distribution<float> dis(rlo[d], rhi[d]);
+                xf[d] = dis(eng);
+            }
+            bool inside = true;
+            for (int d = RD; d < DF && inside; ++d) inside = xf[d] >= rlo[d] && xf[d] <= rhi[d];
+            if (inside) sol += f(xf) * factor;
+        }
+        return sol;
+    }
+};
+
+/* Residual samples: monte_carlo_per_bin(rng, 1) seeds an engine from the bin's, the
+   capturing lambda copies it once more; each call is MonteCarloPerBin::integrate with
+   one bin and one sample (src/monte-carlo/monte-carlo-per-bin.h:37-62): rest dim 0
+   subranged to [lo, lo + (hi - lo) / 1], sol += f * (volume / 1). */
+template <int RD, int DF, typename F>
+struct FubiniSamples {
+    const F& f;
+    float rlo[DF], rhi[DF];
+    float vol;
+    int copies = 1;  // copies after the first seeding (the capture by value)
+    auto bind(std::mt19937& rng) const {
+        std::mt19937 e = copied_engine(std::mt19937(rng()), copies);
+        float slo[DF], shi[DF];
+        for (int d = RD; d < DF; ++d) {
+            slo[d] = rlo[d];
+            shi[d] = rhi[d];
+        }
+        const float drange = (rhi[RD] - rlo[RD]) / float(1);
+        slo[RD] = rlo[RD] + float(std::size_t(0)) * drange;
+        shi[RD] = rlo[RD] + float(std::size_t(1)) * drange;
+        const double factor = double(vol) / double(1);
+        const F* fp = &f;
+        return [fp, e, slo, shi, factor](const float* x) mutable {
+            float xf[DF];
+            for (int d = 0; d < RD; ++d) xf[d] = x[d];
+            for (int d = RD; d < DF; ++d) {
+                std::uniform_real_distribution<float> dis(slo[d], shi[d]);
+                xf[d] = dis(e);
+            }
+            float sol = 0.0f;
+            sol += (*fp)(xf) * factor;
+            return sol;
+        };
+    }
+};
+
 /* Per-bin control variate + residual, variance-reduction form
    (src/control-variates/regions-integrator-parallel-variance-reduction.h:32-109):
    per-bin mt19937 seeded in dim-0-fastest bin order from mt19937(seed); uniform
@@ -516,8 +620,9 @@ void integrate_regions_vr(const std::vector<Region<T, D>>& regs, const F& f, con
                 rng.seed(seeds[b]);
                 // monte_carlo_per_bin(rng, 1) binds the bin's engine by lvalue and seeds
                 // its own copy from one draw (…parallel-variance-reduction.h:69,
-                // src/monte-carlo/monte-carlo-per-bin.h:28); that draw is consumed here.
-                (void)rng();
+                // src/monte-carlo/monte-carlo-per-bin.h:28); that draw is consumed here,
+                // and a Fubini integrand takes its per-bin engine from it.
+                auto fb = bind_bin(f, rng);
                 T blo[D], bhi[D];
                 for (int d = 0; d < D; ++d) {
                     blo[d] = lo[d];
@@ -559,7 +664,7 @@ void integrate_regions_vr(const std::vector<Region<T, D>>& regs, const F& f, con
                             x[d] = dis(rng);
                         }
                         const T sfactor = box_volume<T, D>(slo, shi);
-                        acc.push(T(f(x) * double(factor) * rrfactor * sfactor),
+                        acc.push(T(fb(x) * double(factor) * rrfactor * sfactor),
                                  T(approximation_at(r, x) * double(factor) * rrfactor * sfactor));
                     }
                 }

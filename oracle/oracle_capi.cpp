@@ -258,6 +258,57 @@ struct HeuristicJob {
 
 }  // namespace
 
+/* integrator_crespo2021_infinite<RD> (src/control-variates/integrator-crespo2021.h:24-44)
+   over a float range of DF dims: regions and bins over the first RD, node values and the
+   residual integrate the rest by Monte Carlo (see vmo::FubiniNodes / FubiniSamples).
+   gen_copies / bin_copies: copy-constructions of the Monte Carlo objects between their
+   seeding and use (each reseeds from one draw). */
+template <int RD, int DF>
+int fubini_run(const vmx_integrand* f, const double* lo, const double* hi, int db, const uint64_t* res,
+               uint64_t iterations, uint64_t mc, uint64_t spp, uint64_t seed, int gen_copies, int bin_copies,
+               int nthreads, double* bins_inout, vmo::Stats* st) {
+    if (db < 1 || db > RD || db > 3) {
+        g_err = "bin dims must be 1..min(3, region dims)";
+        return -1;
+    }
+    if (mc < 1) {
+        g_err = "mc samples must be >= 1";
+        return -1;
+    }
+    float flo[DF], fhi[DF];
+    load_box<float, DF>(lo, hi, flo, fhi);
+    return with_f<float, DF>(f, [&](const auto& fn) {
+        using FN = std::decay_t<decltype(fn)>;
+        float vol = 1.0f;
+        for (int d = RD; d < DF; ++d) vol *= (fhi[d] - flo[d]);
+        vmo::FubiniNodes<RD, DF, FN> nodes{fn, {}, {}, vol, mc,
+                                           vmo::copied_engine(std::mt19937(2 * seed + 1), gen_copies)};
+        vmo::FubiniSamples<RD, DF, FN> samples{fn, {}, {}, vol, bin_copies};
+        for (int d = 0; d < DF; ++d) {
+            nodes.rlo[d] = samples.rlo[d] = flo[d];
+            nodes.rhi[d] = samples.rhi[d] = fhi[d];
+        }
+        Timer tm;
+        uint64_t ev = 0;
+        auto regs = vmo::generate<float, RD>(nodes, flo, fhi, vmo::HEUR_SIZE_RELATIVE, iterations, &ev);
+        const double tg = tm.lap();
+        uint64_t nb = 1;
+        for (int i = 0; i < db; ++i) nb *= res[i];
+        std::vector<float> out(nb);
+        for (uint64_t b = 0; b < nb; ++b) out[b] = float(bins_inout[b]);
+        vmo::integrate_regions_vr<float, RD, decltype(samples), vmo::AccOptimize<float>>(
+            regs, samples, flo, fhi, db, res, spp, seed, 1.0, nthreads, out.data(), st);
+        for (uint64_t b = 0; b < nb; ++b) bins_inout[b] = double(out[b]);
+        if (st) {
+            st->regions = regs.size();
+            st->evals_regions = ev * mc;
+            st->t_generate = tg;
+            st->t_bins = tm.lap();
+        }
+        return 0;
+    });
+}
+
 extern "C" {
 
 typedef struct vmo_stats {
@@ -308,6 +359,31 @@ int vmo_integrate_regions(int scalar_bytes, int integrator, double alpha, const 
     vmo::Stats s;
     int rc = dispatch<RegionsJob>(scalar_bytes, f->dim, integrator, alpha, f, lo, hi, nregions, reg_lo, reg_hi,
                                   reg_nodes, db, res, spp, seed, nthreads, bins_inout, &s);
+    copy_stats(s, st);
+    return rc;
+}
+
+/* The Fubini form above for (rd, f->dim) with 1 <= rd <= 3, rd < f->dim <= 5. */
+int vmo_integrate_fubini(int rd, const vmx_integrand* f, const double* lo, const double* hi, int db,
+                         const uint64_t* res, uint64_t iterations, uint64_t mc, uint64_t spp, uint64_t seed,
+                         int gen_copies, int bin_copies, int nthreads, double* bins_inout, vmo_stats* st) {
+    vmo::Stats s;
+    int rc = -1;
+    try {
+#define VMO_FB(R, F)                                                                                          \
+    if (rd == R && f->dim == F)                                                                               \
+        rc = fubini_run<R, F>(f, lo, hi, db, res, iterations, mc, spp, seed, gen_copies, bin_copies, nthreads, \
+                              bins_inout, &s);                                                                \
+    else
+        VMO_FB(1, 2) VMO_FB(1, 3) VMO_FB(1, 5) VMO_FB(2, 3) VMO_FB(2, 4) VMO_FB(2, 5) VMO_FB(3, 4) VMO_FB(3, 5) {
+            g_err = "fubini: region dims 1..3 below the integrand's dims (2..5)";
+            rc = -1;
+        }
+#undef VMO_FB
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        rc = -3;
+    }
     copy_stats(s, st);
     return rc;
 }

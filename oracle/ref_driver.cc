@@ -183,6 +183,38 @@ int integrate_any(int integrator, int heuristic, double alpha, const vmx_integra
     return -1;
 }
 
+/* integrator_crespo2021_infinite<RD>(iterations, mc, spp, seed)
+   (src/control-variates/integrator-crespo2021.h:24-44) over a DF-dimensional float range:
+   regions over the first RD dims, node values Monte Carlo over the rest.  The reference
+   compiles it for float ranges only (the rest-dims lambda of
+   src/combination/fubini.h:51-54 takes std::array<float, ...>). */
+template <std::size_t RD, std::size_t DF, std::size_t DB>
+int fubini_db(const vmx_integrand* f, const double* lo, const double* hi, const uint64_t* res, uint64_t iterations,
+              uint64_t mc, uint64_t spp, uint64_t seed, double* bins_inout) {
+    auto range = make_range<float, DF>(lo, hi);
+    std::array<std::size_t, DB> r;
+    std::size_t nb = 1;
+    for (std::size_t i = 0; i < DB; ++i) {
+        r[i] = res[i];
+        nb *= res[i];
+    }
+    std::vector<float> out(nb);
+    for (std::size_t b = 0; b < nb; ++b) out[b] = float(bins_inout[b]);
+    auto bins = [&](const std::array<std::size_t, DB>& p) -> float& {
+        std::size_t k = p[0];
+        for (std::size_t i = 1; i < DB; ++i) k = k * r[i] + p[i];
+        return out[k];
+    };
+    int rc = with_f<float, DF>(f, [&](const auto& fn) {
+        auto in = viltrum::integrator_crespo2021_infinite<RD>(std::size_t(iterations), std::size_t(mc),
+                                                               std::size_t(spp), std::size_t(seed));
+        viltrum::integrate(in, bins, r, fn, range);
+        return 0;
+    });
+    for (std::size_t b = 0; b < nb; ++b) bins_inout[b] = double(out[b]);
+    return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -243,6 +275,26 @@ int vmr_integrate(int scalar_bytes, int integrator, int heuristic, double alpha,
                 return -1;
         }
 #undef VMR_I
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -3;
+    }
+}
+
+/* integrator_crespo2021_infinite<rd> on a float range of f->dim dims with bins over the
+   first db of the rd region dims: (rd, dim, db) in {(1, 2, 1), (2, 3, 2), (2, 5, 2)}. */
+int vmr_integrate_fubini(int rd, const vmx_integrand* f, const double* lo, const double* hi, int db,
+                         const uint64_t* res, uint64_t iterations, uint64_t mc, uint64_t spp, uint64_t seed,
+                         double* bins_inout) {
+    try {
+        if (rd == 1 && f->dim == 2 && db == 1)
+            return fubini_db<1, 2, 1>(f, lo, hi, res, iterations, mc, spp, seed, bins_inout);
+        if (rd == 2 && f->dim == 3 && db == 2)
+            return fubini_db<2, 3, 2>(f, lo, hi, res, iterations, mc, spp, seed, bins_inout);
+        if (rd == 2 && f->dim == 5 && db == 2)
+            return fubini_db<2, 5, 2>(f, lo, hi, res, iterations, mc, spp, seed, bins_inout);
+        g_err = "fubini: (region dims, dims, bin dims) not built";
+        return -1;
     } catch (const std::exception& e) {
         g_err = e.what();
         return -3;

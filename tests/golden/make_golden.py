@@ -53,6 +53,17 @@ BIN_CASES = [  # (integrand, scalar, form, heuristic, res, iterations, spp, seed
     ("lobes5", 8, 0, 0, (6, 6), 40, 2, 9, 1.0),
 ]
 
+FUBINI_CASES = [  # (integrand, region dims, res, iterations, mc samples, spp, seed): integrator_crespo2021_infinite
+    ("blobs3d", 2, (8, 8), 150, 3, 4, 11),
+    ("blobs3d", 2, (12, 10), 60, 1, 0, 4),
+    ("lobes5", 2, (5, 6), 60, 2, 3, 5),
+    ("poly2", 1, (12,), 80, 5, 6, 2),
+]
+
+
+def fubini_key(name, rd, res, it, mc, spp, seed):
+    return "fubini/%s/rd%d/r%s/it%d/mc%d/spp%d/seed%d" % (name, rd, "x".join(map(str, res)), it, mc, spp, seed)
+
 
 def main():
     if O.ref() is None:
@@ -75,6 +86,9 @@ def main():
         key = "bins/%s/s%d/f%d/h%d/r%s/it%d/spp%d/seed%d/a%g" % (name, scalar, form, heur, "x".join(map(str, res)),
                                                                 it, spp, seed, alpha)
         out[key] = b
+    for name, rd, res, it, mc, spp, seed in FUBINI_CASES:
+        b, _ = O.integrate_fubini(fs[name], rd, res, it, mc, spp, seed, use_ref=True)
+        out[fubini_key(name, rd, res, it, mc, spp, seed)] = b
     path = os.path.join(HERE, "reference_fixtures.npz")
     np.savez_compressed(path, **out)
     print("wrote %s (%d arrays, %.0f KB)" % (path, len(out), os.path.getsize(path) / 1024))

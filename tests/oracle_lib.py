@@ -137,6 +137,37 @@ def integrate(f, res, iterations, spp, seed, scalar=8, integrator=INT_VR_OPTIMIZ
     return bins.reshape(res), st
 
 
+FUBINI_GEN_COPIES, FUBINI_BIN_COPIES = 3, 1  # copy-constructions of the Monte Carlo objects (see oracle.hpp)
+
+
+def integrate_fubini(f, rd, res, iterations, mc, spp, seed, lo=None, hi=None, nthreads=1, use_ref=False,
+                     gen_copies=FUBINI_GEN_COPIES, bin_copies=FUBINI_BIN_COPIES):
+    """integrator_crespo2021_infinite<rd>(iterations, mc, spp, seed) over f's float range:
+    regions and bins over the first rd dims.  Bins in C order; returns (bins, stats)."""
+    D = f.dim
+    lo, hi = _box(D, lo, hi)
+    res = tuple(int(r) for r in res)
+    r = np.ascontiguousarray(np.array(res, dtype=np.uint64))
+    bins = np.zeros(int(np.prod(res)))
+    if use_ref:
+        lib = ref()
+        rc = lib.vmr_integrate_fubini(C.c_int(rd), f.ptr(), _ptr(lo), _ptr(hi), C.c_int(len(res)), _ptr(r, _up),
+                                      C.c_uint64(iterations), C.c_uint64(mc), C.c_uint64(spp), C.c_uint64(seed),
+                                      _ptr(bins))
+        if rc < 0:
+            raise RuntimeError(lib.vmr_last_error().decode())
+        return bins.reshape(res), None
+    lib = oracle()
+    st = OracleStats()
+    rc = lib.vmo_integrate_fubini(C.c_int(rd), f.ptr(), _ptr(lo), _ptr(hi), C.c_int(len(res)), _ptr(r, _up),
+                                  C.c_uint64(iterations), C.c_uint64(mc), C.c_uint64(spp), C.c_uint64(seed),
+                                  C.c_int(gen_copies), C.c_int(bin_copies), C.c_int(nthreads), _ptr(bins),
+                                  C.byref(st))
+    if rc < 0:
+        raise RuntimeError(lib.vmo_last_error().decode())
+    return bins.reshape(res), st
+
+
 def integrate_regions(f, regs_lo, regs_hi, regs_nodes, res, spp, seed, scalar=8, integrator=INT_VR_OPTIMIZE,
                       alpha=1.0, lo=None, hi=None, nthreads=1, bins_in=None):
     """Per-bin phase of the oracle on a caller-given region list."""

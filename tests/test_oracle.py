@@ -47,6 +47,38 @@ def test_integrate_matches_reference_fixture(case):
     assert np.array_equal(b, FIX[key])
 
 
+@pytest.mark.parametrize("case", MG.FUBINI_CASES, ids=lambda c: "%s-rd%d-mc%d-spp%d" % (c[0], c[1], c[4], c[5]))
+def test_fubini_matches_reference_fixture(case):
+    """integrator_crespo2021_infinite: node values Monte Carlo over the dims after the
+    regions', every engine seeded as the reference's Monte Carlo objects are copied."""
+    name, rd, res, it, mc, spp, seed = case
+    b, st = O.integrate_fubini(FS[name], rd, res, it, mc, spp, seed, nthreads=2)
+    assert st.regions == it + 1
+    assert np.array_equal(b, FIX[MG.fubini_key(name, rd, res, it, mc, spp, seed)])
+
+
+@pytest.mark.skipif(O.ref() is None, reason="reference build absent (no /root/reference where this runs)")
+@pytest.mark.parametrize("seed", range(3))
+def test_fubini_oracle_matches_live_reference(seed):
+    rng = np.random.default_rng(40 + seed)
+    f = I.moving_blobs(50 + seed, int(rng.integers(1, 9)))
+    res = (int(rng.integers(3, 17)), int(rng.integers(3, 17)))
+    it, mc, spp = int(rng.integers(0, 250)), int(rng.integers(1, 5)), int(rng.integers(0, 9))
+    a, _ = O.integrate_fubini(f, 2, res, it, mc, spp, seed)
+    b, _ = O.integrate_fubini(f, 2, res, it, mc, spp, seed, use_ref=True)
+    assert np.array_equal(a, b)
+
+
+def test_fubini_with_a_tensor_quadratic_in_the_region_dims_is_exact():
+    """f(x, y, t) = x^2 y + y + 3 t with regions over (x, y): each node value is
+    x^2 y + y plus a Monte Carlo estimate of the integral of 3 t (1.5), so with spp = 0
+    the mean bin is 1/6 + 1/2 + 1.5 up to the node values' Monte Carlo error."""
+    f = I.poly([(1.0, (2, 1, 0)), (1.0, (0, 1, 0)), (3.0, (0, 0, 1))], 3)
+    b, _ = O.integrate_fubini(f, 2, (4, 4), 30, 4000, 0, 3)
+    # sum over bins / B = integral over [0,1]^2 of x^2 y + y + E[3 t] = 1/6 + 1/2 + 1.5
+    assert abs(b.mean() - (1.0 / 6 + 0.5 + 1.5)) < 5e-3
+
+
 @pytest.mark.skipif(O.ref() is None, reason="reference build absent (no /root/reference where this runs)")
 @pytest.mark.parametrize("seed", range(4))
 def test_oracle_matches_live_reference(seed):
