@@ -43,7 +43,8 @@ extern "C" {
 enum vmx_family {
     VMX_FAMILY_POLY = 1,
     VMX_FAMILY_BLOBS = 2,
-    VMX_FAMILY_LOBES5 = 3
+    VMX_FAMILY_LOBES5 = 3,
+    VMX_FAMILY_FUBINI = 4   /* an inner integrand, its trailing dims integrated by Monte Carlo */
 };
 
 typedef struct vmx_poly_params {
@@ -74,6 +75,22 @@ typedef struct vmx_lobes5_params {
     double k[5];                            /* integer frequencies of the cosine */
     double amp;                             /* cosine amplitude */
 } vmx_lobes5_params;
+
+/* The Fubini form of integrator_crespo2021_infinite<DIM> (src/control-variates/
+   integrator-crespo2021.h:24-44): regions and bins over the descriptor's `dim` leading
+   dims of an inner integrand of inner_dim dims; its remaining dims are integrated by
+   Monte Carlo over [rest_lo, rest_hi] (entries dim .. inner_dim-1), with mc_samples
+   uniform samples per region node value and one per residual sample.  FP32 only, as
+   the reference. */
+typedef struct vmx_fubini_params {
+    int32_t inner_family;                   /* VMX_FAMILY_POLY, _BLOBS (3-D) or _LOBES5 */
+    int32_t inner_dim;                      /* > the descriptor's dim, <= 5 */
+    uint32_t mc_samples;                    /* >= 1 */
+    uint32_t seed;                          /* stream of the Monte Carlo samples */
+    double rest_lo[VMX_MAX_DIM];            /* indexed by inner dim */
+    double rest_hi[VMX_MAX_DIM];
+    const void* inner;                      /* host pointer to the inner family's parameter block */
+} vmx_fubini_params;
 
 /* Integrand descriptor handed to every entry point. */
 typedef struct vmx_integrand {

@@ -353,6 +353,10 @@ class Integrator:
         self.refiner = refiner
         self.last_stats = None
 
+    def prepare(self, f, rng):
+        """(integrand descriptor, range) the problem is posed on."""
+        return f, rng
+
     def problem(self, f, res, rng, scalar):
         if rng.dims != f.dim:
             raise ValueError("range has %d dims, integrand %d" % (rng.dims, f.dim))
@@ -375,6 +379,28 @@ class Integrator:
         p.spp = self.spp
         p.seed = self.seed
         return p
+
+
+class FubiniIntegrator(Integrator):
+    """integrator_crespo2021_infinite<rd>(iterations, mc_samples, spp, seed)
+    (src/control-variates/integrator-crespo2021.h:24-44): crespo2021 over the first rd
+    dims of the range, the remaining dims integrated by Monte Carlo (mc_samples per region
+    node value, one per residual sample).  FP32 bins only, as the reference."""
+
+    def __init__(self, rd, iterations, mc_samples, spp, seed=0, **kw):
+        super().__init__(iterations, spp, seed, HEUR_SIZE_RELATIVE, FORM_VR_OPTIMIZE, **kw)
+        self.rd = int(rd)
+        self.mc_samples = int(mc_samples)
+
+    def prepare(self, f, rng):
+        if rng.dims != f.dim:
+            raise ValueError("range has %d dims, integrand %d" % (rng.dims, f.dim))
+        g = integrands.fubini(f, self.rd, self.mc_samples, 2 * self.seed + 1, rng.lo, rng.hi)
+        return g, Range(rng.lo[:self.rd], rng.hi[:self.rd])
+
+
+def integrator_crespo2021_infinite(rd, iterations, mc_samples, spp, seed=0, **kw):
+    return FubiniIntegrator(rd, iterations, mc_samples, spp, seed, **kw)
 
 
 def integrator_crespo2021(iterations, spp, seed=0, **kw):
@@ -404,8 +430,9 @@ def integrate(integrator, *args, ctx=None):
         if bins.dtype not in (np.float32, np.float64) or not bins.flags.c_contiguous:
             raise TypeError("bins must be a C-contiguous float32 or float64 array")
         scalar = F32 if bins.dtype == np.float32 else F64
-        p = integrator.problem(f, bins.shape, rng, scalar)
-        integrator.last_stats = ctx.integrate_host(p, f, bins)
+        g, grng = integrator.prepare(f, rng)
+        p = integrator.problem(g, bins.shape, grng, scalar)
+        integrator.last_stats = ctx.integrate_host(p, g, bins)
         return bins
     f, rng = args[:2]
     dtype = args[2] if len(args) > 2 else np.float64

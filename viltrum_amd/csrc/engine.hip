@@ -180,6 +180,10 @@ template <typename T>
 struct ParamOf<vmx::Lobes5<T>> {
     using type = vmx_lobes5_params;
 };
+template <typename T, typename Inner, int DF, int RD>
+struct ParamOf<vmx::Fubini<T, Inner, DF, RD>> {
+    using type = vmx_fubini_params;
+};
 
 double ms_since(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -275,6 +279,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     bool learn = false;
     for (const auto& jb : jobs) learn |= jb.learn != 0;
     if (learn) {
+        if (vmx::IsFubini<P>::value) throw VmxError(VMX_ERR_UNSUPPORTED, "learned refiner: not for the Fubini form");
         if (!std::is_same<T, float>::value || D > LEARN_MAX_DIM)
             throw VmxError(VMX_ERR_UNSUPPORTED, "learned refiner: FP32 problems of dimension 1..4 only");
         if (!ctx->nn_ok[D]) throw VmxError(VMX_ERR_UNSUPPORTED, "learned refiner: no network for this dimension");
@@ -482,7 +487,9 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     }
     const double ms_gen = ms_since(t_start);
 
-    if (pairs_req) {
+    if constexpr (vmx::IsFubini<P>::value) {
+        if (pairs_req) throw VmxError(VMX_ERR_UNSUPPORTED, "training pairs: not for the Fubini form");
+    } else if (pairs_req) {
         PairsReq* pairs = pairs_req;
         // training pairs: every node of every job, job-major in slot order
         std::vector<uint64_t> base(J);
@@ -853,6 +860,25 @@ int dispatch_T(vmx_ctx* ctx, int family, int D, const std::vector<JobIn>& jobs, 
         case VMX_FAMILY_LOBES5:
             if (D == 5) return run_batch<T, 5, vmx::Lobes5<T>, Math>(ctx, jobs, st, pairs);
             break;
+        case VMX_FAMILY_FUBINI:
+            // integrator_crespo2021_infinite is float-only in the reference; built here for
+            // (region dims, inner family, inner dims) = (2, blobs, 3), (2, lobes5, 5),
+            // (2, poly, 3), (1, poly, 2)
+            if constexpr (std::is_same<T, float>::value) {
+                const auto* fp = static_cast<const vmx_fubini_params*>(jobs[0].params);
+                const int fam = fp->inner_family, dd = fp->inner_dim;
+                if (D == 2 && fam == VMX_FAMILY_BLOBS && dd == 3)
+                    return run_batch<T, 2, vmx::Fubini<T, vmx::Blobs<T>, 3, 2>, Math>(ctx, jobs, st, pairs);
+                if (D == 2 && fam == VMX_FAMILY_LOBES5 && dd == 5)
+                    return run_batch<T, 2, vmx::Fubini<T, vmx::Lobes5<T>, 5, 2>, Math>(ctx, jobs, st, pairs);
+                if (D == 2 && fam == VMX_FAMILY_POLY && dd == 3)
+                    return run_batch<T, 2, vmx::Fubini<T, vmx::Poly<T>, 3, 2>, Math>(ctx, jobs, st, pairs);
+                if (D == 1 && fam == VMX_FAMILY_POLY && dd == 2)
+                    return run_batch<T, 1, vmx::Fubini<T, vmx::Poly<T>, 2, 1>, Math>(ctx, jobs, st, pairs);
+            } else {
+                throw VmxError(VMX_ERR_UNSUPPORTED, "Fubini form: FP32 only (as the reference)");
+            }
+            break;
         default: break;
     }
     throw VmxError(VMX_ERR_UNSUPPORTED, "integrand family / dimension not built");
@@ -870,8 +896,13 @@ int integrate_groups(vmx_ctx* ctx, uint32_t n, const vmx_problem* const* ps, con
     for (uint32_t i = 0; i < n; ++i) {
         if (done[i]) continue;
         std::vector<JobIn> group;
+        auto inner = [](const vmx_integrand* f) {
+            if (f->family != VMX_FAMILY_FUBINI) return std::pair<int, int>(0, 0);
+            const auto* fp = static_cast<const vmx_fubini_params*>(f->params);
+            return std::pair<int, int>(fp->inner_family, fp->inner_dim);
+        };
         for (uint32_t k = i; k < n; ++k)
-            if (!done[k] && fs[k]->family == fs[i]->family && fs[k]->dim == fs[i]->dim) {
+            if (!done[k] && fs[k]->family == fs[i]->family && fs[k]->dim == fs[i]->dim && inner(fs[k]) == inner(fs[i])) {
                 group.push_back({ps[k], fs[k]->params, bins[k], stats[k]});
                 done[k] = 1;
             }
@@ -904,6 +935,19 @@ void validate(const vmx_problem* p, const vmx_integrand* f) {
         throw VmxError(VMX_ERR_INVALID, "unknown refiner");
     for (int d = 0; d < f->dim; ++d)
         if (!(p->hi[d] > p->lo[d])) throw VmxError(VMX_ERR_INVALID, "range must have hi > lo");
+    if (f->family == VMX_FAMILY_FUBINI) {
+        const auto* fp = static_cast<const vmx_fubini_params*>(f->params);
+        if (!fp->inner) throw VmxError(VMX_ERR_INVALID, "Fubini form: null inner parameters");
+        if (fp->inner_dim <= f->dim || fp->inner_dim > 5 || f->dim > 3)
+            throw VmxError(VMX_ERR_INVALID, "Fubini form: 1..3 region dims below the inner integrand's 2..5");
+        if (fp->mc_samples < 1 || fp->mc_samples >= 0x80000000u)
+            throw VmxError(VMX_ERR_INVALID, "Fubini form: mc_samples must be 1..2^31-1");
+        for (int d = f->dim; d < fp->inner_dim; ++d)
+            if (!(fp->rest_hi[d] > fp->rest_lo[d])) throw VmxError(VMX_ERR_INVALID, "Fubini form: rest range must have hi > lo");
+        if (p->scalar != VMX_F32) throw VmxError(VMX_ERR_UNSUPPORTED, "Fubini form: FP32 only (as the reference)");
+        if (p->refiner != VMX_REFINER_HEURISTIC)
+            throw VmxError(VMX_ERR_UNSUPPORTED, "Fubini form: heuristic refinement only");
+    }
 }
 
 int guarded(vmx_ctx* ctx, const std::function<int()>& body) {

@@ -22,6 +22,7 @@
 
 #include <type_traits>
 
+#include "fubini.h"
 #include "kernels.h"
 #include "refiner.h"
 
@@ -1329,7 +1330,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D <= 3
                 x[d] = mul_add<FP32>(u.coord(d), ihi[d] - ilo[d], ilo[d]);
                 t[d] = sample_pos(x[d], rl, rh);
             }
-            const T fx = bj >= 0 ? f_lds.template eval<M, D>(x) : fs[j].template eval<M, D>(x);
+            const T fx = bj >= 0 ? residual_eval<M, D>(f_lds, x) : residual_eval<M, D>(fs[j], x);
             const T px = eval_monomials<T, D, FP32>(r.coef + size_t(reg) * K, t);
             const T vol = box_volume<T, D>(ilo, ihi);
             if (jb.form == VMX_FORM_CV_ALPHA1) {

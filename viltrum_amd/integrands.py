@@ -17,6 +17,7 @@ MAX_BLOBS = 64
 FAMILY_POLY = 1
 FAMILY_BLOBS = 2
 FAMILY_LOBES5 = 3
+FAMILY_FUBINI = 4
 
 
 class PolyParams(C.Structure):
@@ -49,6 +50,18 @@ class Lobes5Params(C.Structure):
         ("step_d", C.c_double * 2),
         ("k", C.c_double * 5),
         ("amp", C.c_double),
+    ]
+
+
+class FubiniParams(C.Structure):
+    _fields_ = [
+        ("inner_family", C.c_int32),
+        ("inner_dim", C.c_int32),
+        ("mc_samples", C.c_uint32),
+        ("seed", C.c_uint32),
+        ("rest_lo", C.c_double * MAX_DIM),
+        ("rest_hi", C.c_double * MAX_DIM),
+        ("inner", C.c_void_p),
     ]
 
 
@@ -125,6 +138,26 @@ def lobes5(centre, inv2s2, step_n, step_d, k, amp, name="lobes5"):
         p.k[j] = k[j]
     p.amp = amp
     return Integrand(FAMILY_LOBES5, 5, p, name)
+
+
+def fubini(inner, rd, mc_samples, seed, rest_lo, rest_hi, name=None):
+    """The Fubini form of integrator_crespo2021_infinite<rd>: regions and bins over the
+    first rd dims of `inner`; its other dims (rest_lo/hi, indexed by inner dim) are
+    integrated by Monte Carlo, mc_samples per node value and one per residual sample."""
+    if not 1 <= rd < inner.dim:
+        raise ValueError("region dims must be 1..%d" % (inner.dim - 1))
+    p = FubiniParams()
+    p.inner_family = inner.family
+    p.inner_dim = inner.dim
+    p.mc_samples = int(mc_samples)
+    p.seed = int(seed) & 0xFFFFFFFF
+    for d in range(inner.dim):
+        p.rest_lo[d] = rest_lo[d]
+        p.rest_hi[d] = rest_hi[d]
+    p.inner = C.cast(C.pointer(inner.params), C.c_void_p)
+    f = Integrand(FAMILY_FUBINI, rd, p, name or "fubini(%s)" % inner.name)
+    f.inner = inner  # keeps the inner parameter block alive
+    return f
 
 
 # ---------------------------------------------------------------- seeded generators
@@ -254,9 +287,11 @@ def config3_batch(seed=0, count=256):
 
 def flops_per_eval(f):
     """Static FP32 operation count of one integrand evaluation, read off the source in
-    include/viltrum_amd/integrands.h (add/sub/mul/neg/compare = 1, FMA = 2; the
-    functions are written without FMAs).  Returns (flops, special) where `special`
+    include/viltrum_amd/integrands.h (add/sub/mul/neg/compare = 1, a*b + c = 2 whether
+    or not the FP32 path fuses it).  Returns (flops, special) where `special`
     counts exp/sin/cos calls, never converted to flops."""
+    if f.family == FAMILY_FUBINI:  # a residual sample evaluates the inner integrand once
+        return flops_per_eval(f.inner)
     if f.family == FAMILY_BLOBS:
         p = f.params
         flops = 14 * p.nblobs + 1  # per blob: 6 (centre, dx, dy) + 8 (quadratic form, exp arg, w*, s+)
