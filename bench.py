@@ -262,6 +262,56 @@ def run_config4(args, V, hip):
     print(json.dumps(out), flush=True)
 
 
+def run_fubini(args, V, hip):
+    """Fubini control variate (integrator_crespo2021_infinite<2>) on the config-2 integrand:
+    16384 regions over (x, y), the time dim integrated by Monte Carlo (4 samples per
+    region node value, one per residual sample), 64 spp, 512x512 FP32 bins.  RMSE against
+    the same FP64 stratified Monte Carlo truth as config 2.  value = bins / step time."""
+    from viltrum_amd import integrands as I
+    ctx = V.Context(0)
+    f = I.config2()
+    res = (512, 512)
+    nb = res[0] * res[1]
+    integ = V.integrator_crespo2021_infinite(2, 16383, 4, 64, seed=0)
+    g, grng = integ.prepare(f, V.range_primary(3))
+    p = integ.problem(g, res, grng, V.F32)
+    buf = hip.DeviceBuffer(nb * 4)
+    stream = hip.Stream()
+    tune = ctx.autotune([p], [g], [buf.address], stream.address)
+    for _ in range(args.warmup):
+        ctx.integrate_device(p, g, buf.address, stream.address)
+    hip.device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = ctx.integrate_device(p, g, buf.address, stream.address)
+    hip.device_synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    out = np.zeros(res, dtype=np.float32)
+    buf.to_host(out)
+    truth, tse = ctx.integrate_mc(f, res, args.truth_spp, seed=12345, rng=V.range_primary(3), stratified=True)
+    flops = st.evals_residual * I.flops_per_eval(g)[0]
+    ms_k = float(st.ms_bins)
+    achieved = flops / (ms_k * 1e-3) / 1e12
+    print(json.dumps({
+        "metric": METRIC, "value": nb / (ms * 1e-3), "unit": "bins/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "Fubini CV: config-2 integrand, regions over (x, y), t by Monte Carlo (4 per node, "
+                               "1 per residual sample), 512x512 fp32 bins, 16384 regions, 64 spp",
+                   "bins": nb, "regions": 16384, "mc_samples": 4, "spp": 64, "path": "fubini"},
+        "rmse_vs_fp64": float(np.sqrt(np.mean((out.astype(np.float64) - truth) ** 2))),
+        "fp64_truth": {"kind": "GPU FP64 stratified per-bin MC of the 3-D integrand", "spp": args.truth_spp,
+                       "stderr_rms": float(np.sqrt(np.mean(tse ** 2)))},
+        "evals_per_s": (st.evals_regions + st.evals_residual) / (ms * 1e-3),
+        "phase_ms": {k: float(getattr(st, k)) for k in ("ms_generate", "ms_select", "ms_stratify", "ms_bins",
+                                                         "ms_total")},
+        "roofline": {"bound": "valu", "kernel": "k_bins1 (per-bin CV + residual Monte Carlo, Fubini form)",
+                     "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel_ms": ms_k},
+        "autotune": tune,
+    }), flush=True)
+
+
 def run_config5(args, V, hip):
     """5-D stress: 16 integrals (3 Gaussian lobes over the integrated dims x 2 half-space
     steps x a cosine, seeds 0..15) at 1024x1024 FP32 bins, 65536 regions, 32 spp, in one
@@ -388,6 +438,8 @@ def main():
         return run_config4(args, V, hip)
     if args.workload == "config5":
         return run_config5(args, V, hip)
+    if args.workload == "fubini":
+        return run_fubini(args, V, hip)
     w = workloads()[args.workload]
     f = w["f"]
     ctx = V.Context(0)

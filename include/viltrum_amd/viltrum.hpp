@@ -14,6 +14,8 @@
  *     integrator_adaptive_control_variates_parallel(iterations, spp, seed, heuristic)
  *                                                                       (integrator-adaptive-control-variates.h:17-20)
  *     integrator_variance_reduction_fixed(alpha, iterations, spp, seed) (regions-integrator-parallel-variance-reduction.h:127-130)
+ *     integrator_crespo2021_infinite<DIM>(iterations, mc_samples, spp, seed)
+ *                                                                       (integrator-crespo2021.h:24-44; float bins)
  *
  * What differs: the integrand is a device-evaluable descriptor (viltrum_amd::blobs,
  * poly, lobes5) rather than an arbitrary host lambda, which a GPU cannot call; passing
@@ -210,6 +212,61 @@ inline IntegratorGpu integrator_variance_reduction_fixed(double alpha, std::size
                                                          std::size_t seed = 0,
                                                          int heuristic = VMX_HEUR_SIZE_RELATIVE) {
     return IntegratorGpu(iterations, spp, seed, heuristic, VMX_FORM_VR_FIXED, alpha);
+}
+
+/* integrator_crespo2021_infinite<RD> (src/control-variates/integrator-crespo2021.h:24-44):
+   crespo2021 over the first RD dims of the range; the remaining dims of the integrand are
+   integrated by Monte Carlo, mc_samples per region node value and one per residual
+   sample.  Float bins only, as the reference (its rest-dims lambda takes float arrays). */
+template <std::size_t RD>
+class IntegratorGpuFubini {
+    IntegratorGpu base_;
+    std::size_t mc_, seed_;
+
+public:
+    IntegratorGpuFubini(std::size_t iterations, std::size_t mc_samples, std::size_t spp, std::size_t seed)
+        : base_(iterations, spp, seed, VMX_HEUR_SIZE_RELATIVE, VMX_FORM_VR_OPTIMIZE, 1.0), mc_(mc_samples),
+          seed_(seed) {}
+
+    template <typename Bins, std::size_t DB, typename Float, std::size_t DIM, typename Logger>
+    void integrate(Bins& bins, const std::array<std::size_t, DB>& res, const Integrand& f,
+                   const Range<Float, DIM>& range, Logger& logger, Context& ctx = Context::thread_default()) const {
+        static_assert(RD >= 1 && RD < DIM && DB <= RD, "region dims 1..DIM-1, bins over region dims");
+        using T = std::decay_t<decltype(bins(std::array<std::size_t, DB>()))>;
+        static_assert(std::is_same<T, float>::value, "the Fubini form integrates into float bins");
+        if (f.desc.dim != int(DIM)) throw std::invalid_argument("viltrum_amd: range and integrand dims differ");
+        vmx_fubini_params fp{};
+        fp.inner_family = f.desc.family;
+        fp.inner_dim = f.desc.dim;
+        fp.mc_samples = uint32_t(mc_);
+        fp.seed = uint32_t(2 * seed_ + 1);
+        for (std::size_t d = 0; d < DIM; ++d) {
+            fp.rest_lo[d] = double(range.min(d));
+            fp.rest_hi[d] = double(range.max(d));
+        }
+        fp.inner = f.desc.params;
+        Integrand g;
+        g.desc = vmx_integrand{VMX_FAMILY_FUBINI, int(RD), &fp};
+        std::array<Float, RD> lo, hi;
+        for (std::size_t d = 0; d < RD; ++d) {
+            lo[d] = range.min(d);
+            hi[d] = range.max(d);
+        }
+        base_.integrate(bins, res, g, Range<Float, RD>(lo, hi), logger, ctx);
+    }
+
+    template <typename Bins, std::size_t DB, typename F, typename Float, std::size_t DIM, typename Logger>
+    void integrate(Bins&, const std::array<std::size_t, DB>&, const F&, const Range<Float, DIM>&, Logger&) const {
+        static_assert(std::is_same<F, Integrand>::value,
+                      "viltrum_amd integrates device-evaluable integrand descriptors (viltrum_amd::blobs, poly, "
+                      "lobes5); a host lambda cannot run on the GPU");
+    }
+};
+
+template <std::size_t RD>
+IntegratorGpuFubini<RD> integrator_crespo2021_infinite(std::size_t iterations, std::size_t mc_samples,
+                                                       std::size_t spp, std::size_t seed = 0) {
+    return IntegratorGpuFubini<RD>(iterations, mc_samples, spp, seed);
 }
 
 /* ---------------------------------------------------------------- integrate overloads

@@ -38,5 +38,19 @@ int main() {
     for (auto& row : bins)
         for (float x : row) s += x;
     std::printf("bins %.9g %.9g %.9g\n", double(bins[3][7]), double(bins[20][11]), s);
+
+    // Fubini form: one blob moving in t, regions over (x, y), t integrated by Monte Carlo
+    vmx_blobs_params mp{};
+    mp.nblobs = 1;
+    const double moving[7] = {0.4, 0.5, 0.2, -0.1, 40.0, 60.0, 1.0};
+    for (int k = 0; k < 7; ++k) mp.blob[0][k] = moving[k];
+    auto h = viltrum::blobs(mp, 3);
+    std::vector<std::vector<float>> fb(16, std::vector<float>(16, 0.f));
+    viltrum::integrate(viltrum::integrator_crespo2021_infinite<2>(200, 2, 8, 4), fb, h,
+                       viltrum::range_primary<3, float>());
+    double sf = 0;
+    for (auto& row : fb)
+        for (float x : row) sf += x;
+    std::printf("fubini %.9g %.9g %.9g\n", double(fb[5][9]), double(fb[12][3]), sf);
     return 0;
 }

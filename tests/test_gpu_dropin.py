@@ -24,3 +24,10 @@ def test_cpp_dropin_matches_python(ctx):
     # 9 significant digits identify a float32 exactly
     assert np.float32(vals[0]) == b[3, 7] and np.float32(vals[1]) == b[20, 11]
     assert abs(vals[2] - float(b.astype(np.float64).sum())) < 1e-6 * abs(vals[2])
+    # the Fubini form through the drop-in header and through the Python mirror
+    f3 = I.blobs([(0.4, 0.5, 0.2, -0.1, 40.0, 60.0, 1.0)], dim=3)
+    fb = np.zeros((16, 16), dtype=np.float32)
+    V.integrate(V.integrator_crespo2021_infinite(2, 200, 2, 8, seed=4), fb, f3, V.range_primary(3), ctx=ctx)
+    vals = [float(x) for x in out[2].split()[1:]]
+    assert np.float32(vals[0]) == fb[5, 9] and np.float32(vals[1]) == fb[12, 3]
+    assert abs(vals[2] - float(fb.astype(np.float64).sum())) < 1e-6 * abs(vals[2])

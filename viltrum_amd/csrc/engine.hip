@@ -828,6 +828,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         s->pairs = pairs;
         s->footprints = nfp;
         s->evals_regions = uint64_t(K) + (s->nodes - 1) * uint64_t(vmx::pow3(D - 1));
+        if constexpr (vmx::IsFubini<P>::value) s->evals_regions *= fparams[j].mc;  // mc samples per node value
         s->evals_residual = jb.nbins * jb.spp;
         s->ties_unresolved = rt[j].ties;
         s->ms_generate = ms_gen;
