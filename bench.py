@@ -101,9 +101,9 @@ def run_config4(args, V, hip):
     generates on 512 seeded config-1 style integrands), then take 64 held-out integrands
     at 512x512 FP32 bins with their FP64 closed-form bins.  Target RMSE: the plain path
     (heuristic priority, one integral per call) at 16384 regions and 16 spp.  Each path
-    then picks, over region budgets {4096, 16384}, the spp that reaches the target
+    then picks, over region budgets {2048, 4096, 8192, 16384}, the spp that reaches the target
     (RMSE ~ 1/sqrt(spp) from a 16-spp probe, then checked and stepped) and the faster
-    of the two budgets.  A step is one learned+batched pass over the 64 integrals at its
+    of the budgets.  A step is one learned+batched pass over the 64 integrals at its
     chosen (budget, spp); value = their bins / step time.  The plain path at its chosen
     point is timed the same way beside it."""
     from viltrum_amd import integrands as I
@@ -170,7 +170,7 @@ def run_config4(args, V, hip):
                          "traffic": TRAFFIC_BYTES["config4"],
                          "kernel_ms": ms_k}}), flush=True)
         return
-    budgets = (4096, 16384)
+    budgets = (2048, 4096, 8192, 16384)
     # launch shapes of each path's workload class, picked by the library's autotuner
     # (setup, untimed, like the training)
     tune = {"learned_batched": ctx.autotune(problems(4096, 16, True), fs, ptrs, stream.address),
@@ -255,9 +255,9 @@ def run_config4(args, V, hip):
         "autotune": tune,
     }
     if not args.no_cpu_baseline:
-        w = dict(f=fs[0], res=res, regions=bp["regions"], spp=bp["spp"], seed=0)
-        cb = cpu_baseline(w, min(16, os.cpu_count() or 1), runs=1)
-        cb["sample"] = "1 of the %d integrals, plain method at the plain path's point: " % len(fs) + cb["sample"]
+        ws = [dict(f=f, res=res, regions=bp["regions"], spp=bp["spp"], seed=0) for f in fs]
+        cb = cpu_baseline(ws, min(16, os.cpu_count() or 1))
+        cb["sample"] = "the held-out integrals in turn, plain method at the plain path's point: " + cb["sample"]
         out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)
 
@@ -378,22 +378,34 @@ def run_config5(args, V, hip):
     print(json.dumps(res_out), flush=True)
 
 
-def cpu_baseline(w, nthreads, runs=3):
+def cpu_baseline(ws, nthreads, budget_s=10.0, min_runs=3):
     """The oracle (C++ restatement of integrator_crespo2021, FP32, per-bin mt19937,
-    threads over dim-0 rows) on the host cores: median of 3 full runs."""
+    threads over dim-0 rows) on the host cores, over a bounded sample: one warm-up run,
+    then the workloads of `ws` in turn (cycling) until about budget_s seconds of CPU work
+    and at least min_runs runs; value = bins integrated / time."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
-    times = []
-    for _ in range(runs):
-        t0 = time.perf_counter()
+    if isinstance(ws, dict):
+        ws = [ws]
+
+    def run(w):
         O.integrate(w["f"], w["res"], w["regions"] - 1, w["spp"], w["seed"], scalar=4,
                     integrator=O.INT_VR_OPTIMIZE, heuristic=O.HEUR_SIZE_RELATIVE, nthreads=nthreads)
-        times.append(time.perf_counter() - t0)
-    t = sorted(times)[len(times) // 2]
-    nb = int(np.prod(w["res"]))
-    return {"value": nb / t, "unit": "bins/s", "cores": nthreads, "kind": "port",
-            "sample": "%d bins, %d regions, %d spp, median of %d runs, %.2f s each"
-                      % (nb, w["regions"], w["spp"], runs, t)}
+
+    run(ws[0])
+    total_t, total_bins, n = 0.0, 0, 0
+    while n < min_runs or total_t < budget_s:
+        w = ws[n % len(ws)]
+        t0 = time.perf_counter()
+        run(w)
+        total_t += time.perf_counter() - t0
+        total_bins += int(np.prod(w["res"]))
+        n += 1
+    w0 = ws[0]
+    return {"value": total_bins / total_t, "unit": "bins/s", "cores": nthreads, "kind": "port",
+            "sample": "%d runs (%d distinct integrals) of %s bins, %d regions, %d spp after one warm-up: "
+                      "%.1f s of CPU work" % (n, min(n, len(ws)), "x".join(map(str, w0["res"])), w0["regions"],
+                                              w0["spp"], total_t)}
 
 
 def main():
