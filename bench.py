@@ -12,6 +12,9 @@ Workloads:
   config2            3-D moving-blob integrand (32 blobs x step occluder), 512x512 bins,
                      16384 regions, 64 spp, plain GPU path
   config3            256 uneven integrals in one batched call vs one call each
+  config5            16 5-D integrals at 1024x1024 bins, 65536 regions, 32 spp, one
+                     batched call with the learned priority (heuristic beside it)
+  fubini             integrator_crespo2021_infinite<2> on a 3-D integrand
   config1            2-D config-1 integrand, 64x64 bins, 1024 regions, 16 spp
 """
 import argparse
@@ -312,68 +315,109 @@ def run_fubini(args, V, hip):
     }), flush=True)
 
 
+def c5_truth(ctx, V, fs, res, spp=4096, side=8, grid=4):
+    """FP64 stratified per-bin Monte Carlo (spp samples per bin) on grid x grid blocks of
+    side x side bins spread evenly over the bin grid of each integral: the same bin boxes
+    as the full grid.  integrate_mc's bins are (block range volume) x (bin mean), the full
+    grid's (bin count) x (bin integral): factor (res / side)^2.  Returns per integral a
+    list of (i0, j0, truth block), the summed squared standard error and the bin count."""
+    starts = [[int((g + 0.5) * r / grid) - side // 2 for g in range(grid)] for r in res]
+    scale = (res[0] / side) * (res[1] / side)
+    out, tse, n = [], 0.0, 0
+    for f in fs:
+        blocks = []
+        for i0 in starts[0]:
+            for j0 in starts[1]:
+                rng = V.Range([i0 / res[0], j0 / res[1], 0.0, 0.0, 0.0],
+                              [(i0 + side) / res[0], (j0 + side) / res[1], 1.0, 1.0, 1.0])
+                t, se = ctx.integrate_mc(f, (side, side), spp, seed=777, rng=rng, stratified=True)
+                blocks.append((i0, j0, t * scale))
+                tse += float(np.sum((se * scale) ** 2))
+                n += side * side
+        out.append(blocks)
+    return out, tse, n
+
+
 def run_config5(args, V, hip):
     """5-D stress: 16 integrals (3 Gaussian lobes over the integrated dims x 2 half-space
     steps x a cosine, seeds 0..15) at 1024x1024 FP32 bins, 65536 regions, 32 spp, in one
-    batched call (heuristic priority: the learned one is built for dims 1..4).  RMSE per
-    bin against FP64 stratified Monte Carlo at 4096 spp on a 32x32 corner of every
-    integral.  value = total bins / batched time."""
+    batched call with the learned priority.  Setup (untimed): the D=5 network is trained
+    on the GPU on the pairs the heuristic method generates for seeded integrands of the
+    same family (seeds disjoint from the evaluated ones).  The same batch with the
+    heuristic priority is timed beside it.  RMSE per bin against FP64 stratified Monte
+    Carlo at 4096 spp on 16 blocks of 8x8 bins spread over the grid of every integral.  value = total bins /
+    learned+batched time."""
     from viltrum_amd import integrands as I
+    from viltrum_amd import refiner as R
     ctx = V.Context(0)
+    rep = R.train(ctx, count=args.c5_train_integrands, regions=args.c5_train_regions, res=(1024, 1024),
+                  epochs=args.train_epochs, seed=0, stride=1, dim=5)
     n = args.integrals if args.integrals != 64 else 16
     fs = [I.config5(s) for s in range(n)]
     res = (1024, 1024)
     nb = res[0] * res[1]
-    it = V.integrator_crespo2021(65535, 32, 0)
-    ps = [it.problem(f, res, V.range_primary(5), V.F32) for f in fs]
     bufs = [hip.DeviceBuffer(nb * 4) for _ in fs]
     ptrs = [b.address for b in bufs]
     stream = hip.Stream()
-    tune = ctx.autotune(ps, fs, ptrs, stream.address)
-    for _ in range(args.warmup):
-        ctx.integrate_batch(ps, fs, ptrs, stream.address)
-    hip.device_synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        st = ctx.integrate_batch(ps, fs, ptrs, stream.address)
-    hip.device_synchronize()
-    ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    # truth on the 32x32 corner of each integral: the same bin boxes, FP64 MC at 4096 spp
-    crop = 32
-    out = np.zeros(res, dtype=np.float32)
-    se2, n_sq, tse = 0.0, 0, 0.0
-    for b, f in zip(bufs, fs):
-        b.to_host(out)
-        rng = V.Range([0.0] * 5, [crop / 1024.0, crop / 1024.0, 1.0, 1.0, 1.0])
-        truth, tse_b = ctx.integrate_mc(f, (crop, crop), 4096, seed=777, rng=rng, stratified=True)
-        # integrate_mc's bins are (crop range volume) x (bin mean); the full grid's are
-        # (bin count) x (bin integral): same boxes, factor (1024 / crop)^2
-        scale = (1024.0 / crop) ** 2
-        se2 += float(np.sum((out[:crop, :crop].astype(np.float64) - truth * scale) ** 2))
-        tse += float(np.sum((tse_b * scale) ** 2))
-        n_sq += crop * crop
+
+    def timed(learned):
+        it = R.integrator(65535, 32, seed=0, learned=learned)
+        ps = [it.problem(f, res, V.range_primary(5), V.F32) for f in fs]
+        tune = ctx.autotune(ps, fs, ptrs, stream.address)
+        for _ in range(args.warmup):
+            ctx.integrate_batch(ps, fs, ptrs, stream.address)
+        hip.device_synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            st = ctx.integrate_batch(ps, fs, ptrs, stream.address)
+        hip.device_synchronize()
+        return (time.perf_counter() - t0) * 1e3 / args.steps, st, tune
+
+    truths, tse, n_sq = c5_truth(ctx, V, fs, res)
+
+    def rmse():
+        out = np.zeros(res, dtype=np.float32)
+        se2 = 0.0
+        for b, blocks in zip(bufs, truths):
+            b.to_host(out)
+            for (i0, j0, t) in blocks:
+                se2 += float(np.sum((out[i0:i0 + t.shape[0], j0:j0 + t.shape[1]].astype(np.float64) - t) ** 2))
+        return float(np.sqrt(se2 / n_sq))
+
+    ms_h, st_h, tune_h = timed(False)
+    rmse_h = rmse()
+    ms, st, tune = timed(True)
+    rmse_l = rmse()
     total_bins = nb * len(fs)
     flops = sum(x.evals_residual for x in st) * I.flops_per_eval(fs[0])[0]
     ms_k = float(st[0].ms_bins)
     achieved = flops / (ms_k * 1e-3) / 1e12
+    phases = ("ms_generate", "ms_select", "ms_stratify", "ms_bins", "ms_total")
     res_out = {
         "metric": METRIC, "value": total_bins / (ms * 1e-3), "unit": "bins/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": "5-D lobes x steps x cosine, %d integrals, 1024x1024 fp32 bins, 65536 regions, "
-                               "32 spp, one batched call, heuristic priority" % len(fs),
-                   "integrals": len(fs), "bins": total_bins, "regions": 65536, "spp": 32, "path": "batched"},
-        "rmse_vs_fp64": float(np.sqrt(se2 / n_sq)),
-        "fp64_truth": {"kind": "GPU FP64 stratified per-bin MC on a 32x32 corner of every integral",
-                       "spp": 4096, "stderr_rms": float(np.sqrt(tse / n_sq))},
+                               "32 spp, one batched call, learned priority" % len(fs),
+                   "integrals": len(fs), "bins": total_bins, "regions": 65536, "spp": 32, "path": "learned+batched"},
+        "rmse_vs_fp64": {"learned_batched": rmse_l, "heuristic_batched": rmse_h},
+        # the truth's own Monte Carlo error adds in quadrature: removed here
+        "rmse_vs_fp64_less_truth_noise": {k: float(np.sqrt(max(0.0, v * v - tse / n_sq)))
+                                          for k, v in (("learned_batched", rmse_l), ("heuristic_batched", rmse_h))},
+        "fp64_truth": {"kind": "GPU FP64 stratified per-bin MC on 16 blocks of 8x8 bins spread over the grid "
+                               "of every integral (1024 bins each)", "spp": 4096,
+                       "stderr_rms": float(np.sqrt(tse / n_sq))},
+        "heuristic_batched_bins_per_s": total_bins / (ms_h * 1e-3),
         "evals_per_s": sum(x.evals_regions + x.evals_residual for x in st) / (ms * 1e-3),
-        "phase_ms": {k: float(getattr(st[0], k)) for k in ("ms_generate", "ms_select", "ms_stratify", "ms_bins",
-                                                            "ms_total")},
+        "phase_ms": {k: float(getattr(st[0], k)) for k in phases},
+        "heuristic_phase_ms": {k: float(getattr(st_h[0], k)) for k in phases},
         "roofline": {"bound": "valu", "kernel": "k_bins1 (per-bin CV + residual Monte Carlo, batched)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel_ms": ms_k},
-        "stats": {"nodes": int(sum(x.nodes for x in st)), "rounds": int(st[0].rounds), "pairs": int(st[0].pairs)},
-        "autotune": tune,
+        "stats": {"nodes": int(sum(x.nodes for x in st)), "rounds": int(st[0].rounds),
+                  "rounds_heuristic": int(st_h[0].rounds)},
+        "training": rep,
+        "autotune": {"learned_batched": tune, "heuristic_batched": tune_h},
     }
     print(json.dumps(res_out), flush=True)
 
@@ -422,6 +466,9 @@ def main():
                     help="config4: training budgets cycle through 1/4, 1/2, 1, 2 times this")
     ap.add_argument("--train-epochs", type=int, default=8)
     ap.add_argument("--train-stride", type=int, default=2, help="config4: keep every n-th generated region")
+    ap.add_argument("--c5-train-integrands", type=int, default=32, help="config5: training integrands")
+    ap.add_argument("--c5-train-regions", type=int, default=32768,
+                    help="config5: training budgets cycle through 1/4, 1/2, 1, 2 times this")
     ap.add_argument("--only-timed", default="", metavar="REGIONS,SPP",
                     help="config4: run only the learned+batched step at this point (for kernel profiles)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",

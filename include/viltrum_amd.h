@@ -84,7 +84,7 @@ enum vmx_order {
 enum vmx_refiner {
     VMX_REFINER_HEURISTIC = 0, /* the error heuristic above */
     VMX_REFINER_LEARNED = 1    /* the context's priority network (vmx_refiner_train / _set);
-                                  FP32, dims 1..4, canonical order */
+                                  FP32, dims 1..5, canonical order */
 };
 
 typedef struct vmx_problem {
@@ -178,12 +178,14 @@ typedef struct vmx_train_report {
     double ms_pairs, ms_train;
 } vmx_train_report;
 
-/* Sizes for integrands of dimension dim (1..4): parameter count, feature count F and
+/* Sizes for integrands of dimension dim (1..5): parameter count, feature count F and
    the padded feature row width KP of vmx_refiner_pairs / _forward / _grad rows. */
 int vmx_refiner_size(int32_t dim, uint64_t* n_params, int32_t* n_features, int32_t* row_width);
 
-/* Training pairs of the regions the heuristic method generates for n problems of one
-   family and dim (every stride-th node of the refinement rounds): features as BF16 bit
+/* Training pairs of the regions the method generates for n problems of one family and
+   dim (every stride-th node of the refinement rounds), each problem with its own
+   refiner: the heuristic, or the context's current network for VMX_REFINER_LEARNED
+   (pairs from the trees the network itself builds): features as BF16 bit
    patterns (count x KP) and targets log2(E / (s^2 vol)), E the FP64 target of
    opts->target (bins of each problem's resolution), s = max |node value|.  Only
    opts->target and opts->stride are read (NULL: defaults).  *count is set to the number

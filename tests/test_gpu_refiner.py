@@ -15,7 +15,7 @@ from test_refiner_oracle import layout, random_net, random_rows
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("D,n", [(2, 1000), (3, 77), (4, 130)])
+@pytest.mark.parametrize("D,n", [(2, 1000), (3, 77), (4, 130), (5, 150)])
 def test_forward_matches_oracle(ctx, D, n):
     F, kp = O.nn_dims(D)
     assert V.refiner_size(D) == (layout(kp)["trained"] + 2, F, kp)
@@ -30,7 +30,7 @@ def test_forward_matches_oracle(ctx, D, n):
     assert np.max(diff) < 2e-2 * (1 + np.max(np.abs(ye)))
 
 
-@pytest.mark.parametrize("D,n", [(2, 200), (3, 64)])
+@pytest.mark.parametrize("D,n", [(2, 200), (3, 64), (5, 130)])
 def test_grad_matches_oracle(ctx, D, n):
     F, kp = O.nn_dims(D)
     p = random_net(kp, F, 70 + D)
@@ -60,6 +60,31 @@ def _codes(lo, hi, rlo, rhi):
         idx = np.rint((lo[:, d] - rlo[d]) / (hi[:, d] - lo[:, d])).astype(np.int64)
         out[:, d] = (1 << n) + idx
     return out
+
+
+def fused_priority_check(ctx, D, f, res, regions):
+    """Priorities of the final regions of a learned run against the standalone forward
+    kernel on the oracle's features of the same regions.  The expansion kernel stores
+    2^(lp/4), lp = the predicted log2 energy, lowered where a parent caps it; returns the
+    share of regions whose priority is the uncapped prediction."""
+    bins = np.zeros(res, dtype=np.float32)
+    V.integrate(R.integrator(regions - 1, 0, 0, learned=True), bins, f, V.range_primary(D), ctx=ctx)
+    regs = ctx.export_regions(D)
+    rl, rh = [0.0] * D, [1.0] * D
+    codes = _codes(regs["lo"], regs["hi"], rl, rh)
+    lres = O.lres_of(res, D)
+    rows, sc, dep = [], [], []
+    for r in range(regs["lo"].shape[0]):
+        x, s, d = O.region_features(D, regs["lo"][r], regs["hi"][r], rl, rh, regs["nodes"][r], codes[r], lres=lres)
+        rows.append(O.f2bf(x))
+        sc.append(s)
+        dep.append(d)
+    y = ctx.refiner_forward(D, np.array(rows, dtype=np.uint16)).astype(np.float64)
+    p = ctx.refiner_get(D)
+    lp = y * float(p[-1]) + float(p[-2]) + 2 * np.log2(np.maximum(np.array(sc), 2.0 ** -100)) - np.array(dep)
+    got = np.log2(regs["err"].astype(np.float64))
+    assert np.all(got <= lp / 4 + 1e-3)
+    return float(np.mean(np.abs(got - lp / 4) < 1e-3))
 
 
 def children_nodes(f, lo, hi, v, dim):
@@ -165,7 +190,9 @@ def test_set_get_roundtrip(ctx, trained):
 def test_learned_refinement_is_a_valid_partition(ctx, trained):
     f = R.held_out(1)[0]
     bins = np.zeros((64, 64), dtype=np.float32)
-    it = R.integrator(2047, 4, 3, learned=True)
+    # weight alpha = 1: exactly unbiased for any partition (crespo2021's optimised weight
+    # is a ratio estimator whose bias falls as 1/spp)
+    it = V.integrator_variance_reduction_fixed(1.0, 2047, 4, 3, refiner=V.REFINER_LEARNED)
     V.integrate(it, bins, f, V.range_primary(2), ctx=ctx)
     st = it.last_stats
     assert st.regions == 2048 and np.all(np.isfinite(bins))
@@ -173,11 +200,18 @@ def test_learned_refinement_is_a_valid_partition(ctx, trained):
     vol = np.prod(regs["hi"] - regs["lo"], axis=1)
     assert abs(vol.sum() - 1.0) < 1e-6
     assert np.all(regs["err"] > 0)
-    # unbiased: the control variate is the exact integral of the fitted polynomial
+    # unbiased: the control variate is the exact integral of the fitted polynomial and
+    # the residual is weighted 1
     truth = O.blobs2d_exact_bins(f, 64, 64)
     err = bins.astype(np.float64) - truth
     se = np.std(err) / np.sqrt(err.size)
     assert abs(err.mean()) < 5 * se + 1e-7
+
+
+def test_fused_priority_matches_forward(ctx, trained):
+    share = fused_priority_check(ctx, 2, R.held_out(2)[1], (64, 64), 2048)
+    print("uncapped share", share)
+    assert share > 0.5
 
 
 def rmse_pair(ctx, f, res, regions, spp, seed):

@@ -80,7 +80,7 @@ def test_forward_emulated_close_to_exact():
     assert np.any(ye != yx)  # the rounding is really applied
 
 
-@pytest.mark.parametrize("D", [2, 3])
+@pytest.mark.parametrize("D", [2, 3, 5])
 def test_grad_exact_matches_finite_differences(D):
     F, kp = O.nn_dims(D)
     p = random_net(kp, F, 10 + D)

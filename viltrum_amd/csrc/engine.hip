@@ -206,7 +206,7 @@ struct PairsReq {
     std::function<void(uint64_t, uint16_t**, float**)> alloc;
 };
 
-constexpr int LEARN_MAX_DIM = 4;  // the fused network fits the expansion kernel's LDS up to here
+constexpr int LEARN_MAX_DIM = 5;  // the fused network fits the expansion kernel's LDS up to here
 
 template <typename T, int D, typename P, typename Math>
 void launch_expand(bool learn, const vmx::GenArgs<T>& g, const P* d_f, int par, uint32_t blocks, hipStream_t st) {
@@ -272,7 +272,11 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         n_reg += p.regions;
         jb.form = p.form;
         jb.heur = p.heuristic;
-        jb.learn = (!pairs_req && p.refiner == VMX_REFINER_LEARNED) ? 1 : 0;
+        // training pairs follow each problem's own refiner too: the regions a learned
+        // priority generates are where its network must be right (on-policy pairs)
+        jb.learn = p.refiner == VMX_REFINER_LEARNED ? 1 : 0;
+        // the size floor of the learned priority costs at most 1/64 of the budget
+        jb.lfloor = jb.learn ? std::max(0, (ilog2(p.regions) - 6) / D) : 0;
         for (int d = 0; d < VMX_MAX_DIM; ++d) jb.lres[d] = d < jb.db ? float(std::log2(double(p.res[d]))) : 0.f;
         fparams[j] = P::from(*static_cast<const typename ParamOf<P>::type*>(in[j].params));
     }
@@ -281,7 +285,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     if (learn) {
         if (vmx::IsFubini<P>::value) throw VmxError(VMX_ERR_UNSUPPORTED, "learned refiner: not for the Fubini form");
         if (!std::is_same<T, float>::value || D > LEARN_MAX_DIM)
-            throw VmxError(VMX_ERR_UNSUPPORTED, "learned refiner: FP32 problems of dimension 1..4 only");
+            throw VmxError(VMX_ERR_UNSUPPORTED, "learned refiner: FP32 problems of dimension 1..5 only");
         if (!ctx->nn_ok[D]) throw VmxError(VMX_ERR_UNSUPPORTED, "learned refiner: no network for this dimension");
         if (order != VMX_ORDER_CANONICAL)
             throw VmxError(VMX_ERR_INVALID, "learned refiner: canonical region order only");
@@ -1034,7 +1038,8 @@ int kp_of(int dim) {
         case 2: return vmx::nn::Dims<2>::KP;
         case 3: return vmx::nn::Dims<3>::KP;
         case 4: return vmx::nn::Dims<4>::KP;
-        default: throw VmxError(VMX_ERR_UNSUPPORTED, "learned refiner: dimension 1..4");
+        case 5: return vmx::nn::Dims<5>::KP;
+        default: throw VmxError(VMX_ERR_UNSUPPORTED, "learned refiner: dimension 1..5");
     }
 }
 int nf_of(int dim) {
@@ -1043,7 +1048,8 @@ int nf_of(int dim) {
         case 2: return vmx::nn::Dims<2>::F;
         case 3: return vmx::nn::Dims<3>::F;
         case 4: return vmx::nn::Dims<4>::F;
-        default: throw VmxError(VMX_ERR_UNSUPPORTED, "learned refiner: dimension 1..4");
+        case 5: return vmx::nn::Dims<5>::F;
+        default: throw VmxError(VMX_ERR_UNSUPPORTED, "learned refiner: dimension 1..5");
     }
 }
 uint64_t n_trained(int kp) { return uint64_t(kp) * 64 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1; }
@@ -1059,6 +1065,8 @@ void with_kp(int kp, Fn&& fn) {
         fn(std::integral_constant<int, 96>{});
     else if (kp == 128)
         fn(std::integral_constant<int, 128>{});
+    else if (kp == 288)
+        fn(std::integral_constant<int, 288>{});
     else
         throw VmxError(VMX_ERR_UNSUPPORTED, "learned refiner: unsupported feature width");
 }

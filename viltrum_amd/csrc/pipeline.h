@@ -44,7 +44,8 @@ struct Job {
     uint32_t seed_lo, seed_hi, stream;
     uint32_t N, node_base, node_cap, reg_base;
     int32_t db, mb, form, heur;
-    int32_t learn, pad_;  // learn: refinement priority from the network (refiner.h)
+    int32_t learn;   // refinement priority from the network (refiner.h)
+    int32_t lfloor;  // with learn: every dim is first halved lfloor times, widest first
     float lres[VMX_MAX_DIM];  // log2 of the bin count per dim (0 for integrated dims)
 };
 
@@ -269,7 +270,7 @@ __global__ void __launch_bounds__(256) k_roots(const Job<T>* __restrict__ jobs, 
         // only region), so its children are ranked by their own priorities
         const T pe = jb.learn ? T(INFINITY) : e;
         t.err[s] = pe;
-        t.dim[s] = dm;
+        t.dim[s] = jb.learn && jb.lfloor > 0 ? 0 : dm;
         t.b[s] = pe;
         t.b1[s] = T(INFINITY);
         t.parent[s] = -1;
@@ -320,7 +321,92 @@ struct LearnLds {
     alignas(16) uint16_t xs[R][KP + 8];
     alignas(16) uint16_t h[2][R][nn::WS];
     float y[R];
+    float terms[TPB][2 * D * pow3(D - 1)];  // per team: the fold terms of both children
 };
+
+/* nn::region_features of the two children of one expansion (boxes L.box, nodes L.cv,
+   the parent's codes pcode, split along dim), computed by the TEAM lanes of a team
+   (lane 0..TEAM-1) and written as BF16 to rows[0..1]; jb == nullptr writes zero rows.
+   The values are those of nn::region_features bit for bit: s is a max, and each fold
+   adds its line terms in the same order.  Lane c < 2 gets child c's s and total
+   depth.  Every lane of the block calls it (it contains __syncthreads). */
+template <typename T, int D, int TEAM>
+__device__ inline void team_child_features(const TeamLds<T, D>& L, const Job<T>* jb, const uint32_t* pcode, int dim,
+                                           int lane, uint16_t (*rows)[nn::Dims<D>::KP + 8], float* terms,
+                                           float& s_lane, int& depth_lane) {
+    constexpr int K = pow3(D), KM = pow3(D - 1), KP = nn::Dims<D>::KP;
+    // s of both children: lane-strided maxima, then a butterfly inside the team's lanes
+    float m0 = 0.f, m1 = 0.f;
+    if (jb)
+        for (int k = lane; k < K; k += TEAM) {
+            m0 = vmax(m0, vabs(float(L.cv[0][k])));
+            m1 = vmax(m1, vabs(float(L.cv[1][k])));
+        }
+#pragma unroll
+    for (int o = TEAM / 2; o > 0; o >>= 1) {
+        m0 = vmax(m0, __shfl_xor(m0, o));
+        m1 = vmax(m1, __shfl_xor(m1, o));
+    }
+    const float inv0 = 1.f / vmax(m0, 0x1p-100f), inv1 = 1.f / vmax(m1, 0x1p-100f);
+    for (int q = lane; q < 2 * K; q += TEAM) {
+        const int c = q / K, k = q % K;
+        rows[c][3 * D + k] = jb ? nn::f2bf(float(L.cv[c][k]) * (c ? inv1 : inv0)) : uint16_t(0);
+    }
+    if (jb)
+        for (int q = lane; q < 2 * D * KM; q += TEAM) {
+            const int c = q / (D * KM), r = q % (D * KM), d = r / KM, jl = r % KM;
+            int below = 1;
+            for (int e = 0; e < d; ++e) below *= 3;
+            const int full = (jl % below) + (jl / below) * below * 3;
+            const float p0 = float(L.cv[c][full]), p1 = float(L.cv[c][full + below]),
+                        p2 = float(L.cv[c][full + 2 * below]);
+            const float diff = vabs(simpson(p0, p1, p2) - trapezoid(p0, p2));
+            float w = 1.f;
+            int rr = jl;
+            for (int e = 0; e < D - 1; ++e) {
+                const int dg = rr % 3;
+                rr /= 3;
+                w *= (dg == 1) ? (4.f / 6.f) : (1.f / 6.f);
+            }
+            terms[q] = w * diff;
+        }
+    __syncthreads();
+    if (lane < 2 * D) {
+        const int c = lane / D, d = lane % D;
+        float acc = 0.f;
+        if (jb)
+            for (int jl = 0; jl < KM; ++jl) acc += terms[(c * D + d) * KM + jl];
+        rows[c][3 * D + K + d] = jb ? nn::f2bf(acc * (c ? inv1 : inv0)) : uint16_t(0);
+    }
+    for (int q = lane; q < 2 * (KP - 4 * D - K - 1); q += TEAM) {
+        const int c = q / (KP - 4 * D - K - 1), i = 4 * D + K + 1 + q % (KP - 4 * D - K - 1);
+        rows[c][i] = 0;
+    }
+    s_lane = 0.f;
+    depth_lane = 0;
+    if (lane < 2) {
+        const int c = lane;
+        uint16_t* row = rows[c];
+        if (jb) {
+            int depth = 0;
+            for (int d = 0; d < D; ++d) {
+                const T ctr = T((T(0.5) * (L.box[c][0][d] + L.box[c][1][d]) - jb->lo[d]) / (jb->hi[d] - jb->lo[d]));
+                row[d] = nn::f2bf(float(ctr));
+                const uint32_t code = d == dim ? pcode[d] * 2u + uint32_t(c) : pcode[d];
+                const int n = nn::code_depth(code);
+                depth += n;
+                row[D + d] = nn::f2bf(-float(n) * (1.f / 16.f));
+                row[2 * D + d] = nn::f2bf((jb->lres[d] - float(n)) * (1.f / 16.f));
+            }
+            row[4 * D + K] = nn::f2bf(float(depth) * (1.f / 32.f));
+            s_lane = c ? m1 : m0;
+            depth_lane = depth;
+        } else {
+            for (int i = 0; i < 3 * D; ++i) row[i] = 0;
+            row[4 * D + K] = 0;
+        }
+    }
+}
 
 /* Expansion of one candidate per team; all lanes of the block call it the same number
    of times (block-uniform loop), teams without a candidate idle through the syncs. */
@@ -437,28 +523,45 @@ __device__ inline void team_expand(const GenArgs<T>& g, const F* __restrict__ fs
         const bool lrn = go && g.jobs[L.job].learn != 0;
         float fsc = 0.f;
         int fdepth = 0;
-        if (lane < 2) {
-            uint16_t* row = ll->xs[team * 2 + lane];
-            if (lrn) {
-                const int c = lane;
-                const Job<T>& jb = g.jobs[L.job];
-                uint32_t code[D];
-                for (int d = 0; d < D; ++d) code[d] = t.code[size_t(L.node) * D + d];
-                code[L.dim] = code[L.dim] * 2u + uint32_t(c);
-                nn::region_features<T, D, KP>(
-                    L.box[c][0], L.box[c][1], jb.lo, jb.hi, L.cv[c], code, jb.lres,
-                    [row](int i, float x) { row[i] = nn::f2bf(x); }, fsc, fdepth);
-            } else {
-                for (int k = 0; k < KP; ++k) row[k] = 0;
-            }
-        }
+        uint32_t pcode[D];
+        for (int d = 0; d < D; ++d) pcode[d] = lrn ? t.code[size_t(L.node) * D + d] : 0u;
+        team_child_features<T, D, TEAM>(L, lrn ? &g.jobs[L.job] : nullptr, pcode, lrn ? L.dim : 0, lane,
+                                        &ll->xs[team * 2], ll->terms[team], fsc, fdepth);
         __syncthreads();
         nn::forward_block<KP>(ll->net, &ll->xs[0][0], LearnLds<D>::R, &ll->h[0][0][0], &ll->h[1][0][0],
                               &ll->h[0][0][0], ll->y);
         if (lrn && lane < 2) {
+            // lp = log2 of the predicted energy.  Stored as 2^(lp/4): the same order, and
+            // the whole span of lp (about -450 for regions of zeros .. +50) stays inside
+            // the FP32 range, so no clamp ties distinct predictions.  A child is held
+            // below 2^(-1/8) of its parent's predicted energy (2^(-1/32) of its stored
+            // priority; the root's is +inf): priorities fall strictly along every path,
+            // so b is the node's own priority and a subtree the network overrates cannot
+            // tie with its ancestor and grow round after round.
             const float lp = ll->y[team * 2 + lane] * ll->net.sigma + ll->net.mu +
                              2.f * log2f(vmax(fsc, 0x1p-100f)) - float(fdepth);
-            if (lp == lp) e = T(exp2f(vmin(vmax(lp, -126.f), 126.f)));
+            if (lp == lp) {
+                const float cap = log2f(float(t.err[L.node])) - 0.03125f;
+                e = T(exp2f(vmin(vmax(vmin(0.25f * lp, cap), -126.f), 60.f)));
+            }
+            // size floor: a child still wider than 2^-lfloor along some dim goes before
+            // every network-ranked region (2^(126 - depth/4) > 2^60, falling along
+            // paths) and is split along its widest dim.  Large regions are rare in the
+            // training pairs, so the network's ranking of them is the least reliable.
+            const Job<T>& jb = g.jobs[L.job];
+            int nmin = 1 << 30, dmin = 0;
+            for (int d = 0; d < D; ++d) {
+                const uint32_t code = d == L.dim ? pcode[d] * 2u + uint32_t(lane) : pcode[d];
+                const int n = nn::code_depth(code);
+                if (n < nmin) {
+                    nmin = n;
+                    dmin = d;
+                }
+            }
+            if (nmin < jb.lfloor) {
+                e = T(exp2f(126.f - 0.25f * float(fdepth)));
+                dm = dmin;
+            }
         }
     }
     if (go && lane < 2) {
