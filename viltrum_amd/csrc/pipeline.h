@@ -317,11 +317,13 @@ struct LearnLds {
     static constexpr int KP = nn::Dims<D>::KP;
     static constexpr int TPB = GEN_THREADS / TeamShape<D>::TEAM;
     static constexpr int R = (2 * TPB + 15) / 16 * 16;
-    nn::NetW<KP> net;
+    // the first layer's weights (5 KB at D = 2 .. 37 KB at D = 5) are read from device
+    // memory (L2): the LDS they would take costs a block per CU at every D
+    static constexpr bool W0_DEV = true;
+    typename std::conditional<W0_DEV, nn::NetTail, nn::NetW<KP>>::type net;
     alignas(16) uint16_t xs[R][KP + 8];
     alignas(16) uint16_t h[2][R][nn::WS];
     float y[R];
-    float terms[TPB][2 * D * pow3(D - 1)];  // per team: the fold terms of both children
 };
 
 /* nn::region_features of the two children of one expansion (boxes L.box, nodes L.cv,
@@ -335,6 +337,7 @@ __device__ inline void team_child_features(const TeamLds<T, D>& L, const Job<T>*
                                            int lane, uint16_t (*rows)[nn::Dims<D>::KP + 8], float* terms,
                                            float& s_lane, int& depth_lane) {
     constexpr int K = pow3(D), KM = pow3(D - 1), KP = nn::Dims<D>::KP;
+    __syncthreads();  // terms overwrite the heuristic's line errors
     // s of both children: lane-strided maxima, then a butterfly inside the team's lanes
     float m0 = 0.f, m1 = 0.f;
     if (jb)
@@ -525,11 +528,17 @@ __device__ inline void team_expand(const GenArgs<T>& g, const F* __restrict__ fs
         int fdepth = 0;
         uint32_t pcode[D];
         for (int d = 0; d < D; ++d) pcode[d] = lrn ? t.code[size_t(L.node) * D + d] : 0u;
+        static_assert(sizeof(L.e) >= sizeof(float) * 2 * D * pow3(D - 1), "fold terms fit the line errors");
         team_child_features<T, D, TEAM>(L, lrn ? &g.jobs[L.job] : nullptr, pcode, lrn ? L.dim : 0, lane,
-                                        &ll->xs[team * 2], ll->terms[team], fsc, fdepth);
+                                        &ll->xs[team * 2], reinterpret_cast<float*>(&L.e[0][0]), fsc, fdepth);
         __syncthreads();
-        nn::forward_block<KP>(ll->net, &ll->xs[0][0], LearnLds<D>::R, &ll->h[0][0][0], &ll->h[1][0][0],
-                              &ll->h[0][0][0], ll->y);
+        if constexpr (LearnLds<D>::W0_DEV)
+            nn::forward_block_w0dev<KP>(&static_cast<const nn::NetW<KP>*>(g.net)->w0t[0][0], ll->net,
+                                        &ll->xs[0][0], LearnLds<D>::R, &ll->h[0][0][0], &ll->h[1][0][0],
+                                        &ll->h[0][0][0], ll->y);
+        else
+            nn::forward_block<KP>(ll->net, &ll->xs[0][0], LearnLds<D>::R, &ll->h[0][0][0], &ll->h[1][0][0],
+                                  &ll->h[0][0][0], ll->y);
         if (lrn && lane < 2) {
             // lp = log2 of the predicted energy.  Stored as 2^(lp/4): the same order, and
             // the whole span of lp (about -450 for regions of zeros .. +50) stays inside
@@ -661,9 +670,10 @@ __global__ void __launch_bounds__(GEN_THREADS) k_expand_round(GenArgs<T> g, cons
     if constexpr (LEARN) {
         ll = &learn_lds;
         constexpr int KP = LearnLds<D>::KP;
-        const uint4* src = static_cast<const uint4*>(g.net);
+        const size_t skip = LearnLds<D>::W0_DEV ? nn::net_tail_offset<KP>() : 0;  // bytes left in device memory
+        const uint4* src = reinterpret_cast<const uint4*>(static_cast<const char*>(g.net) + skip);
         uint4* dst = reinterpret_cast<uint4*>(&ll->net);
-        for (int i = threadIdx.x; i < int(sizeof(nn::NetW<KP>) / 16); i += blockDim.x) dst[i] = src[i];
+        for (int i = threadIdx.x; i < int(sizeof(ll->net) / 16); i += blockDim.x) dst[i] = src[i];
         for (int i = threadIdx.x + 2 * TPB * KP; i < LearnLds<D>::R * KP; i += blockDim.x)
             ll->xs[i / KP][i % KP] = 0;  // padding rows stay zero
         __syncthreads();

@@ -75,6 +75,22 @@ struct alignas(16) NetW {  // size a multiple of 16 bytes (copied as 16-byte wor
     float b0[W], b1[W], b2[W], w3[W];
     float b3, mu, sigma, pad;
 };
+/* NetW<KP> from w1t on, with the layout it has there: the part of the forward weights
+   a kernel keeps in LDS when it reads the first layer from device memory. */
+struct alignas(16) NetTail {
+    uint16_t w1t[W][WS];
+    uint16_t w2t[W][WS];
+    float b0[W], b1[W], b2[W], w3[W];
+    float b3, mu, sigma, pad;
+};
+template <int KP>
+constexpr size_t net_tail_offset() {
+    return sizeof(uint16_t) * W * (KP + 8);
+}
+
+static_assert(sizeof(NetW<288>) == net_tail_offset<288>() + sizeof(NetTail), "NetTail is the tail of NetW");
+static_assert(sizeof(NetW<128>) == net_tail_offset<128>() + sizeof(NetTail), "NetTail is the tail of NetW");
+
 /* Input-major copies for back-propagation. */
 struct alignas(16) NetWN {
     uint16_t w1n[W][WS];
@@ -151,10 +167,10 @@ __device__ inline void dense_out(const float* w3, float b3, const uint16_t* h, i
 
 /* Forward pass of R rows of BF16 features xs[R][KP+8]: hidden activations into h0, h1,
    h2 ([R][WS] each; h2 may alias h0), raw outputs into y.  Contains __syncthreads. */
-template <int KP>
-__device__ inline void forward_block(const NetW<KP>& n, const uint16_t* xs, int R, uint16_t* h0, uint16_t* h1,
-                                     uint16_t* h2, float* y) {
-    dense_relu(&n.w0t[0][0], KP + 8, n.b0, xs, KP + 8, KP, R, h0);
+template <int KP, typename Net>
+__device__ inline void forward_rows(const uint16_t* w0t, const Net& n, const uint16_t* xs, int R, uint16_t* h0,
+                                    uint16_t* h1, uint16_t* h2, float* y) {
+    dense_relu(w0t, KP + 8, n.b0, xs, KP + 8, KP, R, h0);
     __syncthreads();
     dense_relu(&n.w1t[0][0], WS, n.b1, h0, WS, W, R, h1);
     __syncthreads();
@@ -162,6 +178,18 @@ __device__ inline void forward_block(const NetW<KP>& n, const uint16_t* xs, int 
     __syncthreads();
     dense_out(n.w3, n.b3, h2, R, y);
     __syncthreads();
+}
+template <int KP>
+__device__ inline void forward_block(const NetW<KP>& n, const uint16_t* xs, int R, uint16_t* h0, uint16_t* h1,
+                                     uint16_t* h2, float* y) {
+    forward_rows<KP>(&n.w0t[0][0], n, xs, R, h0, h1, h2, y);
+}
+/* The same with the first layer's weights w0t ([W][KP+8], BF16) read from device memory
+   (the packed image of NetW<KP>) and the rest from LDS. */
+template <int KP>
+__device__ inline void forward_block_w0dev(const uint16_t* w0t, const NetTail& n, const uint16_t* xs, int R,
+                                           uint16_t* h0, uint16_t* h1, uint16_t* h2, float* y) {
+    forward_rows<KP>(w0t, n, xs, R, h0, h1, h2, y);
 }
 
 /* ---------------------------------------------------------------- inference
