@@ -374,6 +374,17 @@ def run_config5(args, V, hip):
         hip.device_synchronize()
         return (time.perf_counter() - t0) * 1e3 / args.steps, st, tune
 
+    if args.only_timed:
+        # the learned+batched step alone at a given (regions, spp), launch shapes from the
+        # VMX_* environment (no tuning runs): for kernel profiles of the timed step
+        regions, spp = (int(x) for x in args.only_timed.split(","))
+        ms, st, _ = timed(True, regions, spp, tune=False)
+        print(json.dumps({"workload": "config5 learned+batched", "regions": regions, "spp": spp, "ms_per_step": ms,
+                          "kernel_ms": float(st[0].ms_bins),
+                          "phase_ms": {k: float(getattr(st[0], k)) for k in ("ms_generate", "ms_select",
+                                                                              "ms_stratify", "ms_bins")},
+                          "rounds": int(st[0].rounds), "nodes": int(sum(x.nodes for x in st))}), flush=True)
+        return
     truths, tse, n_sq = c5_truth(ctx, V, fs, res)
 
     def rmse():
@@ -490,7 +501,7 @@ def main():
     ap.add_argument("--c5-train-regions", type=int, default=32768,
                     help="config5: training budgets cycle through 1/4, 1/2, 1, 2 times this")
     ap.add_argument("--only-timed", default="", metavar="REGIONS,SPP",
-                    help="config4: run only the learned+batched step at this point (for kernel profiles)")
+                    help="config4, config5: run only the learned+batched step at this point (for kernel profiles)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device (a known-size kernel for HBM counter scaling)")
     args = ap.parse_args()

@@ -72,6 +72,9 @@ struct vmx_ctx {
         int gen_per_cu = 4, round_chunk = 8, bins_xcd = 0;
     };
     std::map<std::array<uint64_t, 6>, Shape> tune;
+    // node capacity per region (times N) the last call of a workload class needed, so a
+    // class whose rounds overflow 8N does not pay a failed attempt on every call
+    std::map<std::array<uint64_t, 6>, uint64_t> cap_hint;
     const Shape* trial = nullptr;
     std::vector<std::array<uint64_t, 6>> tuned_keys;
     // last result (for vmx_export_regions): gathered from device memory on demand
@@ -317,6 +320,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     uint32_t rounds = 0;
     uint32_t max_depth[VMX_MAX_DIM] = {};
     uint64_t cap_mult = 8;
+    if (const auto h = ctx->cap_hint.find(tune_key); h != ctx->cap_hint.end()) cap_mult = h->second;
     vmx::TreeArrays<T> t{};
     vmx::Job<T>* d_jobs = nullptr;
     vmx::JobRt* d_rt = nullptr;
@@ -487,6 +491,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
             cap_mult *= 2;
             continue;
         }
+        ctx->cap_hint[tune_key] = cap_mult;
         break;
     }
     const double ms_gen = ms_since(t_start);
