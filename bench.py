@@ -34,7 +34,7 @@ PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 peak
 # WRITE_SIZE x1.0: the factors a 256 MiB device copy needs to read its known size), at
 # the profiled point of each workload; the commands and raw counts are in
 # profiles/k_bins1_config4_learned_batched.md and profiles/k_bins1_config2_plain.md
-TRAFFIC_BYTES = {"config4": (136791 * 2.0 + 65536) * 1024, "config2": (8274 * 2.0 + 3072) * 1024}
+TRAFFIC_BYTES = {"config4": (136249 * 2.0 + 65536) * 1024, "config2": (8274 * 2.0 + 3072) * 1024}
 PEAK_HBM_GBS = 8000.0
 
 
@@ -496,7 +496,7 @@ def main():
     ap.add_argument("--train-regions", type=int, default=4096,
                     help="config4: training budgets cycle through 1/4, 1/2, 1, 2 times this")
     ap.add_argument("--train-epochs", type=int, default=8)
-    ap.add_argument("--train-stride", type=int, default=2, help="config4: keep every n-th generated region")
+    ap.add_argument("--train-stride", type=int, default=1, help="config4: keep every n-th generated region")
     ap.add_argument("--c5-train-integrands", type=int, default=32, help="config5: training integrands")
     ap.add_argument("--c5-train-regions", type=int, default=32768,
                     help="config5: training budgets cycle through 1/4, 1/2, 1, 2 times this")

@@ -1161,8 +1161,34 @@ void train_net(vmx_ctx* ctx, int dim, uint64_t count, const vmx_train_opts& o, v
     HIP_CHECK(hipMemcpyAsync(th.data(), tg, sizeof(float) * count, hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
     std::mt19937_64 rng(o.seed);
+    // the device writes pairs in the order it claimed node slots, which varies between
+    // calls; the seeded shuffle starts from the rows sorted by a hash of their content,
+    // so the trained network depends on the pair set and the seed only
     std::vector<uint32_t> perm(count);
-    for (uint64_t i = 0; i < count; ++i) perm[i] = uint32_t(i);
+    {
+        size_t sort_bytes = 0;
+        HIP_CHECK(hipError_t(rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint64_t*)nullptr, (uint64_t*)nullptr,
+                                                       (uint32_t*)nullptr, (uint32_t*)nullptr, size_t(count), 0, 64,
+                                                       st)));
+        Measure ms;
+        ms.take<uint64_t>(count);
+        ms.take<uint64_t>(count);
+        ms.take<uint32_t>(count);
+        ms.take<uint32_t>(count);
+        ms.take<char>(sort_bytes);
+        ensure(ctx->tr_idx, ms.used);
+        Carve cs(ctx->tr_idx);
+        uint64_t* k0 = cs.take<uint64_t>(count);
+        uint64_t* k1 = cs.take<uint64_t>(count);
+        uint32_t* i0 = cs.take<uint32_t>(count);
+        uint32_t* i1 = cs.take<uint32_t>(count);
+        void* tmp = cs.take<char>(sort_bytes);
+        vmx::nn::k_row_hash<<<unsigned((count + 255) / 256), 256, 0, st>>>(X, tg, count, kp, k0, i0);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipError_t(rocprim::radix_sort_pairs(tmp, sort_bytes, k0, k1, i0, i1, size_t(count), 0, 64, st)));
+        HIP_CHECK(hipMemcpyAsync(perm.data(), i1, sizeof(uint32_t) * count, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+    }
     std::shuffle(perm.begin(), perm.end(), rng);
     const uint64_t n_valid = std::max<uint64_t>(64, uint64_t(double(count) * holdout));
     const uint64_t n_train = count - n_valid;

@@ -325,6 +325,25 @@ __global__ void __launch_bounds__(256) k_mlp_grad(const float* __restrict__ para
     }
 }
 
+/* 64-bit FNV-1a over the 32-bit words of row i (kp BF16 features, then the target's
+   bits): a content key whose sort gives the pairs one order whatever order the device
+   wrote them in. */
+__global__ void k_row_hash(const uint16_t* __restrict__ X, const float* __restrict__ tgt, uint64_t n, int kp,
+                           uint64_t* __restrict__ key, uint32_t* __restrict__ idx) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(X + i * uint64_t(kp));
+    uint64_t h = 0xcbf29ce484222325ull;
+    for (int k = 0; k < kp / 2; ++k) {
+        h ^= w[k];
+        h *= 0x100000001b3ull;
+    }
+    h ^= __builtin_bit_cast(uint32_t, tgt[i]);
+    h *= 0x100000001b3ull;
+    key[i] = h;
+    idx[i] = uint32_t(i);
+}
+
 /* Sum of the per-block partial gradients in block order. */
 __global__ void k_sum_parts(const float* __restrict__ part, uint32_t nblk, uint32_t P, float* __restrict__ grad) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
