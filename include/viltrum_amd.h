@@ -231,6 +231,7 @@ typedef struct vmx_tune_report {
     int32_t gen_blocks_per_cu; /* chosen: expansion workgroups per compute unit */
     int32_t round_chunk;       /* chosen: refinement rounds per graph launch */
     int32_t bins_xcd;          /* chosen: 1 = each XCD takes a contiguous run of bin tiles */
+    int32_t deepen;            /* chosen: 1 = children that qualify are split in the same round */
     int32_t trials;            /* shapes timed */
     double ms_best;            /* wall time of one call at the chosen shape */
     double ms_default;         /* ... at the built-in default shape */

@@ -28,4 +28,4 @@ def test_struct_layouts_match_header():
     # sizes of the C structs as seen by ctypes (x86-64 SysV layout)
     assert C.sizeof(V.Problem) == 6 * 4 + 8 + 12 * 8 + 3 * 8 + 3 * 8
     assert C.sizeof(V.Stats) == 8 * 8 + 5 * 8
-    assert C.sizeof(V.TuneReport) == 4 * 4 + 2 * 8
+    assert C.sizeof(V.TuneReport) == 5 * 4 + 4 + 2 * 8  # five int32, padding to the doubles

@@ -99,7 +99,9 @@ def _run_batch(ctx, items, scalar=V.F32):
 
 
 @pytest.mark.parametrize("env", [{"VMX_BINS_XCD": "1"}, {"VMX_GEN_PER_CU": "2", "VMX_ROUND_CHUNK": "4"},
-                                 {"VMX_GEN_PER_CU": "8", "VMX_BINS_XCD": "1"}], ids=["xcd", "g2r4", "g8xcd"])
+                                 {"VMX_GEN_PER_CU": "8", "VMX_BINS_XCD": "1"}, {"VMX_DEEPEN": "1"},
+                                 {"VMX_DEEPEN": "1", "VMX_GEN_PER_CU": "2", "VMX_ROUND_CHUNK": "4"}],
+                         ids=["xcd", "g2r4", "g8xcd", "deepen", "deepen_g2r4"])
 def test_launch_shapes_do_not_change_results(ctx, monkeypatch, env):
     items = [it for it in _problems() if it[0].dim == 2]
     base = _run_batch(ctx, items)
@@ -110,14 +112,40 @@ def test_launch_shapes_do_not_change_results(ctx, monkeypatch, env):
         assert np.array_equal(base[k], other[k]), k
 
 
+@pytest.mark.parametrize("scalar", [V.F32, V.F64])
+def test_deepened_rounds_give_the_same_regions_and_bins(ctx, monkeypatch, scalar):
+    """Splitting qualifying children in the round that creates them adds only splits a
+    later round would consider: the selected regions (every box, error and node value)
+    and the bins are unchanged, in fewer rounds.  All problems, 2-D and 3-D, FP32 and
+    FP64."""
+    items = _problems()
+    runs = []
+    for deepen in ("0", "1"):
+        monkeypatch.setenv("VMX_DEEPEN", deepen)
+        regs, bins, rounds = [], [], []
+        for f, res, integ in items:
+            dt = np.float32 if scalar == V.F32 else np.float64
+            b = np.zeros(res, dtype=dt)
+            V.integrate(integ, b, f, V.range_primary(f.dim), ctx=ctx)
+            bins.append(b)
+            rounds.append(integ.last_stats.rounds)
+            regs.append(ctx.export_regions(f.dim))
+        runs.append((regs, bins, rounds))
+    for k in range(len(items)):
+        for key in ("lo", "hi", "nodes", "err"):
+            assert np.array_equal(runs[0][0][k][key], runs[1][0][k][key]), (k, key)
+        assert np.array_equal(runs[0][1][k], runs[1][1][k]), k
+    assert sum(runs[1][2]) < sum(runs[0][2]), (runs[0][2], runs[1][2])
+
+
 def test_autotune_picks_a_shape_and_keeps_results(ctx):
     items = [it for it in _problems() if it[0].dim == 2]
     base = _run_batch(ctx, items)
     bufs = [hip.DeviceBuffer(int(np.prod(res)) * 4) for _, res, _ in items]
     probs = [integ.problem(f, res, V.range_primary(f.dim), V.F32) for f, res, integ in items]
     rep = ctx.autotune(probs, [f for f, _, _ in items], [b.address for b in bufs])
-    assert rep["trials"] == 12 and rep["gen_blocks_per_cu"] in (2, 4, 8) and rep["round_chunk"] in (4, 8)
-    assert rep["bins_xcd"] in (0, 1) and 0 < rep["ms_best"] <= rep["ms_default"]
+    assert rep["trials"] == 24 and rep["gen_blocks_per_cu"] in (2, 4, 8) and rep["round_chunk"] in (4, 8)
+    assert rep["bins_xcd"] in (0, 1) and rep["deepen"] in (0, 1) and 0 < rep["ms_best"] <= rep["ms_default"]
     for k, (_, res, _) in enumerate(items):
         assert np.array_equal(bufs[k].to_host(np.zeros(res, dtype=np.float32)), base[k]), k
     for b in bufs:

@@ -214,6 +214,25 @@ def test_fused_priority_matches_forward(ctx, trained):
     assert share > 0.5
 
 
+def deepen_check(ctx, monkeypatch, D, f, res, regions):
+    """The learned priority gives the same regions and bins with deepened rounds."""
+    out = []
+    for deepen in ("0", "1"):
+        monkeypatch.setenv("VMX_DEEPEN", deepen)
+        bins = np.zeros(res, dtype=np.float32)
+        it = R.integrator(regions - 1, 4, 0, learned=True)
+        V.integrate(it, bins, f, V.range_primary(D), ctx=ctx)
+        out.append((ctx.export_regions(D), bins, it.last_stats.rounds))
+    for key in ("lo", "hi", "nodes", "err"):
+        assert np.array_equal(out[0][0][key], out[1][0][key]), key
+    assert np.array_equal(out[0][1], out[1][1])
+    assert out[1][2] < out[0][2], (out[0][2], out[1][2])
+
+
+def test_learned_deepened_rounds(ctx, trained, monkeypatch):
+    deepen_check(ctx, monkeypatch, 2, R.held_out(3)[2], (128, 128), 4096)
+
+
 def rmse_pair(ctx, f, res, regions, spp, seed):
     truth = O.blobs2d_exact_bins(f, res[0], res[1])
     out = []

@@ -10,7 +10,7 @@ import viltrum_amd as V
 from viltrum_amd import integrands as I
 from viltrum_amd import refiner as R
 
-from test_gpu_refiner import _codes, fused_priority_check
+from test_gpu_refiner import _codes, deepen_check, fused_priority_check
 
 pytestmark = pytest.mark.gpu
 
@@ -97,3 +97,7 @@ def test_fused_priority_matches_forward_d5(ctx, trained5):
     share = fused_priority_check(ctx, 5, R.held_out(1, dim=5)[0], (64, 64), 2048)
     print("uncapped share", share)
     assert share > 0.5
+
+
+def test_learned_deepened_rounds_d5(ctx, trained5, monkeypatch):
+    deepen_check(ctx, monkeypatch, 5, R.held_out(2, dim=5)[1], (32, 32), 2048)

@@ -84,7 +84,7 @@ class TrainReport(C.Structure):
 
 class TuneReport(C.Structure):
     _fields_ = [("gen_blocks_per_cu", C.c_int32), ("round_chunk", C.c_int32), ("bins_xcd", C.c_int32),
-                ("trials", C.c_int32), ("ms_best", C.c_double), ("ms_default", C.c_double)]
+                ("deepen", C.c_int32), ("trials", C.c_int32), ("ms_best", C.c_double), ("ms_default", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

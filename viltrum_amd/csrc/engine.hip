@@ -69,7 +69,7 @@ struct vmx_ctx {
     // launch shapes: the autotuner's choices per workload class, and the shape a tuning
     // trial forces (keys of the classes the trial ran are collected in tuned_keys)
     struct Shape {
-        int gen_per_cu = 4, round_chunk = 8, bins_xcd = 0;
+        int gen_per_cu = 4, round_chunk = 8, bins_xcd = 0, deepen = 0;
     };
     std::map<std::array<uint64_t, 6>, Shape> tune;
     // node capacity per region (times N) the last call of a workload class needed, so a
@@ -313,6 +313,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         if (const char* e = std::getenv("VMX_GEN_PER_CU")) shape.gen_per_cu = std::max(1, std::atoi(e));
         if (const char* e = std::getenv("VMX_ROUND_CHUNK")) shape.round_chunk = std::max(2, std::atoi(e) & ~1);
         if (const char* e = std::getenv("VMX_BINS_XCD")) shape.bins_xcd = std::atoi(e) != 0;
+        if (const char* e = std::getenv("VMX_DEEPEN")) shape.deepen = std::atoi(e) != 0;
     }
     const uint32_t gen_blocks = uint32_t(n_cu) * uint32_t(shape.gen_per_cu);
     const int round_chunk = shape.round_chunk;
@@ -401,6 +402,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         g.depth = small + 8;
         g.J = J;
         g.net = learn ? ctx->nn_lds[D].p : nullptr;
+        g.deepen = shape.deepen;
         // rounds in chunks of round_chunk (even, so each chunk starts at parity 0); after
         // each chunk the candidate counts, job states and round counter are copied to
         // pinned memory.  The host checks chunk k while chunk k+1 runs, so the device is
@@ -1553,19 +1555,22 @@ int vmx_autotune(vmx_ctx* ctx, uint32_t n, const vmx_problem* problems, const vm
         int trials = 1;
         for (int g : {2, 4, 8})
             for (int rc : {4, 8})
-                for (int x : {0, 1}) {
-                    vmx_ctx::Shape sh;
-                    sh.gen_per_cu = g;
-                    sh.round_chunk = rc;
-                    sh.bins_xcd = x;
-                    if (g == def.gen_per_cu && rc == def.round_chunk && x == def.bins_xcd) continue;
-                    const double ms = time_shape(sh);
-                    ++trials;
-                    if (ms < ms_best) {
-                        ms_best = ms;
-                        best = sh;
+                for (int x : {0, 1})
+                    for (int dp : {0, 1}) {
+                        vmx_ctx::Shape sh;
+                        sh.gen_per_cu = g;
+                        sh.round_chunk = rc;
+                        sh.bins_xcd = x;
+                        sh.deepen = dp;
+                        if (g == def.gen_per_cu && rc == def.round_chunk && x == def.bins_xcd && dp == def.deepen)
+                            continue;
+                        const double ms = time_shape(sh);
+                        ++trials;
+                        if (ms < ms_best) {
+                            ms_best = ms;
+                            best = sh;
+                        }
                     }
-                }
         for (const auto& k : ctx->tuned_keys) ctx->tune[k] = best;
         ctx->tuned_keys.clear();
         // leave the chosen shape's results in the bins
@@ -1575,6 +1580,7 @@ int vmx_autotune(vmx_ctx* ctx, uint32_t n, const vmx_problem* problems, const vm
             report->gen_blocks_per_cu = best.gen_per_cu;
             report->round_chunk = best.round_chunk;
             report->bins_xcd = best.bins_xcd;
+            report->deepen = best.deepen;
             report->trials = trials;
             report->ms_best = ms_best;
             report->ms_default = ms_def;

@@ -155,6 +155,7 @@ struct TeamLds {
     T box[2][2][D];      // child c: lo, hi
     T pbox[2][D];
     int32_t node, slot, dim, job, go;
+    int32_t deep[2];  // children that qualify in the same round (slot, or -1)
 };
 
 /* Line errors and folds for `nc` regions held in L.cv[c] with boxes L.box[c]; lanes
@@ -298,6 +299,7 @@ struct GenArgs {
     uint32_t J;
     uint32_t max_rounds;
     const void* net;   // the priority network's LDS image (nn::NetW<KP>, packed once per network)
+    int32_t deepen;    // 1: a split's children with b >= theta are split in the same round
 };
 
 /* Per-team slot requests of one expansion step, combined into one atomic per block and
@@ -415,8 +417,8 @@ __device__ inline void team_child_features(const TeamLds<T, D>& L, const Job<T>*
    of times (block-uniform loop), teams without a candidate idle through the syncs. */
 template <typename T, int D, typename F, typename M, bool LEARN>
 __device__ inline void team_expand(const GenArgs<T>& g, const F* __restrict__ fs, TeamLds<T, D>& L, int par,
-                                   const uint32_t* pre, uint64_t w, bool has, LearnLds<D>* ll, int team,
-                                   SlotLds<GEN_THREADS / TeamShape<D>::TEAM>& sl) {
+                                   const uint32_t* pre, uint64_t w, bool has, int32_t direct, bool may_deepen,
+                                   LearnLds<D>* ll, int team, SlotLds<GEN_THREADS / TeamShape<D>::TEAM>& sl) {
     constexpr int K = pow3(D), KM = pow3(D - 1);
     constexpr int TEAM = TeamShape<D>::TEAM;
     constexpr int TPB = GEN_THREADS / TEAM;
@@ -425,21 +427,30 @@ __device__ inline void team_expand(const GenArgs<T>& g, const F* __restrict__ fs
     TreeArrays<T> t = g.t;
     if (lane == 0) {
         L.go = 0;
+        L.deep[0] = L.deep[1] = -1;
         sl.want[team] = 0;
         if (has) {
-            uint32_t lo = 0, hi = J;
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) / 2;
-                if (pre[mid] <= w)
-                    lo = mid;
-                else
-                    hi = mid;
+            uint32_t j;
+            int32_t node;
+            if (direct >= 0) {  // a child split in the round that created it
+                node = direct;
+                j = t.job[node];
+            } else {  // candidate w of the round
+                uint32_t lo = 0, hi = J;
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) / 2;
+                    if (pre[mid] <= w)
+                        lo = mid;
+                    else
+                        hi = mid;
+                }
+                j = lo;
+                node = int32_t(g.cstart[par * J + j] + (uint32_t(w) - pre[j]));
             }
-            const uint32_t j = lo;
-            const int32_t node = int32_t(g.cstart[par * J + j] + (uint32_t(w) - pre[j]));
             L.node = node;
             sl.job[team] = j;
-            sl.want[team] = t.b[node] >= g.theta[j] ? 1 : 0;
+            // a candidate already split (in the round that created it) is not split again
+            sl.want[team] = t.b[node] >= g.theta[j] && t.child[node] < 0 ? 1 : 0;
         }
     }
     __syncthreads();
@@ -603,6 +614,7 @@ __device__ inline void team_expand(const GenArgs<T>& g, const F* __restrict__ fs
         }
         t.b[s] = nb;
         t.b1[s] = nb1;
+        L.deep[c] = may_deepen && nb >= g.theta[j] ? int32_t(s) : -1;
         t.parent[s] = node;
         t.child[s] = -1;
         t.job[s] = j;
@@ -678,9 +690,25 @@ __global__ void __launch_bounds__(GEN_THREADS) k_expand_round(GenArgs<T> g, cons
             ll->xs[i / KP][i % KP] = 0;  // padding rows stay zero
         __syncthreads();
     }
+    // pass 0 splits candidate w; with deepen, passes 1 and 2 split its children that
+    // already qualify (b >= theta of this round), one level deeper: every split they
+    // add is one a later round would consider, so the selection is unchanged and
+    // rounds advance two levels.  Pass predicates are block-wide (team_expand syncs).
+    const int passes = g.deepen ? 3 : 1;
     for (uint64_t base = uint64_t(blockIdx.x) * TPB; base < total; base += uint64_t(gridDim.x) * TPB) {
         const uint64_t w = base + team;
-        team_expand<T, D, F, M, LEARN>(g, fs, lds[team], par, pre, w, w < total, ll, team, slots);
+        int32_t deep0 = -1, deep1 = -1;
+        for (int pass = 0; pass < passes; ++pass) {
+            const int32_t direct = pass == 0 ? -1 : (pass == 1 ? deep0 : deep1);
+            const bool has = pass == 0 ? w < total : direct >= 0;
+            if (pass > 0 && !__syncthreads_or(has)) continue;
+            team_expand<T, D, F, M, LEARN>(g, fs, lds[team], par, pre, w, has, direct, pass == 0 && passes > 1, ll,
+                                           team, slots);
+            if (pass == 0) {
+                deep0 = lds[team].deep[0];
+                deep1 = lds[team].deep[1];
+            }
+        }
     }
     __syncthreads();
     if (threadIdx.x < D && slots.depth[threadIdx.x] != 0) atomicMax(&g.depth[threadIdx.x], slots.depth[threadIdx.x]);
