@@ -222,7 +222,9 @@ int vmx_exact_bins(vmx_ctx* ctx, const vmx_problem* p, const vmx_integrand* f, d
 /* ---- Launch-shape autotuner.  No counterpart in the reference (its parallel loops have
    no launch shapes).  Runs the batch of vmx_integrate_batch once per candidate shape
    (expansion workgroups per CU 2/4/8, refinement rounds per graph launch 4/8, the
-   per-bin kernel's workgroup-to-bin mapping contiguous per XCD or round-robin), keeps
+   per-bin kernel's workgroup-to-bin mapping contiguous per XCD or round-robin,
+   deepened rounds off/on: qualifying children split in the round that creates them,
+   which selects the same regions in fewer rounds), keeps
    the fastest for later calls of the same family, dimension, scalar type, refiner,
    batch size and bin count (each within a factor of 2), and leaves that batch's
    results in bins_device.  Launch shapes never change results: the random streams and
