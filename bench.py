@@ -477,7 +477,14 @@ def cpu_baseline(ws, nthreads, budget_s=10.0, min_runs=3):
         total_bins += int(np.prod(w["res"]))
         n += 1
     w0 = ws[0]
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": total_bins / total_t, "unit": "bins/s", "cores": nthreads, "kind": "port",
+            "cpu_model": model, "host_cpus": os.cpu_count(),
             "sample": "%d runs (%d distinct integrals) of %s bins, %d regions, %d spp after one warm-up: "
                       "%.1f s of CPU work" % (n, min(n, len(ws)), "x".join(map(str, w0["res"])), w0["regions"],
                                               w0["spp"], total_t)}
