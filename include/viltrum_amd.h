@@ -135,6 +135,20 @@ int vmx_integrate_device(vmx_ctx* ctx, const vmx_problem* p, const vmx_integrand
 int vmx_integrate_batch(vmx_ctx* ctx, uint32_t n, const vmx_problem* problems, const vmx_integrand* integrands,
                         void* const* bins_device, void* stream, vmx_stats* stats);
 
+/* The per-bin phase alone over a caller-given region list: control-variate integral and
+   residual Monte Carlo per bin, as vmx_integrate with generation skipped.  Replaces
+   RegionsIntegratorParallelVarianceReduction::integrate_regions(bins, res, seq_regions,
+   f, range, logger) (src/control-variates/regions-integrator-parallel-variance-reduction.h:32-109)
+   and its alpha = 1 counterpart (…-parallel-control-variates.h:17-66), which take the
+   region sequence a generator produced; here it is the list vmx_export_regions writes
+   (lo/hi: n*D, nodes: n*3^D, dim 0 fastest), so a run can be replayed with more samples,
+   another seed or another resolution.  The list order is kept (regions are not merged
+   into shared footprints).  p->regions and p->order are ignored; p->form, spp, seed, res
+   and bin_dims apply.  Regions must be non-empty boxes inside the range; with spp > 0
+   every bin must meet one. */
+int vmx_integrate_regions(vmx_ctx* ctx, const vmx_problem* p, const vmx_integrand* f, uint64_t n, const double* lo,
+                          const double* hi, const double* nodes, void* bins, vmx_stats* stats);
+
 /* Region list of the last vmx_integrate* call on this context, in its final order.
    Pass NULL arrays to query *n.  lo/hi: n*D, nodes: n*3^D, err: n, dim: n. */
 int vmx_export_regions(vmx_ctx* ctx, uint64_t* n, double* lo, double* hi, double* nodes, double* err,

@@ -131,6 +131,43 @@ public:
     void log(const Data&) {}
 };
 
+/* A region list in the layout of vmx_export_regions / vmx_integrate_regions: boxes
+   count x dim, node values count x 3^dim (dim 0 fastest), error and split dim. */
+struct RegionList {
+    std::size_t dim = 0, count = 0;
+    std::vector<double> lo, hi, nodes, err;
+    std::vector<int32_t> split_dim;
+};
+
+/* What an integrator hands to logger.log after generation, in place of the reference's
+   region sequence (src/newton-cotes/integrator-region-based.h:19): the list is read
+   from the device only if the logger asks for it, and only until the context's next
+   call. */
+class RegionLog {
+    Context* ctx_;
+    std::size_t dim_;
+
+public:
+    RegionLog(Context& ctx, std::size_t dim) : ctx_(&ctx), dim_(dim) {}
+    RegionList fetch() const {
+        RegionList r;
+        r.dim = dim_;
+        uint64_t n = 0;
+        ctx_->check(vmx_export_regions(ctx_->get(), &n, nullptr, nullptr, nullptr, nullptr, nullptr));
+        std::size_t k = 1;
+        for (std::size_t d = 0; d < dim_; ++d) k *= 3;
+        r.count = n;
+        r.lo.resize(n * dim_);
+        r.hi.resize(n * dim_);
+        r.nodes.resize(n * k);
+        r.err.resize(n);
+        r.split_dim.resize(n);
+        ctx_->check(vmx_export_regions(ctx_->get(), &n, r.lo.data(), r.hi.data(), r.nodes.data(), r.err.data(),
+                                       r.split_dim.data()));
+        return r;
+    }
+};
+
 /* ---------------------------------------------------------------- integrators */
 class IntegratorGpu {
     vmx_problem base_{};
@@ -188,6 +225,53 @@ public:
         if (p.form == VMX_FORM_CV_ALPHA1) for_each_pos([&](std::size_t k, const auto& pos) { flat[k] = bins(pos); });
         vmx_stats st{};
         ctx.check(vmx_integrate(ctx.get(), &p, &f.desc, flat.data(), &st));
+        for_each_pos([&](std::size_t k, const auto& pos) { bins(pos) = flat[k]; });
+        logger.log(st);
+        logger.log(RegionLog(ctx, DIM));
+    }
+
+    /* RegionsIntegrator::integrate_regions
+       (src/control-variates/regions-integrator-parallel-variance-reduction.h:32-34): this
+       integrator's per-bin phase (form, spp, seed) over a given region list, e.g. one a
+       RegionLog fetched, replayed at another spp, seed or resolution. */
+    template <typename Bins, std::size_t DB, typename Float, std::size_t DIM, typename Logger>
+    void integrate_regions(Bins& bins, const std::array<std::size_t, DB>& res, const RegionList& regions,
+                           const Integrand& f, const Range<Float, DIM>& range, Logger& logger,
+                           Context& ctx = Context::thread_default()) const {
+        static_assert(DB >= 1 && DB <= 3 && DB <= DIM, "1..3 bin dimensions, no more than the integrand's");
+        using T = std::decay_t<decltype(bins(std::array<std::size_t, DB>()))>;
+        static_assert(std::is_same<T, float>::value || std::is_same<T, double>::value,
+                      "bins must hold float (FP32 path) or double (FP64 path)");
+        if (f.desc.dim != int(DIM) || regions.dim != DIM)
+            throw std::invalid_argument("viltrum_amd: range, integrand and region dims differ");
+        vmx_problem p = base_;
+        p.scalar = sizeof(T) == 4 ? VMX_F32 : VMX_F64;
+        p.bin_dims = int(DB);
+        for (std::size_t d = 0; d < DIM; ++d) {
+            p.lo[d] = double(range.min(d));
+            p.hi[d] = double(range.max(d));
+        }
+        std::size_t nb = 1;
+        for (std::size_t i = 0; i < DB; ++i) {
+            p.res[i] = res[i];
+            nb *= res[i];
+        }
+        std::vector<T> flat(nb, T(0));
+        auto for_each_pos = [&](auto&& fn) {
+            std::array<std::size_t, DB> pos{};
+            for (std::size_t k = 0; k < nb; ++k) {
+                std::size_t rem = k;
+                for (std::size_t i = DB; i-- > 0;) {
+                    pos[i] = rem % res[i];
+                    rem /= res[i];
+                }
+                fn(k, pos);
+            }
+        };
+        if (p.form == VMX_FORM_CV_ALPHA1) for_each_pos([&](std::size_t k, const auto& pos) { flat[k] = bins(pos); });
+        vmx_stats st{};
+        ctx.check(vmx_integrate_regions(ctx.get(), &p, &f.desc, regions.count, regions.lo.data(), regions.hi.data(),
+                                        regions.nodes.data(), flat.data(), &st));
         for_each_pos([&](std::size_t k, const auto& pos) { bins(pos) = flat[k]; });
         logger.log(st);
     }
@@ -293,6 +377,48 @@ void integrate(const Integrator& integrator, std::vector<std::vector<T>>& bins, 
     auto b = [&bins](const std::array<std::size_t, 2>& i) -> T& { return bins[i[0]][i[1]]; };
     const std::array<std::size_t, 2> res{bins.size(), max1};
     integrate(integrator, b, res, f, range);
+}
+
+template <typename Integrator, typename T, typename Float, std::size_t DIM, typename Logger>
+void integrate(const Integrator& integrator, std::vector<std::vector<T>>& bins, const Integrand& f,
+               const Range<Float, DIM>& range, Logger& logger) {
+    std::size_t max1 = 0;
+    for (const auto& v : bins) max1 = v.size() > max1 ? v.size() : max1;
+    for (auto& v : bins) v.resize(max1);
+    auto b = [&bins](const std::array<std::size_t, 2>& i) -> T& { return bins[i[0]][i[1]]; };
+    const std::array<std::size_t, 2> res{bins.size(), max1};
+    integrate(integrator, b, res, f, range, logger);
+}
+
+/* src/integrate.h:150-167: three bin dims, inner vectors padded to the longest */
+template <typename Integrator, typename T, typename Float, std::size_t DIM, typename Logger>
+void integrate(const Integrator& integrator, std::vector<std::vector<std::vector<T>>>& bins, const Integrand& f,
+               const Range<Float, DIM>& range, Logger& logger) {
+    std::size_t max1 = 0, max2 = 0;
+    for (const auto& v : bins) max1 = v.size() > max1 ? v.size() : max1;
+    for (auto& v : bins) v.resize(max1);
+    for (const auto& vv : bins)
+        for (const auto& v : vv) max2 = v.size() > max2 ? v.size() : max2;
+    for (auto& vv : bins)
+        for (auto& v : vv) v.resize(max2);
+    auto b = [&bins](const std::array<std::size_t, 3>& i) -> T& { return bins[i[0]][i[1]][i[2]]; };
+    const std::array<std::size_t, 3> res{bins.size(), max1, max2};
+    integrate(integrator, b, res, f, range, logger);
+}
+
+template <typename Integrator, typename T, typename Float, std::size_t DIM>
+void integrate(const Integrator& integrator, std::vector<std::vector<std::vector<T>>>& bins, const Integrand& f,
+               const Range<Float, DIM>& range) {
+    LoggerNull log;
+    integrate(integrator, bins, f, range, log);
+}
+
+template <typename Integrator, typename T, typename Float, std::size_t DIM, typename Logger>
+void integrate(const Integrator& integrator, std::vector<T>& bins, const Integrand& f, const Range<Float, DIM>& range,
+               Logger& logger) {
+    auto b = [&bins](const std::array<std::size_t, 1>& i) -> T& { return bins[i[0]]; };
+    const std::array<std::size_t, 1> res{bins.size()};
+    integrate(integrator, b, res, f, range, logger);
 }
 
 template <typename Integrator, typename T, typename Float, std::size_t DIM>

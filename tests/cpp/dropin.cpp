@@ -6,6 +6,14 @@
 
 #include "viltrum_amd/viltrum.hpp"
 
+// a logger (src/log.h:10-19 shape) that keeps the region list an integrator logs
+struct KeepRegions {
+    viltrum::RegionList regions;
+    void log(const viltrum::RegionLog& r) { regions = r.fetch(); }
+    template <typename Data>
+    void log(const Data&) {}
+};
+
 int main() {
     // x^2 + y^2 over [0,1]^2 with integrator_crespo2021(16, 64): 2/3
     vmx_poly_params pp{};
@@ -52,5 +60,29 @@ int main() {
     for (auto& row : fb)
         for (float x : row) sf += x;
     std::printf("fubini %.9g %.9g %.9g\n", double(fb[5][9]), double(fb[12][3]), sf);
+
+    // three bin dims (src/integrate.h:150-167): the moving blob into 8x8x4 float bins
+    std::vector<std::vector<std::vector<float>>> b3(8, std::vector<std::vector<float>>(8, std::vector<float>(4, 0.f)));
+    viltrum::integrate(viltrum::integrator_crespo2021(300, 4, 2), b3, h, viltrum::range_primary<3, float>());
+    double s3 = 0;
+    for (auto& plane : b3)
+        for (auto& row : plane)
+            for (float x : row) s3 += x;
+    std::printf("bins3 %.9g %.9g %.9g\n", double(b3[1][2][3]), double(b3[6][5][0]), s3);
+
+    // a logger that keeps the generated regions, then the per-bin phase replayed over
+    // them with more samples and another seed
+    KeepRegions keep;
+    std::vector<std::vector<float>> b1(32, std::vector<float>(32, 0.f));
+    viltrum::integrate(viltrum::integrator_crespo2021(255, 0, 1), b1, g, viltrum::range_primary<2, float>(), keep);
+    std::vector<std::vector<float>> b2(32, std::vector<float>(32, 0.f));
+    auto acc = [&b2](const std::array<std::size_t, 2>& i) -> float& { return b2[i[0]][i[1]]; };
+    viltrum::LoggerNull quiet;
+    viltrum::integrator_crespo2021(0, 64, 9).integrate_regions(acc, std::array<std::size_t, 2>{32, 32}, keep.regions, g,
+                                                               viltrum::range_primary<2, float>(), quiet);
+    double s2 = 0;
+    for (auto& row : b2)
+        for (float x : row) s2 += x;
+    std::printf("replay %zu %.9g %.9g %.9g\n", keep.regions.count, double(b2[3][7]), double(b2[20][11]), s2);
     return 0;
 }

@@ -31,3 +31,20 @@ def test_cpp_dropin_matches_python(ctx):
     vals = [float(x) for x in out[2].split()[1:]]
     assert np.float32(vals[0]) == fb[5, 9] and np.float32(vals[1]) == fb[12, 3]
     assert abs(vals[2] - float(fb.astype(np.float64).sum())) < 1e-6 * abs(vals[2])
+    # three bin dims (vector<vector<vector<float>>>)
+    b3 = np.zeros((8, 8, 4), dtype=np.float32)
+    V.integrate(V.integrator_crespo2021(300, 4, seed=2), b3, f3, V.range_primary(3), ctx=ctx)
+    vals = [float(x) for x in out[3].split()[1:]]
+    assert np.float32(vals[0]) == b3[1, 2, 3] and np.float32(vals[1]) == b3[6, 5, 0]
+    assert abs(vals[2] - float(b3.astype(np.float64).sum())) < 1e-6 * abs(vals[2])
+    # the logged regions replayed with more samples, through the header and through Python
+    b1 = np.zeros((32, 32), dtype=np.float32)
+    V.integrate(V.integrator_crespo2021(255, 0, seed=1), b1, I.config1(), V.range_primary(2), ctx=ctx)
+    regs = ctx.export_regions(2)
+    b2 = np.zeros((32, 32), dtype=np.float32)
+    V.integrate_regions(V.integrator_crespo2021(0, 64, seed=9), b2, I.config1(), V.range_primary(2), regs, ctx=ctx)
+    parts = out[4].split()
+    assert int(parts[1]) == regs["lo"].shape[0] == 256
+    vals = [float(x) for x in parts[2:]]
+    assert np.float32(vals[0]) == b2[3, 7] and np.float32(vals[1]) == b2[20, 11]
+    assert abs(vals[2] - float(b2.astype(np.float64).sum())) < 1e-6 * abs(vals[2])

@@ -37,7 +37,7 @@ REFINER_HEURISTIC, REFINER_LEARNED = 0, 1
 SYMBOLS = ["vmx_ctx_create", "vmx_ctx_destroy", "vmx_last_error", "vmx_integrate", "vmx_integrate_device",
            "vmx_integrate_batch", "vmx_export_regions", "vmx_integrate_mc", "vmx_eval", "vmx_refiner_size",
            "vmx_refiner_pairs", "vmx_refiner_train", "vmx_refiner_get", "vmx_refiner_set", "vmx_refiner_forward",
-           "vmx_refiner_grad", "vmx_exact_bins", "vmx_autotune"]
+           "vmx_refiner_grad", "vmx_exact_bins", "vmx_autotune", "vmx_integrate_regions"]
 
 
 class VmxError(RuntimeError):
@@ -113,6 +113,8 @@ def lib():
                                    C.POINTER(TuneReport)]
         L.vmx_export_regions.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_void_p, C.c_void_p]
+        L.vmx_integrate_regions.argtypes = [C.c_void_p, C.POINTER(Problem), C.c_void_p, C.c_uint64, C.c_void_p,
+                                            C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Stats)]
         L.vmx_eval.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_uint64, C.c_void_p, C.c_void_p]
         L.vmx_integrate_mc.argtypes = [C.c_void_p, C.POINTER(Problem), C.c_void_p, C.c_int32, C.c_void_p,
                                        C.c_void_p]
@@ -200,6 +202,19 @@ class Context:
         self._check(lib().vmx_autotune(self.ptr, n, C.cast(P, C.c_void_p), C.cast(F, C.c_void_p),
                                        C.cast(B, C.c_void_p), C.c_void_p(stream or 0), C.byref(rep)))
         return rep.as_dict()
+
+    def integrate_regions_host(self, problem, f, regions, bins):
+        """The per-bin phase over a given region list (dict lo, hi, nodes as export_regions
+        returns); bins: contiguous numpy array of the evaluation type, filled in place."""
+        lo = np.ascontiguousarray(regions["lo"], dtype=np.float64)
+        hi = np.ascontiguousarray(regions["hi"], dtype=np.float64)
+        nodes = np.ascontiguousarray(regions["nodes"], dtype=np.float64)
+        st = Stats()
+        self._check(lib().vmx_integrate_regions(self.ptr, C.byref(problem), C.cast(f.ptr(), C.c_void_p),
+                                                lo.shape[0], lo.ctypes.data_as(C.c_void_p),
+                                                hi.ctypes.data_as(C.c_void_p), nodes.ctypes.data_as(C.c_void_p),
+                                                bins.ctypes.data_as(C.c_void_p), C.byref(st)))
+        return st
 
     def export_regions(self, D):
         n = C.c_uint64(0)
@@ -419,6 +434,20 @@ def integrator_variance_reduction_fixed(alpha, iterations, spp, seed=0, heuristi
     """cv_fixed_weight(alpha) residual
     (src/control-variates/regions-integrator-parallel-variance-reduction.h:127-130)."""
     return Integrator(iterations, spp, seed, heuristic, FORM_VR_FIXED, alpha=alpha, **kw)
+
+
+def integrate_regions(integrator, bins, f, rng, regions, ctx=None):
+    """RegionsIntegrator::integrate_regions: the integrator's per-bin phase (form, spp,
+    seed) over a given region list (dict lo, hi, nodes, e.g. from Context.export_regions)
+    instead of generating one; fills bins."""
+    ctx = ctx or default_context()
+    if bins.dtype not in (np.float32, np.float64) or not bins.flags.c_contiguous:
+        raise TypeError("bins must be a C-contiguous float32 or float64 array")
+    scalar = F32 if bins.dtype == np.float32 else F64
+    g, grng = integrator.prepare(f, rng)
+    p = integrator.problem(g, bins.shape, grng, scalar)
+    integrator.last_stats = ctx.integrate_regions_host(p, g, regions, bins)
+    return bins
 
 
 def integrate(integrator, *args, ctx=None):

@@ -193,11 +193,19 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
 }
 
 /* One (scalar, D, family) batch through the device pipeline (pipeline.h). */
+/* A caller-given region list (host memory, the layout vmx_export_regions writes):
+   boxes n x D and node values n x 3^D, dim 0 fastest. */
+struct GivenRegions {
+    uint64_t n;
+    const double *lo, *hi, *nodes;
+};
+
 struct JobIn {
     const vmx_problem* p;
     const void* params;
     void* bins;  // device pointer
     vmx_stats* stats;
+    const GivenRegions* given = nullptr;  // set: skip generation, integrate over these regions
 };
 
 /* Training-pair mode of run_batch: stop after generation, report the node count and,
@@ -238,7 +246,13 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     const int order = in[0].p->order;
     for (const JobIn& x : in)
         if (x.p->order != order) throw VmxError(VMX_ERR_INVALID, "one region order per batch");
-    const bool merge = order == VMX_ORDER_CANONICAL;
+    // a given region list keeps its own order (like the reference order): no merging of
+    // regions into shared footprints, so per-bin sums follow the list
+    const bool replay = in[0].given != nullptr;
+    for (const JobIn& x : in)
+        if ((x.given != nullptr) != replay) throw VmxError(VMX_ERR_INVALID, "given regions for all or none");
+    if (replay && pairs_req) throw VmxError(VMX_ERR_INVALID, "training pairs need generation");
+    const bool merge = order == VMX_ORDER_CANONICAL && !replay;
 
     std::vector<vmx::Job<T>> jobs(J);
     std::vector<P> fparams(J);
@@ -270,14 +284,14 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         jb.seed_lo = uint32_t(p.seed);
         jb.seed_hi = uint32_t(p.seed >> 32);
         jb.stream = 0;  // random streams depend on the seed only, not on the batch position
-        jb.N = uint32_t(p.regions);
+        jb.N = uint32_t(replay ? in[j].given->n : p.regions);
         jb.reg_base = uint32_t(n_reg);
-        n_reg += p.regions;
+        n_reg += jb.N;
         jb.form = p.form;
         jb.heur = p.heuristic;
         // training pairs follow each problem's own refiner too: the regions a learned
         // priority generates are where its network must be right (on-policy pairs)
-        jb.learn = p.refiner == VMX_REFINER_LEARNED ? 1 : 0;
+        jb.learn = p.refiner == VMX_REFINER_LEARNED && !replay ? 1 : 0;
         // the size floor of the learned priority costs at most 1/64 of the budget
         jb.lfloor = jb.learn ? std::max(0, (ilog2(p.regions) - 6) / D) : 0;
         for (int d = 0; d < VMX_MAX_DIM; ++d) jb.lres[d] = d < jb.db ? float(std::log2(double(p.res[d]))) : 0.f;
@@ -336,7 +350,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         uint64_t cap64 = 0;
         for (auto& jb : jobs) {
             jb.node_base = uint32_t(cap64);
-            jb.node_cap = uint32_t(cap_mult * jb.N + 64);
+            jb.node_cap = replay ? jb.N : uint32_t(cap_mult * jb.N + 64);
             cap64 += jb.node_cap;
         }
         if (cap64 > 0x7FFFFFF0ull) throw VmxError(VMX_ERR_CAPACITY, "region budget too large for the node arena");
@@ -388,6 +402,40 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         HIP_CHECK(hipMemsetAsync(d_hist16, 0, sizeof(uint32_t) * J * 65536, st));
         HIP_CHECK(hipMemsetAsync(d_ncand, 0, sizeof(uint32_t) * 2 * J, st));
         HIP_CHECK(hipMemsetAsync(small, 0, sizeof(uint32_t) * 16, st));
+        if (replay) {  // the given regions become the nodes, in list order
+            for (uint32_t j = 0; j < J; ++j) {
+                const GivenRegions& gr = *in[j].given;
+                const vmx::Job<T>& jb = jobs[j];
+                const size_t n = gr.n;
+                std::vector<T> lo(n * D), hi(n * D), v(n * K), err(n, T(0));
+                std::vector<uint32_t> job(n, j);
+                std::vector<int32_t> dim(n, 0);
+                for (size_t i = 0; i < n * D; ++i) {
+                    const int d = int(i % D);
+                    lo[i] = T(gr.lo[i]);
+                    hi[i] = T(gr.hi[i]);
+                    if (!(lo[i] < hi[i]) || lo[i] < jb.lo[d] || hi[i] > jb.hi[d])
+                        throw VmxError(VMX_ERR_INVALID, "given region outside the range or empty");
+                }
+                for (size_t i = 0; i < n * K; ++i) {
+                    v[i] = T(gr.nodes[i]);
+                    if (!std::isfinite(double(v[i]))) throw VmxError(VMX_ERR_INVALID, "non-finite node value");
+                }
+                const size_t b = jb.node_base;
+                HIP_CHECK(hipMemcpyAsync(t.lo + b * D, lo.data(), sizeof(T) * n * D, hipMemcpyHostToDevice, st));
+                HIP_CHECK(hipMemcpyAsync(t.hi + b * D, hi.data(), sizeof(T) * n * D, hipMemcpyHostToDevice, st));
+                HIP_CHECK(hipMemcpyAsync(t.v + b * K, v.data(), sizeof(T) * n * K, hipMemcpyHostToDevice, st));
+                HIP_CHECK(hipMemcpyAsync(t.err + b, err.data(), sizeof(T) * n, hipMemcpyHostToDevice, st));
+                HIP_CHECK(hipMemcpyAsync(t.dim + b, dim.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+                HIP_CHECK(hipMemcpyAsync(t.job + b, job.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, st));
+                HIP_CHECK(hipStreamSynchronize(st));  // the host vectors go out of scope
+                rt[j] = vmx::JobRt{};
+                rt[j].count = uint32_t(b + n);
+            }
+            HIP_CHECK(hipMemcpyAsync(d_rt, rt.data(), sizeof(vmx::JobRt) * J, hipMemcpyHostToDevice, st));
+            HIP_CHECK(hipStreamSynchronize(st));
+            break;
+        }
         vmx::k_roots<T, D, P, Math><<<J, 256, 0, st>>>(d_jobs, d_rt, d_f, t, d_hist16, d_cstart, d_ncand);
         HIP_CHECK(hipGetLastError());
         vmx::GenArgs<T> g{};
@@ -575,7 +623,13 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     void* d_sort_tmp = cs.take<char>(sort_bytes + 256);
     const unsigned cap_blocks = (cap_total + 255) / 256;
     const unsigned reg_blocks = unsigned((n_reg + 255) / 256);
-    if (order == VMX_ORDER_CANONICAL) {
+    if (replay) {
+        std::vector<int32_t> all_leaves(n_reg);
+        for (uint32_t j = 0; j < J; ++j)
+            for (uint32_t i = 0; i < jobs[j].N; ++i) all_leaves[jobs[j].reg_base + i] = int32_t(jobs[j].node_base + i);
+        HIP_CHECK(hipMemcpyAsync(d_leaf, all_leaves.data(), sizeof(int32_t) * n_reg, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+    } else if (order == VMX_ORDER_CANONICAL) {
         HIP_CHECK(hipMemsetAsync(d_leaf, 0xFF, sizeof(int32_t) * n_reg, st));
         vmx::k_sel_bound<T><<<J, 256, 0, st>>>(d_jobs, d_rt, d_hist16);
         vmx::k_sel_gather<T><<<cap_blocks, 256, 0, st>>>(d_jobs, d_rt, t, cap_total, d_bucket);
@@ -720,7 +774,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         std::memcpy(&nfp, pin + 4, 4);
         std::memcpy(rt.data(), pin + 16, sizeof(vmx::JobRt) * J);
     }
-    if (order == VMX_ORDER_CANONICAL)
+    if (order == VMX_ORDER_CANONICAL && !replay)
         for (uint32_t j = 0; j < J; ++j)
             if ((rt[j].flags & vmx::JF_LEAVES) || rt[j].nleaves != jobs[j].N)
                 throw VmxError(VMX_ERR_CAPACITY, "region selection did not close");
@@ -898,7 +952,8 @@ int dispatch_T(vmx_ctx* ctx, int family, int D, const std::vector<JobIn>& jobs, 
 
 /* Groups a batch by (family, dim) and runs each group through the device pipeline. */
 int integrate_groups(vmx_ctx* ctx, uint32_t n, const vmx_problem* const* ps, const vmx_integrand* const* fs,
-                     void* const* bins, vmx_stats* const* stats, hipStream_t st) {
+                     void* const* bins, vmx_stats* const* stats, hipStream_t st,
+                     const GivenRegions* const* given = nullptr) {
     const int scalar = ps[0]->scalar;
     std::vector<int> done(n, 0);
     for (uint32_t i = 0; i < n; ++i) {
@@ -915,7 +970,7 @@ int integrate_groups(vmx_ctx* ctx, uint32_t n, const vmx_problem* const* ps, con
         };
         for (uint32_t k = i; k < n; ++k)
             if (!done[k] && fs[k]->family == fs[i]->family && fs[k]->dim == fs[i]->dim && inner(fs[k]) == inner(fs[i])) {
-                group.push_back({ps[k], fs[k]->params, bins[k], stats[k]});
+                group.push_back({ps[k], fs[k]->params, bins[k], stats[k], given ? given[k] : nullptr});
                 done[k] = 1;
             }
         if (scalar == VMX_F32)
@@ -1599,6 +1654,31 @@ int vmx_integrate(vmx_ctx* ctx, const vmx_problem* p, const vmx_integrand* f, vo
         void* d = ensure(ctx->bins, bytes);
         HIP_CHECK(hipMemcpyAsync(d, bins, bytes, hipMemcpyHostToDevice, ctx->stream));
         int rc = integrate_groups(ctx, 1, &p, &f, &d, &stats, ctx->stream);
+        HIP_CHECK(hipMemcpyAsync(bins, d, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return rc;
+    });
+}
+
+int vmx_integrate_regions(vmx_ctx* ctx, const vmx_problem* p, const vmx_integrand* f, uint64_t n, const double* lo,
+                          const double* hi, const double* nodes, void* bins, vmx_stats* stats) {
+    return guarded(ctx, [&]() -> int {
+        if (!p) throw VmxError(VMX_ERR_INVALID, "null problem");
+        if (!bins || !lo || !hi || !nodes) throw VmxError(VMX_ERR_INVALID, "null bins or region arrays");
+        if (n < 1 || n > (1ull << 26)) throw VmxError(VMX_ERR_INVALID, "1..2^26 regions");
+        vmx_problem q = *p;
+        q.regions = n;
+        q.refiner = VMX_REFINER_HEURISTIC;
+        validate(&q, f);
+        const GivenRegions gr{n, lo, hi, nodes};
+        const GivenRegions* grp = &gr;
+        uint64_t nb = 1;
+        for (int i = 0; i < q.bin_dims; ++i) nb *= q.res[i];
+        const size_t bytes = nb * size_t(q.scalar);
+        void* d = ensure(ctx->bins, bytes);
+        HIP_CHECK(hipMemcpyAsync(d, bins, bytes, hipMemcpyHostToDevice, ctx->stream));
+        const vmx_problem* qp = &q;
+        int rc = integrate_groups(ctx, 1, &qp, &f, &d, &stats, ctx->stream, &grp);
         HIP_CHECK(hipMemcpyAsync(bins, d, bytes, hipMemcpyDeviceToHost, ctx->stream));
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
         return rc;
