@@ -63,7 +63,13 @@ struct HostMath {
 constexpr int pow3(int d) { return d <= 0 ? 1 : 3 * pow3(d - 1); }
 
 enum Heuristic { HEUR_SIZE_RELATIVE = 0, HEUR_DEFAULT_ABSOLUTE = 1 };
-enum Integrator { INT_VR_OPTIMIZE = 0, INT_VR_FIXED = 1, INT_CV_ALPHA1 = 2 };
+enum Integrator {
+    INT_VR_OPTIMIZE = 0,
+    INT_VR_FIXED = 1,
+    INT_CV_ALPHA1 = 2,
+    INT_VR_OPTIMIZE_IMPORTANCE = 3,  // the two forms above with region_sampling_importance
+    INT_VR_FIXED_IMPORTANCE = 4
+};
 
 /* A region: box [lo, hi], its cached volume and 3^D node values, dim 0 fastest
    (src/newton-cotes/region.h:23-47, src/multiarray/multiarray.h:20-25,
@@ -577,13 +583,186 @@ struct FubiniSamples {
     }
 };
 
+/* ---- importance sampling of a region's polynomial (region_sampling_importance,
+   src/control-variates/region-sampling.h:22-44, NormDefault) ----
+   The Simpson pdf of a line is |p| shifted so that the quadratic's interior minimum is
+   0 (src/newton-cotes/rules.h:104-145); its CDF is inverted by Cardano's or Viete's
+   formula (:166-213) with the library's mixed float/double arithmetic kept as written:
+   NormDefault::sign returns float (src/control-variates/norm.h:28-29), the cubic's
+   coefficients are double, pow/sqrt/acos/cos are the double ones. */
+namespace imp {
+template <typename T>
+inline std::array<T, 3> coefs(const std::array<T, 3>& p) {
+    return {p[0], -3 * p[0] + 4 * p[1] - p[2], 2 * p[0] - 4 * p[1] + 2 * p[2]};
+}
+template <typename T>
+inline T at(T t, const std::array<T, 3>& p) {
+    const auto c = coefs(p);
+    return (c[2] * t + c[1]) * t + c[0];
+}
+template <typename T>
+inline T subrange(T a, T b, const std::array<T, 3>& p) {
+    const auto c = coefs(p);
+    return T(((c[2] * b / 3.0 + c[1] / 2.0) * b + c[0]) * b - ((c[2] * a / 3.0 + c[1] / 2.0) * a + c[0]) * a);
+}
+template <typename T>
+inline T cdf(T t, const std::array<T, 3>& p) {
+    const auto c = coefs(p);
+    return T(((c[2] * t / 3.0 + c[1] / 2.0) * t + c[0]) * t);
+}
+template <typename T>
+inline std::array<T, 3> pdf_points(const std::array<T, 3>& p) {  // rules.h:104-145
+    std::array<T, 3> q{std::abs(p[0]), std::abs(p[1]), std::abs(p[2])};
+    const auto cs = coefs(q);
+    T ymin = 0;
+    if (cs[2] > 0) {
+        const T tmin = T(-cs[1] / (2.0 * cs[2]));
+        if ((tmin > 0) && (tmin < 1)) {
+            const T ytmin = (cs[2] * tmin + cs[1]) * tmin + cs[0];
+            if (ytmin < ymin) ymin = ytmin;
+        }
+    }
+    for (int i = 0; i < 3; ++i) q[i] -= ymin;
+    return q;
+}
+template <typename T>
+inline T pdf_integral(T a, T b, const std::array<T, 3>& p) {  // rules.h:149-153
+    return subrange(a, b, pdf_points(p));
+}
+template <typename T>
+inline T pdf_unnormalized(T t, const std::array<T, 3>& p) {  // rules.h:156-160
+    return at(t, pdf_points(p));
+}
+inline float sign_of(float v) { return v; }   // norm.h:28
+inline float sign_of(double v) { return float(v); }  // norm.h:29: float as well
+inline double cuberoot(double x) { return (x < 0) ? (-1 * std::pow(-x, 1. / 3.)) : std::pow(x, 1. / 3.); }
+template <typename T>
+inline std::vector<T> inv_cdf(T x, const std::array<T, 3>& points) {  // rules.h:178-213
+    std::vector<T> solutions;
+    const auto coeff = coefs(points);
+    auto a = sign_of(coeff[2]) / 3.;
+    auto b = sign_of(coeff[1]) / 2.;
+    auto c = sign_of(coeff[0]);
+    auto d = -x;
+    const auto sum = a + b + c + d;
+    a /= sum;
+    b /= sum;
+    c /= sum;
+    d /= sum;
+    if (std::abs(a) < 1.e-3) {
+        if (std::abs(b) < 1.e-3) {
+            if (std::abs(c) >= 1.e-3) solutions.push_back(T(-d / c));
+        } else {
+            const auto disc = c * c - 4 * b * d;
+            if (disc >= 0) {
+                solutions.push_back(T((-c + std::sqrt(disc)) / (2 * b)));
+                solutions.push_back(T((-c - std::sqrt(disc)) / (2 * b)));
+            }
+        }
+    } else {
+        const auto pp = c / a - std::pow(b, 2.) / (3. * std::pow(a, 2.));
+        const auto qq = 2 * std::pow(b, 3.) / (27. * std::pow(a, 3.)) - b * c / (3. * std::pow(a, 2.)) + d / a;
+        const auto sqr = std::pow(qq / 2., 2.) + std::pow(pp / 3., 3.);
+        if (sqr >= 0.) {
+            solutions.push_back(T(cuberoot(-qq / 2. - std::sqrt(sqr)) + cuberoot(-qq / 2. + std::sqrt(sqr)) - b / (3 * a)));
+        } else {
+            for (int i = 0; i < 3; ++i)
+                solutions.push_back(T(2. * std::sqrt(-pp / 3.) *
+                                          std::cos(1. / 3. * std::acos(3. * qq / (2. * pp) * std::sqrt(-3. / pp)) -
+                                                   2. * M_PI * i / 3.) -
+                                      b / (3. * a)));
+        }
+    }
+    return solutions;
+}
+template <typename T>
+inline T sample_line(T s, T t0, T t1, const std::array<T, 3>& p) {  // rules.h:216-237
+    const auto ps = pdf_points(p);
+    const T x = s * cdf(t1, ps) + (T(1) - s) * cdf(t0, ps);
+    for (T r : inv_cdf(x, ps)) {
+        if ((r < t0) && (std::abs(t0 - r) < 1.e-5)) r = t0;
+        if ((r > t1) && (std::abs(r - t1) < 1.e-5)) r = t1;
+        if ((r >= t0) && (r <= t1) && (!std::isnan(r))) return r;
+    }
+    return s * (t1 - t0) + t0;
+}
+/* fold of the last dim of an m-dim tensor (3^m values, dim 0 fastest) */
+template <typename T, typename Fn>
+inline std::vector<T> fold_last(const std::vector<T>& t, Fn&& fn) {
+    const size_t n = t.size() / 3;
+    std::vector<T> out(n);
+    for (size_t i = 0; i < n; ++i) out[i] = fn(std::array<T, 3>{t[i], t[i + n], t[i + 2 * n]});
+    return out;
+}
+/* fold of dim 0 */
+template <typename T, typename Fn>
+inline std::vector<T> fold_first(const std::vector<T>& t, Fn&& fn) {
+    std::vector<T> out(t.size() / 3);
+    for (size_t j = 0; j < out.size(); ++j) out[j] = fn(std::array<T, 3>{t[3 * j], t[3 * j + 1], t[3 * j + 2]});
+    return out;
+}
+/* Region::sample_subrange (region.h:220-265) on normalised bounds an, bn: dim k is
+   drawn from the marginal of the remaining dims (later dims integrated over their
+   bounds, last first), then the tensor is conditioned on it. */
+template <typename T, int D>
+inline void sample_subrange(const Region<T, D>& r, const T* s, const T* an, const T* bn, T* out) {
+    std::vector<T> t(r.v, r.v + Region<T, D>::K);
+    for (int k = 0; k < D; ++k) {
+        std::vector<T> m = t;
+        for (int d = D - 1; d > k; --d)
+            m = fold_last(m, [&](const std::array<T, 3>& v) { return pdf_integral(an[d], bn[d], v); });
+        const T sol = sample_line(s[k], an[k], bn[k], std::array<T, 3>{m[0], m[1], m[2]});
+        out[k] = sol;
+        if (k + 1 < D) t = fold_first(t, [&](const std::array<T, 3>& v) { return pdf_unnormalized(sol, v); });
+    }
+}
+/* Region::pdf_subrange (region.h:324-330): the unnormalised pdf at the point over its
+   integral over the box */
+template <typename T, int D>
+inline T pdf_subrange(const Region<T, D>& r, const T* pn, const T* an, const T* bn) {
+    std::vector<T> t(r.v, r.v + Region<T, D>::K), u(r.v, r.v + Region<T, D>::K);
+    for (int d = D - 1; d > 0; --d) {
+        t = fold_last(t, [&](const std::array<T, 3>& v) { return pdf_unnormalized(pn[d], v); });
+        u = fold_last(u, [&](const std::array<T, 3>& v) { return pdf_integral(an[d], bn[d], v); });
+    }
+    const T num = pdf_unnormalized(pn[0], std::array<T, 3>{t[0], t[1], t[2]});
+    const T den = r.vol * pdf_integral(an[0], bn[0], std::array<T, 3>{u[0], u[1], u[2]});
+    return num / den;
+}
+/* region_sampling_importance::sample over the box [slo, shi] of region r */
+template <typename T, int D, typename RNG>
+inline T sample(const Region<T, D>& r, const T* slo, const T* shi, RNG& rng, T* x) {
+    const T svol = box_volume<T, D>(slo, shi);
+    if (svol < 1.e-5) {  // region_sampling_uniform
+        for (int d = 0; d < D; ++d) {
+            std::uniform_real_distribution<T> dis(slo[d], shi[d]);
+            x[d] = dis(rng);
+        }
+        return svol;
+    }
+    T s[D], an[D], bn[D], pn[D], un[D];
+    std::uniform_real_distribution<T> dis(0, 1);
+    for (int d = 0; d < D; ++d) s[d] = dis(rng);
+    auto in_range = [&](T p, int d) { return (r.lo[d] >= r.hi[d]) ? r.lo[d] : (p - r.lo[d]) / (r.hi[d] - r.lo[d]); };
+    for (int d = 0; d < D; ++d) {
+        an[d] = in_range(slo[d], d);
+        bn[d] = in_range(shi[d], d);
+    }
+    sample_subrange<T, D>(r, s, an, bn, un);
+    for (int d = 0; d < D; ++d) x[d] = un[d] * (r.hi[d] - r.lo[d]) + r.lo[d];
+    for (int d = 0; d < D; ++d) pn[d] = in_range(x[d], d);
+    const T pdf = pdf_subrange<T, D>(r, pn, an, bn);
+    return T((pdf < 1.e-5) ? 0.0 : (1.0 / pdf));
+}
+}  // namespace imp
+
 /* Per-bin control variate + residual, variance-reduction form
    (src/control-variates/regions-integrator-parallel-variance-reduction.h:32-109):
    per-bin mt19937 seeded in dim-0-fastest bin order from mt19937(seed); uniform
    region roulette (region-russian-roulette.h:9-28); uniform in-cell sampling
    (region-sampling.h:9-20); bins assigned.  Threads split dim-0 rows as
    src/foreach.h:43-78 does; results do not depend on the thread count. */
-template <typename T, int D, typename F, typename Acc>
+template <typename T, int D, typename F, typename Acc, bool IMPORTANCE = false>
 void integrate_regions_vr(const std::vector<Region<T, D>>& regs, const F& f, const T* lo, const T* hi, int db,
                           const uint64_t* res, uint64_t spp, uint64_t seed, double alpha, int nthreads, T* out,
                           Stats* st) {
@@ -659,11 +838,16 @@ void integrate_regions_vr(const std::vector<Region<T, D>>& regs, const F& f, con
                         const T* slo = &ilo[chosen * D];
                         const T* shi = &ihi[chosen * D];
                         T x[D];
-                        for (int d = 0; d < D; ++d) {
-                            std::uniform_real_distribution<T> dis(slo[d], shi[d]);
-                            x[d] = dis(rng);
+                        T sfactor;
+                        if constexpr (IMPORTANCE) {
+                            sfactor = imp::sample<T, D>(r, slo, shi, rng, x);
+                        } else {
+                            for (int d = 0; d < D; ++d) {
+                                std::uniform_real_distribution<T> dis(slo[d], shi[d]);
+                                x[d] = dis(rng);
+                            }
+                            sfactor = box_volume<T, D>(slo, shi);
                         }
-                        const T sfactor = box_volume<T, D>(slo, shi);
                         acc.push(T(fb(x) * double(factor) * rrfactor * sfactor),
                                  T(approximation_at(r, x) * double(factor) * rrfactor * sfactor));
                     }

@@ -153,6 +153,12 @@ int run_regions(int integrator, double alpha, const F& fn, const std::vector<vmo
                                                                out.data(), st);
     else if (integrator == vmo::INT_CV_ALPHA1)
         vmo::integrate_regions_cv<T, D, F>(regs, fn, tlo, thi, db, res, spp, seed, out.data(), st);
+    else if (integrator == vmo::INT_VR_OPTIMIZE_IMPORTANCE)
+        vmo::integrate_regions_vr<T, D, F, vmo::AccOptimize<T>, true>(regs, fn, tlo, thi, db, res, spp, seed, alpha,
+                                                                        nthreads, out.data(), st);
+    else if (integrator == vmo::INT_VR_FIXED_IMPORTANCE)
+        vmo::integrate_regions_vr<T, D, F, vmo::AccFixed<T>, true>(regs, fn, tlo, thi, db, res, spp, seed, alpha,
+                                                                     nthreads, out.data(), st);
     else {
         g_err = "unknown integrator";
         return -1;

@@ -51,6 +51,13 @@ BIN_CASES = [  # (integrand, scalar, form, heuristic, res, iterations, spp, seed
     ("blobs3d", 8, 0, 0, (4, 4, 4), 100, 3, 5, 1.0),
     ("poly3", 8, 0, 0, (8, 8), 150, 0, 0, 1.0),
     ("lobes5", 8, 0, 0, (6, 6), 40, 2, 9, 1.0),
+    # region_sampling_importance (forms 3 and 4: optimised and fixed weight; float only)
+    ("config1", 4, 3, 0, (16, 16), 255, 8, 5, 1.0),
+    ("config1", 4, 4, 0, (8, 8), 63, 16, 3, 0.7),
+    ("config1", 4, 3, 0, (40,), 100, 5, 2, 1.0),
+    ("poly2", 4, 3, 0, (8, 8), 100, 8, 5, 1.0),
+    ("poly3", 4, 4, 0, (4, 4), 200, 4, 2, 1.0),
+    ("blobs3d", 4, 3, 0, (8, 8), 300, 8, 5, 1.0),  # regions of zeros: the reference's 0/0 pdf gives NaN bins
 ]
 
 FUBINI_CASES = [  # (integrand, region dims, res, iterations, mc samples, spp, seed): integrator_crespo2021_infinite

@@ -44,7 +44,7 @@ def test_integrate_matches_reference_fixture(case):
                        nthreads=3)
     key = "bins/%s/s%d/f%d/h%d/r%s/it%d/spp%d/seed%d/a%g" % (name, scalar, form, heur, "x".join(map(str, res)), it,
                                                             spp, seed, alpha)
-    assert np.array_equal(b, FIX[key])
+    assert np.array_equal(b, FIX[key], equal_nan=True)
 
 
 @pytest.mark.parametrize("case", MG.FUBINI_CASES, ids=lambda c: "%s-rd%d-mc%d-spp%d" % (c[0], c[1], c[4], c[5]))
@@ -87,7 +87,9 @@ def test_oracle_matches_live_reference(seed):
     res = (int(rng.integers(3, 20)), int(rng.integers(3, 20)))
     it, spp = int(rng.integers(0, 300)), int(rng.integers(0, 9))
     for scalar in (4, 8):
-        for form, heur in ((0, 0), (1, 0), (2, 1)):
+        for form, heur in ((0, 0), (1, 0), (2, 1), (3, 0), (4, 0)):
+            if form >= 3 and scalar == 8:
+                continue  # region_sampling_importance: the reference compiles it for float only
             a, _ = O.integrate(f, res, it, spp, seed, scalar=scalar, integrator=form, heuristic=heur, alpha=0.3)
             b, _ = O.integrate(f, res, it, spp, seed, scalar=scalar, integrator=form, heuristic=heur, alpha=0.3,
                                use_ref=True)
