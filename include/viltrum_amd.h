@@ -71,7 +71,13 @@ enum vmx_heuristic {
 enum vmx_form {
     VMX_FORM_VR_OPTIMIZE = 0, /* bins = residual with cv_optimize_weight; integrator_crespo2021 */
     VMX_FORM_VR_FIXED = 1,    /* bins = residual with cv_fixed_weight(alpha) */
-    VMX_FORM_CV_ALPHA1 = 2    /* bins += CV + residual, alpha = 1 */
+    VMX_FORM_CV_ALPHA1 = 2,   /* bins += CV + residual, alpha = 1 */
+    /* the two variance-reduction forms with region_sampling_importance in place of
+       region_sampling_uniform (src/control-variates/region-sampling.h:22-44): residual
+       samples drawn from each region polynomial's |P| pdf over the bin's box, weighted
+       by 1/pdf.  FP32, dims 1..3 (the reference compiles it for float regions only). */
+    VMX_FORM_VR_OPTIMIZE_IMPORTANCE = 3,
+    VMX_FORM_VR_FIXED_IMPORTANCE = 4
 };
 
 /* Order of the final region list (and so of every per-bin region list). */

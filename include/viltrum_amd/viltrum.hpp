@@ -14,6 +14,7 @@
  *     integrator_adaptive_control_variates_parallel(iterations, spp, seed, heuristic)
  *                                                                       (integrator-adaptive-control-variates.h:17-20)
  *     integrator_variance_reduction_fixed(alpha, iterations, spp, seed) (regions-integrator-parallel-variance-reduction.h:127-130)
+ *     integrator_variance_reduction_importance[_fixed](...) (region_sampling_importance, region-sampling.h:22-44)
  *     integrator_crespo2021_infinite<DIM>(iterations, mc_samples, spp, seed)
  *                                                                       (integrator-crespo2021.h:24-44; float bins)
  *
@@ -296,6 +297,20 @@ inline IntegratorGpu integrator_variance_reduction_fixed(double alpha, std::size
                                                          std::size_t seed = 0,
                                                          int heuristic = VMX_HEUR_SIZE_RELATIVE) {
     return IntegratorGpu(iterations, spp, seed, heuristic, VMX_FORM_VR_FIXED, alpha);
+}
+/* regions_integrator_parallel_variance_reduction(rr_uniform_region(), cv_optimize_weight(),
+   region_sampling_importance(), rng, spp) behind the crespo2021 generator
+   (src/control-variates/region-sampling.h:22-44); float bins, dims 1..3 */
+inline IntegratorGpu integrator_variance_reduction_importance(std::size_t iterations, std::size_t spp,
+                                                              std::size_t seed = 0,
+                                                              int heuristic = VMX_HEUR_SIZE_RELATIVE) {
+    return IntegratorGpu(iterations, spp, seed, heuristic, VMX_FORM_VR_OPTIMIZE_IMPORTANCE, 1.0);
+}
+/* ... with cv_fixed_weight(alpha) */
+inline IntegratorGpu integrator_variance_reduction_importance_fixed(double alpha, std::size_t iterations,
+                                                                    std::size_t spp, std::size_t seed = 0,
+                                                                    int heuristic = VMX_HEUR_SIZE_RELATIVE) {
+    return IntegratorGpu(iterations, spp, seed, heuristic, VMX_FORM_VR_FIXED_IMPORTANCE, alpha);
 }
 
 /* integrator_crespo2021_infinite<RD> (src/control-variates/integrator-crespo2021.h:24-44):

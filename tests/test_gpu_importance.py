@@ -1,0 +1,87 @@
+"""Importance-sampled residual (region_sampling_importance,
+src/control-variates/region-sampling.h:22-44): the GPU's Philox stream and the oracle's
+mt19937 stream (the oracle is pinned bit for bit to the reference build, including this
+sampler) run the same estimator, so per-bin means over independent seeds agree within
+their standard errors, with the same spread."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import viltrum_amd as V
+from viltrum_amd import integrands as I
+
+from test_gpu_stats import SEEDS, _check
+from test_gpu_parity import poly_cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _runs(ctx, f, res, it, spp, form, alpha):
+    g, o = [], []
+    for s in range(SEEDS):
+        integ = V.Integrator(it, spp, 2000 + s, form=form, alpha=alpha)
+        b = np.zeros(res, dtype=np.float32)
+        V.integrate(integ, b, f, V.range_primary(f.dim), ctx=ctx)
+        g.append(b.astype(np.float64))
+        ob, _ = O.integrate(f, res, it, spp, 2000 + s, scalar=4, integrator=form, alpha=alpha)
+        o.append(ob)
+    return np.array(g), np.array(o)
+
+
+@pytest.mark.parametrize("f,res,it,spp", [
+    (I.config1(), (8, 8), 127, 8),
+    (I.c1_family(12, 4), (6, 5), 200, 8),
+    (poly_cases()[1], (8, 8), 100, 8),
+    (poly_cases()[2], (5, 5), 150, 8),
+    (I.config1(), (20,), 60, 8),
+], ids=lambda v: getattr(v, "name", str(v)))
+@pytest.mark.parametrize("form,alpha", [(V.FORM_VR_FIXED_IMPORTANCE, 1.0), (V.FORM_VR_OPTIMIZE_IMPORTANCE, 1.0)],
+                         ids=["fixed1", "optimize"])
+def test_importance_matches_oracle(ctx, f, res, it, spp, form, alpha):
+    g, o = _runs(ctx, f, res, it, spp, form, alpha)
+    assert np.all(np.isfinite(g)) and np.all(np.isfinite(o))
+    _check(g, o, rel_floor=1e-6)
+
+
+def test_importance_is_unbiased_and_not_uniform(ctx):
+    """Weight 1: unbiased against the closed-form bins; and the samples really follow the
+    polynomial (bins differ from uniform sampling's at the same seed)."""
+    f = I.config1()
+    res = (8, 8)
+    truth = O.blobs2d_exact_bins(f, *res)
+    runs = []
+    for s in range(SEEDS):
+        b = np.zeros(res, dtype=np.float32)
+        V.integrate(V.integrator_variance_reduction_importance(127, 4, 3000 + s, alpha=1.0), b, f, V.range_primary(2),
+                    ctx=ctx)
+        runs.append(b.astype(np.float64))
+    runs = np.array(runs)
+    err = runs.mean(0) - truth
+    se = runs.std(0, ddof=1) / np.sqrt(SEEDS)
+    z = err[se > 0] / se[se > 0]
+    assert abs(z.mean()) < 5 / np.sqrt(z.size) + 0.05 and np.max(np.abs(z)) < 5.5, z
+    u = np.zeros(res, dtype=np.float32)
+    V.integrate(V.integrator_variance_reduction_fixed(1.0, 127, 4, 3000), u, f, V.range_primary(2), ctx=ctx)
+    assert not np.array_equal(u, runs[0].astype(np.float32))
+
+
+def test_importance_regions_of_zeros_stay_finite(ctx):
+    """Where the region polynomial is 0 over the box the reference divides 0 by 0 (NaN
+    bins, reproduced by the oracle); the GPU samples such boxes uniformly instead."""
+    f = I.moving_blobs(5, 8)
+    res = (8, 8)
+    ob, _ = O.integrate(f, res, 300, 8, 5, scalar=4, integrator=V.FORM_VR_OPTIMIZE_IMPORTANCE)
+    assert np.isnan(ob).any()
+    b = np.zeros(res, dtype=np.float32)
+    V.integrate(V.Integrator(300, 8, 5, form=V.FORM_VR_OPTIMIZE_IMPORTANCE), b, f, V.range_primary(3), ctx=ctx)
+    assert np.all(np.isfinite(b))
+
+
+def test_importance_limits(ctx):
+    f = I.config1()
+    with pytest.raises(V.VmxError):  # FP64: the reference compiles the sampler for float only
+        V.integrate(V.Integrator(63, 4, 0, form=V.FORM_VR_OPTIMIZE_IMPORTANCE), np.zeros((4, 4)), f,
+                    V.range_primary(2), ctx=ctx)
+    with pytest.raises(V.VmxError):  # 3^D tensors in registers: dims 1..3
+        V.integrate(V.Integrator(63, 4, 0, form=V.FORM_VR_OPTIMIZE_IMPORTANCE), np.zeros((4, 4), np.float32),
+                    I.config5(1), V.range_primary(5), ctx=ctx)

@@ -31,6 +31,7 @@ LIB_PATH = os.path.join(_HERE, "libviltrum_amd.so")
 F32, F64 = 4, 8
 HEUR_SIZE_RELATIVE, HEUR_DEFAULT_ABSOLUTE = 0, 1
 FORM_VR_OPTIMIZE, FORM_VR_FIXED, FORM_CV_ALPHA1 = 0, 1, 2
+FORM_VR_OPTIMIZE_IMPORTANCE, FORM_VR_FIXED_IMPORTANCE = 3, 4  # with region_sampling_importance
 ORDER_CANONICAL, ORDER_REFERENCE = 0, 1
 REFINER_HEURISTIC, REFINER_LEARNED = 0, 1
 
@@ -428,6 +429,16 @@ def integrator_adaptive_control_variates_parallel(iterations, spp, seed=0, heuri
     """alpha = 1 control variates, bins accumulated
     (src/control-variates/integrator-adaptive-control-variates.h:17-20)."""
     return Integrator(iterations, spp, seed, heuristic, FORM_CV_ALPHA1, **kw)
+
+
+def integrator_variance_reduction_importance(iterations, spp, seed=0, alpha=None, heuristic=HEUR_SIZE_RELATIVE, **kw):
+    """regions_integrator_parallel_variance_reduction(rr_uniform_region(), cv_optimize_weight()
+    or cv_fixed_weight(alpha), region_sampling_importance(), ...)
+    (src/control-variates/region-sampling.h:22-44): residual samples from each region
+    polynomial's |P| pdf over the bin's box, weighted by 1/pdf.  FP32, dims 1..3."""
+    if alpha is None:
+        return Integrator(iterations, spp, seed, heuristic, FORM_VR_OPTIMIZE_IMPORTANCE, **kw)
+    return Integrator(iterations, spp, seed, heuristic, FORM_VR_FIXED_IMPORTANCE, alpha=alpha, **kw)
 
 
 def integrator_variance_reduction_fixed(alpha, iterations, spp, seed=0, heuristic=HEUR_SIZE_RELATIVE, **kw):

@@ -708,6 +708,10 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     mr.take<T>(n_reg * D);
     mr.take<T>(n_reg);
     mr.take<T>(n_reg * K);
+    bool any_importance = false;
+    for (const auto& jb : jobs)
+        any_importance |= jb.form == VMX_FORM_VR_OPTIMIZE_IMPORTANCE || jb.form == VMX_FORM_VR_FIXED_IMPORTANCE;
+    if (any_importance) mr.take<T>(n_reg * K);
     mr.take<double>(n_reg * mb_max);
     mr.take<uint32_t>(n_reg);
     mr.take<uint32_t>(n_reg + 1);
@@ -729,6 +733,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     ra.hi = cr.take<T>(n_reg * D);
     ra.vol = cr.take<T>(n_reg);
     ra.coef = cr.take<T>(n_reg * K);
+    ra.v = any_importance ? cr.take<T>(n_reg * K) : nullptr;
     ra.marg = cr.take<double>(n_reg * mb_max);
     ra.job = cr.take<uint32_t>(n_reg);
     ra.fpflag = cr.take<uint32_t>(n_reg + 1);
@@ -822,16 +827,29 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     const unsigned bins_grid = unsigned((total_bins + 255) / 256);
     HIP_CHECK(hipEventRecord(ctx->ev[0], st));
     const int db = jobs[0].db;
-    if (db == 1)
-        vmx::k_bins1<T, D, 1, P, Math><<<bins_grid, 256, 0, st>>>(d_jobs, J, total_bins, ra, fa, d_off, d_lst, d_f,
-                                                                  d_rt, lp, shape.bins_xcd);
-    else if constexpr (D >= 2) {
-        if (db == 2)
-            vmx::k_bins1<T, D, 2, P, Math><<<bins_grid, 256, 0, st>>>(d_jobs, J, total_bins, ra, fa, d_off, d_lst,
-                                                                      d_f, d_rt, lp, shape.bins_xcd);
-        else if constexpr (D >= 3)
-            vmx::k_bins1<T, D, 3, P, Math><<<bins_grid, 256, 0, st>>>(d_jobs, J, total_bins, ra, fa, d_off, d_lst,
-                                                                      d_f, d_rt, lp, shape.bins_xcd);
+    // the importance-sampling variant is its own instantiation: its register footprint
+    // stays out of the uniform kernel
+    auto bins_kernel = [&](auto imp_tag) {
+        constexpr bool IMP = decltype(imp_tag)::value;
+        if (db == 1)
+            vmx::k_bins1<T, D, 1, P, Math, IMP><<<bins_grid, 256, 0, st>>>(d_jobs, J, total_bins, ra, fa, d_off,
+                                                                           d_lst, d_f, d_rt, lp, shape.bins_xcd);
+        else if constexpr (D >= 2) {
+            if (db == 2)
+                vmx::k_bins1<T, D, 2, P, Math, IMP><<<bins_grid, 256, 0, st>>>(d_jobs, J, total_bins, ra, fa, d_off,
+                                                                               d_lst, d_f, d_rt, lp, shape.bins_xcd);
+            else if constexpr (D >= 3)
+                vmx::k_bins1<T, D, 3, P, Math, IMP><<<bins_grid, 256, 0, st>>>(
+                    d_jobs, J, total_bins, ra, fa, d_off, d_lst, d_f, d_rt, lp, shape.bins_xcd);
+        }
+    };
+    if constexpr (std::is_same<T, float>::value && D <= vmx::imp::MAX_DIM && !vmx::IsFubini<P>::value) {
+        if (any_importance)
+            bins_kernel(std::true_type{});
+        else
+            bins_kernel(std::false_type{});
+    } else {
+        bins_kernel(std::false_type{});
     }
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipEventRecord(ctx->ev[1], st));
@@ -995,7 +1013,11 @@ void validate(const vmx_problem* p, const vmx_integrand* f) {
     if (p->regions < 1 || p->regions > (1ull << 26)) throw VmxError(VMX_ERR_INVALID, "regions must be 1..2^26");
     if (p->heuristic != VMX_HEUR_SIZE_RELATIVE && p->heuristic != VMX_HEUR_DEFAULT_ABSOLUTE)
         throw VmxError(VMX_ERR_INVALID, "unknown heuristic");
-    if (p->form < VMX_FORM_VR_OPTIMIZE || p->form > VMX_FORM_CV_ALPHA1) throw VmxError(VMX_ERR_INVALID, "unknown form");
+    if (p->form < VMX_FORM_VR_OPTIMIZE || p->form > VMX_FORM_VR_FIXED_IMPORTANCE)
+        throw VmxError(VMX_ERR_INVALID, "unknown form");
+    if (p->form >= VMX_FORM_VR_OPTIMIZE_IMPORTANCE &&
+        (p->scalar != VMX_F32 || f->dim > vmx::imp::MAX_DIM || f->family == VMX_FAMILY_FUBINI))
+        throw VmxError(VMX_ERR_UNSUPPORTED, "importance sampling: FP32 integrands of dimension 1..3");
     if (p->order != VMX_ORDER_CANONICAL && p->order != VMX_ORDER_REFERENCE)
         throw VmxError(VMX_ERR_INVALID, "unknown region order");
     if (p->refiner != VMX_REFINER_HEURISTIC && p->refiner != VMX_REFINER_LEARNED)

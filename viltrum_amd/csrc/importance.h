@@ -1,0 +1,205 @@
+/*
+ * Importance sampling of a region's polynomial over a box (region_sampling_importance,
+ * src/control-variates/region-sampling.h:22-44, with NormDefault): a point is drawn
+ * from the density proportional to the Simpson "pdf" of the region's node values,
+ * dimension by dimension (the first dim from its marginal, the later dims integrated
+ * over the box, last first; then the tensor is conditioned on the drawn coordinate,
+ * Region::sample_subrange, src/newton-cotes/region.h:220-265), and weighted by 1/pdf
+ * (Region::pdf_subrange, :324-330).  Along one line the pdf is |p| shifted so that the
+ * quadratic's interior minimum is 0 (src/newton-cotes/rules.h:104-145); its CDF, a
+ * cubic, is inverted by Cardano's formula or Viete's trigonometric one (:166-237).
+ *
+ * The arithmetic follows the library's mixed float/double expressions as written (the
+ * sign of a coefficient is a float, NormDefault::sign at src/control-variates/norm.h:28-29;
+ * the cubic's coefficients and the root formulas are double), with the device's double
+ * pow/sqrt/acos/cos.  One deviation: where the pdf integrates to 0 over the box (a
+ * region whose nodes are all 0), the reference divides 0 by 0 and its bin becomes NaN;
+ * here sample() reports it and the caller samples the box uniformly instead.
+ */
+#ifndef VMX_IMPORTANCE_H
+#define VMX_IMPORTANCE_H
+
+#include "region_math.h"
+
+namespace vmx {
+namespace imp {
+
+constexpr int MAX_DIM = 3;  // tensors of 3^D values live in registers
+
+template <typename T>
+VMX_HD inline T at3(T t, const T* p) {
+    const T c0 = p[0], c1 = -3 * p[0] + 4 * p[1] - p[2], c2 = 2 * p[0] - 4 * p[1] + 2 * p[2];
+    return (c2 * t + c1) * t + c0;
+}
+template <typename T>
+VMX_HD inline T subrange3(T a, T b, const T* p) {
+    const T c0 = p[0], c1 = -3 * p[0] + 4 * p[1] - p[2], c2 = 2 * p[0] - 4 * p[1] + 2 * p[2];
+    return T(((c2 * b / 3.0 + c1 / 2.0) * b + c0) * b - ((c2 * a / 3.0 + c1 / 2.0) * a + c0) * a);
+}
+template <typename T>
+VMX_HD inline T cdf3(T t, const T* p) {
+    const T c0 = p[0], c1 = -3 * p[0] + 4 * p[1] - p[2], c2 = 2 * p[0] - 4 * p[1] + 2 * p[2];
+    return T(((c2 * t / 3.0 + c1 / 2.0) * t + c0) * t);
+}
+template <typename T>
+VMX_HD inline void pdf_points(const T* p, T* q) {
+    q[0] = vabs(p[0]);
+    q[1] = vabs(p[1]);
+    q[2] = vabs(p[2]);
+    const T c1 = -3 * q[0] + 4 * q[1] - q[2], c2 = 2 * q[0] - 4 * q[1] + 2 * q[2], c0 = q[0];
+    T ymin = 0;
+    if (c2 > 0) {
+        const T tmin = T(-c1 / (2.0 * c2));
+        if ((tmin > 0) && (tmin < 1)) {
+            const T ytmin = (c2 * tmin + c1) * tmin + c0;
+            if (ytmin < ymin) ymin = ytmin;
+        }
+    }
+    for (int i = 0; i < 3; ++i) q[i] -= ymin;
+}
+template <typename T>
+VMX_HD inline T pdf_integral(T a, T b, const T* p) {
+    T q[3];
+    pdf_points(p, q);
+    return subrange3(a, b, q);
+}
+template <typename T>
+VMX_HD inline T pdf_unnormalized(T t, const T* p) {
+    T q[3];
+    pdf_points(p, q);
+    return at3(t, q);
+}
+VMX_HD inline float sign_of(float v) { return v; }
+VMX_HD inline float sign_of(double v) { return float(v); }
+VMX_HD inline double cuberoot(double x) { return (x < 0) ? (-1 * pow(-x, 1. / 3.)) : pow(x, 1. / 3.); }
+
+/* roots of cdf(t) = x in order; returns their count (0..3) */
+template <typename T>
+VMX_HD inline int inv_cdf(T x, const T* points, T* sol) {
+    const T c0 = points[0], c1 = -3 * points[0] + 4 * points[1] - points[2],
+            c2 = 2 * points[0] - 4 * points[1] + 2 * points[2];
+    double a = sign_of(c2) / 3.;
+    double b = sign_of(c1) / 2.;
+    float c = sign_of(c0);
+    T d = -x;
+    const double sum = a + b + c + d;
+    a /= sum;
+    b /= sum;
+    c = float(c / sum);
+    d = T(d / sum);
+    int n = 0;
+    if (fabs(a) < 1.e-3) {
+        if (fabs(b) < 1.e-3) {
+            if (fabs(c) >= 1.e-3) sol[n++] = T(-d / c);
+        } else {
+            const double disc = c * c - 4 * b * d;
+            if (disc >= 0) {
+                sol[n++] = T((-c + sqrt(disc)) / (2 * b));
+                sol[n++] = T((-c - sqrt(disc)) / (2 * b));
+            }
+        }
+    } else {
+        const double pp = c / a - pow(b, 2.) / (3. * pow(a, 2.));
+        const double qq = 2 * pow(b, 3.) / (27. * pow(a, 3.)) - b * c / (3. * pow(a, 2.)) + d / a;
+        const double sqr = pow(qq / 2., 2.) + pow(pp / 3., 3.);
+        if (sqr >= 0.) {
+            sol[n++] = T(cuberoot(-qq / 2. - sqrt(sqr)) + cuberoot(-qq / 2. + sqrt(sqr)) - b / (3 * a));
+        } else {
+            const double pi = 3.14159265358979323846;
+            for (int i = 0; i < 3; ++i)
+                sol[n++] = T(2. * sqrt(-pp / 3.) * cos(1. / 3. * acos(3. * qq / (2. * pp) * sqrt(-3. / pp)) - 2. * pi * i / 3.) -
+                             b / (3. * a));
+        }
+    }
+    return n;
+}
+
+/* one coordinate in [t0, t1] from the line's pdf, s uniform in [0, 1) */
+template <typename T>
+VMX_HD inline T sample_line(T s, T t0, T t1, const T* p) {
+    T q[3];
+    pdf_points(p, q);
+    const T x = s * cdf3(t1, q) + (T(1) - s) * cdf3(t0, q);
+    T sol[3];
+    const int n = inv_cdf(x, q, sol);
+    for (int i = 0; i < n; ++i) {
+        T r = sol[i];
+        if ((r < t0) && (vabs(t0 - r) < 1.e-5)) r = t0;
+        if ((r > t1) && (vabs(r - t1) < 1.e-5)) r = t1;
+        if ((r >= t0) && (r <= t1) && !(r != r)) return r;
+    }
+    return s * (t1 - t0) + t0;
+}
+
+/* A point of the box [slo, shi] inside region [rlo, rhi] (volume rvol, node values v,
+   dim 0 fastest) from the uniforms s[D], and its weight 1/pdf (0 where pdf < 1e-5).
+   Returns false where the pdf's integral over the box is not positive and finite. */
+template <typename T, int D>
+VMX_HD inline bool sample(const T* v, const T* rlo, const T* rhi, T rvol, const T* slo, const T* shi, const T* s,
+                          T* x, T& weight) {
+    static_assert(D >= 1 && D <= MAX_DIM, "importance sampling: 1..3 dims");
+    constexpr int K = pow3(D);
+    T an[D], bn[D], un[D], pn[D];
+    for (int d = 0; d < D; ++d) {
+        an[d] = (rlo[d] >= rhi[d]) ? rlo[d] : (slo[d] - rlo[d]) / (rhi[d] - rlo[d]);
+        bn[d] = (rlo[d] >= rhi[d]) ? rlo[d] : (shi[d] - rlo[d]) / (rhi[d] - rlo[d]);
+    }
+    // normalisation first: the pdf's integral over the box (folds last dim first)
+    T u[K];
+    for (int i = 0; i < K; ++i) u[i] = v[i];
+    int len = K;
+    for (int d = D - 1; d > 0; --d) {
+        len /= 3;
+        for (int i = 0; i < len; ++i) {
+            const T l[3] = {u[i], u[i + len], u[i + 2 * len]};
+            u[i] = pdf_integral(an[d], bn[d], l);
+        }
+    }
+    const T den = rvol * pdf_integral(an[0], bn[0], u);
+    if (!(den > T(0)) || !(den < T(INFINITY))) return false;
+    // draw: dim k from the marginal, then condition on it
+    T t[K];
+    for (int i = 0; i < K; ++i) t[i] = v[i];
+    int tl = K;  // 3^(D-k)
+    for (int k = 0; k < D; ++k) {
+        T m[K];
+        for (int i = 0; i < tl; ++i) m[i] = t[i];
+        int ml = tl;
+        for (int d = D - 1; d > k; --d) {
+            ml /= 3;
+            for (int i = 0; i < ml; ++i) {
+                const T l[3] = {m[i], m[i + ml], m[i + 2 * ml]};
+                m[i] = pdf_integral(an[d], bn[d], l);
+            }
+        }
+        const T sol = sample_line(s[k], an[k], bn[k], m);
+        un[k] = sol;
+        if (k + 1 < D) {
+            tl /= 3;
+            for (int j = 0; j < tl; ++j) {
+                const T l[3] = {t[3 * j], t[3 * j + 1], t[3 * j + 2]};
+                t[j] = pdf_unnormalized(sol, l);
+            }
+        }
+    }
+    for (int d = 0; d < D; ++d) x[d] = un[d] * (rhi[d] - rlo[d]) + rlo[d];
+    for (int d = 0; d < D; ++d) pn[d] = (rlo[d] >= rhi[d]) ? rlo[d] : (x[d] - rlo[d]) / (rhi[d] - rlo[d]);
+    // pdf at the point: unnormalised folds at its coordinates, last dim first
+    for (int i = 0; i < K; ++i) t[i] = v[i];
+    len = K;
+    for (int d = D - 1; d > 0; --d) {
+        len /= 3;
+        for (int i = 0; i < len; ++i) {
+            const T l[3] = {t[i], t[i + len], t[i + 2 * len]};
+            t[i] = pdf_unnormalized(pn[d], l);
+        }
+    }
+    const T pdf = pdf_unnormalized(pn[0], t) / den;
+    weight = T((pdf < 1.e-5) ? 0.0 : (1.0 / pdf));
+    return true;
+}
+
+}  // namespace imp
+}  // namespace vmx
+
+#endif

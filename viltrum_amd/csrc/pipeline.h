@@ -23,6 +23,7 @@
 #include <type_traits>
 
 #include "fubini.h"
+#include "importance.h"
 #include "kernels.h"
 #include "refiner.h"
 
@@ -997,6 +998,7 @@ __global__ void k_leaf_keys64(const int32_t* __restrict__ leaf_slot, const uint3
 template <typename T>
 struct RegionArrays {
     T *lo, *hi, *vol, *coef;
+    T* v;  // node values (3^D per region) for importance sampling, else null
     double* marg;
     uint32_t* job;
     uint32_t* fpflag;  // [n+1]
@@ -1039,6 +1041,8 @@ __global__ void k_prepare2(const Job<T>* __restrict__ jobs, const int32_t* __res
         r.hi[size_t(i) * D + d] = hi[d];
     }
     for (int k = 0; k < K; ++k) v[k] = t.v[s * K + k];
+    if (r.v)
+        for (int k = 0; k < K; ++k) r.v[size_t(i) * K + k] = v[k];
     r.vol[i] = box_volume<T, D>(lo, hi);
     double m[27];
     bin_marginal<T, double, D>(v, jb.db, m);
@@ -1333,8 +1337,8 @@ __device__ inline float sample_pos<float>(float x, float lo, float hi) {
    Moments use the reference's shift by the bin's first sample; sums are
    Kahan-compensated in the evaluation type and finished in FP64.  Summation depends on
    the job alone, so a batched integral gives exactly what it gives alone. */
-template <typename T, int D, int DB, typename F, typename M>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D <= 3 && sizeof(T) == 4 ? 4 : 1))) k_bins1(const Job<T>* __restrict__ jobs, uint32_t J, uint64_t total_bins,
+template <typename T, int D, int DB, typename F, typename M, bool IMP = false>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!IMP && D <= 3 && sizeof(T) == 4 ? 4 : 1))) k_bins1(const Job<T>* __restrict__ jobs, uint32_t J, uint64_t total_bins,
                                                RegionArrays<T> r, FootArrays<T> fa,
                                                const uint32_t* __restrict__ off, const uint32_t* __restrict__ lst,
                                                const F* __restrict__ fs, JobRt* rt, ListPrefix lp, int xcd) {
@@ -1453,6 +1457,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D <= 3
         };
         constexpr bool FP32 = sizeof(T) == 4;
         const T wn = T(factor * double(n));  // FP32 sample weight factor B * n
+        const bool imp_form = jb.form == VMX_FORM_VR_OPTIMIZE_IMPORTANCE || jb.form == VMX_FORM_VR_FIXED_IMPORTANCE;
         KahanSum<T> sf, sa, mf, ma, maa, mfa;
         double acc1 = 0.0;  // alpha = 1 form
         T k_f = 0, k_a = 0;
@@ -1471,9 +1476,30 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D <= 3
                 x[d] = mul_add<FP32>(u.coord(d), ihi[d] - ilo[d], ilo[d]);
                 t[d] = sample_pos(x[d], rl, rh);
             }
+            T vol = box_volume<T, D>(ilo, ihi);
+            if constexpr (IMP && D <= imp::MAX_DIM && FP32) {
+                // region_sampling_importance: boxes of volume >= 1e-5 are sampled from the
+                // region polynomial's pdf with weight 1/pdf, smaller ones uniformly
+                if (imp_form && !empty && vol >= T(1.e-5)) {
+                    T rlo[D], rhi[D], xi[D], s01[D], wgt;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        rlo[d] = r.lo[size_t(reg) * D + d];
+                        rhi[d] = r.hi[size_t(reg) * D + d];
+                        s01[d] = u.coord(d);
+                    }
+                    if (imp::sample<T, D>(r.v + size_t(reg) * K, rlo, rhi, r.vol[reg], ilo, ihi, s01, xi, wgt)) {
+#pragma unroll
+                        for (int d = 0; d < D; ++d) {
+                            x[d] = xi[d];
+                            t[d] = sample_pos(x[d], rlo[d], rhi[d]);
+                        }
+                        vol = wgt;
+                    }
+                }
+            }
             const T fx = bj >= 0 ? residual_eval<M, D>(f_lds, x) : residual_eval<M, D>(fs[j], x);
             const T px = eval_monomials<T, D, FP32>(r.coef + size_t(reg) * K, t);
-            const T vol = box_volume<T, D>(ilo, ihi);
             if (jb.form == VMX_FORM_CV_ALPHA1) {
                 if (!empty) acc1 += double(fx - px) * (factor * double(n) * double(vol)) / double(spp);
                 continue;
@@ -1506,12 +1532,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D <= 3
         }
         double result;
         const double nsd = double(spp);
-        if (!sampling || (jb.form != VMX_FORM_VR_FIXED && spp < 2)) {
+        const bool fixed_w = jb.form == VMX_FORM_VR_FIXED || jb.form == VMX_FORM_VR_FIXED_IMPORTANCE;
+        if (!sampling || (!fixed_w && spp < 2)) {
             result = approx;
         } else {
             const double Sf = double(sf.s) - double(sf.c);
             const double Sa = double(sa.s) - double(sa.c);
-            if (jb.form == VMX_FORM_VR_FIXED) {
+            if (fixed_w) {
                 result = (Sf - jb.alpha * Sa) / nsd + jb.alpha * approx;
             } else {
                 const double Ef = double(mf.s) - double(mf.c);
