@@ -15,6 +15,7 @@ Workloads:
   config5            16 5-D integrals at 1024x1024 bins, 65536 regions, 32 spp, one
                      batched call with the learned priority (heuristic beside it)
   fubini             integrator_crespo2021_infinite<2> on a 3-D integrand
+  importance         importance-sampled residual vs uniform, 64 integrals at 256x256 bins
   config1            2-D config-1 integrand, 64x64 bins, 1024 regions, 16 spp
 """
 import argparse
@@ -315,6 +316,62 @@ def run_fubini(args, V, hip):
     }), flush=True)
 
 
+def run_importance(args, V, hip):
+    """Importance-sampled residual (region_sampling_importance) against uniform sampling:
+    64 held-out config-1 style integrands at 256x256 FP32 bins (bin boxes above the
+    sampler's 1e-5 volume threshold), 4096 regions, 16 spp, one batched call each way.
+    RMSE per bin against the FP64 closed-form bins.  value = bins/s with importance
+    sampling."""
+    from viltrum_amd import integrands as I
+    from viltrum_amd import refiner as R
+    ctx = V.Context(0)
+    fs = R.held_out(args.integrals)
+    res = (256, 256)
+    nb = res[0] * res[1]
+    truths = [ctx.exact_bins(f, res) for f in fs]
+    bufs = [hip.DeviceBuffer(nb * 4) for _ in fs]
+    ptrs = [b.address for b in bufs]
+    stream = hip.Stream()
+    out = {}
+    for name, form in (("uniform", V.FORM_VR_OPTIMIZE), ("importance", V.FORM_VR_OPTIMIZE_IMPORTANCE)):
+        ps = [V.Integrator(4095, 16, 0, form=form).problem(f, res, V.range_primary(2), V.F32) for f in fs]
+        tune = ctx.autotune(ps, fs, ptrs, stream.address)
+        for _ in range(args.warmup):
+            ctx.integrate_batch(ps, fs, ptrs, stream.address)
+        hip.device_synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            st = ctx.integrate_batch(ps, fs, ptrs, stream.address)
+        hip.device_synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        se2 = 0.0
+        h = np.zeros(res, dtype=np.float32)
+        for b, t in zip(bufs, truths):
+            b.to_host(h)
+            se2 += float(np.sum((h.astype(np.float64) - t) ** 2))
+        out[name] = {"ms": ms, "bins_per_s": nb * len(fs) / (ms * 1e-3), "rmse": float(np.sqrt(se2 / (nb * len(fs)))),
+                     "kernel_ms": float(st[0].ms_bins), "autotune": tune}
+    imp = out["importance"]
+    flops = sum(x.evals_residual for x in st) * I.flops_per_eval(fs[0])[0]
+    achieved = flops / (imp["kernel_ms"] * 1e-3) / 1e12
+    res_out = {
+        "metric": METRIC, "value": imp["bins_per_s"], "unit": "bins/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": imp["ms"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "importance-sampled residual: %d held-out config-1 style integrands, 256x256 fp32 "
+                               "bins, 4096 regions, 16 spp, one batched call" % len(fs),
+                   "integrals": len(fs), "bins": nb * len(fs), "regions": 4096, "spp": 16,
+                   "path": "batched, region_sampling_importance"},
+        "rmse_vs_fp64": {"importance": imp["rmse"], "uniform": out["uniform"]["rmse"]},
+        "fp64_truth": {"kind": "FP64 closed-form per-bin integrals (erf, polygon area, separable wave)"},
+        "uniform": out["uniform"],
+        "roofline": {"bound": "valu", "kernel": "k_bins1 (importance-sampling instantiation, batched)",
+                     "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel_ms": imp["kernel_ms"]},
+    }
+    print(json.dumps(res_out), flush=True)
+
+
 def c5_truth(ctx, V, fs, res, spp=4096, side=8, grid=4):
     """FP64 stratified per-bin Monte Carlo (spp samples per bin) on grid x grid blocks of
     side x side bins spread evenly over the bin grid of each integral: the same bin boxes
@@ -537,6 +594,8 @@ def main():
         return run_config5(args, V, hip)
     if args.workload == "fubini":
         return run_fubini(args, V, hip)
+    if args.workload == "importance":
+        return run_importance(args, V, hip)
     w = workloads()[args.workload]
     f = w["f"]
     ctx = V.Context(0)
