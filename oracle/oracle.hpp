@@ -68,7 +68,9 @@ enum Integrator {
     INT_VR_FIXED = 1,
     INT_CV_ALPHA1 = 2,
     INT_VR_OPTIMIZE_IMPORTANCE = 3,  // the two forms above with region_sampling_importance
-    INT_VR_FIXED_IMPORTANCE = 4
+    INT_VR_FIXED_IMPORTANCE = 4,
+    INT_VR_OPTIMIZE_RR_INTEGRAL = 5,  // ... with the rr_integral_region roulette
+    INT_VR_FIXED_RR_INTEGRAL = 6
 };
 
 /* A region: box [lo, hi], its cached volume and 3^D node values, dim 0 fastest
@@ -762,7 +764,7 @@ inline T sample(const Region<T, D>& r, const T* slo, const T* shi, RNG& rng, T* 
    region roulette (region-russian-roulette.h:9-28); uniform in-cell sampling
    (region-sampling.h:9-20); bins assigned.  Threads split dim-0 rows as
    src/foreach.h:43-78 does; results do not depend on the thread count. */
-template <typename T, int D, typename F, typename Acc, bool IMPORTANCE = false>
+template <typename T, int D, typename F, typename Acc, bool IMPORTANCE = false, bool RR_INTEGRAL = false>
 void integrate_regions_vr(const std::vector<Region<T, D>>& regs, const F& f, const T* lo, const T* hi, int db,
                           const uint64_t* res, uint64_t spp, uint64_t seed, double alpha, int nthreads, T* out,
                           Stats* st) {
@@ -831,9 +833,32 @@ void integrate_regions_vr(const std::vector<Region<T, D>>& regs, const F& f, con
                 if (spp > 0 && n == 0) throw std::runtime_error("bin with no region (undefined in the reference)");
                 if (spp > 0) {
                     std::uniform_int_distribution<std::size_t> rr(0, n - 1);
+                    // rr_integral_region (region-russian-roulette.h:30-67): weights |integral of
+                    // P_r over the bin's box|, floored at 1% of their mean (all 1 if they sum
+                    // to <= 0), std::discrete_distribution, factor 1/p
+                    std::discrete_distribution<std::size_t> rrw;
+                    if constexpr (RR_INTEGRAL) {
+                        std::vector<double> w(n);
+                        for (uint64_t k = 0; k < n; ++k)
+                            w[k] = std::abs(integral_subrange(regs[lst[off[b] + k]], &ilo[k * D], &ihi[k * D]));
+                        double sum = 0.0;
+                        for (double x : w) sum += x;
+                        if (sum <= 0.0)
+                            for (double& x : w) x = 1.0;
+                        else
+                            for (double& x : w) x = std::max(x, 0.01 * sum / double(w.size()));
+                        rrw = std::discrete_distribution<std::size_t>(w.begin(), w.end());
+                    }
                     for (uint64_t s = 0; s < spp; ++s) {
-                        const std::size_t chosen = rr(rng);
-                        const double rrfactor = double(rr.max() + 1);
+                        std::size_t chosen;
+                        double rrfactor;
+                        if constexpr (RR_INTEGRAL) {
+                            chosen = rrw(rng);
+                            rrfactor = 1.0 / rrw.probabilities()[chosen];
+                        } else {
+                            chosen = rr(rng);
+                            rrfactor = double(rr.max() + 1);
+                        }
                         const Region<T, D>& r = regs[lst[off[b] + chosen]];
                         const T* slo = &ilo[chosen * D];
                         const T* shi = &ihi[chosen * D];

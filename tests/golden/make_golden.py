@@ -58,6 +58,11 @@ BIN_CASES = [  # (integrand, scalar, form, heuristic, res, iterations, spp, seed
     ("poly2", 4, 3, 0, (8, 8), 100, 8, 5, 1.0),
     ("poly3", 4, 4, 0, (4, 4), 200, 4, 2, 1.0),
     ("blobs3d", 4, 3, 0, (8, 8), 300, 8, 5, 1.0),  # regions of zeros: the reference's 0/0 pdf gives NaN bins
+    # rr_integral_region (forms 5 and 6: optimised and fixed weight)
+    ("config1", 8, 5, 0, (16, 16), 255, 8, 5, 1.0),
+    ("config1", 4, 6, 0, (8, 8), 63, 16, 3, 0.7),
+    ("blobs3d", 8, 5, 0, (8, 8), 300, 4, 5, 1.0),
+    ("lobes5", 4, 5, 0, (3, 3), 60, 4, 2, 1.0),
 ]
 
 FUBINI_CASES = [  # (integrand, region dims, res, iterations, mc samples, spp, seed): integrator_crespo2021_infinite
