@@ -77,7 +77,14 @@ enum vmx_form {
        samples drawn from each region polynomial's |P| pdf over the bin's box, weighted
        by 1/pdf.  FP32, dims 1..3 (the reference compiles it for float regions only). */
     VMX_FORM_VR_OPTIMIZE_IMPORTANCE = 3,
-    VMX_FORM_VR_FIXED_IMPORTANCE = 4
+    VMX_FORM_VR_FIXED_IMPORTANCE = 4,
+    /* the two variance-reduction forms with rr_integral_region in place of
+       rr_uniform_region (src/control-variates/region-russian-roulette.h:30-67): a
+       residual sample picks region r of its bin with probability proportional to
+       |integral of P_r over the bin's box| (floored at 1% of the mean), weight 1/p.
+       A batch holding these forms keeps one footprint per region (no merging). */
+    VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL = 5,
+    VMX_FORM_VR_FIXED_RR_INTEGRAL = 6
 };
 
 /* Order of the final region list (and so of every per-bin region list). */

@@ -306,6 +306,19 @@ inline IntegratorGpu integrator_variance_reduction_importance(std::size_t iterat
                                                               int heuristic = VMX_HEUR_SIZE_RELATIVE) {
     return IntegratorGpu(iterations, spp, seed, heuristic, VMX_FORM_VR_OPTIMIZE_IMPORTANCE, 1.0);
 }
+/* regions_integrator_parallel_variance_reduction(rr_integral_region(), cv_optimize_weight(),
+   region_sampling_uniform(), rng, spp) (src/control-variates/region-russian-roulette.h:30-67) */
+inline IntegratorGpu integrator_variance_reduction_rr_integral(std::size_t iterations, std::size_t spp,
+                                                               std::size_t seed = 0,
+                                                               int heuristic = VMX_HEUR_SIZE_RELATIVE) {
+    return IntegratorGpu(iterations, spp, seed, heuristic, VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL, 1.0);
+}
+/* ... with cv_fixed_weight(alpha) */
+inline IntegratorGpu integrator_variance_reduction_rr_integral_fixed(double alpha, std::size_t iterations,
+                                                                     std::size_t spp, std::size_t seed = 0,
+                                                                     int heuristic = VMX_HEUR_SIZE_RELATIVE) {
+    return IntegratorGpu(iterations, spp, seed, heuristic, VMX_FORM_VR_FIXED_RR_INTEGRAL, alpha);
+}
 /* ... with cv_fixed_weight(alpha) */
 inline IntegratorGpu integrator_variance_reduction_importance_fixed(double alpha, std::size_t iterations,
                                                                     std::size_t spp, std::size_t seed = 0,

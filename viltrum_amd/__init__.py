@@ -32,6 +32,7 @@ F32, F64 = 4, 8
 HEUR_SIZE_RELATIVE, HEUR_DEFAULT_ABSOLUTE = 0, 1
 FORM_VR_OPTIMIZE, FORM_VR_FIXED, FORM_CV_ALPHA1 = 0, 1, 2
 FORM_VR_OPTIMIZE_IMPORTANCE, FORM_VR_FIXED_IMPORTANCE = 3, 4  # with region_sampling_importance
+FORM_VR_OPTIMIZE_RR_INTEGRAL, FORM_VR_FIXED_RR_INTEGRAL = 5, 6  # with the rr_integral_region roulette
 ORDER_CANONICAL, ORDER_REFERENCE = 0, 1
 REFINER_HEURISTIC, REFINER_LEARNED = 0, 1
 
@@ -439,6 +440,17 @@ def integrator_variance_reduction_importance(iterations, spp, seed=0, alpha=None
     if alpha is None:
         return Integrator(iterations, spp, seed, heuristic, FORM_VR_OPTIMIZE_IMPORTANCE, **kw)
     return Integrator(iterations, spp, seed, heuristic, FORM_VR_FIXED_IMPORTANCE, alpha=alpha, **kw)
+
+
+def integrator_variance_reduction_rr_integral(iterations, spp, seed=0, alpha=None, heuristic=HEUR_SIZE_RELATIVE,
+                                              **kw):
+    """regions_integrator_parallel_variance_reduction(rr_integral_region(), cv_optimize_weight()
+    or cv_fixed_weight(alpha), region_sampling_uniform(), ...)
+    (src/control-variates/region-russian-roulette.h:30-67): a residual sample picks the
+    bin's region r with probability ~ |integral of P_r over the bin's box|, weight 1/p."""
+    if alpha is None:
+        return Integrator(iterations, spp, seed, heuristic, FORM_VR_OPTIMIZE_RR_INTEGRAL, **kw)
+    return Integrator(iterations, spp, seed, heuristic, FORM_VR_FIXED_RR_INTEGRAL, alpha=alpha, **kw)
 
 
 def integrator_variance_reduction_fixed(alpha, iterations, spp, seed=0, heuristic=HEUR_SIZE_RELATIVE, **kw):

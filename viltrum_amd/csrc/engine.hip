@@ -252,7 +252,14 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     for (const JobIn& x : in)
         if ((x.given != nullptr) != replay) throw VmxError(VMX_ERR_INVALID, "given regions for all or none");
     if (replay && pairs_req) throw VmxError(VMX_ERR_INVALID, "training pairs need generation");
-    const bool merge = order == VMX_ORDER_CANONICAL && !replay;
+    // the rr_integral_region roulette weighs each region of a bin: one footprint per region
+    bool any_rr = false, any_imp = false;
+    for (const JobIn& x : in) {
+        any_rr |= x.p->form == VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL || x.p->form == VMX_FORM_VR_FIXED_RR_INTEGRAL;
+        any_imp |= x.p->form == VMX_FORM_VR_OPTIMIZE_IMPORTANCE || x.p->form == VMX_FORM_VR_FIXED_IMPORTANCE;
+    }
+    if (any_rr && any_imp) throw VmxError(VMX_ERR_INVALID, "importance sampling and rr_integral_region: one per batch");
+    const bool merge = order == VMX_ORDER_CANONICAL && !replay && !any_rr;
 
     std::vector<vmx::Job<T>> jobs(J);
     std::vector<P> fparams(J);
@@ -795,6 +802,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     mp.take<uint32_t>(size_t(pairs) / vmx::SORT_SHORT + 2);   // queue of long lists
     mp.take<uint32_t>(size_t(pairs) + 1);                     // prefix counts of long lists
     mp.take<uint32_t>(size_t(pairs) + 1);                     // first regions of long lists
+    if (any_rr) mp.take<double>(size_t(pairs) + 1);          // roulette weights per pair
     mp.take<char>(scan_bins_bytes + 256);
     ensure(ctx->bins2, mp.used);
     Carve cp(ctx->bins2);
@@ -803,6 +811,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     uint32_t* d_lst_tmp = cp.take<uint32_t>(size_t(pairs) + 1);
     uint32_t* d_qlong = cp.take<uint32_t>(size_t(pairs) / vmx::SORT_SHORT + 2);
     vmx::ListPrefix lp{cp.take<uint32_t>(size_t(pairs) + 1), cp.take<uint32_t>(size_t(pairs) + 1)};
+    double* d_rrw = any_rr ? cp.take<double>(size_t(pairs) + 1) : nullptr;
     void* d_scan_bins_tmp = cp.take<char>(scan_bins_bytes + 256);
     HIP_CHECK(hipMemsetAsync(d_cnt, 0, sizeof(uint32_t) * (total_bins + 1), st));
     HIP_CHECK(hipMemsetAsync(d_qlong, 0, sizeof(uint32_t), st));
@@ -829,27 +838,36 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     const int db = jobs[0].db;
     // the importance-sampling variant is its own instantiation: its register footprint
     // stays out of the uniform kernel
-    auto bins_kernel = [&](auto imp_tag) {
-        constexpr bool IMP = decltype(imp_tag)::value;
+    auto bins_kernel = [&](auto mode_tag) {
+        constexpr int MODE = decltype(mode_tag)::value;
         if (db == 1)
-            vmx::k_bins1<T, D, 1, P, Math, IMP><<<bins_grid, 256, 0, st>>>(d_jobs, J, total_bins, ra, fa, d_off,
-                                                                           d_lst, d_f, d_rt, lp, shape.bins_xcd);
+            vmx::k_bins1<T, D, 1, P, Math, MODE><<<bins_grid, 256, 0, st>>>(
+                d_jobs, J, total_bins, ra, fa, d_off, d_lst, d_f, d_rt, lp, shape.bins_xcd, d_rrw);
         else if constexpr (D >= 2) {
             if (db == 2)
-                vmx::k_bins1<T, D, 2, P, Math, IMP><<<bins_grid, 256, 0, st>>>(d_jobs, J, total_bins, ra, fa, d_off,
-                                                                               d_lst, d_f, d_rt, lp, shape.bins_xcd);
+                vmx::k_bins1<T, D, 2, P, Math, MODE><<<bins_grid, 256, 0, st>>>(
+                    d_jobs, J, total_bins, ra, fa, d_off, d_lst, d_f, d_rt, lp, shape.bins_xcd, d_rrw);
             else if constexpr (D >= 3)
-                vmx::k_bins1<T, D, 3, P, Math, IMP><<<bins_grid, 256, 0, st>>>(
-                    d_jobs, J, total_bins, ra, fa, d_off, d_lst, d_f, d_rt, lp, shape.bins_xcd);
+                vmx::k_bins1<T, D, 3, P, Math, MODE><<<bins_grid, 256, 0, st>>>(
+                    d_jobs, J, total_bins, ra, fa, d_off, d_lst, d_f, d_rt, lp, shape.bins_xcd, d_rrw);
         }
     };
-    if constexpr (std::is_same<T, float>::value && D <= vmx::imp::MAX_DIM && !vmx::IsFubini<P>::value) {
-        if (any_importance)
-            bins_kernel(std::true_type{});
-        else
-            bins_kernel(std::false_type{});
+    if constexpr (vmx::IsFubini<P>::value) {
+        bins_kernel(std::integral_constant<int, 0>{});
     } else {
-        bins_kernel(std::false_type{});
+        bool done = false;
+        if constexpr (std::is_same<T, float>::value && D <= vmx::imp::MAX_DIM) {
+            if (any_importance) {
+                bins_kernel(std::integral_constant<int, 1>{});
+                done = true;
+            }
+        }
+        if (!done) {
+            if (any_rr)
+                bins_kernel(std::integral_constant<int, 2>{});
+            else
+                bins_kernel(std::integral_constant<int, 0>{});
+        }
     }
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipEventRecord(ctx->ev[1], st));
@@ -1013,9 +1031,11 @@ void validate(const vmx_problem* p, const vmx_integrand* f) {
     if (p->regions < 1 || p->regions > (1ull << 26)) throw VmxError(VMX_ERR_INVALID, "regions must be 1..2^26");
     if (p->heuristic != VMX_HEUR_SIZE_RELATIVE && p->heuristic != VMX_HEUR_DEFAULT_ABSOLUTE)
         throw VmxError(VMX_ERR_INVALID, "unknown heuristic");
-    if (p->form < VMX_FORM_VR_OPTIMIZE || p->form > VMX_FORM_VR_FIXED_IMPORTANCE)
+    if (p->form < VMX_FORM_VR_OPTIMIZE || p->form > VMX_FORM_VR_FIXED_RR_INTEGRAL)
         throw VmxError(VMX_ERR_INVALID, "unknown form");
-    if (p->form >= VMX_FORM_VR_OPTIMIZE_IMPORTANCE &&
+    if (p->form >= VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL && f->family == VMX_FAMILY_FUBINI)
+        throw VmxError(VMX_ERR_UNSUPPORTED, "rr_integral_region: not for the Fubini form");
+    if ((p->form == VMX_FORM_VR_OPTIMIZE_IMPORTANCE || p->form == VMX_FORM_VR_FIXED_IMPORTANCE) &&
         (p->scalar != VMX_F32 || f->dim > vmx::imp::MAX_DIM || f->family == VMX_FAMILY_FUBINI))
         throw VmxError(VMX_ERR_UNSUPPORTED, "importance sampling: FP32 integrands of dimension 1..3");
     if (p->order != VMX_ORDER_CANONICAL && p->order != VMX_ORDER_REFERENCE)

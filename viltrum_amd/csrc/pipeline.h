@@ -1337,11 +1337,16 @@ __device__ inline float sample_pos<float>(float x, float lo, float hi) {
    Moments use the reference's shift by the bin's first sample; sums are
    Kahan-compensated in the evaluation type and finished in FP64.  Summation depends on
    the job alone, so a batched integral gives exactly what it gives alone. */
-template <typename T, int D, int DB, typename F, typename M, bool IMP = false>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!IMP && D <= 3 && sizeof(T) == 4 ? 4 : 1))) k_bins1(const Job<T>* __restrict__ jobs, uint32_t J, uint64_t total_bins,
+template <typename T, int D, int DB, typename F, typename M, int MODE = 0>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE != 1 && D <= 3 && sizeof(T) == 4 ? 4 : 1))) k_bins1(const Job<T>* __restrict__ jobs, uint32_t J, uint64_t total_bins,
                                                RegionArrays<T> r, FootArrays<T> fa,
                                                const uint32_t* __restrict__ off, const uint32_t* __restrict__ lst,
-                                               const F* __restrict__ fs, JobRt* rt, ListPrefix lp, int xcd) {
+                                               const F* __restrict__ fs, JobRt* rt, ListPrefix lp, int xcd,
+                                               double* __restrict__ rrw) {
+    // MODE 0: uniform region roulette and in-box sampling; 1: importance sampling in the
+    // box; 2: rr_integral_region roulette (rrw: per-list cumulative weights)
+    constexpr bool IMP = MODE == 1;
+    constexpr bool RRI = MODE == 2;
     constexpr int K = pow3(D);
     // per lane: first region and running region count of its first LANE_MAXF footprints
     __shared__ uint32_t s_first[LANE_MAXF][256];
@@ -1422,9 +1427,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!IMP &
                 aa[d] = double(pos_in(il, rl, rh));
                 bb[d] = double(pos_in(ih, rl, rh));
             }
-            approx += empty ? 0.0
-                            : factor * (fa.scale[fi] * marginal_subrange_n<double, DB>(fa.marg + size_t(fi) * jb.mb,
-                                                                                       aa, bb));
+            const double integ = empty ? 0.0
+                                       : fa.scale[fi] * marginal_subrange_n<double, DB>(fa.marg + size_t(fi) * jb.mb,
+                                                                                        aa, bb);
+            approx += empty ? 0.0 : factor * integ;
+            if constexpr (RRI) rrw[o + k] = vabs(integ);  // one region per footprint here
         }
         const uint64_t spp = jb.spp;
         const bool sampling = spp > 0 && n > 0;
@@ -1457,6 +1464,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!IMP &
         };
         constexpr bool FP32 = sizeof(T) == 4;
         const T wn = T(factor * double(n));  // FP32 sample weight factor B * n
+        const bool rr_form = RRI && (jb.form == VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL || jb.form == VMX_FORM_VR_FIXED_RR_INTEGRAL);
+        double rr_total = 0.0;
+        if (rr_form && spp > 0 && n > 0) {  // floor at 1% of the mean weight, then cumulate
+            double sum = 0.0;
+            for (uint32_t k = 0; k < n; ++k) sum += rrw[o + k];
+            double cum = 0.0;
+            for (uint32_t k = 0; k < n; ++k) {
+                const double w = sum <= 0.0 ? 1.0 : vmax(rrw[o + k], 0.01 * sum / double(n));
+                cum += w;
+                rrw[o + k] = cum;
+            }
+            rr_total = cum;
+        }
         const bool imp_form = jb.form == VMX_FORM_VR_OPTIMIZE_IMPORTANCE || jb.form == VMX_FORM_VR_FIXED_IMPORTANCE;
         KahanSum<T> sf, sa, mf, ma, maa, mfa;
         double acc1 = 0.0;  // alpha = 1 form
@@ -1464,7 +1484,25 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!IMP &
         const uint64_t ns_run = sampling ? spp : 0;
         for (uint64_t s = 0; s < ns_run; ++s) {
             const SampleDraws<T, D> u(uint32_t(s), b, jb.stream, jb.seed_lo, jb.seed_hi);
-            const uint32_t reg = region_of(uniform_index(u.u[0], n));
+            uint32_t pick = 0;
+            double rrfactor = double(n);
+            if (rr_form) {  // first cumulative weight above u * total
+                const double target = double(u.u[0]) * 0x1p-32 * rr_total;
+                uint32_t lo = 0, hi = n - 1;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) / 2;
+                    if (rrw[o + mid] > target)
+                        hi = mid;
+                    else
+                        lo = mid + 1;
+                }
+                pick = lo;
+                const double wk = rrw[o + pick] - (pick > 0 ? rrw[o + pick - 1] : 0.0);
+                rrfactor = rr_total / wk;
+            } else {
+                pick = uniform_index(u.u[0], n);
+            }
+            const uint32_t reg = region_of(pick);
             T x[D], t[D], ilo[D], ihi[D];
             bool empty = false;
 #pragma unroll
@@ -1506,11 +1544,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!IMP &
             }
             T fsv, asv;
             if constexpr (FP32) {  // weights in FP32: B * n * vol is O(n), far from the range limits
-                const T w = wn * vol;
+                const T w = (rr_form ? T(factor * rrfactor) : wn) * vol;
                 fsv = fx * w;
                 asv = px * w;
             } else {
-                const double w = factor * double(n) * double(vol);
+                const double w = factor * rrfactor * double(vol);
                 fsv = T(double(fx) * w);
                 asv = T(double(px) * w);
             }
@@ -1532,7 +1570,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!IMP &
         }
         double result;
         const double nsd = double(spp);
-        const bool fixed_w = jb.form == VMX_FORM_VR_FIXED || jb.form == VMX_FORM_VR_FIXED_IMPORTANCE;
+        const bool fixed_w = jb.form == VMX_FORM_VR_FIXED || jb.form == VMX_FORM_VR_FIXED_IMPORTANCE ||
+                             jb.form == VMX_FORM_VR_FIXED_RR_INTEGRAL;
         if (!sampling || (!fixed_w && spp < 2)) {
             result = approx;
         } else {
