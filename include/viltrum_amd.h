@@ -84,7 +84,11 @@ enum vmx_form {
        |integral of P_r over the bin's box| (floored at 1% of the mean), weight 1/p.
        A batch holding these forms keeps one footprint per region (no merging). */
     VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL = 5,
-    VMX_FORM_VR_FIXED_RR_INTEGRAL = 6
+    VMX_FORM_VR_FIXED_RR_INTEGRAL = 6,
+    /* ... with rr_error_region (region-russian-roulette.h:68-104): probability
+       proportional to |Region::error()| times the share of the region inside the bin */
+    VMX_FORM_VR_OPTIMIZE_RR_ERROR = 7,
+    VMX_FORM_VR_FIXED_RR_ERROR = 8
 };
 
 /* Order of the final region list (and so of every per-bin region list). */

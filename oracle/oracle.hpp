@@ -70,7 +70,9 @@ enum Integrator {
     INT_VR_OPTIMIZE_IMPORTANCE = 3,  // the two forms above with region_sampling_importance
     INT_VR_FIXED_IMPORTANCE = 4,
     INT_VR_OPTIMIZE_RR_INTEGRAL = 5,  // ... with the rr_integral_region roulette
-    INT_VR_FIXED_RR_INTEGRAL = 6
+    INT_VR_FIXED_RR_INTEGRAL = 6,
+    INT_VR_OPTIMIZE_RR_ERROR = 7,  // ... with the rr_error_region roulette
+    INT_VR_FIXED_RR_ERROR = 8
 };
 
 /* A region: box [lo, hi], its cached volume and 3^D node values, dim 0 fastest
@@ -764,7 +766,20 @@ inline T sample(const Region<T, D>& r, const T* slo, const T* shi, RNG& rng, T* 
    region roulette (region-russian-roulette.h:9-28); uniform in-cell sampling
    (region-sampling.h:9-20); bins assigned.  Threads split dim-0 rows as
    src/foreach.h:43-78 does; results do not depend on the thread count. */
-template <typename T, int D, typename F, typename Acc, bool IMPORTANCE = false, bool RR_INTEGRAL = false>
+/* Region::error() (src/newton-cotes/region.h:421-424): volume times the difference of the
+   full-tensor Simpson and trapezoid (outer nodes) folds, dim 0 folded first. */
+template <typename T, int D>
+T region_error(const Region<T, D>& r) {
+    std::vector<T> s(r.v, r.v + Region<T, D>::K), t(s);
+    for (size_t n = s.size(); n > 1; n /= 3)
+        for (size_t j = 0; j < n / 3; ++j) {
+            s[j] = simpson(s[3 * j], s[3 * j + 1], s[3 * j + 2]);
+            t[j] = trapezoid(t[3 * j], t[3 * j + 2]);
+        }
+    return r.vol * (s[0] - t[0]);
+}
+
+template <typename T, int D, typename F, typename Acc, bool IMPORTANCE = false, int RR = 0>
 void integrate_regions_vr(const std::vector<Region<T, D>>& regs, const F& f, const T* lo, const T* hi, int db,
                           const uint64_t* res, uint64_t spp, uint64_t seed, double alpha, int nthreads, T* out,
                           Stats* st) {
@@ -836,11 +851,18 @@ void integrate_regions_vr(const std::vector<Region<T, D>>& regs, const F& f, con
                     // rr_integral_region (region-russian-roulette.h:30-67): weights |integral of
                     // P_r over the bin's box|, floored at 1% of their mean (all 1 if they sum
                     // to <= 0), std::discrete_distribution, factor 1/p
+                    // rr_error_region (:68-104): weights |Region::error()| times the share of
+                    // the region's volume inside the bin
                     std::discrete_distribution<std::size_t> rrw;
-                    if constexpr (RR_INTEGRAL) {
+                    if constexpr (RR != 0) {
                         std::vector<double> w(n);
-                        for (uint64_t k = 0; k < n; ++k)
-                            w[k] = std::abs(integral_subrange(regs[lst[off[b] + k]], &ilo[k * D], &ihi[k * D]));
+                        for (uint64_t k = 0; k < n; ++k) {
+                            const Region<T, D>& rk = regs[lst[off[b] + k]];
+                            if constexpr (RR == 1)
+                                w[k] = std::abs(integral_subrange(rk, &ilo[k * D], &ihi[k * D]));
+                            else
+                                w[k] = std::abs(region_error(rk)) * box_volume<T, D>(&ilo[k * D], &ihi[k * D]) / rk.vol;
+                        }
                         double sum = 0.0;
                         for (double x : w) sum += x;
                         if (sum <= 0.0)
@@ -852,7 +874,7 @@ void integrate_regions_vr(const std::vector<Region<T, D>>& regs, const F& f, con
                     for (uint64_t s = 0; s < spp; ++s) {
                         std::size_t chosen;
                         double rrfactor;
-                        if constexpr (RR_INTEGRAL) {
+                        if constexpr (RR != 0) {
                             chosen = rrw(rng);
                             rrfactor = 1.0 / rrw.probabilities()[chosen];
                         } else {

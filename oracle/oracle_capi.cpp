@@ -160,11 +160,17 @@ int run_regions(int integrator, double alpha, const F& fn, const std::vector<vmo
         vmo::integrate_regions_vr<T, D, F, vmo::AccFixed<T>, true>(regs, fn, tlo, thi, db, res, spp, seed, alpha,
                                                                      nthreads, out.data(), st);
     else if (integrator == vmo::INT_VR_OPTIMIZE_RR_INTEGRAL)
-        vmo::integrate_regions_vr<T, D, F, vmo::AccOptimize<T>, false, true>(regs, fn, tlo, thi, db, res, spp, seed,
-                                                                               alpha, nthreads, out.data(), st);
-    else if (integrator == vmo::INT_VR_FIXED_RR_INTEGRAL)
-        vmo::integrate_regions_vr<T, D, F, vmo::AccFixed<T>, false, true>(regs, fn, tlo, thi, db, res, spp, seed,
+        vmo::integrate_regions_vr<T, D, F, vmo::AccOptimize<T>, false, 1>(regs, fn, tlo, thi, db, res, spp, seed,
                                                                             alpha, nthreads, out.data(), st);
+    else if (integrator == vmo::INT_VR_FIXED_RR_INTEGRAL)
+        vmo::integrate_regions_vr<T, D, F, vmo::AccFixed<T>, false, 1>(regs, fn, tlo, thi, db, res, spp, seed,
+                                                                         alpha, nthreads, out.data(), st);
+    else if (integrator == vmo::INT_VR_OPTIMIZE_RR_ERROR)
+        vmo::integrate_regions_vr<T, D, F, vmo::AccOptimize<T>, false, 2>(regs, fn, tlo, thi, db, res, spp, seed,
+                                                                            alpha, nthreads, out.data(), st);
+    else if (integrator == vmo::INT_VR_FIXED_RR_ERROR)
+        vmo::integrate_regions_vr<T, D, F, vmo::AccFixed<T>, false, 2>(regs, fn, tlo, thi, db, res, spp, seed,
+                                                                         alpha, nthreads, out.data(), st);
     else {
         g_err = "unknown integrator";
         return -1;

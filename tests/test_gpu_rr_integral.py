@@ -1,6 +1,7 @@
-"""rr_integral_region (src/control-variates/region-russian-roulette.h:30-67): residual
-samples pick a bin's region with probability proportional to |integral of its
-polynomial over the bin's box| (floored at 1% of the mean), weight 1/p.  The GPU's
+"""rr_integral_region and rr_error_region (src/control-variates/region-russian-roulette.h:30-104):
+residual samples pick a bin's region with probability proportional to |integral of its
+polynomial over the bin's box|, or to |Region::error()| times the region's share inside
+the bin (floored at 1% of the mean), weight 1/p.  The GPU's
 Philox stream and the oracle's mt19937 + std::discrete_distribution (the oracle is
 pinned bit for bit to the reference build) run the same estimator: per-bin means over
 independent seeds agree within their standard errors, with the same spread."""
@@ -23,7 +24,8 @@ pytestmark = pytest.mark.gpu
     (poly_cases()[2], (6, 6), 150, 8),
     (poly_cases()[3], (4, 4), 400, 8),
 ], ids=lambda v: getattr(v, "name", str(v)))
-@pytest.mark.parametrize("form", [V.FORM_VR_FIXED_RR_INTEGRAL, V.FORM_VR_OPTIMIZE_RR_INTEGRAL], ids=["fixed1", "optimize"])
+@pytest.mark.parametrize("form", [V.FORM_VR_FIXED_RR_INTEGRAL, V.FORM_VR_OPTIMIZE_RR_INTEGRAL, V.FORM_VR_FIXED_RR_ERROR,
+                                  V.FORM_VR_OPTIMIZE_RR_ERROR], ids=["int_fixed1", "int_optimize", "err_fixed1", "err_optimize"])
 @pytest.mark.parametrize("scalar", [4, 8])
 def test_rr_integral_matches_oracle(ctx, f, res, it, spp, form, scalar):
     dt = np.float32 if scalar == 4 else np.float64

@@ -253,9 +253,10 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         if ((x.given != nullptr) != replay) throw VmxError(VMX_ERR_INVALID, "given regions for all or none");
     if (replay && pairs_req) throw VmxError(VMX_ERR_INVALID, "training pairs need generation");
     // the rr_integral_region roulette weighs each region of a bin: one footprint per region
-    bool any_rr = false, any_imp = false;
+    bool any_rr = false, any_imp = false, any_rr_err = false;
     for (const JobIn& x : in) {
-        any_rr |= x.p->form == VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL || x.p->form == VMX_FORM_VR_FIXED_RR_INTEGRAL;
+        any_rr |= x.p->form >= VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL && x.p->form <= VMX_FORM_VR_FIXED_RR_ERROR;
+        any_rr_err |= x.p->form == VMX_FORM_VR_OPTIMIZE_RR_ERROR || x.p->form == VMX_FORM_VR_FIXED_RR_ERROR;
         any_imp |= x.p->form == VMX_FORM_VR_OPTIMIZE_IMPORTANCE || x.p->form == VMX_FORM_VR_FIXED_IMPORTANCE;
     }
     if (any_rr && any_imp) throw VmxError(VMX_ERR_INVALID, "importance sampling and rr_integral_region: one per batch");
@@ -719,6 +720,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     for (const auto& jb : jobs)
         any_importance |= jb.form == VMX_FORM_VR_OPTIMIZE_IMPORTANCE || jb.form == VMX_FORM_VR_FIXED_IMPORTANCE;
     if (any_importance) mr.take<T>(n_reg * K);
+    if (any_rr_err) mr.take<T>(n_reg);
     mr.take<double>(n_reg * mb_max);
     mr.take<uint32_t>(n_reg);
     mr.take<uint32_t>(n_reg + 1);
@@ -741,6 +743,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     ra.vol = cr.take<T>(n_reg);
     ra.coef = cr.take<T>(n_reg * K);
     ra.v = any_importance ? cr.take<T>(n_reg * K) : nullptr;
+    ra.err = any_rr_err ? cr.take<T>(n_reg) : nullptr;
     ra.marg = cr.take<double>(n_reg * mb_max);
     ra.job = cr.take<uint32_t>(n_reg);
     ra.fpflag = cr.take<uint32_t>(n_reg + 1);
@@ -1031,7 +1034,7 @@ void validate(const vmx_problem* p, const vmx_integrand* f) {
     if (p->regions < 1 || p->regions > (1ull << 26)) throw VmxError(VMX_ERR_INVALID, "regions must be 1..2^26");
     if (p->heuristic != VMX_HEUR_SIZE_RELATIVE && p->heuristic != VMX_HEUR_DEFAULT_ABSOLUTE)
         throw VmxError(VMX_ERR_INVALID, "unknown heuristic");
-    if (p->form < VMX_FORM_VR_OPTIMIZE || p->form > VMX_FORM_VR_FIXED_RR_INTEGRAL)
+    if (p->form < VMX_FORM_VR_OPTIMIZE || p->form > VMX_FORM_VR_FIXED_RR_ERROR)
         throw VmxError(VMX_ERR_INVALID, "unknown form");
     if (p->form >= VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL && f->family == VMX_FAMILY_FUBINI)
         throw VmxError(VMX_ERR_UNSUPPORTED, "rr_integral_region: not for the Fubini form");

@@ -998,7 +998,8 @@ __global__ void k_leaf_keys64(const int32_t* __restrict__ leaf_slot, const uint3
 template <typename T>
 struct RegionArrays {
     T *lo, *hi, *vol, *coef;
-    T* v;  // node values (3^D per region) for importance sampling, else null
+    T* v;    // node values (3^D per region) for importance sampling, else null
+    T* err;  // Region::error() per region for the rr_error_region roulette, else null
     double* marg;
     uint32_t* job;
     uint32_t* fpflag;  // [n+1]
@@ -1044,6 +1045,16 @@ __global__ void k_prepare2(const Job<T>* __restrict__ jobs, const int32_t* __res
     if (r.v)
         for (int k = 0; k < K; ++k) r.v[size_t(i) * K + k] = v[k];
     r.vol[i] = box_volume<T, D>(lo, hi);
+    if (r.err) {  // Region::error() (src/newton-cotes/region.h:421-424): dim 0 folded first
+        T s[K], tz[K];
+        for (int k = 0; k < K; ++k) s[k] = tz[k] = v[k];
+        for (int m = K; m > 1; m /= 3)
+            for (int q = 0; q < m / 3; ++q) {
+                s[q] = simpson(s[3 * q], s[3 * q + 1], s[3 * q + 2]);
+                tz[q] = trapezoid(tz[3 * q], tz[3 * q + 2]);
+            }
+        r.err[i] = r.vol[i] * (s[0] - tz[0]);
+    }
     double m[27];
     bin_marginal<T, double, D>(v, jb.db, m);
     for (int k = 0; k < mb; ++k) r.marg[size_t(i) * mb + k] = m[k];
@@ -1431,7 +1442,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE !
                                        : fa.scale[fi] * marginal_subrange_n<double, DB>(fa.marg + size_t(fi) * jb.mb,
                                                                                         aa, bb);
             approx += empty ? 0.0 : factor * integ;
-            if constexpr (RRI) rrw[o + k] = vabs(integ);  // one region per footprint here
+            if constexpr (RRI) {  // one region per footprint here
+                if (jb.form == VMX_FORM_VR_OPTIMIZE_RR_ERROR || jb.form == VMX_FORM_VR_FIXED_RR_ERROR) {
+                    double share = empty ? 0.0 : 1.0;  // the bin's share of the region
+#pragma unroll
+                    for (int d = 0; d < DB; ++d) share *= bb[d] - aa[d];
+                    rrw[o + k] = vabs(double(r.err[fa.first[fi]])) * share;
+                } else {
+                    rrw[o + k] = vabs(integ);
+                }
+            }
         }
         const uint64_t spp = jb.spp;
         const bool sampling = spp > 0 && n > 0;
@@ -1464,7 +1484,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE !
         };
         constexpr bool FP32 = sizeof(T) == 4;
         const T wn = T(factor * double(n));  // FP32 sample weight factor B * n
-        const bool rr_form = RRI && (jb.form == VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL || jb.form == VMX_FORM_VR_FIXED_RR_INTEGRAL);
+        const bool rr_form = RRI && jb.form >= VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL && jb.form <= VMX_FORM_VR_FIXED_RR_ERROR;
         double rr_total = 0.0;
         if (rr_form && spp > 0 && n > 0) {  // floor at 1% of the mean weight, then cumulate
             double sum = 0.0;
@@ -1571,7 +1591,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE !
         double result;
         const double nsd = double(spp);
         const bool fixed_w = jb.form == VMX_FORM_VR_FIXED || jb.form == VMX_FORM_VR_FIXED_IMPORTANCE ||
-                             jb.form == VMX_FORM_VR_FIXED_RR_INTEGRAL;
+                             jb.form == VMX_FORM_VR_FIXED_RR_INTEGRAL || jb.form == VMX_FORM_VR_FIXED_RR_ERROR;
         if (!sampling || (!fixed_w && spp < 2)) {
             result = approx;
         } else {
