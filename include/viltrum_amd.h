@@ -88,7 +88,12 @@ enum vmx_form {
     /* ... with rr_error_region (region-russian-roulette.h:68-104): probability
        proportional to |Region::error()| times the share of the region inside the bin */
     VMX_FORM_VR_OPTIMIZE_RR_ERROR = 7,
-    VMX_FORM_VR_FIXED_RR_ERROR = 8
+    VMX_FORM_VR_FIXED_RR_ERROR = 8,
+    /* ... with rr_pdf_region (region-russian-roulette.h:106-149): probability
+       proportional to the integral of the importance pdf over the bin's box
+       (Region::pdf_integral_subrange), floored at 1% of the mean */
+    VMX_FORM_VR_OPTIMIZE_RR_PDF = 9,
+    VMX_FORM_VR_FIXED_RR_PDF = 10
 };
 
 /* Order of the final region list (and so of every per-bin region list). */

@@ -330,6 +330,17 @@ inline IntegratorGpu integrator_variance_reduction_rr_error_fixed(double alpha, 
                                                                   int heuristic = VMX_HEUR_SIZE_RELATIVE) {
     return IntegratorGpu(iterations, spp, seed, heuristic, VMX_FORM_VR_FIXED_RR_ERROR, alpha);
 }
+/* ... with rr_pdf_region() (src/control-variates/region-russian-roulette.h:106-149) */
+inline IntegratorGpu integrator_variance_reduction_rr_pdf(std::size_t iterations, std::size_t spp,
+                                                          std::size_t seed = 0,
+                                                          int heuristic = VMX_HEUR_SIZE_RELATIVE) {
+    return IntegratorGpu(iterations, spp, seed, heuristic, VMX_FORM_VR_OPTIMIZE_RR_PDF, 1.0);
+}
+inline IntegratorGpu integrator_variance_reduction_rr_pdf_fixed(double alpha, std::size_t iterations,
+                                                                std::size_t spp, std::size_t seed = 0,
+                                                                int heuristic = VMX_HEUR_SIZE_RELATIVE) {
+    return IntegratorGpu(iterations, spp, seed, heuristic, VMX_FORM_VR_FIXED_RR_PDF, alpha);
+}
 /* ... with cv_fixed_weight(alpha) */
 inline IntegratorGpu integrator_variance_reduction_importance_fixed(double alpha, std::size_t iterations,
                                                                     std::size_t spp, std::size_t seed = 0,

@@ -72,7 +72,9 @@ enum Integrator {
     INT_VR_OPTIMIZE_RR_INTEGRAL = 5,  // ... with the rr_integral_region roulette
     INT_VR_FIXED_RR_INTEGRAL = 6,
     INT_VR_OPTIMIZE_RR_ERROR = 7,  // ... with the rr_error_region roulette
-    INT_VR_FIXED_RR_ERROR = 8
+    INT_VR_FIXED_RR_ERROR = 8,
+    INT_VR_OPTIMIZE_RR_PDF = 9,  // ... with the rr_pdf_region roulette
+    INT_VR_FIXED_RR_PDF = 10
 };
 
 /* A region: box [lo, hi], its cached volume and 3^D node values, dim 0 fastest
@@ -733,6 +735,20 @@ inline T pdf_subrange(const Region<T, D>& r, const T* pn, const T* an, const T* 
     const T den = r.vol * pdf_integral(an[0], bn[0], std::array<T, 3>{u[0], u[1], u[2]});
     return num / den;
 }
+/* Region::pdf_integral_subrange (region.h:287-303): the region volume times the pdf's
+   integral over the normalised box, folds last dim first */
+template <typename T, int D>
+inline T pdf_integral_box(const Region<T, D>& r, const T* slo, const T* shi) {
+    T an[D], bn[D];
+    for (int d = 0; d < D; ++d) {
+        an[d] = (r.lo[d] >= r.hi[d]) ? r.lo[d] : (slo[d] - r.lo[d]) / (r.hi[d] - r.lo[d]);
+        bn[d] = (r.lo[d] >= r.hi[d]) ? r.lo[d] : (shi[d] - r.lo[d]) / (r.hi[d] - r.lo[d]);
+    }
+    std::vector<T> u(r.v, r.v + Region<T, D>::K);
+    for (int d = D - 1; d > 0; --d)
+        u = fold_last(u, [&](const std::array<T, 3>& v) { return pdf_integral(an[d], bn[d], v); });
+    return r.vol * pdf_integral(an[0], bn[0], std::array<T, 3>{u[0], u[1], u[2]});
+}
 /* region_sampling_importance::sample over the box [slo, shi] of region r */
 template <typename T, int D, typename RNG>
 inline T sample(const Region<T, D>& r, const T* slo, const T* shi, RNG& rng, T* x) {
@@ -853,6 +869,8 @@ void integrate_regions_vr(const std::vector<Region<T, D>>& regs, const F& f, con
                     // to <= 0), std::discrete_distribution, factor 1/p
                     // rr_error_region (:68-104): weights |Region::error()| times the share of
                     // the region's volume inside the bin
+                    // rr_pdf_region (:106-149): the importance pdf's integral over the bin's box,
+                    // floored at factor_prob = 0.01f (a float) of the mean
                     std::discrete_distribution<std::size_t> rrw;
                     if constexpr (RR != 0) {
                         std::vector<double> w(n);
@@ -860,15 +878,17 @@ void integrate_regions_vr(const std::vector<Region<T, D>>& regs, const F& f, con
                             const Region<T, D>& rk = regs[lst[off[b] + k]];
                             if constexpr (RR == 1)
                                 w[k] = std::abs(integral_subrange(rk, &ilo[k * D], &ihi[k * D]));
-                            else
+                            else if constexpr (RR == 2)
                                 w[k] = std::abs(region_error(rk)) * box_volume<T, D>(&ilo[k * D], &ihi[k * D]) / rk.vol;
+                            else  // rr_pdf_region (:106-149): the importance pdf's integral over the box
+                                w[k] = imp::pdf_integral_box<T, D>(rk, &ilo[k * D], &ihi[k * D]);
                         }
                         double sum = 0.0;
                         for (double x : w) sum += x;
                         if (sum <= 0.0)
                             for (double& x : w) x = 1.0;
                         else
-                            for (double& x : w) x = std::max(x, 0.01 * sum / double(w.size()));
+                            for (double& x : w) x = std::max(x, (RR == 3 ? double(0.01f) : 0.01) * sum / double(w.size()));
                         rrw = std::discrete_distribution<std::size_t>(w.begin(), w.end());
                     }
                     for (uint64_t s = 0; s < spp; ++s) {

@@ -171,6 +171,12 @@ int run_regions(int integrator, double alpha, const F& fn, const std::vector<vmo
     else if (integrator == vmo::INT_VR_FIXED_RR_ERROR)
         vmo::integrate_regions_vr<T, D, F, vmo::AccFixed<T>, false, 2>(regs, fn, tlo, thi, db, res, spp, seed,
                                                                          alpha, nthreads, out.data(), st);
+    else if (integrator == vmo::INT_VR_OPTIMIZE_RR_PDF)
+        vmo::integrate_regions_vr<T, D, F, vmo::AccOptimize<T>, false, 3>(regs, fn, tlo, thi, db, res, spp, seed,
+                                                                            alpha, nthreads, out.data(), st);
+    else if (integrator == vmo::INT_VR_FIXED_RR_PDF)
+        vmo::integrate_regions_vr<T, D, F, vmo::AccFixed<T>, false, 3>(regs, fn, tlo, thi, db, res, spp, seed,
+                                                                         alpha, nthreads, out.data(), st);
     else {
         g_err = "unknown integrator";
         return -1;

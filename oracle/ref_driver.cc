@@ -175,6 +175,18 @@ int integrate_db(int integrator, int heuristic, double alpha, const vmx_integran
                              viltrum::rr_error_region(), viltrum::cv_fixed_weight(alpha),
                              viltrum::region_sampling_uniform(), std::mt19937(std::size_t(seed)), spp));
                 viltrum::integrate(in, bins, r, fn, range);
+            } else if (integrator == 9) {
+                auto in = viltrum::integrator_region_based(
+                    gen, viltrum::regions_integrator_parallel_variance_reduction(
+                             viltrum::rr_pdf_region(), viltrum::cv_optimize_weight(),
+                             viltrum::region_sampling_uniform(), std::mt19937(std::size_t(seed)), spp));
+                viltrum::integrate(in, bins, r, fn, range);
+            } else if (integrator == 10) {
+                auto in = viltrum::integrator_region_based(
+                    gen, viltrum::regions_integrator_parallel_variance_reduction(
+                             viltrum::rr_pdf_region(), viltrum::cv_fixed_weight(alpha),
+                             viltrum::region_sampling_uniform(), std::mt19937(std::size_t(seed)), spp));
+                viltrum::integrate(in, bins, r, fn, range);
             } else if (integrator == 3 || integrator == 4) {
                 // region_sampling_importance compiles for float regions only: the
                 // reference's pop() exists for std::array<float, N> (src/array.h:52)

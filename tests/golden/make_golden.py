@@ -67,6 +67,11 @@ BIN_CASES = [  # (integrand, scalar, form, heuristic, res, iterations, spp, seed
     ("config1", 4, 7, 0, (16, 16), 255, 8, 5, 1.0),
     ("poly3", 8, 8, 0, (4, 4), 200, 4, 2, 0.7),
     ("lobes5", 8, 7, 0, (3, 3), 60, 4, 2, 1.0),
+    # rr_pdf_region (forms 9 and 10)
+    ("config1", 4, 9, 0, (16, 16), 255, 8, 5, 1.0),
+    ("poly3", 8, 10, 0, (4, 4), 200, 4, 2, 0.7),
+    ("blobs3d", 4, 9, 0, (8, 8), 300, 4, 5, 1.0),
+    ("lobes5", 8, 9, 0, (3, 3), 60, 4, 2, 1.0),
 ]
 
 FUBINI_CASES = [  # (integrand, region dims, res, iterations, mc samples, spp, seed): integrator_crespo2021_infinite

@@ -15,6 +15,7 @@ INT_VR_OPTIMIZE, INT_VR_FIXED, INT_CV_ALPHA1 = 0, 1, 2
 INT_VR_OPTIMIZE_IMPORTANCE, INT_VR_FIXED_IMPORTANCE = 3, 4  # with region_sampling_importance
 INT_VR_OPTIMIZE_RR_INTEGRAL, INT_VR_FIXED_RR_INTEGRAL = 5, 6  # with the rr_integral_region roulette
 INT_VR_OPTIMIZE_RR_ERROR, INT_VR_FIXED_RR_ERROR = 7, 8  # with the rr_error_region roulette
+INT_VR_OPTIMIZE_RR_PDF, INT_VR_FIXED_RR_PDF = 9, 10  # with the rr_pdf_region roulette
 HEUR_SIZE_RELATIVE, HEUR_DEFAULT_ABSOLUTE = 0, 1
 
 _dp = C.POINTER(C.c_double)

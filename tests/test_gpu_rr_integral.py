@@ -1,7 +1,8 @@
-"""rr_integral_region and rr_error_region (src/control-variates/region-russian-roulette.h:30-104):
-residual samples pick a bin's region with probability proportional to |integral of its
-polynomial over the bin's box|, or to |Region::error()| times the region's share inside
-the bin (floored at 1% of the mean), weight 1/p.  The GPU's
+"""rr_integral_region, rr_error_region and rr_pdf_region
+(src/control-variates/region-russian-roulette.h:30-149): residual samples pick a bin's
+region with probability proportional to |integral of its polynomial over the bin's box|,
+to |Region::error()| times the region's share inside the bin, or to the integral of the
+region's importance pdf over the bin's box (floored at 1% of the mean), weight 1/p.  The GPU's
 Philox stream and the oracle's mt19937 + std::discrete_distribution (the oracle is
 pinned bit for bit to the reference build) run the same estimator: per-bin means over
 independent seeds agree within their standard errors, with the same spread."""
@@ -25,7 +26,8 @@ pytestmark = pytest.mark.gpu
     (poly_cases()[3], (4, 4), 400, 8),
 ], ids=lambda v: getattr(v, "name", str(v)))
 @pytest.mark.parametrize("form", [V.FORM_VR_FIXED_RR_INTEGRAL, V.FORM_VR_OPTIMIZE_RR_INTEGRAL, V.FORM_VR_FIXED_RR_ERROR,
-                                  V.FORM_VR_OPTIMIZE_RR_ERROR], ids=["int_fixed1", "int_optimize", "err_fixed1", "err_optimize"])
+                                  V.FORM_VR_OPTIMIZE_RR_ERROR, V.FORM_VR_FIXED_RR_PDF, V.FORM_VR_OPTIMIZE_RR_PDF],
+                         ids=["int_fixed1", "int_optimize", "err_fixed1", "err_optimize", "pdf_fixed1", "pdf_optimize"])
 @pytest.mark.parametrize("scalar", [4, 8])
 def test_rr_integral_matches_oracle(ctx, f, res, it, spp, form, scalar):
     dt = np.float32 if scalar == 4 else np.float64
@@ -68,3 +70,35 @@ def test_rr_integral_and_importance_do_not_share_a_batch(ctx):
         ctx.integrate_batch(ps, [f, f], [b.address for b in bufs])
     for b in bufs:
         b.free()
+
+
+def test_rr_pdf_is_unbiased(ctx):
+    f = I.config1()
+    res = (16, 16)
+    truth = O.blobs2d_exact_bins(f, *res)
+    runs = []
+    for s in range(SEEDS):
+        b = np.zeros(res, dtype=np.float32)
+        V.integrate(V.integrator_variance_reduction_rr_pdf(255, 4, 6000 + s, alpha=1.0), b, f, V.range_primary(2),
+                    ctx=ctx)
+        runs.append(b.astype(np.float64))
+    runs = np.array(runs)
+    se = runs.std(0, ddof=1) / np.sqrt(SEEDS)
+    ok = se > 0
+    z = (runs.mean(0) - truth)[ok] / se[ok]
+    assert abs(z.mean()) < 5 / np.sqrt(z.size) + 0.05 and np.max(np.abs(z)) < 5.5
+
+
+def test_rr_pdf_in_five_dimensions_matches_oracle(ctx):
+    """The depth-first pdf fold at D = 5 (243 node values per region)."""
+    f = I.config5(3)
+    res = (6, 6)
+    g, o = [], []
+    for s in range(SEEDS):
+        b = np.zeros(res, dtype=np.float64)
+        V.integrate(V.Integrator(60, 8, 7000 + s, form=V.FORM_VR_FIXED_RR_PDF, alpha=1.0), b, f,
+                    V.range_primary(f.dim), ctx=ctx)
+        g.append(b)
+        ob, _ = O.integrate(f, res, 60, 8, 7000 + s, scalar=8, integrator=V.FORM_VR_FIXED_RR_PDF, alpha=1.0)
+        o.append(ob)
+    _check(np.array(g), np.array(o), rel_floor=1e-6, spread=(0.6, 1.6))

@@ -34,6 +34,7 @@ FORM_VR_OPTIMIZE, FORM_VR_FIXED, FORM_CV_ALPHA1 = 0, 1, 2
 FORM_VR_OPTIMIZE_IMPORTANCE, FORM_VR_FIXED_IMPORTANCE = 3, 4  # with region_sampling_importance
 FORM_VR_OPTIMIZE_RR_INTEGRAL, FORM_VR_FIXED_RR_INTEGRAL = 5, 6  # with the rr_integral_region roulette
 FORM_VR_OPTIMIZE_RR_ERROR, FORM_VR_FIXED_RR_ERROR = 7, 8  # with the rr_error_region roulette
+FORM_VR_OPTIMIZE_RR_PDF, FORM_VR_FIXED_RR_PDF = 9, 10  # with the rr_pdf_region roulette
 ORDER_CANONICAL, ORDER_REFERENCE = 0, 1
 REFINER_HEURISTIC, REFINER_LEARNED = 0, 1
 
@@ -460,6 +461,14 @@ def integrator_variance_reduction_rr_error(iterations, spp, seed=0, alpha=None, 
     if alpha is None:
         return Integrator(iterations, spp, seed, heuristic, FORM_VR_OPTIMIZE_RR_ERROR, **kw)
     return Integrator(iterations, spp, seed, heuristic, FORM_VR_FIXED_RR_ERROR, alpha=alpha, **kw)
+
+
+def integrator_variance_reduction_rr_pdf(iterations, spp, seed=0, alpha=None, heuristic=HEUR_SIZE_RELATIVE, **kw):
+    """... with rr_pdf_region() (src/control-variates/region-russian-roulette.h:106-149):
+    probability ~ the integral of the region's importance pdf over the bin's box."""
+    if alpha is None:
+        return Integrator(iterations, spp, seed, heuristic, FORM_VR_OPTIMIZE_RR_PDF, **kw)
+    return Integrator(iterations, spp, seed, heuristic, FORM_VR_FIXED_RR_PDF, alpha=alpha, **kw)
 
 
 def integrator_variance_reduction_fixed(alpha, iterations, spp, seed=0, heuristic=HEUR_SIZE_RELATIVE, **kw):

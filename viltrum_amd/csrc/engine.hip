@@ -255,7 +255,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     // the rr_integral_region roulette weighs each region of a bin: one footprint per region
     bool any_rr = false, any_imp = false, any_rr_err = false;
     for (const JobIn& x : in) {
-        any_rr |= x.p->form >= VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL && x.p->form <= VMX_FORM_VR_FIXED_RR_ERROR;
+        any_rr |= x.p->form >= VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL && x.p->form <= VMX_FORM_VR_FIXED_RR_PDF;
         any_rr_err |= x.p->form == VMX_FORM_VR_OPTIMIZE_RR_ERROR || x.p->form == VMX_FORM_VR_FIXED_RR_ERROR;
         any_imp |= x.p->form == VMX_FORM_VR_OPTIMIZE_IMPORTANCE || x.p->form == VMX_FORM_VR_FIXED_IMPORTANCE;
     }
@@ -716,10 +716,13 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     mr.take<T>(n_reg * D);
     mr.take<T>(n_reg);
     mr.take<T>(n_reg * K);
-    bool any_importance = false;
-    for (const auto& jb : jobs)
+    bool any_importance = false, keep_nodes = false;  // node values per region: importance sampling, rr_pdf_region
+    for (const auto& jb : jobs) {
         any_importance |= jb.form == VMX_FORM_VR_OPTIMIZE_IMPORTANCE || jb.form == VMX_FORM_VR_FIXED_IMPORTANCE;
-    if (any_importance) mr.take<T>(n_reg * K);
+        keep_nodes |= jb.form == VMX_FORM_VR_OPTIMIZE_RR_PDF || jb.form == VMX_FORM_VR_FIXED_RR_PDF;
+    }
+    keep_nodes |= any_importance;
+    if (keep_nodes) mr.take<T>(n_reg * K);
     if (any_rr_err) mr.take<T>(n_reg);
     mr.take<double>(n_reg * mb_max);
     mr.take<uint32_t>(n_reg);
@@ -742,7 +745,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     ra.hi = cr.take<T>(n_reg * D);
     ra.vol = cr.take<T>(n_reg);
     ra.coef = cr.take<T>(n_reg * K);
-    ra.v = any_importance ? cr.take<T>(n_reg * K) : nullptr;
+    ra.v = keep_nodes ? cr.take<T>(n_reg * K) : nullptr;
     ra.err = any_rr_err ? cr.take<T>(n_reg) : nullptr;
     ra.marg = cr.take<double>(n_reg * mb_max);
     ra.job = cr.take<uint32_t>(n_reg);
@@ -1034,10 +1037,10 @@ void validate(const vmx_problem* p, const vmx_integrand* f) {
     if (p->regions < 1 || p->regions > (1ull << 26)) throw VmxError(VMX_ERR_INVALID, "regions must be 1..2^26");
     if (p->heuristic != VMX_HEUR_SIZE_RELATIVE && p->heuristic != VMX_HEUR_DEFAULT_ABSOLUTE)
         throw VmxError(VMX_ERR_INVALID, "unknown heuristic");
-    if (p->form < VMX_FORM_VR_OPTIMIZE || p->form > VMX_FORM_VR_FIXED_RR_ERROR)
+    if (p->form < VMX_FORM_VR_OPTIMIZE || p->form > VMX_FORM_VR_FIXED_RR_PDF)
         throw VmxError(VMX_ERR_INVALID, "unknown form");
     if (p->form >= VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL && f->family == VMX_FAMILY_FUBINI)
-        throw VmxError(VMX_ERR_UNSUPPORTED, "rr_integral_region: not for the Fubini form");
+        throw VmxError(VMX_ERR_UNSUPPORTED, "region roulettes: not for the Fubini form");
     if ((p->form == VMX_FORM_VR_OPTIMIZE_IMPORTANCE || p->form == VMX_FORM_VR_FIXED_IMPORTANCE) &&
         (p->scalar != VMX_F32 || f->dim > vmx::imp::MAX_DIM || f->family == VMX_FAMILY_FUBINI))
         throw VmxError(VMX_ERR_UNSUPPORTED, "importance sampling: FP32 integrands of dimension 1..3");

@@ -114,6 +114,25 @@ VMX_HD inline int inv_cdf(T x, const T* points, T* sol) {
     return n;
 }
 
+/* The pdf's integral over the normalised box [an, bn] of the 3^D node values v (dim 0
+   fastest), without the region volume (Region::pdf_sub, src/newton-cotes/region.h:
+   271-283): folds the last dim first, here depth first over the slices, so any D keeps
+   3 values per level live; each fold sees the same three values as in the breadth-first
+   order.  Call with L = 0 and base = 0. */
+template <typename T, int D, int L>
+VMX_HD inline T pdf_fold(const T* v, int base, const T* an, const T* bn) {
+    constexpr int stride = pow3(L);
+    T w[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        if constexpr (L == D - 1)
+            w[j] = v[base + j * stride];
+        else
+            w[j] = pdf_fold<T, D, L + 1>(v, base + j * stride, an, bn);
+    }
+    return pdf_integral(an[L], bn[L], w);
+}
+
 /* one coordinate in [t0, t1] from the line's pdf, s uniform in [0, 1) */
 template <typename T>
 VMX_HD inline T sample_line(T s, T t0, T t1, const T* p) {

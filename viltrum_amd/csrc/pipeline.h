@@ -1355,7 +1355,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE !
                                                const F* __restrict__ fs, JobRt* rt, ListPrefix lp, int xcd,
                                                double* __restrict__ rrw) {
     // MODE 0: uniform region roulette and in-box sampling; 1: importance sampling in the
-    // box; 2: rr_integral_region roulette (rrw: per-list cumulative weights)
+    // box; 2: the region roulettes (rr_integral_region, rr_error_region, rr_pdf_region;
+    // rrw: per-list cumulative weights)
     constexpr bool IMP = MODE == 1;
     constexpr bool RRI = MODE == 2;
     constexpr int K = pow3(D);
@@ -1448,6 +1449,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE !
 #pragma unroll
                     for (int d = 0; d < DB; ++d) share *= bb[d] - aa[d];
                     rrw[o + k] = vabs(double(r.err[fa.first[fi]])) * share;
+                } else if (jb.form == VMX_FORM_VR_OPTIMIZE_RR_PDF || jb.form == VMX_FORM_VR_FIXED_RR_PDF) {
+                    // the pdf's integral over the bin's box; dims past the bins span the region
+                    const uint32_t rg = fa.first[fi];
+                    T an[D], bn[D];
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        an[d] = d < DB ? T(aa[d]) : T(0);
+                        bn[d] = d < DB ? T(bb[d]) : T(1);
+                    }
+                    rrw[o + k] = empty ? 0.0 : double(r.vol[rg] * imp::pdf_fold<T, D, 0>(r.v + size_t(rg) * K, 0, an, bn));
                 } else {
                     rrw[o + k] = vabs(integ);
                 }
@@ -1484,14 +1495,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE !
         };
         constexpr bool FP32 = sizeof(T) == 4;
         const T wn = T(factor * double(n));  // FP32 sample weight factor B * n
-        const bool rr_form = RRI && jb.form >= VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL && jb.form <= VMX_FORM_VR_FIXED_RR_ERROR;
+        const bool rr_form = RRI && jb.form >= VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL && jb.form <= VMX_FORM_VR_FIXED_RR_PDF;
         double rr_total = 0.0;
         if (rr_form && spp > 0 && n > 0) {  // floor at 1% of the mean weight, then cumulate
+            // rr_pdf_region keeps its factor 0.01 in a float
+            const double fl = jb.form >= VMX_FORM_VR_OPTIMIZE_RR_PDF ? double(0.01f) : 0.01;
             double sum = 0.0;
             for (uint32_t k = 0; k < n; ++k) sum += rrw[o + k];
             double cum = 0.0;
             for (uint32_t k = 0; k < n; ++k) {
-                const double w = sum <= 0.0 ? 1.0 : vmax(rrw[o + k], 0.01 * sum / double(n));
+                const double w = sum <= 0.0 ? 1.0 : vmax(rrw[o + k], fl * sum / double(n));
                 cum += w;
                 rrw[o + k] = cum;
             }
@@ -1591,7 +1604,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE !
         double result;
         const double nsd = double(spp);
         const bool fixed_w = jb.form == VMX_FORM_VR_FIXED || jb.form == VMX_FORM_VR_FIXED_IMPORTANCE ||
-                             jb.form == VMX_FORM_VR_FIXED_RR_INTEGRAL || jb.form == VMX_FORM_VR_FIXED_RR_ERROR;
+                             jb.form == VMX_FORM_VR_FIXED_RR_INTEGRAL || jb.form == VMX_FORM_VR_FIXED_RR_ERROR ||
+                             jb.form == VMX_FORM_VR_FIXED_RR_PDF;
         if (!sampling || (!fixed_w && spp < 2)) {
             result = approx;
         } else {
