@@ -93,7 +93,19 @@ enum vmx_form {
        proportional to the integral of the importance pdf over the bin's box
        (Region::pdf_integral_subrange), floored at 1% of the mean */
     VMX_FORM_VR_OPTIMIZE_RR_PDF = 9,
-    VMX_FORM_VR_FIXED_RR_PDF = 10
+    VMX_FORM_VR_FIXED_RR_PDF = 10,
+    /* A roulette and region_sampling_importance together
+       (regions_integrator_parallel_variance_reduction(rr, cv, region_sampling_importance(),
+       ...), regions-integrator-parallel-variance-reduction.h:94-100, as in
+       main/compilation-tests/multiple-parameters-2d.cc:101-103): the region is drawn by
+       the roulette, the point in the bin's box from the region's pdf; weight 1/p times
+       1/pdf.  FP32 integrands of dimension 1..3, as for importance sampling alone. */
+    VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL_IMPORTANCE = 11,
+    VMX_FORM_VR_FIXED_RR_INTEGRAL_IMPORTANCE = 12,
+    VMX_FORM_VR_OPTIMIZE_RR_ERROR_IMPORTANCE = 13,
+    VMX_FORM_VR_FIXED_RR_ERROR_IMPORTANCE = 14,
+    VMX_FORM_VR_OPTIMIZE_RR_PDF_IMPORTANCE = 15,
+    VMX_FORM_VR_FIXED_RR_PDF_IMPORTANCE = 16
 };
 
 /* Order of the final region list (and so of every per-bin region list). */

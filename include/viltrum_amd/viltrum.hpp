@@ -341,6 +341,20 @@ inline IntegratorGpu integrator_variance_reduction_rr_pdf_fixed(double alpha, st
                                                                 int heuristic = VMX_HEUR_SIZE_RELATIVE) {
     return IntegratorGpu(iterations, spp, seed, heuristic, VMX_FORM_VR_FIXED_RR_PDF, alpha);
 }
+/* ... with rr_pdf_region() and region_sampling_importance() (the region by its pdf mass,
+   the point from its pdf; main/compilation-tests/multiple-parameters-2d.cc:101-103).  The
+   other roulettes with importance sampling: IntegratorGpu with the forms
+   VMX_FORM_VR_{OPTIMIZE,FIXED}_RR_{INTEGRAL,ERROR}_IMPORTANCE. */
+inline IntegratorGpu integrator_variance_reduction_rr_pdf_importance(std::size_t iterations, std::size_t spp,
+                                                                     std::size_t seed = 0,
+                                                                     int heuristic = VMX_HEUR_SIZE_RELATIVE) {
+    return IntegratorGpu(iterations, spp, seed, heuristic, VMX_FORM_VR_OPTIMIZE_RR_PDF_IMPORTANCE, 1.0);
+}
+inline IntegratorGpu integrator_variance_reduction_rr_pdf_importance_fixed(double alpha, std::size_t iterations,
+                                                                           std::size_t spp, std::size_t seed = 0,
+                                                                           int heuristic = VMX_HEUR_SIZE_RELATIVE) {
+    return IntegratorGpu(iterations, spp, seed, heuristic, VMX_FORM_VR_FIXED_RR_PDF_IMPORTANCE, alpha);
+}
 /* ... with cv_fixed_weight(alpha) */
 inline IntegratorGpu integrator_variance_reduction_importance_fixed(double alpha, std::size_t iterations,
                                                                     std::size_t spp, std::size_t seed = 0,

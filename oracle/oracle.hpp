@@ -74,7 +74,14 @@ enum Integrator {
     INT_VR_OPTIMIZE_RR_ERROR = 7,  // ... with the rr_error_region roulette
     INT_VR_FIXED_RR_ERROR = 8,
     INT_VR_OPTIMIZE_RR_PDF = 9,  // ... with the rr_pdf_region roulette
-    INT_VR_FIXED_RR_PDF = 10
+    INT_VR_FIXED_RR_PDF = 10,
+    // a roulette and region_sampling_importance together (optimised / fixed weight):
+    INT_VR_OPTIMIZE_RR_INTEGRAL_IMPORTANCE = 11,
+    INT_VR_FIXED_RR_INTEGRAL_IMPORTANCE = 12,
+    INT_VR_OPTIMIZE_RR_ERROR_IMPORTANCE = 13,
+    INT_VR_FIXED_RR_ERROR_IMPORTANCE = 14,
+    INT_VR_OPTIMIZE_RR_PDF_IMPORTANCE = 15,
+    INT_VR_FIXED_RR_PDF_IMPORTANCE = 16
 };
 
 /* A region: box [lo, hi], its cached volume and 3^D node values, dim 0 fastest

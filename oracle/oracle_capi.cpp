@@ -177,7 +177,25 @@ int run_regions(int integrator, double alpha, const F& fn, const std::vector<vmo
     else if (integrator == vmo::INT_VR_FIXED_RR_PDF)
         vmo::integrate_regions_vr<T, D, F, vmo::AccFixed<T>, false, 3>(regs, fn, tlo, thi, db, res, spp, seed,
                                                                          alpha, nthreads, out.data(), st);
-    else {
+    else if (integrator >= vmo::INT_VR_OPTIMIZE_RR_INTEGRAL_IMPORTANCE &&
+             integrator <= vmo::INT_VR_FIXED_RR_PDF_IMPORTANCE) {
+        // codes 11..16: roulette (integral, error, pdf) x weight (optimised, fixed), importance sampled
+        const int k = integrator - vmo::INT_VR_OPTIMIZE_RR_INTEGRAL_IMPORTANCE;
+        auto run = [&](auto acc, auto rr) {
+            vmo::integrate_regions_vr<T, D, F, decltype(acc), true, decltype(rr)::value>(
+                regs, fn, tlo, thi, db, res, spp, seed, alpha, nthreads, out.data(), st);
+        };
+        using R1 = std::integral_constant<int, 1>;
+        using R2 = std::integral_constant<int, 2>;
+        using R3 = std::integral_constant<int, 3>;
+        const bool fixed = k % 2 == 1;
+        if (k / 2 == 0)
+            fixed ? run(vmo::AccFixed<T>{}, R1{}) : run(vmo::AccOptimize<T>{}, R1{});
+        else if (k / 2 == 1)
+            fixed ? run(vmo::AccFixed<T>{}, R2{}) : run(vmo::AccOptimize<T>{}, R2{});
+        else
+            fixed ? run(vmo::AccFixed<T>{}, R3{}) : run(vmo::AccOptimize<T>{}, R3{});
+    } else {
         g_err = "unknown integrator";
         return -1;
     }

@@ -187,6 +187,27 @@ int integrate_db(int integrator, int heuristic, double alpha, const vmx_integran
                              viltrum::rr_pdf_region(), viltrum::cv_fixed_weight(alpha),
                              viltrum::region_sampling_uniform(), std::mt19937(std::size_t(seed)), spp));
                 viltrum::integrate(in, bins, r, fn, range);
+            } else if (integrator >= 11 && integrator <= 16) {
+                // a roulette with region_sampling_importance (float regions only, as below)
+                if constexpr (std::is_same<T, float>::value) {
+                    auto run = [&](auto rr, auto cv) {
+                        auto in = viltrum::integrator_region_based(
+                            gen, viltrum::regions_integrator_parallel_variance_reduction(
+                                     std::move(rr), std::move(cv), viltrum::region_sampling_importance(),
+                                     std::mt19937(std::size_t(seed)), spp));
+                        viltrum::integrate(in, bins, r, fn, range);
+                    };
+                    switch (integrator) {
+                        case 11: run(viltrum::rr_integral_region(), viltrum::cv_optimize_weight()); break;
+                        case 12: run(viltrum::rr_integral_region(), viltrum::cv_fixed_weight(alpha)); break;
+                        case 13: run(viltrum::rr_error_region(), viltrum::cv_optimize_weight()); break;
+                        case 14: run(viltrum::rr_error_region(), viltrum::cv_fixed_weight(alpha)); break;
+                        case 15: run(viltrum::rr_pdf_region(), viltrum::cv_optimize_weight()); break;
+                        default: run(viltrum::rr_pdf_region(), viltrum::cv_fixed_weight(alpha)); break;
+                    }
+                } else {
+                    throw std::runtime_error("importance sampling: float regions only");
+                }
             } else if (integrator == 3 || integrator == 4) {
                 // region_sampling_importance compiles for float regions only: the
                 // reference's pop() exists for std::array<float, N> (src/array.h:52)

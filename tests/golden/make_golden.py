@@ -72,6 +72,12 @@ BIN_CASES = [  # (integrand, scalar, form, heuristic, res, iterations, spp, seed
     ("poly3", 8, 10, 0, (4, 4), 200, 4, 2, 0.7),
     ("blobs3d", 4, 9, 0, (8, 8), 300, 4, 5, 1.0),
     ("lobes5", 8, 9, 0, (3, 3), 60, 4, 2, 1.0),
+    # a roulette with region_sampling_importance (forms 11..16; float only)
+    ("config1", 4, 11, 0, (16, 16), 255, 8, 5, 1.0),
+    ("poly3", 4, 14, 0, (4, 4), 200, 4, 2, 0.7),
+    ("config1", 4, 15, 0, (16, 16), 255, 8, 5, 1.0),
+    ("blobs3d", 4, 16, 0, (8, 8), 300, 4, 5, 0.5),
+    ("config1", 4, 15, 0, (40,), 100, 5, 2, 1.0),
 ]
 
 FUBINI_CASES = [  # (integrand, region dims, res, iterations, mc samples, spp, seed): integrator_crespo2021_infinite

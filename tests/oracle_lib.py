@@ -16,6 +16,10 @@ INT_VR_OPTIMIZE_IMPORTANCE, INT_VR_FIXED_IMPORTANCE = 3, 4  # with region_sampli
 INT_VR_OPTIMIZE_RR_INTEGRAL, INT_VR_FIXED_RR_INTEGRAL = 5, 6  # with the rr_integral_region roulette
 INT_VR_OPTIMIZE_RR_ERROR, INT_VR_FIXED_RR_ERROR = 7, 8  # with the rr_error_region roulette
 INT_VR_OPTIMIZE_RR_PDF, INT_VR_FIXED_RR_PDF = 9, 10  # with the rr_pdf_region roulette
+# a roulette together with region_sampling_importance (optimised / fixed weight)
+INT_VR_OPTIMIZE_RR_INTEGRAL_IMPORTANCE, INT_VR_FIXED_RR_INTEGRAL_IMPORTANCE = 11, 12
+INT_VR_OPTIMIZE_RR_ERROR_IMPORTANCE, INT_VR_FIXED_RR_ERROR_IMPORTANCE = 13, 14
+INT_VR_OPTIMIZE_RR_PDF_IMPORTANCE, INT_VR_FIXED_RR_PDF_IMPORTANCE = 15, 16
 HEUR_SIZE_RELATIVE, HEUR_DEFAULT_ABSOLUTE = 0, 1
 
 _dp = C.POINTER(C.c_double)

@@ -2,7 +2,8 @@
 (src/control-variates/region-russian-roulette.h:30-149): residual samples pick a bin's
 region with probability proportional to |integral of its polynomial over the bin's box|,
 to |Region::error()| times the region's share inside the bin, or to the integral of the
-region's importance pdf over the bin's box (floored at 1% of the mean), weight 1/p.  The GPU's
+region's importance pdf over the bin's box (floored at 1% of the mean), weight 1/p; also
+each of them with region_sampling_importance drawing the point.  The GPU's
 Philox stream and the oracle's mt19937 + std::discrete_distribution (the oracle is
 pinned bit for bit to the reference build) run the same estimator: per-bin means over
 independent seeds agree within their standard errors, with the same spread."""
@@ -60,16 +61,26 @@ def test_rr_integral_is_unbiased(ctx):
     assert abs(z.mean()) < 5 / np.sqrt(z.size) + 0.05 and np.max(np.abs(z)) < 5.5
 
 
-def test_rr_integral_and_importance_do_not_share_a_batch(ctx):
+def test_roulettes_importance_and_uniform_share_a_batch(ctx):
+    """One batch holding a roulette, importance sampling, both, and neither: each job's
+    bins are what it gives alone (the batch keeps one footprint per region for all of
+    them, so control-variate sums may differ in the last bit)."""
     from viltrum_amd import hip
     f = I.config1()
-    ps = [V.Integrator(63, 4, 0, form=V.FORM_VR_OPTIMIZE_RR_INTEGRAL).problem(f, (8, 8), V.range_primary(2), V.F32),
-          V.Integrator(63, 4, 0, form=V.FORM_VR_OPTIMIZE_IMPORTANCE).problem(f, (8, 8), V.range_primary(2), V.F32)]
+    res = (8, 8)
+    forms = [V.FORM_VR_OPTIMIZE_RR_INTEGRAL, V.FORM_VR_OPTIMIZE_IMPORTANCE, V.FORM_VR_OPTIMIZE_RR_PDF_IMPORTANCE,
+             V.FORM_VR_OPTIMIZE]
+    integs = [V.Integrator(63, 4, 10 + k, form=fm) for k, fm in enumerate(forms)]
+    ps = [ig.problem(f, res, V.range_primary(2), V.F32) for ig in integs]
     bufs = [hip.DeviceBuffer(64 * 4) for _ in ps]
-    with pytest.raises(V.VmxError):
-        ctx.integrate_batch(ps, [f, f], [b.address for b in bufs])
+    ctx.integrate_batch(ps, [f] * len(ps), [b.address for b in bufs])
+    got = [b.to_host(np.zeros(res, dtype=np.float32)) for b in bufs]
     for b in bufs:
         b.free()
+    for ig, g in zip(integs, got):
+        alone = np.zeros(res, dtype=np.float32)
+        V.integrate(ig, alone, f, V.range_primary(2), ctx=ctx)
+        np.testing.assert_allclose(g, alone, rtol=1e-5, atol=1e-7)
 
 
 def test_rr_pdf_is_unbiased(ctx):
@@ -102,3 +113,56 @@ def test_rr_pdf_in_five_dimensions_matches_oracle(ctx):
         ob, _ = O.integrate(f, res, 60, 8, 7000 + s, scalar=8, integrator=V.FORM_VR_FIXED_RR_PDF, alpha=1.0)
         o.append(ob)
     _check(np.array(g), np.array(o), rel_floor=1e-6, spread=(0.6, 1.6))
+
+
+@pytest.mark.parametrize("f,res,it,spp", [
+    (I.config1(), (8, 8), 127, 8),
+    (poly_cases()[1], (8, 8), 100, 8),
+    (poly_cases()[2], (5, 5), 150, 8),
+    (I.config1(), (20,), 60, 8),
+], ids=lambda v: getattr(v, "name", str(v)))
+@pytest.mark.parametrize("form", [V.FORM_VR_FIXED_RR_INTEGRAL_IMPORTANCE, V.FORM_VR_OPTIMIZE_RR_INTEGRAL_IMPORTANCE,
+                                  V.FORM_VR_FIXED_RR_ERROR_IMPORTANCE, V.FORM_VR_OPTIMIZE_RR_ERROR_IMPORTANCE,
+                                  V.FORM_VR_FIXED_RR_PDF_IMPORTANCE, V.FORM_VR_OPTIMIZE_RR_PDF_IMPORTANCE],
+                         ids=["int_fixed1", "int_optimize", "err_fixed1", "err_optimize", "pdf_fixed1", "pdf_optimize"])
+def test_roulette_with_importance_matches_oracle(ctx, f, res, it, spp, form):
+    """A roulette picks the region, region_sampling_importance the point (FP32)."""
+    g, o = [], []
+    for s in range(SEEDS):
+        b = np.zeros(res, dtype=np.float32)
+        V.integrate(V.Integrator(it, spp, 8000 + s, form=form, alpha=1.0), b, f, V.range_primary(f.dim), ctx=ctx)
+        g.append(b.astype(np.float64))
+        ob, _ = O.integrate(f, res, it, spp, 8000 + s, scalar=4, integrator=form, alpha=1.0)
+        o.append(ob)
+    g, o = np.array(g), np.array(o)
+    assert np.all(np.isfinite(g)) and np.all(np.isfinite(o))
+    _check(g, o, rel_floor=1e-6, spread=(0.6, 1.6))
+
+
+def test_full_importance_sampling_is_unbiased(ctx):
+    """rr_pdf_region with region_sampling_importance, weight 1, against the closed form."""
+    f = I.config1()
+    res = (16, 16)
+    truth = O.blobs2d_exact_bins(f, *res)
+    runs = []
+    for s in range(SEEDS):
+        b = np.zeros(res, dtype=np.float32)
+        V.integrate(V.integrator_variance_reduction_rr_pdf(255, 4, 9000 + s, alpha=1.0, importance=True), b, f,
+                    V.range_primary(2), ctx=ctx)
+        runs.append(b.astype(np.float64))
+    runs = np.array(runs)
+    se = runs.std(0, ddof=1) / np.sqrt(SEEDS)
+    ok = se > 0
+    z = (runs.mean(0) - truth)[ok] / se[ok]
+    assert abs(z.mean()) < 5 / np.sqrt(z.size) + 0.05 and np.max(np.abs(z)) < 5.5
+
+
+def test_roulette_with_importance_rejects_fp64_and_five_dimensions(ctx):
+    f = I.config1()
+    with pytest.raises(V.VmxError):
+        V.integrate(V.integrator_variance_reduction_rr_pdf(63, 4, 0, importance=True), np.zeros((4, 4), np.float64), f,
+                    V.range_primary(2), ctx=ctx)
+    f5 = I.config5(1)
+    with pytest.raises(V.VmxError):
+        V.integrate(V.integrator_variance_reduction_rr_integral(40, 2, 0, importance=True), np.zeros((3, 3), np.float32),
+                    f5, V.range_primary(5), ctx=ctx)

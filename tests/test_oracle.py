@@ -87,8 +87,9 @@ def test_oracle_matches_live_reference(seed):
     res = (int(rng.integers(3, 20)), int(rng.integers(3, 20)))
     it, spp = int(rng.integers(0, 300)), int(rng.integers(0, 9))
     for scalar in (4, 8):
-        for form, heur in ((0, 0), (1, 0), (2, 1), (3, 0), (4, 0), (5, 0), (6, 0), (7, 0), (8, 0), (9, 0), (10, 0)):
-            if form in (3, 4) and scalar == 8:
+        for form in range(17):
+            heur = 1 if form == 2 else 0
+            if (form in (3, 4) or form >= 11) and scalar == 8:
                 continue  # region_sampling_importance: the reference compiles it for float only
             a, _ = O.integrate(f, res, it, spp, seed, scalar=scalar, integrator=form, heuristic=heur, alpha=0.3)
             b, _ = O.integrate(f, res, it, spp, seed, scalar=scalar, integrator=form, heuristic=heur, alpha=0.3,

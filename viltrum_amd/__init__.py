@@ -35,6 +35,10 @@ FORM_VR_OPTIMIZE_IMPORTANCE, FORM_VR_FIXED_IMPORTANCE = 3, 4  # with region_samp
 FORM_VR_OPTIMIZE_RR_INTEGRAL, FORM_VR_FIXED_RR_INTEGRAL = 5, 6  # with the rr_integral_region roulette
 FORM_VR_OPTIMIZE_RR_ERROR, FORM_VR_FIXED_RR_ERROR = 7, 8  # with the rr_error_region roulette
 FORM_VR_OPTIMIZE_RR_PDF, FORM_VR_FIXED_RR_PDF = 9, 10  # with the rr_pdf_region roulette
+# a roulette together with region_sampling_importance (optimised / fixed weight)
+FORM_VR_OPTIMIZE_RR_INTEGRAL_IMPORTANCE, FORM_VR_FIXED_RR_INTEGRAL_IMPORTANCE = 11, 12
+FORM_VR_OPTIMIZE_RR_ERROR_IMPORTANCE, FORM_VR_FIXED_RR_ERROR_IMPORTANCE = 13, 14
+FORM_VR_OPTIMIZE_RR_PDF_IMPORTANCE, FORM_VR_FIXED_RR_PDF_IMPORTANCE = 15, 16
 ORDER_CANONICAL, ORDER_REFERENCE = 0, 1
 REFINER_HEURISTIC, REFINER_LEARNED = 0, 1
 
@@ -445,27 +449,41 @@ def integrator_variance_reduction_importance(iterations, spp, seed=0, alpha=None
 
 
 def integrator_variance_reduction_rr_integral(iterations, spp, seed=0, alpha=None, heuristic=HEUR_SIZE_RELATIVE,
-                                              **kw):
+                                              importance=False, **kw):
     """regions_integrator_parallel_variance_reduction(rr_integral_region(), cv_optimize_weight()
     or cv_fixed_weight(alpha), region_sampling_uniform(), ...)
     (src/control-variates/region-russian-roulette.h:30-67): a residual sample picks the
     bin's region r with probability ~ |integral of P_r over the bin's box|, weight 1/p."""
+    if importance:  # with region_sampling_importance as the point sampler (float, dims 1..3)
+        if alpha is None:
+            return Integrator(iterations, spp, seed, heuristic, FORM_VR_OPTIMIZE_RR_INTEGRAL_IMPORTANCE, **kw)
+        return Integrator(iterations, spp, seed, heuristic, FORM_VR_FIXED_RR_INTEGRAL_IMPORTANCE, alpha=alpha, **kw)
     if alpha is None:
         return Integrator(iterations, spp, seed, heuristic, FORM_VR_OPTIMIZE_RR_INTEGRAL, **kw)
     return Integrator(iterations, spp, seed, heuristic, FORM_VR_FIXED_RR_INTEGRAL, alpha=alpha, **kw)
 
 
-def integrator_variance_reduction_rr_error(iterations, spp, seed=0, alpha=None, heuristic=HEUR_SIZE_RELATIVE, **kw):
+def integrator_variance_reduction_rr_error(iterations, spp, seed=0, alpha=None, heuristic=HEUR_SIZE_RELATIVE,
+                                           importance=False, **kw):
     """... with rr_error_region() (src/control-variates/region-russian-roulette.h:68-104):
     probability ~ |Region::error()| times the share of the region inside the bin."""
+    if importance:  # with region_sampling_importance as the point sampler (float, dims 1..3)
+        if alpha is None:
+            return Integrator(iterations, spp, seed, heuristic, FORM_VR_OPTIMIZE_RR_ERROR_IMPORTANCE, **kw)
+        return Integrator(iterations, spp, seed, heuristic, FORM_VR_FIXED_RR_ERROR_IMPORTANCE, alpha=alpha, **kw)
     if alpha is None:
         return Integrator(iterations, spp, seed, heuristic, FORM_VR_OPTIMIZE_RR_ERROR, **kw)
     return Integrator(iterations, spp, seed, heuristic, FORM_VR_FIXED_RR_ERROR, alpha=alpha, **kw)
 
 
-def integrator_variance_reduction_rr_pdf(iterations, spp, seed=0, alpha=None, heuristic=HEUR_SIZE_RELATIVE, **kw):
+def integrator_variance_reduction_rr_pdf(iterations, spp, seed=0, alpha=None, heuristic=HEUR_SIZE_RELATIVE,
+                                         importance=False, **kw):
     """... with rr_pdf_region() (src/control-variates/region-russian-roulette.h:106-149):
     probability ~ the integral of the region's importance pdf over the bin's box."""
+    if importance:  # with region_sampling_importance as the point sampler (float, dims 1..3)
+        if alpha is None:
+            return Integrator(iterations, spp, seed, heuristic, FORM_VR_OPTIMIZE_RR_PDF_IMPORTANCE, **kw)
+        return Integrator(iterations, spp, seed, heuristic, FORM_VR_FIXED_RR_PDF_IMPORTANCE, alpha=alpha, **kw)
     if alpha is None:
         return Integrator(iterations, spp, seed, heuristic, FORM_VR_OPTIMIZE_RR_PDF, **kw)
     return Integrator(iterations, spp, seed, heuristic, FORM_VR_FIXED_RR_PDF, alpha=alpha, **kw)

@@ -252,14 +252,12 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     for (const JobIn& x : in)
         if ((x.given != nullptr) != replay) throw VmxError(VMX_ERR_INVALID, "given regions for all or none");
     if (replay && pairs_req) throw VmxError(VMX_ERR_INVALID, "training pairs need generation");
-    // the rr_integral_region roulette weighs each region of a bin: one footprint per region
-    bool any_rr = false, any_imp = false, any_rr_err = false;
+    // the region roulettes weigh each region of a bin: one footprint per region
+    bool any_rr = false, any_rr_err = false;
     for (const JobIn& x : in) {
-        any_rr |= x.p->form >= VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL && x.p->form <= VMX_FORM_VR_FIXED_RR_PDF;
-        any_rr_err |= x.p->form == VMX_FORM_VR_OPTIMIZE_RR_ERROR || x.p->form == VMX_FORM_VR_FIXED_RR_ERROR;
-        any_imp |= x.p->form == VMX_FORM_VR_OPTIMIZE_IMPORTANCE || x.p->form == VMX_FORM_VR_FIXED_IMPORTANCE;
+        any_rr |= vmx::form_roulette(x.p->form) != 0;
+        any_rr_err |= vmx::form_roulette(x.p->form) == 2;
     }
-    if (any_rr && any_imp) throw VmxError(VMX_ERR_INVALID, "importance sampling and rr_integral_region: one per batch");
     const bool merge = order == VMX_ORDER_CANONICAL && !replay && !any_rr;
 
     std::vector<vmx::Job<T>> jobs(J);
@@ -718,8 +716,8 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     mr.take<T>(n_reg * K);
     bool any_importance = false, keep_nodes = false;  // node values per region: importance sampling, rr_pdf_region
     for (const auto& jb : jobs) {
-        any_importance |= jb.form == VMX_FORM_VR_OPTIMIZE_IMPORTANCE || jb.form == VMX_FORM_VR_FIXED_IMPORTANCE;
-        keep_nodes |= jb.form == VMX_FORM_VR_OPTIMIZE_RR_PDF || jb.form == VMX_FORM_VR_FIXED_RR_PDF;
+        any_importance |= vmx::form_importance(jb.form);
+        keep_nodes |= vmx::form_roulette(jb.form) == 3;
     }
     keep_nodes |= any_importance;
     if (keep_nodes) mr.take<T>(n_reg * K);
@@ -864,7 +862,10 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         bool done = false;
         if constexpr (std::is_same<T, float>::value && D <= vmx::imp::MAX_DIM) {
             if (any_importance) {
-                bins_kernel(std::integral_constant<int, 1>{});
+                if (any_rr)
+                    bins_kernel(std::integral_constant<int, 3>{});
+                else
+                    bins_kernel(std::integral_constant<int, 1>{});
                 done = true;
             }
         }
@@ -1037,11 +1038,11 @@ void validate(const vmx_problem* p, const vmx_integrand* f) {
     if (p->regions < 1 || p->regions > (1ull << 26)) throw VmxError(VMX_ERR_INVALID, "regions must be 1..2^26");
     if (p->heuristic != VMX_HEUR_SIZE_RELATIVE && p->heuristic != VMX_HEUR_DEFAULT_ABSOLUTE)
         throw VmxError(VMX_ERR_INVALID, "unknown heuristic");
-    if (p->form < VMX_FORM_VR_OPTIMIZE || p->form > VMX_FORM_VR_FIXED_RR_PDF)
+    if (p->form < VMX_FORM_VR_OPTIMIZE || p->form > VMX_FORM_VR_FIXED_RR_PDF_IMPORTANCE)
         throw VmxError(VMX_ERR_INVALID, "unknown form");
     if (p->form >= VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL && f->family == VMX_FAMILY_FUBINI)
         throw VmxError(VMX_ERR_UNSUPPORTED, "region roulettes: not for the Fubini form");
-    if ((p->form == VMX_FORM_VR_OPTIMIZE_IMPORTANCE || p->form == VMX_FORM_VR_FIXED_IMPORTANCE) &&
+    if (vmx::form_importance(p->form) &&
         (p->scalar != VMX_F32 || f->dim > vmx::imp::MAX_DIM || f->family == VMX_FAMILY_FUBINI))
         throw VmxError(VMX_ERR_UNSUPPORTED, "importance sampling: FP32 integrands of dimension 1..3");
     if (p->order != VMX_ORDER_CANONICAL && p->order != VMX_ORDER_REFERENCE)

@@ -1348,17 +1348,42 @@ __device__ inline float sample_pos<float>(float x, float lo, float hi) {
    Moments use the reference's shift by the bin's first sample; sums are
    Kahan-compensated in the evaluation type and finished in FP64.  Summation depends on
    the job alone, so a batched integral gives exactly what it gives alone. */
+/* the residual estimator's parts per form: region roulette (0 uniform, 1 rr_integral_region,
+   2 rr_error_region, 3 rr_pdf_region), point sampler, fixed weight */
+VMX_HD constexpr int form_roulette(int f) {
+    return (f == VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL || f == VMX_FORM_VR_FIXED_RR_INTEGRAL ||
+            f == VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL_IMPORTANCE || f == VMX_FORM_VR_FIXED_RR_INTEGRAL_IMPORTANCE)
+               ? 1
+           : (f == VMX_FORM_VR_OPTIMIZE_RR_ERROR || f == VMX_FORM_VR_FIXED_RR_ERROR ||
+              f == VMX_FORM_VR_OPTIMIZE_RR_ERROR_IMPORTANCE || f == VMX_FORM_VR_FIXED_RR_ERROR_IMPORTANCE)
+               ? 2
+           : (f == VMX_FORM_VR_OPTIMIZE_RR_PDF || f == VMX_FORM_VR_FIXED_RR_PDF ||
+              f == VMX_FORM_VR_OPTIMIZE_RR_PDF_IMPORTANCE || f == VMX_FORM_VR_FIXED_RR_PDF_IMPORTANCE)
+               ? 3
+               : 0;
+}
+VMX_HD constexpr bool form_importance(int f) {
+    return f == VMX_FORM_VR_OPTIMIZE_IMPORTANCE || f == VMX_FORM_VR_FIXED_IMPORTANCE ||
+           (f >= VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL_IMPORTANCE && f <= VMX_FORM_VR_FIXED_RR_PDF_IMPORTANCE);
+}
+VMX_HD constexpr bool form_fixed(int f) {
+    return f == VMX_FORM_VR_FIXED || f == VMX_FORM_VR_FIXED_IMPORTANCE || f == VMX_FORM_VR_FIXED_RR_INTEGRAL ||
+           f == VMX_FORM_VR_FIXED_RR_ERROR || f == VMX_FORM_VR_FIXED_RR_PDF ||
+           f == VMX_FORM_VR_FIXED_RR_INTEGRAL_IMPORTANCE || f == VMX_FORM_VR_FIXED_RR_ERROR_IMPORTANCE ||
+           f == VMX_FORM_VR_FIXED_RR_PDF_IMPORTANCE;
+}
+
 template <typename T, int D, int DB, typename F, typename M, int MODE = 0>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE != 1 && D <= 3 && sizeof(T) == 4 ? 4 : 1))) k_bins1(const Job<T>* __restrict__ jobs, uint32_t J, uint64_t total_bins,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((MODE & 1) == 0 && D <= 3 && sizeof(T) == 4 ? 4 : 1))) k_bins1(const Job<T>* __restrict__ jobs, uint32_t J, uint64_t total_bins,
                                                RegionArrays<T> r, FootArrays<T> fa,
                                                const uint32_t* __restrict__ off, const uint32_t* __restrict__ lst,
                                                const F* __restrict__ fs, JobRt* rt, ListPrefix lp, int xcd,
                                                double* __restrict__ rrw) {
-    // MODE 0: uniform region roulette and in-box sampling; 1: importance sampling in the
-    // box; 2: the region roulettes (rr_integral_region, rr_error_region, rr_pdf_region;
-    // rrw: per-list cumulative weights)
-    constexpr bool IMP = MODE == 1;
-    constexpr bool RRI = MODE == 2;
+    // MODE bit 0: importance sampling in the box can occur; bit 1: the region roulettes
+    // (rr_integral_region, rr_error_region, rr_pdf_region; rrw: per-list cumulative
+    // weights) can; 0: uniform region and uniform point only
+    constexpr bool IMP = (MODE & 1) != 0;
+    constexpr bool RRI = (MODE & 2) != 0;
     constexpr int K = pow3(D);
     // per lane: first region and running region count of its first LANE_MAXF footprints
     __shared__ uint32_t s_first[LANE_MAXF][256];
@@ -1444,12 +1469,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE !
                                                                                         aa, bb);
             approx += empty ? 0.0 : factor * integ;
             if constexpr (RRI) {  // one region per footprint here
-                if (jb.form == VMX_FORM_VR_OPTIMIZE_RR_ERROR || jb.form == VMX_FORM_VR_FIXED_RR_ERROR) {
+                const int rk = form_roulette(jb.form);
+                if (rk == 2) {
                     double share = empty ? 0.0 : 1.0;  // the bin's share of the region
 #pragma unroll
                     for (int d = 0; d < DB; ++d) share *= bb[d] - aa[d];
                     rrw[o + k] = vabs(double(r.err[fa.first[fi]])) * share;
-                } else if (jb.form == VMX_FORM_VR_OPTIMIZE_RR_PDF || jb.form == VMX_FORM_VR_FIXED_RR_PDF) {
+                } else if (rk == 3) {
                     // the pdf's integral over the bin's box; dims past the bins span the region
                     const uint32_t rg = fa.first[fi];
                     T an[D], bn[D];
@@ -1495,11 +1521,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE !
         };
         constexpr bool FP32 = sizeof(T) == 4;
         const T wn = T(factor * double(n));  // FP32 sample weight factor B * n
-        const bool rr_form = RRI && jb.form >= VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL && jb.form <= VMX_FORM_VR_FIXED_RR_PDF;
+        const bool rr_form = RRI && form_roulette(jb.form) != 0;
         double rr_total = 0.0;
         if (rr_form && spp > 0 && n > 0) {  // floor at 1% of the mean weight, then cumulate
             // rr_pdf_region keeps its factor 0.01 in a float
-            const double fl = jb.form >= VMX_FORM_VR_OPTIMIZE_RR_PDF ? double(0.01f) : 0.01;
+            const double fl = form_roulette(jb.form) == 3 ? double(0.01f) : 0.01;
             double sum = 0.0;
             for (uint32_t k = 0; k < n; ++k) sum += rrw[o + k];
             double cum = 0.0;
@@ -1510,7 +1536,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE !
             }
             rr_total = cum;
         }
-        const bool imp_form = jb.form == VMX_FORM_VR_OPTIMIZE_IMPORTANCE || jb.form == VMX_FORM_VR_FIXED_IMPORTANCE;
+        const bool imp_form = form_importance(jb.form);
         KahanSum<T> sf, sa, mf, ma, maa, mfa;
         double acc1 = 0.0;  // alpha = 1 form
         T k_f = 0, k_a = 0;
@@ -1603,9 +1629,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE !
         }
         double result;
         const double nsd = double(spp);
-        const bool fixed_w = jb.form == VMX_FORM_VR_FIXED || jb.form == VMX_FORM_VR_FIXED_IMPORTANCE ||
-                             jb.form == VMX_FORM_VR_FIXED_RR_INTEGRAL || jb.form == VMX_FORM_VR_FIXED_RR_ERROR ||
-                             jb.form == VMX_FORM_VR_FIXED_RR_PDF;
+        const bool fixed_w = form_fixed(jb.form);
         if (!sampling || (!fixed_w && spp < 2)) {
             result = approx;
         } else {
