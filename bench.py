@@ -317,11 +317,12 @@ def run_fubini(args, V, hip):
 
 
 def run_importance(args, V, hip):
-    """Importance-sampled residual (region_sampling_importance) against uniform sampling:
-    64 held-out config-1 style integrands at 256x256 FP32 bins (bin boxes above the
-    sampler's 1e-5 volume threshold), 4096 regions, 16 spp, one batched call each way.
-    RMSE per bin against the FP64 closed-form bins.  value = bins/s with importance
-    sampling."""
+    """Importance-sampled residual (region_sampling_importance) against uniform sampling,
+    and the region roulettes (rr_integral_region, rr_error_region, rr_pdf_region, and
+    rr_pdf_region with importance sampling): 64 held-out config-1 style integrands at
+    256x256 FP32 bins (bin boxes above the sampler's 1e-5 volume threshold), 4096 regions,
+    16 spp, one batched call per variant.  RMSE per bin against the FP64 closed-form
+    bins.  value = bins/s with importance sampling."""
     from viltrum_amd import integrands as I
     from viltrum_amd import refiner as R
     ctx = V.Context(0)
@@ -333,7 +334,11 @@ def run_importance(args, V, hip):
     ptrs = [b.address for b in bufs]
     stream = hip.Stream()
     out = {}
-    for name, form in (("uniform", V.FORM_VR_OPTIMIZE), ("importance", V.FORM_VR_OPTIMIZE_IMPORTANCE)):
+    variants = (("uniform", V.FORM_VR_OPTIMIZE), ("importance", V.FORM_VR_OPTIMIZE_IMPORTANCE),
+                ("rr_integral", V.FORM_VR_OPTIMIZE_RR_INTEGRAL), ("rr_error", V.FORM_VR_OPTIMIZE_RR_ERROR),
+                ("rr_pdf", V.FORM_VR_OPTIMIZE_RR_PDF), ("rr_pdf_importance", V.FORM_VR_OPTIMIZE_RR_PDF_IMPORTANCE))
+    sts = {}
+    for name, form in variants:
         ps = [V.Integrator(4095, 16, 0, form=form).problem(f, res, V.range_primary(2), V.F32) for f in fs]
         tune = ctx.autotune(ps, fs, ptrs, stream.address)
         for _ in range(args.warmup):
@@ -351,7 +356,10 @@ def run_importance(args, V, hip):
             se2 += float(np.sum((h.astype(np.float64) - t) ** 2))
         out[name] = {"ms": ms, "bins_per_s": nb * len(fs) / (ms * 1e-3), "rmse": float(np.sqrt(se2 / (nb * len(fs)))),
                      "kernel_ms": float(st[0].ms_bins), "autotune": tune}
+        sts[name] = st
+        print("variant %s: %.3f ms, rmse %.3e" % (name, ms, out[name]["rmse"]), file=sys.stderr, flush=True)
     imp = out["importance"]
+    st = sts["importance"]
     flops = sum(x.evals_residual for x in st) * I.flops_per_eval(fs[0])[0]
     achieved = flops / (imp["kernel_ms"] * 1e-3) / 1e12
     res_out = {
@@ -362,9 +370,11 @@ def run_importance(args, V, hip):
                                "bins, 4096 regions, 16 spp, one batched call" % len(fs),
                    "integrals": len(fs), "bins": nb * len(fs), "regions": 4096, "spp": 16,
                    "path": "batched, region_sampling_importance"},
-        "rmse_vs_fp64": {"importance": imp["rmse"], "uniform": out["uniform"]["rmse"]},
+        "rmse_vs_fp64": {k: v["rmse"] for k, v in out.items()},
         "fp64_truth": {"kind": "FP64 closed-form per-bin integrals (erf, polygon area, separable wave)"},
         "uniform": out["uniform"],
+        "variants": {k: {"bins_per_s": v["bins_per_s"], "ms": v["ms"], "kernel_ms": v["kernel_ms"], "rmse": v["rmse"]}
+                     for k, v in out.items()},
         "roofline": {"bound": "valu", "kernel": "k_bins1 (importance-sampling instantiation, batched)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel_ms": imp["kernel_ms"]},
