@@ -11,10 +11,13 @@
  *
  * The arithmetic follows the library's mixed float/double expressions as written (the
  * sign of a coefficient is a float, NormDefault::sign at src/control-variates/norm.h:28-29;
- * the cubic's coefficients and the root formulas are double), with the device's double
- * pow/sqrt/acos/cos.  One deviation: where the pdf integrates to 0 over the box (a
- * region whose nodes are all 0), the reference divides 0 by 0 and its bin becomes NaN;
- * here sample() reports it and the caller samples the box uniformly instead.
+ * the cubic's coefficients and the root formulas are double), with cheaper forms of the
+ * same values: integer powers (the library's pow(x, 2.) and pow(x, 3.)) as products, its
+ * cube root pow(|x|, 1/3.) as cbrt, and Viete's three cosines from one sincos by the
+ * angle-difference identity, each within a few ulps of the library's double result.
+ * One deviation: where the pdf integrates to 0 over the box (a region whose nodes are
+ * all 0), the reference divides 0 by 0 and its bin becomes NaN; here sample() reports it
+ * and the caller samples the box uniformly instead.
  */
 #ifndef VMX_IMPORTANCE_H
 #define VMX_IMPORTANCE_H
@@ -71,7 +74,7 @@ VMX_HD inline T pdf_unnormalized(T t, const T* p) {
 }
 VMX_HD inline float sign_of(float v) { return v; }
 VMX_HD inline float sign_of(double v) { return float(v); }
-VMX_HD inline double cuberoot(double x) { return (x < 0) ? (-1 * pow(-x, 1. / 3.)) : pow(x, 1. / 3.); }
+VMX_HD inline double cuberoot(double x) { return cbrt(x); }
 
 /* roots of cdf(t) = x in order; returns their count (0..3) */
 template <typename T>
@@ -99,16 +102,24 @@ VMX_HD inline int inv_cdf(T x, const T* points, T* sol) {
             }
         }
     } else {
-        const double pp = c / a - pow(b, 2.) / (3. * pow(a, 2.));
-        const double qq = 2 * pow(b, 3.) / (27. * pow(a, 3.)) - b * c / (3. * pow(a, 2.)) + d / a;
-        const double sqr = pow(qq / 2., 2.) + pow(pp / 3., 3.);
+        const double a2 = a * a, a3 = a2 * a, b2 = b * b, b3 = b2 * b;
+        const double pp = c / a - b2 / (3. * a2);
+        const double qq = 2 * b3 / (27. * a3) - b * c / (3. * a2) + d / a;
+        const double hq = qq / 2., tp = pp / 3.;
+        const double sqr = hq * hq + tp * tp * tp;
         if (sqr >= 0.) {
             sol[n++] = T(cuberoot(-qq / 2. - sqrt(sqr)) + cuberoot(-qq / 2. + sqrt(sqr)) - b / (3 * a));
         } else {
-            const double pi = 3.14159265358979323846;
-            for (int i = 0; i < 3; ++i)
-                sol[n++] = T(2. * sqrt(-pp / 3.) * cos(1. / 3. * acos(3. * qq / (2. * pp) * sqrt(-3. / pp)) - 2. * pi * i / 3.) -
-                             b / (3. * a));
+            // Viete: 2 sqrt(-p/3) cos(phi - 2 pi i / 3) - b / 3a, phi = acos(...) / 3; the
+            // three cosines from one sincos(phi) by the angle-difference identity
+            const double phi = 1. / 3. * acos(3. * qq / (2. * pp) * sqrt(-3. / pp));
+            double sphi, cphi;
+            sincos(phi, &sphi, &cphi);
+            const double r = 2. * sqrt(-pp / 3.), sh = b / (3. * a);
+            const double h3 = 0.86602540378443864676;  // sin(2 pi / 3)
+            sol[n++] = T(r * cphi - sh);
+            sol[n++] = T(r * (-0.5 * cphi + h3 * sphi) - sh);
+            sol[n++] = T(r * (-0.5 * cphi - h3 * sphi) - sh);
         }
     }
     return n;
@@ -159,59 +170,62 @@ VMX_HD inline bool sample(const T* v, const T* rlo, const T* rhi, T rvol, const 
     static_assert(D >= 1 && D <= MAX_DIM, "importance sampling: 1..3 dims");
     constexpr int K = pow3(D);
     T an[D], bn[D], un[D], pn[D];
+#pragma unroll
     for (int d = 0; d < D; ++d) {
         an[d] = (rlo[d] >= rhi[d]) ? rlo[d] : (slo[d] - rlo[d]) / (rhi[d] - rlo[d]);
         bn[d] = (rlo[d] >= rhi[d]) ? rlo[d] : (shi[d] - rlo[d]) / (rhi[d] - rlo[d]);
     }
     // normalisation first: the pdf's integral over the box (folds last dim first)
-    T u[K];
-    for (int i = 0; i < K; ++i) u[i] = v[i];
-    int len = K;
-    for (int d = D - 1; d > 0; --d) {
-        len /= 3;
-        for (int i = 0; i < len; ++i) {
-            const T l[3] = {u[i], u[i + len], u[i + 2 * len]};
-            u[i] = pdf_integral(an[d], bn[d], l);
-        }
-    }
-    const T den = rvol * pdf_integral(an[0], bn[0], u);
+    const T den = rvol * pdf_fold<T, D, 0>(v, 0, an, bn);
     if (!(den > T(0)) || !(den < T(INFINITY))) return false;
     // draw: dim k from the marginal, then condition on it
     T t[K];
+#pragma unroll
     for (int i = 0; i < K; ++i) t[i] = v[i];
-    int tl = K;  // 3^(D-k)
+#pragma unroll
     for (int k = 0; k < D; ++k) {
+        const int tl = pow3(D - k);  // entries of t still live
         T m[K];
-        for (int i = 0; i < tl; ++i) m[i] = t[i];
-        int ml = tl;
+#pragma unroll
+        for (int i = 0; i < K; ++i)
+            if (i < tl) m[i] = t[i];
+#pragma unroll
         for (int d = D - 1; d > k; --d) {
-            ml /= 3;
-            for (int i = 0; i < ml; ++i) {
-                const T l[3] = {m[i], m[i + ml], m[i + 2 * ml]};
-                m[i] = pdf_integral(an[d], bn[d], l);
-            }
+            const int ml = pow3(d - k);  // t spans dims k..D-1: folding dim d leaves 3^(d-k) lines
+#pragma unroll
+            for (int i = 0; i < K / 3; ++i)
+                if (i < ml) {
+                    const T l[3] = {m[i], m[i + ml], m[i + 2 * ml]};
+                    m[i] = pdf_integral(an[d], bn[d], l);
+                }
         }
         const T sol = sample_line(s[k], an[k], bn[k], m);
         un[k] = sol;
         if (k + 1 < D) {
-            tl /= 3;
-            for (int j = 0; j < tl; ++j) {
-                const T l[3] = {t[3 * j], t[3 * j + 1], t[3 * j + 2]};
-                t[j] = pdf_unnormalized(sol, l);
-            }
+#pragma unroll
+            for (int j = 0; j < K / 3; ++j)
+                if (j < tl / 3) {
+                    const T l[3] = {t[3 * j], t[3 * j + 1], t[3 * j + 2]};
+                    t[j] = pdf_unnormalized(sol, l);
+                }
         }
     }
+#pragma unroll
     for (int d = 0; d < D; ++d) x[d] = un[d] * (rhi[d] - rlo[d]) + rlo[d];
+#pragma unroll
     for (int d = 0; d < D; ++d) pn[d] = (rlo[d] >= rhi[d]) ? rlo[d] : (x[d] - rlo[d]) / (rhi[d] - rlo[d]);
     // pdf at the point: unnormalised folds at its coordinates, last dim first
+#pragma unroll
     for (int i = 0; i < K; ++i) t[i] = v[i];
-    len = K;
+#pragma unroll
     for (int d = D - 1; d > 0; --d) {
-        len /= 3;
-        for (int i = 0; i < len; ++i) {
-            const T l[3] = {t[i], t[i + len], t[i + 2 * len]};
-            t[i] = pdf_unnormalized(pn[d], l);
-        }
+        const int len = pow3(d);
+#pragma unroll
+        for (int i = 0; i < K / 3; ++i)
+            if (i < len) {
+                const T l[3] = {t[i], t[i + len], t[i + 2 * len]};
+                t[i] = pdf_unnormalized(pn[d], l);
+            }
     }
     const T pdf = pdf_unnormalized(pn[0], t) / den;
     weight = T((pdf < 1.e-5) ? 0.0 : (1.0 / pdf));
