@@ -166,3 +166,11 @@ def test_roulette_with_importance_rejects_fp64_and_five_dimensions(ctx):
     with pytest.raises(V.VmxError):
         V.integrate(V.integrator_variance_reduction_rr_integral(40, 2, 0, importance=True), np.zeros((3, 3), np.float32),
                     f5, V.range_primary(5), ctx=ctx)
+
+
+def test_unknown_forms_are_rejected(ctx):
+    f = I.config1()
+    for form in (-1, 17):
+        with pytest.raises((ValueError, V.VmxError)):
+            V.integrate(V.Integrator(63, 4, 0, form=form), np.zeros((4, 4), np.float32), f, V.range_primary(2),
+                        ctx=ctx)
