@@ -13,8 +13,10 @@
  * sign of a coefficient is a float, NormDefault::sign at src/control-variates/norm.h:28-29;
  * the cubic's coefficients and the root formulas are double), with cheaper forms of the
  * same values: integer powers (the library's pow(x, 2.) and pow(x, 3.)) as products, its
- * cube root pow(|x|, 1/3.) as cbrt, and Viete's three cosines from one sincos by the
- * angle-difference identity, each within a few ulps of the library's double result.
+ * cube root pow(|x|, 1/3.) as cbrt, Viete's three cosines from one sincos by the
+ * angle-difference identity, divisions by constants and by one value repeatedly as
+ * products with reciprocals, and the line minimum's position in T, each within a few
+ * ulps of the library's double result.
  * One deviation: where the pdf integrates to 0 over the box (a region whose nodes are
  * all 0), the reference divides 0 by 0 and its bin becomes NaN; here sample() reports it
  * and the caller samples the box uniformly instead.
@@ -37,12 +39,12 @@ VMX_HD inline T at3(T t, const T* p) {
 template <typename T>
 VMX_HD inline T subrange3(T a, T b, const T* p) {
     const T c0 = p[0], c1 = -3 * p[0] + 4 * p[1] - p[2], c2 = 2 * p[0] - 4 * p[1] + 2 * p[2];
-    return T(((c2 * b / 3.0 + c1 / 2.0) * b + c0) * b - ((c2 * a / 3.0 + c1 / 2.0) * a + c0) * a);
+    return T(((c2 * b * (1.0 / 3.0) + c1 * 0.5) * b + c0) * b - ((c2 * a * (1.0 / 3.0) + c1 * 0.5) * a + c0) * a);
 }
 template <typename T>
 VMX_HD inline T cdf3(T t, const T* p) {
     const T c0 = p[0], c1 = -3 * p[0] + 4 * p[1] - p[2], c2 = 2 * p[0] - 4 * p[1] + 2 * p[2];
-    return T(((c2 * t / 3.0 + c1 / 2.0) * t + c0) * t);
+    return T(((c2 * t * (1.0 / 3.0) + c1 * 0.5) * t + c0) * t);
 }
 template <typename T>
 VMX_HD inline void pdf_points(const T* p, T* q) {
@@ -52,7 +54,7 @@ VMX_HD inline void pdf_points(const T* p, T* q) {
     const T c1 = -3 * q[0] + 4 * q[1] - q[2], c2 = 2 * q[0] - 4 * q[1] + 2 * q[2], c0 = q[0];
     T ymin = 0;
     if (c2 > 0) {
-        const T tmin = T(-c1 / (2.0 * c2));
+        const T tmin = -c1 / (T(2) * c2);
         if ((tmin > 0) && (tmin < 1)) {
             const T ytmin = (c2 * tmin + c1) * tmin + c0;
             if (ytmin < ymin) ymin = ytmin;
@@ -85,11 +87,11 @@ VMX_HD inline int inv_cdf(T x, const T* points, T* sol) {
     double b = sign_of(c1) / 2.;
     float c = sign_of(c0);
     T d = -x;
-    const double sum = a + b + c + d;
-    a /= sum;
-    b /= sum;
-    c = float(c / sum);
-    d = T(d / sum);
+    const double sum = a + b + c + d, isum = 1.0 / sum;
+    a *= isum;
+    b *= isum;
+    c = float(c * isum);
+    d = T(d * isum);
     int n = 0;
     if (fabs(a) < 1.e-3) {
         if (fabs(b) < 1.e-3) {
@@ -102,20 +104,22 @@ VMX_HD inline int inv_cdf(T x, const T* points, T* sol) {
             }
         }
     } else {
-        const double a2 = a * a, a3 = a2 * a, b2 = b * b, b3 = b2 * b;
-        const double pp = c / a - b2 / (3. * a2);
-        const double qq = 2 * b3 / (27. * a3) - b * c / (3. * a2) + d / a;
+        const double ia = 1.0 / a, ia2 = ia * ia, b2 = b * b, b3 = b2 * b;
+        const double pp = c * ia - b2 * ia2 * (1. / 3.);
+        const double qq = b3 * ia2 * ia * (2. / 27.) - b * c * ia2 * (1. / 3.) + d * ia;
         const double hq = qq / 2., tp = pp / 3.;
         const double sqr = hq * hq + tp * tp * tp;
         if (sqr >= 0.) {
-            sol[n++] = T(cuberoot(-qq / 2. - sqrt(sqr)) + cuberoot(-qq / 2. + sqrt(sqr)) - b / (3 * a));
+            const double sq = sqrt(sqr);
+            sol[n++] = T(cuberoot(-hq - sq) + cuberoot(-hq + sq) - b * ia * (1. / 3.));
         } else {
             // Viete: 2 sqrt(-p/3) cos(phi - 2 pi i / 3) - b / 3a, phi = acos(...) / 3; the
             // three cosines from one sincos(phi) by the angle-difference identity
-            const double phi = 1. / 3. * acos(3. * qq / (2. * pp) * sqrt(-3. / pp));
+            const double ipp = 1.0 / pp;
+            const double phi = 1. / 3. * acos(1.5 * qq * ipp * sqrt(-3. * ipp));
             double sphi, cphi;
             sincos(phi, &sphi, &cphi);
-            const double r = 2. * sqrt(-pp / 3.), sh = b / (3. * a);
+            const double r = 2. * sqrt(-pp * (1. / 3.)), sh = b * ia * (1. / 3.);
             const double h3 = 0.86602540378443864676;  // sin(2 pi / 3)
             sol[n++] = T(r * cphi - sh);
             sol[n++] = T(r * (-0.5 * cphi + h3 * sphi) - sh);
