@@ -127,6 +127,8 @@ double vmh_monomial(int scalar, int D, const double* rlo, const double* rhi, con
     VMH_DISPATCH(mono_n, rlo, rhi, nodes, x)
     return 0;
 }
+uint64_t vmh_max_pairs(void) { return vmx::MAX_PAIRS; }
+
 void vmh_pixel_range(int scalar, double rlo, double rhi, double lo, double hi, uint64_t res, uint64_t* se) {
     if (scalar == 4)
         vmx::pixel_range<float>(float(rlo), float(rhi), float(lo), float(hi), res, se[0], se[1]);

@@ -189,3 +189,37 @@ def test_bf16_rounding_matches_oracle():
     np.testing.assert_array_equal(got, want)
     # round to nearest even at the halfway points
     assert H.vmh_f2bf(1.00390625) == 0x3F80 and H.vmh_f2bf(1.01171875) == 0x3F82
+
+
+# a 3-D integrand that varies only along the integrated dim t: its regions stay wide in
+# the bin dims, so at a fine resolution each region covers millions of bins
+PAIRS_CASE = dict(terms=[(1.0, (0, 0, 4))], iterations=1099, res=(8192, 4096))
+
+
+def pair_total(g, res):
+    se = np.zeros(2, dtype=np.uint64)
+    up = C.POINTER(C.c_uint64)
+    H.vmh_pixel_range.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint64, up]
+    total = 0
+    for lo, hi in zip(g.lo, g.hi):
+        n = 1
+        for d in range(2):
+            H.vmh_pixel_range(4, lo[d], hi[d], 0.0, 1.0, res[d], P(se, up))
+            n *= int(se[1] - se[0])
+        total += n
+    return total
+
+
+def test_pair_total_guard_covers_the_t_only_case():
+    """The (bin, footprint) pair total of the t-only case passes what 32-bit list offsets
+    hold (so the engine must refuse it; tests/test_gpu_capacity.py checks that it does),
+    while the same regions at 1024^2 bins stay far below."""
+    H.vmh_max_pairs.restype = C.c_uint64
+    limit = H.vmh_max_pairs()
+    assert limit == 2 ** 32 - 2
+    f = I.poly(PAIRS_CASE["terms"], 3)
+    g = O.generate(f, PAIRS_CASE["iterations"], scalar=4)
+    big = pair_total(g, PAIRS_CASE["res"])
+    assert big > limit
+    assert big % 2 ** 32 < big  # a 32-bit count would wrap
+    assert pair_total(g, (1024, 1024)) < limit // 16

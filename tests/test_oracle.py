@@ -57,7 +57,7 @@ def test_fubini_matches_reference_fixture(case):
     assert np.array_equal(b, FIX[MG.fubini_key(name, rd, res, it, mc, spp, seed)])
 
 
-@pytest.mark.skipif(O.ref() is None, reason="reference build absent (no /root/reference where this runs)")
+@pytest.mark.skipif(not os.path.exists(O.REF_SO), reason="reference build absent (no /root/reference where this runs)")
 @pytest.mark.parametrize("seed", range(3))
 def test_fubini_oracle_matches_live_reference(seed):
     rng = np.random.default_rng(40 + seed)
@@ -79,7 +79,7 @@ def test_fubini_with_a_tensor_quadratic_in_the_region_dims_is_exact():
     assert abs(b.mean() - (1.0 / 6 + 0.5 + 1.5)) < 5e-3
 
 
-@pytest.mark.skipif(O.ref() is None, reason="reference build absent (no /root/reference where this runs)")
+@pytest.mark.skipif(not os.path.exists(O.REF_SO), reason="reference build absent (no /root/reference where this runs)")
 @pytest.mark.parametrize("seed", range(4))
 def test_oracle_matches_live_reference(seed):
     rng = np.random.default_rng(seed)

@@ -1,952 +1,13 @@
 /*
- * Host orchestration and C ABI of the MI355X per-bin control-variate integrator.
- *
- * Pipeline of a batch of integrals of one (scalar, D, family) (run_batch; one
- * integral is a batch of one):
- *   1. k_roots, then per refinement round k_theta_round + k_expand_round (launched in
- *      chunks; the host checks for remaining candidates between chunks).
- *   2. exact selection of the N-1 split nodes per integral on the device (canonical
- *      order), or, for the reference's heap-array order, libstdc++ push_heap/pop_heap
- *      replayed on the host over the GPU-computed errors
- *      (src/nested/regions-generator-adaptive-heap.h:21-41).
- *   3. k_prepare2 (boxes, volumes, FP64 bin-dim marginals, monomial coefficients),
- *      footprints, per-bin footprint lists (pairs emitted per footprint at scanned
- *      offsets, stable radix sort by bin, boundary offsets).
- *   4. k_bins1: per-bin control variate + residual Monte Carlo, one lane per bin.
- * Host synchronisation: after generation (capacity check) and once to size the bin
- * lists.
+ * C ABI of the MI355X per-bin control-variate integrator: validation, grouping of
+ * batches by (scalar, D, family), plain Monte Carlo, device evaluation, the priority
+ * network's training and inference entry points.  The batch pipeline itself is
+ * run_batch.h.
  */
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <chrono>
-#include <cmath>
-#include <cstdlib>
-#include <cstring>
-#include <array>
-#include <functional>
-#include <map>
-#include <random>
-#include <limits>
-#include <type_traits>
-#include <typeinfo>
-#include <new>
-#include <stdexcept>
-#include <string>
-#include <vector>
-
-#include <cstring>
-#include <rocprim/rocprim.hpp>
-
-#include "../../include/viltrum_amd.h"
+#include "engine_core.h"
 #include "exact.h"
-#include "pipeline.h"
 
-struct vmx_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    // device memory pool: one arena per purpose, grown on demand
-    struct Buf {
-        void* p = nullptr;
-        size_t bytes = 0;
-    };
-    Buf tree, regs, lists, misc, params, bins, bins2;
-    Buf nn[VMX_MAX_DIM + 1];                       // FP32 priority-network parameters per dim
-    Buf nn_lds[VMX_MAX_DIM + 1];                   // ... and their LDS image (nn::NetW<KP>)
-    bool nn_ok[VMX_MAX_DIM + 1] = {};
-    Buf tr_x, tr_t, tr_idx, tr_part, tr_loss, tr_state;  // training work space
-    hipEvent_t ev[8] = {};
-    void* pinned = nullptr;  // host staging for small device-to-host readbacks
-    size_t pinned_bytes = 0;
-    // one chunk of refinement rounds captured as a graph, reused while its launch
-    // arguments (buffers, shapes, stream) stay the same
-    struct RoundGraph {
-        std::vector<unsigned char> key;
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-    } rounds;
-    // launch shapes: the autotuner's choices per workload class, and the shape a tuning
-    // trial forces (keys of the classes the trial ran are collected in tuned_keys)
-    struct Shape {
-        int gen_per_cu = 4, round_chunk = 8, bins_xcd = 0, deepen = 0;
-    };
-    std::map<std::array<uint64_t, 6>, Shape> tune;
-    // node capacity per region (times N) the last call of a workload class needed, so a
-    // class whose rounds overflow 8N does not pay a failed attempt on every call
-    std::map<std::array<uint64_t, 6>, uint64_t> cap_hint;
-    const Shape* trial = nullptr;
-    std::vector<std::array<uint64_t, 6>> tuned_keys;
-    // last result (for vmx_export_regions): gathered from device memory on demand
-    int last_D = 0;
-    uint64_t last_n = 0;
-    std::function<void(double*, double*, double*, double*, int32_t*)> last_gather;
-};
-
-namespace {
-
-struct VmxError : std::runtime_error {
-    int code;
-    VmxError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-#define HIP_CHECK(x)                                                                                   \
-    do {                                                                                               \
-        hipError_t e_ = (x);                                                                           \
-        if (e_ != hipSuccess)                                                                          \
-            throw VmxError(VMX_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_));            \
-    } while (0)
-
-void* ensure(vmx_ctx::Buf& b, size_t bytes) {
-    if (bytes == 0) bytes = 256;
-    if (b.bytes < bytes) {
-        if (b.p) HIP_CHECK(hipFree(b.p));
-        b.p = nullptr;
-        b.bytes = 0;
-        HIP_CHECK(hipMalloc(&b.p, bytes));
-        b.bytes = bytes;
-    }
-    return b.p;
-}
-
-void* ensure_pinned(vmx_ctx* ctx, size_t bytes) {
-    if (ctx->pinned_bytes < bytes) {
-        if (ctx->pinned) HIP_CHECK(hipHostFree(ctx->pinned));
-        ctx->pinned = nullptr;
-        ctx->pinned_bytes = 0;
-        const size_t want = std::max<size_t>(bytes, 1 << 16);
-        HIP_CHECK(hipHostMalloc(&ctx->pinned, want, hipHostMallocDefault));
-        ctx->pinned_bytes = want;
-    }
-    return ctx->pinned;
-}
-
-/* Carves aligned sub-arrays out of one allocation. */
-inline int ilog2(uint64_t x) {
-    int r = 0;
-    while (x > 1) {
-        x >>= 1;
-        ++r;
-    }
-    return r;
-}
-
-/* Sub-allocations of one buffer, 256-byte aligned; taking more than the buffer holds
-   is a bug in the caller's size computation and throws. */
-struct Carve {
-    char* base;
-    size_t used = 0, cap;
-    explicit Carve(const vmx_ctx::Buf& b) : base(static_cast<char*>(b.p)), cap(b.bytes) {}
-    template <typename X>
-    X* take(size_t n) {
-        used = (used + 255) & ~size_t(255);
-        if (used + n * sizeof(X) > cap) throw VmxError(VMX_ERR_CAPACITY, "work buffer carved past its size");
-        X* r = reinterpret_cast<X*>(base + used);
-        used += n * sizeof(X);
-        return r;
-    }
-};
-struct Measure {
-    size_t used = 0;
-    template <typename X>
-    void take(size_t n) {
-        used = (used + 255) & ~size_t(255);
-        used += n * sizeof(X);
-    }
-};
-
-template <typename P>
-struct ParamOf;
-template <typename P>
-struct F64Of;
-template <typename T>
-struct F64Of<vmx::Poly<T>> {
-    using type = vmx::Poly<double>;
-};
-template <typename T>
-struct F64Of<vmx::Blobs<T>> {
-    using type = vmx::Blobs<double>;
-};
-template <typename T>
-struct F64Of<vmx::Lobes5<T>> {
-    using type = vmx::Lobes5<double>;
-};
-template <typename T>
-struct ParamOf<vmx::Poly<T>> {
-    using type = vmx_poly_params;
-};
-template <typename T>
-struct ParamOf<vmx::Blobs<T>> {
-    using type = vmx_blobs_params;
-};
-template <typename T>
-struct ParamOf<vmx::Lobes5<T>> {
-    using type = vmx_lobes5_params;
-};
-template <typename T, typename Inner, int DF, int RD>
-struct ParamOf<vmx::Fubini<T, Inner, DF, RD>> {
-    using type = vmx_fubini_params;
-};
-
-double ms_since(std::chrono::steady_clock::time_point t0) {
-    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-}
-
-/* One (scalar, D, family) batch through the device pipeline (pipeline.h). */
-/* A caller-given region list (host memory, the layout vmx_export_regions writes):
-   boxes n x D and node values n x 3^D, dim 0 fastest. */
-struct GivenRegions {
-    uint64_t n;
-    const double *lo, *hi, *nodes;
-};
-
-struct JobIn {
-    const vmx_problem* p;
-    const void* params;
-    void* bins;  // device pointer
-    vmx_stats* stats;
-    const GivenRegions* given = nullptr;  // set: skip generation, integrate over these regions
-};
-
-/* Training-pair mode of run_batch: stop after generation, report the node count and,
-   if alloc hands back device arrays for that many rows, fill them. */
-struct PairsReq {
-    uint64_t count = 0;
-    int kind = vmx::nn::TARGET_ENERGY;
-    uint32_t stride = 1;
-    std::function<void(uint64_t, uint16_t**, float**)> alloc;
-};
-
-constexpr int LEARN_MAX_DIM = 5;  // the fused network fits the expansion kernel's LDS up to here
-
-template <typename T, int D, typename P, typename Math>
-void launch_expand(bool learn, const vmx::GenArgs<T>& g, const P* d_f, int par, uint32_t blocks, hipStream_t st) {
-    const size_t pre_bytes = size_t(g.J + 1) * sizeof(uint32_t);  // the per-job candidate prefix
-    if constexpr (std::is_same<T, float>::value && D <= LEARN_MAX_DIM) {
-        if (learn) {
-            vmx::k_expand_round<T, D, P, Math, true><<<blocks, vmx::GEN_THREADS, pre_bytes, st>>>(g, d_f, par);
-            return;
-        }
-    }
-    vmx::k_expand_round<T, D, P, Math, false><<<blocks, vmx::GEN_THREADS, pre_bytes, st>>>(g, d_f, par);
-}
-
-template <typename T>
-struct RocprimTemp {
-    size_t sort_bytes = 0, scan_bytes = 0;
-};
-
-template <typename T, int D, typename P, typename Math>
-int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsReq* pairs_req = nullptr) {
-    constexpr int K = vmx::pow3(D);
-    const auto t_start = std::chrono::steady_clock::now();
-    const uint32_t J = uint32_t(in.size());
-    if (J == 0) return VMX_OK;
-    if (J > uint32_t(vmx::MAX_JOBS)) throw VmxError(VMX_ERR_INVALID, "at most 4096 integrals per batch");
-    const int order = in[0].p->order;
-    for (const JobIn& x : in)
-        if (x.p->order != order) throw VmxError(VMX_ERR_INVALID, "one region order per batch");
-    // a given region list keeps its own order (like the reference order): no merging of
-    // regions into shared footprints, so per-bin sums follow the list
-    const bool replay = in[0].given != nullptr;
-    for (const JobIn& x : in)
-        if ((x.given != nullptr) != replay) throw VmxError(VMX_ERR_INVALID, "given regions for all or none");
-    if (replay && pairs_req) throw VmxError(VMX_ERR_INVALID, "training pairs need generation");
-    // the region roulettes weigh each region of a bin: one footprint per region
-    bool any_rr = false, any_rr_err = false;
-    for (const JobIn& x : in) {
-        any_rr |= vmx::form_roulette(x.p->form) != 0;
-        any_rr_err |= vmx::form_roulette(x.p->form) == 2;
-    }
-    const bool merge = order == VMX_ORDER_CANONICAL && !replay && !any_rr;
-
-    std::vector<vmx::Job<T>> jobs(J);
-    std::vector<P> fparams(J);
-    uint64_t total_bins = 0, n_reg = 0;
-    for (uint32_t j = 0; j < J; ++j) {
-        const vmx_problem& p = *in[j].p;
-        vmx::Job<T>& jb = jobs[j];
-        std::memset(&jb, 0, sizeof(jb));
-        for (int d = 0; d < D; ++d) {
-            jb.lo[d] = T(p.lo[d]);
-            jb.hi[d] = T(p.hi[d]);
-        }
-        jb.db = p.bin_dims;
-        jb.mb = 1;
-        jb.nbins = 1;
-        for (int i = 0; i < 3; ++i) {
-            jb.res[i] = i < jb.db ? p.res[i] : 1;
-            jb.drange[i] = i < jb.db ? (jb.hi[i] - jb.lo[i]) / T(p.res[i]) : T(0);
-            if (i < jb.db) {
-                jb.mb *= 3;
-                jb.nbins *= p.res[i];
-            }
-        }
-        jb.bin_base = total_bins;
-        total_bins += jb.nbins;
-        jb.spp = p.spp;
-        jb.bins = in[j].bins;
-        jb.alpha = p.alpha;
-        jb.seed_lo = uint32_t(p.seed);
-        jb.seed_hi = uint32_t(p.seed >> 32);
-        jb.stream = 0;  // random streams depend on the seed only, not on the batch position
-        jb.N = uint32_t(replay ? in[j].given->n : p.regions);
-        jb.reg_base = uint32_t(n_reg);
-        n_reg += jb.N;
-        jb.form = p.form;
-        jb.heur = p.heuristic;
-        // training pairs follow each problem's own refiner too: the regions a learned
-        // priority generates are where its network must be right (on-policy pairs)
-        jb.learn = p.refiner == VMX_REFINER_LEARNED && !replay ? 1 : 0;
-        // the size floor of the learned priority costs at most 1/64 of the budget
-        jb.lfloor = jb.learn ? std::max(0, (ilog2(p.regions) - 6) / D) : 0;
-        for (int d = 0; d < VMX_MAX_DIM; ++d) jb.lres[d] = d < jb.db ? float(std::log2(double(p.res[d]))) : 0.f;
-        fparams[j] = P::from(*static_cast<const typename ParamOf<P>::type*>(in[j].params));
-    }
-    bool learn = false;
-    for (const auto& jb : jobs) learn |= jb.learn != 0;
-    if (learn) {
-        if (vmx::IsFubini<P>::value) throw VmxError(VMX_ERR_UNSUPPORTED, "learned refiner: not for the Fubini form");
-        if (!std::is_same<T, float>::value || D > LEARN_MAX_DIM)
-            throw VmxError(VMX_ERR_UNSUPPORTED, "learned refiner: FP32 problems of dimension 1..5 only");
-        if (!ctx->nn_ok[D]) throw VmxError(VMX_ERR_UNSUPPORTED, "learned refiner: no network for this dimension");
-        if (order != VMX_ORDER_CANONICAL)
-            throw VmxError(VMX_ERR_INVALID, "learned refiner: canonical region order only");
-    }
-    if (n_reg > 0x7FFFFFF0ull || total_bins > 0xFFFFFFF0ull) throw VmxError(VMX_ERR_CAPACITY, "batch too large");
-    int mb_max = 1;
-    for (const auto& jb : jobs) mb_max = std::max(mb_max, int(jb.mb));
-
-    // launch shapes: a tuning trial's, else VMX_* environment overrides, else the
-    // autotuner's choice for this workload class, else the defaults
-    int dev = ctx->device, n_cu = 0;
-    HIP_CHECK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
-    const std::array<uint64_t, 6> tune_key = {uint64_t(typeid(P).hash_code()), uint64_t(D), sizeof(T),
-                                              uint64_t(learn), uint64_t(ilog2(J)), uint64_t(ilog2(total_bins))};
-    vmx_ctx::Shape shape;
-    if (ctx->trial) {
-        shape = *ctx->trial;
-        ctx->tuned_keys.push_back(tune_key);
-    } else {
-        const auto it = ctx->tune.find(tune_key);
-        if (it != ctx->tune.end()) shape = it->second;
-        if (const char* e = std::getenv("VMX_GEN_PER_CU")) shape.gen_per_cu = std::max(1, std::atoi(e));
-        if (const char* e = std::getenv("VMX_ROUND_CHUNK")) shape.round_chunk = std::max(2, std::atoi(e) & ~1);
-        if (const char* e = std::getenv("VMX_BINS_XCD")) shape.bins_xcd = std::atoi(e) != 0;
-        if (const char* e = std::getenv("VMX_DEEPEN")) shape.deepen = std::atoi(e) != 0;
-    }
-    const uint32_t gen_blocks = uint32_t(n_cu) * uint32_t(shape.gen_per_cu);
-    const int round_chunk = shape.round_chunk;
-
-    uint32_t rounds = 0;
-    uint32_t max_depth[VMX_MAX_DIM] = {};
-    uint64_t cap_mult = 8;
-    if (const auto h = ctx->cap_hint.find(tune_key); h != ctx->cap_hint.end()) cap_mult = h->second;
-    vmx::TreeArrays<T> t{};
-    vmx::Job<T>* d_jobs = nullptr;
-    vmx::JobRt* d_rt = nullptr;
-    P* d_f = nullptr;
-    uint32_t *d_hist16 = nullptr, *d_ncand = nullptr, *d_rounds = nullptr;
-    T* d_theta = nullptr;
-    uint32_t* d_cstart = nullptr;
-    uint32_t cap_total = 0;
-    std::vector<vmx::JobRt> rt(J);
-    // ---------------- 1. generation (retried with more node capacity if needed)
-    for (int attempt = 0;; ++attempt) {
-        uint64_t cap64 = 0;
-        for (auto& jb : jobs) {
-            jb.node_base = uint32_t(cap64);
-            jb.node_cap = replay ? jb.N : uint32_t(cap_mult * jb.N + 64);
-            cap64 += jb.node_cap;
-        }
-        if (cap64 > 0x7FFFFFF0ull) throw VmxError(VMX_ERR_CAPACITY, "region budget too large for the node arena");
-        cap_total = uint32_t(cap64);
-        Measure m;
-        m.take<vmx::Job<T>>(J);
-        m.take<vmx::JobRt>(J);
-        m.take<P>(J);
-        m.take<T>(size_t(cap_total) * D);
-        m.take<T>(size_t(cap_total) * D);
-        m.take<T>(size_t(cap_total) * K);
-        for (int i = 0; i < 3; ++i) m.take<T>(cap_total);
-        for (int i = 0; i < 3; ++i) m.take<int32_t>(cap_total);
-        m.take<uint32_t>(size_t(cap_total) * D);
-        m.take<uint32_t>(cap_total);
-        m.take<uint8_t>(cap_total);
-        m.take<uint32_t>(size_t(J) * 65536);
-        m.take<uint32_t>(2 * J);
-        m.take<uint32_t>(2 * J);
-        m.take<T>(J);
-        m.take<uint32_t>(16);
-        ensure(ctx->tree, m.used);
-        Carve c(ctx->tree);
-        d_jobs = c.take<vmx::Job<T>>(J);
-        d_rt = c.take<vmx::JobRt>(J);
-        d_f = c.take<P>(J);
-        t.lo = c.take<T>(size_t(cap_total) * D);
-        t.hi = c.take<T>(size_t(cap_total) * D);
-        t.v = c.take<T>(size_t(cap_total) * K);
-        t.err = c.take<T>(cap_total);
-        t.b = c.take<T>(cap_total);
-        t.b1 = c.take<T>(cap_total);
-        t.dim = c.take<int32_t>(cap_total);
-        t.parent = c.take<int32_t>(cap_total);
-        t.child = c.take<int32_t>(cap_total);
-        t.code = c.take<uint32_t>(size_t(cap_total) * D);
-        t.job = c.take<uint32_t>(cap_total);
-        t.popped = c.take<uint8_t>(cap_total);
-        d_hist16 = c.take<uint32_t>(size_t(J) * 65536);
-        d_cstart = c.take<uint32_t>(2 * J);
-        d_ncand = c.take<uint32_t>(2 * J);
-        d_theta = c.take<T>(J);
-        uint32_t* small = c.take<uint32_t>(16);  // [0] rounds, [8 + d] deepest split count along dim d
-        d_rounds = small;
-        HIP_CHECK(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(vmx::Job<T>) * J, hipMemcpyHostToDevice, st));
-        HIP_CHECK(hipMemcpyAsync(d_f, fparams.data(), sizeof(P) * J, hipMemcpyHostToDevice, st));
-        HIP_CHECK(hipMemsetAsync(d_rt, 0, sizeof(vmx::JobRt) * J, st));
-        HIP_CHECK(hipMemsetAsync(t.job, 0xFF, sizeof(uint32_t) * cap_total, st));
-        HIP_CHECK(hipMemsetAsync(d_hist16, 0, sizeof(uint32_t) * J * 65536, st));
-        HIP_CHECK(hipMemsetAsync(d_ncand, 0, sizeof(uint32_t) * 2 * J, st));
-        HIP_CHECK(hipMemsetAsync(small, 0, sizeof(uint32_t) * 16, st));
-        if (replay) {  // the given regions become the nodes, in list order
-            for (uint32_t j = 0; j < J; ++j) {
-                const GivenRegions& gr = *in[j].given;
-                const vmx::Job<T>& jb = jobs[j];
-                const size_t n = gr.n;
-                std::vector<T> lo(n * D), hi(n * D), v(n * K), err(n, T(0));
-                std::vector<uint32_t> job(n, j);
-                std::vector<int32_t> dim(n, 0);
-                for (size_t i = 0; i < n * D; ++i) {
-                    const int d = int(i % D);
-                    lo[i] = T(gr.lo[i]);
-                    hi[i] = T(gr.hi[i]);
-                    if (!(lo[i] < hi[i]) || lo[i] < jb.lo[d] || hi[i] > jb.hi[d])
-                        throw VmxError(VMX_ERR_INVALID, "given region outside the range or empty");
-                }
-                for (size_t i = 0; i < n * K; ++i) {
-                    v[i] = T(gr.nodes[i]);
-                    if (!std::isfinite(double(v[i]))) throw VmxError(VMX_ERR_INVALID, "non-finite node value");
-                }
-                const size_t b = jb.node_base;
-                HIP_CHECK(hipMemcpyAsync(t.lo + b * D, lo.data(), sizeof(T) * n * D, hipMemcpyHostToDevice, st));
-                HIP_CHECK(hipMemcpyAsync(t.hi + b * D, hi.data(), sizeof(T) * n * D, hipMemcpyHostToDevice, st));
-                HIP_CHECK(hipMemcpyAsync(t.v + b * K, v.data(), sizeof(T) * n * K, hipMemcpyHostToDevice, st));
-                HIP_CHECK(hipMemcpyAsync(t.err + b, err.data(), sizeof(T) * n, hipMemcpyHostToDevice, st));
-                HIP_CHECK(hipMemcpyAsync(t.dim + b, dim.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
-                HIP_CHECK(hipMemcpyAsync(t.job + b, job.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, st));
-                HIP_CHECK(hipStreamSynchronize(st));  // the host vectors go out of scope
-                rt[j] = vmx::JobRt{};
-                rt[j].count = uint32_t(b + n);
-            }
-            HIP_CHECK(hipMemcpyAsync(d_rt, rt.data(), sizeof(vmx::JobRt) * J, hipMemcpyHostToDevice, st));
-            HIP_CHECK(hipStreamSynchronize(st));
-            break;
-        }
-        vmx::k_roots<T, D, P, Math><<<J, 256, 0, st>>>(d_jobs, d_rt, d_f, t, d_hist16, d_cstart, d_ncand);
-        HIP_CHECK(hipGetLastError());
-        vmx::GenArgs<T> g{};
-        g.jobs = d_jobs;
-        g.rt = d_rt;
-        g.t = t;
-        g.hist16 = d_hist16;
-        g.cstart = d_cstart;
-        g.ncand = d_ncand;
-        g.theta = d_theta;
-        g.rounds = d_rounds;
-        g.depth = small + 8;
-        g.J = J;
-        g.net = learn ? ctx->nn_lds[D].p : nullptr;
-        g.deepen = shape.deepen;
-        // rounds in chunks of round_chunk (even, so each chunk starts at parity 0); after
-        // each chunk the candidate counts, job states and round counter are copied to
-        // pinned memory.  The host checks chunk k while chunk k+1 runs, so the device is
-        // never idle waiting for the host; the chunk after the last busy one finds no
-        // candidates and its kernels exit at once.
-        const size_t slot_bytes = ((sizeof(uint32_t) * 2 * J + sizeof(vmx::JobRt) * J + 64) + 255) & ~size_t(255);
-        char* pin = static_cast<char*>(ensure_pinned(ctx, 2 * slot_bytes));
-        auto slot = [&](int k) { return pin + size_t(k & 1) * slot_bytes; };
-        // the chunk's 2 * round_chunk kernels as a graph: one launch instead of many
-        // small ones (the late rounds are launch-bound); recaptured only when an argument
-        // changes.  VMX_NO_GRAPH=1 launches them one by one (e.g. under a kernel tracer
-        // that does not follow graph launches).
-        static const bool use_graph = !(std::getenv("VMX_NO_GRAPH") && std::atoi(std::getenv("VMX_NO_GRAPH")));
-        if (use_graph) {
-            std::vector<unsigned char> key(sizeof(g) + 64);
-            std::memcpy(key.data(), &g, sizeof(g));
-            const void* fn = reinterpret_cast<const void*>(&launch_expand<T, D, P, Math>);
-            const void* parts[4] = {fn, d_f, st, reinterpret_cast<const void*>(uintptr_t(learn))};
-            std::memcpy(key.data() + sizeof(g), parts, sizeof(parts));
-            const uint32_t shape[2] = {gen_blocks, uint32_t(round_chunk)};
-            std::memcpy(key.data() + sizeof(g) + sizeof(parts), shape, sizeof(shape));
-            if (ctx->rounds.key != key || !ctx->rounds.exec) {
-                if (ctx->rounds.exec) HIP_CHECK(hipGraphExecDestroy(ctx->rounds.exec));
-                if (ctx->rounds.graph) HIP_CHECK(hipGraphDestroy(ctx->rounds.graph));
-                ctx->rounds.exec = nullptr;
-                ctx->rounds.graph = nullptr;
-                ctx->rounds.key.clear();
-                HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
-                for (int r = 0; r < round_chunk; ++r) {
-                    vmx::k_theta_round<T><<<J, vmx::THETA_THREADS, 0, st>>>(g, r & 1);
-                    launch_expand<T, D, P, Math>(learn, g, d_f, r & 1, gen_blocks, st);
-                }
-                hipGraph_t graph = nullptr;
-                const hipError_t e1 = hipStreamEndCapture(st, &graph);
-                HIP_CHECK(e1);
-                HIP_CHECK(hipGetLastError());
-                HIP_CHECK(hipGraphInstantiate(&ctx->rounds.exec, graph, nullptr, nullptr, 0));
-                ctx->rounds.graph = graph;
-                ctx->rounds.key = key;
-            }
-        }
-        auto launch_chunk = [&](int k) {
-            if (use_graph) {
-                HIP_CHECK(hipGraphLaunch(ctx->rounds.exec, st));
-            } else {
-                for (int r = 0; r < round_chunk; ++r) {
-                    vmx::k_theta_round<T><<<J, vmx::THETA_THREADS, 0, st>>>(g, r & 1);
-                    launch_expand<T, D, P, Math>(learn, g, d_f, r & 1, gen_blocks, st);
-                }
-                HIP_CHECK(hipGetLastError());
-            }
-            char* sl = slot(k);
-            HIP_CHECK(hipMemcpyAsync(sl, d_ncand, sizeof(uint32_t) * 2 * J, hipMemcpyDeviceToHost, st));
-            HIP_CHECK(hipMemcpyAsync(sl + sizeof(uint32_t) * 2 * J, d_rt, sizeof(vmx::JobRt) * J,
-                                     hipMemcpyDeviceToHost, st));
-            HIP_CHECK(hipMemcpyAsync(sl + sizeof(uint32_t) * 2 * J + sizeof(vmx::JobRt) * J, small, 64,
-                                     hipMemcpyDeviceToHost, st));
-            HIP_CHECK(hipEventRecord(ctx->ev[2 + (k & 1)], st));
-        };
-        launch_chunk(0);
-        int k = 0;
-        for (;; ++k) {
-            if ((k + 2) * round_chunk > 4096) {
-                HIP_CHECK(hipStreamSynchronize(st));
-                throw VmxError(VMX_ERR_CAPACITY, "more than 4096 refinement rounds");
-            }
-            launch_chunk(k + 1);
-            HIP_CHECK(hipEventSynchronize(ctx->ev[2 + (k & 1)]));
-            const uint32_t* nc = reinterpret_cast<const uint32_t*>(slot(k));
-            bool pending = false;
-            for (uint32_t j = 0; j < J; ++j) pending |= nc[j] != 0;
-            if (!pending) break;
-        }
-        std::memcpy(rt.data(), slot(k) + sizeof(uint32_t) * 2 * J, sizeof(vmx::JobRt) * J);
-        uint32_t small_h[16];
-        std::memcpy(small_h, slot(k) + sizeof(uint32_t) * 2 * J + sizeof(vmx::JobRt) * J, 64);
-        rounds = small_h[0];
-        for (int d = 0; d < D; ++d) max_depth[d] = small_h[8 + d];
-        bool cap_fail = false, too_deep = false;
-        for (const auto& r : rt) {
-            too_deep |= (r.flags & vmx::JF_DEPTH) != 0;
-            cap_fail |= (r.flags & vmx::JF_CAPACITY) != 0;
-        }
-        if (too_deep || cap_fail) HIP_CHECK(hipStreamSynchronize(st));  // before buffers change hands
-        if (too_deep) throw VmxError(VMX_ERR_CAPACITY, "region tree deeper than 31 splits in one dim");
-        if (cap_fail) {
-            if (attempt >= 3) throw VmxError(VMX_ERR_CAPACITY, "node capacity exceeded");
-            cap_mult *= 2;
-            continue;
-        }
-        ctx->cap_hint[tune_key] = cap_mult;
-        break;
-    }
-    const double ms_gen = ms_since(t_start);
-
-    if constexpr (vmx::IsFubini<P>::value) {
-        if (pairs_req) throw VmxError(VMX_ERR_UNSUPPORTED, "training pairs: not for the Fubini form");
-    } else if (pairs_req) {
-        PairsReq* pairs = pairs_req;
-        // training pairs: every node of every job, job-major in slot order
-        std::vector<uint64_t> base(J);
-        uint64_t total = 0;
-        const uint32_t stride = std::max<uint32_t>(1, pairs->stride);
-        for (uint32_t j = 0; j < J; ++j) {
-            base[j] = total;
-            total += (rt[j].count - jobs[j].node_base + stride - 1) / stride;
-        }
-        pairs->count = total;
-        uint16_t* X = nullptr;
-        float* tg = nullptr;
-        HIP_CHECK(hipStreamSynchronize(st));  // the last (idle) round chunk is done
-        if (pairs->alloc) pairs->alloc(total, &X, &tg);
-        if (X && tg && total > 0) {
-            using F64 = typename F64Of<P>::type;
-            std::vector<F64> f64(J);
-            for (uint32_t j = 0; j < J; ++j)
-                f64[j] = F64::from(*static_cast<const typename ParamOf<P>::type*>(in[j].params));
-            ensure(ctx->misc, sizeof(uint64_t) * J + sizeof(F64) * J + 1024);
-            Carve c(ctx->misc);
-            uint64_t* d_base = c.take<uint64_t>(J);
-            F64* d_f64 = c.take<F64>(J);
-            HIP_CHECK(hipMemcpyAsync(d_base, base.data(), sizeof(uint64_t) * J, hipMemcpyHostToDevice, st));
-            HIP_CHECK(hipMemcpyAsync(d_f64, f64.data(), sizeof(F64) * J, hipMemcpyHostToDevice, st));
-            vmx::k_train_pairs<T, D, F64><<<(cap_total + 127) / 128, 128, 0, st>>>(d_jobs, t, cap_total, d_base, d_f64,
-                                                                                pairs->kind, stride, X, tg);
-            HIP_CHECK(hipGetLastError());
-            HIP_CHECK(hipStreamSynchronize(st));
-        }
-        return VMX_OK;
-    }
-
-    // ---------------- 2. selection and canonical leaf order
-    const auto t_sel = std::chrono::steady_clock::now();
-    // canonical order in one sort when (job, code_0, ..., code_{D-1}) fits a 64-bit key:
-    // code_d < 2^(max_depth_d + 1); LSD passes over 32-bit keys otherwise
-    vmx::Widths<D> widths{};
-    int job_bits = 0;
-    while (job_bits < 32 && (uint64_t(1) << job_bits) < J) ++job_bits;
-    int key_bits64 = job_bits;
-    for (int d = 0; d < D; ++d) {
-        widths.w[d] = int(max_depth[d]) + 1;
-        key_bits64 += widths.w[d];
-    }
-    const bool one_sort = key_bits64 <= 64;
-    size_t sort_bytes = 0, sort64_bytes = 0;
-    HIP_CHECK(hipError_t(rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                                  (int32_t*)nullptr, (int32_t*)nullptr, size_t(n_reg), 0, 32, st)));
-    HIP_CHECK(hipError_t(rocprim::radix_sort_pairs(nullptr, sort64_bytes, (uint64_t*)nullptr, (uint64_t*)nullptr,
-                                                  (int32_t*)nullptr, (int32_t*)nullptr, size_t(n_reg), 0,
-                                                  std::max(1, std::min(64, key_bits64)), st)));
-    sort_bytes = std::max(sort_bytes, sort64_bytes);
-    Measure ms;
-    ms.take<int32_t>(cap_total);   // bucket
-    ms.take<uint8_t>(cap_total);   // bstate
-    ms.take<int32_t>(n_reg);       // leaf slots
-    ms.take<int32_t>(n_reg);       // sort values tmp
-    ms.take<uint64_t>(n_reg);      // sort keys
-    ms.take<uint64_t>(n_reg);      // sort keys tmp
-    ms.take<char>(sort_bytes + 256);
-    ensure(ctx->regs, ms.used);
-    Carve cs(ctx->regs);
-    int32_t* d_bucket = cs.take<int32_t>(cap_total);
-    uint8_t* d_bstate = cs.take<uint8_t>(cap_total);
-    int32_t* d_leaf = cs.take<int32_t>(n_reg);
-    int32_t* d_leaf_tmp = cs.take<int32_t>(n_reg);
-    uint64_t* d_keys64 = cs.take<uint64_t>(n_reg);
-    uint64_t* d_keys64_tmp = cs.take<uint64_t>(n_reg);
-    uint32_t* d_keys = reinterpret_cast<uint32_t*>(d_keys64);
-    uint32_t* d_keys_tmp = reinterpret_cast<uint32_t*>(d_keys64_tmp);
-    void* d_sort_tmp = cs.take<char>(sort_bytes + 256);
-    const unsigned cap_blocks = (cap_total + 255) / 256;
-    const unsigned reg_blocks = unsigned((n_reg + 255) / 256);
-    if (replay) {
-        std::vector<int32_t> all_leaves(n_reg);
-        for (uint32_t j = 0; j < J; ++j)
-            for (uint32_t i = 0; i < jobs[j].N; ++i) all_leaves[jobs[j].reg_base + i] = int32_t(jobs[j].node_base + i);
-        HIP_CHECK(hipMemcpyAsync(d_leaf, all_leaves.data(), sizeof(int32_t) * n_reg, hipMemcpyHostToDevice, st));
-        HIP_CHECK(hipStreamSynchronize(st));
-    } else if (order == VMX_ORDER_CANONICAL) {
-        HIP_CHECK(hipMemsetAsync(d_leaf, 0xFF, sizeof(int32_t) * n_reg, st));
-        vmx::k_sel_bound<T><<<J, 256, 0, st>>>(d_jobs, d_rt, d_hist16);
-        vmx::k_sel_gather<T><<<cap_blocks, 256, 0, st>>>(d_jobs, d_rt, t, cap_total, d_bucket);
-        vmx::k_sel_resolve<T, D><<<J, 256, 0, st>>>(d_jobs, d_rt, t, d_bucket, d_bstate);
-        vmx::k_leaves<T><<<cap_blocks, 256, 0, st>>>(d_jobs, d_rt, t, cap_total, d_leaf);
-        HIP_CHECK(hipGetLastError());
-        int32_t* vin = d_leaf;
-        int32_t* vout = d_leaf_tmp;
-        if (one_sort) {
-            vmx::k_leaf_keys64<D><<<reg_blocks, 256, 0, st>>>(d_leaf, t.code, t.job, uint32_t(n_reg), widths, d_keys64,
-                                                              d_leaf_tmp);
-            size_t bytes = sort_bytes;
-            HIP_CHECK(hipError_t(rocprim::radix_sort_pairs(d_sort_tmp, bytes, d_keys64, d_keys64_tmp, d_leaf_tmp, d_leaf,
-                                                          size_t(n_reg), 0, std::max(1, key_bits64), st)));
-        } else {
-            // LSD passes: code_{D-1} ... code_0, then job
-            for (int which = D - 1; which >= (J > 1 ? -1 : 0); --which) {
-                vmx::k_leaf_keys<D><<<reg_blocks, 256, 0, st>>>(vin, t.code, t.job, uint32_t(n_reg), which, d_keys);
-                size_t bytes = sort_bytes;
-                HIP_CHECK(hipError_t(rocprim::radix_sort_pairs(d_sort_tmp, bytes, d_keys, d_keys_tmp, vin, vout,
-                                                              size_t(n_reg), 0, 32, st)));
-                std::swap(vin, vout);
-            }
-        }
-        if (vin != d_leaf)
-            HIP_CHECK(hipMemcpyAsync(d_leaf, vin, sizeof(int32_t) * n_reg, hipMemcpyDeviceToDevice, st));
-        // a job with fewer than N leaves leaves -1 slots behind: k_prepare2 turns them into
-        // empty regions and the host rejects the result at its next synchronisation
-    } else {
-        // the reference's heap-array order: replay push_heap / pop_heap on the errors
-        std::vector<int32_t> all_leaves(n_reg);
-        for (uint32_t j = 0; j < J; ++j) {
-            const auto& jb = jobs[j];
-            const uint32_t cnt = rt[j].count - jb.node_base;
-            std::vector<T> err(cnt);
-            std::vector<int32_t> child(cnt);
-            HIP_CHECK(hipMemcpyAsync(err.data(), t.err + jb.node_base, sizeof(T) * cnt, hipMemcpyDeviceToHost, st));
-            HIP_CHECK(hipMemcpyAsync(child.data(), t.child + jb.node_base, sizeof(int32_t) * cnt,
-                                     hipMemcpyDeviceToHost, st));
-            HIP_CHECK(hipStreamSynchronize(st));
-            struct E {
-                T err;
-                int32_t node;
-            };
-            auto cmp = [](const E& a, const E& b) { return a.err < b.err; };
-            std::vector<E> heap;
-            heap.reserve(jb.N + 1);
-            heap.push_back({err[0], 0});
-            for (uint32_t i = 0; i + 1 < jb.N; ++i) {
-                const E top = heap.front();
-                const int32_t ch = child[top.node];
-                if (ch < 0) throw VmxError(VMX_ERR_CAPACITY, "heap replay reached an unexpanded node");
-                const int32_t cl = ch - int32_t(jb.node_base);
-                std::pop_heap(heap.begin(), heap.end(), cmp);
-                heap.pop_back();
-                heap.push_back({err[cl], cl});
-                std::push_heap(heap.begin(), heap.end(), cmp);
-                heap.push_back({err[cl + 1], cl + 1});
-                std::push_heap(heap.begin(), heap.end(), cmp);
-            }
-            for (uint32_t i = 0; i < jb.N; ++i) all_leaves[jb.reg_base + i] = int32_t(jb.node_base) + heap[i].node;
-        }
-        HIP_CHECK(hipMemcpyAsync(d_leaf, all_leaves.data(), sizeof(int32_t) * n_reg, hipMemcpyHostToDevice, st));
-        HIP_CHECK(hipStreamSynchronize(st));
-    }
-    const double ms_sel = ms_since(t_sel);
-
-    // ---------------- 3. regions, footprints, bin lists
-    const auto t_str = std::chrono::steady_clock::now();
-    size_t scan_reg_bytes = 0;
-    HIP_CHECK(hipError_t(rocprim::exclusive_scan(nullptr, scan_reg_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr, 0u,
-                                                size_t(n_reg + 1), rocprim::plus<uint32_t>(), st)));
-
-    Measure mr;
-    mr.take<T>(n_reg * D);
-    mr.take<T>(n_reg * D);
-    mr.take<T>(n_reg);
-    mr.take<T>(n_reg * K);
-    bool any_importance = false, keep_nodes = false;  // node values per region: importance sampling, rr_pdf_region
-    for (const auto& jb : jobs) {
-        any_importance |= vmx::form_importance(jb.form);
-        keep_nodes |= vmx::form_roulette(jb.form) == 3;
-    }
-    keep_nodes |= any_importance;
-    if (keep_nodes) mr.take<T>(n_reg * K);
-    if (any_rr_err) mr.take<T>(n_reg);
-    mr.take<double>(n_reg * mb_max);
-    mr.take<uint32_t>(n_reg);
-    mr.take<uint32_t>(n_reg + 1);
-    mr.take<uint32_t>(n_reg + 1);
-    mr.take<T>(n_reg * 3);
-    mr.take<T>(n_reg * 3);
-    mr.take<double>(n_reg);
-    mr.take<double>(n_reg * mb_max);
-    for (int i = 0; i < 3; ++i) mr.take<uint32_t>(n_reg);
-    mr.take<uint32_t>(n_reg * 6);
-    mr.take<uint32_t>(n_reg + 1);
-    mr.take<uint32_t>(n_reg + 1);
-    mr.take<uint32_t>(total_bins + 1);
-    mr.take<char>(scan_reg_bytes + 256);
-    ensure(ctx->lists, mr.used);
-    Carve cr(ctx->lists);
-    vmx::RegionArrays<T> ra{};
-    ra.lo = cr.take<T>(n_reg * D);
-    ra.hi = cr.take<T>(n_reg * D);
-    ra.vol = cr.take<T>(n_reg);
-    ra.coef = cr.take<T>(n_reg * K);
-    ra.v = keep_nodes ? cr.take<T>(n_reg * K) : nullptr;
-    ra.err = any_rr_err ? cr.take<T>(n_reg) : nullptr;
-    ra.marg = cr.take<double>(n_reg * mb_max);
-    ra.job = cr.take<uint32_t>(n_reg);
-    ra.fpflag = cr.take<uint32_t>(n_reg + 1);
-    ra.fpid = cr.take<uint32_t>(n_reg + 1);
-    vmx::FootArrays<T> fa{};
-    fa.lo = cr.take<T>(n_reg * 3);
-    fa.hi = cr.take<T>(n_reg * 3);
-    fa.scale = cr.take<double>(n_reg);
-    fa.marg = cr.take<double>(n_reg * mb_max);
-    fa.first = cr.take<uint32_t>(n_reg);
-    fa.count = cr.take<uint32_t>(n_reg);
-    fa.job = cr.take<uint32_t>(n_reg);
-    fa.box = cr.take<uint32_t>(n_reg * 6);
-    fa.npairs = cr.take<uint32_t>(n_reg + 1);
-    uint32_t* d_pofs = cr.take<uint32_t>(n_reg + 1);
-    uint32_t* d_off = cr.take<uint32_t>(total_bins + 1);
-    void* d_scan_tmp = cr.take<char>(scan_reg_bytes + 256);
-    // mb differs per job only through db; all jobs of a batch share the footprint stride
-    for (const auto& jb : jobs)
-        if (jb.mb != mb_max) throw VmxError(VMX_ERR_INVALID, "one bin_dims per batch");
-    vmx::k_prepare2<T, D><<<reg_blocks, 256, 0, st>>>(d_jobs, d_leaf, t, ra, uint32_t(n_reg), mb_max, merge);
-    HIP_CHECK(hipGetLastError());
-    size_t bytes = scan_reg_bytes;
-    HIP_CHECK(hipError_t(rocprim::exclusive_scan(d_scan_tmp, bytes, ra.fpflag, ra.fpid, 0u, size_t(n_reg + 1),
-                                                rocprim::plus<uint32_t>(), st)));
-    HIP_CHECK(hipMemsetAsync(fa.npairs, 0, sizeof(uint32_t) * (n_reg + 1), st));
-    vmx::k_footprints<T, D><<<reg_blocks, 256, 0, st>>>(d_jobs, ra, fa, uint32_t(n_reg), mb_max, merge);
-    HIP_CHECK(hipGetLastError());
-    const uint32_t* d_nfp = ra.fpid + n_reg;
-    // per-bin footprint lists: pairs emitted per footprint at scanned offsets, then a
-    // stable radix sort by bin (each list ends up in ascending footprint order)
-    bytes = scan_reg_bytes;
-    HIP_CHECK(hipError_t(rocprim::exclusive_scan(d_scan_tmp, bytes, fa.npairs, d_pofs, 0u, size_t(n_reg + 1),
-                                                rocprim::plus<uint32_t>(), st)));
-    uint32_t pairs = 0, nfp = 0;
-    {
-        char* pin = static_cast<char*>(ensure_pinned(ctx, 16 + sizeof(vmx::JobRt) * J));
-        HIP_CHECK(hipMemcpyAsync(pin, d_pofs + n_reg, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-        HIP_CHECK(hipMemcpyAsync(pin + 4, d_nfp, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-        HIP_CHECK(hipMemcpyAsync(pin + 16, d_rt, sizeof(vmx::JobRt) * J, hipMemcpyDeviceToHost, st));
-        HIP_CHECK(hipStreamSynchronize(st));
-        std::memcpy(&pairs, pin, 4);
-        std::memcpy(&nfp, pin + 4, 4);
-        std::memcpy(rt.data(), pin + 16, sizeof(vmx::JobRt) * J);
-    }
-    if (order == VMX_ORDER_CANONICAL && !replay)
-        for (uint32_t j = 0; j < J; ++j)
-            if ((rt[j].flags & vmx::JF_LEAVES) || rt[j].nleaves != jobs[j].N)
-                throw VmxError(VMX_ERR_CAPACITY, "region selection did not close");
-    // per-bin footprint lists by counting sort: count, scan, place, then every list in
-    // ascending footprint id (region order)
-    size_t scan_bins_bytes = 0;
-    HIP_CHECK(hipError_t(rocprim::exclusive_scan(nullptr, scan_bins_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr, 0u,
-                                                size_t(total_bins + 1), rocprim::plus<uint32_t>(), st)));
-    Measure mp;
-    mp.take<uint32_t>(total_bins + 1);                        // counts
-    mp.take<uint32_t>(size_t(pairs) + 1);                     // lists
-    mp.take<uint32_t>(size_t(pairs) + 1);                     // copies of very long lists
-    mp.take<uint32_t>(size_t(pairs) / vmx::SORT_SHORT + 2);   // queue of long lists
-    mp.take<uint32_t>(size_t(pairs) + 1);                     // prefix counts of long lists
-    mp.take<uint32_t>(size_t(pairs) + 1);                     // first regions of long lists
-    if (any_rr) mp.take<double>(size_t(pairs) + 1);          // roulette weights per pair
-    mp.take<char>(scan_bins_bytes + 256);
-    ensure(ctx->bins2, mp.used);
-    Carve cp(ctx->bins2);
-    uint32_t* d_cnt = cp.take<uint32_t>(total_bins + 1);
-    uint32_t* d_lst = cp.take<uint32_t>(size_t(pairs) + 1);
-    uint32_t* d_lst_tmp = cp.take<uint32_t>(size_t(pairs) + 1);
-    uint32_t* d_qlong = cp.take<uint32_t>(size_t(pairs) / vmx::SORT_SHORT + 2);
-    vmx::ListPrefix lp{cp.take<uint32_t>(size_t(pairs) + 1), cp.take<uint32_t>(size_t(pairs) + 1)};
-    double* d_rrw = any_rr ? cp.take<double>(size_t(pairs) + 1) : nullptr;
-    void* d_scan_bins_tmp = cp.take<char>(scan_bins_bytes + 256);
-    HIP_CHECK(hipMemsetAsync(d_cnt, 0, sizeof(uint32_t) * (total_bins + 1), st));
-    HIP_CHECK(hipMemsetAsync(d_qlong, 0, sizeof(uint32_t), st));
-    const unsigned fp_blocks = unsigned((uint64_t(nfp) * 64 + 255) / 256);
-    if (pairs > 0)
-        vmx::k_bin_pass<T, false><<<fp_blocks, 256, 0, st>>>(d_jobs, fa, d_nfp, d_cnt, nullptr, nullptr);
-    bytes = scan_bins_bytes;
-    HIP_CHECK(hipError_t(rocprim::exclusive_scan(d_scan_bins_tmp, bytes, d_cnt, d_off, 0u, size_t(total_bins + 1),
-                                                rocprim::plus<uint32_t>(), st)));
-    if (pairs > 0) {
-        vmx::k_bin_pass<T, true><<<fp_blocks, 256, 0, st>>>(d_jobs, fa, d_nfp, d_cnt, d_off, d_lst);
-        vmx::k_sort_lists<T><<<unsigned((total_bins + 255) / 256), 256, 0, st>>>(d_off, total_bins, d_lst, d_qlong,
-                                                                                 fa, lp);
-        vmx::k_sort_long<<<1024, 256, 0, st>>>(d_off, d_qlong, d_lst, d_lst_tmp);
-        vmx::k_list_prefix<T><<<1024, 256, 0, st>>>(d_off, d_qlong, d_lst, fa, lp);
-    }
-    HIP_CHECK(hipGetLastError());
-    const double ms_str = ms_since(t_str);
-
-    // ---------------- 4. bins
-    // one lane per bin, one block per 256 consecutive bins
-    const unsigned bins_grid = unsigned((total_bins + 255) / 256);
-    HIP_CHECK(hipEventRecord(ctx->ev[0], st));
-    const int db = jobs[0].db;
-    // the importance-sampling variant is its own instantiation: its register footprint
-    // stays out of the uniform kernel
-    auto bins_kernel = [&](auto mode_tag) {
-        constexpr int MODE = decltype(mode_tag)::value;
-        if (db == 1)
-            vmx::k_bins1<T, D, 1, P, Math, MODE><<<bins_grid, 256, 0, st>>>(
-                d_jobs, J, total_bins, ra, fa, d_off, d_lst, d_f, d_rt, lp, shape.bins_xcd, d_rrw);
-        else if constexpr (D >= 2) {
-            if (db == 2)
-                vmx::k_bins1<T, D, 2, P, Math, MODE><<<bins_grid, 256, 0, st>>>(
-                    d_jobs, J, total_bins, ra, fa, d_off, d_lst, d_f, d_rt, lp, shape.bins_xcd, d_rrw);
-            else if constexpr (D >= 3)
-                vmx::k_bins1<T, D, 3, P, Math, MODE><<<bins_grid, 256, 0, st>>>(
-                    d_jobs, J, total_bins, ra, fa, d_off, d_lst, d_f, d_rt, lp, shape.bins_xcd, d_rrw);
-        }
-    };
-    if constexpr (vmx::IsFubini<P>::value) {
-        bins_kernel(std::integral_constant<int, 0>{});
-    } else {
-        bool done = false;
-        if constexpr (std::is_same<T, float>::value && D <= vmx::imp::MAX_DIM) {
-            if (any_importance) {
-                if (any_rr)
-                    bins_kernel(std::integral_constant<int, 3>{});
-                else
-                    bins_kernel(std::integral_constant<int, 1>{});
-                done = true;
-            }
-        }
-        if (!done) {
-            if (any_rr)
-                bins_kernel(std::integral_constant<int, 2>{});
-            else
-                bins_kernel(std::integral_constant<int, 0>{});
-        }
-    }
-    HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipEventRecord(ctx->ev[1], st));
-    {
-        char* pin = static_cast<char*>(ensure_pinned(ctx, sizeof(vmx::JobRt) * J));
-        HIP_CHECK(hipMemcpyAsync(pin, d_rt, sizeof(vmx::JobRt) * J, hipMemcpyDeviceToHost, st));
-        HIP_CHECK(hipStreamSynchronize(st));
-        std::memcpy(rt.data(), pin, sizeof(vmx::JobRt) * J);
-    }
-    float ms_k = 0.f;
-    HIP_CHECK(hipEventElapsedTime(&ms_k, ctx->ev[0], ctx->ev[1]));
-    for (uint32_t j = 0; j < J; ++j)
-        if (rt[j].flags & vmx::JF_EMPTY_BIN) throw VmxError(VMX_ERR_NUMERIC, "a bin overlaps no region while spp > 0");
-
-    // region export, gathered lazily from this call's device buffers
-    ctx->last_D = D;
-    ctx->last_n = n_reg;
-    ctx->last_gather = [ctx, t, ra, d_leaf, n_reg](double* olo, double* ohi, double* onodes, double* oerr,
-                                                   int32_t* odim) {
-        const size_t N = n_reg;
-        const size_t total = N * (2 * D + K + 1);
-        ensure(ctx->misc, sizeof(T) * total + sizeof(int32_t) * N + 4096);
-        Carve c(ctx->misc);
-        T* tmp = c.take<T>(total);
-        int32_t* tdim = c.take<int32_t>(N);
-        const unsigned g1 = unsigned((N * K + 255) / 256);
-        vmx::k_gather<T><<<g1, 256, 0, ctx->stream>>>(t.v, d_leaf, uint32_t(N), K, tmp);
-        vmx::k_gather<T><<<unsigned((N + 255) / 256), 256, 0, ctx->stream>>>(t.err, d_leaf, uint32_t(N), 1,
-                                                                          tmp + N * (2 * D + K));
-        vmx::k_gather<int32_t><<<unsigned((N + 255) / 256), 256, 0, ctx->stream>>>(t.dim, d_leaf, uint32_t(N), 1,
-                                                                                tdim);
-        HIP_CHECK(hipGetLastError());
-        HIP_CHECK(hipMemcpyAsync(tmp + N * K, ra.lo, sizeof(T) * N * D, hipMemcpyDeviceToDevice, ctx->stream));
-        HIP_CHECK(hipMemcpyAsync(tmp + N * K + N * D, ra.hi, sizeof(T) * N * D, hipMemcpyDeviceToDevice, ctx->stream));
-        std::vector<T> h(total);
-        std::vector<int32_t> hd(N);
-        HIP_CHECK(hipMemcpyAsync(h.data(), tmp, sizeof(T) * total, hipMemcpyDeviceToHost, ctx->stream));
-        HIP_CHECK(hipMemcpyAsync(hd.data(), tdim, sizeof(int32_t) * N, hipMemcpyDeviceToHost, ctx->stream));
-        HIP_CHECK(hipStreamSynchronize(ctx->stream));
-        for (size_t i = 0; i < N * K; ++i)
-            if (onodes) onodes[i] = double(h[i]);
-        for (size_t i = 0; i < N * D; ++i) {
-            if (olo) olo[i] = double(h[N * K + i]);
-            if (ohi) ohi[i] = double(h[N * K + N * D + i]);
-        }
-        for (size_t i = 0; i < N; ++i) {
-            if (oerr) oerr[i] = double(h[N * (2 * D + K) + i]);
-            if (odim) odim[i] = hd[i];
-        }
-    };
-    const double ms_total = ms_since(t_start);
-    for (uint32_t j = 0; j < J; ++j) {
-        vmx_stats* s = in[j].stats;
-        if (!s) continue;
-        const auto& jb = jobs[j];
-        s->regions = jb.N;
-        s->nodes = rt[j].count - jb.node_base;
-        s->rounds = rounds;
-        s->pairs = pairs;
-        s->footprints = nfp;
-        s->evals_regions = uint64_t(K) + (s->nodes - 1) * uint64_t(vmx::pow3(D - 1));
-        if constexpr (vmx::IsFubini<P>::value) s->evals_regions *= fparams[j].mc;  // mc samples per node value
-        s->evals_residual = jb.nbins * jb.spp;
-        s->ties_unresolved = rt[j].ties;
-        s->ms_generate = ms_gen;
-        s->ms_select = ms_sel;
-        s->ms_stratify = ms_str;
-        s->ms_bins = ms_k;
-        s->ms_total = ms_total;
-    }
-    return VMX_OK;
-}
+namespace vmxe {
 
 template <typename T, typename Math>
 int dispatch_T(vmx_ctx* ctx, int family, int D, const std::vector<JobIn>& jobs, hipStream_t st,
@@ -1485,7 +546,9 @@ int mc_dispatch(vmx_ctx* ctx, const vmx_problem& p, const vmx_integrand& f, int 
     throw VmxError(VMX_ERR_UNSUPPORTED, "integrand family / dimension not built");
 }
 
-}  // namespace
+}  // namespace vmxe
+
+using namespace vmxe;
 
 namespace {
 /* device copies of host network inputs, in ctx->misc */

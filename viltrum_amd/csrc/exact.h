@@ -71,7 +71,7 @@ struct ExactBlobs2D {
     double step[5], wave[3];
 };
 
-__global__ void k_exact_bins(const ExactBlobs2D* __restrict__ p, double lo0, double lo1, double dx, double dy,
+static __global__ void k_exact_bins(const ExactBlobs2D* __restrict__ p, double lo0, double lo1, double dx, double dy,
                              uint64_t res0, uint64_t res1, double* __restrict__ out) {
     const uint64_t b = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (b >= res0 * res1) return;

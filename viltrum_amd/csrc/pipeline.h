@@ -1119,6 +1119,20 @@ __global__ void k_footprints(const Job<T>* __restrict__ jobs, RegionArrays<T> r,
     }
 }
 
+/* *total += sum of x[0..n) in 64 bits.  The per-bin list offsets are 32-bit scans of
+   these counts, so the host checks this total before it sizes the lists. */
+static __global__ void __launch_bounds__(256) k_sum_u64(const uint32_t* __restrict__ x, uint32_t n,
+                                                 unsigned long long* __restrict__ total) {
+    uint64_t s = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += x[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    __shared__ uint64_t part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(total, (unsigned long long)(part[0] + part[1] + part[2] + part[3]));
+}
+
 /* q = c / n for the box walks below: a float estimate, then exact integer fix-up. */
 __device__ inline uint32_t div_fix(uint32_t c, uint32_t n, float inv) {
     uint32_t q = uint32_t(float(c) * inv);
@@ -1232,7 +1246,7 @@ __global__ void __launch_bounds__(256) k_sort_lists(const uint32_t* __restrict__
    lists from a copy in `tmp` read back with device-scope loads.  Blocks stride over the
    queue; the loop bounds are block-uniform. */
 constexpr uint32_t SORT_LDS = 8192;
-__global__ void __launch_bounds__(256) k_sort_long(const uint32_t* __restrict__ off, const uint32_t* __restrict__ qlong,
+static __global__ void __launch_bounds__(256) k_sort_long(const uint32_t* __restrict__ off, const uint32_t* __restrict__ qlong,
                                                    uint32_t* __restrict__ lst, uint32_t* __restrict__ tmp) {
     __shared__ uint32_t sb[SORT_LDS];
     const uint32_t nq = qlong[0];

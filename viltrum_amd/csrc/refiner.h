@@ -328,7 +328,7 @@ __global__ void __launch_bounds__(256) k_mlp_grad(const float* __restrict__ para
 /* 64-bit FNV-1a over the 32-bit words of row i (kp BF16 features, then the target's
    bits): a content key whose sort gives the pairs one order whatever order the device
    wrote them in. */
-__global__ void k_row_hash(const uint16_t* __restrict__ X, const float* __restrict__ tgt, uint64_t n, int kp,
+static __global__ void k_row_hash(const uint16_t* __restrict__ X, const float* __restrict__ tgt, uint64_t n, int kp,
                            uint64_t* __restrict__ key, uint32_t* __restrict__ idx) {
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -345,7 +345,7 @@ __global__ void k_row_hash(const uint16_t* __restrict__ X, const float* __restri
 }
 
 /* Sum of the per-block partial gradients in block order. */
-__global__ void k_sum_parts(const float* __restrict__ part, uint32_t nblk, uint32_t P, float* __restrict__ grad) {
+static __global__ void k_sum_parts(const float* __restrict__ part, uint32_t nblk, uint32_t P, float* __restrict__ grad) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P) return;
     float s = 0.f;
@@ -355,7 +355,7 @@ __global__ void k_sum_parts(const float* __restrict__ part, uint32_t nblk, uint3
 
 /* Adam (Kingma & Ba 2015) on the trained parameters; the gradient is the block-ordered
    sum of the partials.  c1, c2 are the bias corrections 1/(1-beta^t). */
-__global__ void k_adam(float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
+static __global__ void k_adam(float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
                        const float* __restrict__ part, uint32_t nblk, uint32_t P, float lr, float b1, float b2,
                        float eps, float c1, float c2) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;

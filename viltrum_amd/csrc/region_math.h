@@ -247,6 +247,11 @@ VMX_HD inline A marginal_subrange(const A* m, int db, const A* a, const A* b) {
     return marginal_subrange_n<A, 3>(m, a, b);
 }
 
+/* The per-bin footprint lists are indexed by 32-bit offsets (an exclusive scan of the
+   pair counts, one extra slot for the end): a batch's (bin, footprint) pair total must
+   not exceed this. */
+constexpr uint64_t MAX_PAIRS = 0xFFFFFFFEull;
+
 /* pixels_in_region: bin index box [start, end) along bin dim i. */
 template <typename T>
 VMX_HD inline void pixel_range(T rlo, T rhi, T lo, T hi, uint64_t res, uint64_t& start, uint64_t& end) {
