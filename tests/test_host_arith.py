@@ -211,9 +211,10 @@ def pair_total(g, res):
 
 
 def test_pair_total_guard_covers_the_t_only_case():
-    """The (bin, footprint) pair total of the t-only case passes what 32-bit list offsets
-    hold (so the engine must refuse it; tests/test_gpu_capacity.py checks that it does),
-    while the same regions at 1024^2 bins stay far below."""
+    """The (bin, region) pair total of the t-only case, one footprint per region as in
+    the reference's region order, passes what 32-bit list offsets hold (so the engine
+    must refuse it; tests/test_gpu_capacity.py checks that it does), while the same
+    regions at 1024^2 bins stay far below."""
     H.vmh_max_pairs.restype = C.c_uint64
     limit = H.vmh_max_pairs()
     assert limit == 2 ** 32 - 2
