@@ -162,6 +162,20 @@ int vmx_integrate(vmx_ctx* ctx, const vmx_problem* p, const vmx_integrand* f, vo
 int vmx_integrate_device(vmx_ctx* ctx, const vmx_problem* p, const vmx_integrand* f, void* bins_device,
                          void* stream, vmx_stats* stats);
 
+/* vmx_integrate_device that allocates nothing: no hipMalloc, hipFree, hipHostMalloc or
+   graph instantiation.  No counterpart in the reference; it is for callers that cannot
+   allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+   it).  Every work array of the call is carved from one fixed device array of
+   VMX_NOALLOC_SCRATCH_BYTES per GPU (resolved at vmx_ctx_create and held until the
+   process ends); a problem needing more returns VMX_ERR_CAPACITY.  Results are those of
+   vmx_integrate_device bit for bit.  All calls on one GPU share that array and it keeps
+   the last call's contents: calls that overlap on one GPU (other threads, other
+   contexts, other streams) race and must be serialised by the caller.  Afterwards
+   vmx_export_regions reports no regions. */
+#define VMX_NOALLOC_SCRATCH_BYTES (64u << 20)
+int vmx_integrate_device_noalloc(vmx_ctx* ctx, const vmx_problem* p, const vmx_integrand* f, void* bins_device,
+                                 void* stream, vmx_stats* stats);
+
 /* n independent problems in one launch sequence per (family, dim) group (every problem
    keeps its own range, resolution, budget, spp, seed, form and refiner).  All problems
    share the scalar type, region order and bin_dims.

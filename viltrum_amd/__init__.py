@@ -45,7 +45,8 @@ REFINER_HEURISTIC, REFINER_LEARNED = 0, 1
 SYMBOLS = ["vmx_ctx_create", "vmx_ctx_destroy", "vmx_last_error", "vmx_integrate", "vmx_integrate_device",
            "vmx_integrate_batch", "vmx_export_regions", "vmx_integrate_mc", "vmx_eval", "vmx_refiner_size",
            "vmx_refiner_pairs", "vmx_refiner_train", "vmx_refiner_get", "vmx_refiner_set", "vmx_refiner_forward",
-           "vmx_refiner_grad", "vmx_exact_bins", "vmx_autotune", "vmx_integrate_regions"]
+           "vmx_refiner_grad", "vmx_exact_bins", "vmx_autotune", "vmx_integrate_regions",
+           "vmx_integrate_device_noalloc"]
 
 
 class VmxError(RuntimeError):
@@ -115,6 +116,8 @@ def lib():
         L.vmx_integrate.argtypes = [C.c_void_p, C.POINTER(Problem), C.c_void_p, C.c_void_p, C.POINTER(Stats)]
         L.vmx_integrate_device.argtypes = [C.c_void_p, C.POINTER(Problem), C.c_void_p, C.c_void_p, C.c_void_p,
                                            C.POINTER(Stats)]
+        L.vmx_integrate_device_noalloc.argtypes = [C.c_void_p, C.POINTER(Problem), C.c_void_p, C.c_void_p,
+                                                   C.c_void_p, C.POINTER(Stats)]
         L.vmx_integrate_batch.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                           C.c_void_p]
         L.vmx_autotune.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -186,6 +189,15 @@ class Context:
         st = Stats()
         self._check(lib().vmx_integrate_device(self.ptr, C.byref(problem), C.cast(f.ptr(), C.c_void_p),
                                                C.c_void_p(bins_ptr), C.c_void_p(stream or 0), C.byref(st)))
+        return st
+
+    def integrate_device_noalloc(self, problem, f, bins_ptr, stream=None):
+        """integrate_device without any allocation: work memory comes from the library's
+        fixed per-GPU scratch array (VMX_NOALLOC_SCRATCH_BYTES); calls on one GPU must
+        not overlap."""
+        st = Stats()
+        self._check(lib().vmx_integrate_device_noalloc(self.ptr, C.byref(problem), C.cast(f.ptr(), C.c_void_p),
+                                                       C.c_void_p(bins_ptr), C.c_void_p(stream or 0), C.byref(st)))
         return st
 
     def integrate_batch(self, problems, fs, bins_ptrs, stream=None):

@@ -4,6 +4,8 @@
  * network's training and inference entry points.  The batch pipeline itself is
  * run_batch.h.
  */
+#include <mutex>
+
 #include "engine_core.h"
 #include "exact.h"
 
@@ -550,6 +552,62 @@ int mc_dispatch(vmx_ctx* ctx, const vmx_problem& p, const vmx_integrand& f, int 
 
 using namespace vmxe;
 
+/* Work memory of vmx_integrate_device_noalloc: every arena of its pipeline is carved
+   from this array (one per GPU, held until the process ends), so that entry point
+   calls no hipMalloc.  Host code never passes `scratch` itself to a kernel or a copy:
+   on the host the name is only a stand-in, and the array's device address on each GPU
+   comes from hipGetSymbolAddress (noalloc_scratch below). */
+static __device__ __align__(256) unsigned char scratch[VMX_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+/* The device address of `scratch` on `device` (the current device), resolved once per
+   device and kept. */
+hipError_t noalloc_scratch(int device, char** out) {
+    static std::mutex mu;
+    static std::map<int, char*> addr;
+    std::lock_guard<std::mutex> lock(mu);
+    const auto it = addr.find(device);
+    if (it != addr.end()) {
+        *out = it->second;
+        return hipSuccess;
+    }
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) return e != hipSuccess ? e : hipErrorInvalidSymbol;
+    addr[device] = static_cast<char*>(ptr);
+    *out = static_cast<char*>(ptr);
+    return hipSuccess;
+}
+
+/* For the duration of a no-alloc call: the pipeline's arenas become views into the
+   device's scratch array (bump-allocated from its start), and graphs and pinned growth
+   are off.  The context's own arenas are put back afterwards. */
+struct NoAllocScope {
+    vmx_ctx* ctx;
+    vmx_ctx::Fixed fixed;
+    vmx_ctx::Buf saved[5];
+    vmx_ctx::Buf* bufs[5];
+    explicit NoAllocScope(vmx_ctx* c) : ctx(c), bufs{&c->tree, &c->regs, &c->lists, &c->misc, &c->bins2} {
+        fixed.base = c->scratch;
+        fixed.cap = VMX_NOALLOC_SCRATCH_BYTES;
+        fixed.used = 0;
+        for (int i = 0; i < 5; ++i) {
+            saved[i] = *bufs[i];
+            *bufs[i] = vmx_ctx::Buf{};
+            bufs[i]->fixed = &fixed;
+        }
+        c->noalloc = true;
+    }
+    ~NoAllocScope() {
+        for (int i = 0; i < 5; ++i) *bufs[i] = saved[i];
+        ctx->noalloc = false;
+        // the call's regions live in the shared scratch: nothing to export afterwards
+        ctx->last_n = 0;
+        ctx->last_gather = nullptr;
+    }
+};
+}  // namespace
+
 namespace {
 /* device copies of host network inputs, in ctx->misc */
 struct NetIn {
@@ -622,9 +680,17 @@ int vmx_ctx_create(int device, vmx_ctx** out) {
     }
     for (hipEvent_t& e : c->ev)
         if (hipEventCreate(&e) != hipSuccess) {
-            delete c;
+            vmx_ctx_destroy(c);
             return VMX_ERR_DEVICE;
         }
+    // set-up for vmx_integrate_device_noalloc: the scratch array's address on this device
+    // and the pinned staging its readbacks use, so that entry point allocates nothing
+    if (noalloc_scratch(device, &c->scratch) != hipSuccess ||
+        hipHostMalloc(&c->pinned, 1 << 16, hipHostMallocDefault) != hipSuccess) {
+        vmx_ctx_destroy(c);
+        return VMX_ERR_DEVICE;
+    }
+    c->pinned_bytes = 1 << 16;
     *out = c;
     return VMX_OK;
 }
@@ -657,6 +723,25 @@ int vmx_integrate_device(vmx_ctx* ctx, const vmx_problem* p, const vmx_integrand
         if (!bins_device) throw VmxError(VMX_ERR_INVALID, "null bins");
         hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
         return integrate_groups(ctx, 1, &p, &f, &bins_device, &stats, st);
+    });
+}
+
+int vmx_integrate_device_noalloc(vmx_ctx* ctx, const vmx_problem* p, const vmx_integrand* f, void* bins_device,
+                                 void* stream, vmx_stats* stats) {
+    return guarded(ctx, [&]() -> int {
+        validate(p, f);
+        if (!bins_device) throw VmxError(VMX_ERR_INVALID, "null bins");
+        if (!ctx->scratch) throw VmxError(VMX_ERR_DEVICE, "no-alloc scratch not set up");
+        hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+        NoAllocScope scope(ctx);
+        try {
+            return integrate_groups(ctx, 1, &p, &f, &bins_device, &stats, st);
+        } catch (...) {
+            // work already queued may still read the scratch: let it finish before the
+            // array is handed to the next call
+            (void)hipStreamSynchronize(st);
+            throw;
+        }
     });
 }
 

@@ -38,10 +38,17 @@ struct vmx_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
+    // bump allocator over a fixed device array, for the entry point that allocates
+    // nothing: arenas that point at it are carved from it instead of hipMalloc
+    struct Fixed {
+        char* base = nullptr;
+        size_t cap = 0, used = 0;
+    };
     // device memory pool: one arena per purpose, grown on demand
     struct Buf {
         void* p = nullptr;
         size_t bytes = 0;
+        Fixed* fixed = nullptr;  // set: a view into a Fixed array, never hipMalloc'd or freed
     };
     Buf tree, regs, lists, misc, params, bins, bins2;
     Buf nn[VMX_MAX_DIM + 1];                       // FP32 priority-network parameters per dim
@@ -73,6 +80,10 @@ struct vmx_ctx {
     int last_D = 0;
     uint64_t last_n = 0;
     std::function<void(double*, double*, double*, double*, int32_t*)> last_gather;
+    // the no-alloc entry point: this device's scratch array (its device address, resolved
+    // at context creation) and whether the current call runs on it
+    char* scratch = nullptr;
+    bool noalloc = false;
 };
 
 
@@ -93,6 +104,19 @@ struct VmxError : std::runtime_error {
 
 inline void* ensure(vmx_ctx::Buf& b, size_t bytes) {
     if (bytes == 0) bytes = 256;
+    if (b.fixed) {  // carve from the fixed array: grow in place when b is its last carve
+        if (b.bytes >= bytes) return b.p;
+        vmx_ctx::Fixed& f = *b.fixed;
+        const bool last = b.p && static_cast<char*>(b.p) + b.bytes == f.base + f.used;
+        const size_t start = last ? size_t(static_cast<char*>(b.p) - f.base) : (f.used + 255) & ~size_t(255);
+        if (start + bytes > f.cap)
+            throw VmxError(VMX_ERR_CAPACITY, "problem needs more work memory than the no-alloc scratch holds (" +
+                                                 std::to_string(f.cap >> 20) + " MiB)");
+        b.p = f.base + start;
+        b.bytes = bytes;
+        f.used = start + bytes;
+        return b.p;
+    }
     if (b.bytes < bytes) {
         if (b.p) HIP_CHECK(hipFree(b.p));
         b.p = nullptr;
@@ -112,6 +136,8 @@ inline void check_pair_total(uint64_t pairs) {
 
 inline void* ensure_pinned(vmx_ctx* ctx, size_t bytes) {
     if (ctx->pinned_bytes < bytes) {
+        if (ctx->noalloc)
+            throw VmxError(VMX_ERR_CAPACITY, "no-alloc call needs more pinned staging than the context set up");
         if (ctx->pinned) HIP_CHECK(hipHostFree(ctx->pinned));
         ctx->pinned = nullptr;
         ctx->pinned_bytes = 0;

@@ -256,7 +256,9 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         // small ones (the late rounds are launch-bound); recaptured only when an argument
         // changes.  VMX_NO_GRAPH=1 launches them one by one (e.g. under a kernel tracer
         // that does not follow graph launches).
-        static const bool use_graph = !(std::getenv("VMX_NO_GRAPH") && std::atoi(std::getenv("VMX_NO_GRAPH")));
+        // (nor under the no-alloc entry point: instantiating a graph allocates)
+        static const bool graph_env = !(std::getenv("VMX_NO_GRAPH") && std::atoi(std::getenv("VMX_NO_GRAPH")));
+        const bool use_graph = graph_env && !ctx->noalloc;
         if (use_graph) {
             std::vector<unsigned char> key(sizeof(g) + 64);
             std::memcpy(key.data(), &g, sizeof(g));
