@@ -160,6 +160,19 @@ def run_config3(args, V, hip):
     print(json.dumps(res), flush=True)
 
 
+def trained(ctx, R, args, dim, train):
+    """The context's dim-`dim` priority network: trained on the GPU (`train()`), or loaded
+    from --load-net (a file an earlier --save-net run wrote: training is deterministic, so
+    it is the network the same command trains).  Returns the training report."""
+    if args.load_net:
+        ctx.refiner_set(dim, R.load(args.load_net))
+        return {"loaded": args.load_net}
+    rep = train()
+    if args.save_net:
+        R.save(args.save_net, ctx.refiner_get(dim))
+    return rep
+
+
 def run_config4(args, V, hip):
     """Learned refinement + batching against the plain GPU path at equal RMSE.
 
@@ -175,8 +188,8 @@ def run_config4(args, V, hip):
     from viltrum_amd import integrands as I
     from viltrum_amd import refiner as R
     ctx = V.Context(0)
-    rep = R.train(ctx, count=args.train_integrands, regions=args.train_regions, epochs=args.train_epochs, seed=0,
-                  stride=args.train_stride)
+    rep = trained(ctx, R, args, 2, lambda: R.train(ctx, count=args.train_integrands, regions=args.train_regions,
+                                                   epochs=args.train_epochs, seed=0, stride=args.train_stride))
     fs = R.held_out(args.integrals)
     res = (512, 512)
     nb = res[0] * res[1]
@@ -317,7 +330,7 @@ def run_config4(args, V, hip):
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": TRAFFIC_BYTES["config4"], "kernel_ms": ms_k},
         "training": {k: rep[k] for k in ("pairs", "steps", "loss_first", "loss_train", "loss_valid", "ms_pairs",
-                                         "ms_train")},
+                                         "ms_train", "loaded") if k in rep},
         "autotune": tune,
     }
     if not args.no_cpu_baseline:
@@ -480,8 +493,9 @@ def run_config5(args, V, hip):
     from viltrum_amd import integrands as I
     from viltrum_amd import refiner as R
     ctx = V.Context(0)
-    rep = R.train(ctx, count=args.c5_train_integrands, regions=args.c5_train_regions, res=(1024, 1024),
-                  epochs=args.train_epochs, seed=0, stride=1, dim=5)
+    rep = trained(ctx, R, args, 5, lambda: R.train(ctx, count=args.c5_train_integrands,
+                                                   regions=args.c5_train_regions, res=(1024, 1024),
+                                                   epochs=args.train_epochs, seed=0, stride=1, dim=5))
     n = args.integrals if args.integrals != 64 else 16
     fs = [I.config5(s) for s in range(n)]
     res = (1024, 1024)
@@ -638,6 +652,9 @@ def main():
                     help="config5: training budgets cycle through 1/4, 1/2, 1, 2 times this")
     ap.add_argument("--only-timed", default="", metavar="REGIONS,SPP",
                     help="config4, config5: run only the learned+batched step at this point (for kernel profiles)")
+    ap.add_argument("--save-net", default="", metavar="PATH", help="config4, config5: save the trained network")
+    ap.add_argument("--load-net", default="", metavar="PATH",
+                    help="config4, config5: load the network instead of training it (e.g. for counter passes)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device (a known-size kernel for HBM counter scaling)")
     args = ap.parse_args()

@@ -35,7 +35,9 @@ def test_fubini_statistics_match_oracle(ctx, f, rd, res, it, mc, spp):
         g.append(b.astype(np.float64))
         ob, _ = O.integrate_fubini(f, rd, res, it, mc, spp, 300 + s)
         o.append(ob)
-    _check(np.array(g), np.array(o), rel_floor=1e-5)
+    # small grids with few samples: the median variance ratio of the 3-D polynomial case
+    # (36 bins, 4 spp) ranged over 0.88..1.26 across four seed bases
+    _check(np.array(g), np.array(o), rel_floor=1e-5, spread=(0.75, 1.33))
 
 
 def test_fubini_is_unbiased_against_full_monte_carlo(ctx):

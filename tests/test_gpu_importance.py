@@ -40,7 +40,11 @@ def _runs(ctx, f, res, it, spp, form, alpha):
 def test_importance_matches_oracle(ctx, f, res, it, spp, form, alpha):
     g, o = _runs(ctx, f, res, it, spp, form, alpha)
     assert np.all(np.isfinite(g)) and np.all(np.isfinite(o))
-    _check(g, o, rel_floor=1e-6)
+    # the 1/pdf weights are heavy-tailed where |P| is small, so per-bin variance
+    # estimates over 48 seeds scatter widely: on the 3-D polynomial at 5x5 bins the
+    # median GPU/oracle variance ratio ranged over 0.67..1.45 across four seed bases,
+    # for this implementation and for an earlier draw layout alike
+    _check(g, o, rel_floor=1e-6, spread=(0.6, 1.67))
 
 
 def test_importance_is_unbiased_and_not_uniform(ctx):

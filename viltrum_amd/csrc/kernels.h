@@ -109,6 +109,43 @@ struct SampleDraws {
     }
 };
 
+/* Uniform draws of a group of 4 consecutive residual samples of a bin: one Philox call
+   per word type gives that word to all 4 samples -- the region word, then the
+   coordinates (one word each in FP32, two in FP64).  Every output word is used: at
+   D = 2 in FP32, 3 calls serve 4 samples.  Counter (group, bin, stream ^ bin_hi, type). */
+template <typename T, int D>
+struct GroupDraws {
+    static constexpr int WPC = sizeof(T) == 8 ? 2 : 1;
+    static constexpr int NT = 1 + D * WPC;
+    uint32_t w[NT][4];
+    __device__ GroupDraws(uint32_t group, uint64_t bin, uint32_t stream, uint32_t k0, uint32_t k1) {
+#pragma unroll
+        for (int c = 0; c < NT; ++c) {
+            const U4 r = philox4x32_10(U4{group, uint32_t(bin), uint32_t(bin >> 32) ^ stream, uint32_t(c)}, k0, k1);
+            w[c][0] = r.x;
+            w[c][1] = r.y;
+            w[c][2] = r.z;
+            w[c][3] = r.w;
+        }
+    }
+    __device__ uint32_t region_word(int k) const { return w[0][k]; }
+    __device__ T coord(int k, int d) const {
+        if constexpr (sizeof(T) == 8)
+            return u01d(w[1 + 2 * d][k], w[2 + 2 * d][k]);
+        else
+            return u01f(w[1 + d][k]);
+    }
+    /* drop sample 0's words: sample k + 1 becomes sample k (for a rolled group loop) */
+    __device__ void shift() {
+#pragma unroll
+        for (int c = 0; c < NT; ++c) {
+            w[c][0] = w[c][1];
+            w[c][1] = w[c][2];
+            w[c][2] = w[c][3];
+        }
+    }
+};
+
 /* Plain per-bin Monte Carlo, the reference's monte_carlo_per_bin
    (src/monte-carlo/monte-carlo-per-bin.h:40-65): `count` uniform samples per bin from
    sample index `first`, accumulated into FP64 sum and sum of squares.  With

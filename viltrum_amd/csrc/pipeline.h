@@ -44,6 +44,7 @@ struct Job {
     double alpha;
     uint32_t seed_lo, seed_hi, stream;
     uint32_t N, node_base, node_cap, reg_base;
+    uint32_t tile_base;  // first tile (of 256 bins) of this job in the per-bin kernel's grid
     int32_t db, mb, form, heur;
     int32_t learn;   // refinement priority from the network (refiner.h)
     int32_t lfloor;  // with learn: every dim is first halved lfloor times, widest first
@@ -51,7 +52,8 @@ struct Job {
 };
 
 struct JobRt {
-    uint32_t count, flags, kbound, above, nbucket, nleaves, ties, pad;
+    uint32_t count, flags, kbound, above, nbucket, nleaves, ties;
+    uint32_t row_floor;  // during the rounds: coarse row of the last boundary bucket
 };
 
 template <typename T>
@@ -92,19 +94,23 @@ __device__ inline void suffix_sum256(uint32_t* s) {
    than kept by atomics: nodes of a round cluster in few coarse buckets. */
 template <int NT>
 __device__ inline void boundary_bucket(const uint32_t* h16, uint32_t need, uint32_t* sh, uint32_t& k_out,
-                                       uint32_t& above_out) {
+                                       uint32_t& above_out, uint32_t row_floor = 0) {
     static_assert(NT >= 256 && NT % 256 == 0, "whole rows per thread group");
     constexpr int PER = NT / 256;
     const int t = threadIdx.x;
     __shared__ uint32_t part[NT];
     __shared__ uint32_t sel_c, sel_above;
     {
+        // rows below row_floor cannot hold the boundary (the caller knows it lies at or
+        // above that row): they count as empty and are not read
         const int row = t / PER, q = t % PER;
-        const uint4* p = reinterpret_cast<const uint4*>(h16 + size_t(row) * 256) + q * (64 / PER);
         uint32_t acc = 0;
-        for (int i = 0; i < 64 / PER; ++i) {
-            const uint4 v = p[i];
-            acc += v.x + v.y + v.z + v.w;
+        if (uint32_t(row) >= row_floor) {
+            const uint4* p = reinterpret_cast<const uint4*>(h16 + size_t(row) * 256) + q * (64 / PER);
+            for (int i = 0; i < 64 / PER; ++i) {
+                const uint4 v = p[i];
+                acc += v.x + v.y + v.z + v.w;
+            }
         }
         part[t] = acc;
     }
@@ -652,9 +658,12 @@ __global__ void __launch_bounds__(THETA_THREADS) k_theta_round(GenArgs<T> g, int
     } else if (total < need) {
         th = T(-INFINITY);
     } else {
+        // nodes are only added, so the need-th largest b -- and its bucket -- only rise
+        // from round to round: the rows below last round's boundary row are skipped
         uint32_t k, above;
-        boundary_bucket<THETA_THREADS>(g.hist16 + size_t(j) * 65536, need, sh, k, above);
+        boundary_bucket<THETA_THREADS>(g.hist16 + size_t(j) * 65536, need, sh, k, above, g.rt[j].row_floor);
         key16_floor(k, th);
+        if (threadIdx.x == 0) g.rt[j].row_floor = k >> 8;
     }
     if (threadIdx.x == 0) g.theta[j] = th;
 }
@@ -995,9 +1004,19 @@ __global__ void k_leaf_keys64(const int32_t* __restrict__ leaf_slot, const uint3
 }
 
 /* ------------------------------------------------------------------ regions */
+/* Per-region record the residual samples read (one gather per sample): box lo[D],
+   hi[D], the inverse extents 1/(hi - lo) (0 for an empty extent), then the 3^D monomial
+   coefficients; padded to a multiple of 4 values so a record starts 16-byte aligned
+   (64 bytes at D = 2 in FP32: one cache line). */
+template <int D>
+VMX_HD constexpr int samp_stride() {
+    return (3 * D + pow3(D) + 3) & ~3;
+}
+
 template <typename T>
 struct RegionArrays {
-    T *lo, *hi, *vol, *coef;
+    T *lo, *hi, *vol;
+    T* samp;  // sampling records, samp_stride<D>() values per region
     T* v;    // node values (3^D per region) for importance sampling, else null
     T* err;  // Region::error() per region for the rr_error_region roulette, else null
     double* marg;
@@ -1025,7 +1044,7 @@ __global__ void k_prepare2(const Job<T>* __restrict__ jobs, const int32_t* __res
         for (int d = 0; d < D; ++d) r.lo[size_t(i) * D + d] = r.hi[size_t(i) * D + d] = T(0);
         r.vol[i] = T(0);
         for (int k = 0; k < mb; ++k) r.marg[size_t(i) * mb + k] = 0.0;
-        for (int k = 0; k < K; ++k) r.coef[size_t(i) * K + k] = T(0);
+        for (int k = 0; k < samp_stride<D>(); ++k) r.samp[size_t(i) * samp_stride<D>() + k] = T(0);
         r.job[i] = 0;
         r.fpflag[i] = 1;
         if (i == 0) r.fpflag[n] = 0;
@@ -1060,7 +1079,14 @@ __global__ void k_prepare2(const Job<T>* __restrict__ jobs, const int32_t* __res
     for (int k = 0; k < mb; ++k) r.marg[size_t(i) * mb + k] = m[k];
     T c[K];
     monomials<T, T, D>(v, c);
-    for (int k = 0; k < K; ++k) r.coef[size_t(i) * K + k] = c[k];
+    T* rec = r.samp + size_t(i) * samp_stride<D>();
+    for (int d = 0; d < D; ++d) {
+        rec[d] = lo[d];
+        rec[D + d] = hi[d];
+        rec[2 * D + d] = hi[d] > lo[d] ? T(1) / (hi[d] - lo[d]) : T(0);
+    }
+    for (int k = 0; k < K; ++k) rec[3 * D + k] = c[k];
+    for (int k = 3 * D + K; k < samp_stride<D>(); ++k) rec[k] = T(0);
     r.job[i] = j;
     uint32_t flag = 1;
     if (merge && i != jb.reg_base && leaf_slot[i - 1] >= 0) {
@@ -1339,28 +1365,21 @@ __device__ inline uint32_t xcd_tile(uint32_t w, uint32_t G) {
 }
 
 
-/* A residual sample's position in its region.  FP64: pos_in, the reference's quotient.
-   FP32: x times the hardware reciprocal (about 1 ulp); only the polynomial's value at
-   the sample depends on it, while the control-variate integral keeps pos_in. */
-template <typename T>
-__device__ inline T sample_pos(T x, T lo, T hi) {
-    return pos_in(x, lo, hi);
-}
-template <>
-__device__ inline float sample_pos<float>(float x, float lo, float hi) {
-    if (lo >= hi) return lo;
-    return (x - lo) * __builtin_amdgcn_rcpf(hi - lo);
-}
+/* A residual sample's position t in its region: FP64, pos_in (the reference's
+   quotient); FP32, (x - lo) times the region's stored inverse extent (one multiply; only
+   the polynomial's value at the sample depends on it, while the control-variate
+   integral keeps pos_in). */
 
-/* One lane per bin (all jobs of the batch), one block per 256 consecutive bins.  The
-   lane sums the control variate over the
+/* One lane per bin (all jobs of the batch), one block per tile of 256 consecutive bins
+   of one job.  The lane sums the control variate over the
    bin's footprints in list order (FP64), then draws the spp residual samples -- a region
    uniform among the bin's regions (binary search over footprint counts cached in LDS),
    a point uniform in bin x region -- and forms the estimator of the job's form
    (…regions-integrator-parallel-variance-reduction.h:67-103,
    …regions-integrator-parallel-control-variates.h:39-65, weight-strategy.h:7-110).
-   Moments use the reference's shift by the bin's first sample; sums are
-   Kahan-compensated in the evaluation type and finished in FP64.  Summation depends on
+   Moments use the reference's shift by the bin's first sample; each group of 4 samples
+   is summed plainly and the group sums are Kahan-compensated in the evaluation type,
+   then finished in FP64.  Summation depends on
    the job alone, so a batched integral gives exactly what it gives alone. */
 /* the residual estimator's parts per form: region roulette (0 uniform, 1 rr_integral_region,
    2 rr_error_region, 3 rr_pdf_region), point sampler, fixed weight */
@@ -1398,25 +1417,32 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((MODE 
     // weights) can; 0: uniform region and uniform point only
     constexpr bool IMP = (MODE & 1) != 0;
     constexpr bool RRI = (MODE & 2) != 0;
+    constexpr bool FP32 = sizeof(T) == 4;
     constexpr int K = pow3(D);
     // per lane: first region and running region count of its first LANE_MAXF footprints
     __shared__ uint32_t s_first[LANE_MAXF][256];
     __shared__ uint32_t s_cum[LANE_MAXF + 1][256];
-    // the integrand's parameters in LDS when the block's bins belong to one job (the
-    // common case): every evaluation then reads them as LDS broadcasts
+    // the integrand's parameters in LDS: every evaluation reads them as LDS broadcasts
     __shared__ alignas(16) unsigned char f_raw[sizeof(F)];
-    __shared__ int block_job;
     const int tid = threadIdx.x;
-    const uint64_t base = uint64_t(xcd ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x) * blockDim.x;
-    if (tid == 0) {
-        const uint64_t last = (base + blockDim.x < total_bins ? base + blockDim.x : total_bins) - 1;
-        const uint32_t j0 = job_of_bin(jobs, J, base), j1 = job_of_bin(jobs, J, last);
-        block_job = j0 == j1 ? int(j0) : -1;
+    // the block's job: every job's bins start a fresh tile of blockDim.x bins (tile_base),
+    // so a block holds one job's bins and the job's fields are wave-uniform (scalar)
+    const uint32_t tile = xcd ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+    uint32_t j = 0;
+    {
+        uint32_t lo = 0, hi = J;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (jobs[mid].tile_base <= tile)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        j = __builtin_amdgcn_readfirstlane(lo);
     }
-    __syncthreads();
-    const int bj = block_job;
-    if (bj >= 0) {
-        const unsigned char* src = reinterpret_cast<const unsigned char*>(&fs[bj]);
+    const Job<T>& jb = jobs[j];
+    {
+        const unsigned char* src = reinterpret_cast<const unsigned char*>(&fs[j]);
         for (int i = tid; i < int(sizeof(F) / 4); i += blockDim.x)
             reinterpret_cast<uint32_t*>(f_raw)[i] = reinterpret_cast<const uint32_t*>(src)[i];
         if (tid == 0)
@@ -1425,19 +1451,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((MODE 
     __syncthreads();
     const F& f_lds = *reinterpret_cast<const F*>(f_raw);
     {
-        const uint64_t gb = base + tid;
-        if (gb >= total_bins) return;
-        const uint32_t j = job_of_bin(jobs, J, gb);
-        const Job<T>& jb = jobs[j];
+        const uint64_t b = uint64_t(tile - jb.tile_base) * blockDim.x + tid;  // bin within the job
+        if (b >= jb.nbins) return;
+        const uint64_t gb = jb.bin_base + b;
         const uint32_t o = off[gb], nf = off[gb + 1] - o;
-        const uint64_t b = gb - jb.bin_base;
         uint32_t p[3] = {0, 0, 0};
         {
-            uint64_t rem = b;
+            uint32_t rem = uint32_t(b);  // a job holds at most 2^31 bins
 #pragma unroll
             for (int i = DB - 1; i >= 0; --i) {
-                p[i] = uint32_t(rem % jb.res[i]);
-                rem /= jb.res[i];
+                const uint32_t ri = uint32_t(jb.res[i]);
+                p[i] = i > 0 ? rem % ri : rem;
+                rem /= ri;
             }
         }
         T blo[D], bhi[D];
@@ -1478,9 +1503,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((MODE 
                 aa[d] = double(pos_in(il, rl, rh));
                 bb[d] = double(pos_in(ih, rl, rh));
             }
+            // FP64 bins keep the reference's divisions by 3; the FP32 path multiplies by 1/3
             const double integ = empty ? 0.0
-                                       : fa.scale[fi] * marginal_subrange_n<double, DB>(fa.marg + size_t(fi) * jb.mb,
-                                                                                        aa, bb);
+                                       : fa.scale[fi] * marginal_subrange_n<double, DB, !FP32>(
+                                                            fa.marg + size_t(fi) * jb.mb, aa, bb);
             approx += empty ? 0.0 : factor * integ;
             if constexpr (RRI) {  // one region per footprint here
                 const int rk = form_roulette(jb.form);
@@ -1533,7 +1559,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((MODE 
             }
             return lp.pfirst[o + lo] + (idx - lp.pcum[o + lo]);
         };
-        constexpr bool FP32 = sizeof(T) == 4;
         const T wn = T(factor * double(n));  // FP32 sample weight factor B * n
         const bool rr_form = RRI && form_roulette(jb.form) != 0;
         double rr_total = 0.0;
@@ -1553,89 +1578,125 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((MODE 
         const bool imp_form = form_importance(jb.form);
         KahanSum<T> sf, sa, mf, ma, maa, mfa;
         double acc1 = 0.0;  // alpha = 1 form
+        const double a1w = spp > 0 ? factor * double(n) / double(spp) : 0.0;
         T k_f = 0, k_a = 0;
         const uint64_t ns_run = sampling ? spp : 0;
-        for (uint64_t s = 0; s < ns_run; ++s) {
-            const SampleDraws<T, D> u(uint32_t(s), b, jb.stream, jb.seed_lo, jb.seed_hi);
-            uint32_t pick = 0;
-            double rrfactor = double(n);
-            if (rr_form) {  // first cumulative weight above u * total
-                const double target = double(u.u[0]) * 0x1p-32 * rr_total;
-                uint32_t lo = 0, hi = n - 1;
-                while (lo < hi) {
-                    const uint32_t mid = (lo + hi) / 2;
-                    if (rrw[o + mid] > target)
-                        hi = mid;
-                    else
-                        lo = mid + 1;
-                }
-                pick = lo;
-                const double wk = rrw[o + pick] - (pick > 0 ? rrw[o + pick - 1] : 0.0);
-                rrfactor = rr_total / wk;
-            } else {
-                pick = uniform_index(u.u[0], n);
-            }
-            const uint32_t reg = region_of(pick);
-            T x[D], t[D], ilo[D], ihi[D];
-            bool empty = false;
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                const T rl = r.lo[size_t(reg) * D + d], rh = r.hi[size_t(reg) * D + d];
-                ilo[d] = vmax(blo[d], rl);
-                ihi[d] = vmax(ilo[d], vmin(bhi[d], rh));
-                if (ilo[d] >= ihi[d]) empty = true;
-                x[d] = mul_add<FP32>(u.coord(d), ihi[d] - ilo[d], ilo[d]);
-                t[d] = sample_pos(x[d], rl, rh);
-            }
-            T vol = box_volume<T, D>(ilo, ihi);
-            if constexpr (IMP && D <= imp::MAX_DIM && FP32) {
-                // region_sampling_importance: boxes of volume >= 1e-5 are sampled from the
-                // region polynomial's pdf with weight 1/pdf, smaller ones uniformly
-                if (imp_form && !empty && vol >= T(1.e-5)) {
-                    T rlo[D], rhi[D], xi[D], s01[D], wgt;
-#pragma unroll
-                    for (int d = 0; d < D; ++d) {
-                        rlo[d] = r.lo[size_t(reg) * D + d];
-                        rhi[d] = r.hi[size_t(reg) * D + d];
-                        s01[d] = u.coord(d);
+        constexpr int SR = samp_stride<D>();
+        // samples in groups of 4: one draw per word type serves the group, and the
+        // group's partial sums are compensated once
+        for (uint64_t s0 = 0; s0 < ns_run; s0 += 4) {
+            GroupDraws<T, D> u(uint32_t(s0 >> 2), b, jb.stream, jb.seed_lo, jb.seed_hi);
+            T gf = 0, ga = 0, gmf = 0, gma = 0, gmaa = 0, gmfa = 0;
+            // sample s0 + k of the bin, drawing from word column kw of the group
+            const uint64_t left = ns_run - s0;  // samples of this group that exist: min(left, 4)
+            auto sample = [&](int k, int kw) {
+                if (uint64_t(k) >= left) return;
+                uint32_t pick = 0;
+                double rrfactor = double(n);
+                const uint32_t uw = u.region_word(kw);
+                if (rr_form) {  // first cumulative weight above u * total
+                    const double target = double(uw) * 0x1p-32 * rr_total;
+                    uint32_t lo = 0, hi = n - 1;
+                    while (lo < hi) {
+                        const uint32_t mid = (lo + hi) / 2;
+                        if (rrw[o + mid] > target)
+                            hi = mid;
+                        else
+                            lo = mid + 1;
                     }
-                    if (imp::sample<T, D>(r.v + size_t(reg) * K, rlo, rhi, r.vol[reg], ilo, ihi, s01, xi, wgt)) {
+                    pick = lo;
+                    const double wk = rrw[o + pick] - (pick > 0 ? rrw[o + pick - 1] : 0.0);
+                    rrfactor = rr_total / wk;
+                } else {
+                    pick = uniform_index(uw, n);
+                }
+                const uint32_t reg = region_of(pick);
+                const T* rec = r.samp + size_t(reg) * SR;  // lo[D], hi[D], 1/extent[D], coefficients
+                T x[D], t[D], ilo[D], ihi[D];
+                bool empty = false;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    const T rl = rec[d], rh = rec[D + d];
+                    if constexpr (FP32) {
+                        ilo[d] = __builtin_fmaxf(blo[d], rl);
+                        ihi[d] = __builtin_fmaxf(ilo[d], __builtin_fminf(bhi[d], rh));
+                    } else {
+                        ilo[d] = vmax(blo[d], rl);
+                        ihi[d] = vmax(ilo[d], vmin(bhi[d], rh));
+                    }
+                    if (ilo[d] >= ihi[d]) empty = true;
+                    x[d] = mul_add<FP32>(u.coord(kw, d), ihi[d] - ilo[d], ilo[d]);
+                    if constexpr (FP32)
+                        t[d] = (x[d] - rl) * rec[2 * D + d];
+                    else
+                        t[d] = pos_in(x[d], rl, rh);
+                }
+                T vol = box_volume<T, D>(ilo, ihi);
+                if constexpr (IMP && D <= imp::MAX_DIM && FP32) {
+                    // region_sampling_importance: boxes of volume >= 1e-5 are sampled from
+                    // the region polynomial's pdf with weight 1/pdf, smaller ones uniformly
+                    if (imp_form && !empty && vol >= T(1.e-5)) {
+                        T rlo[D], rhi[D], xi[D], s01[D], wgt;
 #pragma unroll
                         for (int d = 0; d < D; ++d) {
-                            x[d] = xi[d];
-                            t[d] = sample_pos(x[d], rlo[d], rhi[d]);
+                            rlo[d] = rec[d];
+                            rhi[d] = rec[D + d];
+                            s01[d] = u.coord(kw, d);
                         }
-                        vol = wgt;
+                        if (imp::sample<T, D>(r.v + size_t(reg) * K, rlo, rhi, r.vol[reg], ilo, ihi, s01, xi, wgt)) {
+#pragma unroll
+                            for (int d = 0; d < D; ++d) {
+                                x[d] = xi[d];
+                                t[d] = (x[d] - rlo[d]) * rec[2 * D + d];
+                            }
+                            vol = wgt;
+                        }
                     }
                 }
-            }
-            const T fx = bj >= 0 ? residual_eval<M, D>(f_lds, x) : residual_eval<M, D>(fs[j], x);
-            const T px = eval_monomials<T, D, FP32>(r.coef + size_t(reg) * K, t);
-            if (jb.form == VMX_FORM_CV_ALPHA1) {
-                if (!empty) acc1 += double(fx - px) * (factor * double(n) * double(vol)) / double(spp);
-                continue;
-            }
-            T fsv, asv;
-            if constexpr (FP32) {  // weights in FP32: B * n * vol is O(n), far from the range limits
-                const T w = (rr_form ? T(factor * rrfactor) : wn) * vol;
-                fsv = fx * w;
-                asv = px * w;
+                const T fx = residual_eval<M, D>(f_lds, x);
+                const T px = eval_monomials<T, D, FP32>(rec + 3 * D, t);
+                if (jb.form == VMX_FORM_CV_ALPHA1) {
+                    if (!empty) acc1 += double(fx - px) * (a1w * double(vol));
+                    return;
+                }
+                T fsv, asv;
+                if constexpr (FP32) {  // weights in FP32: B * n * vol is O(n), far from the range limits
+                    const T w = (rr_form ? T(factor * rrfactor) : wn) * vol;
+                    fsv = fx * w;
+                    asv = px * w;
+                } else {
+                    const double w = factor * rrfactor * double(vol);
+                    fsv = T(double(fx) * w);
+                    asv = T(double(px) * w);
+                }
+                if (k == 0 && s0 == 0) {  // the bin's first sample shifts the moments
+                    k_f = vabs(fsv);
+                    k_a = vabs(asv);
+                }
+                gf += fsv;
+                ga += asv;
+                const T df = vabs(fsv) - k_f, da = vabs(asv) - k_a;
+                gmf += df;
+                gma += da;
+                gmaa += da * da;
+                gmfa += df * da;
+            };
+            if constexpr (IMP) {  // the importance sampler is large: one copy, words shifted in
+#pragma unroll 1
+                for (int k = 0; k < 4; ++k) {
+                    sample(k, 0);
+                    u.shift();
+                }
             } else {
-                const double w = factor * rrfactor * double(vol);
-                fsv = T(double(fx) * w);
-                asv = T(double(px) * w);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) sample(k, k);
             }
-            if (s == 0) {  // the bin's first sample shifts the moments
-                k_f = vabs(fsv);
-                k_a = vabs(asv);
-            }
-            sf.add(fsv);
-            sa.add(asv);
-            const T df = vabs(fsv) - k_f, da = vabs(asv) - k_a;
-            mf.add(df);
-            ma.add(da);
-            maa.add(da * da);
-            mfa.add(df * da);
+            sf.add(gf);
+            sa.add(ga);
+            mf.add(gmf);
+            ma.add(gma);
+            maa.add(gmaa);
+            mfa.add(gmfa);
         }
         if (jb.form == VMX_FORM_CV_ALPHA1) {
             bins[b] = T(approx + acc1);

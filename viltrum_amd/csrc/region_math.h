@@ -77,6 +77,17 @@ VMX_HD inline T simpson_subrange(T a, T b, T p0, T p1, T p2) {
     return T(((c2 * b / 3.0 + c1 / 2.0) * b + c0) * b - ((c2 * a / 3.0 + c1 / 2.0) * a + c0) * a);
 }
 
+/* simpson_subrange with the divisions by 3 replaced by products with 1/3 (within an ulp
+   of the quotient): for sums whose result is rounded to FP32 afterwards. */
+template <typename T>
+VMX_HD inline T simpson_subrange_mul(T a, T b, T p0, T p1, T p2) {
+    const T third = T(1) / T(3);
+    const T c0 = p0;
+    const T c1 = -3 * p0 + 4 * p1 - p2;
+    const T c2 = 2 * p0 - 4 * p1 + 2 * p2;
+    return ((c2 * b * third + c1 * T(0.5)) * b + c0) * b - ((c2 * a * third + c1 * T(0.5)) * a + c0) * a;
+}
+
 template <typename T>
 VMX_HD inline T metric_relative(T a, T b) {  // error_metric_relative(1e-37)
     const T ma = vmax(vabs(a), vabs(b));
@@ -227,7 +238,7 @@ VMX_HD inline void bin_marginal(const T* v, int db, A* out) {
 
 /* integral_subrange restricted to the bin dims on a marginal m (3^DB entries): folds
    dim DB-1 first, dim 0 last.  a/b are normalised bounds. */
-template <typename A, int DB>
+template <typename A, int DB, bool EXACT = true>
 VMX_HD inline A marginal_subrange_n(const A* m, const A* a, const A* b) {
     A w[pow3(DB)];
 #pragma unroll
@@ -236,7 +247,9 @@ VMX_HD inline A marginal_subrange_n(const A* m, const A* a, const A* b) {
     for (int d = DB - 1; d >= 0; --d) {
         const int n = pow3(d);
 #pragma unroll
-        for (int j = 0; j < n; ++j) w[j] = simpson_subrange(a[d], b[d], w[j], w[j + n], w[j + 2 * n]);
+        for (int j = 0; j < n; ++j)
+            w[j] = EXACT ? simpson_subrange(a[d], b[d], w[j], w[j + n], w[j + 2 * n])
+                         : simpson_subrange_mul(a[d], b[d], w[j], w[j + n], w[j + 2 * n]);
     }
     return w[0];
 }

@@ -49,7 +49,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
 
     std::vector<vmx::Job<T>> jobs(J);
     std::vector<P> fparams(J);
-    uint64_t total_bins = 0, n_reg = 0;
+    uint64_t total_bins = 0, n_reg = 0, total_tiles = 0;
     for (uint32_t j = 0; j < J; ++j) {
         const vmx_problem& p = *in[j].p;
         vmx::Job<T>& jb = jobs[j];
@@ -71,6 +71,8 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         }
         jb.bin_base = total_bins;
         total_bins += jb.nbins;
+        jb.tile_base = uint32_t(total_tiles);
+        total_tiles += (jb.nbins + 255) / 256;
         jb.spp = p.spp;
         jb.bins = in[j].bins;
         jb.alpha = p.alpha;
@@ -502,7 +504,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     mr.take<T>(n_reg * D);
     mr.take<T>(n_reg * D);
     mr.take<T>(n_reg);
-    mr.take<T>(n_reg * K);
+    mr.take<T>(n_reg * vmx::samp_stride<D>());
     bool any_importance = false, keep_nodes = false;  // node values per region: importance sampling, rr_pdf_region
     for (const auto& jb : jobs) {
         any_importance |= vmx::form_importance(jb.form);
@@ -532,7 +534,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     ra.lo = cr.take<T>(n_reg * D);
     ra.hi = cr.take<T>(n_reg * D);
     ra.vol = cr.take<T>(n_reg);
-    ra.coef = cr.take<T>(n_reg * K);
+    ra.samp = cr.take<T>(n_reg * vmx::samp_stride<D>());
     ra.v = keep_nodes ? cr.take<T>(n_reg * K) : nullptr;
     ra.err = any_rr_err ? cr.take<T>(n_reg) : nullptr;
     ra.marg = cr.take<double>(n_reg * mb_max);
@@ -636,8 +638,8 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     const double ms_str = ms_since(t_str);
 
     // ---------------- 4. bins
-    // one lane per bin, one block per 256 consecutive bins
-    const unsigned bins_grid = unsigned((total_bins + 255) / 256);
+    // one lane per bin, one block per tile of 256 consecutive bins of one job
+    const unsigned bins_grid = unsigned(total_tiles);  // every job starts a fresh tile
     HIP_CHECK(hipEventRecord(ctx->ev[0], st));
     const int db = jobs[0].db;
     // the importance-sampling variant is its own instantiation: its register footprint
