@@ -33,9 +33,14 @@ METRIC = "bins/sec at fixed RMSE vs FP64 oracle: learned+batched vs plain GPU pa
 PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 peak
 # HBM bytes per launch of k_bins1 from two rocprofv3 PMC passes (FETCH_SIZE x2.0,
 # WRITE_SIZE x1.0: the factors a 256 MiB device copy needs to read its known size), at
-# the profiled point of each workload; the commands and raw counts are in
-# profiles/k_bins1_config4_learned_batched.md and profiles/k_bins1_config2_plain.md
-TRAFFIC_BYTES = {"config4": (136249 * 2.0 + 65536) * 1024, "config2": (8274 * 2.0 + 3072) * 1024}
+# the profiled point of each workload; the commands and raw counts are in the profiles
+# named in TRAFFIC_SOURCE
+TRAFFIC_BYTES = {"config4": (131929 * 2.0 + 69097) * 1024, "config2": (8274 * 2.0 + 3072) * 1024}
+TRAFFIC_SOURCE = {
+    "config4": "PMC FETCH_SIZE/WRITE_SIZE of k_bins1 at this line's point (4096 regions, 16 spp) with the network "
+               "this bench trains (profiles/k_bins1_config4_learned_batched.md)",
+    "config2": "PMC FETCH_SIZE/WRITE_SIZE of k_bins1 at 16384 regions, 64 spp (profiles/k_bins1_config2_plain.md)",
+}
 PEAK_HBM_GBS = 8000.0
 
 
@@ -328,7 +333,8 @@ def run_config4(args, V, hip):
                                         ("ms_generate", "ms_select", "ms_stratify", "ms_bins", "ms_total")},
         "roofline": {"bound": "valu", "kernel": "k_bins1 (per-bin CV + residual Monte Carlo, batched)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": TRAFFIC_BYTES["config4"], "kernel_ms": ms_k},
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": TRAFFIC_BYTES["config4"],
+                     "traffic_source": TRAFFIC_SOURCE["config4"], "kernel_ms": ms_k},
         "training": {k: rep[k] for k in ("pairs", "steps", "loss_first", "loss_train", "loss_valid", "ms_pairs",
                                          "ms_train", "loaded") if k in rep},
         "autotune": tune,
@@ -743,7 +749,8 @@ def main():
                      ("ms_generate", "ms_select", "ms_stratify", "ms_bins", "ms_total")},
         "roofline": {"bound": "valu", "kernel": "k_bins1 (per-bin CV + residual Monte Carlo)", "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": TRAFFIC_BYTES.get(args.workload), "flops_per_eval": flops,
+                     "traffic": TRAFFIC_BYTES.get(args.workload),
+                     "traffic_source": TRAFFIC_SOURCE.get(args.workload), "flops_per_eval": flops,
                      "special_per_eval": special,
                      "kernel_ms": ms_k},
         "stats": {k: st[k] for k in ("regions", "nodes", "rounds", "pairs", "evals_regions", "evals_residual",
