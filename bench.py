@@ -400,8 +400,8 @@ def run_fubini(args, V, hip):
 
 def run_importance(args, V, hip):
     """Importance-sampled residual (region_sampling_importance) against uniform sampling,
-    and the region roulettes (rr_integral_region, rr_error_region, rr_pdf_region, and
-    rr_pdf_region with importance sampling): 64 held-out config-1 style integrands at
+    the region roulettes (rr_integral_region, rr_error_region, rr_pdf_region, and
+    rr_pdf_region with importance sampling) and region_sampling_russian_roulette: 64 held-out config-1 style integrands at
     256x256 FP32 bins (bin boxes above the sampler's 1e-5 volume threshold), 4096 regions,
     16 spp, one batched call per variant.  RMSE per bin against the FP64 closed-form
     bins.  value = bins/s with importance sampling."""
@@ -418,7 +418,8 @@ def run_importance(args, V, hip):
     out = {}
     variants = (("uniform", V.FORM_VR_OPTIMIZE), ("importance", V.FORM_VR_OPTIMIZE_IMPORTANCE),
                 ("rr_integral", V.FORM_VR_OPTIMIZE_RR_INTEGRAL), ("rr_error", V.FORM_VR_OPTIMIZE_RR_ERROR),
-                ("rr_pdf", V.FORM_VR_OPTIMIZE_RR_PDF), ("rr_pdf_importance", V.FORM_VR_OPTIMIZE_RR_PDF_IMPORTANCE))
+                ("rr_pdf", V.FORM_VR_OPTIMIZE_RR_PDF), ("rr_pdf_importance", V.FORM_VR_OPTIMIZE_RR_PDF_IMPORTANCE),
+                ("sampling_russian_roulette", V.FORM_VR_OPTIMIZE_SAMPLING_RR))
     sts = {}
     for name, form in variants:
         ps = [V.Integrator(4095, 16, 0, form=form).problem(f, res, V.range_primary(2), V.F32) for f in fs]
@@ -431,13 +432,25 @@ def run_importance(args, V, hip):
             st = ctx.integrate_batch(ps, fs, ptrs, stream.address)
         hip.device_synchronize()
         ms = (time.perf_counter() - t0) * 1e3 / args.steps
-        se2 = 0.0
-        h = np.zeros(res, dtype=np.float32)
-        for b, t in zip(bufs, truths):
-            b.to_host(h)
-            se2 += float(np.sum((h.astype(np.float64) - t) ** 2))
-        out[name] = {"ms": ms, "bins_per_s": nb * len(fs) / (ms * 1e-3), "rmse": float(np.sqrt(se2 / (nb * len(fs)))),
-                     "kernel_ms": float(st[0].ms_bins), "autotune": tune}
+
+        def rmse():
+            se2 = 0.0
+            h = np.zeros(res, dtype=np.float32)
+            for b, t in zip(bufs, truths):
+                b.to_host(h)
+                se2 += float(np.sum((h.astype(np.float64) - t) ** 2))
+            return float(np.sqrt(se2 / (nb * len(fs))))
+
+        # the 1/pdf weights make the importance samplers' error heavy-tailed (one bin can
+        # carry most of it), so the RMSE is taken over seeds 0..3 and the median reported
+        per_seed = [rmse()]
+        for seed in (1, 2, 3):
+            ctx.integrate_batch([V.Integrator(4095, 16, seed, form=form).problem(f, res, V.range_primary(2), V.F32)
+                                 for f in fs], fs, ptrs, stream.address)
+            hip.device_synchronize()
+            per_seed.append(rmse())
+        out[name] = {"ms": ms, "bins_per_s": nb * len(fs) / (ms * 1e-3), "rmse": float(np.median(per_seed)),
+                     "rmse_per_seed": per_seed, "kernel_ms": float(st[0].ms_bins), "autotune": tune}
         sts[name] = st
         print("variant %s: %.3f ms, rmse %.3e" % (name, ms, out[name]["rmse"]), file=sys.stderr, flush=True)
     imp = out["importance"]
@@ -455,8 +468,9 @@ def run_importance(args, V, hip):
         "rmse_vs_fp64": {k: v["rmse"] for k, v in out.items()},
         "fp64_truth": {"kind": "FP64 closed-form per-bin integrals (erf, polygon area, separable wave)"},
         "uniform": out["uniform"],
-        "variants": {k: {"bins_per_s": v["bins_per_s"], "ms": v["ms"], "kernel_ms": v["kernel_ms"], "rmse": v["rmse"]}
-                     for k, v in out.items()},
+        "variants": {k: {"bins_per_s": v["bins_per_s"], "ms": v["ms"], "kernel_ms": v["kernel_ms"], "rmse": v["rmse"],
+                         "rmse_per_seed": v["rmse_per_seed"]} for k, v in out.items()},
+        "rmse_note": "median over seeds 0..3 of the RMSE against the closed-form bins",
         "roofline": {"bound": "valu", "kernel": "k_bins1 (importance-sampling instantiation, batched)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel_ms": imp["kernel_ms"]},

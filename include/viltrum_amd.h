@@ -105,7 +105,15 @@ enum vmx_form {
     VMX_FORM_VR_OPTIMIZE_RR_ERROR_IMPORTANCE = 13,
     VMX_FORM_VR_FIXED_RR_ERROR_IMPORTANCE = 14,
     VMX_FORM_VR_OPTIMIZE_RR_PDF_IMPORTANCE = 15,
-    VMX_FORM_VR_FIXED_RR_PDF_IMPORTANCE = 16
+    VMX_FORM_VR_FIXED_RR_PDF_IMPORTANCE = 16,
+    /* region_sampling_russian_roulette in place of uniform sampling
+       (src/control-variates/region-sampling.h:47-84; uniform region roulette): each
+       residual sample chooses importance sampling with probability pdf_integral /
+       (pdf_integral + volume) of the bin's box, else uniform sampling, and is weighted
+       by 1/pdf of the strategy it took (0 where pdf < 1e-10), as the reference does.
+       FP32 integrands of dimension 1..3, as for importance sampling. */
+    VMX_FORM_VR_OPTIMIZE_SAMPLING_RR = 17,
+    VMX_FORM_VR_FIXED_SAMPLING_RR = 18
 };
 
 /* Order of the final region list (and so of every per-bin region list). */

@@ -15,6 +15,8 @@
  *                                                                       (integrator-adaptive-control-variates.h:17-20)
  *     integrator_variance_reduction_fixed(alpha, iterations, spp, seed) (regions-integrator-parallel-variance-reduction.h:127-130)
  *     integrator_variance_reduction_importance[_fixed](...) (region_sampling_importance, region-sampling.h:22-44)
+ *     integrator_variance_reduction_sampling_russian_roulette[_fixed](...)
+ *                                                (region_sampling_russian_roulette, region-sampling.h:47-84)
  *     integrator_crespo2021_infinite<DIM>(iterations, mc_samples, spp, seed)
  *                                                                       (integrator-crespo2021.h:24-44; float bins)
  *
@@ -354,6 +356,19 @@ inline IntegratorGpu integrator_variance_reduction_rr_pdf_importance_fixed(doubl
                                                                            std::size_t spp, std::size_t seed = 0,
                                                                            int heuristic = VMX_HEUR_SIZE_RELATIVE) {
     return IntegratorGpu(iterations, spp, seed, heuristic, VMX_FORM_VR_FIXED_RR_PDF_IMPORTANCE, alpha);
+}
+/* regions_integrator_parallel_variance_reduction(rr_uniform_region(), cv_optimize_weight(),
+   region_sampling_russian_roulette(), rng, spp) (src/control-variates/region-sampling.h:
+   47-84): importance or uniform sampling per sample by a roulette; float bins, dims 1..3 */
+inline IntegratorGpu integrator_variance_reduction_sampling_russian_roulette(std::size_t iterations, std::size_t spp,
+                                                                             std::size_t seed = 0,
+                                                                             int heuristic = VMX_HEUR_SIZE_RELATIVE) {
+    return IntegratorGpu(iterations, spp, seed, heuristic, VMX_FORM_VR_OPTIMIZE_SAMPLING_RR, 1.0);
+}
+inline IntegratorGpu integrator_variance_reduction_sampling_russian_roulette_fixed(
+    double alpha, std::size_t iterations, std::size_t spp, std::size_t seed = 0,
+    int heuristic = VMX_HEUR_SIZE_RELATIVE) {
+    return IntegratorGpu(iterations, spp, seed, heuristic, VMX_FORM_VR_FIXED_SAMPLING_RR, alpha);
 }
 /* ... with cv_fixed_weight(alpha) */
 inline IntegratorGpu integrator_variance_reduction_importance_fixed(double alpha, std::size_t iterations,

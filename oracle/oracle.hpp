@@ -81,7 +81,10 @@ enum Integrator {
     INT_VR_OPTIMIZE_RR_ERROR_IMPORTANCE = 13,
     INT_VR_FIXED_RR_ERROR_IMPORTANCE = 14,
     INT_VR_OPTIMIZE_RR_PDF_IMPORTANCE = 15,
-    INT_VR_FIXED_RR_PDF_IMPORTANCE = 16
+    INT_VR_FIXED_RR_PDF_IMPORTANCE = 16,
+    // region_sampling_russian_roulette (uniform region roulette; optimised / fixed weight)
+    INT_VR_OPTIMIZE_SAMPLING_RR = 17,
+    INT_VR_FIXED_SAMPLING_RR = 18
 };
 
 /* A region: box [lo, hi], its cached volume and 3^D node values, dim 0 fastest
@@ -781,6 +784,36 @@ inline T sample(const Region<T, D>& r, const T* slo, const T* shi, RNG& rng, T* 
     const T pdf = pdf_subrange<T, D>(r, pn, an, bn);
     return T((pdf < 1.e-5) ? 0.0 : (1.0 / pdf));
 }
+/* region_sampling_russian_roulette::sample (src/control-variates/region-sampling.h:
+   47-84) over the box [slo, shi] of region r: D uniforms, then one more that picks
+   importance sampling with probability pdf_integral / (pdf_integral + volume), else
+   uniform sampling; the weight is 1/pdf of the strategy taken (0 where pdf < 1e-10). */
+template <typename T, int D, typename RNG>
+inline T sample_rr(const Region<T, D>& r, const T* slo, const T* shi, RNG& rng, T* x) {
+    T s[D];
+    std::uniform_real_distribution<T> dis(0, 1);
+    for (int d = 0; d < D; ++d) s[d] = dis(rng);
+    const T pdf_importance = pdf_integral_box<T, D>(r, slo, shi);
+    const T pdf_uniform = box_volume<T, D>(slo, shi);
+    const T rr_sample = dis(rng);
+    T pdf = 0;
+    if (rr_sample < (pdf_importance / (pdf_importance + pdf_uniform))) {
+        T an[D], bn[D], un[D], pn[D];
+        auto in_range = [&](T p, int d) { return (r.lo[d] >= r.hi[d]) ? r.lo[d] : (p - r.lo[d]) / (r.hi[d] - r.lo[d]); };
+        for (int d = 0; d < D; ++d) {
+            an[d] = in_range(slo[d], d);
+            bn[d] = in_range(shi[d], d);
+        }
+        sample_subrange<T, D>(r, s, an, bn, un);
+        for (int d = 0; d < D; ++d) x[d] = un[d] * (r.hi[d] - r.lo[d]) + r.lo[d];
+        for (int d = 0; d < D; ++d) pn[d] = in_range(x[d], d);
+        pdf = pdf_subrange<T, D>(r, pn, an, bn);
+    } else {
+        for (int d = 0; d < D; ++d) x[d] = slo[d] + s[d] * (shi[d] - slo[d]);
+        pdf = T(1) / pdf_uniform;
+    }
+    return T((pdf < 1.e-10) ? 0.0 : (1.0 / pdf));
+}
 }  // namespace imp
 
 /* Per-bin control variate + residual, variance-reduction form
@@ -802,7 +835,10 @@ T region_error(const Region<T, D>& r) {
     return r.vol * (s[0] - t[0]);
 }
 
-template <typename T, int D, typename F, typename Acc, bool IMPORTANCE = false, int RR = 0>
+/* SAMPLER: 0 region_sampling_uniform, 1 region_sampling_importance, 2
+   region_sampling_russian_roulette; RR: 0 rr_uniform_region, 1 rr_integral_region,
+   2 rr_error_region, 3 rr_pdf_region. */
+template <typename T, int D, typename F, typename Acc, int SAMPLER = 0, int RR = 0>
 void integrate_regions_vr(const std::vector<Region<T, D>>& regs, const F& f, const T* lo, const T* hi, int db,
                           const uint64_t* res, uint64_t spp, uint64_t seed, double alpha, int nthreads, T* out,
                           Stats* st) {
@@ -913,8 +949,10 @@ void integrate_regions_vr(const std::vector<Region<T, D>>& regs, const F& f, con
                         const T* shi = &ihi[chosen * D];
                         T x[D];
                         T sfactor;
-                        if constexpr (IMPORTANCE) {
+                        if constexpr (SAMPLER == 1) {
                             sfactor = imp::sample<T, D>(r, slo, shi, rng, x);
+                        } else if constexpr (SAMPLER == 2) {
+                            sfactor = imp::sample_rr<T, D>(r, slo, shi, rng, x);
                         } else {
                             for (int d = 0; d < D; ++d) {
                                 std::uniform_real_distribution<T> dis(slo[d], shi[d]);

@@ -177,6 +177,12 @@ int run_regions(int integrator, double alpha, const F& fn, const std::vector<vmo
     else if (integrator == vmo::INT_VR_FIXED_RR_PDF)
         vmo::integrate_regions_vr<T, D, F, vmo::AccFixed<T>, false, 3>(regs, fn, tlo, thi, db, res, spp, seed,
                                                                          alpha, nthreads, out.data(), st);
+    else if (integrator == vmo::INT_VR_OPTIMIZE_SAMPLING_RR)
+        vmo::integrate_regions_vr<T, D, F, vmo::AccOptimize<T>, 2>(regs, fn, tlo, thi, db, res, spp, seed, alpha,
+                                                                     nthreads, out.data(), st);
+    else if (integrator == vmo::INT_VR_FIXED_SAMPLING_RR)
+        vmo::integrate_regions_vr<T, D, F, vmo::AccFixed<T>, 2>(regs, fn, tlo, thi, db, res, spp, seed, alpha,
+                                                                  nthreads, out.data(), st);
     else if (integrator >= vmo::INT_VR_OPTIMIZE_RR_INTEGRAL_IMPORTANCE &&
              integrator <= vmo::INT_VR_FIXED_RR_PDF_IMPORTANCE) {
         // codes 11..16: roulette (integral, error, pdf) x weight (optimised, fixed), importance sampled
@@ -428,6 +434,50 @@ int vmo_integrate_fubini(int rd, const vmx_integrand* f, const double* lo, const
     }
     copy_stats(s, st);
     return rc;
+}
+
+/* Rules::sample_line (src/newton-cotes/rules.h:216-237, float): a point of [t0, t1] from
+   the uniform s and the line's node values p[3]. */
+float vmo_imp_sample_line(float s, float t0, float t1, const float* p) {
+    return vmo::imp::sample_line<float>(s, t0, t1, std::array<float, 3>{p[0], p[1], p[2]});
+}
+
+/* region_sampling_importance's draw over the box [slo, shi] of the float region [rlo, rhi]
+   with node values v, from given uniforms s[D] (Region::sample_subrange and pdf_subrange,
+   src/newton-cotes/region.h:220-330): the point x[D] and its weight 1/pdf (0 where
+   pdf < 1e-5). */
+int vmo_imp_sample(int D, const float* v, const float* rlo, const float* rhi, const float* slo, const float* shi,
+                   const float* s, float* x, float* w) {
+    auto run = [&](auto dd) {
+        constexpr int DD = decltype(dd)::value;
+        vmo::Region<float, DD> r;
+        for (int d = 0; d < DD; ++d) {
+            r.lo[d] = rlo[d];
+            r.hi[d] = rhi[d];
+        }
+        r.vol = vmo::box_volume<float, DD>(r.lo, r.hi);
+        for (int k = 0; k < vmo::Region<float, DD>::K; ++k) r.v[k] = v[k];
+        float an[DD], bn[DD], un[DD], pn[DD];
+        auto in_range = [&](float p, int d) {
+            return (r.lo[d] >= r.hi[d]) ? r.lo[d] : (p - r.lo[d]) / (r.hi[d] - r.lo[d]);
+        };
+        for (int d = 0; d < DD; ++d) {
+            an[d] = in_range(slo[d], d);
+            bn[d] = in_range(shi[d], d);
+        }
+        vmo::imp::sample_subrange<float, DD>(r, s, an, bn, un);
+        for (int d = 0; d < DD; ++d) x[d] = un[d] * (r.hi[d] - r.lo[d]) + r.lo[d];
+        for (int d = 0; d < DD; ++d) pn[d] = in_range(x[d], d);
+        const float pdf = vmo::imp::pdf_subrange<float, DD>(r, pn, an, bn);
+        *w = float((pdf < 1.e-5) ? 0.0 : (1.0 / pdf));
+        return 1;
+    };
+    switch (D) {
+        case 1: return run(std::integral_constant<int, 1>{});
+        case 2: return run(std::integral_constant<int, 2>{});
+        case 3: return run(std::integral_constant<int, 3>{});
+        default: return -1;
+    }
 }
 
 int vmo_eval(int scalar_bytes, const vmx_integrand* f, uint64_t n, const double* x, double* out) {

@@ -208,6 +208,24 @@ int integrate_db(int integrator, int heuristic, double alpha, const vmx_integran
                 } else {
                     throw std::runtime_error("importance sampling: float regions only");
                 }
+            } else if (integrator == 17 || integrator == 18) {
+                // region_sampling_russian_roulette (float regions only, like the importance
+                // sampler whose sample_subrange / pdf_subrange it calls)
+                if constexpr (std::is_same<T, float>::value) {
+                    auto run = [&](auto cv) {
+                        auto in = viltrum::integrator_region_based(
+                            gen, viltrum::regions_integrator_parallel_variance_reduction(
+                                     viltrum::rr_uniform_region(), std::move(cv),
+                                     viltrum::region_sampling_russian_roulette(), std::mt19937(std::size_t(seed)), spp));
+                        viltrum::integrate(in, bins, r, fn, range);
+                    };
+                    if (integrator == 17)
+                        run(viltrum::cv_optimize_weight());
+                    else
+                        run(viltrum::cv_fixed_weight(alpha));
+                } else {
+                    throw std::runtime_error("region_sampling_russian_roulette: float regions only");
+                }
             } else if (integrator == 3 || integrator == 4) {
                 // region_sampling_importance compiles for float regions only: the
                 // reference's pop() exists for std::array<float, N> (src/array.h:52)

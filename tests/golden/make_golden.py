@@ -78,6 +78,12 @@ BIN_CASES = [  # (integrand, scalar, form, heuristic, res, iterations, spp, seed
     ("config1", 4, 15, 0, (16, 16), 255, 8, 5, 1.0),
     ("blobs3d", 4, 16, 0, (8, 8), 300, 4, 5, 0.5),
     ("config1", 4, 15, 0, (40,), 100, 5, 2, 1.0),
+    # region_sampling_russian_roulette (forms 17 and 18; float only)
+    ("config1", 4, 17, 0, (16, 16), 255, 8, 5, 1.0),
+    ("config1", 4, 18, 0, (8, 8), 63, 16, 3, 0.7),
+    ("poly3", 4, 17, 0, (4, 4), 200, 4, 2, 1.0),
+    ("blobs3d", 4, 18, 0, (8, 8), 300, 4, 5, 0.5),
+    ("config1", 4, 17, 0, (40,), 100, 5, 2, 1.0),
 ]
 
 FUBINI_CASES = [  # (integrand, region dims, res, iterations, mc samples, spp, seed): integrator_crespo2021_infinite

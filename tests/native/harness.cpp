@@ -10,6 +10,7 @@
 #include "../../viltrum_amd/csrc/philox.h"
 #include "../../viltrum_amd/csrc/region_math.h"
 #include "../../viltrum_amd/csrc/features.h"
+#include "../../viltrum_amd/csrc/importance.h"
 
 namespace {
 template <typename T, int D>
@@ -128,6 +129,24 @@ double vmh_monomial(int scalar, int D, const double* rlo, const double* rhi, con
     return 0;
 }
 uint64_t vmh_max_pairs(void) { return vmx::MAX_PAIRS; }
+
+/* the importance sampler (importance.h) on one line: a point of [t0, t1] from the
+   uniform s and the line's node values p[3] */
+float vmh_imp_sample_line(float s, float t0, float t1, const float* p) {
+    return vmx::imp::sample_line<float>(s, t0, t1, p);
+}
+
+/* ... on a region: point x[D] of the box [slo, shi] from the uniforms s[D], weight 1/pdf;
+   returns 0 where the pdf's integral over the box is not positive and finite */
+int vmh_imp_sample(int D, const float* v, const float* rlo, const float* rhi, float rvol, const float* slo,
+                   const float* shi, const float* s, float* x, float* w) {
+    switch (D) {
+        case 1: return vmx::imp::sample<float, 1>(v, rlo, rhi, rvol, slo, shi, s, x, *w) ? 1 : 0;
+        case 2: return vmx::imp::sample<float, 2>(v, rlo, rhi, rvol, slo, shi, s, x, *w) ? 1 : 0;
+        case 3: return vmx::imp::sample<float, 3>(v, rlo, rhi, rvol, slo, shi, s, x, *w) ? 1 : 0;
+        default: return -1;
+    }
+}
 
 void vmh_pixel_range(int scalar, double rlo, double rhi, double lo, double hi, uint64_t res, uint64_t* se) {
     if (scalar == 4)

@@ -20,6 +20,8 @@ INT_VR_OPTIMIZE_RR_PDF, INT_VR_FIXED_RR_PDF = 9, 10  # with the rr_pdf_region ro
 INT_VR_OPTIMIZE_RR_INTEGRAL_IMPORTANCE, INT_VR_FIXED_RR_INTEGRAL_IMPORTANCE = 11, 12
 INT_VR_OPTIMIZE_RR_ERROR_IMPORTANCE, INT_VR_FIXED_RR_ERROR_IMPORTANCE = 13, 14
 INT_VR_OPTIMIZE_RR_PDF_IMPORTANCE, INT_VR_FIXED_RR_PDF_IMPORTANCE = 15, 16
+# region_sampling_russian_roulette as the point sampler (uniform region roulette)
+INT_VR_OPTIMIZE_SAMPLING_RR, INT_VR_FIXED_SAMPLING_RR = 17, 18
 HEUR_SIZE_RELATIVE, HEUR_DEFAULT_ABSOLUTE = 0, 1
 
 _dp = C.POINTER(C.c_double)

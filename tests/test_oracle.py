@@ -87,10 +87,10 @@ def test_oracle_matches_live_reference(seed):
     res = (int(rng.integers(3, 20)), int(rng.integers(3, 20)))
     it, spp = int(rng.integers(0, 300)), int(rng.integers(0, 9))
     for scalar in (4, 8):
-        for form in range(17):
+        for form in range(19):
             heur = 1 if form == 2 else 0
             if (form in (3, 4) or form >= 11) and scalar == 8:
-                continue  # region_sampling_importance: the reference compiles it for float only
+                continue  # the importance and roulette samplers: the reference compiles them for float only
             a, _ = O.integrate(f, res, it, spp, seed, scalar=scalar, integrator=form, heuristic=heur, alpha=0.3)
             b, _ = O.integrate(f, res, it, spp, seed, scalar=scalar, integrator=form, heuristic=heur, alpha=0.3,
                                use_ref=True)

@@ -39,6 +39,8 @@ FORM_VR_OPTIMIZE_RR_PDF, FORM_VR_FIXED_RR_PDF = 9, 10  # with the rr_pdf_region 
 FORM_VR_OPTIMIZE_RR_INTEGRAL_IMPORTANCE, FORM_VR_FIXED_RR_INTEGRAL_IMPORTANCE = 11, 12
 FORM_VR_OPTIMIZE_RR_ERROR_IMPORTANCE, FORM_VR_FIXED_RR_ERROR_IMPORTANCE = 13, 14
 FORM_VR_OPTIMIZE_RR_PDF_IMPORTANCE, FORM_VR_FIXED_RR_PDF_IMPORTANCE = 15, 16
+# with region_sampling_russian_roulette (importance or uniform sampling by a roulette)
+FORM_VR_OPTIMIZE_SAMPLING_RR, FORM_VR_FIXED_SAMPLING_RR = 17, 18
 ORDER_CANONICAL, ORDER_REFERENCE = 0, 1
 REFINER_HEURISTIC, REFINER_LEARNED = 0, 1
 
@@ -458,6 +460,18 @@ def integrator_variance_reduction_importance(iterations, spp, seed=0, alpha=None
     if alpha is None:
         return Integrator(iterations, spp, seed, heuristic, FORM_VR_OPTIMIZE_IMPORTANCE, **kw)
     return Integrator(iterations, spp, seed, heuristic, FORM_VR_FIXED_IMPORTANCE, alpha=alpha, **kw)
+
+
+def integrator_variance_reduction_sampling_russian_roulette(iterations, spp, seed=0, alpha=None,
+                                                            heuristic=HEUR_SIZE_RELATIVE, **kw):
+    """regions_integrator_parallel_variance_reduction(rr_uniform_region(), cv_optimize_weight()
+    or cv_fixed_weight(alpha), region_sampling_russian_roulette(), ...)
+    (src/control-variates/region-sampling.h:47-84): each residual sample is drawn from the
+    region polynomial's pdf with probability pdf_integral / (pdf_integral + volume) of the
+    bin's box, else uniformly, and weighted by 1/pdf of the strategy taken.  FP32, dims 1..3."""
+    if alpha is None:
+        return Integrator(iterations, spp, seed, heuristic, FORM_VR_OPTIMIZE_SAMPLING_RR, **kw)
+    return Integrator(iterations, spp, seed, heuristic, FORM_VR_FIXED_SAMPLING_RR, alpha=alpha, **kw)
 
 
 def integrator_variance_reduction_rr_integral(iterations, spp, seed=0, alpha=None, heuristic=HEUR_SIZE_RELATIVE,

@@ -101,7 +101,7 @@ void validate(const vmx_problem* p, const vmx_integrand* f) {
     if (p->regions < 1 || p->regions > (1ull << 26)) throw VmxError(VMX_ERR_INVALID, "regions must be 1..2^26");
     if (p->heuristic != VMX_HEUR_SIZE_RELATIVE && p->heuristic != VMX_HEUR_DEFAULT_ABSOLUTE)
         throw VmxError(VMX_ERR_INVALID, "unknown heuristic");
-    if (p->form < VMX_FORM_VR_OPTIMIZE || p->form > VMX_FORM_VR_FIXED_RR_PDF_IMPORTANCE)
+    if (p->form < VMX_FORM_VR_OPTIMIZE || p->form > VMX_FORM_VR_FIXED_SAMPLING_RR)
         throw VmxError(VMX_ERR_INVALID, "unknown form");
     if (p->form >= VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL && f->family == VMX_FAMILY_FUBINI)
         throw VmxError(VMX_ERR_UNSUPPORTED, "region roulettes: not for the Fubini form");

@@ -166,11 +166,19 @@ VMX_HD inline T sample_line(T s, T t0, T t1, const T* p) {
 }
 
 /* A point of the box [slo, shi] inside region [rlo, rhi] (volume rvol, node values v,
-   dim 0 fastest) from the uniforms s[D], and its weight 1/pdf (0 where pdf < 1e-5).
-   Returns false where the pdf's integral over the box is not positive and finite. */
+   dim 0 fastest) from the uniforms s[D], and its weight.
+   - strategy_rr false (region_sampling_importance, src/control-variates/
+     region-sampling.h:22-44): drawn from the polynomial's pdf, weight 1/pdf (0 where
+     pdf < 1e-5).  Returns false where the pdf's integral over the box is not positive
+     and finite.
+   - strategy_rr true (region_sampling_russian_roulette, region-sampling.h:47-84): with
+     probability pdf_integral / (pdf_integral + svol) (svol the box volume, the uniform
+     rr in [0, 1) decides) drawn from the pdf, else uniformly (x = slo + s (shi - slo),
+     pdf = 1 / svol); weight 1/pdf of the strategy taken, 0 where pdf < 1e-10.  Returns
+     true. */
 template <typename T, int D>
 VMX_HD inline bool sample(const T* v, const T* rlo, const T* rhi, T rvol, const T* slo, const T* shi, const T* s,
-                          T* x, T& weight) {
+                          T* x, T& weight, bool strategy_rr = false, T rr = T(0), T svol = T(0)) {
     static_assert(D >= 1 && D <= MAX_DIM, "importance sampling: 1..3 dims");
     constexpr int K = pow3(D);
     T an[D], bn[D], un[D], pn[D];
@@ -181,6 +189,13 @@ VMX_HD inline bool sample(const T* v, const T* rlo, const T* rhi, T rvol, const 
     }
     // normalisation first: the pdf's integral over the box (folds last dim first)
     const T den = rvol * pdf_fold<T, D, 0>(v, 0, an, bn);
+    if (strategy_rr && !(rr < den / (den + svol))) {  // the roulette chose uniform sampling
+#pragma unroll
+        for (int d = 0; d < D; ++d) x[d] = slo[d] + s[d] * (shi[d] - slo[d]);
+        const T pdf = T(1) / svol;
+        weight = T((pdf < 1.e-10) ? 0.0 : (1.0 / pdf));
+        return true;
+    }
     if (!(den > T(0)) || !(den < T(INFINITY))) return false;
     // draw: dim k from the marginal, then condition on it
     T t[K];
@@ -232,7 +247,7 @@ VMX_HD inline bool sample(const T* v, const T* rlo, const T* rhi, T rvol, const 
             }
     }
     const T pdf = pdf_unnormalized(pn[0], t) / den;
-    weight = T((pdf < 1.e-5) ? 0.0 : (1.0 / pdf));
+    weight = T((pdf < (strategy_rr ? 1.e-10 : 1.e-5)) ? 0.0 : (1.0 / pdf));
     return true;
 }
 

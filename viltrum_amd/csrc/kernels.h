@@ -110,13 +110,16 @@ struct SampleDraws {
 };
 
 /* Uniform draws of a group of 4 consecutive residual samples of a bin: one Philox call
-   per word type gives that word to all 4 samples -- the region word, then the
-   coordinates (one word each in FP32, two in FP64).  Every output word is used: at
-   D = 2 in FP32, 3 calls serve 4 samples.  Counter (group, bin, stream ^ bin_hi, type). */
-template <typename T, int D>
+   per word type gives that word to all 4 samples -- the region word, the coordinates
+   (one word each in FP32, two in FP64), then XW extra words (the roulette between
+   sampling strategies).  The region and coordinate words do not depend on XW, so a job
+   draws the same samples in every kernel instantiation.  Every output word is used: at
+   D = 2 in FP32 with XW = 0, 3 calls serve 4 samples.  Counter (group, bin,
+   stream ^ bin_hi, type). */
+template <typename T, int D, int XW = 0>
 struct GroupDraws {
     static constexpr int WPC = sizeof(T) == 8 ? 2 : 1;
-    static constexpr int NT = 1 + D * WPC;
+    static constexpr int NT = 1 + XW + D * WPC;
     uint32_t w[NT][4];
     __device__ GroupDraws(uint32_t group, uint64_t bin, uint32_t stream, uint32_t k0, uint32_t k1) {
 #pragma unroll
@@ -129,6 +132,7 @@ struct GroupDraws {
         }
     }
     __device__ uint32_t region_word(int k) const { return w[0][k]; }
+    __device__ uint32_t extra_word(int k, int e) const { return w[1 + D * WPC + e][k]; }
     __device__ T coord(int k, int d) const {
         if constexpr (sizeof(T) == 8)
             return u01d(w[1 + 2 * d][k], w[2 + 2 * d][k]);

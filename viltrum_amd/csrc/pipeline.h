@@ -1395,15 +1395,21 @@ VMX_HD constexpr int form_roulette(int f) {
                ? 3
                : 0;
 }
+/* the point sampler draws from the region polynomial's pdf (always, or by the roulette
+   between strategies of region_sampling_russian_roulette) */
+VMX_HD constexpr bool form_sampling_rr(int f) {
+    return f == VMX_FORM_VR_OPTIMIZE_SAMPLING_RR || f == VMX_FORM_VR_FIXED_SAMPLING_RR;
+}
 VMX_HD constexpr bool form_importance(int f) {
     return f == VMX_FORM_VR_OPTIMIZE_IMPORTANCE || f == VMX_FORM_VR_FIXED_IMPORTANCE ||
-           (f >= VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL_IMPORTANCE && f <= VMX_FORM_VR_FIXED_RR_PDF_IMPORTANCE);
+           (f >= VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL_IMPORTANCE && f <= VMX_FORM_VR_FIXED_RR_PDF_IMPORTANCE) ||
+           form_sampling_rr(f);
 }
 VMX_HD constexpr bool form_fixed(int f) {
     return f == VMX_FORM_VR_FIXED || f == VMX_FORM_VR_FIXED_IMPORTANCE || f == VMX_FORM_VR_FIXED_RR_INTEGRAL ||
            f == VMX_FORM_VR_FIXED_RR_ERROR || f == VMX_FORM_VR_FIXED_RR_PDF ||
            f == VMX_FORM_VR_FIXED_RR_INTEGRAL_IMPORTANCE || f == VMX_FORM_VR_FIXED_RR_ERROR_IMPORTANCE ||
-           f == VMX_FORM_VR_FIXED_RR_PDF_IMPORTANCE;
+           f == VMX_FORM_VR_FIXED_RR_PDF_IMPORTANCE || f == VMX_FORM_VR_FIXED_SAMPLING_RR;
 }
 
 template <typename T, int D, int DB, typename F, typename M, int MODE = 0>
@@ -1585,7 +1591,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((MODE 
         // samples in groups of 4: one draw per word type serves the group, and the
         // group's partial sums are compensated once
         for (uint64_t s0 = 0; s0 < ns_run; s0 += 4) {
-            GroupDraws<T, D> u(uint32_t(s0 >> 2), b, jb.stream, jb.seed_lo, jb.seed_hi);
+            GroupDraws<T, D, IMP ? 1 : 0> u(uint32_t(s0 >> 2), b, jb.stream, jb.seed_lo, jb.seed_hi);
             T gf = 0, ga = 0, gmf = 0, gma = 0, gmaa = 0, gmfa = 0;
             // sample s0 + k of the bin, drawing from word column kw of the group
             const uint64_t left = ns_run - s0;  // samples of this group that exist: min(left, 4)
@@ -1634,8 +1640,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((MODE 
                 T vol = box_volume<T, D>(ilo, ihi);
                 if constexpr (IMP && D <= imp::MAX_DIM && FP32) {
                     // region_sampling_importance: boxes of volume >= 1e-5 are sampled from
-                    // the region polynomial's pdf with weight 1/pdf, smaller ones uniformly
-                    if (imp_form && !empty && vol >= T(1.e-5)) {
+                    // the region polynomial's pdf with weight 1/pdf, smaller ones uniformly;
+                    // region_sampling_russian_roulette: every box, importance or uniform by
+                    // a roulette on the pdf's integral against the volume
+                    const bool srr = form_sampling_rr(jb.form);
+                    if (imp_form && !empty && (srr || vol >= T(1.e-5))) {
                         T rlo[D], rhi[D], xi[D], s01[D], wgt;
 #pragma unroll
                         for (int d = 0; d < D; ++d) {
@@ -1643,7 +1652,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((MODE 
                             rhi[d] = rec[D + d];
                             s01[d] = u.coord(kw, d);
                         }
-                        if (imp::sample<T, D>(r.v + size_t(reg) * K, rlo, rhi, r.vol[reg], ilo, ihi, s01, xi, wgt)) {
+                        if (imp::sample<T, D>(r.v + size_t(reg) * K, rlo, rhi, r.vol[reg], ilo, ihi, s01, xi, wgt,
+                                              srr, u01f(u.extra_word(kw, 0)), vol)) {
 #pragma unroll
                             for (int d = 0; d < D; ++d) {
                                 x[d] = xi[d];
