@@ -476,6 +476,8 @@ int vmo_imp_sample(int D, const float* v, const float* rlo, const float* rhi, co
         case 1: return run(std::integral_constant<int, 1>{});
         case 2: return run(std::integral_constant<int, 2>{});
         case 3: return run(std::integral_constant<int, 3>{});
+        case 4: return run(std::integral_constant<int, 4>{});
+        case 5: return run(std::integral_constant<int, 5>{});
         default: return -1;
     }
 }

@@ -84,6 +84,10 @@ BIN_CASES = [  # (integrand, scalar, form, heuristic, res, iterations, spp, seed
     ("poly3", 4, 17, 0, (4, 4), 200, 4, 2, 1.0),
     ("blobs3d", 4, 18, 0, (8, 8), 300, 4, 5, 0.5),
     ("config1", 4, 17, 0, (40,), 100, 5, 2, 1.0),
+    # importance sampling and the sampling roulette at D = 5
+    ("lobes5", 4, 3, 0, (3, 3), 60, 4, 2, 1.0),
+    ("lobes5", 4, 18, 0, (3, 3), 60, 4, 2, 0.7),
+    ("lobes5", 4, 15, 0, (3, 3), 60, 4, 2, 1.0),
 ]
 
 FUBINI_CASES = [  # (integrand, region dims, res, iterations, mc samples, spp, seed): integrator_crespo2021_infinite

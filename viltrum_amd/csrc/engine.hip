@@ -107,7 +107,7 @@ void validate(const vmx_problem* p, const vmx_integrand* f) {
         throw VmxError(VMX_ERR_UNSUPPORTED, "region roulettes: not for the Fubini form");
     if (vmx::form_importance(p->form) &&
         (p->scalar != VMX_F32 || f->dim > vmx::imp::MAX_DIM || f->family == VMX_FAMILY_FUBINI))
-        throw VmxError(VMX_ERR_UNSUPPORTED, "importance sampling: FP32 integrands of dimension 1..3");
+        throw VmxError(VMX_ERR_UNSUPPORTED, "importance sampling: FP32 integrands of dimension 1..5");
     if (p->order != VMX_ORDER_CANONICAL && p->order != VMX_ORDER_REFERENCE)
         throw VmxError(VMX_ERR_INVALID, "unknown region order");
     if (p->refiner != VMX_REFINER_HEURISTIC && p->refiner != VMX_REFINER_LEARNED)

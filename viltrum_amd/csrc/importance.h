@@ -29,7 +29,7 @@
 namespace vmx {
 namespace imp {
 
-constexpr int MAX_DIM = 3;  // tensors of 3^D values live in registers
+constexpr int MAX_DIM = 5;  // dims 1..3 keep the 3^D tensors in registers, 4 and 5 fold depth first
 
 template <typename T>
 VMX_HD inline T at3(T t, const T* p) {
@@ -166,7 +166,8 @@ VMX_HD inline T sample_line(T s, T t0, T t1, const T* p) {
 }
 
 /* A point of the box [slo, shi] inside region [rlo, rhi] (volume rvol, node values v,
-   dim 0 fastest) from the uniforms s[D], and its weight.
+   dim 0 fastest) from the uniforms s[D], and its weight (dims 1..3: the tensors in
+   registers).
    - strategy_rr false (region_sampling_importance, src/control-variates/
      region-sampling.h:22-44): drawn from the polynomial's pdf, weight 1/pdf (0 where
      pdf < 1e-5).  Returns false where the pdf's integral over the box is not positive
@@ -177,9 +178,9 @@ VMX_HD inline T sample_line(T s, T t0, T t1, const T* p) {
      pdf = 1 / svol); weight 1/pdf of the strategy taken, 0 where pdf < 1e-10.  Returns
      true. */
 template <typename T, int D>
-VMX_HD inline bool sample(const T* v, const T* rlo, const T* rhi, T rvol, const T* slo, const T* shi, const T* s,
-                          T* x, T& weight, bool strategy_rr = false, T rr = T(0), T svol = T(0)) {
-    static_assert(D >= 1 && D <= MAX_DIM, "importance sampling: 1..3 dims");
+VMX_HD inline bool sample_regs(const T* v, const T* rlo, const T* rhi, T rvol, const T* slo, const T* shi, const T* s,
+                               T* x, T& weight, bool strategy_rr, T rr, T svol) {
+    static_assert(D >= 1 && D <= 3, "register tensors: 1..3 dims");
     constexpr int K = pow3(D);
     T an[D], bn[D], un[D], pn[D];
 #pragma unroll
@@ -249,6 +250,105 @@ VMX_HD inline bool sample(const T* v, const T* rlo, const T* rhi, T rvol, const 
     const T pdf = pdf_unnormalized(pn[0], t) / den;
     weight = T((pdf < (strategy_rr ? 1.e-10 : 1.e-5)) ? 0.0 : (1.0 / pdf));
     return true;
+}
+
+/* ---- dimensions 4 and 5: the same draw, depth first --------------------------------
+   The register-tensor form above keeps 3^D values live; from D = 4 on, the marginal of
+   dim k and the pdf at the point are evaluated as nested folds straight from the node
+   values (3 live values per fold level), in the reference's nesting: the drawn dims
+   0..k-1 conditioned first (dim 0 innermost, pdf_unnormalized at the drawn coordinate),
+   then the later dims folded by pdf_integral over the box, the last dim innermost. */
+
+/* conditioned chain over dims L..0 of the block at `base`: pdf_unnormalized(sol[L], .) */
+template <typename T, int L>
+VMX_HD inline T cond_fold(const T* v, int base, const T* sol) {
+    if constexpr (L < 0) {
+        return v[base];
+    } else {
+        T w[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) w[j] = cond_fold<T, L - 1>(v, base + j * pow3(L), sol);
+        return pdf_unnormalized(sol[L], w);
+    }
+}
+/* marginal folds over dims L..D-1 (dim K's index fixed in base), leaves the chain over
+   dims K-1..0 */
+template <typename T, int D, int K, int L>
+VMX_HD inline T marg_fold(const T* v, int base, const T* sol, const T* an, const T* bn) {
+    if constexpr (L == D) {
+        return cond_fold<T, K - 1>(v, base, sol);
+    } else {
+        T w[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) w[j] = marg_fold<T, D, K, L + 1>(v, base + j * pow3(L), sol, an, bn);
+        return pdf_integral(an[L], bn[L], w);
+    }
+}
+/* the unnormalised pdf at normalised coordinates pn: folds last dim first */
+template <typename T, int D, int L>
+VMX_HD inline T point_fold(const T* v, int base, const T* pn) {
+    T w[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        if constexpr (L == D - 1)
+            w[j] = v[base + j * pow3(L)];
+        else
+            w[j] = point_fold<T, D, L + 1>(v, base + j * pow3(L), pn);
+    }
+    return pdf_unnormalized(pn[L], w);
+}
+/* draw dims K..D-1 in turn from their marginals given the ones drawn before */
+template <typename T, int D, int K>
+VMX_HD inline void draw_dims(const T* v, const T* an, const T* bn, const T* s, T* sol) {
+    if constexpr (K < D) {
+        T m[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) m[i] = marg_fold<T, D, K, K + 1>(v, i * pow3(K), sol, an, bn);
+        sol[K] = sample_line(s[K], an[K], bn[K], m);
+        draw_dims<T, D, K + 1>(v, an, bn, s, sol);
+    }
+}
+
+/* sample() for dimensions 4 and 5: the same arguments and results */
+template <typename T, int D>
+__attribute__((noinline)) VMX_HD bool sample_deep(const T* v, const T* rlo, const T* rhi, T rvol, const T* slo,
+                                                  const T* shi, const T* s, T* x, T& weight, bool strategy_rr, T rr,
+                                                  T svol) {
+    T an[D], bn[D], un[D], pn[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        an[d] = (rlo[d] >= rhi[d]) ? rlo[d] : (slo[d] - rlo[d]) / (rhi[d] - rlo[d]);
+        bn[d] = (rlo[d] >= rhi[d]) ? rlo[d] : (shi[d] - rlo[d]) / (rhi[d] - rlo[d]);
+    }
+    const T den = rvol * pdf_fold<T, D, 0>(v, 0, an, bn);
+    if (strategy_rr && !(rr < den / (den + svol))) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) x[d] = slo[d] + s[d] * (shi[d] - slo[d]);
+        const T pdf = T(1) / svol;
+        weight = T((pdf < 1.e-10) ? 0.0 : (1.0 / pdf));
+        return true;
+    }
+    if (!(den > T(0)) || !(den < T(INFINITY))) return false;
+    draw_dims<T, D, 0>(v, an, bn, s, un);
+#pragma unroll
+    for (int d = 0; d < D; ++d) x[d] = un[d] * (rhi[d] - rlo[d]) + rlo[d];
+#pragma unroll
+    for (int d = 0; d < D; ++d) pn[d] = (rlo[d] >= rhi[d]) ? rlo[d] : (x[d] - rlo[d]) / (rhi[d] - rlo[d]);
+    const T pdf = point_fold<T, D, 0>(v, 0, pn) / den;
+    weight = T((pdf < (strategy_rr ? 1.e-10 : 1.e-5)) ? 0.0 : (1.0 / pdf));
+    return true;
+}
+
+/* A point of the box and its weight (sample_regs above for dims 1..3, sample_deep for 4
+   and 5); arguments and results as described at sample_regs. */
+template <typename T, int D>
+VMX_HD inline bool sample(const T* v, const T* rlo, const T* rhi, T rvol, const T* slo, const T* shi, const T* s, T* x,
+                          T& weight, bool strategy_rr = false, T rr = T(0), T svol = T(0)) {
+    static_assert(D >= 1 && D <= MAX_DIM, "importance sampling: 1..5 dims");
+    if constexpr (D <= 3)
+        return sample_regs<T, D>(v, rlo, rhi, rvol, slo, shi, s, x, weight, strategy_rr, rr, svol);
+    else
+        return sample_deep<T, D>(v, rlo, rhi, rvol, slo, shi, s, x, weight, strategy_rr, rr, svol);
 }
 
 }  // namespace imp
