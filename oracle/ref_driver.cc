@@ -336,8 +336,9 @@ int vmr_generate(int scalar_bytes, const vmx_integrand* f, const double* lo, con
             VMR_G(3)
             VMR_G(4)
             VMR_G(5)
+            VMR_G(6)  // the reference's region cap (VILTRUM_MAX_DIMENSIONS_REGION)
             default:
-                g_err = "D must be 1..5";
+                g_err = "D must be 1..6";
                 return -1;
         }
 #undef VMR_G

@@ -28,6 +28,8 @@ def integrand_cases():
         "poly2": I.poly([(1.0, (3, 1)), (-2.0, (1, 4)), (0.5, (0, 0))], 2, step=([1.0, 1.0], 1.0, 0.5)),
         "poly3": I.poly([(1.0, (2, 1, 3)), (0.25, (0, 5, 1))], 3, step=([1.0, -0.5, 0.7], 0.3, 1.0)),
         "lobes5": I.config5(2),
+        "poly6": I.poly([(1.0, (1, 2, 0, 1, 0, 2)), (0.5, (0, 0, 3, 0, 1, 0)), (0.25, (2, 0, 0, 0, 0, 1))], 6,
+                        step=([1, 1, 1, 1, 1, 1], 3.0, 0.5)),
     }
 
 
@@ -36,6 +38,7 @@ REGION_CASES = [  # (integrand, scalar, heuristic, iterations)
     ("config1", 8, 0, 1023), ("config1", 4, 0, 1023), ("config1", 8, 1, 63), ("config1", 4, 1, 255),
     ("blobs3d", 8, 0, 300), ("blobs3d", 4, 0, 300), ("c1fam", 8, 0, 400), ("poly2", 8, 0, 200),
     ("poly2", 8, 1, 200), ("poly3", 8, 0, 150), ("lobes5", 8, 0, 40), ("lobes5", 4, 0, 40),
+    ("poly6", 8, 0, 60), ("poly6", 4, 0, 40), ("poly6", 8, 1, 30),
 ]
 
 BIN_CASES = [  # (integrand, scalar, form, heuristic, res, iterations, spp, seed, alpha)

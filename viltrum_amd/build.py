@@ -14,7 +14,7 @@ LIB = os.path.join(HERE, "libviltrum_amd.so")
 CSRC = os.path.join(HERE, "csrc")
 SOURCES = [os.path.join(CSRC, s) for s in (
     "engine.hip", "batch_poly_f32_a.hip", "batch_poly_f32_b.hip", "batch_poly_f64_a.hip", "batch_poly_f64_b.hip",
-    "batch_blobs.hip", "batch_lobes5.hip",
+    "batch_poly_6.hip", "batch_blobs.hip", "batch_lobes5.hip",
     "batch_fubini.hip")]
 
 HIPCC_FLAGS = [

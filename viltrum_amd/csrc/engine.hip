@@ -22,6 +22,7 @@ int dispatch_T(vmx_ctx* ctx, int family, int D, const std::vector<JobIn>& jobs, 
                 case 3: return run_batch<T, 3, vmx::Poly<T>, Math>(ctx, jobs, st, pairs);
                 case 4: return run_batch<T, 4, vmx::Poly<T>, Math>(ctx, jobs, st, pairs);
                 case 5: return run_batch<T, 5, vmx::Poly<T>, Math>(ctx, jobs, st, pairs);
+                case 6: return run_batch<T, 6, vmx::Poly<T>, Math>(ctx, jobs, st, pairs);
                 default: break;
             }
             break;
@@ -187,6 +188,7 @@ int eval_T(vmx_ctx* ctx, const vmx_integrand& f, uint64_t n, const double* x, do
             case 3: launch(PolyTag{}, std::integral_constant<int, 3>{}); break;
             case 4: launch(PolyTag{}, std::integral_constant<int, 4>{}); break;
             case 5: launch(PolyTag{}, std::integral_constant<int, 5>{}); break;
+            case 6: launch(PolyTag{}, std::integral_constant<int, 6>{}); break;
             default: ok = false;
         }
     } else if (f.family == VMX_FAMILY_BLOBS && D == 2) {
@@ -533,6 +535,7 @@ int mc_dispatch(vmx_ctx* ctx, const vmx_problem& p, const vmx_integrand& f, int 
                 case 3: return mc_run<T, Math, 3, vmx::Poly<T>>(ctx, p, f.params, stratified, bins, se);
                 case 4: return mc_run<T, Math, 4, vmx::Poly<T>>(ctx, p, f.params, stratified, bins, se);
                 case 5: return mc_run<T, Math, 5, vmx::Poly<T>>(ctx, p, f.params, stratified, bins, se);
+                case 6: return mc_run<T, Math, 6, vmx::Poly<T>>(ctx, p, f.params, stratified, bins, se);
                 default: break;
             }
             break;

@@ -150,7 +150,8 @@ __device__ inline void boundary_bucket(const uint32_t* h16, uint32_t need, uint3
    the results are bit-identical. */
 template <int D>
 struct TeamShape {
-    static constexpr int TEAM = D <= 2 ? 8 : (D == 3 ? 32 : 64);
+    // D = 6: one team per block (its 3^6-value tensors take 32 KB of LDS in FP32)
+    static constexpr int TEAM = D <= 2 ? 8 : (D == 3 ? 32 : (D <= 5 ? 64 : 256));
 };
 
 template <typename T, int D>

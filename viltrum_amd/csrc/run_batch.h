@@ -343,8 +343,8 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     }
     const double ms_gen = ms_since(t_start);
 
-    if constexpr (vmx::IsFubini<P>::value) {
-        if (pairs_req) throw VmxError(VMX_ERR_UNSUPPORTED, "training pairs: not for the Fubini form");
+    if constexpr (vmx::IsFubini<P>::value || D > LEARN_MAX_DIM) {
+        if (pairs_req) throw VmxError(VMX_ERR_UNSUPPORTED, "training pairs: not for the Fubini form or D > 5");
     } else if (pairs_req) {
         PairsReq* pairs = pairs_req;
         // training pairs: every node of every job, job-major in slot order
