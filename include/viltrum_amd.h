@@ -113,7 +113,20 @@ enum vmx_form {
        by 1/pdf of the strategy it took (0 where pdf < 1e-10), as the reference does.
        FP32 integrands of dimension 1..5, as for importance sampling. */
     VMX_FORM_VR_OPTIMIZE_SAMPLING_RR = 17,
-    VMX_FORM_VR_FIXED_SAMPLING_RR = 18
+    VMX_FORM_VR_FIXED_SAMPLING_RR = 18,
+    /* regions_integrator_parallel_variance_reduction_optimized with
+       region_stratification_uniform and region_sampling_uniform
+       (src/control-variates/regions-integrator-parallel-variance-reduction-optimized.h:54-142,
+       region-stratification.h:9-27), cv_optimize_weight / cv_fixed_weight(alpha): a bin's
+       spp residual samples are split over its n regions, spp / n each and one more for
+       spp % n of them taken cyclically from a random start, each weighted by
+       spp / (its region's count).  As
+       integrator_adaptive_fubini_variance_reduction_parallel_optimized
+       (integrator-adaptive-fubini-variance-reduction-optimized.h:17-23), which the
+       reference builds over infinite ranges only: Fubini integrands with inner_dim
+       VMX_INFINITE_DIM; the residual sample is f itself, without the rest's volume. */
+    VMX_FORM_VR_OPTIMIZE_STRATIFIED = 19,
+    VMX_FORM_VR_FIXED_STRATIFIED = 20
 };
 
 /* Order of the final region list (and so of every per-bin region list). */

@@ -44,8 +44,14 @@ enum vmx_family {
     VMX_FAMILY_POLY = 1,
     VMX_FAMILY_BLOBS = 2,
     VMX_FAMILY_LOBES5 = 3,
-    VMX_FAMILY_FUBINI = 4   /* an inner integrand, its trailing dims integrated by Monte Carlo */
+    VMX_FAMILY_FUBINI = 4,  /* an inner integrand, its trailing dims integrated by Monte Carlo */
+    VMX_FAMILY_DYSON = 5    /* a function of an unbounded sequence: only as the inner integrand of
+                               the Fubini form over an infinite range (inner_dim VMX_INFINITE_DIM) */
 };
+
+/* inner_dim of a Fubini descriptor whose rest is an infinite range (RangeInfinite,
+   src/range-infinite.h:16-87): the inner integrand reads an unbounded sequence */
+#define VMX_INFINITE_DIM (-1)
 
 typedef struct vmx_poly_params {
     int32_t nterms;                         /* 0..VMX_POLY_MAX_TERMS */
@@ -80,17 +86,31 @@ typedef struct vmx_lobes5_params {
    integrator-crespo2021.h:24-44): regions and bins over the descriptor's `dim` leading
    dims of an inner integrand of inner_dim dims; its remaining dims are integrated by
    Monte Carlo over [rest_lo, rest_hi] (entries dim .. inner_dim-1), with mc_samples
-   uniform samples per region node value and one per residual sample.  FP32 only, as
-   the reference. */
+   uniform samples per region node value and one per residual sample.  With inner_dim
+   VMX_INFINITE_DIM the rest is an infinite range (RangeInfinite): the inner integrand
+   (VMX_FAMILY_DYSON) reads the region point followed by an unbounded sequence of uniform
+   draws, entry k of the sequence in [rest_lo[dim + k], rest_hi[dim + k]] for the first
+   rest_count entries and in [0, 1] after them (RangeInfinite::min(i) / max(i)); the
+   rest's volume is the product of the first rest_count extents.  FP32 only, as the
+   reference. */
 typedef struct vmx_fubini_params {
-    int32_t inner_family;                   /* VMX_FAMILY_POLY, _BLOBS (3-D) or _LOBES5 */
-    int32_t inner_dim;                      /* > the descriptor's dim, <= 5 */
+    int32_t inner_family;                   /* VMX_FAMILY_POLY, _BLOBS (3-D), _LOBES5, or _DYSON */
+    int32_t inner_dim;                      /* > the descriptor's dim, <= 5; or VMX_INFINITE_DIM */
     uint32_t mc_samples;                    /* >= 1 */
     uint32_t seed;                          /* stream of the Monte Carlo samples */
     double rest_lo[VMX_MAX_DIM];            /* indexed by inner dim */
     double rest_hi[VMX_MAX_DIM];
     const void* inner;                      /* host pointer to the inner family's parameter block */
+    int32_t rest_count;                     /* infinite rest: explicit entries (0 .. VMX_MAX_DIM - dim) */
 } vmx_fubini_params;
+
+/* Dyson-series integrand of a sequence s (the reference's infinite-range example,
+   main/compilation-tests/crespo21.cc:6-25): sum = 1, range = 1; while the current entry
+   is below `decay`, step to the next entry u, x = range * u, range = x,
+   sum += x * range_before (x / (1 / range_before)); step again. */
+typedef struct vmx_dyson_params {
+    double decay;                           /* in (0, 1) */
+} vmx_dyson_params;
 
 /* Integrand descriptor handed to every entry point. */
 typedef struct vmx_integrand {
@@ -250,6 +270,39 @@ struct Lobes5 {
         T ph = T(0);
         for (int j = 0; j < 5; ++j) ph = M::fma(k[j], x[j], ph);
         return M::exp(-q) * (T(1) + amp * M::cos2pi(ph));
+    }
+};
+
+/* A function of an unbounded sequence (src/range-infinite.h, the integrand of
+   crespo21.cc): `eval_seq(s)` reads entries through a cursor with `cur()` (the current
+   entry) and `next()` (step to the following one), as the reference's integrand walks
+   a sequence iterator.  The walk stops after 4096 steps, which a decay below 1 reaches
+   with probability decay^4096 (never, in practice). */
+template <typename T>
+struct Dyson {
+    T decay;
+
+    static Dyson from(const vmx_dyson_params& p) {
+        Dyson q{};
+        q.decay = T(p.decay);
+        return q;
+    }
+
+    /* s: cur() reads the current entry, next() steps to the following one.  At most 4096
+       terms (the reference's loop has no bound; reaching it has probability below
+       decay^4096). */
+    template <class Seq>
+    VMX_HD T eval_seq(Seq& s) const {
+        T sum = T(1), range = T(1);
+        for (int step = 0; step < 4096 && s.cur() < decay; ++step) {
+            s.next();
+            const T x = range * s.cur();
+            const T prob = T(1.0 / double(range));
+            range = x;
+            sum += x / prob;
+            s.next();
+        }
+        return sum;
     }
 };
 

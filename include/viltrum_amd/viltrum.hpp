@@ -78,6 +78,46 @@ Range<T, N> range_primary() {
     return Range<T, N>(a, b);
 }
 
+/* A range of unbounded dimension (src/range-infinite.h:16-87): explicit entries
+   [a[i], b[i]], then [0, 1] for every entry after them. */
+template <typename T>
+class RangeInfinite {
+    std::vector<T> lo_, hi_;
+
+public:
+    using value_type = T;
+    RangeInfinite(const std::vector<T>& a = std::vector<T>(), const std::vector<T>& b = std::vector<T>())
+        : lo_(a), hi_(b) {
+        if (a.size() != b.size()) throw std::invalid_argument("viltrum_amd: range_infinite entries differ");
+    }
+    const std::vector<T>& min() const { return lo_; }
+    const std::vector<T>& max() const { return hi_; }
+    T min(std::size_t i) const { return i < lo_.size() ? lo_[i] : T(0); }
+    T max(std::size_t i) const { return i < hi_.size() ? hi_[i] : T(1); }
+    T volume() const {
+        T v = T(1);
+        for (std::size_t i = 0; i < lo_.size(); ++i) v *= (hi_[i] - lo_[i]);
+        return v;
+    }
+};
+
+template <typename T>
+RangeInfinite<T> range_infinite(const std::vector<T>& a, const std::vector<T>& b) {
+    return RangeInfinite<T>(a, b);
+}
+
+template <typename T = float>
+RangeInfinite<T> range_primary_infinite() {
+    return RangeInfinite<T>();
+}
+
+template <typename R>
+struct is_range : std::false_type {};
+template <typename T, std::size_t DIM>
+struct is_range<Range<T, DIM>> : std::true_type {};
+template <typename T>
+struct is_range<RangeInfinite<T>> : std::true_type {};
+
 /* ---------------------------------------------------------------- integrands */
 struct Integrand {
     vmx_integrand desc{};
@@ -103,6 +143,16 @@ inline Integrand lobes5(const vmx_lobes5_params& p) {
     auto q = std::make_shared<vmx_lobes5_params>(p);
     f.params = q;
     f.desc = vmx_integrand{VMX_FAMILY_LOBES5, 5, q.get()};
+    return f;
+}
+
+/* The Dyson series of a sequence (the reference's infinite-range example integrand,
+   main/compilation-tests/crespo21.cc:6-25); integrated over an infinite range only. */
+inline Integrand dyson(double decay = 0.75) {
+    Integrand f;
+    auto q = std::make_shared<vmx_dyson_params>(vmx_dyson_params{decay});
+    f.params = q;
+    f.desc = vmx_integrand{VMX_FAMILY_DYSON, VMX_INFINITE_DIM, q.get()};
     return f;
 }
 
@@ -418,11 +468,51 @@ public:
         base_.integrate(bins, res, g, Range<Float, RD>(lo, hi), logger, ctx);
     }
 
+    /* over an infinite range: regions and bins over its first RD entries, the sequence
+       after them Monte Carlo (the integrand reads an unbounded sequence) */
+    template <typename Bins, std::size_t DB, typename Float, typename Logger>
+    void integrate(Bins& bins, const std::array<std::size_t, DB>& res, const Integrand& f,
+                   const RangeInfinite<Float>& range, Logger& logger, Context& ctx = Context::thread_default()) const {
+        integrate_infinite<RD>(base_, uint32_t(2 * seed_ + 1), mc_, bins, res, f, range, logger, ctx);
+    }
+
     template <typename Bins, std::size_t DB, typename F, typename Float, std::size_t DIM, typename Logger>
     void integrate(Bins&, const std::array<std::size_t, DB>&, const F&, const Range<Float, DIM>&, Logger&) const {
         static_assert(std::is_same<F, Integrand>::value,
                       "viltrum_amd integrates device-evaluable integrand descriptors (viltrum_amd::blobs, poly, "
                       "lobes5); a host lambda cannot run on the GPU");
+    }
+
+    template <std::size_t R, typename Bins, std::size_t DB, typename Float, typename Logger>
+    static void integrate_infinite(const IntegratorGpu& base, uint32_t mc_seed, std::size_t mc, Bins& bins,
+                                   const std::array<std::size_t, DB>& res, const Integrand& f,
+                                   const RangeInfinite<Float>& range, Logger& logger, Context& ctx) {
+        static_assert(R >= 1 && R <= 4 && DB <= R && DB <= 3, "1..4 region dims, bins over region dims");
+        using T = std::decay_t<decltype(bins(std::array<std::size_t, DB>()))>;
+        static_assert(std::is_same<T, float>::value, "the Fubini form integrates into float bins");
+        if (f.desc.dim != VMX_INFINITE_DIM)
+            throw std::invalid_argument("viltrum_amd: an infinite range goes with an integrand of a sequence");
+        const std::size_t nrest = range.min().size() > R ? range.min().size() - R : 0;
+        if (R + nrest > VMX_MAX_DIM) throw std::invalid_argument("viltrum_amd: too many explicit range entries");
+        vmx_fubini_params fp{};
+        fp.inner_family = f.desc.family;
+        fp.inner_dim = VMX_INFINITE_DIM;
+        fp.mc_samples = uint32_t(mc);
+        fp.seed = mc_seed;
+        for (std::size_t d = 0; d < R + nrest; ++d) {
+            fp.rest_lo[d] = double(range.min(d));
+            fp.rest_hi[d] = double(range.max(d));
+        }
+        fp.rest_count = int32_t(nrest);
+        fp.inner = f.desc.params;
+        Integrand g;
+        g.desc = vmx_integrand{VMX_FAMILY_FUBINI, int(R), &fp};
+        std::array<Float, R> lo, hi;
+        for (std::size_t d = 0; d < R; ++d) {
+            lo[d] = range.min(d);
+            hi[d] = range.max(d);
+        }
+        base.integrate(bins, res, g, Range<Float, R>(lo, hi), logger, ctx);
     }
 };
 
@@ -432,24 +522,62 @@ IntegratorGpuFubini<RD> integrator_crespo2021_infinite(std::size_t iterations, s
     return IntegratorGpuFubini<RD>(iterations, mc_samples, spp, seed);
 }
 
+/* integrator_adaptive_fubini_variance_reduction_parallel_optimized<RD>(nested(simpson,
+   trapezoidal), error_heuristic_size(error_metric_relative(), 1e-5), iterations,
+   mc_samples, region_stratification_uniform(), cv_optimize_weight() or
+   cv_fixed_weight(alpha), region_sampling_uniform(), spp, seed)
+   (src/control-variates/integrator-adaptive-fubini-variance-reduction-optimized.h:17-23):
+   node values Monte Carlo from seed, residual samples from seed + 1, spread evenly over
+   each bin's regions.  Over infinite ranges only, as the reference builds it. */
+template <std::size_t RD>
+class IntegratorGpuFubiniOptimized {
+    IntegratorGpu base_;
+    std::size_t mc_, seed_;
+
+public:
+    IntegratorGpuFubiniOptimized(std::size_t iterations, std::size_t mc_samples, std::size_t spp, std::size_t seed,
+                                 int form, double alpha)
+        : base_(iterations, spp, seed + 1, VMX_HEUR_SIZE_RELATIVE, form, alpha), mc_(mc_samples), seed_(seed) {}
+
+    template <typename Bins, std::size_t DB, typename Float, typename Logger>
+    void integrate(Bins& bins, const std::array<std::size_t, DB>& res, const Integrand& f,
+                   const RangeInfinite<Float>& range, Logger& logger, Context& ctx = Context::thread_default()) const {
+        IntegratorGpuFubini<RD>::template integrate_infinite<RD>(base_, uint32_t(seed_), mc_, bins, res, f, range,
+                                                                 logger, ctx);
+    }
+};
+
+template <std::size_t RD>
+IntegratorGpuFubiniOptimized<RD> integrator_adaptive_fubini_variance_reduction_parallel_optimized(
+    std::size_t iterations, std::size_t mc_samples, std::size_t spp, std::size_t seed = 0) {
+    return IntegratorGpuFubiniOptimized<RD>(iterations, mc_samples, spp, seed, VMX_FORM_VR_OPTIMIZE_STRATIFIED, 1.0);
+}
+
+/* the same with cv_fixed_weight(alpha) */
+template <std::size_t RD>
+IntegratorGpuFubiniOptimized<RD> integrator_adaptive_fubini_variance_reduction_parallel_optimized_fixed(
+    double alpha, std::size_t iterations, std::size_t mc_samples, std::size_t spp, std::size_t seed = 0) {
+    return IntegratorGpuFubiniOptimized<RD>(iterations, mc_samples, spp, seed, VMX_FORM_VR_FIXED_STRATIFIED, alpha);
+}
+
 /* ---------------------------------------------------------------- integrate overloads
    (src/integrate.h:72-79, 93-97, 105-130, 132-148, 169-173) */
-template <typename Integrator, typename Bins, std::size_t DB, typename Float, std::size_t DIM, typename Logger>
+template <typename Integrator, typename Bins, std::size_t DB, typename Rg, typename Logger>
 void integrate(const Integrator& integrator, Bins& bins, const std::array<std::size_t, DB>& resolution,
-               const Integrand& f, const Range<Float, DIM>& range, Logger& logger) {
+               const Integrand& f, const Rg& range, Logger& logger) {
     integrator.integrate(bins, resolution, f, range, logger);
 }
 
-template <typename Integrator, typename Bins, std::size_t DB, typename Float, std::size_t DIM>
+template <typename Integrator, typename Bins, std::size_t DB, typename Rg>
 void integrate(const Integrator& integrator, Bins& bins, const std::array<std::size_t, DB>& resolution,
-               const Integrand& f, const Range<Float, DIM>& range) {
+               const Integrand& f, const Rg& range) {
     LoggerNull log;
     integrate(integrator, bins, resolution, f, range, log);
 }
 
-template <typename Integrator, typename T, typename Float, std::size_t DIM>
+template <typename Integrator, typename T, typename Rg>
 void integrate(const Integrator& integrator, std::vector<std::vector<T>>& bins, const Integrand& f,
-               const Range<Float, DIM>& range) {
+               const Rg& range) {
     std::size_t max1 = 0;
     for (const auto& v : bins) max1 = v.size() > max1 ? v.size() : max1;
     for (auto& v : bins) v.resize(max1);
@@ -458,9 +586,9 @@ void integrate(const Integrator& integrator, std::vector<std::vector<T>>& bins, 
     integrate(integrator, b, res, f, range);
 }
 
-template <typename Integrator, typename T, typename Float, std::size_t DIM, typename Logger>
+template <typename Integrator, typename T, typename Rg, typename Logger>
 void integrate(const Integrator& integrator, std::vector<std::vector<T>>& bins, const Integrand& f,
-               const Range<Float, DIM>& range, Logger& logger) {
+               const Rg& range, Logger& logger) {
     std::size_t max1 = 0;
     for (const auto& v : bins) max1 = v.size() > max1 ? v.size() : max1;
     for (auto& v : bins) v.resize(max1);
@@ -470,9 +598,9 @@ void integrate(const Integrator& integrator, std::vector<std::vector<T>>& bins, 
 }
 
 /* src/integrate.h:150-167: three bin dims, inner vectors padded to the longest */
-template <typename Integrator, typename T, typename Float, std::size_t DIM, typename Logger>
+template <typename Integrator, typename T, typename Rg, typename Logger>
 void integrate(const Integrator& integrator, std::vector<std::vector<std::vector<T>>>& bins, const Integrand& f,
-               const Range<Float, DIM>& range, Logger& logger) {
+               const Rg& range, Logger& logger) {
     std::size_t max1 = 0, max2 = 0;
     for (const auto& v : bins) max1 = v.size() > max1 ? v.size() : max1;
     for (auto& v : bins) v.resize(max1);
@@ -485,30 +613,31 @@ void integrate(const Integrator& integrator, std::vector<std::vector<std::vector
     integrate(integrator, b, res, f, range, logger);
 }
 
-template <typename Integrator, typename T, typename Float, std::size_t DIM>
+template <typename Integrator, typename T, typename Rg>
 void integrate(const Integrator& integrator, std::vector<std::vector<std::vector<T>>>& bins, const Integrand& f,
-               const Range<Float, DIM>& range) {
+               const Rg& range) {
     LoggerNull log;
     integrate(integrator, bins, f, range, log);
 }
 
-template <typename Integrator, typename T, typename Float, std::size_t DIM, typename Logger>
-void integrate(const Integrator& integrator, std::vector<T>& bins, const Integrand& f, const Range<Float, DIM>& range,
+template <typename Integrator, typename T, typename Rg, typename Logger>
+void integrate(const Integrator& integrator, std::vector<T>& bins, const Integrand& f, const Rg& range,
                Logger& logger) {
     auto b = [&bins](const std::array<std::size_t, 1>& i) -> T& { return bins[i[0]]; };
     const std::array<std::size_t, 1> res{bins.size()};
     integrate(integrator, b, res, f, range, logger);
 }
 
-template <typename Integrator, typename T, typename Float, std::size_t DIM>
-void integrate(const Integrator& integrator, std::vector<T>& bins, const Integrand& f, const Range<Float, DIM>& range) {
+template <typename Integrator, typename T, typename Rg>
+void integrate(const Integrator& integrator, std::vector<T>& bins, const Integrand& f, const Rg& range) {
     auto b = [&bins](const std::array<std::size_t, 1>& i) -> T& { return bins[i[0]]; };
     const std::array<std::size_t, 1> res{bins.size()};
     integrate(integrator, b, res, f, range);
 }
 
-template <typename Integrator, typename Float, std::size_t DIM>
-Float integrate(const Integrator& integrator, const Integrand& f, const Range<Float, DIM>& range) {
+template <typename Integrator, typename Rg, typename = std::enable_if_t<is_range<Rg>::value>>
+typename Rg::value_type integrate(const Integrator& integrator, const Integrand& f, const Rg& range) {
+    using Float = typename Rg::value_type;
     Float sol(0);
     auto b = [&sol](const std::array<std::size_t, 1>&) -> Float& { return sol; };
     const std::array<std::size_t, 1> res{1};

@@ -36,6 +36,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <random>
 #include <stdexcept>
 #include <thread>
@@ -84,7 +85,13 @@ enum Integrator {
     INT_VR_FIXED_RR_PDF_IMPORTANCE = 16,
     // region_sampling_russian_roulette (uniform region roulette; optimised / fixed weight)
     INT_VR_OPTIMIZE_SAMPLING_RR = 17,
-    INT_VR_FIXED_SAMPLING_RR = 18
+    INT_VR_FIXED_SAMPLING_RR = 18,
+    // the optimised integrator with region_stratification_uniform (deterministic samples
+    // per region), uniform point sampling; optimised / fixed weight.  The reference builds
+    // it over infinite ranges only (MonteCarloFunction reads Range::DIM, which a finite
+    // Range lacks): vmo_integrate_fubini_inf
+    INT_VR_OPTIMIZE_STRATIFIED = 19,
+    INT_VR_FIXED_STRATIFIED = 20
 };
 
 /* A region: box [lo, hi], its cached volume and 3^D node values, dim 0 fastest
@@ -414,7 +421,7 @@ void stratify(const std::vector<Region<T, D>>& regs, int db, const uint64_t* res
     uint64_t nb = 1;
     for (int i = 0; i < db; ++i) nb *= res[i];
     off.assign(nb + 1, 0);
-    uint64_t s[3], e[3];
+    uint64_t s[3] = {0, 0, 0}, e[3] = {1, 1, 1};
     auto for_box = [&](auto&& fn) {
         for (size_t r = 0; r < regs.size(); ++r) {
             pixels_in_region(regs[r], db, res, lo, hi, s, e);
@@ -596,6 +603,141 @@ struct FubiniSamples {
             sol += (*fp)(xf) * factor;
             return sol;
         };
+    }
+};
+
+/* ---- infinite rest ranges (RangeInfinite, src/range-infinite.h:16-87) ----
+   The rest of an infinite range: entry i in [lo[i], hi[i]] for the explicit entries,
+   [0, 1] after them (RangeInfinite::min(i) / max(i)); volume the product of the
+   explicit extents in float (RangeInfinite's constructor). */
+struct RestInf {
+    std::vector<float> lo, hi;
+    float min(std::size_t i) const { return i < lo.size() ? lo[i] : 0.0f; }
+    float max(std::size_t i) const { return i < hi.size() ? hi[i] : 1.0f; }
+    float volume() const {
+        float v = 1.0f;
+        for (std::size_t i = 0; i < std::max(lo.size(), hi.size()); ++i) v *= (max(i) - min(i));
+        return v;
+    }
+};
+
+/* RandomSequenceRNG (src/monte-carlo/random-sequence-rng.h:9-40): its own engine; entry
+   i drawn by uniform_real_distribution(min(i), max(i)) when the cursor reaches it. */
+struct SeqRng {
+    std::mt19937 rng;
+    const RestInf* range;
+    std::size_t i = 0;
+    float n;
+    SeqRng(std::mt19937 e, const RestInf* r) : rng(e), range(r) {
+        n = std::uniform_real_distribution<float>(range->min(0), range->max(0))(rng);
+    }
+    float cur() const { return n; }
+    void next() {
+        ++i;
+        n = std::uniform_real_distribution<float>(range->min(i), range->max(i))(rng);
+    }
+};
+
+/* RandomSequenceRefDis (random-sequence-ref-dis.h:9-40): draws from an engine it refers
+   to, dis(0, 1) * (max(i) - min(i)) + min(i); the first entry is drawn when the cursor
+   is made. */
+struct SeqRefDis {
+    std::mt19937* rng;
+    std::uniform_real_distribution<float>* dis;
+    const RestInf* range;
+    std::size_t i = 0;
+    float n;
+    SeqRefDis(std::mt19937* e, std::uniform_real_distribution<float>* d, const RestInf* r) : rng(e), dis(d), range(r) {
+        n = (*dis)(*rng) * (range->max(0) - range->min(0)) + range->min(0);
+    }
+    float cur() const { return n; }
+    void next() {
+        ++i;
+        n = (*dis)(*rng) * (range->max(i) - range->min(i)) + range->min(i);
+    }
+};
+
+/* concat(x, seq) (src/combination/concat.h:8-33): the RD region coordinates, then the
+   sequence (made, and so its first entry drawn, before the integrand reads anything) */
+template <int RD, typename S>
+struct SeqConcat {
+    const float* x;
+    S s;
+    int i1 = 0;
+    float cur() const { return i1 < RD ? x[i1] : s.cur(); }
+    void next() {
+        if (i1 < RD)
+            ++i1;
+        else
+            s.next();
+    }
+};
+
+/* Node values over an infinite rest: MonteCarlo::integrate over RangeInfinite
+   (src/monte-carlo/monte-carlo.h:62-84, one bin): each sample is a RandomSequenceRNG
+   seeded by one draw of the Monte Carlo engine (random_sequence(range, seed),
+   random-sequence.h:13-16), f(concat(x, sequence)) * (1 * volume / mc) added into a float. */
+template <int RD, typename F>
+struct FubiniNodesInf {
+    const F& f;
+    RestInf rest;
+    uint64_t mc;
+    mutable std::mt19937 eng;
+    float operator()(const float* x) const {
+        const double factor = 1.0 * double(rest.volume()) / double(mc);
+        float sol = 0.0f;
+        for (uint64_t i = 0; i < mc; ++i) {
+            const unsigned seed = unsigned(std::size_t(eng()));
+            SeqConcat<RD, SeqRng> s{x, SeqRng(std::mt19937(seed), &rest)};
+            sol += f.eval_seq(s) * factor;
+        }
+        return sol;
+    }
+};
+
+/* Residual samples over an infinite rest: as FubiniSamples, then MonteCarloPerBin over
+   RangeInfinite with one bin and one sample (monte-carlo-per-bin.h:64-87): rest entry 0
+   subranged to [min0, min0 + (max0 - min0) / 1] (an explicit entry from then on),
+   f(concat(x, RandomSequenceRefDis over the per-bin engine)) * (volume / 1). */
+template <int RD, typename F>
+struct FubiniSamplesInf {
+    const F& f;
+    RestInf rest;
+    int copies = 1;
+    auto bind(std::mt19937& rng) const {
+        auto st = std::make_shared<std::pair<std::mt19937, std::uniform_real_distribution<float>>>(
+            copied_engine(std::mt19937(rng()), copies), std::uniform_real_distribution<float>(0.0f, 1.0f));
+        auto sub = std::make_shared<RestInf>(rest);
+        const float drange = (rest.max(0) - rest.min(0)) / float(1);
+        if (sub->lo.empty()) sub->lo.resize(1, 0.0f);
+        if (sub->hi.empty()) sub->hi.resize(1, 1.0f);
+        sub->lo[0] = rest.min(0) + float(std::size_t(0)) * drange;
+        sub->hi[0] = rest.min(0) + float(std::size_t(1)) * drange;
+        const double factor = double(rest.volume()) / double(1);
+        const F* fp = &f;
+        return [fp, st, sub, factor](const float* x) {
+            // a fresh distribution object per sequence (random_sequence_ref_dis makes one)
+            std::uniform_real_distribution<float> dis(0.0f, 1.0f);
+            SeqConcat<RD, SeqRefDis> s{x, SeqRefDis(&st->first, &dis, sub.get())};
+            float sol = 0.0f;
+            sol += fp->eval_seq(s) * factor;
+            return sol;
+        };
+    }
+};
+
+/* The rest function of the optimised integrator over an infinite rest
+   (MonteCarloFunction<F, RNG, RangeInfinite>, regions-integrator-parallel-variance-
+   reduction-optimized.h:36-47): f(concat(x, random_sequence(rest, rng))) drawing from
+   the bin's own engine. */
+template <int RD, typename F>
+struct RestFunctionInf {
+    const F& f;
+    RestInf rest;
+    float operator()(const float* x, std::mt19937& rng) const {
+        std::uniform_real_distribution<float> dis(0.0f, 1.0f);
+        SeqConcat<RD, SeqRefDis> s{x, SeqRefDis(&rng, &dis, &rest)};
+        return f.eval_seq(s);
     }
 };
 
@@ -962,6 +1104,115 @@ void integrate_regions_vr(const std::vector<Region<T, D>>& regs, const F& f, con
                         }
                         acc.push(T(fb(x) * double(factor) * rrfactor * sfactor),
                                  T(approximation_at(r, x) * double(factor) * rrfactor * sfactor));
+                    }
+                }
+                out[out_index(db, res, p)] = acc.integral(approximation);
+            }
+        }
+    };
+    const int nt = std::max(1, nthreads);
+    if (nt == 1) {
+        work();
+    } else {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; ++t) th.emplace_back(work);
+        for (auto& t : th) t.join();
+    }
+    if (st) st->evals_residual = nb * spp;
+}
+
+/* The optimised variance-reduction integrator with region_stratification_uniform
+   (src/control-variates/regions-integrator-parallel-variance-reduction-optimized.h:
+   55-113, region-stratification.h:9-25): per bin (engines seeded as above), the control
+   variate as above; spp / n samples per region of the bin's list, the remainder spread
+   one each over consecutive regions from a uniform random start; each sample uniform
+   in bin x region (region_sampling_uniform) and weighted by spp / (samples of its
+   region).  `rest(x, rng)` is the integrand at a sample, drawing any rest dims from the
+   bin's engine (MonteCarloFunction, :16-47). */
+template <typename T, int D, typename RF, typename Acc>
+void integrate_regions_strat(const std::vector<Region<T, D>>& regs, const RF& rest, const T* lo, const T* hi, int db,
+                             const uint64_t* res, uint64_t spp, uint64_t seed, double alpha, int nthreads, T* out,
+                             Stats* st) {
+    std::vector<uint64_t> off;
+    std::vector<uint32_t> lst;
+    stratify(regs, db, res, lo, hi, off, lst);
+    if (st) st->pairs = lst.size();
+    uint64_t nb = 1;
+    T drange[3];
+    for (int i = 0; i < db; ++i) {
+        nb *= res[i];
+        drange[i] = (hi[i] - lo[i]) / T(res[i]);
+    }
+    const uint64_t factor = nb;
+    std::vector<uint32_t> seeds(nb);
+    {
+        std::mt19937 master(seed);
+        for (uint64_t b = 0; b < nb; ++b) seeds[b] = uint32_t(master());
+    }
+    const uint64_t rows = res[0];
+    std::atomic<uint64_t> next_row{0};
+    auto work = [&]() {
+        std::vector<T> ilo, ihi;
+        for (;;) {
+            const uint64_t i0 = next_row.fetch_add(1);
+            if (i0 >= rows) break;
+            const uint64_t inner = nb / rows;
+            for (uint64_t q = 0; q < inner; ++q) {
+                uint64_t p[3] = {i0, 0, 0};
+                if (db > 1) p[1] = q % res[1];
+                if (db > 2) p[2] = q / res[1];
+                const uint64_t b = p[0] + (db > 1 ? res[0] * (p[1] + (db > 2 ? res[1] * p[2] : 0)) : 0);
+                std::mt19937 rng;
+                rng.seed(seeds[b]);
+                T blo[D], bhi[D];
+                for (int d = 0; d < D; ++d) {
+                    blo[d] = lo[d];
+                    bhi[d] = hi[d];
+                }
+                for (int i = 0; i < db; ++i) {
+                    blo[i] = lo[i] + p[i] * drange[i];
+                    bhi[i] = lo[i] + (p[i] + 1) * drange[i];
+                }
+                const uint64_t n = off[b + 1] - off[b];
+                ilo.resize(n * D);
+                ihi.resize(n * D);
+                T approximation = 0;
+                for (uint64_t k = 0; k < n; ++k) {
+                    const Region<T, D>& r = regs[lst[off[b] + k]];
+                    bool empty = false;
+                    for (int d = 0; d < D; ++d) {
+                        const T a = ref_max(blo[d], r.lo[d]);
+                        ilo[k * D + d] = a;
+                        ihi[k * D + d] = ref_max(a, ref_min(bhi[d], r.hi[d]));
+                        if (ilo[k * D + d] >= ihi[k * D + d]) empty = true;
+                    }
+                    if (!empty) approximation += double(factor) * integral_subrange(r, &ilo[k * D], &ihi[k * D]);
+                }
+                if (spp > 0 && n == 0) throw std::runtime_error("bin with no region (undefined in the reference)");
+                std::vector<unsigned long> spr(n, n ? spp / n : 0);
+                if (n > 0) {
+                    const unsigned long remainder = spp % n;
+                    if (remainder > 0) {
+                        const unsigned long start = std::uniform_int_distribution<unsigned long>(0, n - 1)(rng);
+                        for (unsigned long i = 0; i < remainder; ++i) spr[(start + i) % n]++;
+                    }
+                }
+                Acc acc;
+                acc.set_alpha(alpha);
+                for (uint64_t k = 0; k < n; ++k) {
+                    const Region<T, D>& r = regs[lst[off[b] + k]];
+                    const T* slo = &ilo[k * D];
+                    const T* shi = &ihi[k * D];
+                    for (unsigned long s = 0; s < spr[k]; ++s) {
+                        T x[D];
+                        for (int d = 0; d < D; ++d) {
+                            std::uniform_real_distribution<T> dis(slo[d], shi[d]);
+                            x[d] = dis(rng);
+                        }
+                        const T sfactor = box_volume<T, D>(slo, shi);
+                        const T fx = rest(x, rng);
+                        acc.push(T(fx * double(factor) * sfactor * double(spp) / double(spr[k])),
+                                 T(approximation_at(r, x) * double(factor) * sfactor * double(spp) / double(spr[k])));
                     }
                 }
                 out[out_index(db, res, p)] = acc.integral(approximation);

@@ -436,6 +436,80 @@ int vmo_integrate_fubini(int rd, const vmx_integrand* f, const double* lo, const
     return rc;
 }
 
+/* integrator_crespo2021_infinite<rd> (form 0) or
+   integrator_adaptive_fubini_variance_reduction_parallel_optimized<rd> with
+   region_stratification_uniform, region_sampling_uniform and cv_optimize_weight (form 1)
+   or cv_fixed_weight(alpha) (form 2), over an infinite range: region dims
+   [lo, hi] (rd entries), the rest's first rest_count entries [rest_lo, rest_hi], [0, 1]
+   after them; the integrand is the Dyson series of the sequence. */
+int vmo_integrate_fubini_inf(int rd, int form, const vmx_dyson_params* dp, const double* lo, const double* hi,
+                             int rest_count, const double* rest_lo, const double* rest_hi, int db, const uint64_t* res,
+                             uint64_t iterations, uint64_t mc, uint64_t spp, uint64_t seed, double alpha,
+                             int gen_copies, int bin_copies, int nthreads, double* bins_inout, vmo_stats* st) {
+    vmo::Stats s;
+    int rc = -1;
+    try {
+        if (db < 1 || db > rd || db > 3) throw std::runtime_error("bin dims must be 1..min(3, region dims)");
+        if (mc < 1) throw std::runtime_error("mc samples must be >= 1");
+        auto run = [&](auto rdc) {
+            constexpr int RD = decltype(rdc)::value;
+            const vmx::Dyson<float> dy = vmx::Dyson<float>::from(*dp);
+            vmo::RestInf rest;
+            for (int k = 0; k < rest_count; ++k) {
+                rest.lo.push_back(float(rest_lo[k]));
+                rest.hi.push_back(float(rest_hi[k]));
+            }
+            float flo[RD], fhi[RD];
+            load_box<float, RD>(lo, hi, flo, fhi);
+            const uint64_t gen_seed = form == 0 ? 2 * seed + 1 : seed;
+            vmo::FubiniNodesInf<RD, vmx::Dyson<float>> nodes{dy, rest, mc,
+                                                              vmo::copied_engine(std::mt19937(gen_seed), gen_copies)};
+            Timer tm;
+            uint64_t ev = 0;
+            auto regs = vmo::generate<float, RD>(nodes, flo, fhi, vmo::HEUR_SIZE_RELATIVE, iterations, &ev);
+            const double tg = tm.lap();
+            uint64_t nb = 1;
+            for (int i = 0; i < db; ++i) nb *= res[i];
+            std::vector<float> out(nb);
+            for (uint64_t b = 0; b < nb; ++b) out[b] = float(bins_inout[b]);
+            if (form == 0) {
+                vmo::FubiniSamplesInf<RD, vmx::Dyson<float>> samples{dy, rest, bin_copies};
+                vmo::integrate_regions_vr<float, RD, decltype(samples), vmo::AccOptimize<float>>(
+                    regs, samples, flo, fhi, db, res, spp, seed, 1.0, nthreads, out.data(), &s);
+            } else {
+                vmo::RestFunctionInf<RD, vmx::Dyson<float>> restf{dy, rest};
+                if (form == 1)
+                    vmo::integrate_regions_strat<float, RD, decltype(restf), vmo::AccOptimize<float>>(
+                        regs, restf, flo, fhi, db, res, spp, seed + 1, alpha, nthreads, out.data(), &s);
+                else
+                    vmo::integrate_regions_strat<float, RD, decltype(restf), vmo::AccFixed<float>>(
+                        regs, restf, flo, fhi, db, res, spp, seed + 1, alpha, nthreads, out.data(), &s);
+            }
+            for (uint64_t b = 0; b < nb; ++b) bins_inout[b] = double(out[b]);
+            s.regions = regs.size();
+            s.evals_regions = ev * mc;
+            s.t_generate = tg;
+            s.t_bins = tm.lap();
+            return 0;
+        };
+        if (rd == 1)
+            rc = run(std::integral_constant<int, 1>{});
+        else if (rd == 2)
+            rc = run(std::integral_constant<int, 2>{});
+        else if (rd == 3)
+            rc = run(std::integral_constant<int, 3>{});
+        else if (rd == 4)
+            rc = run(std::integral_constant<int, 4>{});
+        else
+            throw std::runtime_error("infinite Fubini form: region dims 1..4");
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        rc = -3;
+    }
+    copy_stats(s, st);
+    return rc;
+}
+
 /* Rules::sample_line (src/newton-cotes/rules.h:216-237, float): a point of [t0, t1] from
    the uniform s and the line's node values p[3]. */
 float vmo_imp_sample_line(float s, float t0, float t1, const float* p) {

@@ -310,6 +310,81 @@ int fubini_db(const vmx_integrand* f, const double* lo, const double* hi, const 
     return rc;
 }
 
+/* The project's Dyson series (vmx::Dyson, include/viltrum_amd/integrands.h) read through
+   the reference's sequence iterator: the sequence the reference hands an integrand over
+   an infinite range is consumed front to back. */
+template <class It>
+struct IterSeq {
+    It it;
+    float cur() const { return *it; }
+    void next() { ++it; }
+};
+struct DysonRef {
+    vmx::Dyson<float> d;
+    template <class Seq>
+    float operator()(const Seq& seq) const {
+        IterSeq<decltype(seq.begin())> s{seq.begin()};
+        return d.eval_seq(s);
+    }
+};
+
+/* Over a RangeInfinite of RD region dims then rest_count rest dims ([0, 1] after them):
+   form 0 integrator_crespo2021_infinite<RD> (src/control-variates/integrator-crespo2021.h:24-44);
+   form 1 / 2 integrator_adaptive_fubini_variance_reduction_parallel_optimized<RD>
+   (src/control-variates/integrator-adaptive-fubini-variance-reduction-optimized.h:17-23) with
+   nested(simpson, trapezoidal), error_heuristic_size(error_metric_relative(), 1e-5),
+   region_stratification_uniform, cv_optimize_weight (1) or cv_fixed_weight(alpha) (2),
+   region_sampling_uniform. */
+template <std::size_t RD, std::size_t DB>
+int fubini_inf_db(int form, const vmx_dyson_params* dp, const double* lo, const double* hi, int rest_count,
+                  const double* rest_lo, const double* rest_hi, const uint64_t* res, uint64_t iterations, uint64_t mc,
+                  uint64_t spp, uint64_t seed, double alpha, double* bins_inout) {
+    std::vector<float> a, b;
+    for (std::size_t d = 0; d < RD; ++d) {
+        a.push_back(float(lo[d]));
+        b.push_back(float(hi[d]));
+    }
+    for (int k = 0; k < rest_count; ++k) {
+        a.push_back(float(rest_lo[k]));
+        b.push_back(float(rest_hi[k]));
+    }
+    const viltrum::RangeInfinite<float> range(a, b);
+    std::array<std::size_t, DB> r;
+    std::size_t nb = 1;
+    for (std::size_t i = 0; i < DB; ++i) {
+        r[i] = res[i];
+        nb *= res[i];
+    }
+    std::vector<float> out(nb);
+    for (std::size_t k = 0; k < nb; ++k) out[k] = float(bins_inout[k]);
+    auto bins = [&](const std::array<std::size_t, DB>& p) -> float& {
+        std::size_t k = p[0];
+        for (std::size_t i = 1; i < DB; ++i) k = k * r[i] + p[i];
+        return out[k];
+    };
+    const DysonRef fn{vmx::Dyson<float>::from(*dp)};
+    if (form == 0) {
+        auto in = viltrum::integrator_crespo2021_infinite<RD>(std::size_t(iterations), std::size_t(mc),
+                                                               std::size_t(spp), std::size_t(seed));
+        viltrum::integrate(in, bins, r, fn, range);
+    } else {
+        auto run = [&](auto cv) {
+            auto in = viltrum::integrator_adaptive_fubini_variance_reduction_parallel_optimized<RD>(
+                viltrum::nested(viltrum::simpson, viltrum::trapezoidal),
+                viltrum::error_heuristic_size(viltrum::error_metric_relative(), 1.e-5), std::size_t(iterations),
+                (unsigned long)mc, viltrum::region_stratification_uniform(), std::move(cv),
+                viltrum::region_sampling_uniform(), (unsigned long)spp, std::size_t(seed));
+            viltrum::integrate(in, bins, r, fn, range);
+        };
+        if (form == 1)
+            run(viltrum::cv_optimize_weight());
+        else
+            run(viltrum::cv_fixed_weight(alpha));
+    }
+    for (std::size_t k = 0; k < nb; ++k) bins_inout[k] = double(out[k]);
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -390,6 +465,33 @@ int vmr_integrate_fubini(int rd, const vmx_integrand* f, const double* lo, const
         if (rd == 2 && f->dim == 5 && db == 2)
             return fubini_db<2, 5, 2>(f, lo, hi, res, iterations, mc, spp, seed, bins_inout);
         g_err = "fubini: (region dims, dims, bin dims) not built";
+        return -1;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -3;
+    }
+}
+
+/* fubini_inf_db for (rd, db) in {(1, 1), (2, 1), (2, 2), (3, 1..3), (4, 1), (4, 2)}. */
+int vmr_integrate_fubini_inf(int rd, int form, const vmx_dyson_params* dp, const double* lo, const double* hi,
+                             int rest_count, const double* rest_lo, const double* rest_hi, int db,
+                             const uint64_t* res, uint64_t iterations, uint64_t mc, uint64_t spp, uint64_t seed,
+                             double alpha, double* bins_inout) {
+    try {
+#define VMR_FI(R, B)                                                                                            \
+    if (rd == R && db == B)                                                                                     \
+        return fubini_inf_db<R, B>(form, dp, lo, hi, rest_count, rest_lo, rest_hi, res, iterations, mc, spp, seed, \
+                                   alpha, bins_inout);
+        VMR_FI(1, 1)
+        VMR_FI(2, 1)
+        VMR_FI(2, 2)
+        VMR_FI(3, 1)
+        VMR_FI(3, 2)
+        VMR_FI(3, 3)
+        VMR_FI(4, 1)
+        VMR_FI(4, 2)
+#undef VMR_FI
+        g_err = "infinite fubini: (region dims, bin dims) not built";
         return -1;
     } catch (const std::exception& e) {
         g_err = e.what();

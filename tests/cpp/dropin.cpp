@@ -84,5 +84,21 @@ int main() {
     for (auto& row : b2)
         for (float x : row) s2 += x;
     std::printf("replay %zu %.9g %.9g %.9g\n", keep.regions.count, double(b2[3][7]), double(b2[20][11]), s2);
+
+    // the Dyson series over an infinite range (main/compilation-tests/crespo21.cc:27-35):
+    // integrator_crespo2021_infinite<4> into 10 bins, and the optimised Fubini integrator
+    auto dy = viltrum::dyson(0.75);
+    std::vector<float> sol_bins(10, 0.0f);
+    viltrum::integrate(viltrum::integrator_crespo2021_infinite<4>(1280, 8, 256), sol_bins, dy,
+                       viltrum::range_primary_infinite<float>());
+    double si = 0;
+    for (float x : sol_bins) si += x;
+    std::printf("infinite %.9g %.9g %.9g\n", double(sol_bins[0]), double(sol_bins[9]), si);
+    std::vector<float> opt_bins(10, 0.0f);
+    viltrum::integrate(viltrum::integrator_adaptive_fubini_variance_reduction_parallel_optimized<2>(300, 4, 32, 7),
+                       opt_bins, dy, viltrum::range_infinite<float>({0.0f, 0.0f, 0.1f}, {1.0f, 1.0f, 0.9f}));
+    double so = 0;
+    for (float x : opt_bins) so += x;
+    std::printf("optimized %.9g %.9g %.9g\n", double(opt_bins[0]), double(opt_bins[9]), so);
     return 0;
 }

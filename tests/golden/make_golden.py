@@ -105,6 +105,34 @@ def fubini_key(name, rd, res, it, mc, spp, seed):
     return "fubini/%s/rd%d/r%s/it%d/mc%d/spp%d/seed%d" % (name, rd, "x".join(map(str, res)), it, mc, spp, seed)
 
 
+# the Dyson series over an infinite range: (form (0 integrator_crespo2021_infinite, 1 / 2
+# the optimised Fubini integrator with cv_optimize_weight / cv_fixed_weight(alpha)),
+# decay, region dims, res, iterations, mc samples, spp, seed, alpha, region range
+# (lo, hi) or None for [0, 1], explicit rest entries (lo, hi))
+FUBINI_INF_CASES = [
+    (0, 0.75, 4, (10,), 1280, 8, 256, 3, 1.0, None, ((), ())),  # main/compilation-tests/crespo21.cc
+    (0, 0.6, 1, (12,), 100, 3, 8, 5, 1.0, None, ((0.1,), (0.8,))),
+    (0, 0.75, 2, (6, 5), 200, 2, 16, 7, 1.0, ((-0.5, 0.0), (1.0, 2.0)), ((), ())),
+    (1, 0.75, 2, (4, 4), 150, 4, 32, 9, 1.0, None, ((), ())),
+    (1, 0.5, 4, (3, 2), 120, 2, 5, 2, 1.0, None, ((0.2, 0.0), (0.7, 0.5))),  # spp below most bins' region counts
+    (2, 0.8, 1, (9,), 60, 5, 12, 4, 0.4, None, ((), ())),
+    (2, 0.75, 2, (8,), 300, 3, 7, 11, 0.7, ((0.0, 0.25), (1.0, 0.75)), ((0.3,), (0.9,))),
+]
+
+
+def fubini_inf_key(form, decay, rd, res, it, mc, spp, seed, alpha, box, rest):
+    return "fubini_inf/f%d/d%g/rd%d/r%s/it%d/mc%d/spp%d/seed%d/a%g/box%s/rest%s" % (
+        form, decay, rd, "x".join(map(str, res)), it, mc, spp, seed, alpha, "" if box is None else repr(box),
+        repr(rest))
+
+
+def run_fubini_inf(case, **kw):
+    form, decay, rd, res, it, mc, spp, seed, alpha, box, rest = case
+    lo, hi = (None, None) if box is None else box
+    return O.integrate_fubini_inf(decay, rd, res, it, mc, spp, seed, form=form, alpha=alpha, lo=lo, hi=hi,
+                                  rest_lo=rest[0], rest_hi=rest[1], **kw)
+
+
 def main():
     if O.ref() is None:
         sys.exit("reference build missing: make -C oracle (needs /root/reference)")
@@ -129,6 +157,9 @@ def main():
     for name, rd, res, it, mc, spp, seed in FUBINI_CASES:
         b, _ = O.integrate_fubini(fs[name], rd, res, it, mc, spp, seed, use_ref=True)
         out[fubini_key(name, rd, res, it, mc, spp, seed)] = b
+    for case in FUBINI_INF_CASES:
+        b, _ = run_fubini_inf(case, use_ref=True)
+        out[fubini_inf_key(*case)] = b
     path = os.path.join(HERE, "reference_fixtures.npz")
     np.savez_compressed(path, **out)
     print("wrote %s (%d arrays, %.0f KB)" % (path, len(out), os.path.getsize(path) / 1024))

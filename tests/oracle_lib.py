@@ -178,6 +178,58 @@ def integrate_fubini(f, rd, res, iterations, mc, spp, seed, lo=None, hi=None, nt
     return bins.reshape(res), st
 
 
+class DysonParams(C.Structure):
+    _fields_ = [("decay", C.c_double)]
+
+
+# forms of integrate_fubini_inf
+INF_CRESPO2021, INF_OPTIMIZE_STRATIFIED, INF_FIXED_STRATIFIED = 0, 1, 2
+# copy-constructions of the Monte Carlo objects over an infinite range, per form (see oracle.hpp)
+FUBINI_INF_GEN_COPIES = {INF_CRESPO2021: 3, INF_OPTIMIZE_STRATIFIED: 3, INF_FIXED_STRATIFIED: 3}
+FUBINI_INF_BIN_COPIES = 1
+
+
+def integrate_fubini_inf(decay, rd, res, iterations, mc, spp, seed, form=INF_CRESPO2021, alpha=1.0, lo=None,
+                         hi=None, rest_lo=(), rest_hi=(), nthreads=1, use_ref=False, gen_copies=None,
+                         bin_copies=FUBINI_INF_BIN_COPIES):
+    """The Dyson series (decay) over an infinite range: region dims [lo, hi] (rd entries),
+    then the rest's explicit entries [rest_lo, rest_hi] and [0, 1] after them.  form 0:
+    integrator_crespo2021_infinite<rd>; 1 / 2: the optimised Fubini integrator with
+    region_stratification_uniform and cv_optimize_weight / cv_fixed_weight(alpha).
+    Bins in C order over the first len(res) region dims; returns (bins, stats)."""
+    lo, hi = _box(rd, lo, hi)
+    rl = np.ascontiguousarray(np.array(rest_lo, dtype=np.float64).reshape(-1))
+    rh = np.ascontiguousarray(np.array(rest_hi, dtype=np.float64).reshape(-1))
+    assert rl.size == rh.size
+    dp = DysonParams(float(decay))
+    res = tuple(int(r) for r in res)
+    r = np.ascontiguousarray(np.array(res, dtype=np.uint64))
+    bins = np.zeros(int(np.prod(res)))
+    rlp = _ptr(rl) if rl.size else None
+    rhp = _ptr(rh) if rh.size else None
+    if use_ref:
+        lib = ref()
+        rc = lib.vmr_integrate_fubini_inf(C.c_int(rd), C.c_int(form), C.byref(dp), _ptr(lo), _ptr(hi),
+                                          C.c_int(rl.size), rlp, rhp, C.c_int(len(res)), _ptr(r, _up),
+                                          C.c_uint64(iterations), C.c_uint64(mc), C.c_uint64(spp), C.c_uint64(seed),
+                                          C.c_double(alpha), _ptr(bins))
+        if rc < 0:
+            raise RuntimeError(lib.vmr_last_error().decode())
+        return bins.reshape(res), None
+    if gen_copies is None:
+        gen_copies = FUBINI_INF_GEN_COPIES[form]
+    lib = oracle()
+    st = OracleStats()
+    rc = lib.vmo_integrate_fubini_inf(C.c_int(rd), C.c_int(form), C.byref(dp), _ptr(lo), _ptr(hi), C.c_int(rl.size),
+                                      rlp, rhp, C.c_int(len(res)), _ptr(r, _up), C.c_uint64(iterations),
+                                      C.c_uint64(mc), C.c_uint64(spp), C.c_uint64(seed), C.c_double(alpha),
+                                      C.c_int(gen_copies), C.c_int(bin_copies), C.c_int(nthreads), _ptr(bins),
+                                      C.byref(st))
+    if rc < 0:
+        raise RuntimeError(lib.vmo_last_error().decode())
+    return bins.reshape(res), st
+
+
 def integrate_regions(f, regs_lo, regs_hi, regs_nodes, res, spp, seed, scalar=8, integrator=INT_VR_OPTIMIZE,
                       alpha=1.0, lo=None, hi=None, nthreads=1, bins_in=None):
     """Per-bin phase of the oracle on a caller-given region list."""

@@ -48,3 +48,16 @@ def test_cpp_dropin_matches_python(ctx):
     vals = [float(x) for x in parts[2:]]
     assert np.float32(vals[0]) == b2[3, 7] and np.float32(vals[1]) == b2[20, 11]
     assert abs(vals[2] - float(b2.astype(np.float64).sum())) < 1e-6 * abs(vals[2])
+    # the Dyson series over infinite ranges, both Fubini integrators
+    dy = I.dyson(0.75)
+    bi = np.zeros(10, dtype=np.float32)
+    V.integrate(V.integrator_crespo2021_infinite(4, 1280, 8, 256), bi, dy, V.range_primary_infinite(), ctx=ctx)
+    vals = [float(x) for x in out[5].split()[1:]]
+    assert np.float32(vals[0]) == bi[0] and np.float32(vals[1]) == bi[9]
+    assert abs(vals[2] - float(bi.astype(np.float64).sum())) < 1e-6 * abs(vals[2])
+    bo = np.zeros(10, dtype=np.float32)
+    V.integrate(V.integrator_adaptive_fubini_variance_reduction_parallel_optimized(2, 300, 4, 32, seed=7), bo, dy,
+                V.range_infinite([0.0, 0.0, 0.1], [1.0, 1.0, 0.9]), ctx=ctx)
+    vals = [float(x) for x in out[6].split()[1:]]
+    assert np.float32(vals[0]) == bo[0] and np.float32(vals[1]) == bo[9]
+    assert abs(vals[2] - float(bo.astype(np.float64).sum())) < 1e-6 * abs(vals[2])

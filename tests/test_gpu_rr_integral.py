@@ -166,7 +166,7 @@ def test_roulette_with_importance_rejects_fp64(ctx):
 
 def test_unknown_forms_are_rejected(ctx):
     f = I.config1()
-    for form in (-1, 19):
+    for form in (-1, 21):
         with pytest.raises((ValueError, V.VmxError)):
             V.integrate(V.Integrator(63, 4, 0, form=form), np.zeros((4, 4), np.float32), f, V.range_primary(2),
                         ctx=ctx)

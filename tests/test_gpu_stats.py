@@ -28,11 +28,12 @@ def _runs(ctx, f, res, it, spp, form, heur, alpha, scalar):
     return np.array(g), np.array(o)
 
 
-def _check(g, o, rel_floor=0.0, spread=(0.8, 1.25)):
+def _check(g, o, rel_floor=0.0, spread=(0.8, 1.25), flat_rel=1e-12):
     mg, mo = g.mean(0), o.mean(0)
     vg, vo = g.var(0, ddof=1), o.var(0, ddof=1)
     se = np.sqrt((vg + vo) / SEEDS)
-    ok = se > 1e-12 * (1 + np.abs(mo))
+    # bins whose spread is at the rounding level are compared by value (last assertion)
+    ok = se > flat_rel * (1 + np.abs(mo))
     z = (mg - mo)[ok] / se[ok]
     assert ok.sum() > 0.5 * ok.size
     B = z.size

@@ -79,6 +79,95 @@ def test_fubini_with_a_tensor_quadratic_in_the_region_dims_is_exact():
     assert abs(b.mean() - (1.0 / 6 + 0.5 + 1.5)) < 5e-3
 
 
+@pytest.mark.parametrize("case", MG.FUBINI_INF_CASES,
+                         ids=lambda c: "f%d-rd%d-r%s-spp%d" % (c[0], c[2], "x".join(map(str, c[3])), c[6]))
+def test_fubini_infinite_matches_reference_fixture(case):
+    """The Dyson series over an infinite range: integrator_crespo2021_infinite (form 0) and
+    the optimised Fubini integrator with region_stratification_uniform (forms 1, 2)."""
+    b, st = MG.run_fubini_inf(case, nthreads=2)
+    assert st.regions == case[4] + 1
+    assert np.array_equal(b, FIX[MG.fubini_inf_key(*case)])
+
+
+@pytest.mark.skipif(not os.path.exists(O.REF_SO), reason="reference build absent (no /root/reference where this runs)")
+@pytest.mark.parametrize("seed", range(8))
+def test_fubini_infinite_oracle_matches_live_reference(seed):
+    rng = np.random.default_rng(70 + seed)
+    form = seed % 3
+    rd, res = [(1, (7,)), (2, (5, 3)), (4, (4, 2)), (3, (3, 2, 2))][seed % 4]
+    it, mc, spp = int(rng.integers(5, 300)), int(rng.integers(1, 6)), int(rng.integers(1, 30))
+    nr = int(rng.integers(0, 3))
+    rl = rng.uniform(0, 0.4, nr)
+    rh = rl + rng.uniform(0.2, 0.6, nr)
+    lo = rng.uniform(-1, 0, rd)
+    hi = lo + rng.uniform(0.5, 2, rd)
+    kw = dict(form=form, alpha=0.6, lo=lo, hi=hi, rest_lo=tuple(rl), rest_hi=tuple(rh))
+    decay = float(rng.uniform(0.3, 0.9))
+    a, _ = O.integrate_fubini_inf(decay, rd, res, it, mc, spp, 100 + seed, nthreads=3, **kw)
+    b, _ = O.integrate_fubini_inf(decay, rd, res, it, mc, spp, 100 + seed, use_ref=True, **kw)
+    assert np.array_equal(a, b)
+
+
+def dyson_bins_closed_form(decay, rd, res):
+    """Per-bin value (bin count times the integral over the bin) of the Dyson series over
+    the primary infinite range with regions over the first rd entries, bins over the
+    first len(res) of them.  The series reads entry 0 (continue while below d), then
+    entry 1 (its first term), then alternately; the region coordinates are its first
+    entries and the rest are uniform on [0, 1].  Term k after the first multiplies the
+    previous range squared by a fresh uniform and survives with probability d^k, so with
+    G = (1/2) / (1 - d/3):
+      rd 1: E[f | x0] = 1 + [x0 < d] G
+      rd 2: E[f | x0, x1] = 1 + [x0 < d] (x1 + d G x1^2)."""
+    d = decay
+    G = 0.5 / (1.0 - d / 3.0)
+    if rd == 1:
+        (n,) = res
+        e = np.linspace(0, 1, n + 1)
+        below = np.clip(np.minimum(e[1:], d) - e[:-1], 0, None) * n  # share of the bin below d
+        return 1.0 + below * G
+    n0, n1 = res
+    e0, e1 = np.linspace(0, 1, n0 + 1), np.linspace(0, 1, n1 + 1)
+    below = np.clip(np.minimum(e0[1:], d) - e0[:-1], 0, None) * n0
+    m1 = (e1[1:] ** 2 - e1[:-1] ** 2) / 2 * n1  # mean of x1 over the bin
+    m2 = (e1[1:] ** 3 - e1[:-1] ** 3) / 3 * n1  # mean of x1^2
+    return 1.0 + below[:, None] * (m1 + d * G * m2)[None, :]
+
+
+@pytest.mark.parametrize("form,rd,res", [(0, 1, (8,)), (1, 1, (8,)), (0, 2, (4, 4)), (2, 2, (4, 4))])
+def test_fubini_infinite_oracle_is_unbiased_against_the_closed_form(form, rd, res):
+    """Mean over seeds of the oracle's bins against the closed form of the Dyson series:
+    pins the sequence order (region coordinates first), the node values' Monte Carlo and
+    both residual estimators (a dropped term, a swapped entry or a wrong weight moves
+    the mean by far more than the tolerance)."""
+    d = 0.75
+    truth = dyson_bins_closed_form(d, rd, res)
+    runs = np.array([O.integrate_fubini_inf(d, rd, res, 60, 2, 16, s, form=form, alpha=0.5, nthreads=4)[0]
+                     for s in range(64)])
+    se = runs.std(0, ddof=1) / np.sqrt(runs.shape[0])
+    flat = se == 0  # bins past the decay: the series is 1 there, every estimate exact
+    np.testing.assert_allclose(runs.mean(0)[flat], truth[flat], rtol=1e-6)
+    z = (runs.mean(0) - truth)[~flat] / se[~flat]
+    assert np.max(np.abs(z)) < 4.5 and abs(z.mean()) < 4.0 / np.sqrt(z.size) + 0.1, z
+    # the total: 1 + 3 d / (2 (3 - d)) = 1.5 at d = 0.75
+    assert abs(runs.mean() - 1.5) < 5 * runs.mean(axis=tuple(range(1, runs.ndim))).std() / np.sqrt(64)
+
+
+def test_stratified_estimator_is_biased_with_fewer_samples_than_regions():
+    """region_stratification_uniform with fewer samples than a bin's regions: a region
+    that draws no sample adds nothing to the residual (the weights spp / count cover the
+    sampled regions only), so the estimate is biased; with enough samples every region
+    is sampled and the bins are unbiased again.  This is the reference's estimator (the
+    fixtures above are its output); the GPU keeps it."""
+    d, res = 0.75, (2, 2)
+    truth = dyson_bins_closed_form(d, 2, res)
+    few = np.array([O.integrate_fubini_inf(d, 2, res, 300, 2, 16, 700 + s, form=1, nthreads=4)[0] for s in range(64)])
+    many = np.array([O.integrate_fubini_inf(d, 2, res, 300, 2, 400, 700 + s, form=1, nthreads=4)[0] for s in range(64)])
+    z_few = (few.mean(0) - truth) / (few.std(0, ddof=1) / 8)
+    z_many = (many.mean(0) - truth) / (many.std(0, ddof=1) / 8)
+    assert z_few.min() < -4.0, z_few
+    assert np.max(np.abs(z_many)) < 4.5, z_many
+
+
 @pytest.mark.skipif(not os.path.exists(O.REF_SO), reason="reference build absent (no /root/reference where this runs)")
 @pytest.mark.parametrize("seed", range(4))
 def test_oracle_matches_live_reference(seed):

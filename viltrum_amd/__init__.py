@@ -41,6 +41,8 @@ FORM_VR_OPTIMIZE_RR_ERROR_IMPORTANCE, FORM_VR_FIXED_RR_ERROR_IMPORTANCE = 13, 14
 FORM_VR_OPTIMIZE_RR_PDF_IMPORTANCE, FORM_VR_FIXED_RR_PDF_IMPORTANCE = 15, 16
 # with region_sampling_russian_roulette (importance or uniform sampling by a roulette)
 FORM_VR_OPTIMIZE_SAMPLING_RR, FORM_VR_FIXED_SAMPLING_RR = 17, 18
+# region_stratification_uniform (the optimised integrator; the Fubini form over an infinite range)
+FORM_VR_OPTIMIZE_STRATIFIED, FORM_VR_FIXED_STRATIFIED = 19, 20
 ORDER_CANONICAL, ORDER_REFERENCE = 0, 1
 REFINER_HEURISTIC, REFINER_LEARNED = 0, 1
 
@@ -374,6 +376,34 @@ def range_primary(D):
     return Range([0.0] * D, [1.0] * D)
 
 
+class RangeInfinite:
+    """A range of unbounded dimension (src/range-infinite.h:16-87): explicit entries
+    [lo[i], hi[i]], then [0, 1] for every entry after them."""
+    dims = integrands.INFINITE_DIM
+
+    def __init__(self, lo=(), hi=()):
+        self.lo = [float(v) for v in lo]
+        self.hi = [float(v) for v in hi]
+        if len(self.lo) != len(self.hi):
+            raise ValueError("lo/hi entries differ")
+
+    def min(self, i):
+        return self.lo[i] if i < len(self.lo) else 0.0
+
+    def max(self, i):
+        return self.hi[i] if i < len(self.hi) else 1.0
+
+
+def range_primary_infinite():
+    """[0, 1] in every entry (src/range-infinite.h:129-132)."""
+    return RangeInfinite()
+
+
+def range_infinite(lo, hi):
+    """Explicit entries [lo[i], hi[i]], [0, 1] after them (src/range-infinite.h:106-126)."""
+    return RangeInfinite(lo, hi)
+
+
 class Integrator:
     """Parameters of a region-based CV integrator (IntegratorRegionBased,
     src/newton-cotes/integrator-region-based.h:5-29)."""
@@ -429,15 +459,60 @@ class FubiniIntegrator(Integrator):
         self.rd = int(rd)
         self.mc_samples = int(mc_samples)
 
+    # seed of the node values' Monte Carlo: monte_carlo(mc_samples, 2 * seed + 1)
+    def mc_seed(self):
+        return 2 * self.seed + 1
+
     def prepare(self, f, rng):
+        if isinstance(rng, RangeInfinite) or f.dim == integrands.INFINITE_DIM:
+            if not (isinstance(rng, RangeInfinite) and f.dim == integrands.INFINITE_DIM):
+                raise ValueError("an integrand of a sequence goes with an infinite range")
+            rd = self.rd
+            lo = [rng.min(i) for i in range(rd)]
+            hi = [rng.max(i) for i in range(rd)]
+            g = integrands.fubini_infinite(f, rd, self.mc_samples, self.mc_seed(), lo, hi, rng.lo[rd:], rng.hi[rd:])
+            return g, Range(lo, hi)
         if rng.dims != f.dim:
             raise ValueError("range has %d dims, integrand %d" % (rng.dims, f.dim))
-        g = integrands.fubini(f, self.rd, self.mc_samples, 2 * self.seed + 1, rng.lo, rng.hi)
+        g = integrands.fubini(f, self.rd, self.mc_samples, self.mc_seed(), rng.lo, rng.hi)
         return g, Range(rng.lo[:self.rd], rng.hi[:self.rd])
 
 
 def integrator_crespo2021_infinite(rd, iterations, mc_samples, spp, seed=0, **kw):
     return FubiniIntegrator(rd, iterations, mc_samples, spp, seed, **kw)
+
+
+class FubiniOptimizedIntegrator(FubiniIntegrator):
+    """integrator_adaptive_fubini_variance_reduction_parallel_optimized<rd>(nested(simpson,
+    trapezoidal), error_heuristic_size(error_metric_relative(), 1e-5), iterations,
+    mc_samples, region_stratification_uniform(), cv_optimize_weight() or
+    cv_fixed_weight(alpha), region_sampling_uniform(), spp, seed)
+    (src/control-variates/integrator-adaptive-fubini-variance-reduction-optimized.h:17-23):
+    node values Monte Carlo from seed, residual samples split evenly over each bin's
+    regions from seed + 1.  The reference builds it over infinite ranges only."""
+
+    def __init__(self, rd, iterations, mc_samples, spp, seed=0, alpha=None, **kw):
+        super().__init__(rd, iterations, mc_samples, spp, seed, **kw)
+        self.form = FORM_VR_OPTIMIZE_STRATIFIED if alpha is None else FORM_VR_FIXED_STRATIFIED
+        self.alpha = 1.0 if alpha is None else float(alpha)
+
+    def mc_seed(self):
+        return self.seed
+
+    def prepare(self, f, rng):
+        if not isinstance(rng, RangeInfinite):
+            raise ValueError("the optimised Fubini integrator needs an infinite range (as the reference)")
+        return super().prepare(f, rng)
+
+    def problem(self, f, res, rng, scalar):
+        p = super().problem(f, res, rng, scalar)
+        p.seed = self.seed + 1
+        return p
+
+
+def integrator_adaptive_fubini_variance_reduction_parallel_optimized(rd, iterations, mc_samples, spp, seed=0,
+                                                                     alpha=None, **kw):
+    return FubiniOptimizedIntegrator(rd, iterations, mc_samples, spp, seed, alpha=alpha, **kw)
 
 
 def integrator_crespo2021(iterations, spp, seed=0, **kw):

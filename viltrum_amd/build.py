@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 SOURCES = [os.path.join(CSRC, s) for s in (
     "engine.hip", "batch_poly_f32_a.hip", "batch_poly_f32_b.hip", "batch_poly_f64_a.hip", "batch_poly_f64_b.hip",
     "batch_poly_6.hip", "batch_blobs.hip", "batch_lobes5.hip",
-    "batch_fubini.hip")]
+    "batch_fubini.hip", "batch_fubini_inf.hip")]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -48,6 +48,21 @@ int dispatch_T(vmx_ctx* ctx, int family, int D, const std::vector<JobIn>& jobs, 
                     return run_batch<T, 2, vmx::Fubini<T, vmx::Poly<T>, 3, 2>, Math>(ctx, jobs, st, pairs);
                 if (D == 1 && fam == VMX_FAMILY_POLY && dd == 2)
                     return run_batch<T, 1, vmx::Fubini<T, vmx::Poly<T>, 2, 1>, Math>(ctx, jobs, st, pairs);
+                // over an infinite range: the Dyson series, 1..4 region dims
+                if (fam == VMX_FAMILY_DYSON && dd == VMX_INFINITE_DIM) {
+                    if (D == 1)
+                        return run_batch<T, 1, vmx::Fubini<T, vmx::Dyson<T>, VMX_INFINITE_DIM, 1>, Math>(ctx, jobs, st,
+                                                                                                       pairs);
+                    if (D == 2)
+                        return run_batch<T, 2, vmx::Fubini<T, vmx::Dyson<T>, VMX_INFINITE_DIM, 2>, Math>(ctx, jobs, st,
+                                                                                                       pairs);
+                    if (D == 3)
+                        return run_batch<T, 3, vmx::Fubini<T, vmx::Dyson<T>, VMX_INFINITE_DIM, 3>, Math>(ctx, jobs, st,
+                                                                                                       pairs);
+                    if (D == 4)
+                        return run_batch<T, 4, vmx::Fubini<T, vmx::Dyson<T>, VMX_INFINITE_DIM, 4>, Math>(ctx, jobs, st,
+                                                                                                       pairs);
+                }
             } else {
                 throw VmxError(VMX_ERR_UNSUPPORTED, "Fubini form: FP32 only (as the reference)");
             }
@@ -102,10 +117,16 @@ void validate(const vmx_problem* p, const vmx_integrand* f) {
     if (p->regions < 1 || p->regions > (1ull << 26)) throw VmxError(VMX_ERR_INVALID, "regions must be 1..2^26");
     if (p->heuristic != VMX_HEUR_SIZE_RELATIVE && p->heuristic != VMX_HEUR_DEFAULT_ABSOLUTE)
         throw VmxError(VMX_ERR_INVALID, "unknown heuristic");
-    if (p->form < VMX_FORM_VR_OPTIMIZE || p->form > VMX_FORM_VR_FIXED_SAMPLING_RR)
+    if (p->form < VMX_FORM_VR_OPTIMIZE || p->form > VMX_FORM_VR_FIXED_STRATIFIED)
         throw VmxError(VMX_ERR_INVALID, "unknown form");
-    if (p->form >= VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL && f->family == VMX_FAMILY_FUBINI)
+    if (p->form >= VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL && p->form <= VMX_FORM_VR_FIXED_SAMPLING_RR &&
+        f->family == VMX_FAMILY_FUBINI)
         throw VmxError(VMX_ERR_UNSUPPORTED, "region roulettes: not for the Fubini form");
+    const bool infinite = f->family == VMX_FAMILY_FUBINI &&
+                          static_cast<const vmx_fubini_params*>(f->params)->inner_dim == VMX_INFINITE_DIM;
+    if (vmx::form_stratified(p->form) && !infinite)
+        throw VmxError(VMX_ERR_UNSUPPORTED,
+                       "stratified forms: the Fubini form over an infinite range only (as the reference)");
     if (vmx::form_importance(p->form) &&
         (p->scalar != VMX_F32 || f->dim > vmx::imp::MAX_DIM || f->family == VMX_FAMILY_FUBINI))
         throw VmxError(VMX_ERR_UNSUPPORTED, "importance sampling: FP32 integrands of dimension 1..5");
@@ -118,12 +139,28 @@ void validate(const vmx_problem* p, const vmx_integrand* f) {
     if (f->family == VMX_FAMILY_FUBINI) {
         const auto* fp = static_cast<const vmx_fubini_params*>(f->params);
         if (!fp->inner) throw VmxError(VMX_ERR_INVALID, "Fubini form: null inner parameters");
-        if (fp->inner_dim <= f->dim || fp->inner_dim > 5 || f->dim > 3)
-            throw VmxError(VMX_ERR_INVALID, "Fubini form: 1..3 region dims below the inner integrand's 2..5");
         if (fp->mc_samples < 1 || fp->mc_samples >= 0x80000000u)
             throw VmxError(VMX_ERR_INVALID, "Fubini form: mc_samples must be 1..2^31-1");
-        for (int d = f->dim; d < fp->inner_dim; ++d)
-            if (!(fp->rest_hi[d] > fp->rest_lo[d])) throw VmxError(VMX_ERR_INVALID, "Fubini form: rest range must have hi > lo");
+        if (infinite) {
+            if (fp->inner_family != VMX_FAMILY_DYSON)
+                throw VmxError(VMX_ERR_UNSUPPORTED, "Fubini form over an infinite range: the Dyson series only");
+            if (f->dim > 4) throw VmxError(VMX_ERR_INVALID, "Fubini form over an infinite range: 1..4 region dims");
+            if (fp->rest_count < 0 || fp->rest_count > VMX_MAX_DIM - f->dim)
+                throw VmxError(VMX_ERR_INVALID, "Fubini form over an infinite range: rest_count must be 0..6 - dim");
+            for (int j = 0; j < fp->rest_count; ++j)
+                if (!(fp->rest_hi[f->dim + j] > fp->rest_lo[f->dim + j]))
+                    throw VmxError(VMX_ERR_INVALID, "Fubini form: rest range must have hi > lo");
+            const double decay = static_cast<const vmx_dyson_params*>(fp->inner)->decay;
+            if (!(decay > 0.0 && decay < 1.0)) throw VmxError(VMX_ERR_INVALID, "Dyson series: decay must be in (0, 1)");
+        } else {
+            if (fp->inner_family == VMX_FAMILY_DYSON)
+                throw VmxError(VMX_ERR_INVALID, "Dyson series: only over an infinite range (inner_dim VMX_INFINITE_DIM)");
+            if (fp->inner_dim <= f->dim || fp->inner_dim > 5 || f->dim > 3)
+                throw VmxError(VMX_ERR_INVALID, "Fubini form: 1..3 region dims below the inner integrand's 2..5");
+            for (int d = f->dim; d < fp->inner_dim; ++d)
+                if (!(fp->rest_hi[d] > fp->rest_lo[d]))
+                    throw VmxError(VMX_ERR_INVALID, "Fubini form: rest range must have hi > lo");
+        }
         if (p->scalar != VMX_F32) throw VmxError(VMX_ERR_UNSUPPORTED, "Fubini form: FP32 only (as the reference)");
         if (p->refiner != VMX_REFINER_HEURISTIC)
             throw VmxError(VMX_ERR_UNSUPPORTED, "Fubini form: heuristic refinement only");

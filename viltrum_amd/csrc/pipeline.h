@@ -1410,7 +1410,12 @@ VMX_HD constexpr bool form_fixed(int f) {
     return f == VMX_FORM_VR_FIXED || f == VMX_FORM_VR_FIXED_IMPORTANCE || f == VMX_FORM_VR_FIXED_RR_INTEGRAL ||
            f == VMX_FORM_VR_FIXED_RR_ERROR || f == VMX_FORM_VR_FIXED_RR_PDF ||
            f == VMX_FORM_VR_FIXED_RR_INTEGRAL_IMPORTANCE || f == VMX_FORM_VR_FIXED_RR_ERROR_IMPORTANCE ||
-           f == VMX_FORM_VR_FIXED_RR_PDF_IMPORTANCE || f == VMX_FORM_VR_FIXED_SAMPLING_RR;
+           f == VMX_FORM_VR_FIXED_RR_PDF_IMPORTANCE || f == VMX_FORM_VR_FIXED_SAMPLING_RR ||
+           f == VMX_FORM_VR_FIXED_STRATIFIED;
+}
+/* region_stratification_uniform in place of the uniform region roulette */
+VMX_HD constexpr bool form_stratified(int f) {
+    return f == VMX_FORM_VR_OPTIMIZE_STRATIFIED || f == VMX_FORM_VR_FIXED_STRATIFIED;
 }
 
 template <typename T, int D, int DB, typename F, typename M, int MODE = 0>
@@ -1424,6 +1429,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((MODE 
     // weights) can; 0: uniform region and uniform point only
     constexpr bool IMP = (MODE & 1) != 0;
     constexpr bool RRI = (MODE & 2) != 0;
+    // the stratified split of the samples over the regions: the Fubini form over an
+    // infinite range only (the reference builds the optimised integrator for no other)
+    constexpr bool STRAT = IsFubiniInf<F>::value;
     constexpr bool FP32 = sizeof(T) == 4;
     constexpr int K = pow3(D);
     // per lane: first region and running region count of its first LANE_MAXF footprints
@@ -1582,6 +1590,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((MODE 
             }
             rr_total = cum;
         }
+        // region_stratification_uniform (region-stratification.h:11-24): sample s of the
+        // bin goes to region (start + s) mod n, so region i gets spp / n samples plus one
+        // when (i - start) mod n < spp % n -- the reference's counts -- and weight
+        // spp / count; start uniform in [0, n) from the bin's own Philox word
+        const bool strat = STRAT && form_stratified(jb.form);
+        uint32_t st_start = 0;
+        uint64_t st_q = 0, st_rem = 0;
+        if (strat && sampling) {
+            const U4 r0 = philox4x32_10(U4{0xFFFFFFFFu, uint32_t(b), uint32_t(b >> 32) ^ jb.stream, 0x53545254u},
+                                        jb.seed_lo, jb.seed_hi);
+            st_start = uniform_index(r0.x, n);
+            st_q = spp / n;
+            st_rem = spp % n;
+        }
         const bool imp_form = form_importance(jb.form);
         KahanSum<T> sf, sa, mf, ma, maa, mfa;
         double acc1 = 0.0;  // alpha = 1 form
@@ -1614,6 +1636,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((MODE 
                     pick = lo;
                     const double wk = rrw[o + pick] - (pick > 0 ? rrw[o + pick - 1] : 0.0);
                     rrfactor = rr_total / wk;
+                } else if (strat) {
+                    const uint64_t s = s0 + uint64_t(k);
+                    const uint64_t sm = s % n;
+                    pick = uint32_t((uint64_t(st_start) + sm) % n);
+                    rrfactor = double(spp) / double(st_q + (sm < st_rem ? 1u : 0u));
                 } else {
                     pick = uniform_index(uw, n);
                 }
@@ -1664,7 +1691,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((MODE 
                         }
                     }
                 }
-                const T fx = residual_eval<M, D>(f_lds, x);
+                const T fx = residual_eval<M, D>(f_lds, x, strat);
                 const T px = eval_monomials<T, D, FP32>(rec + 3 * D, t);
                 if (jb.form == VMX_FORM_CV_ALPHA1) {
                     if (!empty) acc1 += double(fx - px) * (a1w * double(vol));
@@ -1672,7 +1699,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((MODE 
                 }
                 T fsv, asv;
                 if constexpr (FP32) {  // weights in FP32: B * n * vol is O(n), far from the range limits
-                    const T w = (rr_form ? T(factor * rrfactor) : wn) * vol;
+                    const T w = (rr_form || strat ? T(factor * rrfactor) : wn) * vol;
                     fsv = fx * w;
                     asv = px * w;
                 } else {

@@ -18,6 +18,8 @@ FAMILY_POLY = 1
 FAMILY_BLOBS = 2
 FAMILY_LOBES5 = 3
 FAMILY_FUBINI = 4
+FAMILY_DYSON = 5
+INFINITE_DIM = -1  # the dimension of an integrand of an unbounded sequence
 
 
 class PolyParams(C.Structure):
@@ -62,7 +64,12 @@ class FubiniParams(C.Structure):
         ("rest_lo", C.c_double * MAX_DIM),
         ("rest_hi", C.c_double * MAX_DIM),
         ("inner", C.c_void_p),
+        ("rest_count", C.c_int32),
     ]
+
+
+class DysonParams(C.Structure):
+    _fields_ = [("decay", C.c_double)]
 
 
 class IntegrandDesc(C.Structure):
@@ -157,6 +164,44 @@ def fubini(inner, rd, mc_samples, seed, rest_lo, rest_hi, name=None):
     p.inner = C.cast(C.pointer(inner.params), C.c_void_p)
     f = Integrand(FAMILY_FUBINI, rd, p, name or "fubini(%s)" % inner.name)
     f.inner = inner  # keeps the inner parameter block alive
+    return f
+
+
+def dyson(decay=0.75, name="dyson"):
+    """The Dyson series of a sequence (the reference's infinite-range example integrand,
+    main/compilation-tests/crespo21.cc:6-25): sum = 1, range = 1; while the current entry
+    is below `decay`: step, x = range * entry, sum += x * range, range = x; step.  Its
+    integral over the primary infinite range is 1 + 3 decay / (2 (3 - decay))."""
+    if not 0.0 < decay < 1.0:
+        raise ValueError("decay must be in (0, 1)")
+    return Integrand(FAMILY_DYSON, INFINITE_DIM, DysonParams(float(decay)), name)
+
+
+def fubini_infinite(inner, rd, mc_samples, seed, lo, hi, rest_lo=(), rest_hi=(), name=None):
+    """The Fubini form over an infinite range: regions and bins over [lo, hi] (rd dims, the
+    range's first entries), the sequence after them Monte Carlo, entry j in
+    [rest_lo[j], rest_hi[j]] for the explicit entries and in [0, 1] after them."""
+    if inner.family != FAMILY_DYSON:
+        raise ValueError("an integrand of a sequence (the Dyson series) is needed")
+    if not 1 <= rd <= 4:
+        raise ValueError("region dims must be 1..4")
+    if len(rest_lo) != len(rest_hi) or rd + len(rest_lo) > MAX_DIM:
+        raise ValueError("at most %d explicit rest entries" % (MAX_DIM - rd))
+    p = FubiniParams()
+    p.inner_family = inner.family
+    p.inner_dim = INFINITE_DIM
+    p.mc_samples = int(mc_samples)
+    p.seed = int(seed) & 0xFFFFFFFF
+    for d in range(rd):
+        p.rest_lo[d] = lo[d]
+        p.rest_hi[d] = hi[d]
+    for j in range(len(rest_lo)):
+        p.rest_lo[rd + j] = rest_lo[j]
+        p.rest_hi[rd + j] = rest_hi[j]
+    p.rest_count = len(rest_lo)
+    p.inner = C.cast(C.pointer(inner.params), C.c_void_p)
+    f = Integrand(FAMILY_FUBINI, rd, p, name or "fubini_infinite(%s)" % inner.name)
+    f.inner = inner
     return f
 
 
@@ -292,6 +337,8 @@ def flops_per_eval(f):
     counts exp/sin/cos calls, never converted to flops."""
     if f.family == FAMILY_FUBINI:  # a residual sample evaluates the inner integrand once
         return flops_per_eval(f.inner)
+    if f.family == FAMILY_DYSON:  # per term: compare, multiply, divide, multiply-add
+        return 5, 0
     if f.family == FAMILY_BLOBS:
         p = f.params
         flops = 14 * p.nblobs + 1  # per blob: 6 (centre, dx, dy) + 8 (quadratic form, exp arg, w*, s+)
