@@ -189,7 +189,9 @@ def run_config4(args, V, hip):
     (RMSE ~ 1/sqrt(spp) from a 16-spp probe, then checked and stepped) and the faster
     of the budgets.  A step is one learned+batched pass over the 64 integrals at its
     chosen (budget, spp); value = their bins / step time.  The plain path at its chosen
-    point is timed the same way beside it."""
+    point is timed the same way beside it, and so are the two changes apart: batching
+    with the heuristic priority at plain's point, and the learned priority one call per
+    integral at its own chosen point."""
     from viltrum_amd import integrands as I
     from viltrum_amd import refiner as R
     ctx = V.Context(0)
@@ -290,8 +292,11 @@ def run_config4(args, V, hip):
 
     plain_pts = choose(run_plain, False)
     learn_pts = choose(run_batch, True)
+    # the learned priority alone (one call per integral) at its own equal-RMSE point
+    lone_pts = choose(run_plain, True)
     bp = min(plain_pts.values(), key=lambda x: x["ms"])
     bl = min(learn_pts.values(), key=lambda x: x["ms"])
+    bo = min(lone_pts.values(), key=lambda x: x["ms"])
     p_learn = problems(bl["regions"], bl["spp"], True)
     ms_learn, st = timed(run_batch, p_learn, args.steps)
     rmse_learn = rmse()
@@ -299,9 +304,12 @@ def run_config4(args, V, hip):
     ms_plain, st_plain = timed(run_plain, p_plain, max(1, args.steps // 2))
     rmse_plain = rmse()
     # the two changes apart: batching alone (heuristic priority, plain's point) and the
-    # learned priority alone (one call per integral, learned's point)
+    # learned priority alone (one call per integral, at its own point and at the batched
+    # path's point)
     ms_bheur, _ = timed(run_batch, p_plain, max(1, args.steps // 2))
-    ms_lone, _ = timed(run_plain, p_learn, max(1, args.steps // 2))
+    ms_lone, _ = timed(run_plain, problems(bo["regions"], bo["spp"], True), max(1, args.steps // 2))
+    rmse_lone = rmse()
+    ms_lone_b, _ = timed(run_plain, p_learn, max(1, args.steps // 2))
     total_bins = nb * len(fs)
     value = total_bins / (ms_learn * 1e-3)
     plain_bps = total_bins / (ms_plain * 1e-3)
@@ -323,8 +331,12 @@ def run_config4(args, V, hip):
         "speedup_vs_plain": value / plain_bps,
         "batched_heuristic_bins_per_s": total_bins / (ms_bheur * 1e-3),
         "learned_unbatched_bins_per_s": total_bins / (ms_lone * 1e-3),
-        "rmse_vs_fp64": {"target": target, "learned_batched": rmse_learn, "plain": rmse_plain},
-        "sweep": {"plain": list(plain_pts.values()), "learned_batched": list(learn_pts.values())},
+        "learned_unbatched_point": {"regions": bo["regions"], "spp": bo["spp"]},
+        "learned_unbatched_at_batched_point_bins_per_s": total_bins / (ms_lone_b * 1e-3),
+        "rmse_vs_fp64": {"target": target, "learned_batched": rmse_learn, "plain": rmse_plain,
+                         "learned_unbatched": rmse_lone},
+        "sweep": {"plain": list(plain_pts.values()), "learned_batched": list(learn_pts.values()),
+                  "learned_unbatched": list(lone_pts.values())},
         "fp64_truth": {"kind": "FP64 closed-form per-bin integrals (erf, polygon area, separable wave)"},
         "evals_per_s": evals / (ms_learn * 1e-3),
         "phase_ms": {k: float(getattr(st[0], k)) for k in ("ms_generate", "ms_select", "ms_stratify", "ms_bins",
@@ -654,6 +666,43 @@ def cpu_baseline(ws, nthreads, budget_s=10.0, min_runs=3):
                                               w0["spp"], total_t)}
 
 
+def run_mlp(args, V, hip):
+    """The refinement network's kernels alone on synthetic BF16 feature rows: k_mlp_forward
+    (inference) and k_mlp_grad (one backward pass, 64-row tiles), at dims 2 and 5.  Prints
+    their MFMA work per row (the v_mfma_f32_16x16x32_bf16 tiles each issues) for a
+    roofline against a kernel trace; wall times include the host round trip of the C
+    entry points."""
+    ctx = V.Context(0)
+    n = args.mlp_rows
+    out = {"workload": "mlp", "rows": n, "steps": args.steps, "dims": {}}
+    W = 64
+    for D in (2, 5):
+        P, F, kp = V.refiner_size(D)
+        rng = np.random.default_rng(100 + D)
+        p = rng.normal(0.0, 0.05, P).astype(np.float32)
+        p[-2], p[-1] = -7.0, 3.0  # target offset and scale
+        x = np.zeros((n, kp), dtype=np.float32)
+        x[:, :F] = rng.normal(0.0, 0.6, (n, F))
+        X = (x.view(np.uint32) >> 16).astype(np.uint16)  # BF16 by truncation: values do not matter here
+        t = rng.normal(-7.0, 3.0, n).astype(np.float32)
+        ctx.refiner_forward(D, X, params=p)
+        ctx.refiner_grad(D, p, X, t)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.refiner_forward(D, X, params=p)
+        tf = (time.perf_counter() - t0) / args.steps
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.refiner_grad(D, p, X, t)
+        tg = (time.perf_counter() - t0) / args.steps
+        fwd = 2 * W * (kp + 2 * W)  # three MFMA layers: kp -> 64 -> 64 -> 64
+        grad = fwd + 2 * 2 * 2 * W * W + 2 * kp * W  # forward, dW and g_in of layers 2 and 1, dW of layer 0
+        out["dims"][str(D)] = {"kp": kp, "fwd_mfma_flop_per_row": fwd, "grad_mfma_flop_per_row": grad,
+                               "fwd_mfma_flop_per_launch": fwd * n, "grad_mfma_flop_per_launch": grad * n,
+                               "wall_ms_forward": tf * 1e3, "wall_ms_grad": tg * 1e3}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -675,6 +724,7 @@ def main():
     ap.add_argument("--save-net", default="", metavar="PATH", help="config4, config5: save the trained network")
     ap.add_argument("--load-net", default="", metavar="PATH",
                     help="config4, config5: load the network instead of training it (e.g. for counter passes)")
+    ap.add_argument("--mlp-rows", type=int, default=1 << 20, help="mlp: feature rows per launch (<= 2^20)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device (a known-size kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -705,6 +755,8 @@ def main():
         return run_fubini(args, V, hip)
     if args.workload == "importance":
         return run_importance(args, V, hip)
+    if args.workload == "mlp":
+        return run_mlp(args, V, hip)
     w = workloads()[args.workload]
     f = w["f"]
     ctx = V.Context(0)

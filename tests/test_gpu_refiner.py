@@ -30,24 +30,66 @@ def test_forward_matches_oracle(ctx, D, n):
     assert np.max(diff) < 2e-2 * (1 + np.max(np.abs(ye)))
 
 
-@pytest.mark.parametrize("D,n", [(2, 200), (3, 64), (5, 130)])
-def test_grad_matches_oracle(ctx, D, n):
+LAYERS = [("w0", "b0"), ("b0", "w1"), ("w1", "b1"), ("b1", "w2"), ("w2", "b2"), ("b2", "w3"), ("w3", "b3")]
+
+
+def _problem(D, n, rep):
     F, kp = O.nn_dims(D)
-    p = random_net(kp, F, 70 + D)
-    X = random_rows(n, kp, F, 80 + D)
-    t = np.random.default_rng(90 + D).normal(-7, 3, n).astype(np.float32)
+    p = random_net(kp, F, 70 + D + 100 * rep)
+    X = random_rows(n, kp, F, 80 + D + 100 * rep)
+    t = np.random.default_rng(90 + D + 100 * rep).normal(-7, 3, n).astype(np.float32)
+    return kp, p, X, t
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("D,n", [(2, 200), (3, 64), (5, 130), (4, 333), (2, 1000)])
+@pytest.mark.parametrize("rep", range(3))
+def test_grad_matches_oracle(ctx, D, n, rep):
+    """The MFMA backward pass against the oracle that rounds to BF16 where the kernels do
+    (per layer, relative norm): measured at most 2.4e-7 over these 15 cases, so 1e-6
+    leaves a factor of 4 and catches any dropped or doubled row (>= 1/n).  Against the
+    unrounded oracle the BF16 operands alone account for up to 0.086."""
+    kp, p, X, t = _problem(D, n, rep)
     g, loss = ctx.refiner_grad(D, p, X, t)
     ge, le = O.nn_grad(kp, p, X, t, emulate=True)
     gx, lx = O.nn_grad(kp, p, X, t, emulate=False)
-    assert abs(loss - le) < 1e-4 * le
+    assert abs(loss - le) < 1e-6 * le
     L = layout(kp)
-    for a, b in [("w0", "b0"), ("b0", "w1"), ("w1", "b1"), ("b1", "w2"), ("w2", "b2"), ("b2", "w3"), ("w3", "b3")]:
-        ref = ge[L[a]:L[b]]
-        d = np.linalg.norm(g[L[a]:L[b]] - ref) / np.linalg.norm(ref)
-        assert d < 0.05, (a, d)
-        dx = np.linalg.norm(g[L[a]:L[b]] - gx[L[a]:L[b]]) / np.linalg.norm(gx[L[a]:L[b]])
-        assert dx < 0.2, (a, dx)
-    assert abs(g[L["b3"]] - ge[L["b3"]]) < 1e-4 * (1 + abs(ge[L["b3"]]))
+    for a, b in LAYERS:
+        d = _rel(g[L[a]:L[b]], ge[L[a]:L[b]])
+        assert d < 1e-6, (a, d)
+        dx = _rel(g[L[a]:L[b]], gx[L[a]:L[b]])
+        assert dx < 0.25, (a, dx)
+    assert abs(g[L["b3"]] - ge[L["b3"]]) < 1e-6 * (1 + abs(ge[L["b3"]]))
+
+
+@pytest.mark.parametrize("D,n", [(2, 200), (5, 130), (4, 333), (2, 1000)])
+@pytest.mark.parametrize("rep", range(3))
+def test_grad_partial_tile(ctx, D, n, rep):
+    """Rows go through the backward pass in tiles of 64: with n = 64k + r rows,
+    n grad(n) - 64k grad(64k) is the sum of the last r rows' gradients, the partial
+    tile's share.  It matches the oracle's same difference to at most 7.8e-5 per layer
+    (measured over these 12 cases; bound 3e-4), and is the gradient of those r rows
+    (the oracle on them alone, up to the BF16 rounding of the 1/n-scaled output
+    gradient: at most 0.062).  A partial tile dropped or counted twice is off by 1."""
+    kp, p, X, t = _problem(D, n, rep)
+    m = (n // 64) * 64
+    r = n - m
+    assert 0 < r < 64 and m > 0
+    g, _ = ctx.refiner_grad(D, p, X, t)
+    gm, _ = ctx.refiner_grad(D, p, X[:m], t[:m])
+    diff = g.astype(np.float64) * n - gm.astype(np.float64) * m
+    ge, _ = O.nn_grad(kp, p, X, t, emulate=True)
+    gem, _ = O.nn_grad(kp, p, X[:m], t[:m], emulate=True)
+    dife = ge * n - gem * m
+    gr, _ = O.nn_grad(kp, p, X[m:], t[m:], emulate=True)
+    L = layout(kp)
+    for a, b in LAYERS:
+        assert _rel(diff[L[a]:L[b]], dife[L[a]:L[b]]) < 3e-4, a
+        assert _rel(diff[L[a]:L[b]], r * gr[L[a]:L[b]]) < 0.2, a
 
 
 def _codes(lo, hi, rlo, rhi):

@@ -1035,6 +1035,42 @@ struct FootArrays {
     uint32_t* npairs;    // [F+1] bins in the pixel box (exclusive-scanned into pair offsets)
 };
 
+/* the residual estimator's parts per form: region roulette (0 uniform, 1 rr_integral_region,
+   2 rr_error_region, 3 rr_pdf_region), point sampler, fixed weight */
+VMX_HD constexpr int form_roulette(int f) {
+    return (f == VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL || f == VMX_FORM_VR_FIXED_RR_INTEGRAL ||
+            f == VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL_IMPORTANCE || f == VMX_FORM_VR_FIXED_RR_INTEGRAL_IMPORTANCE)
+               ? 1
+           : (f == VMX_FORM_VR_OPTIMIZE_RR_ERROR || f == VMX_FORM_VR_FIXED_RR_ERROR ||
+              f == VMX_FORM_VR_OPTIMIZE_RR_ERROR_IMPORTANCE || f == VMX_FORM_VR_FIXED_RR_ERROR_IMPORTANCE)
+               ? 2
+           : (f == VMX_FORM_VR_OPTIMIZE_RR_PDF || f == VMX_FORM_VR_FIXED_RR_PDF ||
+              f == VMX_FORM_VR_OPTIMIZE_RR_PDF_IMPORTANCE || f == VMX_FORM_VR_FIXED_RR_PDF_IMPORTANCE)
+               ? 3
+               : 0;
+}
+/* the point sampler draws from the region polynomial's pdf (always, or by the roulette
+   between strategies of region_sampling_russian_roulette) */
+VMX_HD constexpr bool form_sampling_rr(int f) {
+    return f == VMX_FORM_VR_OPTIMIZE_SAMPLING_RR || f == VMX_FORM_VR_FIXED_SAMPLING_RR;
+}
+VMX_HD constexpr bool form_importance(int f) {
+    return f == VMX_FORM_VR_OPTIMIZE_IMPORTANCE || f == VMX_FORM_VR_FIXED_IMPORTANCE ||
+           (f >= VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL_IMPORTANCE && f <= VMX_FORM_VR_FIXED_RR_PDF_IMPORTANCE) ||
+           form_sampling_rr(f);
+}
+VMX_HD constexpr bool form_fixed(int f) {
+    return f == VMX_FORM_VR_FIXED || f == VMX_FORM_VR_FIXED_IMPORTANCE || f == VMX_FORM_VR_FIXED_RR_INTEGRAL ||
+           f == VMX_FORM_VR_FIXED_RR_ERROR || f == VMX_FORM_VR_FIXED_RR_PDF ||
+           f == VMX_FORM_VR_FIXED_RR_INTEGRAL_IMPORTANCE || f == VMX_FORM_VR_FIXED_RR_ERROR_IMPORTANCE ||
+           f == VMX_FORM_VR_FIXED_RR_PDF_IMPORTANCE || f == VMX_FORM_VR_FIXED_SAMPLING_RR ||
+           f == VMX_FORM_VR_FIXED_STRATIFIED;
+}
+/* region_stratification_uniform in place of the uniform region roulette */
+VMX_HD constexpr bool form_stratified(int f) {
+    return f == VMX_FORM_VR_OPTIMIZE_STRATIFIED || f == VMX_FORM_VR_FIXED_STRATIFIED;
+}
+
 template <typename T, int D>
 __global__ void k_prepare2(const Job<T>* __restrict__ jobs, const int32_t* __restrict__ leaf_slot, TreeArrays<T> t,
                            RegionArrays<T> r, uint32_t n, int mb, int merge) {
@@ -1090,7 +1126,9 @@ __global__ void k_prepare2(const Job<T>* __restrict__ jobs, const int32_t* __res
     for (int k = 3 * D + K; k < samp_stride<D>(); ++k) rec[k] = T(0);
     r.job[i] = j;
     uint32_t flag = 1;
-    if (merge && i != jb.reg_base && leaf_slot[i - 1] >= 0) {
+    // the region roulettes weigh each region of a bin: their jobs keep one region per
+    // footprint, the batch's other jobs merge as they would alone
+    if (merge && form_roulette(jb.form) == 0 && i != jb.reg_base && leaf_slot[i - 1] >= 0) {
         const size_t sp = size_t(leaf_slot[i - 1]);
         flag = 0;
         for (int d = 0; d < jb.db; ++d)
@@ -1131,7 +1169,7 @@ __global__ void k_footprints(const Job<T>* __restrict__ jobs, RegionArrays<T> r,
         const uint32_t* bx = f.box + size_t(id) * 6;
         f.npairs[id] = (bx[1] - bx[0]) * (bx[3] - bx[2]) * (bx[5] - bx[4]);
     }
-    if (merge) {
+    if (merge && form_roulette(jb.form) == 0) {
         double acc[27];
         for (int k = 0; k < mb; ++k) acc[k] = 0.0;
         for (uint32_t q = i; q < e; ++q) {
@@ -1382,42 +1420,6 @@ __device__ inline uint32_t xcd_tile(uint32_t w, uint32_t G) {
    is summed plainly and the group sums are Kahan-compensated in the evaluation type,
    then finished in FP64.  Summation depends on
    the job alone, so a batched integral gives exactly what it gives alone. */
-/* the residual estimator's parts per form: region roulette (0 uniform, 1 rr_integral_region,
-   2 rr_error_region, 3 rr_pdf_region), point sampler, fixed weight */
-VMX_HD constexpr int form_roulette(int f) {
-    return (f == VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL || f == VMX_FORM_VR_FIXED_RR_INTEGRAL ||
-            f == VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL_IMPORTANCE || f == VMX_FORM_VR_FIXED_RR_INTEGRAL_IMPORTANCE)
-               ? 1
-           : (f == VMX_FORM_VR_OPTIMIZE_RR_ERROR || f == VMX_FORM_VR_FIXED_RR_ERROR ||
-              f == VMX_FORM_VR_OPTIMIZE_RR_ERROR_IMPORTANCE || f == VMX_FORM_VR_FIXED_RR_ERROR_IMPORTANCE)
-               ? 2
-           : (f == VMX_FORM_VR_OPTIMIZE_RR_PDF || f == VMX_FORM_VR_FIXED_RR_PDF ||
-              f == VMX_FORM_VR_OPTIMIZE_RR_PDF_IMPORTANCE || f == VMX_FORM_VR_FIXED_RR_PDF_IMPORTANCE)
-               ? 3
-               : 0;
-}
-/* the point sampler draws from the region polynomial's pdf (always, or by the roulette
-   between strategies of region_sampling_russian_roulette) */
-VMX_HD constexpr bool form_sampling_rr(int f) {
-    return f == VMX_FORM_VR_OPTIMIZE_SAMPLING_RR || f == VMX_FORM_VR_FIXED_SAMPLING_RR;
-}
-VMX_HD constexpr bool form_importance(int f) {
-    return f == VMX_FORM_VR_OPTIMIZE_IMPORTANCE || f == VMX_FORM_VR_FIXED_IMPORTANCE ||
-           (f >= VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL_IMPORTANCE && f <= VMX_FORM_VR_FIXED_RR_PDF_IMPORTANCE) ||
-           form_sampling_rr(f);
-}
-VMX_HD constexpr bool form_fixed(int f) {
-    return f == VMX_FORM_VR_FIXED || f == VMX_FORM_VR_FIXED_IMPORTANCE || f == VMX_FORM_VR_FIXED_RR_INTEGRAL ||
-           f == VMX_FORM_VR_FIXED_RR_ERROR || f == VMX_FORM_VR_FIXED_RR_PDF ||
-           f == VMX_FORM_VR_FIXED_RR_INTEGRAL_IMPORTANCE || f == VMX_FORM_VR_FIXED_RR_ERROR_IMPORTANCE ||
-           f == VMX_FORM_VR_FIXED_RR_PDF_IMPORTANCE || f == VMX_FORM_VR_FIXED_SAMPLING_RR ||
-           f == VMX_FORM_VR_FIXED_STRATIFIED;
-}
-/* region_stratification_uniform in place of the uniform region roulette */
-VMX_HD constexpr bool form_stratified(int f) {
-    return f == VMX_FORM_VR_OPTIMIZE_STRATIFIED || f == VMX_FORM_VR_FIXED_STRATIFIED;
-}
-
 template <typename T, int D, int DB, typename F, typename M, int MODE = 0>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((MODE & 1) == 0 && D <= 3 && sizeof(T) == 4 ? 4 : 1))) k_bins1(const Job<T>* __restrict__ jobs, uint32_t J, uint64_t total_bins,
                                                RegionArrays<T> r, FootArrays<T> fa,

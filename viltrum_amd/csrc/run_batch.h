@@ -39,13 +39,14 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     for (const JobIn& x : in)
         if ((x.given != nullptr) != replay) throw VmxError(VMX_ERR_INVALID, "given regions for all or none");
     if (replay && pairs_req) throw VmxError(VMX_ERR_INVALID, "training pairs need generation");
-    // the region roulettes weigh each region of a bin: one footprint per region
+    // the region roulettes weigh each region of a bin: one footprint per region for
+    // their jobs (k_prepare2 and k_footprints decide per job)
     bool any_rr = false, any_rr_err = false;
     for (const JobIn& x : in) {
         any_rr |= vmx::form_roulette(x.p->form) != 0;
         any_rr_err |= vmx::form_roulette(x.p->form) == 2;
     }
-    const bool merge = order == VMX_ORDER_CANONICAL && !replay && !any_rr;
+    const bool merge = order == VMX_ORDER_CANONICAL && !replay;
 
     std::vector<vmx::Job<T>> jobs(J);
     std::vector<P> fparams(J);
