@@ -45,6 +45,7 @@ struct Job {
     uint32_t seed_lo, seed_hi, stream;
     uint32_t N, node_base, node_cap, reg_base;
     uint32_t tile_base;  // first tile (of 256 bins) of this job in the per-bin kernel's grid
+    uint32_t tile_base2;  // first tile of this job in the region-major per-bin kernel's grid
     int32_t db, mb, form, heur;
     int32_t learn;   // refinement priority from the network (refiner.h)
     int32_t lfloor;  // with learn: every dim is first halved lfloor times, widest first
@@ -1254,10 +1255,14 @@ __global__ void k_bin_pass(const Job<T>* __restrict__ jobs, FootArrays<T> f, con
 
 constexpr uint32_t SORT_SHORT = 16;  // longer lists are sorted by a workgroup each
 constexpr int LANE_MAXF = 16;  // footprints per bin the per-bin kernel caches in LDS
+constexpr int RM_MAXF = 8;     // footprints per bin the region-major per-bin kernel caches in LDS
+constexpr int PREFIX_MIN = 8;  // lists longer than this store their prefix counts
+static_assert(LANE_MAXF >= PREFIX_MIN && RM_MAXF >= PREFIX_MIN,
+              "a list a kernel does not cache whole in LDS has its prefix counts stored");
 
-/* For lists longer than LANE_MAXF the per-bin kernel finds a sample's region by binary
-   search over the list's exclusive prefix region counts `pcum` and first regions
-   `pfirst`, stored beside the list. */
+/* For lists longer than PREFIX_MIN the per-bin kernels find a sample's region beyond
+   their LDS cache by binary search over the list's exclusive prefix region counts
+   `pcum` and first regions `pfirst`, stored beside the list. */
 struct ListPrefix {
     uint32_t* pcum;
     uint32_t* pfirst;
@@ -1295,7 +1300,7 @@ __global__ void __launch_bounds__(256) k_sort_lists(const uint32_t* __restrict__
     }
     if (moved)
         for (uint32_t i = 0; i < n; ++i) lst[o + i] = sv[i][tid];
-    if (n > uint32_t(LANE_MAXF)) {
+    if (n > uint32_t(PREFIX_MIN)) {
         uint32_t c = 0;
         for (uint32_t i = 0; i < n; ++i) {
             const uint32_t fi = sv[i][tid];
@@ -1420,12 +1425,33 @@ __device__ inline uint32_t xcd_tile(uint32_t w, uint32_t G) {
    is summed plainly and the group sums are Kahan-compensated in the evaluation type,
    then finished in FP64.  Summation depends on
    the job alone, so a batched integral gives exactly what it gives alone. */
+/* The region-major per-bin kernel (bins_rm.h) and the jobs it takes. */
+constexpr int RM_SPB = 16;             // samples per bin per pass
+constexpr int RM_NS = 256 * RM_SPB;    // samples per pass of a block
+constexpr int RM_SLOT_BITS = 12;       // log2(RM_NS)
+constexpr uint32_t RM_MAX_REGIONS = 1u << (32 - RM_SLOT_BITS);  // region ids of a job in a key
+constexpr uint32_t RM_NONE = 0xFFFFFFFFu;
+
+/* Whether a job runs through k_bins_rm (the host launches it for FP32 regions of 4..6
+   dims; k_bins1 skips these jobs then). */
+VMX_HD inline bool rm_job(int form, uint32_t N) {
+    return (form == VMX_FORM_VR_OPTIMIZE || form == VMX_FORM_VR_FIXED || form == VMX_FORM_CV_ALPHA1) &&
+           N < RM_MAX_REGIONS;  // a key never equals RM_NONE
+}
+
+/* Tiles of the region-major kernel for a job: 16 x 16 bins with two bin dims, else runs
+   of 256 bins. */
+inline uint32_t rm_tiles(int db, const uint64_t* res, uint64_t nbins) {
+    if (db == 2) return uint32_t(((res[0] + 15) / 16) * ((res[1] + 15) / 16));
+    return uint32_t((nbins + 255) / 256);
+}
+
 template <typename T, int D, int DB, typename F, typename M, int MODE = 0>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((MODE & 1) == 0 && D <= 3 && sizeof(T) == 4 ? 4 : 1))) k_bins1(const Job<T>* __restrict__ jobs, uint32_t J, uint64_t total_bins,
                                                RegionArrays<T> r, FootArrays<T> fa,
                                                const uint32_t* __restrict__ off, const uint32_t* __restrict__ lst,
                                                const F* __restrict__ fs, JobRt* rt, ListPrefix lp, int xcd,
-                                               double* __restrict__ rrw) {
+                                               double* __restrict__ rrw, int rm) {
     // MODE bit 0: importance sampling in the box can occur; bit 1: the region roulettes
     // (rr_integral_region, rr_error_region, rr_pdf_region; rrw: per-list cumulative
     // weights) can; 0: uniform region and uniform point only
@@ -1458,6 +1484,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((MODE 
         j = __builtin_amdgcn_readfirstlane(lo);
     }
     const Job<T>& jb = jobs[j];
+    if (rm && rm_job(jb.form, jb.N)) return;  // block-uniform: k_bins_rm runs this job
     {
         const unsigned char* src = reinterpret_cast<const unsigned char*>(&fs[j]);
         for (int i = tid; i < int(sizeof(F) / 4); i += blockDim.x)
@@ -1773,5 +1800,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((MODE 
 }
 
 }  // namespace vmx
+
+#include "bins_rm.h"
 
 #endif

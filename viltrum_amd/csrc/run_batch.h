@@ -50,7 +50,8 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
 
     std::vector<vmx::Job<T>> jobs(J);
     std::vector<P> fparams(J);
-    uint64_t total_bins = 0, n_reg = 0, total_tiles = 0;
+    uint64_t total_bins = 0, n_reg = 0, total_tiles = 0, total_tiles2 = 0;
+    bool any_rm = false;  // a job the region-major per-bin kernel takes
     for (uint32_t j = 0; j < J; ++j) {
         const vmx_problem& p = *in[j].p;
         vmx::Job<T>& jb = jobs[j];
@@ -74,6 +75,8 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         total_bins += jb.nbins;
         jb.tile_base = uint32_t(total_tiles);
         total_tiles += (jb.nbins + 255) / 256;
+        jb.tile_base2 = uint32_t(total_tiles2);
+        total_tiles2 += vmx::rm_tiles(jb.db, jb.res, jb.nbins);
         jb.spp = p.spp;
         jb.bins = in[j].bins;
         jb.alpha = p.alpha;
@@ -92,6 +95,17 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         jb.lfloor = jb.learn ? std::max(0, (ilog2(p.regions) - 6) / D) : 0;
         for (int d = 0; d < VMX_MAX_DIM; ++d) jb.lres[d] = d < jb.db ? float(std::log2(double(p.res[d]))) : 0.f;
         fparams[j] = P::from(*static_cast<const typename ParamOf<P>::type*>(in[j].params));
+    }
+    // FP32 regions of 4..6 dims with the uniform estimator go through the region-major
+    // per-bin kernel (bins_rm.h); k_bins1 takes the batch's other jobs
+    constexpr bool RM_KIND = std::is_same<T, float>::value && D >= 4 && !vmx::IsFubini<P>::value;
+    bool all_rm = RM_KIND;
+    if constexpr (RM_KIND) {
+        for (const auto& jb : jobs) {
+            const bool r = vmx::rm_job(jb.form, jb.N);
+            any_rm |= r;
+            all_rm &= r;
+        }
     }
     bool learn = false;
     for (const auto& jb : jobs) learn |= jb.learn != 0;
@@ -645,18 +659,34 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     const int db = jobs[0].db;
     // the importance-sampling variant is its own instantiation: its register footprint
     // stays out of the uniform kernel
+    const int rm = any_rm ? 1 : 0;
     auto bins_kernel = [&](auto mode_tag) {
         constexpr int MODE = decltype(mode_tag)::value;
+        if constexpr (RM_KIND) {
+            if (any_rm) {
+                const unsigned grid2 = unsigned(total_tiles2);
+                if (db == 1)
+                    vmx::k_bins_rm<T, D, 1, P, Math><<<grid2, 256, 0, st>>>(d_jobs, J, ra, fa, d_off, d_lst, d_f, d_rt,
+                                                                             lp);
+                else if (db == 2)
+                    vmx::k_bins_rm<T, D, 2, P, Math><<<grid2, 256, 0, st>>>(d_jobs, J, ra, fa, d_off, d_lst, d_f, d_rt,
+                                                                             lp);
+                else
+                    vmx::k_bins_rm<T, D, 3, P, Math><<<grid2, 256, 0, st>>>(d_jobs, J, ra, fa, d_off, d_lst, d_f, d_rt,
+                                                                             lp);
+            }
+        }
+        if (all_rm) return;
         if (db == 1)
             vmx::k_bins1<T, D, 1, P, Math, MODE><<<bins_grid, 256, 0, st>>>(
-                d_jobs, J, total_bins, ra, fa, d_off, d_lst, d_f, d_rt, lp, shape.bins_xcd, d_rrw);
+                d_jobs, J, total_bins, ra, fa, d_off, d_lst, d_f, d_rt, lp, shape.bins_xcd, d_rrw, rm);
         else if constexpr (D >= 2) {
             if (db == 2)
                 vmx::k_bins1<T, D, 2, P, Math, MODE><<<bins_grid, 256, 0, st>>>(
-                    d_jobs, J, total_bins, ra, fa, d_off, d_lst, d_f, d_rt, lp, shape.bins_xcd, d_rrw);
+                    d_jobs, J, total_bins, ra, fa, d_off, d_lst, d_f, d_rt, lp, shape.bins_xcd, d_rrw, rm);
             else if constexpr (D >= 3)
                 vmx::k_bins1<T, D, 3, P, Math, MODE><<<bins_grid, 256, 0, st>>>(
-                    d_jobs, J, total_bins, ra, fa, d_off, d_lst, d_f, d_rt, lp, shape.bins_xcd, d_rrw);
+                    d_jobs, J, total_bins, ra, fa, d_off, d_lst, d_f, d_rt, lp, shape.bins_xcd, d_rrw, rm);
         }
     };
     if constexpr (vmx::IsFubini<P>::value) {
