@@ -29,7 +29,7 @@
 namespace vmx {
 
 template <typename T, int D, int DB, typename F, typename M>
-__global__ void __launch_bounds__(256) k_bins_rm(const Job<T>* __restrict__ jobs, uint32_t J, RegionArrays<T> r,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_bins_rm(const Job<T>* __restrict__ jobs, uint32_t J, RegionArrays<T> r,
                                                  FootArrays<T> fa, const uint32_t* __restrict__ off,
                                                  const uint32_t* __restrict__ lst, const F* __restrict__ fs,
                                                  JobRt* rt, ListPrefix lp) {

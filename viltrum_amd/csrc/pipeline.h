@@ -1426,9 +1426,9 @@ __device__ inline uint32_t xcd_tile(uint32_t w, uint32_t G) {
    then finished in FP64.  Summation depends on
    the job alone, so a batched integral gives exactly what it gives alone. */
 /* The region-major per-bin kernel (bins_rm.h) and the jobs it takes. */
-constexpr int RM_SPB = 16;             // samples per bin per pass
+constexpr int RM_SPB = 8;              // samples per bin per pass
 constexpr int RM_NS = 256 * RM_SPB;    // samples per pass of a block
-constexpr int RM_SLOT_BITS = 12;       // log2(RM_NS)
+constexpr int RM_SLOT_BITS = 11;       // log2(RM_NS)
 constexpr uint32_t RM_MAX_REGIONS = 1u << (32 - RM_SLOT_BITS);  // region ids of a job in a key
 constexpr uint32_t RM_NONE = 0xFFFFFFFFu;
 
