@@ -174,21 +174,30 @@ template <typename T, int D, int TEAM>
 __device__ inline void team_heuristic(TeamLds<T, D>& L, int nc, int heur, int lane, bool active, T& err_out,
                                       int& dim_out) {
     constexpr int KM = pow3(D - 1);
-    const int n_lines = nc * D * KM;
-    if (active)
-        for (int q = lane; q < n_lines; q += TEAM) {
-            const int c = q / (D * KM), rem = q % (D * KM), d = rem / KM, jl = rem % KM;
-            const int below = pow3(d);
-            const int full = (jl % below) + (jl / below) * below * 3;
+    // line (c, d, jl) of region c along dim d lands at (c * D + d) * KM + jl; the loops
+    // over c and d are unrolled so every 3^d below is a compile-time constant
+    if (active) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            if (c >= nc) break;
             const T* v = L.cv[c];
-            const T p0 = v[full], p1 = v[full + below], p2 = v[full + 2 * below];
-            const T sm = simpson(p0, p1, p2), tz = trapezoid(p0, p2);
-            L.e[0][q] = heur == 0 ? metric_relative(sm, tz) : metric_absolute(sm, tz);
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const int below = pow3(d);
+                for (int jl = lane; jl < KM; jl += TEAM) {
+                    const int full = (jl % below) + (jl / below) * below * 3;
+                    const T p0 = v[full], p1 = v[full + below], p2 = v[full + 2 * below];
+                    const T sm = simpson(p0, p1, p2), tz = trapezoid(p0, p2);
+                    L.e[0][(c * D + d) * KM + jl] = heur == 0 ? metric_relative(sm, tz) : metric_absolute(sm, tz);
+                }
+            }
         }
+    }
     __syncthreads();
-    int buf = 0, n = KM;
+    int buf = 0;
+#pragma unroll
     for (int lev = 0; lev < D - 1; ++lev) {
-        n /= 3;
+        const int n = KM / pow3(lev + 1);  // a constant in each unrolled level
         if (active)
             for (int q = lane; q < nc * D * n; q += TEAM) {
                 const int g = q / n, k = q % n;
@@ -366,24 +375,29 @@ __device__ inline void team_child_features(const TeamLds<T, D>& L, const Job<T>*
         const int c = q / K, k = q % K;
         rows[c][3 * D + k] = jb ? nn::f2bf(float(L.cv[c][k]) * (c ? inv1 : inv0)) : uint16_t(0);
     }
-    if (jb)
-        for (int q = lane; q < 2 * D * KM; q += TEAM) {
-            const int c = q / (D * KM), r = q % (D * KM), d = r / KM, jl = r % KM;
-            int below = 1;
-            for (int e = 0; e < d; ++e) below *= 3;
-            const int full = (jl % below) + (jl / below) * below * 3;
-            const float p0 = float(L.cv[c][full]), p1 = float(L.cv[c][full + below]),
-                        p2 = float(L.cv[c][full + 2 * below]);
-            const float diff = vabs(simpson(p0, p1, p2) - trapezoid(p0, p2));
-            float w = 1.f;
-            int rr = jl;
-            for (int e = 0; e < D - 1; ++e) {
-                const int dg = rr % 3;
-                rr /= 3;
-                w *= (dg == 1) ? (4.f / 6.f) : (1.f / 6.f);
+    if (jb) {
+        // term (c, d, jl) at (c * D + d) * KM + jl; c and d unrolled (constant 3^d)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const int below = pow3(d);
+                for (int jl = lane; jl < KM; jl += TEAM) {
+                    const int full = (jl % below) + (jl / below) * below * 3;
+                    const float p0 = float(L.cv[c][full]), p1 = float(L.cv[c][full + below]),
+                                p2 = float(L.cv[c][full + 2 * below]);
+                    const float diff = vabs(simpson(p0, p1, p2) - trapezoid(p0, p2));
+                    float w = 1.f;
+                    int rr = jl;
+                    for (int e = 0; e < D - 1; ++e) {
+                        const int dg = rr % 3;
+                        rr /= 3;
+                        w *= (dg == 1) ? (4.f / 6.f) : (1.f / 6.f);
+                    }
+                    terms[(c * D + d) * KM + jl] = w * diff;
+                }
             }
-            terms[q] = w * diff;
-        }
+    }
     __syncthreads();
     if (lane < 2 * D) {
         const int c = lane / D, d = lane % D;

@@ -185,13 +185,27 @@ VMX_HD inline void split_boxes(const T* lo, const T* hi, int dim, T* lo0, T* hi0
     lo1[dim] = hi0[dim];
 }
 
+/* j with a base-3 digit 0 inserted at position d: (j mod 3^d) + (j div 3^d) * 3^(d+1).
+   The switch keeps every divisor a compile-time constant (an integer division by a
+   run-time value costs tens of instructions on the GPU). */
+VMX_HD inline int insert_digit0(int j, int d) {
+    const unsigned u = unsigned(j);
+    switch (d) {
+        case 0: return int(u * 3u);
+        case 1: return int(u % 3u + (u / 3u) * 9u);
+        case 2: return int(u % 9u + (u / 9u) * 27u);
+        case 3: return int(u % 27u + (u / 27u) * 81u);
+        case 4: return int(u % 81u + (u / 81u) * 243u);
+        default: return int(u % 243u + (u / 243u) * 729u);
+    }
+}
+
 /* Coordinates of the j-th new node of a split (j < 3^(D-1)): child c (0 or 1) gets its
    middle plane at parent-relative .25 / .75 along `dim`; the other dims take {0,.5,1}
    through the parent's node map.  Returns the child's node index the value lands on. */
 template <typename T, int D>
 VMX_HD inline int split_new_node(const T* lo, const T* hi, int dim, int c, int j, T* x) {
-    const int below = pow3(dim);
-    const int idx = (j % below) + below + (j / below) * below * 3;  // digit_dim = 1
+    const int idx = insert_digit0(j, dim) + pow3(dim);  // digit_dim = 1
     for (int d = 0; d < D; ++d)
         x[d] = node_coord<T>(d == dim ? (c == 0 ? 0.25 : 0.75) : double(digit3(idx, d)) / 2.0, lo[d], hi[d]);
     return idx;
