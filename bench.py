@@ -286,7 +286,7 @@ def run_config4(args, V, hip):
                         continue
                     break
                 spp = max(spp + 1, int(np.ceil(spp * 1.1)))
-            ms, _ = timed(fn, problems(n, spp, learned), 2)
+            ms, _ = timed(fn, problems(n, spp, learned), 6)  # enough steps to rank budgets a few % apart
             pts[n] = dict(regions=n, spp=spp, rmse=seen[spp], ms=ms)
         return pts
 

@@ -55,6 +55,8 @@ struct Job {
 struct JobRt {
     uint32_t count, flags, kbound, above, nbucket, nleaves, ties;
     uint32_t row_floor;  // during the rounds: coarse row of the last boundary bucket
+    uint32_t row_ceil;   // no key of the job lies in a coarse row (top 8 key bits) above
+                         // it, +infinity keys (row INF_ROW) aside
 };
 
 template <typename T>
@@ -64,6 +66,20 @@ struct TreeArrays {
     uint32_t *code, *job;
     uint8_t* popped;
 };
+
+/* The coarse row of a +infinity key (key16 of +inf, FP32 and FP64 alike).  It is always
+   read, so +infinity keys (a learned job's root) never raise rt.row_ceil. */
+constexpr uint32_t INF_ROW = 0x7Fu;
+
+/* Counts one priority key in job j's 16-bit histogram; raise also lifts rt.row_ceil to
+   its coarse row.  A node's b is the minimum of its path, so its key is at most its
+   parent's: only a root, a child of a +infinity parent, or a key whose row exceeds its
+   parent's (a negative or NaN priority) has to raise the ceiling.  Every other node
+   lies at or below its parent's row, which the ceiling already covers. */
+__device__ inline void hist_add(uint32_t* hist16, JobRt& rt, uint32_t j, uint32_t k, bool raise) {
+    atomicAdd(&hist16[size_t(j) * 65536 + k], 1u);
+    if (raise) atomicMax(&rt.row_ceil, k >> 8);
+}
 
 /* ------------------------------------------------------------------ block helpers */
 /* In-place inclusive suffix sum of s[0..255] (s[t] = sum_{i>=t}); threads 0..255 of a
@@ -95,7 +111,7 @@ __device__ inline void suffix_sum256(uint32_t* s) {
    than kept by atomics: nodes of a round cluster in few coarse buckets. */
 template <int NT>
 __device__ inline void boundary_bucket(const uint32_t* h16, uint32_t need, uint32_t* sh, uint32_t& k_out,
-                                       uint32_t& above_out, uint32_t row_floor = 0) {
+                                       uint32_t& above_out, uint32_t row_floor = 0, uint32_t row_ceil = 255) {
     static_assert(NT >= 256 && NT % 256 == 0, "whole rows per thread group");
     constexpr int PER = NT / 256;
     const int t = threadIdx.x;
@@ -103,10 +119,11 @@ __device__ inline void boundary_bucket(const uint32_t* h16, uint32_t need, uint3
     __shared__ uint32_t sel_c, sel_above;
     {
         // rows below row_floor cannot hold the boundary (the caller knows it lies at or
-        // above that row): they count as empty and are not read
+        // above that row), rows above row_ceil hold no key but +infinity ones (INF_ROW):
+        // they count as empty and are not read
         const int row = t / PER, q = t % PER;
         uint32_t acc = 0;
-        if (uint32_t(row) >= row_floor) {
+        if (uint32_t(row) >= row_floor && (uint32_t(row) <= row_ceil || uint32_t(row) == INF_ROW)) {
             const uint4* p = reinterpret_cast<const uint4*>(h16 + size_t(row) * 256) + q * (64 / PER);
             for (int i = 0; i < 64 / PER; ++i) {
                 const uint4 v = p[i];
@@ -296,8 +313,7 @@ __global__ void __launch_bounds__(256) k_roots(const Job<T>* __restrict__ jobs, 
         t.child[s] = -1;
         t.job[s] = j;
         rt[j].count = jb.node_base + 1;
-        const uint32_t k = key16(pe);
-        atomicAdd(&hist16[size_t(j) * 65536 + k], 1u);
+        hist_add(hist16, rt[j], j, key16(pe), !(pe == T(INFINITY)));
         cstart0[j] = uint32_t(s);
         ncand0[j] = jb.N > 1 ? 1u : 0u;
     }
@@ -641,8 +657,8 @@ __device__ inline void team_expand(const GenArgs<T>& g, const F* __restrict__ fs
         t.parent[s] = node;
         t.child[s] = -1;
         t.job[s] = j;
-        const uint32_t k = key16(nb);
-        atomicAdd(&g.hist16[size_t(j) * 65536 + k], 1u);
+        const uint32_t kn = key16(nb);
+        hist_add(g.hist16, g.rt[j], j, kn, bp == T(INFINITY) || (kn >> 8) > (key16(bp) >> 8));
         if (c == 0) t.child[node] = int32_t(s);
     }
     __syncthreads();
@@ -677,7 +693,8 @@ __global__ void __launch_bounds__(THETA_THREADS) k_theta_round(GenArgs<T> g, int
         // nodes are only added, so the need-th largest b -- and its bucket -- only rise
         // from round to round: the rows below last round's boundary row are skipped
         uint32_t k, above;
-        boundary_bucket<THETA_THREADS>(g.hist16 + size_t(j) * 65536, need, sh, k, above, g.rt[j].row_floor);
+        boundary_bucket<THETA_THREADS>(g.hist16 + size_t(j) * 65536, need, sh, k, above, g.rt[j].row_floor,
+                                       g.rt[j].row_ceil);
         key16_floor(k, th);
         if (threadIdx.x == 0) g.rt[j].row_floor = k >> 8;
     }
@@ -814,7 +831,7 @@ __global__ void k_sel_bound(const Job<T>* __restrict__ jobs, JobRt* rt, const ui
     const uint32_t j = blockIdx.x;
     const uint32_t need = jobs[j].N - 1;
     uint32_t k = NONE, above = 0;
-    if (need > 0) boundary_bucket<256>(hist16 + size_t(j) * 65536, need, sh, k, above);
+    if (need > 0) boundary_bucket<256>(hist16 + size_t(j) * 65536, need, sh, k, above, 0u, rt[j].row_ceil);
     if (threadIdx.x == 0) {
         rt[j].kbound = k;
         rt[j].above = above;
