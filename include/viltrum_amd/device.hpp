@@ -1,0 +1,106 @@
+/*
+ * User-defined integrands for the C++ drop-in: any device-evaluable functor.
+ *
+ * The reference integrates any callable, f(x0, x1, ...) or f(std::array<Float, D>)
+ * (src/integrate.h:11-70).  A GPU can only call code compiled for it, so a user
+ * integrand is a functor whose call operator is __host__ __device__ (or __device__),
+ * in a translation unit compiled by hipcc for gfx950.  Wrapping it with
+ * viltrum_amd::device_integrand<D>(f) instantiates the whole pipeline (generation,
+ * bin lists, per-bin kernels) for that functor in this translation unit; the result
+ * is an ordinary viltrum_amd::Integrand, so every integrate overload, integrator and
+ * C entry point (vmx_integrate, vmx_integrate_batch, vmx_integrate_device,
+ * vmx_integrate_regions, the refiner's training pairs) takes it:
+ *
+ *     #include <viltrum_amd/device.hpp>
+ *     auto f = viltrum::device_integrand<2>([] __host__ __device__ (float x, float y) {
+ *         return sinf(3.f * x) * expf(y);
+ *     });
+ *     viltrum::integrate(viltrum::integrator_crespo2021(1023, 16), bins, f, viltrum::range_primary<2>());
+ *
+ * The functor is taken as f(const T*), f(const std::array<T, D>&) or f(x0, ..., x_{D-1}),
+ * in that order of preference.  It must be trivially copyable and at most
+ * VMX_CUSTOM_MAX_BYTES: its bytes are copied to device memory per integral (captured
+ * parameters travel with it, pointers in it must be device pointers).
+ *
+ * The scalar types to instantiate follow the dimension argument: device_integrand<D>(f) builds
+ * the FP32 path (float bins), device_integrand<D, double>(f) the FP64 path and
+ * device_integrand<D, float, double>(f) both; bins of a type not instantiated are
+ * refused with VMX_ERR_UNSUPPORTED.  Each instantiation compiles the pipeline's kernels
+ * for the functor (about a minute of hipcc per scalar type).
+ *
+ * Compile with hipcc --offload-arch=gfx950 -std=c++17 -I<repo>/include
+ * -Xclang -target-feature -Xclang -packed-fp32-ops, and link -lviltrum_amd.
+ */
+#ifndef VILTRUM_AMD_DEVICE_HPP
+#define VILTRUM_AMD_DEVICE_HPP
+
+#include <memory>
+#include <tuple>
+#include <type_traits>
+#include <vector>
+
+#include "viltrum.hpp"
+#include "../../viltrum_amd/csrc/custom.h"
+#include "../../viltrum_amd/csrc/run_batch.h"
+
+namespace viltrum_amd {
+namespace detail {
+
+template <typename T, typename... Ts>
+constexpr bool one_of = (std::is_same<T, Ts>::value || ...);
+
+/* The hook the library calls for a group of integrals of this functor type. */
+template <int D, typename Fn, typename... Ts>
+int32_t run_device_integrand(vmx_ctx* ctx, int32_t scalar, const void* group, void* stream, void* pairs) {
+    const auto& jobs = *static_cast<const std::vector<vmxe::JobIn>*>(group);
+    const auto st = static_cast<hipStream_t>(stream);
+    auto* req = static_cast<vmxe::PairsReq*>(pairs);
+    if (scalar == VMX_F32) {
+        if constexpr (one_of<float, Ts...>)
+            return vmxe::run_batch<float, D, vmx::Custom<float, D, Fn>, vmx::FastMath>(ctx, jobs, st, req);
+    } else if (scalar == VMX_F64) {
+        if constexpr (one_of<double, Ts...>)
+            return vmxe::run_batch<double, D, vmx::Custom<double, D, Fn>, vmx::PreciseMath>(ctx, jobs, st, req);
+    }
+    throw vmxe::VmxError(VMX_ERR_UNSUPPORTED,
+                         scalar == VMX_F32 ? "user integrand not instantiated for float bins (device_integrand<D, float>)"
+                                           : "user integrand not instantiated for double bins (device_integrand<D, double>)");
+}
+
+template <typename Fn>
+struct DeviceIntegrandHolder {
+    vmx_custom_integrand desc;
+    Fn fn;
+    explicit DeviceIntegrandHolder(const Fn& f) : desc{}, fn(f) {}
+};
+
+}  // namespace detail
+
+template <int D, typename Fn, typename... Ts>
+Integrand make_device_integrand(const Fn& fn, std::tuple<Ts...>*) {
+    static_assert((... && (std::is_same<Ts, float>::value || std::is_same<Ts, double>::value)),
+                  "scalar types: float and/or double");
+    static_assert((... && vmx::CustomCall<Fn, Ts, D>::ok),
+                  "the functor must be callable as f(const T*), f(std::array<T, D>) or f(x0, ..., x_{D-1})");
+    auto h = std::make_shared<detail::DeviceIntegrandHolder<Fn>>(fn);
+    h->desc.run = &detail::run_device_integrand<D, Fn, Ts...>;
+    h->desc.functor = &h->fn;
+    Integrand f;
+    f.desc = vmx_integrand{VMX_FAMILY_CUSTOM, D, &h->desc};
+    f.params = h;
+    return f;
+}
+
+template <std::size_t D, typename... Ts, typename Fn>
+Integrand device_integrand(const Fn& fn) {
+    static_assert(D >= 1 && D <= VMX_MAX_DIM, "1..6 dimensions (the reference's region cap)");
+    static_assert(std::is_trivially_copyable<Fn>::value,
+                  "a user integrand is copied to the GPU byte for byte: it must be trivially copyable");
+    static_assert(sizeof(Fn) <= VMX_CUSTOM_MAX_BYTES, "a user integrand is staged in LDS: at most 2 KB");
+    using Scalars = std::conditional_t<sizeof...(Ts) == 0, std::tuple<float>, std::tuple<Ts...>>;
+    return make_device_integrand<int(D), Fn>(fn, static_cast<Scalars*>(nullptr));
+}
+
+}  // namespace viltrum_amd
+
+#endif

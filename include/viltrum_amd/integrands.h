@@ -45,8 +45,9 @@ enum vmx_family {
     VMX_FAMILY_BLOBS = 2,
     VMX_FAMILY_LOBES5 = 3,
     VMX_FAMILY_FUBINI = 4,  /* an inner integrand, its trailing dims integrated by Monte Carlo */
-    VMX_FAMILY_DYSON = 5    /* a function of an unbounded sequence: only as the inner integrand of
+    VMX_FAMILY_DYSON = 5,   /* a function of an unbounded sequence: only as the inner integrand of
                                the Fubini form over an infinite range (inner_dim VMX_INFINITE_DIM) */
+    VMX_FAMILY_CUSTOM = 6   /* a user-defined device functor (params: vmx_custom_integrand) */
 };
 
 /* inner_dim of a Fubini descriptor whose rest is an infinite range (RangeInfinite,
@@ -111,6 +112,21 @@ typedef struct vmx_fubini_params {
 typedef struct vmx_dyson_params {
     double decay;                           /* in (0, 1) */
 } vmx_dyson_params;
+
+/* A user-defined integrand (VMX_FAMILY_CUSTOM): a device-evaluable functor for which
+   the caller's own HIP translation unit instantiates the pipeline
+   (viltrum_amd/device.hpp fills this in).  The library hands each group of integrals
+   that share `run` (one functor type and dimension) to run(), which receives the
+   library's internal job list (group) and training-pair request (pairs, or null) and
+   returns a vmx_status.  `functor` points at the functor's bytes in host memory
+   (trivially copyable, at most VMX_CUSTOM_MAX_BYTES: it is staged in LDS).  Not for the
+   Fubini form, vmx_eval or vmx_exact_bins. */
+#define VMX_CUSTOM_MAX_BYTES 2048
+struct vmx_ctx;
+typedef struct vmx_custom_integrand {
+    int32_t (*run)(struct vmx_ctx* ctx, int32_t scalar, const void* group, void* stream, void* pairs);
+    const void* functor;
+} vmx_custom_integrand;
 
 /* Integrand descriptor handed to every entry point. */
 typedef struct vmx_integrand {

@@ -20,9 +20,11 @@
  *     integrator_crespo2021_infinite<DIM>(iterations, mc_samples, spp, seed)
  *                                                                       (integrator-crespo2021.h:24-44; float bins)
  *
- * What differs: the integrand is a device-evaluable descriptor (viltrum_amd::blobs,
- * poly, lobes5) rather than an arbitrary host lambda, which a GPU cannot call; passing
- * a lambda is a compile-time error, not a silent CPU fallback.  The bin value type picks
+ * What differs: the integrand is device-evaluable, either a built-in family descriptor
+ * (viltrum_amd::blobs, poly, lobes5, dyson) or a user functor / __host__ __device__
+ * lambda wrapped by viltrum_amd::device_integrand<D> (viltrum_amd/device.hpp, in a
+ * translation unit compiled by hipcc); a plain host lambda, which a GPU cannot call,
+ * is a compile-time error, not a silent CPU fallback.  The bin value type picks
  * the evaluation precision (float: FP32 evaluation with FP64 assembly; double: FP64).
  *
  * Include this header instead of <viltrum.h>; `namespace viltrum` is an alias of
@@ -332,8 +334,9 @@ public:
     template <typename Bins, std::size_t DB, typename F, typename Float, std::size_t DIM, typename Logger>
     void integrate(Bins&, const std::array<std::size_t, DB>&, const F&, const Range<Float, DIM>&, Logger&) const {
         static_assert(std::is_same<F, Integrand>::value,
-                      "viltrum_amd integrates device-evaluable integrand descriptors (viltrum_amd::blobs, poly, "
-                      "lobes5); a host lambda cannot run on the GPU");
+                      "viltrum_amd integrates device-evaluable integrands: a built-in family (viltrum_amd::blobs, "
+                      "poly, lobes5) or viltrum_amd::device_integrand<D>(functor) from viltrum_amd/device.hpp; a "
+                      "host-only lambda cannot run on the GPU");
     }
 };
 
@@ -479,8 +482,9 @@ public:
     template <typename Bins, std::size_t DB, typename F, typename Float, std::size_t DIM, typename Logger>
     void integrate(Bins&, const std::array<std::size_t, DB>&, const F&, const Range<Float, DIM>&, Logger&) const {
         static_assert(std::is_same<F, Integrand>::value,
-                      "viltrum_amd integrates device-evaluable integrand descriptors (viltrum_amd::blobs, poly, "
-                      "lobes5); a host lambda cannot run on the GPU");
+                      "viltrum_amd integrates device-evaluable integrands: a built-in family (viltrum_amd::blobs, "
+                      "poly, lobes5) or viltrum_amd::device_integrand<D>(functor) from viltrum_amd/device.hpp; a "
+                      "host-only lambda cannot run on the GPU");
     }
 
     template <std::size_t R, typename Bins, std::size_t DB, typename Float, typename Logger>

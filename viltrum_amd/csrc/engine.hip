@@ -5,6 +5,7 @@
  * run_batch.h.
  */
 #include <mutex>
+#include <tuple>
 
 #include "engine_core.h"
 #include "exact.h"
@@ -33,6 +34,11 @@ int dispatch_T(vmx_ctx* ctx, int family, int D, const std::vector<JobIn>& jobs, 
         case VMX_FAMILY_LOBES5:
             if (D == 5) return run_batch<T, 5, vmx::Lobes5<T>, Math>(ctx, jobs, st, pairs);
             break;
+        case VMX_FAMILY_CUSTOM:
+            // a user-defined functor: its pipeline is instantiated in the caller's own
+            // translation unit, reached through the descriptor's hook
+            return static_cast<const vmx_custom_integrand*>(jobs[0].params)
+                ->run(ctx, int32_t(sizeof(T)), &jobs, st, pairs);
         case VMX_FAMILY_FUBINI:
             // integrator_crespo2021_infinite is float-only in the reference; built here for
             // (region dims, inner family, inner dims) = (2, blobs, 3), (2, lobes5, 5),
@@ -72,6 +78,8 @@ int dispatch_T(vmx_ctx* ctx, int family, int D, const std::vector<JobIn>& jobs, 
     throw VmxError(VMX_ERR_UNSUPPORTED, "integrand family / dimension not built");
 }
 
+using custom_hook = decltype(vmx_custom_integrand::run);
+
 /* Groups a batch by (family, dim) and runs each group through the device pipeline. */
 int integrate_groups(vmx_ctx* ctx, uint32_t n, const vmx_problem* const* ps, const vmx_integrand* const* fs,
                      void* const* bins, vmx_stats* const* stats, hipStream_t st,
@@ -85,10 +93,14 @@ int integrate_groups(vmx_ctx* ctx, uint32_t n, const vmx_problem* const* ps, con
     for (uint32_t i = 0; i < n; ++i) {
         if (done[i]) continue;
         std::vector<JobIn> group;
+        // what else must agree within a group: the inner integrand of a Fubini form,
+        // the instantiation hook of a user-defined functor
         auto inner = [](const vmx_integrand* f) {
-            if (f->family != VMX_FAMILY_FUBINI) return std::pair<int, int>(0, 0);
+            if (f->family == VMX_FAMILY_CUSTOM)
+                return std::make_tuple(0, 0, static_cast<const vmx_custom_integrand*>(f->params)->run);
+            if (f->family != VMX_FAMILY_FUBINI) return std::make_tuple(0, 0, custom_hook(nullptr));
             const auto* fp = static_cast<const vmx_fubini_params*>(f->params);
-            return std::pair<int, int>(fp->inner_family, fp->inner_dim);
+            return std::make_tuple(fp->inner_family, fp->inner_dim, custom_hook(nullptr));
         };
         for (uint32_t k = i; k < n; ++k)
             if (!done[k] && fs[k]->family == fs[i]->family && fs[k]->dim == fs[i]->dim && inner(fs[k]) == inner(fs[i])) {
@@ -119,6 +131,10 @@ void validate(const vmx_problem* p, const vmx_integrand* f) {
         throw VmxError(VMX_ERR_INVALID, "unknown heuristic");
     if (p->form < VMX_FORM_VR_OPTIMIZE || p->form > VMX_FORM_VR_FIXED_STRATIFIED)
         throw VmxError(VMX_ERR_INVALID, "unknown form");
+    if (f->family == VMX_FAMILY_CUSTOM) {
+        const auto* c = static_cast<const vmx_custom_integrand*>(f->params);
+        if (!c->run || !c->functor) throw VmxError(VMX_ERR_INVALID, "user integrand: null hook or functor");
+    }
     if (p->form >= VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL && p->form <= VMX_FORM_VR_FIXED_SAMPLING_RR &&
         f->family == VMX_FAMILY_FUBINI)
         throw VmxError(VMX_ERR_UNSUPPORTED, "region roulettes: not for the Fubini form");
@@ -237,7 +253,10 @@ int eval_T(vmx_ctx* ctx, const vmx_integrand& f, uint64_t n, const double* x, do
     } else {
         ok = false;
     }
-    if (!ok) throw VmxError(VMX_ERR_UNSUPPORTED, "integrand family / dimension not built");
+    if (!ok)
+        throw VmxError(VMX_ERR_UNSUPPORTED, f.family == VMX_FAMILY_CUSTOM
+                                                ? "vmx_eval: not for user integrands (call the functor itself)"
+                                                : "integrand family / dimension not built");
     HIP_CHECK(hipMemcpyAsync(out, d_o, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
     HIP_CHECK(hipStreamSynchronize(ctx->stream));
     return VMX_OK;
@@ -329,6 +348,10 @@ uint64_t make_pairs(vmx_ctx* ctx, uint32_t n, const vmx_problem* problems, const
         if (integrands[i].family != integrands[0].family || integrands[i].dim != integrands[0].dim ||
             problems[i].scalar != problems[0].scalar)
             throw VmxError(VMX_ERR_INVALID, "training pairs: one family, dim and scalar per call");
+        if (integrands[i].family == VMX_FAMILY_CUSTOM &&
+            static_cast<const vmx_custom_integrand*>(integrands[i].params)->run !=
+                static_cast<const vmx_custom_integrand*>(integrands[0].params)->run)
+            throw VmxError(VMX_ERR_INVALID, "training pairs: one user integrand type per call");
         if (problems[i].order != VMX_ORDER_CANONICAL) throw VmxError(VMX_ERR_INVALID, "canonical order only");
         group.push_back({&problems[i], integrands[i].params, nullptr, nullptr});
     }
