@@ -261,7 +261,8 @@ def run_config4(args, V, hip):
     # (setup, untimed, like the training)
     tune = {"learned_batched": ctx.autotune(problems(4096, 16, True), fs, ptrs, stream.address),
             "heuristic_batched": ctx.autotune(problems(4096, 16, False), fs, ptrs, stream.address),
-            "plain": ctx.autotune(problems(4096, 16, False)[:1], fs[:1], ptrs[:1], stream.address)}
+            "plain": ctx.autotune(problems(4096, 16, False)[:1], fs[:1], ptrs[:1], stream.address),
+            "learned_unbatched": ctx.autotune(problems(4096, 16, True)[:1], fs[:1], ptrs[:1], stream.address)}
     run_plain(problems(16384, 16, False))
     target = rmse()
 

@@ -280,6 +280,12 @@ __global__ void __launch_bounds__(256) k_roots(const Job<T>* __restrict__ jobs, 
     const F& f = fs[j];
     const size_t s = jb.node_base;
     const int tid = threadIdx.x;
+    // the job's histogram starts empty and its node slots unused (job NONE)
+    {
+        uint4* h = reinterpret_cast<uint4*>(hist16 + size_t(j) * 65536);
+        for (int i = tid; i < 65536 / 4; i += blockDim.x) h[i] = make_uint4(0u, 0u, 0u, 0u);
+        for (uint32_t i = tid; i < jb.node_cap; i += blockDim.x) t.job[s + i] = NONE;
+    }
     if (tid < D) {
         L.box[0][0][tid] = jb.lo[tid];
         L.box[0][1][tid] = jb.hi[tid];
@@ -826,10 +832,12 @@ __global__ void k_train_pairs(const Job<T>* __restrict__ jobs, TreeArrays<T> t, 
 
 /* ------------------------------------------------------------------ selection */
 template <typename T>
-__global__ void k_sel_bound(const Job<T>* __restrict__ jobs, JobRt* rt, const uint32_t* hist16) {
+__global__ void k_sel_bound(const Job<T>* __restrict__ jobs, JobRt* rt, const uint32_t* hist16, int32_t* leaf_slot) {
     __shared__ uint32_t sh[256];
     const uint32_t j = blockIdx.x;
     const uint32_t need = jobs[j].N - 1;
+    // the job's leaf slots start unfilled (k_leaves fills them)
+    for (uint32_t i = threadIdx.x; i <= need; i += blockDim.x) leaf_slot[jobs[j].reg_base + i] = -1;
     uint32_t k = NONE, above = 0;
     if (need > 0) boundary_bucket<256>(hist16 + size_t(j) * 65536, need, sh, k, above, 0u, rt[j].row_ceil);
     if (threadIdx.x == 0) {
@@ -885,14 +893,16 @@ __device__ inline uint32_t sel_digit(const TreeArrays<T>& t, int32_t s, int pass
     return byte_of(uint64_t(~t.code[size_t(s) * D + d]), pass % 4, 4);
 }
 
-/* One workgroup per job: exact radix select inside the boundary bucket.  bstate per
-   bucket item: 0 undecided, 1 split, 2 kept. */
+/* One workgroup of 256 per job: exact radix select inside the boundary bucket.  bstate
+   per bucket item: 0 undecided, 1 split, 2 kept.  Each pass keeps the items whose digit
+   is the one holding the rank-th largest; it stops once a single item is left (keys
+   are unique per node, so that item is the rank-th largest). */
 template <typename T, int D>
-__global__ void k_sel_resolve(const Job<T>* __restrict__ jobs, JobRt* rt, TreeArrays<T> t,
-                              const int32_t* __restrict__ bucket, uint8_t* bstate) {
+__global__ void __launch_bounds__(256) k_sel_resolve(const Job<T>* __restrict__ jobs, JobRt* rt, TreeArrays<T> t,
+                                                     const int32_t* __restrict__ bucket, uint8_t* bstate) {
     constexpr int PASSES = (int(sizeof(T)) - 2) + int(sizeof(T)) + 4 * D;
     __shared__ uint32_t hist[256];
-    __shared__ uint32_t sel_digit_s, sel_above_s;
+    __shared__ uint32_t sel_digit_s, sel_above_s, sel_left_s;
     const uint32_t j = blockIdx.x;
     const Job<T>& jb = jobs[j];
     if (rt[j].kbound == NONE) return;
@@ -908,18 +918,15 @@ __global__ void k_sel_resolve(const Job<T>* __restrict__ jobs, JobRt* rt, TreeAr
         for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x)
             if (st[i] == 0) atomicAdd(&hist[sel_digit<T, D>(t, lst[i], pass)], 1u);
         __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t above = 0;
-            int dsel = 0;
-            for (int dg = 255; dg >= 0; --dg) {
-                if (above + hist[dg] >= rank) {
-                    dsel = dg;
-                    break;
-                }
-                above += hist[dg];
-            }
-            sel_digit_s = uint32_t(dsel);
-            sel_above_s = above;
+        // the digit holding the rank-th largest: suffix counts, then the one place where
+        // they cross rank (1 <= rank <= undecided items)
+        const uint32_t own = hist[threadIdx.x];
+        __syncthreads();
+        suffix_sum256(hist);
+        if (hist[threadIdx.x] >= rank && (threadIdx.x == 255 || hist[threadIdx.x + 1] < rank)) {
+            sel_digit_s = threadIdx.x;
+            sel_above_s = hist[threadIdx.x] - own;
+            sel_left_s = own;
         }
         __syncthreads();
         const uint32_t dsel = sel_digit_s;
@@ -932,7 +939,9 @@ __global__ void k_sel_resolve(const Job<T>* __restrict__ jobs, JobRt* rt, TreeAr
                     st[i] = 2;
             }
         rank -= sel_above_s;
+        const bool last = sel_left_s == 1;
         __syncthreads();
+        if (last) break;
     }
     // exactly one undecided item is left (the key is unique per node): the rank-th
     // largest, split too
@@ -1105,9 +1114,16 @@ VMX_HD constexpr bool form_stratified(int f) {
 
 template <typename T, int D>
 __global__ void k_prepare2(const Job<T>* __restrict__ jobs, const int32_t* __restrict__ leaf_slot, TreeArrays<T> t,
-                           RegionArrays<T> r, uint32_t n, int mb, int merge) {
+                           RegionArrays<T> r, uint32_t n, int mb, int merge, uint32_t* npairs,
+                           unsigned long long* pairs64) {
     constexpr int K = pow3(D);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    // the footprints' pair counts (n + 1 entries) and their 64-bit total start at zero
+    if (i < n) npairs[i] = 0;
+    if (i == 0) {
+        npairs[n] = 0;
+        *pairs64 = 0;
+    }
     if (i >= n) return;
     if (leaf_slot[i] < 0) {  // selection did not close (the host rejects the result): an empty region
         for (int d = 0; d < D; ++d) r.lo[size_t(i) * D + d] = r.hi[size_t(i) * D + d] = T(0);
@@ -1228,6 +1244,21 @@ static __global__ void __launch_bounds__(256) k_sum_u64(const uint32_t* __restri
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
     __syncthreads();
     if (threadIdx.x == 0) atomicAdd(total, (unsigned long long)(part[0] + part[1] + part[2] + part[3]));
+}
+
+/* The stratification totals into the rounds' counter words, read back with the job
+   states in one copy: [2..3] the 64-bit pair total, [4] the 32-bit pair scan's end,
+   [5] the footprint count. */
+static __global__ void k_collect_totals(const unsigned long long* __restrict__ pairs64,
+                                        const uint32_t* __restrict__ pofs_end, const uint32_t* __restrict__ nfp,
+                                        uint32_t* small) {
+    if (threadIdx.x == 0) {
+        const unsigned long long p = *pairs64;
+        small[2] = uint32_t(p);
+        small[3] = uint32_t(p >> 32);
+        small[4] = *pofs_end;
+        small[5] = *nfp;
+    }
 }
 
 /* q = c / n for the box walks below: a float estimate, then exact integer fix-up. */

@@ -155,6 +155,13 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     uint32_t* d_cstart = nullptr;
     uint32_t cap_total = 0;
     std::vector<vmx::JobRt> rt(J);
+    // the rounds' shared state, one block so that one copy reads it back: candidate
+    // counts (2 x J), job states (J) and 16 words of counters ([0] rounds, [2..5] the pair
+    // totals after stratification, [8 + d] deepest split count along dim d)
+    const size_t state_bytes = sizeof(uint32_t) * 2 * J + sizeof(vmx::JobRt) * J + 64;
+    const size_t slot_bytes = (state_bytes + 255) & ~size_t(255);
+    char* d_state = nullptr;
+    uint32_t* small = nullptr;
     // ---------------- 1. generation (retried with more node capacity if needed)
     for (int attempt = 0;; ++attempt) {
         uint64_t cap64 = 0;
@@ -167,8 +174,8 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         cap_total = uint32_t(cap64);
         Measure m;
         m.take<vmx::Job<T>>(J);
-        m.take<vmx::JobRt>(J);
         m.take<P>(J);
+        m.take<char>(state_bytes);
         m.take<T>(size_t(cap_total) * D);
         m.take<T>(size_t(cap_total) * D);
         m.take<T>(size_t(cap_total) * K);
@@ -179,14 +186,16 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         m.take<uint8_t>(cap_total);
         m.take<uint32_t>(size_t(J) * 65536);
         m.take<uint32_t>(2 * J);
-        m.take<uint32_t>(2 * J);
         m.take<T>(J);
-        m.take<uint32_t>(16);
         ensure(ctx->tree, m.used);
         Carve c(ctx->tree);
+        // jobs, parameters and state first: one contiguous image the host uploads at once
         d_jobs = c.take<vmx::Job<T>>(J);
-        d_rt = c.take<vmx::JobRt>(J);
         d_f = c.take<P>(J);
+        d_state = c.take<char>(state_bytes);
+        d_ncand = reinterpret_cast<uint32_t*>(d_state);
+        d_rt = reinterpret_cast<vmx::JobRt*>(d_state + sizeof(uint32_t) * 2 * J);
+        small = reinterpret_cast<uint32_t*>(d_state + sizeof(uint32_t) * 2 * J + sizeof(vmx::JobRt) * J);
         t.lo = c.take<T>(size_t(cap_total) * D);
         t.hi = c.take<T>(size_t(cap_total) * D);
         t.v = c.take<T>(size_t(cap_total) * K);
@@ -201,18 +210,29 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         t.popped = c.take<uint8_t>(cap_total);
         d_hist16 = c.take<uint32_t>(size_t(J) * 65536);
         d_cstart = c.take<uint32_t>(2 * J);
-        d_ncand = c.take<uint32_t>(2 * J);
         d_theta = c.take<T>(J);
-        uint32_t* small = c.take<uint32_t>(16);  // [0] rounds, [8 + d] deepest split count along dim d
         d_rounds = small;
-        HIP_CHECK(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(vmx::Job<T>) * J, hipMemcpyHostToDevice, st));
-        HIP_CHECK(hipMemcpyAsync(d_f, fparams.data(), sizeof(P) * J, hipMemcpyHostToDevice, st));
-        HIP_CHECK(hipMemsetAsync(d_rt, 0, sizeof(vmx::JobRt) * J, st));
-        HIP_CHECK(hipMemsetAsync(t.job, 0xFF, sizeof(uint32_t) * cap_total, st));
-        HIP_CHECK(hipMemsetAsync(d_hist16, 0, sizeof(uint32_t) * J * 65536, st));
-        HIP_CHECK(hipMemsetAsync(d_ncand, 0, sizeof(uint32_t) * 2 * J, st));
-        HIP_CHECK(hipMemsetAsync(small, 0, sizeof(uint32_t) * 16, st));
-        if (replay) {  // the given regions become the nodes, in list order
+        {
+            // jobs, parameters and zeroed state in one copy from pinned staging (after the
+            // round loop's two readback slots) where it fits; k_roots clears each job's
+            // histogram and node slots
+            char* base = reinterpret_cast<char*>(d_jobs);
+            const size_t img = size_t(d_state + state_bytes - base);
+            const size_t need = 2 * slot_bytes + img;
+            if (!ctx->noalloc || need <= ctx->pinned_bytes) {
+                char* pin = static_cast<char*>(ensure_pinned(ctx, need)) + 2 * slot_bytes;
+                std::memcpy(pin, jobs.data(), sizeof(vmx::Job<T>) * J);
+                std::memcpy(pin + (reinterpret_cast<char*>(d_f) - base), fparams.data(), sizeof(P) * J);
+                std::memset(pin + (d_state - base), 0, state_bytes);
+                HIP_CHECK(hipMemcpyAsync(base, pin, img, hipMemcpyHostToDevice, st));
+            } else {
+                HIP_CHECK(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(vmx::Job<T>) * J, hipMemcpyHostToDevice, st));
+                HIP_CHECK(hipMemcpyAsync(d_f, fparams.data(), sizeof(P) * J, hipMemcpyHostToDevice, st));
+                HIP_CHECK(hipMemsetAsync(d_state, 0, state_bytes, st));
+            }
+        }
+        if (replay) {
+            HIP_CHECK(hipMemsetAsync(t.job, 0xFF, sizeof(uint32_t) * cap_total, st));  // the given regions become the nodes, in list order
             for (uint32_t j = 0; j < J; ++j) {
                 const GivenRegions& gr = *in[j].given;
                 const vmx::Job<T>& jb = jobs[j];
@@ -266,7 +286,6 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         // pinned memory.  The host checks chunk k while chunk k+1 runs, so the device is
         // never idle waiting for the host; the chunk after the last busy one finds no
         // candidates and its kernels exit at once.
-        const size_t slot_bytes = ((sizeof(uint32_t) * 2 * J + sizeof(vmx::JobRt) * J + 64) + 255) & ~size_t(255);
         char* pin = static_cast<char*>(ensure_pinned(ctx, 2 * slot_bytes));
         auto slot = [&](int k) { return pin + size_t(k & 1) * slot_bytes; };
         // the chunk's 2 * round_chunk kernels as a graph: one launch instead of many
@@ -314,12 +333,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
                 }
                 HIP_CHECK(hipGetLastError());
             }
-            char* sl = slot(k);
-            HIP_CHECK(hipMemcpyAsync(sl, d_ncand, sizeof(uint32_t) * 2 * J, hipMemcpyDeviceToHost, st));
-            HIP_CHECK(hipMemcpyAsync(sl + sizeof(uint32_t) * 2 * J, d_rt, sizeof(vmx::JobRt) * J,
-                                     hipMemcpyDeviceToHost, st));
-            HIP_CHECK(hipMemcpyAsync(sl + sizeof(uint32_t) * 2 * J + sizeof(vmx::JobRt) * J, small, 64,
-                                     hipMemcpyDeviceToHost, st));
+            HIP_CHECK(hipMemcpyAsync(slot(k), d_state, state_bytes, hipMemcpyDeviceToHost, st));
             HIP_CHECK(hipEventRecord(ctx->ev[2 + (k & 1)], st));
         };
         launch_chunk(0);
@@ -442,8 +456,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         HIP_CHECK(hipMemcpyAsync(d_leaf, all_leaves.data(), sizeof(int32_t) * n_reg, hipMemcpyHostToDevice, st));
         HIP_CHECK(hipStreamSynchronize(st));
     } else if (order == VMX_ORDER_CANONICAL) {
-        HIP_CHECK(hipMemsetAsync(d_leaf, 0xFF, sizeof(int32_t) * n_reg, st));
-        vmx::k_sel_bound<T><<<J, 256, 0, st>>>(d_jobs, d_rt, d_hist16);
+        vmx::k_sel_bound<T><<<J, 256, 0, st>>>(d_jobs, d_rt, d_hist16, d_leaf);
         vmx::k_sel_gather<T><<<cap_blocks, 256, 0, st>>>(d_jobs, d_rt, t, cap_total, d_bucket);
         vmx::k_sel_resolve<T, D><<<J, 256, 0, st>>>(d_jobs, d_rt, t, d_bucket, d_bstate);
         vmx::k_leaves<T><<<cap_blocks, 256, 0, st>>>(d_jobs, d_rt, t, cap_total, d_leaf);
@@ -573,12 +586,12 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     // mb differs per job only through db; all jobs of a batch share the footprint stride
     for (const auto& jb : jobs)
         if (jb.mb != mb_max) throw VmxError(VMX_ERR_INVALID, "one bin_dims per batch");
-    vmx::k_prepare2<T, D><<<reg_blocks, 256, 0, st>>>(d_jobs, d_leaf, t, ra, uint32_t(n_reg), mb_max, merge);
+    vmx::k_prepare2<T, D><<<reg_blocks, 256, 0, st>>>(d_jobs, d_leaf, t, ra, uint32_t(n_reg), mb_max, merge,
+                                                      fa.npairs, d_pairs64);
     HIP_CHECK(hipGetLastError());
     size_t bytes = scan_reg_bytes;
     HIP_CHECK(hipError_t(rocprim::exclusive_scan(d_scan_tmp, bytes, ra.fpflag, ra.fpid, 0u, size_t(n_reg + 1),
                                                 rocprim::plus<uint32_t>(), st)));
-    HIP_CHECK(hipMemsetAsync(fa.npairs, 0, sizeof(uint32_t) * (n_reg + 1), st));
     vmx::k_footprints<T, D><<<reg_blocks, 256, 0, st>>>(d_jobs, ra, fa, uint32_t(n_reg), mb_max, merge);
     HIP_CHECK(hipGetLastError());
     const uint32_t* d_nfp = ra.fpid + n_reg;
@@ -589,22 +602,21 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
                                                 rocprim::plus<uint32_t>(), st)));
     // the pair total in 64 bits: the 32-bit scan above (and the per-bin offsets below)
     // only hold when it fits
-    HIP_CHECK(hipMemsetAsync(d_pairs64, 0, sizeof(unsigned long long), st));
     vmx::k_sum_u64<<<std::min(reg_blocks, 1024u), 256, 0, st>>>(fa.npairs, uint32_t(n_reg), d_pairs64);
+    vmx::k_collect_totals<<<1, 64, 0, st>>>(d_pairs64, d_pofs + n_reg, d_nfp, small);
     HIP_CHECK(hipGetLastError());
     uint32_t pairs = 0, nfp = 0;
     uint64_t pairs64 = 0;
     {
-        char* pin = static_cast<char*>(ensure_pinned(ctx, 16 + sizeof(vmx::JobRt) * J));
-        HIP_CHECK(hipMemcpyAsync(pin, d_pairs64, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-        HIP_CHECK(hipMemcpyAsync(pin + 8, d_pofs + n_reg, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-        HIP_CHECK(hipMemcpyAsync(pin + 12, d_nfp, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-        HIP_CHECK(hipMemcpyAsync(pin + 16, d_rt, sizeof(vmx::JobRt) * J, hipMemcpyDeviceToHost, st));
+        char* pin = static_cast<char*>(ensure_pinned(ctx, state_bytes));
+        HIP_CHECK(hipMemcpyAsync(pin, d_state, state_bytes, hipMemcpyDeviceToHost, st));
         HIP_CHECK(hipStreamSynchronize(st));
-        std::memcpy(&pairs64, pin, 8);
-        std::memcpy(&pairs, pin + 8, 4);
-        std::memcpy(&nfp, pin + 12, 4);
-        std::memcpy(rt.data(), pin + 16, sizeof(vmx::JobRt) * J);
+        uint32_t w[4];
+        std::memcpy(w, pin + sizeof(uint32_t) * 2 * J + sizeof(vmx::JobRt) * J + 2 * sizeof(uint32_t), sizeof(w));
+        pairs64 = uint64_t(w[0]) | (uint64_t(w[1]) << 32);
+        pairs = w[2];
+        nfp = w[3];
+        std::memcpy(rt.data(), pin + sizeof(uint32_t) * 2 * J, sizeof(vmx::JobRt) * J);
     }
     check_pair_total(pairs64);
     if (order == VMX_ORDER_CANONICAL && !replay)
@@ -617,25 +629,26 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     HIP_CHECK(hipError_t(rocprim::exclusive_scan(nullptr, scan_bins_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr, 0u,
                                                 size_t(total_bins + 1), rocprim::plus<uint32_t>(), st)));
     Measure mp;
-    mp.take<uint32_t>(total_bins + 1);                        // counts
+    const size_t qlong_len = size_t(pairs) / vmx::SORT_SHORT + 2;
+    mp.take<uint32_t>(total_bins + 1 + qlong_len);            // counts, then the queue of long lists
     mp.take<uint32_t>(size_t(pairs) + 1);                     // lists
     mp.take<uint32_t>(size_t(pairs) + 1);                     // copies of very long lists
-    mp.take<uint32_t>(size_t(pairs) / vmx::SORT_SHORT + 2);   // queue of long lists
     mp.take<uint32_t>(size_t(pairs) + 1);                     // prefix counts of long lists
     mp.take<uint32_t>(size_t(pairs) + 1);                     // first regions of long lists
     if (any_rr) mp.take<double>(size_t(pairs) + 1);          // roulette weights per pair
     mp.take<char>(scan_bins_bytes + 256);
     ensure(ctx->bins2, mp.used);
     Carve cp(ctx->bins2);
-    uint32_t* d_cnt = cp.take<uint32_t>(total_bins + 1);
+    // the per-bin counts, then the queue of long lists (its word 0 counts the entries
+    // that follow): one fill clears the counts and the queue's counter
+    uint32_t* d_cnt = cp.take<uint32_t>(total_bins + 1 + qlong_len);
+    uint32_t* d_qlong = d_cnt + total_bins + 1;
     uint32_t* d_lst = cp.take<uint32_t>(size_t(pairs) + 1);
     uint32_t* d_lst_tmp = cp.take<uint32_t>(size_t(pairs) + 1);
-    uint32_t* d_qlong = cp.take<uint32_t>(size_t(pairs) / vmx::SORT_SHORT + 2);
     vmx::ListPrefix lp{cp.take<uint32_t>(size_t(pairs) + 1), cp.take<uint32_t>(size_t(pairs) + 1)};
     double* d_rrw = any_rr ? cp.take<double>(size_t(pairs) + 1) : nullptr;
     void* d_scan_bins_tmp = cp.take<char>(scan_bins_bytes + 256);
-    HIP_CHECK(hipMemsetAsync(d_cnt, 0, sizeof(uint32_t) * (total_bins + 1), st));
-    HIP_CHECK(hipMemsetAsync(d_qlong, 0, sizeof(uint32_t), st));
+    HIP_CHECK(hipMemsetAsync(d_cnt, 0, sizeof(uint32_t) * (total_bins + 2), st));
     const unsigned fp_blocks = unsigned((uint64_t(nfp) * 64 + 255) / 256);
     if (pairs > 0)
         vmx::k_bin_pass<T, false><<<fp_blocks, 256, 0, st>>>(d_jobs, fa, d_nfp, d_cnt, nullptr, nullptr);
