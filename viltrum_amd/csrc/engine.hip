@@ -769,8 +769,10 @@ void vmx_ctx_destroy(vmx_ctx* ctx) {
     for (vmx_ctx::Buf& b : ctx->nn_lds)
         if (b.p) (void)hipFree(b.p);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
-    if (ctx->rounds.exec) (void)hipGraphExecDestroy(ctx->rounds.exec);
-    if (ctx->rounds.graph) (void)hipGraphDestroy(ctx->rounds.graph);
+    for (auto* rg : {&ctx->rounds, &ctx->rounds_tail}) {
+        if (rg->exec) (void)hipGraphExecDestroy(rg->exec);
+        if (rg->graph) (void)hipGraphDestroy(rg->graph);
+    }
     for (hipEvent_t e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

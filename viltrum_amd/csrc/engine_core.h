@@ -64,7 +64,7 @@ struct vmx_ctx {
         std::vector<unsigned char> key;
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
-    } rounds;
+    } rounds, rounds_tail;  // chunks of the tuned size, and of 2 rounds near the end
     // launch shapes: the autotuner's choices per workload class, and the shape a tuning
     // trial forces (keys of the classes the trial ran are collected in tuned_keys)
     struct Shape {
@@ -74,6 +74,9 @@ struct vmx_ctx {
     // node capacity per region (times N) the last call of a workload class needed, so a
     // class whose rounds overflow 8N does not pay a failed attempt on every call
     std::map<std::array<uint64_t, 6>, uint64_t> cap_hint;
+    // busy rounds the last call of a workload class (and budget octave) needed: where the
+    // round chunks shrink
+    std::map<std::array<uint64_t, 7>, uint32_t> round_hint;
     const Shape* trial = nullptr;
     std::vector<std::array<uint64_t, 6>> tuned_keys;
     // last result (for vmx_export_regions): gathered from device memory on demand

@@ -359,13 +359,22 @@ struct LearnLds {
     static constexpr int KP = nn::Dims<D>::KP;
     static constexpr int TPB = GEN_THREADS / TeamShape<D>::TEAM;
     static constexpr int R = (2 * TPB + 15) / 16 * 16;
-    // the first layer's weights (5 KB at D = 2 .. 37 KB at D = 5) are read from device
-    // memory (L2): the LDS they would take costs a block per CU at every D
-    static constexpr bool W0_DEV = true;
+    // D = 2: the first layer's weights (5 KB) live in LDS, off the L2 round trip of
+    // every round's forward pass; the room comes from the features, which are dead once
+    // layer 0 has run and share their space with layer 1's output (same block size).
+    // D >= 3: the first layer (9 KB .. 37 KB) would cost a block per CU and is read
+    // from device memory (L2)
+    static constexpr bool W0_DEV = D != 2;
+    static constexpr bool ALIAS = D == 2;
     typename std::conditional<W0_DEV, nn::NetTail, nn::NetW<KP>>::type net;
-    alignas(16) uint16_t xs[R][KP + 8];
-    alignas(16) uint16_t h[2][R][nn::WS];
+    union alignas(16) {
+        uint16_t xs[R][KP + 8];
+        uint16_t h1a[ALIAS ? R : 1][nn::WS];
+    };
+    alignas(16) uint16_t h[ALIAS ? 1 : 2][R][nn::WS];
     float y[R];
+    __device__ uint16_t* h0() { return &h[0][0][0]; }
+    __device__ uint16_t* h1() { return ALIAS ? &h1a[0][0] : &h[ALIAS ? 0 : 1][0][0]; }
 };
 
 /* nn::region_features of the two children of one expansion (boxes L.box, nodes L.cv,
@@ -590,11 +599,9 @@ __device__ inline void team_expand(const GenArgs<T>& g, const F* __restrict__ fs
         __syncthreads();
         if constexpr (LearnLds<D>::W0_DEV)
             nn::forward_block_w0dev<KP>(&static_cast<const nn::NetW<KP>*>(g.net)->w0t[0][0], ll->net,
-                                        &ll->xs[0][0], LearnLds<D>::R, &ll->h[0][0][0], &ll->h[1][0][0],
-                                        &ll->h[0][0][0], ll->y);
+                                        &ll->xs[0][0], LearnLds<D>::R, ll->h0(), ll->h1(), ll->h0(), ll->y);
         else
-            nn::forward_block<KP>(ll->net, &ll->xs[0][0], LearnLds<D>::R, &ll->h[0][0][0], &ll->h[1][0][0],
-                                  &ll->h[0][0][0], ll->y);
+            nn::forward_block<KP>(ll->net, &ll->xs[0][0], LearnLds<D>::R, ll->h0(), ll->h1(), ll->h0(), ll->y);
         if (lrn && lane < 2) {
             // lp = log2 of the predicted energy.  Stored as 2^(lp/4): the same order, and
             // the whole span of lp (about -450 for regions of zeros .. +50) stays inside
@@ -676,7 +683,7 @@ __device__ inline void team_expand(const GenArgs<T>& g, const F* __restrict__ fs
    boundaries order the rounds, so no grid-wide barrier or cache flush is needed inside
    a kernel.  A round with no candidates exits at once; the host launches rounds in
    chunks and checks for remaining candidates between chunks. */
-constexpr int THETA_THREADS = 256;
+constexpr int THETA_THREADS = 1024;  // 4 threads per histogram row: 16 vector loads each
 template <typename T>
 __global__ void __launch_bounds__(THETA_THREADS) k_theta_round(GenArgs<T> g, int par) {
     __shared__ uint32_t sh[256];

@@ -143,6 +143,12 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     const int round_chunk = shape.round_chunk;
 
     uint32_t rounds = 0;
+    // the round-count hint's class: the workload class and the largest budget's octave
+    uint32_t n_max = 1;
+    for (const auto& jb : jobs) n_max = std::max(n_max, jb.N);
+    std::array<uint64_t, 7> hint_key{};
+    std::copy(tune_key.begin(), tune_key.end(), hint_key.begin());
+    hint_key[6] = uint64_t(ilog2(n_max));
     uint32_t max_depth[VMX_MAX_DIM] = {};
     uint64_t cap_mult = 8;
     if (const auto h = ctx->cap_hint.find(tune_key); h != ctx->cap_hint.end()) cap_mult = h->second;
@@ -295,51 +301,65 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         // (nor under the no-alloc entry point: instantiating a graph allocates)
         static const bool graph_env = !(std::getenv("VMX_NO_GRAPH") && std::atoi(std::getenv("VMX_NO_GRAPH")));
         const bool use_graph = graph_env && !ctx->noalloc;
-        if (use_graph) {
+        // chunk sizes: round_chunk, then 2 once the rounds near the count the last call of
+        // this workload class needed (fewer idle rounds after the last busy one)
+        const auto hint_it = ctx->round_hint.find(hint_key);
+        const uint32_t hint = hint_it == ctx->round_hint.end() ? 0u : hint_it->second;
+        const int tail_chunk = std::min(2, round_chunk);
+        auto ensure_graph = [&](vmx_ctx::RoundGraph& rg, int size) {
             std::vector<unsigned char> key(sizeof(g) + 64);
             std::memcpy(key.data(), &g, sizeof(g));
             const void* fn = reinterpret_cast<const void*>(&launch_expand<T, D, P, Math>);
             const void* parts[4] = {fn, d_f, st, reinterpret_cast<const void*>(uintptr_t(learn))};
             std::memcpy(key.data() + sizeof(g), parts, sizeof(parts));
-            const uint32_t shape[2] = {gen_blocks, uint32_t(round_chunk)};
+            const uint32_t shape[2] = {gen_blocks, uint32_t(size)};
             std::memcpy(key.data() + sizeof(g) + sizeof(parts), shape, sizeof(shape));
-            if (ctx->rounds.key != key || !ctx->rounds.exec) {
-                if (ctx->rounds.exec) HIP_CHECK(hipGraphExecDestroy(ctx->rounds.exec));
-                if (ctx->rounds.graph) HIP_CHECK(hipGraphDestroy(ctx->rounds.graph));
-                ctx->rounds.exec = nullptr;
-                ctx->rounds.graph = nullptr;
-                ctx->rounds.key.clear();
-                HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
-                for (int r = 0; r < round_chunk; ++r) {
-                    vmx::k_theta_round<T><<<J, vmx::THETA_THREADS, 0, st>>>(g, r & 1);
-                    launch_expand<T, D, P, Math>(learn, g, d_f, r & 1, gen_blocks, st);
-                }
-                hipGraph_t graph = nullptr;
-                const hipError_t e1 = hipStreamEndCapture(st, &graph);
-                HIP_CHECK(e1);
-                HIP_CHECK(hipGetLastError());
-                HIP_CHECK(hipGraphInstantiate(&ctx->rounds.exec, graph, nullptr, nullptr, 0));
-                ctx->rounds.graph = graph;
-                ctx->rounds.key = key;
+            if (rg.key == key && rg.exec) return;
+            if (rg.exec) HIP_CHECK(hipGraphExecDestroy(rg.exec));
+            if (rg.graph) HIP_CHECK(hipGraphDestroy(rg.graph));
+            rg.exec = nullptr;
+            rg.graph = nullptr;
+            rg.key.clear();
+            HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+            for (int r = 0; r < size; ++r) {
+                vmx::k_theta_round<T><<<J, vmx::THETA_THREADS, 0, st>>>(g, r & 1);
+                launch_expand<T, D, P, Math>(learn, g, d_f, r & 1, gen_blocks, st);
             }
+            hipGraph_t graph = nullptr;
+            const hipError_t e1 = hipStreamEndCapture(st, &graph);
+            HIP_CHECK(e1);
+            HIP_CHECK(hipGetLastError());
+            HIP_CHECK(hipGraphInstantiate(&rg.exec, graph, nullptr, nullptr, 0));
+            rg.graph = graph;
+            rg.key = key;
+        };
+        if (use_graph) {
+            ensure_graph(ctx->rounds, round_chunk);
+            if (hint > 0 && tail_chunk < round_chunk) ensure_graph(ctx->rounds_tail, tail_chunk);
         }
+        uint32_t launched = 0;
+        // sizes are even, so every chunk starts at parity 0
         auto launch_chunk = [&](int k) {
+            const bool tail = hint > 0 && launched + uint32_t(round_chunk) + 2 > hint && launched < hint + 8;
+            const int size = tail ? tail_chunk : round_chunk;
             if (use_graph) {
-                HIP_CHECK(hipGraphLaunch(ctx->rounds.exec, st));
+                HIP_CHECK(hipGraphLaunch(tail && tail_chunk < round_chunk ? ctx->rounds_tail.exec : ctx->rounds.exec,
+                                         st));
             } else {
-                for (int r = 0; r < round_chunk; ++r) {
+                for (int r = 0; r < size; ++r) {
                     vmx::k_theta_round<T><<<J, vmx::THETA_THREADS, 0, st>>>(g, r & 1);
                     launch_expand<T, D, P, Math>(learn, g, d_f, r & 1, gen_blocks, st);
                 }
                 HIP_CHECK(hipGetLastError());
             }
+            launched += uint32_t(size);
             HIP_CHECK(hipMemcpyAsync(slot(k), d_state, state_bytes, hipMemcpyDeviceToHost, st));
             HIP_CHECK(hipEventRecord(ctx->ev[2 + (k & 1)], st));
         };
         launch_chunk(0);
         int k = 0;
         for (;; ++k) {
-            if ((k + 2) * round_chunk > 4096) {
+            if (launched + uint32_t(round_chunk) > 4096) {
                 HIP_CHECK(hipStreamSynchronize(st));
                 throw VmxError(VMX_ERR_CAPACITY, "more than 4096 refinement rounds");
             }
@@ -368,6 +388,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
             continue;
         }
         ctx->cap_hint[tune_key] = cap_mult;
+        ctx->round_hint[hint_key] = rounds;
         break;
     }
     const double ms_gen = ms_since(t_start);
