@@ -50,6 +50,7 @@ struct Job {
     int32_t learn;   // refinement priority from the network (refiner.h)
     int32_t lfloor;  // with learn: every dim is first halved lfloor times, widest first
     float lres[VMX_MAX_DIM];  // log2 of the bin count per dim (0 for integrated dims)
+    uint32_t defer_m;  // deferred rounds: at most about this many expansions per round
 };
 
 struct JobRt {
@@ -340,7 +341,19 @@ struct GenArgs {
     uint32_t max_rounds;
     const void* net;   // the priority network's LDS image (nn::NetW<KP>, packed once per network)
     int32_t deepen;    // 1: a split's children with b >= theta are split in the same round
+    // deferred rounds (defer = 1): a round expands its candidates with b >= max(theta,
+    // tau), tau about the defer_m-th largest candidate b; those in [theta, tau) wait for
+    // a later round in a list, those below theta can never be split
+    int32_t defer;
+    uint32_t cap_total;
+    T* theta_lo;       // [J] theta itself (theta then holds max(theta, tau))
+    uint32_t* ndef;    // [2*J] deferred candidates
+    uint32_t* def;     // [2][cap_total] their slots, job j's from node_base_j
+    uint32_t* nexp;    // [J] the round's expansions
+    uint32_t* exp;     // [cap_total] their slots, job j's from node_base_j
+    uint32_t* chist;   // [2][J][CHIST] the next round's candidates by the top 12 bits of key16(b)
 };
+constexpr int CHIST = 4096;
 
 /* Per-team slot requests of one expansion step, combined into one atomic per block and
    job (teams of a step hold consecutive candidates, so a job's teams are one run). */
@@ -499,7 +512,8 @@ __device__ inline void team_expand(const GenArgs<T>& g, const F* __restrict__ fs
                         hi = mid;
                 }
                 j = lo;
-                node = int32_t(g.cstart[par * J + j] + (uint32_t(w) - pre[j]));
+                node = g.defer ? int32_t(g.exp[g.jobs[j].node_base + (uint32_t(w) - pre[j])])
+                               : int32_t(g.cstart[par * J + j] + (uint32_t(w) - pre[j]));
             }
             L.node = node;
             sl.job[team] = j;
@@ -672,6 +686,7 @@ __device__ inline void team_expand(const GenArgs<T>& g, const F* __restrict__ fs
         t.job[s] = j;
         const uint32_t kn = key16(nb);
         hist_add(g.hist16, g.rt[j], j, kn, bp == T(INFINITY) || (kn >> 8) > (key16(bp) >> 8));
+        if (g.defer) atomicAdd(&g.chist[(size_t(par ^ 1) * J + j) * CHIST + (kn >> 4)], 1u);
         if (c == 0) t.child[node] = int32_t(s);
     }
     __syncthreads();
@@ -688,13 +703,24 @@ template <typename T>
 __global__ void __launch_bounds__(THETA_THREADS) k_theta_round(GenArgs<T> g, int par) {
     __shared__ uint32_t sh[256];
     const uint32_t j = blockIdx.x;
+    const uint32_t J = g.J;
     const Job<T>& jb = g.jobs[j];
-    // this round's children are appended from the job's current count on
+    const uint32_t ncand = g.ncand[par * J + j] + (g.defer ? g.ndef[par * J + j] : 0u);
+    // this round's children are appended from the job's current count on; with deferred
+    // rounds, the next round's list and candidate histogram start empty
     if (threadIdx.x == 0) {
-        g.cstart[(par ^ 1) * g.J + j] = g.rt[j].count;
-        g.ncand[(par ^ 1) * g.J + j] = 0;
+        g.cstart[(par ^ 1) * J + j] = g.rt[j].count;
+        g.ncand[(par ^ 1) * J + j] = 0;
+        if (g.defer) {
+            g.ndef[(par ^ 1) * J + j] = 0;
+            g.nexp[j] = 0;
+        }
     }
-    if (g.ncand[par * g.J + j] == 0) return;
+    if (g.defer) {
+        uint4* h = reinterpret_cast<uint4*>(g.chist + (size_t(par ^ 1) * J + j) * CHIST);
+        for (int i = threadIdx.x; i < CHIST / 4; i += blockDim.x) h[i] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    if (ncand == 0) return;
     const uint32_t total = g.rt[j].count - jb.node_base;
     const uint32_t need = jb.N - 1;
     T th;
@@ -711,7 +737,109 @@ __global__ void __launch_bounds__(THETA_THREADS) k_theta_round(GenArgs<T> g, int
         key16_floor(k, th);
         if (threadIdx.x == 0) g.rt[j].row_floor = k >> 8;
     }
-    if (threadIdx.x == 0) g.theta[j] = th;
+    T eff = th;
+    if (g.defer && jb.defer_m > 0 && ncand > jb.defer_m && need > 0) {
+        // tau: the lower edge of the 12-bit key bucket holding the defer_m-th largest
+        // candidate b (any tau keeps the result exact; this one bounds a round's work)
+        const uint32_t* ch = g.chist + (size_t(par) * J + j) * CHIST;
+        __shared__ uint32_t sel_row, sel_above;
+        if (threadIdx.x < 256) {
+            uint32_t a = 0;
+            for (int i = 0; i < 16; ++i) a += ch[threadIdx.x * 16 + i];
+            sh[threadIdx.x] = a;
+        }
+        __syncthreads();
+        suffix_sum256(sh);
+        const uint32_t m = jb.defer_m;
+        if (threadIdx.x < 256 && sh[threadIdx.x] >= m && (threadIdx.x == 255 || sh[threadIdx.x + 1] < m)) {
+            sel_row = threadIdx.x;
+            sel_above = threadIdx.x == 255 ? 0u : sh[threadIdx.x + 1];
+        }
+        __syncthreads();
+        if (sh[0] >= m) {  // block-uniform: the histogram's total
+            const uint32_t row = sel_row;
+            uint32_t acc = sel_above, b12 = row * 16;
+            for (int i = 15; i >= 0; --i) {
+                acc += ch[row * 16 + i];
+                if (acc >= m) {
+                    b12 = row * 16 + uint32_t(i);
+                    break;
+                }
+            }
+            T tau;
+            key16_floor(b12 << 4, tau);
+            eff = tau > th ? tau : th;
+        }
+    }
+    if (threadIdx.x == 0) {
+        g.theta[j] = eff;
+        if (g.defer) g.theta_lo[j] = th;
+    }
+}
+
+/* Deferred rounds: sorts the round's candidates (the last round's children and the
+   deferred list) into the expansions (b >= max(theta, tau)), the next round's deferred
+   list (theta <= b < tau) and nothing (b < theta: never split, as theta only rises).
+   A grid-stride loop over every job's candidates; appends are wave-aggregated. */
+template <typename T>
+__global__ void __launch_bounds__(256) k_filter_round(GenArgs<T> g, int par) {
+    extern __shared__ uint32_t pre[];  // J + 1 entries
+    const uint32_t J = g.J;
+    for (uint32_t j = threadIdx.x; j < J; j += blockDim.x) pre[j + 1] = g.ncand[par * J + j] + g.ndef[par * J + j];
+    if (threadIdx.x == 0) pre[0] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (uint32_t j = 1; j <= J; ++j) pre[j] += pre[j - 1];
+    __syncthreads();
+    const uint32_t total = pre[J];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    // the loop bound is uniform across the block: waves leave it together
+    for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
+        const uint32_t w = base + threadIdx.x;
+        uint32_t j = 0, slot = 0, kind = 0;  // kind 1: expand, 2: defer
+        T b = T(0);
+        if (w < total) {
+            uint32_t lo = 0, hi = J;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) / 2;
+                if (pre[mid] <= w)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            j = lo;
+            const uint32_t i = w - pre[j], nc = g.ncand[par * J + j];
+            const Job<T>& jb = g.jobs[j];
+            slot = i < nc ? g.cstart[par * J + j] + i : g.def[size_t(par) * g.cap_total + jb.node_base + (i - nc)];
+            b = g.t.b[slot];
+            if (g.t.child[slot] < 0) kind = b >= g.theta[j] ? 1u : (b >= g.theta_lo[j] ? 2u : 0u);
+        }
+        for (uint32_t want = 1; want <= 2; ++want) {
+            const bool me = kind == want;
+            uint64_t act = __ballot(me);
+            while (act) {
+                const int leader = __ffsll((unsigned long long)act) - 1;
+                const uint32_t jl = __shfl(j, leader);
+                const uint64_t same = __ballot(me && j == jl);
+                uint32_t pos0 = 0;
+                const int lane = int(threadIdx.x % 64);
+                if (lane == leader)
+                    pos0 = atomicAdd(want == 1 ? &g.nexp[jl] : &g.ndef[(par ^ 1) * J + jl], uint32_t(__popcll(same)));
+                pos0 = __shfl(pos0, leader);
+                if (me && j == jl) {
+                    const uint32_t pos = pos0 + uint32_t(__popcll(same & ((1ull << lane) - 1ull)));
+                    const uint32_t nb = g.jobs[j].node_base;
+                    if (want == 1) {
+                        g.exp[nb + pos] = slot;
+                    } else {
+                        g.def[size_t(par ^ 1) * g.cap_total + nb + pos] = slot;
+                        atomicAdd(&g.chist[(size_t(par ^ 1) * J + j) * CHIST + (key16(b) >> 4)], 1u);
+                    }
+                }
+                act &= ~same;
+            }
+        }
+    }
 }
 
 template <typename T, int D, typename F, typename M, bool LEARN>
@@ -725,7 +853,7 @@ __global__ void __launch_bounds__(GEN_THREADS) k_expand_round(GenArgs<T> g, cons
     LearnLds<D>* ll = nullptr;
     const uint32_t J = g.J;
     const int team = threadIdx.x / TEAM;
-    for (uint32_t j = threadIdx.x; j < J; j += blockDim.x) pre[j + 1] = g.ncand[par * J + j];
+    for (uint32_t j = threadIdx.x; j < J; j += blockDim.x) pre[j + 1] = g.defer ? g.nexp[j] : g.ncand[par * J + j];
     if (threadIdx.x == 0) pre[0] = 0;
     __syncthreads();
     if (threadIdx.x == 0)

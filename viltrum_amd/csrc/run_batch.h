@@ -117,6 +117,16 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         if (order != VMX_ORDER_CANONICAL)
             throw VmxError(VMX_ERR_INVALID, "learned refiner: canonical region order only");
     }
+    // deferred rounds: a round expands about N / k of a job's candidates, largest b
+    // first (at least 256).  Exact for any k; they pay where rounds hold much work (a
+    // call of 65536 regions or more), and cost an extra kernel per round elsewhere.
+    // VMX_DEFER = k sets k for every call (0: off).  Training pairs (every node the
+    // rounds create) keep the undeferred rounds' nodes
+    int defer_k = n_reg >= 65536 ? DEFER_K : 0;
+    if (const char* e = std::getenv("VMX_DEFER")) defer_k = std::max(0, std::atoi(e));
+    const bool defer = defer_k > 0 && !replay && !pairs_req;
+    if (defer)
+        for (auto& jb : jobs) jb.defer_m = std::max<uint32_t>(256u, jb.N / uint32_t(defer_k));
     if (n_reg > 0x7FFFFFF0ull || total_bins > 0xFFFFFFF0ull) throw VmxError(VMX_ERR_CAPACITY, "batch too large");
     int mb_max = 1;
     for (const auto& jb : jobs) mb_max = std::max(mb_max, int(jb.mb));
@@ -162,9 +172,10 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     uint32_t cap_total = 0;
     std::vector<vmx::JobRt> rt(J);
     // the rounds' shared state, one block so that one copy reads it back: candidate
-    // counts (2 x J), job states (J) and 16 words of counters ([0] rounds, [2..5] the pair
-    // totals after stratification, [8 + d] deepest split count along dim d)
-    const size_t state_bytes = sizeof(uint32_t) * 2 * J + sizeof(vmx::JobRt) * J + 64;
+    // counts (2 x J), job states (J), 16 words of counters ([0] rounds, [2..5] the pair
+    // totals after stratification, [8 + d] deepest split count along dim d) and deferred
+    // candidate counts (2 x J)
+    const size_t state_bytes = sizeof(uint32_t) * 2 * J + sizeof(vmx::JobRt) * J + 64 + sizeof(uint32_t) * 2 * J;
     const size_t slot_bytes = (state_bytes + 255) & ~size_t(255);
     char* d_state = nullptr;
     uint32_t* small = nullptr;
@@ -193,6 +204,13 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         m.take<uint32_t>(size_t(J) * 65536);
         m.take<uint32_t>(2 * J);
         m.take<T>(J);
+        if (defer) {
+            m.take<T>(J);
+            m.take<uint32_t>(J);
+            m.take<uint32_t>(size_t(cap_total) * 2);
+            m.take<uint32_t>(cap_total);
+            m.take<uint32_t>(size_t(J) * 2 * vmx::CHIST);
+        }
         ensure(ctx->tree, m.used);
         Carve c(ctx->tree);
         // jobs, parameters and state first: one contiguous image the host uploads at once
@@ -218,6 +236,16 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         d_cstart = c.take<uint32_t>(2 * J);
         d_theta = c.take<T>(J);
         d_rounds = small;
+        T* d_theta_lo = nullptr;
+        uint32_t *d_nexp = nullptr, *d_def = nullptr, *d_exp = nullptr, *d_chist = nullptr;
+        uint32_t* d_ndef = small + 16;
+        if (defer) {
+            d_theta_lo = c.take<T>(J);
+            d_nexp = c.take<uint32_t>(J);
+            d_def = c.take<uint32_t>(size_t(cap_total) * 2);
+            d_exp = c.take<uint32_t>(cap_total);
+            d_chist = c.take<uint32_t>(size_t(J) * 2 * vmx::CHIST);
+        }
         {
             // jobs, parameters and zeroed state in one copy from pinned staging (after the
             // round loop's two readback slots) where it fits; k_roots clears each job's
@@ -287,6 +315,14 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         g.J = J;
         g.net = learn ? ctx->nn_lds[D].p : nullptr;
         g.deepen = shape.deepen;
+        g.defer = defer ? 1 : 0;
+        g.cap_total = cap_total;
+        g.theta_lo = d_theta_lo;
+        g.ndef = d_ndef;
+        g.def = d_def;
+        g.nexp = d_nexp;
+        g.exp = d_exp;
+        g.chist = d_chist;
         // rounds in chunks of round_chunk (even, so each chunk starts at parity 0); after
         // each chunk the candidate counts, job states and round counter are copied to
         // pinned memory.  The host checks chunk k while chunk k+1 runs, so the device is
@@ -306,6 +342,12 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         const auto hint_it = ctx->round_hint.find(hint_key);
         const uint32_t hint = hint_it == ctx->round_hint.end() ? 0u : hint_it->second;
         const int tail_chunk = std::min(2, round_chunk);
+        const size_t pre_bytes = size_t(J + 1) * sizeof(uint32_t);
+        auto round_kernels = [&](int par) {
+            vmx::k_theta_round<T><<<J, vmx::THETA_THREADS, 0, st>>>(g, par);
+            if (defer) vmx::k_filter_round<T><<<gen_blocks, 256, pre_bytes, st>>>(g, par);
+            launch_expand<T, D, P, Math>(learn, g, d_f, par, gen_blocks, st);
+        };
         auto ensure_graph = [&](vmx_ctx::RoundGraph& rg, int size) {
             std::vector<unsigned char> key(sizeof(g) + 64);
             std::memcpy(key.data(), &g, sizeof(g));
@@ -321,10 +363,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
             rg.graph = nullptr;
             rg.key.clear();
             HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
-            for (int r = 0; r < size; ++r) {
-                vmx::k_theta_round<T><<<J, vmx::THETA_THREADS, 0, st>>>(g, r & 1);
-                launch_expand<T, D, P, Math>(learn, g, d_f, r & 1, gen_blocks, st);
-            }
+            for (int r = 0; r < size; ++r) round_kernels(r & 1);
             hipGraph_t graph = nullptr;
             const hipError_t e1 = hipStreamEndCapture(st, &graph);
             HIP_CHECK(e1);
@@ -346,10 +385,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
                 HIP_CHECK(hipGraphLaunch(tail && tail_chunk < round_chunk ? ctx->rounds_tail.exec : ctx->rounds.exec,
                                          st));
             } else {
-                for (int r = 0; r < size; ++r) {
-                    vmx::k_theta_round<T><<<J, vmx::THETA_THREADS, 0, st>>>(g, r & 1);
-                    launch_expand<T, D, P, Math>(learn, g, d_f, r & 1, gen_blocks, st);
-                }
+                for (int r = 0; r < size; ++r) round_kernels(r & 1);
                 HIP_CHECK(hipGetLastError());
             }
             launched += uint32_t(size);
@@ -366,8 +402,10 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
             launch_chunk(k + 1);
             HIP_CHECK(hipEventSynchronize(ctx->ev[2 + (k & 1)]));
             const uint32_t* nc = reinterpret_cast<const uint32_t*>(slot(k));
+            const uint32_t* nd = reinterpret_cast<const uint32_t*>(slot(k) + sizeof(uint32_t) * 2 * J +
+                                                                   sizeof(vmx::JobRt) * J + 64);
             bool pending = false;
-            for (uint32_t j = 0; j < J; ++j) pending |= nc[j] != 0;
+            for (uint32_t j = 0; j < J; ++j) pending |= nc[j] != 0 || nd[j] != 0;
             if (!pending) break;
         }
         std::memcpy(rt.data(), slot(k) + sizeof(uint32_t) * 2 * J, sizeof(vmx::JobRt) * J);
