@@ -247,8 +247,8 @@ struct PairsReq {
     std::function<void(uint64_t, uint16_t**, float**)> alloc;
 };
 
-constexpr int LEARN_MAX_DIM = 5;
-constexpr int DEFER_K = 32;  // deferred rounds: about N / DEFER_K expansions per job and round  // the fused network fits the expansion kernel's LDS up to here
+constexpr int LEARN_MAX_DIM = 5;  // the fused network fits the expansion kernel's LDS up to here
+constexpr int DEFER_K = 32;        // deferred rounds: about N / DEFER_K expansions per job and round
 
 template <typename T, int D, typename P, typename Math>
 void launch_expand(bool learn, const vmx::GenArgs<T>& g, const P* d_f, int par, uint32_t blocks, hipStream_t st) {
