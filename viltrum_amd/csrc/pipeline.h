@@ -1321,6 +1321,138 @@ __global__ void k_prepare2(const Job<T>* __restrict__ jobs, const int32_t* __res
     if (i == 0) r.fpflag[n] = 0;
 }
 
+/* k_prepare2 for regions of 4..6 dims: one wave per region, its 3^D node values and
+   every intermediate array in LDS (a thread per region spills them to scratch).  Each
+   pass of a fold or of the monomial transform runs the independent lines of one dim
+   across the lanes, with the operations k_prepare2 applies to each line: the results
+   are the same bit for bit. */
+template <int D>
+constexpr int prep_wpb() {  // waves (regions) per block
+    return D >= 6 ? 2 : 4;
+}
+template <typename T, int D>
+struct PrepWave {
+    static constexpr int K = pow3(D);
+    static constexpr int WPB = prep_wpb<D>();
+    T v[K], s[2][K], tz[2][K], c[K];
+    double w[K];
+};
+
+template <typename T, int D>
+__global__ void __launch_bounds__(64 * prep_wpb<D>())
+    k_prepare_wave(const Job<T>* __restrict__ jobs, const int32_t* __restrict__ leaf_slot, TreeArrays<T> t,
+                   RegionArrays<T> r, uint32_t n, int mb, int merge, uint32_t* npairs,
+                   unsigned long long* pairs64) {
+    using PW = PrepWave<T, D>;
+    constexpr int K = PW::K, WPB = PW::WPB;
+    __shared__ PW sw[WPB];
+    const int wave = int(threadIdx.x >> 6), lane = int(threadIdx.x & 63);
+    const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x;
+    // the footprints' pair counts (n + 1 entries) and their 64-bit total start at zero
+    if (gt < n) npairs[gt] = 0;
+    if (gt == 0) {
+        npairs[n] = 0;
+        *pairs64 = 0;
+    }
+    const uint32_t i = blockIdx.x * WPB + uint32_t(wave);
+    PW& P = sw[wave];
+    // every wave runs every pass (the passes end in block barriers); only regions that
+    // exist write results
+    const bool real = i < n;
+    const bool valid = real && leaf_slot[i] >= 0;
+    if (real && !valid) {  // selection did not close (the host rejects the result): an empty region
+        for (int k = lane; k < D; k += 64) r.lo[size_t(i) * D + k] = r.hi[size_t(i) * D + k] = T(0);
+        for (int k = lane; k < mb; k += 64) r.marg[size_t(i) * mb + k] = 0.0;
+        for (int k = lane; k < samp_stride<D>(); k += 64) r.samp[size_t(i) * samp_stride<D>() + k] = T(0);
+        if (lane == 0) {
+            r.vol[i] = T(0);
+            r.job[i] = 0;
+            r.fpflag[i] = 1;
+            if (i == 0) r.fpflag[n] = 0;
+        }
+    }
+    const size_t s = valid ? size_t(leaf_slot[i]) : 0;
+    for (int k = lane; k < K; k += 64) {
+        const T x = valid ? t.v[s * K + k] : T(0);
+        P.v[k] = x;
+        P.s[0][k] = P.tz[0][k] = P.c[k] = x;
+        P.w[k] = double(x);
+        if (valid && r.v) r.v[size_t(i) * K + k] = x;
+    }
+    __syncthreads();
+    // Region::error() (src/newton-cotes/region.h:421-424): dim 0 folded first
+    int cur = 0;
+    if (r.err) {
+        for (int m = K; m > 1; m /= 3) {
+            for (int q = lane; q < m / 3; q += 64) {
+                P.s[cur ^ 1][q] = simpson(P.s[cur][3 * q], P.s[cur][3 * q + 1], P.s[cur][3 * q + 2]);
+                P.tz[cur ^ 1][q] = trapezoid(P.tz[cur][3 * q], P.tz[cur][3 * q + 2]);
+            }
+            cur ^= 1;
+            __syncthreads();
+        }
+    }
+    // FP64 bin-dim marginal (bin_marginal): the non-bin dims folded, the last first
+    int db = 0;
+    for (int q = mb; q > 1; q /= 3) ++db;
+    int nn = K;
+    for (int d = D - 1; d >= db; --d) {
+        nn /= 3;
+        for (int q = lane; q < nn; q += 64) P.w[q] = simpson_subrange(0.0, 1.0, P.w[q], P.w[q + nn], P.w[q + 2 * nn]);
+        __syncthreads();
+    }
+    // monomial coefficients (monomials): dim 0 first, one line per lane
+    int stride = 1;
+    for (int d = 0; d < D; ++d) {
+        for (int q = lane; q < K / 3; q += 64) {
+            const int idx = (q / stride) * stride * 3 + q % stride;
+            const T p0 = P.c[idx], p1 = P.c[idx + stride], p2 = P.c[idx + 2 * stride];
+            P.c[idx] = p0;
+            P.c[idx + stride] = -3 * p0 + 4 * p1 - p2;
+            P.c[idx + 2 * stride] = 2 * p0 - 4 * p1 + 2 * p2;
+        }
+        stride *= 3;
+        __syncthreads();
+    }
+    if (!valid) return;
+    const uint32_t j = t.job[s];
+    const Job<T>& jb = jobs[j];
+    T* rec = r.samp + size_t(i) * samp_stride<D>();
+    for (int k = lane; k < mb; k += 64) r.marg[size_t(i) * mb + k] = P.w[k];
+    for (int k = lane; k < K; k += 64) rec[3 * D + k] = P.c[k];
+    for (int k = 3 * D + K + lane; k < samp_stride<D>(); k += 64) rec[k] = T(0);
+    if (lane < D) {
+        const T lo = t.lo[s * D + lane], hi = t.hi[s * D + lane];
+        r.lo[size_t(i) * D + lane] = lo;
+        r.hi[size_t(i) * D + lane] = hi;
+        rec[lane] = lo;
+        rec[D + lane] = hi;
+        rec[2 * D + lane] = hi > lo ? T(1) / (hi - lo) : T(0);
+    }
+    if (lane == 0) {
+        T lo[D], hi[D];
+        for (int d = 0; d < D; ++d) {
+            lo[d] = t.lo[s * D + d];
+            hi[d] = t.hi[s * D + d];
+        }
+        const T vol = box_volume<T, D>(lo, hi);
+        r.vol[i] = vol;
+        if (r.err) r.err[i] = vol * (P.s[cur][0] - P.tz[cur][0]);
+        r.job[i] = j;
+        uint32_t flag = 1;
+        // the region roulettes weigh each region of a bin: their jobs keep one region per
+        // footprint, the batch's other jobs merge as they would alone
+        if (merge && form_roulette(jb.form) == 0 && i != jb.reg_base && leaf_slot[i - 1] >= 0) {
+            const size_t sp = size_t(leaf_slot[i - 1]);
+            flag = 0;
+            for (int d = 0; d < jb.db; ++d)
+                if (t.code[sp * D + d] != t.code[s * D + d]) flag = 1;
+        }
+        r.fpflag[i] = flag;
+        if (i == 0) r.fpflag[n] = 0;
+    }
+}
+
 /* One thread per footprint: bin-dim box, pixel box, and merged FP64 marginal
    (sum of vol_r * marginal_r over its regions, in region order) or, unmerged, the
    single region's (vol, marginal) so per-region sums keep the reference's rounding. */

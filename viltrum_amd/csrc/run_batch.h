@@ -645,8 +645,14 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     // mb differs per job only through db; all jobs of a batch share the footprint stride
     for (const auto& jb : jobs)
         if (jb.mb != mb_max) throw VmxError(VMX_ERR_INVALID, "one bin_dims per batch");
-    vmx::k_prepare2<T, D><<<reg_blocks, 256, 0, st>>>(d_jobs, d_leaf, t, ra, uint32_t(n_reg), mb_max, merge,
-                                                      fa.npairs, d_pairs64);
+    if constexpr (D >= 4) {
+        constexpr unsigned WPB = vmx::PrepWave<T, D>::WPB;
+        vmx::k_prepare_wave<T, D><<<unsigned((n_reg + WPB - 1) / WPB), 64 * WPB, 0, st>>>(
+            d_jobs, d_leaf, t, ra, uint32_t(n_reg), mb_max, merge, fa.npairs, d_pairs64);
+    } else {
+        vmx::k_prepare2<T, D><<<reg_blocks, 256, 0, st>>>(d_jobs, d_leaf, t, ra, uint32_t(n_reg), mb_max, merge,
+                                                          fa.npairs, d_pairs64);
+    }
     HIP_CHECK(hipGetLastError());
     size_t bytes = scan_reg_bytes;
     HIP_CHECK(hipError_t(rocprim::exclusive_scan(d_scan_tmp, bytes, ra.fpflag, ra.fpid, 0u, size_t(n_reg + 1),
