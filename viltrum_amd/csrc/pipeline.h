@@ -1453,6 +1453,23 @@ __global__ void __launch_bounds__(64 * prep_wpb<D>())
     }
 }
 
+/* The merged FP64 marginal of a footprint's regions [i, e): sum of vol_r * marginal_r
+   in region order, MB = 3^db entries held in registers. */
+template <typename T, int MB>
+__device__ inline void merge_marginals(const RegionArrays<T>& r, const FootArrays<T>& f, uint32_t i, uint32_t e,
+                                       uint32_t id) {
+    double acc[MB];
+#pragma unroll
+    for (int k = 0; k < MB; ++k) acc[k] = 0.0;
+    for (uint32_t q = i; q < e; ++q) {
+        const double vol = double(r.vol[q]);
+#pragma unroll
+        for (int k = 0; k < MB; ++k) acc[k] += vol * r.marg[size_t(q) * MB + k];
+    }
+#pragma unroll
+    for (int k = 0; k < MB; ++k) f.marg[size_t(id) * MB + k] = acc[k];
+}
+
 /* One thread per footprint: bin-dim box, pixel box, and merged FP64 marginal
    (sum of vol_r * marginal_r over its regions, in region order) or, unmerged, the
    single region's (vol, marginal) so per-region sums keep the reference's rounding. */
@@ -1485,13 +1502,12 @@ __global__ void k_footprints(const Job<T>* __restrict__ jobs, RegionArrays<T> r,
         f.npairs[id] = (bx[1] - bx[0]) * (bx[3] - bx[2]) * (bx[5] - bx[4]);
     }
     if (merge && form_roulette(jb.form) == 0) {
-        double acc[27];
-        for (int k = 0; k < mb; ++k) acc[k] = 0.0;
-        for (uint32_t q = i; q < e; ++q) {
-            const double vol = double(r.vol[q]);
-            for (int k = 0; k < mb; ++k) acc[k] += vol * r.marg[size_t(q) * mb + k];
-        }
-        for (int k = 0; k < mb; ++k) f.marg[size_t(id) * mb + k] = acc[k];
+        if (mb == 3)
+            merge_marginals<T, 3>(r, f, i, e, id);
+        else if (mb == 9)
+            merge_marginals<T, 9>(r, f, i, e, id);
+        else
+            merge_marginals<T, 27>(r, f, i, e, id);
         f.scale[id] = 1.0;
     } else {
         for (int k = 0; k < mb; ++k) f.marg[size_t(id) * mb + k] = r.marg[size_t(i) * mb + k];
