@@ -99,8 +99,10 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     // FP32 regions of 4..6 dims with the uniform estimator go through the region-major
     // per-bin kernel (bins_rm.h); k_bins1 takes the batch's other jobs
     constexpr bool RM_KIND = std::is_same<T, float>::value && D >= 4 && !vmx::IsFubini<P>::value;
-    bool all_rm = RM_KIND;
-    if constexpr (RM_KIND) {
+    // VMX_NO_RM=1 sends these jobs through k_bins1 too (the same estimator, other draws)
+    static const bool rm_env = !(std::getenv("VMX_NO_RM") && std::atoi(std::getenv("VMX_NO_RM")));
+    bool all_rm = RM_KIND && rm_env;
+    if (RM_KIND && rm_env) {
         for (const auto& jb : jobs) {
             const bool r = vmx::rm_job(jb.form, jb.N);
             any_rm |= r;
