@@ -619,7 +619,7 @@ def run_config5(args, V, hip):
         "evals_per_s": sum(x.evals_regions + x.evals_residual for x in st) / (ms * 1e-3),
         "phase_ms": {k: float(getattr(st[0], k)) for k in phases},
         "heuristic_phase_ms": {k: float(getattr(st_h[0], k)) for k in phases},
-        "roofline": {"bound": "valu", "kernel": "k_bins1 (per-bin CV + residual Monte Carlo, batched)",
+        "roofline": {"bound": "valu", "kernel": "k_bins_rm (region-major per-bin CV + residual Monte Carlo, batched)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel_ms": ms_k},
         "stats": {"nodes": int(sum(x.nodes for x in st)), "rounds": int(st[0].rounds),
