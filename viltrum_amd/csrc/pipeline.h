@@ -105,6 +105,38 @@ __device__ inline void suffix_sum256(uint32_t* s) {
     __syncthreads();
 }
 
+/* pre[0] = 0, pre[j + 1] = pre[j] + count(j) for j < J, in LDS; every thread of the
+   block calls it (blocks of 64..1024 threads).  Each thread sums a run of consecutive
+   jobs, a shuffle scan and one exchange of the wave totals give the runs' offsets. */
+template <typename Count>
+__device__ inline void block_prefix(uint32_t* pre, uint32_t J, Count count) {
+    __shared__ uint32_t wave_tot[16];
+    const uint32_t per = (J + blockDim.x - 1) / blockDim.x;
+    const uint32_t j0 = threadIdx.x * per;
+    uint32_t local = 0;
+    for (uint32_t k = 0; k < per && j0 + k < J; ++k) {
+        const uint32_t c = count(j0 + k);
+        pre[j0 + k + 1] = c;
+        local += c;
+    }
+    const int lane = int(threadIdx.x & 63), wave = int(threadIdx.x >> 6);
+    uint32_t incl = local;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t v = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += v;
+    }
+    if (lane == 63) wave_tot[wave] = incl;
+    if (threadIdx.x == 0) pre[0] = 0;
+    __syncthreads();
+    uint32_t run = incl - local;
+    for (int w = 0; w < wave; ++w) run += wave_tot[w];
+    for (uint32_t k = 0; k < per && j0 + k < J; ++k) {
+        run += pre[j0 + k + 1];
+        pre[j0 + k + 1] = run;
+    }
+    __syncthreads();
+}
+
 /* Bucket (16-bit key of b) holding the need-th largest b of one job, and the number of
    nodes in strictly higher buckets.  Requires total >= need >= 1.  NT threads (a
    multiple of 256), all of which call it.  The coarse counts (top 8 bits of the key)
@@ -785,12 +817,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_filter_round(GenArgs<T> g, int par) {
     extern __shared__ uint32_t pre[];  // J + 1 entries
     const uint32_t J = g.J;
-    for (uint32_t j = threadIdx.x; j < J; j += blockDim.x) pre[j + 1] = g.ncand[par * J + j] + g.ndef[par * J + j];
-    if (threadIdx.x == 0) pre[0] = 0;
-    __syncthreads();
-    if (threadIdx.x == 0)
-        for (uint32_t j = 1; j <= J; ++j) pre[j] += pre[j - 1];
-    __syncthreads();
+    block_prefix(pre, J, [&](uint32_t j) { return g.ncand[par * J + j] + g.ndef[par * J + j]; });
     const uint32_t total = pre[J];
     const uint32_t stride = gridDim.x * blockDim.x;
     // the loop bound is uniform across the block: waves leave it together
@@ -853,12 +880,7 @@ __global__ void __launch_bounds__(GEN_THREADS) k_expand_round(GenArgs<T> g, cons
     LearnLds<D>* ll = nullptr;
     const uint32_t J = g.J;
     const int team = threadIdx.x / TEAM;
-    for (uint32_t j = threadIdx.x; j < J; j += blockDim.x) pre[j + 1] = g.defer ? g.nexp[j] : g.ncand[par * J + j];
-    if (threadIdx.x == 0) pre[0] = 0;
-    __syncthreads();
-    if (threadIdx.x == 0)
-        for (uint32_t j = 1; j <= J; ++j) pre[j] += pre[j - 1];
-    __syncthreads();
+    block_prefix(pre, J, [&](uint32_t j) { return g.defer ? g.nexp[j] : g.ncand[par * J + j]; });
     const uint32_t total = pre[J];
     if (blockIdx.x == 0 && threadIdx.x == 0 && total > 0) atomicAdd(g.rounds, 1u);
     if (uint64_t(blockIdx.x) * TPB >= total) return;  // no candidate for this block (uniform)
