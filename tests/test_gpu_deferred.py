@@ -2,7 +2,8 @@
 those in [theta, tau) for later rounds) select exactly the regions of the undeferred
 rounds: region lists and bins bit for bit, for any tau (VMX_DEFER = k sets tau near the
 (N/k)-th largest candidate; 0 turns deferral off), in FP64 and FP32, for single calls
-and batches, with the heuristic and the learned priority."""
+and batches, with the heuristic and the learned priority, with the candidates sorted by
+each job's theta block (jobs of up to 32768 regions) and by the grid-wide filter."""
 import os
 
 import numpy as np
@@ -40,7 +41,8 @@ def _single(ctx, f, res, it, spp, scalar, dt):
     (I.config1(), (32, 32), 4095, V.F64),
     (I.poly([(1.0, (1, 2, 0)), (0.5, (2, 0, 1))], 3, step=([1, 1, 1], 1.5, 0.25)), (12, 10), 3000, V.F64),
     (I.config5(3), (16, 16), 4095, V.F32),
-], ids=["config1_fp64", "poly3_fp64", "config5_fp32"])
+    (I.config1(), (64, 64), 39999, V.F64),  # above 32768 regions: the grid-wide filter kernel
+], ids=["config1_fp64", "poly3_fp64", "config5_fp32", "config1_fp64_40k"])
 def test_single_call_regions_match(ctx, defer, f, res, it, scalar):
     dt = np.float64 if scalar == V.F64 else np.float32
     defer(0)

@@ -373,9 +373,10 @@ struct GenArgs {
     uint32_t max_rounds;
     const void* net;   // the priority network's LDS image (nn::NetW<KP>, packed once per network)
     int32_t deepen;    // 1: a split's children with b >= theta are split in the same round
-    // deferred rounds (defer = 1): a round expands its candidates with b >= max(theta,
-    // tau), tau about the defer_m-th largest candidate b; those in [theta, tau) wait for
-    // a later round in a list, those below theta can never be split
+    // deferred rounds (defer = 1 or 2): a round expands its candidates with b >=
+    // max(theta, tau), tau about the defer_m-th largest candidate b; those in [theta,
+    // tau) wait for a later round in a list, those below theta can never be split.  The
+    // candidates are sorted by k_filter_round (1) or by the job's theta block (2)
     int32_t defer;
     uint32_t cap_total;
     T* theta_lo;       // [J] theta itself (theta then holds max(theta, tau))
@@ -806,6 +807,33 @@ __global__ void __launch_bounds__(THETA_THREADS) k_theta_round(GenArgs<T> g, int
     if (threadIdx.x == 0) {
         g.theta[j] = eff;
         if (g.defer) g.theta_lo[j] = th;
+    }
+    if (g.defer == 2) {
+        // jobs of up to 32768 regions: this block sorts its own job's candidates, as
+        // k_filter_round does for larger ones (one launch less per round)
+        __shared__ uint32_t s_nexp, s_ndef;
+        if (threadIdx.x == 0) {
+            s_nexp = 0;
+            s_ndef = 0;
+        }
+        __syncthreads();  // also orders the candidate histogram's clearing before its counts
+        const uint32_t nc = g.ncand[par * J + j], cs = g.cstart[par * J + j];
+        for (uint32_t i = threadIdx.x; i < ncand; i += blockDim.x) {
+            const uint32_t slot = i < nc ? cs + i : g.def[size_t(par) * g.cap_total + jb.node_base + (i - nc)];
+            if (g.t.child[slot] >= 0) continue;
+            const T b = g.t.b[slot];
+            if (b >= eff) {
+                g.exp[jb.node_base + atomicAdd(&s_nexp, 1u)] = slot;
+            } else if (b >= th) {
+                g.def[size_t(par ^ 1) * g.cap_total + jb.node_base + atomicAdd(&s_ndef, 1u)] = slot;
+                atomicAdd(&g.chist[(size_t(par ^ 1) * J + j) * CHIST + (key16(b) >> 4)], 1u);
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            g.nexp[j] = s_nexp;
+            g.ndef[(par ^ 1) * J + j] = s_ndef;
+        }
     }
 }
 

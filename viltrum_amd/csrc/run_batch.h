@@ -317,7 +317,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         g.J = J;
         g.net = learn ? ctx->nn_lds[D].p : nullptr;
         g.deepen = shape.deepen;
-        g.defer = defer ? 1 : 0;
+        g.defer = defer ? (n_max <= 32768 ? 2 : 1) : 0;
         g.cap_total = cap_total;
         g.theta_lo = d_theta_lo;
         g.ndef = d_ndef;
@@ -347,7 +347,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         const size_t pre_bytes = size_t(J + 1) * sizeof(uint32_t);
         auto round_kernels = [&](int par) {
             vmx::k_theta_round<T><<<J, vmx::THETA_THREADS, 0, st>>>(g, par);
-            if (defer) vmx::k_filter_round<T><<<gen_blocks, 256, pre_bytes, st>>>(g, par);
+            if (g.defer == 1) vmx::k_filter_round<T><<<gen_blocks, 256, pre_bytes, st>>>(g, par);
             launch_expand<T, D, P, Math>(learn, g, d_f, par, gen_blocks, st);
         };
         auto ensure_graph = [&](vmx_ctx::RoundGraph& rg, int size) {
