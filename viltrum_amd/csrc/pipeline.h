@@ -358,6 +358,14 @@ __global__ void __launch_bounds__(256) k_roots(const Job<T>* __restrict__ jobs, 
     }
 }
 
+/* A deferred candidate: its slot and b (a deferred node is never split while it waits,
+   so the filter reads neither from the tree). */
+template <typename T>
+struct DefEnt {
+    uint32_t slot;
+    T b;
+};
+
 template <typename T>
 struct GenArgs {
     const Job<T>* jobs;
@@ -381,7 +389,7 @@ struct GenArgs {
     uint32_t cap_total;
     T* theta_lo;       // [J] theta itself (theta then holds max(theta, tau))
     uint32_t* ndef;    // [2*J] deferred candidates
-    uint32_t* def;     // [2][cap_total] their slots, job j's from node_base_j
+    DefEnt<T>* def;    // [2][cap_total] their slots and b, job j's from node_base_j
     uint32_t* nexp;    // [J] the round's expansions
     uint32_t* exp;     // [cap_total] their slots, job j's from node_base_j
     uint32_t* chist;   // [2][J][CHIST] the next round's candidates by the top 12 bits of key16(b)
@@ -819,13 +827,21 @@ __global__ void __launch_bounds__(THETA_THREADS) k_theta_round(GenArgs<T> g, int
         __syncthreads();  // also orders the candidate histogram's clearing before its counts
         const uint32_t nc = g.ncand[par * J + j], cs = g.cstart[par * J + j];
         for (uint32_t i = threadIdx.x; i < ncand; i += blockDim.x) {
-            const uint32_t slot = i < nc ? cs + i : g.def[size_t(par) * g.cap_total + jb.node_base + (i - nc)];
-            if (g.t.child[slot] >= 0) continue;
-            const T b = g.t.b[slot];
+            uint32_t slot;
+            T b;
+            if (i < nc) {  // a child of the last round (split already if a deepened pass took it)
+                slot = cs + i;
+                if (g.t.child[slot] >= 0) continue;
+                b = g.t.b[slot];
+            } else {
+                const DefEnt<T> e = g.def[size_t(par) * g.cap_total + jb.node_base + (i - nc)];
+                slot = e.slot;
+                b = e.b;
+            }
             if (b >= eff) {
                 g.exp[jb.node_base + atomicAdd(&s_nexp, 1u)] = slot;
             } else if (b >= th) {
-                g.def[size_t(par ^ 1) * g.cap_total + jb.node_base + atomicAdd(&s_ndef, 1u)] = slot;
+                g.def[size_t(par ^ 1) * g.cap_total + jb.node_base + atomicAdd(&s_ndef, 1u)] = DefEnt<T>{slot, b};
                 atomicAdd(&g.chist[(size_t(par ^ 1) * J + j) * CHIST + (key16(b) >> 4)], 1u);
             }
         }
@@ -865,9 +881,17 @@ __global__ void __launch_bounds__(256) k_filter_round(GenArgs<T> g, int par) {
             j = lo;
             const uint32_t i = w - pre[j], nc = g.ncand[par * J + j];
             const Job<T>& jb = g.jobs[j];
-            slot = i < nc ? g.cstart[par * J + j] + i : g.def[size_t(par) * g.cap_total + jb.node_base + (i - nc)];
-            b = g.t.b[slot];
-            if (g.t.child[slot] < 0) kind = b >= g.theta[j] ? 1u : (b >= g.theta_lo[j] ? 2u : 0u);
+            bool open = true;
+            if (i < nc) {  // a child of the last round (split already if a deepened pass took it)
+                slot = g.cstart[par * J + j] + i;
+                open = g.t.child[slot] < 0;
+                b = g.t.b[slot];
+            } else {
+                const DefEnt<T> e = g.def[size_t(par) * g.cap_total + jb.node_base + (i - nc)];
+                slot = e.slot;
+                b = e.b;
+            }
+            if (open) kind = b >= g.theta[j] ? 1u : (b >= g.theta_lo[j] ? 2u : 0u);
         }
         for (uint32_t want = 1; want <= 2; ++want) {
             const bool me = kind == want;
@@ -887,7 +911,7 @@ __global__ void __launch_bounds__(256) k_filter_round(GenArgs<T> g, int par) {
                     if (want == 1) {
                         g.exp[nb + pos] = slot;
                     } else {
-                        g.def[size_t(par ^ 1) * g.cap_total + nb + pos] = slot;
+                        g.def[size_t(par ^ 1) * g.cap_total + nb + pos] = DefEnt<T>{slot, b};
                         atomicAdd(&g.chist[(size_t(par ^ 1) * J + j) * CHIST + (key16(b) >> 4)], 1u);
                     }
                 }

@@ -209,7 +209,7 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         if (defer) {
             m.take<T>(J);
             m.take<uint32_t>(J);
-            m.take<uint32_t>(size_t(cap_total) * 2);
+            m.take<vmx::DefEnt<T>>(size_t(cap_total) * 2);
             m.take<uint32_t>(cap_total);
             m.take<uint32_t>(size_t(J) * 2 * vmx::CHIST);
         }
@@ -239,12 +239,13 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
         d_theta = c.take<T>(J);
         d_rounds = small;
         T* d_theta_lo = nullptr;
-        uint32_t *d_nexp = nullptr, *d_def = nullptr, *d_exp = nullptr, *d_chist = nullptr;
+        uint32_t *d_nexp = nullptr, *d_exp = nullptr, *d_chist = nullptr;
+        vmx::DefEnt<T>* d_def = nullptr;
         uint32_t* d_ndef = small + 16;
         if (defer) {
             d_theta_lo = c.take<T>(J);
             d_nexp = c.take<uint32_t>(J);
-            d_def = c.take<uint32_t>(size_t(cap_total) * 2);
+            d_def = c.take<vmx::DefEnt<T>>(size_t(cap_total) * 2);
             d_exp = c.take<uint32_t>(cap_total);
             d_chist = c.take<uint32_t>(size_t(J) * 2 * vmx::CHIST);
         }
