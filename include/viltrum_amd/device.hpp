@@ -28,12 +28,16 @@
  * refused with VMX_ERR_UNSUPPORTED.  Each instantiation compiles the pipeline's kernels
  * for the functor (about a minute of hipcc per scalar type).
  *
+ * As the inner integrand of the Fubini form (integrator_crespo2021_infinite<RD> over a
+ * finite range), wrap it with viltrum_amd::device_integrand_fubini<D[, RD...]>(f).
+ *
  * Compile with hipcc --offload-arch=gfx950 -std=c++17 -I<repo>/include
  * -Xclang -target-feature -Xclang -packed-fp32-ops, and link -lviltrum_amd.
  */
 #ifndef VILTRUM_AMD_DEVICE_HPP
 #define VILTRUM_AMD_DEVICE_HPP
 
+#include <initializer_list>
 #include <memory>
 #include <tuple>
 #include <type_traits>
@@ -65,6 +69,26 @@ int32_t run_device_integrand(vmx_ctx* ctx, int32_t scalar, const void* group, vo
     throw vmxe::VmxError(VMX_ERR_UNSUPPORTED,
                          scalar == VMX_F32 ? "user integrand not instantiated for float bins (device_integrand<D, float>)"
                                            : "user integrand not instantiated for double bins (device_integrand<D, double>)");
+}
+
+/* The hook for the Fubini form with this functor (FP32, D dims) as the inner integrand:
+   region dims RDs... (1..3, below D) are instantiated. */
+template <int D, typename Fn, int... RDs>
+int32_t run_device_integrand_fubini(vmx_ctx* ctx, int32_t rd, const void* group, void* stream) {
+    const auto& jobs = *static_cast<const std::vector<vmxe::JobIn>*>(group);
+    const auto st = static_cast<hipStream_t>(stream);
+    using C = vmx::Custom<float, D, Fn>;
+    int32_t rc = VMX_ERR_UNSUPPORTED;
+    bool done = false;
+    (void)std::initializer_list<int>{
+        (rd == RDs && !done
+             ? (rc = vmxe::run_batch<float, RDs, vmx::Fubini<float, C, D, RDs>, vmx::FastMath>(ctx, jobs, st, nullptr),
+                done = true, 0)
+             : 0)...};
+    if (!done)
+        throw vmxe::VmxError(VMX_ERR_UNSUPPORTED,
+                             "user integrand not instantiated for the Fubini form at these region dims");
+    return rc;
 }
 
 template <typename Fn>
@@ -99,6 +123,28 @@ Integrand device_integrand(const Fn& fn) {
     static_assert(sizeof(Fn) <= VMX_CUSTOM_MAX_BYTES, "a user integrand is staged in LDS: at most 2 KB");
     using Scalars = std::conditional_t<sizeof...(Ts) == 0, std::tuple<float>, std::tuple<Ts...>>;
     return make_device_integrand<int(D), Fn>(fn, static_cast<Scalars*>(nullptr));
+}
+
+/* A functor of D dims for the Fubini form (integrator_crespo2021_infinite<RD> over a
+   finite range, src/integrator-crespo2021.h:24-44) as well as the plain integrators:
+   FP32, region dims RDs... (each 1..3 and below D; all of them if none is given).  Each
+   region-dim count instantiates the pipeline once more. */
+template <std::size_t D, int... RDs, typename Fn>
+Integrand device_integrand_fubini(const Fn& fn) {
+    static_assert(D >= 2 && D <= 5, "the Fubini form takes inner integrands of 2..5 dims");
+    static_assert((... && (RDs >= 1 && RDs <= 3 && RDs < int(D))), "region dims 1..3, below the integrand's");
+    Integrand f = device_integrand<D, float>(fn);
+    auto* desc = const_cast<vmx_custom_integrand*>(static_cast<const vmx_custom_integrand*>(f.desc.params));
+    if constexpr (sizeof...(RDs) > 0) {
+        desc->run_fubini = &detail::run_device_integrand_fubini<int(D), Fn, RDs...>;
+    } else if constexpr (D == 2) {
+        desc->run_fubini = &detail::run_device_integrand_fubini<2, Fn, 1>;
+    } else if constexpr (D == 3) {
+        desc->run_fubini = &detail::run_device_integrand_fubini<3, Fn, 1, 2>;
+    } else {
+        desc->run_fubini = &detail::run_device_integrand_fubini<int(D), Fn, 1, 2, 3>;
+    }
+    return f;
 }
 
 }  // namespace viltrum_amd

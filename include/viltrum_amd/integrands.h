@@ -95,7 +95,7 @@ typedef struct vmx_lobes5_params {
    rest's volume is the product of the first rest_count extents.  FP32 only, as the
    reference. */
 typedef struct vmx_fubini_params {
-    int32_t inner_family;                   /* VMX_FAMILY_POLY, _BLOBS (3-D), _LOBES5, or _DYSON */
+    int32_t inner_family;                   /* VMX_FAMILY_POLY, _BLOBS (3-D), _LOBES5, _CUSTOM or _DYSON */
     int32_t inner_dim;                      /* > the descriptor's dim, <= 5; or VMX_INFINITE_DIM */
     uint32_t mc_samples;                    /* >= 1 */
     uint32_t seed;                          /* stream of the Monte Carlo samples */
@@ -119,13 +119,17 @@ typedef struct vmx_dyson_params {
    that share `run` (one functor type and dimension) to run(), which receives the
    library's internal job list (group) and training-pair request (pairs, or null) and
    returns a vmx_status.  `functor` points at the functor's bytes in host memory
-   (trivially copyable, at most VMX_CUSTOM_MAX_BYTES: it is staged in LDS).  Not for the
-   Fubini form, vmx_eval or vmx_exact_bins. */
+   (trivially copyable, at most VMX_CUSTOM_MAX_BYTES: it is staged in LDS).  As the inner
+   integrand of the Fubini form (VMX_FAMILY_FUBINI with inner_family VMX_FAMILY_CUSTOM),
+   the library calls run_fubini with the form's region dims instead; it is NULL unless
+   the functor was instantiated for that form (viltrum_amd::device_integrand_fubini).
+   Not for vmx_eval or vmx_exact_bins. */
 #define VMX_CUSTOM_MAX_BYTES 2048
 struct vmx_ctx;
 typedef struct vmx_custom_integrand {
     int32_t (*run)(struct vmx_ctx* ctx, int32_t scalar, const void* group, void* stream, void* pairs);
     const void* functor;
+    int32_t (*run_fubini)(struct vmx_ctx* ctx, int32_t region_dims, const void* group, void* stream);
 } vmx_custom_integrand;
 
 /* Integrand descriptor handed to every entry point. */

@@ -245,7 +245,20 @@ int main(int argc, char** argv) {
                     (unsigned long long)na);
     }
 
-    // 7. bins of a type the functor was not instantiated for are refused
+    // 7. the Fubini form (integrator_crespo2021_infinite<2> over a finite 3-D range) with a
+    //    user functor as its inner integrand: the built-in polynomial's bins bit for bit
+    {
+        const PolyFn<3> p3{3, {1.0, 0.5, 0.25, 0.0}, {{1, 2, 0}, {2, 0, 1}, {0, 0, 0}, {0, 0, 0}}, {1.0, 1.0, 1.0},
+                           1.5, 0.25};
+        const auto it = vi::integrator_crespo2021_infinite<2>(300, 4, 8, 6);
+        Grid<float> a(16, 12), b(16, 12);
+        vi::integrate(it, a, std::array<std::size_t, 2>{16, 12}, vi::device_integrand_fubini<3, 2>(p3),
+                      vi::range_primary<3>());
+        vi::integrate(it, b, std::array<std::size_t, 2>{16, 12}, vi::poly(poly_params(p3), 3), vi::range_primary<3>());
+        std::printf("fubini3_f32 %d\n", int(same_bits(a.v, b.v)));
+    }
+
+    // 8. bins of a type the functor was not instantiated for are refused
     {
         Grid<double> g(4, 4);
         std::string msg = "none";

@@ -3,7 +3,9 @@ the way the reference's users write integrands (tests/cpp/custom.hip): a functor
 to a built-in polynomial gives that family's bins and training pairs bit for bit (FP64
 reference order, FP32, and 4 dims through the region-major kernel); lambdas in the
 adapter's call forms f(x, y) and f(std::array) give the closed-form bins; a batch of one
-functor type gives its single calls; bins of a type not instantiated are refused."""
+functor type gives its single calls; as the inner integrand of the Fubini form a functor
+gives the equal built-in polynomial's bins bit for bit; bins of a type not instantiated
+are refused."""
 import os
 import subprocess
 
@@ -31,7 +33,7 @@ def _edges(n):
     return np.arange(n + 1, dtype=np.float64) / n
 
 
-@pytest.mark.parametrize("case", ["poly2_f64", "poly2_f32", "poly4_f32", "batch3_f32", "unsupported"])
+@pytest.mark.parametrize("case", ["poly2_f64", "poly2_f32", "poly4_f32", "batch3_f32", "fubini3_f32", "unsupported"])
 def test_flags(run, case):
     assert run[1][case].split()[0] == "1", case
 

@@ -82,6 +82,11 @@ struct Custom {
     }
 };
 
+template <typename T, int DIM, typename Fn>
+struct FamilyParams<Custom<T, DIM, Fn>> {
+    using type = vmx_custom_integrand;
+};
+
 }  // namespace vmx
 
 namespace vmxe {

@@ -54,6 +54,16 @@ int dispatch_T(vmx_ctx* ctx, int family, int D, const std::vector<JobIn>& jobs, 
                     return run_batch<T, 2, vmx::Fubini<T, vmx::Poly<T>, 3, 2>, Math>(ctx, jobs, st, pairs);
                 if (D == 1 && fam == VMX_FAMILY_POLY && dd == 2)
                     return run_batch<T, 1, vmx::Fubini<T, vmx::Poly<T>, 2, 1>, Math>(ctx, jobs, st, pairs);
+                // a user-defined inner integrand: the Fubini instantiation its own
+                // translation unit made
+                if (fam == VMX_FAMILY_CUSTOM) {
+                    const auto* c = static_cast<const vmx_custom_integrand*>(fp->inner);
+                    if (!c->run_fubini)
+                        throw VmxError(VMX_ERR_UNSUPPORTED,
+                                       "user integrand not instantiated for the Fubini form (device_integrand_fubini)");
+                    if (pairs) throw VmxError(VMX_ERR_UNSUPPORTED, "training pairs: not for the Fubini form");
+                    return c->run_fubini(ctx, D, &jobs, st);
+                }
                 // over an infinite range: the Dyson series, 1..4 region dims
                 if (fam == VMX_FAMILY_DYSON && dd == VMX_INFINITE_DIM) {
                     if (D == 1)
@@ -78,8 +88,6 @@ int dispatch_T(vmx_ctx* ctx, int family, int D, const std::vector<JobIn>& jobs, 
     throw VmxError(VMX_ERR_UNSUPPORTED, "integrand family / dimension not built");
 }
 
-using custom_hook = decltype(vmx_custom_integrand::run);
-
 /* Groups a batch by (family, dim) and runs each group through the device pipeline. */
 int integrate_groups(vmx_ctx* ctx, uint32_t n, const vmx_problem* const* ps, const vmx_integrand* const* fs,
                      void* const* bins, vmx_stats* const* stats, hipStream_t st,
@@ -97,10 +105,14 @@ int integrate_groups(vmx_ctx* ctx, uint32_t n, const vmx_problem* const* ps, con
         // the instantiation hook of a user-defined functor
         auto inner = [](const vmx_integrand* f) {
             if (f->family == VMX_FAMILY_CUSTOM)
-                return std::make_tuple(0, 0, static_cast<const vmx_custom_integrand*>(f->params)->run);
-            if (f->family != VMX_FAMILY_FUBINI) return std::make_tuple(0, 0, custom_hook(nullptr));
+                return std::make_tuple(0, 0,
+                                       reinterpret_cast<const void*>(static_cast<const vmx_custom_integrand*>(f->params)->run));
+            if (f->family != VMX_FAMILY_FUBINI) return std::make_tuple(0, 0, static_cast<const void*>(nullptr));
             const auto* fp = static_cast<const vmx_fubini_params*>(f->params);
-            return std::make_tuple(fp->inner_family, fp->inner_dim, custom_hook(nullptr));
+            const void* hook = nullptr;
+            if (fp->inner_family == VMX_FAMILY_CUSTOM)
+                hook = reinterpret_cast<const void*>(static_cast<const vmx_custom_integrand*>(fp->inner)->run_fubini);
+            return std::make_tuple(fp->inner_family, fp->inner_dim, hook);
         };
         for (uint32_t k = i; k < n; ++k)
             if (!done[k] && fs[k]->family == fs[i]->family && fs[k]->dim == fs[i]->dim && inner(fs[k]) == inner(fs[i])) {
@@ -155,6 +167,8 @@ void validate(const vmx_problem* p, const vmx_integrand* f) {
     if (f->family == VMX_FAMILY_FUBINI) {
         const auto* fp = static_cast<const vmx_fubini_params*>(f->params);
         if (!fp->inner) throw VmxError(VMX_ERR_INVALID, "Fubini form: null inner parameters");
+        if (fp->inner_family == VMX_FAMILY_CUSTOM && !static_cast<const vmx_custom_integrand*>(fp->inner)->functor)
+            throw VmxError(VMX_ERR_INVALID, "Fubini form: user integrand without a functor");
         if (fp->mc_samples < 1 || fp->mc_samples >= 0x80000000u)
             throw VmxError(VMX_ERR_INVALID, "Fubini form: mc_samples must be 1..2^31-1");
         if (infinite) {
