@@ -75,7 +75,7 @@ enum vmx_form {
     /* the two variance-reduction forms with region_sampling_importance in place of
        region_sampling_uniform (src/control-variates/region-sampling.h:22-44): residual
        samples drawn from each region polynomial's |P| pdf over the bin's box, weighted
-       by 1/pdf.  FP32, dims 1..5 (the reference compiles it for float regions only). */
+       by 1/pdf.  FP32, dims 1..6 (the reference compiles it for float regions only). */
     VMX_FORM_VR_OPTIMIZE_IMPORTANCE = 3,
     VMX_FORM_VR_FIXED_IMPORTANCE = 4,
     /* the two variance-reduction forms with rr_integral_region in place of
@@ -99,7 +99,7 @@ enum vmx_form {
        ...), regions-integrator-parallel-variance-reduction.h:94-100, as in
        main/compilation-tests/multiple-parameters-2d.cc:101-103): the region is drawn by
        the roulette, the point in the bin's box from the region's pdf; weight 1/p times
-       1/pdf.  FP32 integrands of dimension 1..5, as for importance sampling alone. */
+       1/pdf.  FP32 integrands of dimension 1..6, as for importance sampling alone. */
     VMX_FORM_VR_OPTIMIZE_RR_INTEGRAL_IMPORTANCE = 11,
     VMX_FORM_VR_FIXED_RR_INTEGRAL_IMPORTANCE = 12,
     VMX_FORM_VR_OPTIMIZE_RR_ERROR_IMPORTANCE = 13,
@@ -111,7 +111,7 @@ enum vmx_form {
        residual sample chooses importance sampling with probability pdf_integral /
        (pdf_integral + volume) of the bin's box, else uniform sampling, and is weighted
        by 1/pdf of the strategy it took (0 where pdf < 1e-10), as the reference does.
-       FP32 integrands of dimension 1..5, as for importance sampling. */
+       FP32 integrands of dimension 1..6, as for importance sampling. */
     VMX_FORM_VR_OPTIMIZE_SAMPLING_RR = 17,
     VMX_FORM_VR_FIXED_SAMPLING_RR = 18,
     /* regions_integrator_parallel_variance_reduction_optimized with

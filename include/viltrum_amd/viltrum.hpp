@@ -355,7 +355,7 @@ inline IntegratorGpu integrator_variance_reduction_fixed(double alpha, std::size
 }
 /* regions_integrator_parallel_variance_reduction(rr_uniform_region(), cv_optimize_weight(),
    region_sampling_importance(), rng, spp) behind the crespo2021 generator
-   (src/control-variates/region-sampling.h:22-44); float bins, dims 1..5 */
+   (src/control-variates/region-sampling.h:22-44); float bins, dims 1..6 */
 inline IntegratorGpu integrator_variance_reduction_importance(std::size_t iterations, std::size_t spp,
                                                               std::size_t seed = 0,
                                                               int heuristic = VMX_HEUR_SIZE_RELATIVE) {
@@ -412,7 +412,7 @@ inline IntegratorGpu integrator_variance_reduction_rr_pdf_importance_fixed(doubl
 }
 /* regions_integrator_parallel_variance_reduction(rr_uniform_region(), cv_optimize_weight(),
    region_sampling_russian_roulette(), rng, spp) (src/control-variates/region-sampling.h:
-   47-84): importance or uniform sampling per sample by a roulette; float bins, dims 1..5 */
+   47-84): importance or uniform sampling per sample by a roulette; float bins, dims 1..6 */
 inline IntegratorGpu integrator_variance_reduction_sampling_russian_roulette(std::size_t iterations, std::size_t spp,
                                                                              std::size_t seed = 0,
                                                                              int heuristic = VMX_HEUR_SIZE_RELATIVE) {

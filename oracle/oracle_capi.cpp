@@ -552,6 +552,7 @@ int vmo_imp_sample(int D, const float* v, const float* rlo, const float* rhi, co
         case 3: return run(std::integral_constant<int, 3>{});
         case 4: return run(std::integral_constant<int, 4>{});
         case 5: return run(std::integral_constant<int, 5>{});
+        case 6: return run(std::integral_constant<int, 6>{});
         default: return -1;
     }
 }

@@ -441,8 +441,9 @@ int vmr_integrate(int scalar_bytes, int integrator, int heuristic, double alpha,
             VMR_I(3)
             VMR_I(4)
             VMR_I(5)
+            VMR_I(6)  // the reference's region cap
             default:
-                g_err = "D must be 1..5";
+                g_err = "D must be 1..6";
                 return -1;
         }
 #undef VMR_I

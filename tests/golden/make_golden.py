@@ -91,6 +91,9 @@ BIN_CASES = [  # (integrand, scalar, form, heuristic, res, iterations, spp, seed
     ("lobes5", 4, 3, 0, (3, 3), 60, 4, 2, 1.0),
     ("lobes5", 4, 18, 0, (3, 3), 60, 4, 2, 0.7),
     ("lobes5", 4, 15, 0, (3, 3), 60, 4, 2, 1.0),
+    # importance sampling and the sampling roulette at D = 6 (the reference's region cap)
+    ("poly6", 4, 3, 0, (2, 2), 30, 3, 2, 1.0),
+    ("poly6", 4, 18, 0, (2, 2), 30, 3, 2, 0.7),
 ]
 
 FUBINI_CASES = [  # (integrand, region dims, res, iterations, mc samples, spp, seed): integrator_crespo2021_infinite

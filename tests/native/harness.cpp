@@ -146,6 +146,7 @@ int vmh_imp_sample(int D, const float* v, const float* rlo, const float* rhi, fl
         case 3: return vmx::imp::sample<float, 3>(v, rlo, rhi, rvol, slo, shi, s, x, *w) ? 1 : 0;
         case 4: return vmx::imp::sample<float, 4>(v, rlo, rhi, rvol, slo, shi, s, x, *w) ? 1 : 0;
         case 5: return vmx::imp::sample<float, 5>(v, rlo, rhi, rvol, slo, shi, s, x, *w) ? 1 : 0;
+        case 6: return vmx::imp::sample<float, 6>(v, rlo, rhi, rvol, slo, shi, s, x, *w) ? 1 : 0;
         default: return -1;
     }
 }

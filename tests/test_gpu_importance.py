@@ -97,6 +97,10 @@ POLY4 = I.poly([(1.0, (1, 2, 0, 3)), (0.5, (2, 0, 1, 0)), (0.25, (0, 0, 0, 0))],
                name="poly4step")
 
 
+POLY6 = I.poly([(1.0, (1, 2, 0, 1, 0, 2)), (0.5, (0, 0, 3, 0, 1, 0)), (0.25, (2, 0, 0, 0, 0, 1))], 6,
+               step=([1, 1, 1, 1, 1, 1], 3.0, 0.5), name="poly6step")
+
+
 @pytest.mark.parametrize("f,res,it,spp", [(POLY4, (4, 3), 100, 4), (I.config5(2), (3, 3), 60, 4)],
                          ids=["poly4", "lobes5"])
 @pytest.mark.parametrize("form,alpha", [(V.FORM_VR_FIXED_IMPORTANCE, 1.0), (V.FORM_VR_OPTIMIZE_IMPORTANCE, 1.0),
@@ -148,3 +152,25 @@ def test_sampling_russian_roulette_takes_both_strategies(ctx):
     V.integrate(V.integrator_variance_reduction_sampling_russian_roulette(300, 4, 5), b, z, V.range_primary(3),
                 ctx=ctx)
     assert np.all(np.isfinite(b))
+
+
+@pytest.mark.parametrize("form", [V.FORM_VR_FIXED_IMPORTANCE, V.FORM_VR_OPTIMIZE_IMPORTANCE,
+                                  V.FORM_VR_OPTIMIZE_SAMPLING_RR], ids=["fixed1", "optimize", "sampling_rr"])
+def test_importance_six_dims_matches_oracle(ctx, form):
+    """Dim 6, the reference's region cap (the oracle matches the reference build there:
+    golden fixtures in test_oracle.py, and the sampler's points and weights match the
+    oracle's sample by sample in test_host_importance.py).  At 6 dims the 1/pdf weights
+    are so heavy-tailed that a per-bin variance over a few hundred seeds swings by
+    factors of 10 between two runs of the same estimator (measured on the oracle and the
+    GPU alike), so the spread is compared by interquartile range, which matched to
+    0.82..1.08 over 512 seeds; means by their standard errors, as elsewhere."""
+    g, o = _runs(ctx, POLY6, (4, 4), 30, 4, form, 1.0)
+    assert np.all(np.isfinite(g)) and np.all(np.isfinite(o))
+    mg, mo = g.mean(0), o.mean(0)
+    se = np.sqrt((g.var(0, ddof=1) + o.var(0, ddof=1)) / g.shape[0])
+    z = ((mg - mo) / se).ravel()
+    assert abs(z.mean()) < 5.0 / np.sqrt(z.size) + 0.05, z.mean()
+    assert np.abs(z).max() < 4.5, np.abs(z).max()
+    iqr = lambda a: np.percentile(a, 75, axis=0) - np.percentile(a, 25, axis=0)  # noqa: E731
+    r = float(np.median(iqr(g) / iqr(o)))
+    assert 0.6 < r < 1.67, r

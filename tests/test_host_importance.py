@@ -62,7 +62,7 @@ def test_sample_line_matches_oracle():
         assert abs(a - b) <= 6e-5, (a, b, s, t0, t1, p.tolist())
 
 
-@pytest.mark.parametrize("D,n", [(1, 6000), (2, 6000), (3, 3000), (4, 1500), (5, 600)])
+@pytest.mark.parametrize("D,n", [(1, 6000), (2, 6000), (3, 3000), (4, 1500), (5, 600), (6, 200)])
 def test_region_sample_matches_oracle(D, n):
     L = _oracle()
     rng = np.random.default_rng(D)

@@ -531,7 +531,7 @@ def integrator_variance_reduction_importance(iterations, spp, seed=0, alpha=None
     """regions_integrator_parallel_variance_reduction(rr_uniform_region(), cv_optimize_weight()
     or cv_fixed_weight(alpha), region_sampling_importance(), ...)
     (src/control-variates/region-sampling.h:22-44): residual samples from each region
-    polynomial's |P| pdf over the bin's box, weighted by 1/pdf.  FP32, dims 1..5."""
+    polynomial's |P| pdf over the bin's box, weighted by 1/pdf.  FP32, dims 1..6."""
     if alpha is None:
         return Integrator(iterations, spp, seed, heuristic, FORM_VR_OPTIMIZE_IMPORTANCE, **kw)
     return Integrator(iterations, spp, seed, heuristic, FORM_VR_FIXED_IMPORTANCE, alpha=alpha, **kw)
@@ -543,7 +543,7 @@ def integrator_variance_reduction_sampling_russian_roulette(iterations, spp, see
     or cv_fixed_weight(alpha), region_sampling_russian_roulette(), ...)
     (src/control-variates/region-sampling.h:47-84): each residual sample is drawn from the
     region polynomial's pdf with probability pdf_integral / (pdf_integral + volume) of the
-    bin's box, else uniformly, and weighted by 1/pdf of the strategy taken.  FP32, dims 1..5."""
+    bin's box, else uniformly, and weighted by 1/pdf of the strategy taken.  FP32, dims 1..6."""
     if alpha is None:
         return Integrator(iterations, spp, seed, heuristic, FORM_VR_OPTIMIZE_SAMPLING_RR, **kw)
     return Integrator(iterations, spp, seed, heuristic, FORM_VR_FIXED_SAMPLING_RR, alpha=alpha, **kw)
@@ -555,7 +555,7 @@ def integrator_variance_reduction_rr_integral(iterations, spp, seed=0, alpha=Non
     or cv_fixed_weight(alpha), region_sampling_uniform(), ...)
     (src/control-variates/region-russian-roulette.h:30-67): a residual sample picks the
     bin's region r with probability ~ |integral of P_r over the bin's box|, weight 1/p."""
-    if importance:  # with region_sampling_importance as the point sampler (float, dims 1..5)
+    if importance:  # with region_sampling_importance as the point sampler (float, dims 1..6)
         if alpha is None:
             return Integrator(iterations, spp, seed, heuristic, FORM_VR_OPTIMIZE_RR_INTEGRAL_IMPORTANCE, **kw)
         return Integrator(iterations, spp, seed, heuristic, FORM_VR_FIXED_RR_INTEGRAL_IMPORTANCE, alpha=alpha, **kw)
@@ -568,7 +568,7 @@ def integrator_variance_reduction_rr_error(iterations, spp, seed=0, alpha=None, 
                                            importance=False, **kw):
     """... with rr_error_region() (src/control-variates/region-russian-roulette.h:68-104):
     probability ~ |Region::error()| times the share of the region inside the bin."""
-    if importance:  # with region_sampling_importance as the point sampler (float, dims 1..5)
+    if importance:  # with region_sampling_importance as the point sampler (float, dims 1..6)
         if alpha is None:
             return Integrator(iterations, spp, seed, heuristic, FORM_VR_OPTIMIZE_RR_ERROR_IMPORTANCE, **kw)
         return Integrator(iterations, spp, seed, heuristic, FORM_VR_FIXED_RR_ERROR_IMPORTANCE, alpha=alpha, **kw)
@@ -581,7 +581,7 @@ def integrator_variance_reduction_rr_pdf(iterations, spp, seed=0, alpha=None, he
                                          importance=False, **kw):
     """... with rr_pdf_region() (src/control-variates/region-russian-roulette.h:106-149):
     probability ~ the integral of the region's importance pdf over the bin's box."""
-    if importance:  # with region_sampling_importance as the point sampler (float, dims 1..5)
+    if importance:  # with region_sampling_importance as the point sampler (float, dims 1..6)
         if alpha is None:
             return Integrator(iterations, spp, seed, heuristic, FORM_VR_OPTIMIZE_RR_PDF_IMPORTANCE, **kw)
         return Integrator(iterations, spp, seed, heuristic, FORM_VR_FIXED_RR_PDF_IMPORTANCE, alpha=alpha, **kw)

@@ -157,7 +157,7 @@ void validate(const vmx_problem* p, const vmx_integrand* f) {
                        "stratified forms: the Fubini form over an infinite range only (as the reference)");
     if (vmx::form_importance(p->form) &&
         (p->scalar != VMX_F32 || f->dim > vmx::imp::MAX_DIM || f->family == VMX_FAMILY_FUBINI))
-        throw VmxError(VMX_ERR_UNSUPPORTED, "importance sampling: FP32 integrands of dimension 1..5");
+        throw VmxError(VMX_ERR_UNSUPPORTED, "importance sampling: FP32 integrands of dimension 1..6");
     if (p->order != VMX_ORDER_CANONICAL && p->order != VMX_ORDER_REFERENCE)
         throw VmxError(VMX_ERR_INVALID, "unknown region order");
     if (p->refiner != VMX_REFINER_HEURISTIC && p->refiner != VMX_REFINER_LEARNED)

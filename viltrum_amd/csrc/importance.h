@@ -29,7 +29,7 @@
 namespace vmx {
 namespace imp {
 
-constexpr int MAX_DIM = 5;  // dims 1..3 keep the 3^D tensors in registers, 4 and 5 fold depth first
+constexpr int MAX_DIM = 6;  // dims 1..3 keep the 3^D tensors in registers, 4..6 fold depth first
 
 template <typename T>
 VMX_HD inline T at3(T t, const T* p) {
@@ -309,7 +309,7 @@ VMX_HD inline void draw_dims(const T* v, const T* an, const T* bn, const T* s, T
     }
 }
 
-/* sample() for dimensions 4 and 5: the same arguments and results */
+/* sample() for dimensions 4..6: the same arguments and results */
 template <typename T, int D>
 __attribute__((noinline)) VMX_HD bool sample_deep(const T* v, const T* rlo, const T* rhi, T rvol, const T* slo,
                                                   const T* shi, const T* s, T* x, T& weight, bool strategy_rr, T rr,
@@ -339,12 +339,12 @@ __attribute__((noinline)) VMX_HD bool sample_deep(const T* v, const T* rlo, cons
     return true;
 }
 
-/* A point of the box and its weight (sample_regs above for dims 1..3, sample_deep for 4
-   and 5); arguments and results as described at sample_regs. */
+/* A point of the box and its weight (sample_regs above for dims 1..3, sample_deep for
+   4..6); arguments and results as described at sample_regs. */
 template <typename T, int D>
 VMX_HD inline bool sample(const T* v, const T* rlo, const T* rhi, T rvol, const T* slo, const T* shi, const T* s, T* x,
                           T& weight, bool strategy_rr = false, T rr = T(0), T svol = T(0)) {
-    static_assert(D >= 1 && D <= MAX_DIM, "importance sampling: 1..5 dims");
+    static_assert(D >= 1 && D <= MAX_DIM, "importance sampling: 1..6 dims");
     if constexpr (D <= 3)
         return sample_regs<T, D>(v, rlo, rhi, rvol, slo, shi, s, x, weight, strategy_rr, rr, svol);
     else
