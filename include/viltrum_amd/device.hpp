@@ -29,7 +29,9 @@
  * for the functor (about a minute of hipcc per scalar type).
  *
  * As the inner integrand of the Fubini form (integrator_crespo2021_infinite<RD> over a
- * finite range), wrap it with viltrum_amd::device_integrand_fubini<D[, RD...]>(f).
+ * finite range), wrap it with viltrum_amd::device_integrand_fubini<D[, RD...]>(f); a
+ * functor of a sequence, for the same form over an infinite range, with
+ * viltrum_amd::device_integrand_infinite<RD...>(f).
  *
  * Compile with hipcc --offload-arch=gfx950 -std=c++17 -I<repo>/include
  * -Xclang -target-feature -Xclang -packed-fp32-ops, and link -lviltrum_amd.
@@ -77,6 +79,8 @@ template <int D, typename Fn, int... RDs>
 int32_t run_device_integrand_fubini(vmx_ctx* ctx, int32_t rd, const void* group, void* stream) {
     const auto& jobs = *static_cast<const std::vector<vmxe::JobIn>*>(group);
     const auto st = static_cast<hipStream_t>(stream);
+    if (static_cast<const vmx_fubini_params*>(jobs[0].params)->inner_dim != D)
+        throw vmxe::VmxError(VMX_ERR_INVALID, "Fubini form: inner_dim differs from the user integrand's dimension");
     using C = vmx::Custom<float, D, Fn>;
     int32_t rc = VMX_ERR_UNSUPPORTED;
     bool done = false;
@@ -89,6 +93,33 @@ int32_t run_device_integrand_fubini(vmx_ctx* ctx, int32_t rd, const void* group,
         throw vmxe::VmxError(VMX_ERR_UNSUPPORTED,
                              "user integrand not instantiated for the Fubini form at these region dims");
     return rc;
+}
+
+/* The hooks of a functor of a sequence: as the inner integrand of the Fubini form over
+   an infinite range, region dims RDs... (1..4) instantiated; alone it is refused (the
+   library never integrates a sequence integrand without a range to draw it from). */
+template <typename Fn, int... RDs>
+int32_t run_device_integrand_infinite(vmx_ctx* ctx, int32_t rd, const void* group, void* stream) {
+    const auto& jobs = *static_cast<const std::vector<vmxe::JobIn>*>(group);
+    const auto st = static_cast<hipStream_t>(stream);
+    if (static_cast<const vmx_fubini_params*>(jobs[0].params)->inner_dim != VMX_INFINITE_DIM)
+        throw vmxe::VmxError(VMX_ERR_INVALID, "an integrand of a sequence goes with an infinite range only");
+    using C = vmx::CustomSeq<float, Fn>;
+    int32_t rc = VMX_ERR_UNSUPPORTED;
+    bool done = false;
+    (void)std::initializer_list<int>{
+        (rd == RDs && !done ? (rc = vmxe::run_batch<float, RDs, vmx::Fubini<float, C, VMX_INFINITE_DIM, RDs>,
+                                                     vmx::FastMath>(ctx, jobs, st, nullptr),
+                               done = true, 0)
+                            : 0)...};
+    if (!done)
+        throw vmxe::VmxError(VMX_ERR_UNSUPPORTED,
+                             "user integrand of a sequence not instantiated for these region dims");
+    return rc;
+}
+
+inline int32_t run_sequence_alone(vmx_ctx*, int32_t, const void*, void*, void*) {
+    throw vmxe::VmxError(VMX_ERR_UNSUPPORTED, "an integrand of a sequence goes with an infinite range only");
 }
 
 template <typename Fn>
@@ -144,6 +175,48 @@ Integrand device_integrand_fubini(const Fn& fn) {
     } else {
         desc->run_fubini = &detail::run_device_integrand_fubini<int(D), Fn, 1, 2, 3>;
     }
+    return f;
+}
+
+/* A functor of a sequence for the Fubini form over an infinite range
+   (integrator_crespo2021_infinite<RD> and the optimised Fubini integrators over a
+   range_infinite / range_primary_infinite, src/integrate.h:80-120): FP32, region dims
+   RDs... (each 1..4; all four if none is given).  The functor is called as f(seq) with
+   seq a single-pass sequence whose begin() gives an iterator it: *it is the current
+   entry, ++it steps on (the point's region coordinates first, then entries drawn
+   uniformly over the rest of the range, [0, 1] past its explicit entries), as the
+   reference's integrands of an infinite range read theirs:
+       struct Series {
+           float decay;
+           template <class Seq>
+           __host__ __device__ float operator()(const Seq& seq) const {
+               auto x = seq.begin();
+               float sum = 0.f, term = 1.f;
+               while (*x < decay) { ++x; term *= 2.f * (*x); ++x; sum += term; }
+               return sum;
+           }
+       };
+       auto f = viltrum_amd::device_integrand_infinite<2>(Series{0.5f});
+   Each region-dim count instantiates the pipeline once. */
+template <int... RDs, typename Fn>
+Integrand device_integrand_infinite(const Fn& fn) {
+    static_assert((... && (RDs >= 1 && RDs <= 4)), "region dims 1..4");
+    static_assert(std::is_trivially_copyable<Fn>::value,
+                  "a user integrand is copied to the GPU byte for byte: it must be trivially copyable");
+    static_assert(sizeof(Fn) <= VMX_CUSTOM_MAX_BYTES, "a user integrand is staged in LDS: at most 2 KB");
+    using Gen = typename vmx::Fubini<float, vmx::CustomSeq<float, Fn>, VMX_INFINITE_DIM, 1>::Seq;
+    static_assert(std::is_invocable<const Fn&, const vmx::SeqView<float, Gen>&>::value,
+                  "the functor must be callable as f(const Sequence&)");
+    auto h = std::make_shared<detail::DeviceIntegrandHolder<Fn>>(fn);
+    h->desc.run = &detail::run_sequence_alone;
+    h->desc.functor = &h->fn;
+    if constexpr (sizeof...(RDs) > 0)
+        h->desc.run_fubini = &detail::run_device_integrand_infinite<Fn, RDs...>;
+    else
+        h->desc.run_fubini = &detail::run_device_integrand_infinite<Fn, 1, 2, 3, 4>;
+    Integrand f;
+    f.desc = vmx_integrand{VMX_FAMILY_CUSTOM, VMX_INFINITE_DIM, &h->desc};
+    f.params = h;
     return f;
 }
 

@@ -89,11 +89,11 @@ typedef struct vmx_lobes5_params {
    Monte Carlo over [rest_lo, rest_hi] (entries dim .. inner_dim-1), with mc_samples
    uniform samples per region node value and one per residual sample.  With inner_dim
    VMX_INFINITE_DIM the rest is an infinite range (RangeInfinite): the inner integrand
-   (VMX_FAMILY_DYSON) reads the region point followed by an unbounded sequence of uniform
-   draws, entry k of the sequence in [rest_lo[dim + k], rest_hi[dim + k]] for the first
-   rest_count entries and in [0, 1] after them (RangeInfinite::min(i) / max(i)); the
-   rest's volume is the product of the first rest_count extents.  FP32 only, as the
-   reference. */
+   (VMX_FAMILY_DYSON, or VMX_FAMILY_CUSTOM with a functor of a sequence) reads the region
+   point followed by an unbounded sequence of uniform draws, entry k of the sequence in
+   [rest_lo[dim + k], rest_hi[dim + k]] for the first rest_count entries and in [0, 1]
+   after them (RangeInfinite::min(i) / max(i)); the rest's volume is the product of the
+   first rest_count extents.  FP32 only, as the reference. */
 typedef struct vmx_fubini_params {
     int32_t inner_family;                   /* VMX_FAMILY_POLY, _BLOBS (3-D), _LOBES5, _CUSTOM or _DYSON */
     int32_t inner_dim;                      /* > the descriptor's dim, <= 5; or VMX_INFINITE_DIM */
@@ -122,7 +122,10 @@ typedef struct vmx_dyson_params {
    (trivially copyable, at most VMX_CUSTOM_MAX_BYTES: it is staged in LDS).  As the inner
    integrand of the Fubini form (VMX_FAMILY_FUBINI with inner_family VMX_FAMILY_CUSTOM),
    the library calls run_fubini with the form's region dims instead; it is NULL unless
-   the functor was instantiated for that form (viltrum_amd::device_integrand_fubini).
+   the functor was instantiated for that form (viltrum_amd::device_integrand_fubini).  A
+   functor of a sequence (the form over an infinite range, inner_dim VMX_INFINITE_DIM)
+   has only run_fubini (viltrum_amd::device_integrand_infinite); each hook refuses a
+   form whose inner_dim is not the one it was built for.
    Not for vmx_eval or vmx_exact_bins. */
 #define VMX_CUSTOM_MAX_BYTES 2048
 struct vmx_ctx;

@@ -85,6 +85,45 @@ bool same_bits(const std::vector<T>& a, const std::vector<T>& b) {
     return a.size() == b.size() && std::memcmp(a.data(), b.data(), sizeof(T) * a.size()) == 0;
 }
 
+/* The built-in Dyson series (vmx::Dyson) written against the sequence a user integrand
+   of an infinite range reads, in the same order of operations. */
+struct DysonFn {
+    float decay;
+    template <class Seq>
+    __host__ __device__ float operator()(const Seq& seq) const {
+        auto it = seq.begin();
+        float sum = 1.f, range = 1.f;
+        for (int step = 0; step < 4096 && *it < decay; ++step) {
+            ++it;
+            const float x = range * (*it);
+            const float prob = float(1.0 / double(range));
+            range = x;
+            sum += x / prob;
+            ++it;
+        }
+        return sum;
+    }
+};
+
+/* The reference's documented integrand of an infinite range (main/doc/ranges-infinite.cc:
+   5-27): a series cut by Russian roulette on every other entry.  Over the primary
+   infinite range with one region dim, E[f | x0] = [x0 < d] / (1 - d). */
+struct SeriesFn {
+    float decaying_factor;
+    template <class Seq>
+    __host__ __device__ float operator()(const Seq& seq) const {
+        auto x = seq.begin();
+        float sum = 0.0f, term = 1.0f;
+        while ((*x) < decaying_factor) {
+            ++x;
+            term *= 2.0f * (*x);
+            ++x;
+            sum += term;
+        }
+        return sum;
+    }
+};
+
 void hip_ok(hipError_t e) {
     if (e != hipSuccess) throw std::runtime_error(hipGetErrorString(e));
 }
@@ -258,7 +297,84 @@ int main(int argc, char** argv) {
         std::printf("fubini3_f32 %d\n", int(same_bits(a.v, b.v)));
     }
 
-    // 8. bins of a type the functor was not instantiated for are refused
+    // 8. the Fubini form over an infinite range with a functor of a sequence: equal to
+    //    the built-in Dyson series bit for bit (crespo2021_infinite at 2 region dims, the
+    //    optimised Fubini integrator's stratified form at 1), and the reference's own
+    //    documented series against its closed form
+    const auto dys = vi::device_integrand_infinite<1, 2>(DysonFn{0.75f});
+    {
+        const auto it = vi::integrator_crespo2021_infinite<2>(300, 4, 8, 6);
+        Grid<float> a(16, 12), b(16, 12);
+        vi::integrate(it, a, std::array<std::size_t, 2>{16, 12}, dys, vi::range_primary_infinite<float>());
+        vi::integrate(it, b, std::array<std::size_t, 2>{16, 12}, vi::dyson(0.75), vi::range_primary_infinite<float>());
+        std::printf("dyson_seq2_f32 %d\n", int(same_bits(a.v, b.v)));
+    }
+    {
+        const auto it = vi::integrator_adaptive_fubini_variance_reduction_parallel_optimized<1>(100, 4, 16, 9);
+        const auto rg = vi::range_infinite<float>({0.0f, 0.25f}, {1.0f, 0.75f});
+        std::vector<float> a(32, 0.f), b(32, 0.f);
+        struct V {
+            std::vector<float>* v;
+            float& operator()(const std::array<std::size_t, 1>& i) const { return (*v)[i[0]]; }
+        };
+        V va{&a}, vb{&b};
+        vi::integrate(it, va, std::array<std::size_t, 1>{32}, dys, rg);
+        vi::integrate(it, vb, std::array<std::size_t, 1>{32}, vi::dyson(0.75), rg);
+        std::printf("dyson_seq1_opt_f32 %d\n", int(same_bits(a, b)));
+    }
+    {
+        const auto it = vi::integrator_crespo2021_infinite<1>(63, 16, 4096, 5);
+        std::vector<float> a(16, 0.f);
+        struct V {
+            std::vector<float>* v;
+            float& operator()(const std::array<std::size_t, 1>& i) const { return (*v)[i[0]]; }
+        };
+        V va{&a};
+        vi::integrate(it, va, std::array<std::size_t, 1>{16}, vi::device_integrand_infinite<1>(SeriesFn{0.5f}),
+                      vi::range_primary_infinite<float>());
+        save(out, "series_seq1", a);
+        std::printf("series_seq1 1\n");
+    }
+    // a sequence functor is refused by the hooks where the form's inner_dim is not an
+    // infinite range, a D-dim functor where it is, and region dims not instantiated
+    {
+        auto msg_of = [&](const vi::Integrand& inner, int inner_dim) {
+            vmx_fubini_params fp{};
+            fp.inner_family = VMX_FAMILY_CUSTOM;
+            fp.inner_dim = inner_dim;
+            fp.mc_samples = 2;
+            for (int d = 0; d < VMX_MAX_DIM; ++d) fp.rest_hi[d] = 1.0;
+            fp.inner = inner.desc.params;
+            vi::Integrand g;
+            g.desc = vmx_integrand{VMX_FAMILY_FUBINI, 2, &fp};
+            Grid<float> gr(4, 4);
+            std::string msg = "none";
+            try {
+                vi::integrate(vi::integrator_crespo2021(15, 2), gr, std::array<std::size_t, 2>{4, 4}, g,
+                              vi::range_primary<2>());
+            } catch (const std::exception& e) {
+                msg = e.what();
+            }
+            return msg;
+        };
+        const PolyFn<3> p3{1, {1.0, 0.0, 0.0, 0.0}, {{1, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, {1.0, 1.0, 1.0},
+                           0.0, 0.0};
+        const std::string m1 = msg_of(dys, 3);
+        const std::string m2 = msg_of(vi::device_integrand_fubini<3, 2>(p3), VMX_INFINITE_DIM);
+        std::string m3 = "none";
+        try {
+            Grid<float> gr(4, 4, 4);
+            vi::integrate(vi::integrator_crespo2021_infinite<3>(15, 2, 2, 0), gr, std::array<std::size_t, 3>{4, 4, 4},
+                          dys, vi::range_primary_infinite<float>());
+        } catch (const std::exception& e) {
+            m3 = e.what();
+        }
+        std::printf("seq_refused %d\n", int(m1.find("infinite range only") != std::string::npos &&
+                                             m2.find("inner_dim differs") != std::string::npos &&
+                                             m3.find("not instantiated") != std::string::npos));
+    }
+
+    // 9. bins of a type the functor was not instantiated for are refused
     {
         Grid<double> g(4, 4);
         std::string msg = "none";

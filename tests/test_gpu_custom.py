@@ -4,8 +4,10 @@ to a built-in polynomial gives that family's bins and training pairs bit for bit
 reference order, FP32, and 4 dims through the region-major kernel); lambdas in the
 adapter's call forms f(x, y) and f(std::array) give the closed-form bins; a batch of one
 functor type gives its single calls; as the inner integrand of the Fubini form a functor
-gives the equal built-in polynomial's bins bit for bit; bins of a type not instantiated
-are refused."""
+gives the equal built-in polynomial's bins bit for bit; over an infinite range a functor
+of a sequence gives the built-in Dyson series' bins bit for bit, and the reference's own
+documented series its closed form; forms and region dims a hook was not built for, and
+bins of a type not instantiated, are refused."""
 import os
 import subprocess
 
@@ -33,7 +35,8 @@ def _edges(n):
     return np.arange(n + 1, dtype=np.float64) / n
 
 
-@pytest.mark.parametrize("case", ["poly2_f64", "poly2_f32", "poly4_f32", "batch3_f32", "fubini3_f32", "unsupported"])
+@pytest.mark.parametrize("case", ["poly2_f64", "poly2_f32", "poly4_f32", "batch3_f32", "fubini3_f32", "dyson_seq2_f32",
+                                  "dyson_seq1_opt_f32", "seq_refused", "unsupported"])
 def test_flags(run, case):
     assert run[1][case].split()[0] == "1", case
 
@@ -75,3 +78,17 @@ def test_array_form_three_dims(run):
     ex, ey = _edges(12), _edges(10)
     truth = (np.outer(np.diff(ex ** 2 / 2), np.diff(ey ** 2 / 2)) + 0.375 * np.outer(np.diff(ex), np.diff(ey))) * 120
     assert np.max(np.abs(b - truth)) < 2e-6 * np.max(np.abs(truth))
+
+
+def test_reference_series_of_a_sequence_matches_closed_form(run):
+    """The reference's documented integrand of an infinite range (a series cut by Russian
+    roulette, decaying factor d = 0.5) over the primary infinite range, regions over entry
+    0, 16 bins: E[f | x0] = [x0 < d] / (1 - d), so bins 0..7 hold 2 and bins 8..15 hold 0
+    (every region node there is 0, so the control variate and the residual vanish).  The
+    series' sd given x0 < d is 2.78 (simulated, 2e6 draws); 4096 residual samples per bin
+    give 0.043 per bin: the bounds are about 6 sd per bin and 5 sd for the mean of eight."""
+    b = _bins(run[0], "series_seq1", np.float32, (16,)).astype(np.float64)
+    print("series_seq1", np.round(b, 3))
+    assert np.all(np.isfinite(b))
+    assert np.max(np.abs(b[:8] - 2.0)) < 0.25 and abs(np.mean(b[:8]) - 2.0) < 0.08
+    assert np.max(np.abs(b[8:])) < 0.05

@@ -87,6 +87,57 @@ struct FamilyParams<Custom<T, DIM, Fn>> {
     using type = vmx_custom_integrand;
 };
 
+/* The sequence a user integrand of an infinite range reads, in the shape of the
+   reference's random sequences (src/monte-carlo/random-sequence-ref-dis.h:10-38):
+   begin() gives a forward iterator, *it is the current entry, ++it steps to the next one,
+   and end() is never reached.  Gen is the Fubini form's lazy generator (cur(), next()):
+   the entries are the point's region coordinates, then the Monte Carlo rest.  The
+   sequence is read in one pass: every iterator steps the same generator. */
+template <typename T, class Gen>
+struct SeqIter {
+    Gen* g;
+    VMX_HD T operator*() const { return g->cur(); }
+    VMX_HD SeqIter& operator++() {
+        g->next();
+        return *this;
+    }
+    VMX_HD bool operator==(const SeqIter&) const { return false; }
+    VMX_HD bool operator!=(const SeqIter&) const { return true; }
+};
+
+template <typename T, class Gen>
+struct SeqView {
+    using value_type = T;
+    using const_iterator = SeqIter<T, Gen>;
+    Gen* g;
+    VMX_HD const_iterator begin() const { return const_iterator{g}; }
+    VMX_HD const_iterator end() const { return const_iterator{g}; }
+};
+
+/* A functor of a sequence, f(const Sequence&) (the reference's integrands of an infinite
+   range, e.g. main/doc/ranges-infinite.cc:5-27), as the inner integrand of the Fubini
+   form over an infinite range. */
+template <typename T, typename Fn>
+struct CustomSeq {
+    alignas(alignof(Fn) > 8 ? alignof(Fn) : 8) unsigned char raw[sizeof(Fn)];
+
+    static CustomSeq from(const vmx_custom_integrand& c) {
+        CustomSeq q;
+        std::memcpy(q.raw, c.functor, sizeof(Fn));
+        return q;
+    }
+
+    template <class Gen>
+    VMX_HD T eval_seq(Gen& s) const {
+        return T((*reinterpret_cast<const Fn*>(raw))(SeqView<T, Gen>{&s}));
+    }
+};
+
+template <typename T, typename Fn>
+struct FamilyParams<CustomSeq<T, Fn>> {
+    using type = vmx_custom_integrand;
+};
+
 }  // namespace vmx
 
 namespace vmxe {

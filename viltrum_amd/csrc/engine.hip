@@ -54,8 +54,8 @@ int dispatch_T(vmx_ctx* ctx, int family, int D, const std::vector<JobIn>& jobs, 
                     return run_batch<T, 2, vmx::Fubini<T, vmx::Poly<T>, 3, 2>, Math>(ctx, jobs, st, pairs);
                 if (D == 1 && fam == VMX_FAMILY_POLY && dd == 2)
                     return run_batch<T, 1, vmx::Fubini<T, vmx::Poly<T>, 2, 1>, Math>(ctx, jobs, st, pairs);
-                // a user-defined inner integrand: the Fubini instantiation its own
-                // translation unit made
+                // a user-defined inner integrand (of D dims, or of a sequence over an
+                // infinite range): the Fubini instantiation its own translation unit made
                 if (fam == VMX_FAMILY_CUSTOM) {
                     const auto* c = static_cast<const vmx_custom_integrand*>(fp->inner);
                     if (!c->run_fubini)
@@ -172,16 +172,20 @@ void validate(const vmx_problem* p, const vmx_integrand* f) {
         if (fp->mc_samples < 1 || fp->mc_samples >= 0x80000000u)
             throw VmxError(VMX_ERR_INVALID, "Fubini form: mc_samples must be 1..2^31-1");
         if (infinite) {
-            if (fp->inner_family != VMX_FAMILY_DYSON)
-                throw VmxError(VMX_ERR_UNSUPPORTED, "Fubini form over an infinite range: the Dyson series only");
+            if (fp->inner_family != VMX_FAMILY_DYSON && fp->inner_family != VMX_FAMILY_CUSTOM)
+                throw VmxError(VMX_ERR_UNSUPPORTED,
+                               "Fubini form over an infinite range: the Dyson series or a user integrand of a sequence");
             if (f->dim > 4) throw VmxError(VMX_ERR_INVALID, "Fubini form over an infinite range: 1..4 region dims");
             if (fp->rest_count < 0 || fp->rest_count > VMX_MAX_DIM - f->dim)
                 throw VmxError(VMX_ERR_INVALID, "Fubini form over an infinite range: rest_count must be 0..6 - dim");
             for (int j = 0; j < fp->rest_count; ++j)
                 if (!(fp->rest_hi[f->dim + j] > fp->rest_lo[f->dim + j]))
                     throw VmxError(VMX_ERR_INVALID, "Fubini form: rest range must have hi > lo");
-            const double decay = static_cast<const vmx_dyson_params*>(fp->inner)->decay;
-            if (!(decay > 0.0 && decay < 1.0)) throw VmxError(VMX_ERR_INVALID, "Dyson series: decay must be in (0, 1)");
+            if (fp->inner_family == VMX_FAMILY_DYSON) {
+                const double decay = static_cast<const vmx_dyson_params*>(fp->inner)->decay;
+                if (!(decay > 0.0 && decay < 1.0))
+                    throw VmxError(VMX_ERR_INVALID, "Dyson series: decay must be in (0, 1)");
+            }
         } else {
             if (fp->inner_family == VMX_FAMILY_DYSON)
                 throw VmxError(VMX_ERR_INVALID, "Dyson series: only over an infinite range (inner_dim VMX_INFINITE_DIM)");

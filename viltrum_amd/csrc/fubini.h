@@ -125,13 +125,15 @@ struct Fubini {
    (MonteCarloFunction, regions-integrator-parallel-variance-reduction-optimized.h:36-47).
    Rest entries come 4 per Philox4x32-10 call: counter (the point's coordinate bits, the
    sample index), key (seed, a constant plus the block of 4).  The integrand reads the
-   sequence lazily, so a Dyson series that stops early draws nothing more. */
-template <typename T, int RD>
-struct Fubini<T, Dyson<T>, VMX_INFINITE_DIM, RD> {
+   sequence lazily, so a Dyson series that stops early draws nothing more.  Inner is any
+   integrand of a sequence with Inner::from(its parameter block) and eval_seq(Seq&): the
+   Dyson series or a user functor (custom.h). */
+template <typename T, typename Inner, int RD>
+struct Fubini<T, Inner, VMX_INFINITE_DIM, RD> {
     static_assert(RD >= 1 && RD <= 4, "1..4 region dims");
     static constexpr uint32_t RESIDUAL = 0x80000000u;  // sample-index tag of residual samples
     static constexpr uint32_t KEY1 = 0x44595349u;
-    Dyson<T> in;
+    Inner in;
     T rlo[VMX_MAX_DIM], rext[VMX_MAX_DIM];  // explicit rest entries: lower end, extent
     int32_t nrest;
     T vol;  // product of the explicit extents, in T (RangeInfinite::volume)
@@ -139,7 +141,7 @@ struct Fubini<T, Dyson<T>, VMX_INFINITE_DIM, RD> {
 
     static Fubini from(const vmx_fubini_params& p) {
         Fubini q{};
-        q.in = Dyson<T>::from(*static_cast<const vmx_dyson_params*>(p.inner));
+        q.in = Inner::from(*static_cast<const typename FamilyParams<Inner>::type*>(p.inner));
         q.nrest = p.rest_count < 0 ? 0 : p.rest_count > VMX_MAX_DIM - RD ? VMX_MAX_DIM - RD : p.rest_count;
         q.vol = T(1);
         for (int j = 0; j < VMX_MAX_DIM; ++j) {
@@ -229,8 +231,8 @@ struct IsFubini<Fubini<T, Inner, DF, RD>> : std::true_type {};
 /* the Fubini form over an infinite range */
 template <typename F>
 struct IsFubiniInf : std::false_type {};
-template <typename T, int RD>
-struct IsFubiniInf<Fubini<T, Dyson<T>, VMX_INFINITE_DIM, RD>> : std::true_type {};
+template <typename T, typename Inner, int RD>
+struct IsFubiniInf<Fubini<T, Inner, VMX_INFINITE_DIM, RD>> : std::true_type {};
 
 /* The integrand of one residual sample: the Fubini form draws one rest sample, every
    other family is its own evaluation.  `bare`: the Fubini form over an infinite range
