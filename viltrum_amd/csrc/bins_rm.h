@@ -29,19 +29,21 @@
 namespace vmx {
 
 template <typename T, int D, int DB, typename F, typename M>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_bins_rm(const Job<T>* __restrict__ jobs, uint32_t J, RegionArrays<T> r,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_bins_rm(const Job<T>* __restrict__ jobs, uint32_t J, RegionArrays<T> r,
                                                  FootArrays<T> fa, const uint32_t* __restrict__ off,
                                                  const uint32_t* __restrict__ lst, const F* __restrict__ fs,
                                                  JobRt* rt, ListPrefix lp) {
     static_assert(sizeof(T) == 4 && D >= 4, "FP32 regions of 4..6 dims");
+    // LDS for four blocks per CU (at most 40 KB each): the bins' boxes are recomputed
+    // where step 3 needs them instead of being kept here
     __shared__ uint32_t s_first[RM_MAXF][256];
-    __shared__ uint32_t s_cum[RM_MAXF + 1][256];
+    __shared__ uint32_t s_cum[RM_MAXF][256];
     __shared__ alignas(16) unsigned char f_raw[sizeof(F)];
     __shared__ uint32_t buf[2][RM_NS];  // keys: as drawn (then the samples' f terms), grouped by region
     __shared__ float resA[RM_NS];       // the samples' P terms
-    __shared__ T s_blo[DB][256], s_bhi[DB][256];
     __shared__ uint32_t s_n[256];
     __shared__ uint32_t s_cnt[256];
+    __shared__ uint32_t s_wtot[4];
     const int tid = threadIdx.x;
     const uint32_t tile = blockIdx.x;
     uint32_t j = 0;
@@ -99,14 +101,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
     for (int i = 0; i < DB; ++i) {
         blo[i] = jb.lo[i] + T(p[i]) * jb.drange[i];
         bhi[i] = jb.lo[i] + T(p[i] + 1) * jb.drange[i];
-        s_blo[i][tid] = blo[i];
-        s_bhi[i][tid] = bhi[i];
     }
     const double factor = double(jb.nbins);
     T* bins = static_cast<T*>(jb.bins);
     // control variate: footprint terms in list order (as k_bins1).  The LDS cache holds
     // every st-th footprint (st = 1 while the list fits): entry m is footprint m * st,
-    // its first region and the regions before it; entry mc holds the total.
+    // its first region and the regions before it (mc entries).
     double approx = 0.0;
     uint32_t o = 0, nf = 0, n = 0, first0 = 0, st = 1, mc = 0;
     if (active) {
@@ -141,7 +141,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
             approx += empty ? 0.0 : factor * integ;
         }
     }
-    s_cum[mc][tid] = n;
     s_n[tid] = n;
     const uint64_t spp = jb.spp;
     const bool sampling = active && spp > 0 && n > 0;
@@ -209,13 +208,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
             atomicAdd(&s_cnt[key == RM_NONE ? 255u : ((key >> RM_SLOT_BITS) & 255u)], 1u);
         }
         __syncthreads();
-        if (tid == 0) {
-            uint32_t run = 0;
-            for (int i = 0; i < 256; ++i) {
-                const uint32_t c = s_cnt[i];
-                s_cnt[i] = run;
-                run += c;
+        {
+            // exclusive prefix of the 256 bucket counts: a scan within each wave, then
+            // the totals of the waves before it
+            const uint32_t c = s_cnt[tid];
+            const int lane = tid & 63, wave = tid >> 6;
+            uint32_t incl = c;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const uint32_t v = __shfl_up(incl, off, 64);
+                if (lane >= off) incl += v;
             }
+            if (lane == 63) s_wtot[wave] = incl;
+            __syncthreads();
+            uint32_t base = 0;
+            for (int w = 0; w < wave; ++w) base += s_wtot[w];
+            s_cnt[tid] = base + incl - c;
         }
         __syncthreads();
         for (int q = 0; q < RM_SPB; ++q) {
@@ -233,14 +241,24 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
             if (key == RM_NONE) continue;
             const uint32_t slot = key & (uint32_t(RM_NS) - 1), lb = slot / RM_SPB, sq = slot % RM_SPB;
             const uint32_t reg = jb.reg_base + (key >> RM_SLOT_BITS);
-            // the slot's bin (job-local index) for the point's Philox words
+            // the slot's bin: job-local index (for the point's Philox words) and position
             uint64_t sb;
+            uint32_t sp[3] = {0, 0, 0};
             if constexpr (DB == 2) {
                 const uint32_t tx_n = uint32_t((jb.res[1] + 15) / 16);
                 const uint32_t row = (tt / tx_n) * 16 + lb / 16, col = (tt % tx_n) * 16 + lb % 16;
                 sb = uint64_t(row) * jb.res[1] + col;
+                sp[0] = row;
+                sp[1] = col;
             } else {
                 sb = uint64_t(tt) * 256 + lb;
+                uint32_t rem = uint32_t(sb);
+#pragma unroll
+                for (int i = DB - 1; i >= 0; --i) {
+                    const uint32_t ri = uint32_t(jb.res[i]);
+                    sp[i] = i > 0 ? rem % ri : rem;
+                    rem /= ri;
+                }
             }
             const uint64_t s = s0 + sq;
             const U4 c0 = philox4x32_10(U4{uint32_t(s), uint32_t(sb), uint32_t(sb >> 32) ^ jb.stream, 0x524D0000u},
@@ -253,8 +271,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
 #pragma unroll
             for (int d = 0; d < D; ++d) {
                 const T rl = rec[d], rh = rec[D + d];
-                const T sbl = d < DB ? s_blo[d < DB ? d : 0][lb] : jb.lo[d];
-                const T sbh = d < DB ? s_bhi[d < DB ? d : 0][lb] : jb.hi[d];
+                // the bin's box, as the bin's own lane computed it
+                const int dd = d < DB ? d : 0;
+                const T sbl = d < DB ? jb.lo[dd] + T(sp[dd]) * jb.drange[dd] : jb.lo[d];
+                const T sbh = d < DB ? jb.lo[dd] + T(sp[dd] + 1) * jb.drange[dd] : jb.hi[d];
                 ilo[d] = __builtin_fmaxf(sbl, rl);
                 ihi[d] = __builtin_fmaxf(ilo[d], __builtin_fminf(sbh, rh));
                 x[d] = mul_add<true>(u01f(cw[d]), ihi[d] - ilo[d], ilo[d]);

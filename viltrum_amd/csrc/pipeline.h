@@ -1674,8 +1674,8 @@ __global__ void k_bin_pass(const Job<T>* __restrict__ jobs, FootArrays<T> f, con
 
 constexpr uint32_t SORT_SHORT = 16;  // longer lists are sorted by a workgroup each
 constexpr int LANE_MAXF = 16;  // footprints per bin the per-bin kernel caches in LDS
-constexpr int RM_MAXF = 8;     // footprints per bin the region-major per-bin kernel caches in LDS
-constexpr int PREFIX_MIN = 8;  // lists longer than this store their prefix counts
+constexpr int RM_MAXF = 6;     // footprints per bin the region-major per-bin kernel caches in LDS
+constexpr int PREFIX_MIN = 6;  // lists longer than this store their prefix counts
 static_assert(LANE_MAXF >= PREFIX_MIN && RM_MAXF >= PREFIX_MIN,
               "a list a kernel does not cache whole in LDS has its prefix counts stored");
 
