@@ -32,7 +32,7 @@ template <typename T, int D, int DB, typename F, typename M>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_bins_rm(const Job<T>* __restrict__ jobs, uint32_t J, RegionArrays<T> r,
                                                  FootArrays<T> fa, const uint32_t* __restrict__ off,
                                                  const uint32_t* __restrict__ lst, const F* __restrict__ fs,
-                                                 JobRt* rt, ListPrefix lp) {
+                                                 JobRt* rt, ListPrefix lp, int xcd) {
     static_assert(sizeof(T) == 4 && D >= 4, "FP32 regions of 4..6 dims");
     // LDS for four blocks per CU (at most 40 KB each): the bins' boxes are recomputed
     // where step 3 needs them instead of being kept here
@@ -45,7 +45,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     __shared__ uint32_t s_cnt[256];
     __shared__ uint32_t s_wtot[4];
     const int tid = threadIdx.x;
-    const uint32_t tile = blockIdx.x;
+    // xcd: each XCD takes one contiguous run of tiles, whose regions then share its L2
+    const uint32_t tile = xcd ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
     uint32_t j = 0;
     {
         uint32_t lo = 0, hi = J;

@@ -748,13 +748,13 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
                 const unsigned grid2 = unsigned(total_tiles2);
                 if (db == 1)
                     vmx::k_bins_rm<T, D, 1, P, Math><<<grid2, 256, 0, st>>>(d_jobs, J, ra, fa, d_off, d_lst, d_f, d_rt,
-                                                                             lp);
+                                                                             lp, shape.bins_xcd);
                 else if (db == 2)
                     vmx::k_bins_rm<T, D, 2, P, Math><<<grid2, 256, 0, st>>>(d_jobs, J, ra, fa, d_off, d_lst, d_f, d_rt,
-                                                                             lp);
+                                                                             lp, shape.bins_xcd);
                 else
                     vmx::k_bins_rm<T, D, 3, P, Math><<<grid2, 256, 0, st>>>(d_jobs, J, ra, fa, d_off, d_lst, d_f, d_rt,
-                                                                             lp);
+                                                                             lp, shape.bins_xcd);
             }
         }
         if (all_rm) return;
