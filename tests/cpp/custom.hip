@@ -298,10 +298,10 @@ int main(int argc, char** argv) {
     }
 
     // 8. the Fubini form over an infinite range with a functor of a sequence: equal to
-    //    the built-in Dyson series bit for bit (crespo2021_infinite at 2 region dims, the
-    //    optimised Fubini integrator's stratified form at 1), and the reference's own
+    //    the built-in Dyson series bit for bit (crespo2021_infinite at 2 and 4 region dims,
+    //    the optimised Fubini integrator's stratified form at 1), and the reference's own
     //    documented series against its closed form
-    const auto dys = vi::device_integrand_infinite<1, 2>(DysonFn{0.75f});
+    const auto dys = vi::device_integrand_infinite<1, 2, 4>(DysonFn{0.75f});
     {
         const auto it = vi::integrator_crespo2021_infinite<2>(300, 4, 8, 6);
         Grid<float> a(16, 12), b(16, 12);
@@ -321,6 +321,15 @@ int main(int argc, char** argv) {
         vi::integrate(it, va, std::array<std::size_t, 1>{32}, dys, rg);
         vi::integrate(it, vb, std::array<std::size_t, 1>{32}, vi::dyson(0.75), rg);
         std::printf("dyson_seq1_opt_f32 %d\n", int(same_bits(a, b)));
+    }
+    {
+        // the reference's own program (main/compilation-tests/crespo21.cc:27-36): 4 region
+        // dims, 10 bins, through the reference's integrate(integrator, bins, f, range)
+        auto range_infinite = vi::range_primary_infinite<float>();
+        std::vector<float> a(10, 0.0f), b(10, 0.0f);
+        vi::integrate(vi::integrator_crespo2021_infinite<4>(1280, 8, 256), a, dys, range_infinite);
+        vi::integrate(vi::integrator_crespo2021_infinite<4>(1280, 8, 256), b, vi::dyson(0.75), range_infinite);
+        std::printf("dyson_seq4_f32 %d\n", int(same_bits(a, b)));
     }
     {
         const auto it = vi::integrator_crespo2021_infinite<1>(63, 16, 4096, 5);

@@ -36,7 +36,7 @@ def _edges(n):
 
 
 @pytest.mark.parametrize("case", ["poly2_f64", "poly2_f32", "poly4_f32", "batch3_f32", "fubini3_f32", "dyson_seq2_f32",
-                                  "dyson_seq1_opt_f32", "seq_refused", "unsupported"])
+                                  "dyson_seq4_f32", "dyson_seq1_opt_f32", "seq_refused", "unsupported"])
 def test_flags(run, case):
     assert run[1][case].split()[0] == "1", case
 
