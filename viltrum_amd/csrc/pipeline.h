@@ -1720,12 +1720,25 @@ __global__ void __launch_bounds__(256) k_sort_lists(const uint32_t* __restrict__
     if (moved)
         for (uint32_t i = 0; i < n; ++i) lst[o + i] = sv[i][tid];
     if (n > uint32_t(PREFIX_MIN)) {
+        // four entries' loads ahead of their stores: the stores could alias the
+        // footprint arrays for all the compiler knows, so loads behind them would
+        // each wait for memory in turn
         uint32_t c = 0;
-        for (uint32_t i = 0; i < n; ++i) {
-            const uint32_t fi = sv[i][tid];
-            lp.pcum[o + i] = c;
-            lp.pfirst[o + i] = fa.first[fi];
-            c += fa.count[fi];
+        for (uint32_t i0 = 0; i0 < n; i0 += 4) {
+            uint32_t fv[4], cv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint32_t fi = sv[i0 + u < n ? i0 + u : 0u][tid];  // past the list: entry 0 again
+                fv[u] = fa.first[fi];
+                cv[u] = fa.count[fi];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (i0 + u < n) {
+                    lp.pcum[o + i0 + u] = c;
+                    lp.pfirst[o + i0 + u] = fv[u];
+                    c += cv[u];
+                }
         }
     }
 }
