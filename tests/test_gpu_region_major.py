@@ -80,3 +80,19 @@ def test_region_major_is_deterministic(ctx):
         V.integrate(V.integrator_crespo2021(4095, 24, seed=5), b, f, V.range_primary(5), ctx=ctx)
         outs.append(b)
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
+@pytest.mark.parametrize("res", [(48, 40), (10, 9, 7)], ids=["9_tiles", "3_tiles_3_bin_dims"])
+def test_region_major_tile_map_does_not_change_results(ctx, monkeypatch, res):
+    """The XCD tile map (each XCD takes one contiguous run of tiles) only reorders which
+    workgroup takes which tile.  9 tiles leave a remainder over the 8 XCDs; 3 tiles are
+    fewer than the XCDs."""
+    f = I.config5(4)
+    outs = []
+    for x in ("0", "1"):
+        monkeypatch.setenv("VMX_BINS_XCD", x)
+        b = np.zeros(res, dtype=np.float32)
+        V.integrate(V.integrator_crespo2021(4095, 24, seed=7), b, f, V.range_primary(5), ctx=ctx)
+        assert np.all(np.isfinite(b))
+        outs.append(b)
+    assert np.array_equal(outs[0], outs[1])
