@@ -1634,14 +1634,17 @@ __device__ inline uint32_t div_fix(uint32_t c, uint32_t n, float inv) {
 }
 
 /* Per-bin footprint lists by counting sort, one wave per footprint walking its pixel
-   box with the last bin dim fastest (consecutive lanes, consecutive bins).
-   PLACE = false: cnt[bin] += 1.  PLACE = true (cnt = the counts, off = their exclusive
-   scan): each footprint id goes to off[bin] + (--cnt[bin]), so every list is filled
-   but in scheduling order; k_sort_lists then puts each list in ascending footprint id
-   (region order), which fixes the per-bin summation order. */
+   box with the last bin dim fastest (consecutive lanes, consecutive bins).  Pair c of
+   footprint w is pair pofs[w] + c of the call.
+   PLACE = false: the pair's rank in its bin, rank = cnt[bin]++ (arrival order).
+   PLACE = true (off = the counts' exclusive scan): each footprint id goes to
+   off[bin] + rank, so every list is filled, in arrival order, with no atomics;
+   k_sort_lists then puts each list in ascending footprint id (region order), which
+   fixes the per-bin summation order. */
 template <typename T, bool PLACE>
 __global__ void k_bin_pass(const Job<T>* __restrict__ jobs, FootArrays<T> f, const uint32_t* __restrict__ nfp_ptr,
-                           uint32_t* __restrict__ cnt, const uint32_t* __restrict__ off, uint32_t* __restrict__ lst) {
+                           uint32_t* __restrict__ cnt, const uint32_t* __restrict__ off, uint32_t* __restrict__ lst,
+                           const uint32_t* __restrict__ pofs, uint32_t* __restrict__ rank) {
     const uint32_t w = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
     const uint32_t lane = threadIdx.x % 64;
     if (w >= *nfp_ptr) return;
@@ -1651,6 +1654,7 @@ __global__ void k_bin_pass(const Job<T>* __restrict__ jobs, FootArrays<T> f, con
     const uint32_t total = f.npairs[w];
     const uint64_t r1 = jb.db > 1 ? jb.res[1] : 1, r2 = jb.db > 2 ? jb.res[2] : 1;
     const float i2 = 1.0f / float(n2), i12 = 1.0f / float(n1);
+    uint32_t* const rk = rank + pofs[w];
     for (uint32_t c = lane; c < total; c += 64) {
         const uint32_t q2 = div_fix(c, n2, i2);  // c = (a0 * n1 + a1) * n2 + a2
         const uint32_t a2 = c - q2 * n2;
@@ -1659,15 +1663,9 @@ __global__ void k_bin_pass(const Job<T>* __restrict__ jobs, FootArrays<T> f, con
         const uint64_t local = (uint64_t(bx[0] + a0) * r1 + (bx[2] + a1)) * r2 + (bx[4] + a2);
         const uint64_t bin = jb.bin_base + local;
         if constexpr (PLACE) {
-            const uint32_t o = off[bin];
-            if (off[bin + 1] - o == 1) {  // the bin's only footprint: no cursor needed
-                lst[o] = w;
-            } else {
-                const uint32_t k = atomicSub(&cnt[bin], 1u) - 1u;
-                lst[o + k] = w;
-            }
+            lst[off[bin] + rk[c]] = w;
         } else {
-            atomicAdd(&cnt[bin], 1u);
+            rk[c] = atomicAdd(&cnt[bin], 1u);
         }
     }
 }

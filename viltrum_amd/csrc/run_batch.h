@@ -719,12 +719,15 @@ int run_batch(vmx_ctx* ctx, const std::vector<JobIn>& in, hipStream_t st, PairsR
     HIP_CHECK(hipMemsetAsync(d_cnt, 0, sizeof(uint32_t) * (total_bins + 2), st));
     const unsigned fp_blocks = unsigned((uint64_t(nfp) * 64 + 255) / 256);
     if (pairs > 0)
-        vmx::k_bin_pass<T, false><<<fp_blocks, 256, 0, st>>>(d_jobs, fa, d_nfp, d_cnt, nullptr, nullptr);
+        vmx::k_bin_pass<T, false><<<fp_blocks, 256, 0, st>>>(d_jobs, fa, d_nfp, d_cnt, nullptr, nullptr, d_pofs,
+                                                             d_lst_tmp);
     bytes = scan_bins_bytes;
     HIP_CHECK(hipError_t(rocprim::exclusive_scan(d_scan_bins_tmp, bytes, d_cnt, d_off, 0u, size_t(total_bins + 1),
                                                 rocprim::plus<uint32_t>(), st)));
     if (pairs > 0) {
-        vmx::k_bin_pass<T, true><<<fp_blocks, 256, 0, st>>>(d_jobs, fa, d_nfp, d_cnt, d_off, d_lst);
+        // the pairs' ranks live in d_lst_tmp until k_sort_long reuses it
+        vmx::k_bin_pass<T, true><<<fp_blocks, 256, 0, st>>>(d_jobs, fa, d_nfp, d_cnt, d_off, d_lst, d_pofs,
+                                                            d_lst_tmp);
         vmx::k_sort_lists<T><<<unsigned((total_bins + 255) / 256), 256, 0, st>>>(d_off, total_bins, d_lst, d_qlong,
                                                                                  fa, lp);
         vmx::k_sort_long<<<1024, 256, 0, st>>>(d_off, d_qlong, d_lst, d_lst_tmp);
